@@ -1,0 +1,257 @@
+#!/usr/bin/env python3
+"""Benchmark of the XPBD substep loop on one MI355X.
+
+Workload (one "step" = one frame of 1/60 s = 15 substeps x 2 iterations):
+the 128 x 128 cloth of the benchmark configuration — 16 384 particles,
+97 026 DISTANCE rows (structural + both shears + bend, compliance
+log-uniform in [1e-8, 1e-3]) and 16 384 FLOOR rows (mu 0.4), Jacobi mode,
+run through the public ``Solver`` API (hipGraph replay per frame).
+
+Metric: constraint projections per second = rows x substeps x iterations /
+frame time (the reference's definition, examples/benchmark.py:123-130).
+
+Also reported on the same JSON line:
+* ``roofline``       — the dominant kernel (the Jacobi projection) at this
+  workload: algorithmic bytes per launch / average launch time, timed with
+  HIP events on the launch stream, against 8 TB/s.
+* ``roofline_scene`` — the same kernel and a full frame on 8 x 1024^2 cloths
+  (58.6M rows, a working set far above the caches), built on the device.
+* ``cpu_baseline``   — the serial CPU restatement of the reference kernels
+  (oracle/) on a bounded number of frames of the same workload, 1 core.
+
+``python bench.py [--gpus 1] [--steps K] [--warmup W]``
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "constraint proj/s (1 GPU); substeps/s at fixed residual, learned vs plain λ"
+HBM_PEAK_GBS = 8000.0
+
+
+def proj_kernel_bytes(n_distance: int, n_floor: int, n_particles: int) -> int:
+    """Algorithmic HBM bytes of one Jacobi projection launch: 24 B per
+    DISTANCE row (a, b, rest, alpha, lambda read + write), 20 B per FLOOR row
+    (a, floor_y, mu, lambda read + write), and per particle 16 B (x + inverse
+    mass read) + 12 B (x_prev read for friction).  The x write (12 B/particle)
+    belongs to the apply pass."""
+    return 24 * n_distance + 20 * n_floor + 28 * n_particles
+
+
+def time_kernel(fn, reps: int) -> float:
+    """Average duration in ms of ``fn`` (which launches on torch's current
+    stream), bracketed by HIP events on that stream."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def jacobi_launcher(N, X, XP, R, ANC, LAM, ACC, h, n_rows, tiling):
+    lib = N.lib()
+    args = (N.ptr(X), N.ptr(XP), N.ptr(R), N.ptr(ANC), N.ptr(LAM), N.ptr(ACC))
+
+    def fn():
+        N.check(lib.xpbd_solve_jacobi(N.current_stream(), *args, h, 0, n_rows, tiling[0],
+                                      tiling[1]), "xpbd_solve_jacobi")
+    return fn
+
+
+def cpu_baseline(h, s, budget_s: float) -> dict:
+    from oracle import oracle as orc
+
+    if not orc.LIB.exists():
+        orc.build()
+    sc = orc.Scene3(h, dt=s.dt, substeps=s.substeps, iterations=s.iterations, gravity=s.gravity,
+                    mode=s.solve_mode, damping=s.damping, relax=s.jacobi_relax,
+                    max_speed=s.max_speed)
+    t = time.perf_counter()
+    sc.step(1)
+    one = time.perf_counter() - t
+    frames = max(1, min(200, int(budget_s / max(one, 1e-6))))
+    t = time.perf_counter()
+    sc.step(frames)
+    dt = time.perf_counter() - t
+    proj = s.num_constraints * s.substeps * s.iterations * frames
+    return {"value": proj / dt, "unit": "proj/s", "cores": 1, "kind": "port",
+            "sample": f"{frames} frames of the same workload after 1 warm-up frame "
+                      f"({dt:.1f} s), serial C restatement (oracle/), "
+                      f"host {platform.processor() or platform.machine()}, "
+                      f"os.cpu_count()={os.cpu_count()}"}
+
+
+def roofline_scene(N, reps: int, frames: int) -> dict:
+    import torch
+
+    from xpbd_amd.scenes import packed_cloth_device
+    from xpbd_amd.solver import MODE_JACOBI
+
+    res, sheets = 1024, 8
+    sc = packed_cloth_device(res, sheets, "cuda:0")
+    torch.cuda.synchronize()
+    n, m, nd = sc["n_particles"], sc["n_rows"], sc["n_distance"]
+    tiling = (2, 4096)
+    h = float(torch.tensor(1.0 / 60.0 / 15.0, dtype=torch.float32))
+    launch = jacobi_launcher(N, sc["X"], sc["XP"], sc["R"], sc["ANC"], sc["LAM"], sc["ACC"], h, m,
+                             tiling)
+    ms = time_kernel(launch, reps)
+    sc["ACC"].zero_()
+    nbytes = proj_kernel_bytes(nd, m - nd, n)
+    d = N.ParticlesDesc()
+    d.x, d.v, d.x_prev, d.acc = (N.ptr(sc["X"]), N.ptr(sc["V"]), N.ptr(sc["XP"]),
+                                 N.ptr(sc["ACC"]))
+    d.rows, d.anchors, d.lam = N.ptr(sc["R"]), N.ptr(sc["ANC"]), N.ptr(sc["LAM"])
+    d.n_particles, d.n_rows = n, m
+    d.color_starts, d.num_colors = None, 0
+    d.mode, d.substeps, d.iterations = MODE_JACOBI, 15, 2
+    d.h, d.gx, d.gy, d.gz = h, 0.0, -9.81, 0.0
+    d.damping, d.relax, d.max_speed = 0.0, 1.0, 40.0
+    d.rows_per_thread, d.window = tiling
+    plan = N.c_void_p()
+    N.check(N.lib().xpbd_frame3_create(N.ctypes.byref(d), N.ctypes.byref(plan)), "create")
+    run = lambda: N.check(N.lib().xpbd_frame3_run(plan, N.current_stream(), 1, 1), "run")  # noqa
+    frame_ms = time_kernel(run, frames)
+    N.lib().xpbd_frame3_destroy(plan)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {
+        "workload": f"{sheets} x {res}^2 cloth: {n} particles, {nd} DISTANCE + {m - nd} FLOOR "
+                    "rows, jacobi 15x2",
+        "kernel": "k_jacobi_project",
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": nbytes,
+        "kernel_ms": round(ms, 4), "frame_ms": round(frame_ms, 3),
+        "proj_per_s": m * 30 / (frame_ms * 1e-3),
+    }
+
+
+def load_traffic(kernel: str):
+    """HBM traffic per launch measured by rocprofv3 --pmc runs, committed
+    under profiles/ (None when no measurement for this kernel exists)."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get(kernel)
+    except (ValueError, OSError):
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--mode", default="jacobi", choices=["jacobi", "gs"])
+    ap.add_argument("--res", type=int, default=128)
+    ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline-scene", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run plain 512 MiB device copies (PMC calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} requested; this build measures one GPU only "
+              "(scenes shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from xpbd_amd import Solver
+    from xpbd_amd import _native as N
+    from xpbd_amd.scenes import cloth_sheet
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    s, _ = cloth_sheet(Solver, res=args.res, solve_mode=args.mode, device="cuda:0")
+    host = s.host_arrays()
+    n_dist = int((host["type"] == 0).sum())
+    n_floor = int((host["type"] == 2).sum())
+    for _ in range(args.warmup):
+        s.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    proj = s.num_constraints * s.substeps * s.iterations
+    value = proj / (ms * 1e-3)
+    print(f"[bench] {args.mode} cloth{args.res}: {ms:.4f} ms/frame, {value / 1e9:.3f} G proj/s",
+          file=sys.stderr, flush=True)
+
+    roof = None
+    if args.mode == "jacobi":
+        h = float(torch.tensor(s.dt / s.substeps, dtype=torch.float32))
+        launch = jacobi_launcher(N, s.X, s.XP, s.R, s.ANC, s.LAM, s.ACC, h, s.num_constraints,
+                                 s._tiling)
+        kms = time_kernel(launch, args.kernel_reps)
+        s.ACC.zero_()
+        nbytes = proj_kernel_bytes(n_dist, n_floor, s.num_bodies)
+        ach = nbytes / (kms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic("k_jacobi_project"),
+                "kernel": "k_jacobi_project", "kernel_ms": round(kms, 5),
+                "bytes_per_launch": nbytes, "tiling": list(s._tiling)}
+        print(f"[bench] jacobi kernel {kms * 1e3:.2f} us, {ach:.1f} GB/s", file=sys.stderr,
+              flush=True)
+
+    if args.calib_copy:
+        a = torch.empty(128 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    scene = None
+    if not args.no_roofline_scene:
+        scene = roofline_scene(N, reps=20, frames=5)
+        print(f"[bench] roofline scene: kernel {scene['kernel_ms']:.3f} ms "
+              f"{scene['achieved']} GB/s, frame {scene['frame_ms']} ms", file=sys.stderr,
+              flush=True)
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(host, s, args.cpu_budget)
+        print(f"[bench] cpu baseline {cpu['value'] / 1e6:.2f} M proj/s", file=sys.stderr,
+              flush=True)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "proj/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"cloth {args.res}x{args.res}: {s.num_bodies} particles, "
+                               f"{n_dist} DISTANCE + {n_floor} FLOOR rows, {args.mode}, "
+                               f"15 substeps x 2 iterations, dt 1/60, hipGraph per frame",
+                   "rows": s.num_constraints, "substeps": s.substeps,
+                   "iterations": s.iterations, "mode": args.mode},
+        "substeps_per_s": s.substeps / (ms * 1e-3),
+        "roofline": roof,
+        "roofline_scene": scene,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,183 @@
+"""CPU-only tests of the host side: the C-ABI library's exports, the host
+coloring routine (against the oracle), row packing and ordering, the host
+broad phase, and the scene generators.  Nothing here launches a kernel."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        names |= set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(xpbd_\w+)\s*\(", text, re.M))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+
+    from xpbd_amd import _native as N
+
+    lib = N.lib()
+    names = _declared_symbols()
+    assert len(names) >= 12, names
+    for name in names:
+        assert hasattr(lib, name), f"{name} declared in include/ but not exported"
+    assert b"gfx950" in lib.xpbd_build_info()
+    assert lib.xpbd_error_string(0)
+    del ctypes
+
+
+def test_device_cpu_is_rejected():
+    from xpbd_amd import Solver
+
+    with pytest.raises(ValueError):
+        Solver(device="cpu")
+
+
+def _random_rows(seed, n_bodies=300, m=2000):
+    rng = np.random.default_rng(seed)
+    a = rng.integers(0, n_bodies, m)
+    b = rng.integers(0, n_bodies, m)
+    b = np.where(b == a, (b + 1) % n_bodies, b)
+    b[rng.random(m) < 0.15] = -1
+    return n_bodies, a, b
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_host_coloring_equals_oracle(oracle, seed):
+    from xpbd_amd.coloring import color_constraints, color_ranges
+
+    n, a, b = _random_rows(seed)
+    got = color_constraints(n, a, b)
+    want = oracle.color_constraints(n, a, b)
+    assert (got == want).all()
+    o1, s1, k1 = color_ranges(got)
+    o2, s2, k2 = oracle.color_ranges(want)
+    assert k1 == k2 and (o1 == o2).all() and (s1 == s2).all()
+
+
+def test_host_coloring_dense_pile_over_64_colors(oracle):
+    """A body shared by > 64 rows forces more than one 64-bit mask word."""
+    from xpbd_amd.coloring import color_constraints
+
+    m = 150
+    a = np.zeros(m, np.int32)
+    b = np.arange(1, m + 1, dtype=np.int32)
+    got = color_constraints(m + 1, a, b)
+    assert sorted(got.tolist()) == list(range(m))
+    assert (got == oracle.color_constraints(m + 1, a, b)).all()
+
+
+def test_host_coloring_config2_cloth(oracle):
+    from xpbd_amd import Solver
+    from xpbd_amd.coloring import color_constraints
+    from xpbd_amd.scenes import cloth_sheet
+
+    s, _ = cloth_sheet(Solver, res=128)
+    h = s.host_arrays()
+    got = color_constraints(s.num_bodies, h["a"], h["b"])
+    assert (got == oracle.color_constraints(s.num_bodies, h["a"], h["b"])).all()
+    assert 13 <= int(got.max()) + 1 <= 20
+
+
+def test_pack_rows_layout():
+    from xpbd_amd.solver import pack_rows
+
+    rows, anc = pack_rows([0, 1, 2, 3], [5, 6, 7, 8], [9, -1, -1, 10],
+                          rest=[0.5, 0.25, 0.0, 0.4], compliance=[1e-6, 2e-6, 0.0, 0.0],
+                          anchor=[[0, 0, 0], [1, 2, 3], [0, 0.7, 0], [0, 0, 0]],
+                          mu=[0.0, 0.0, 0.3, 0.6])
+    assert (rows[:, 0] & 0x0FFFFFFF).tolist() == [5, 6, 7, 8]
+    assert (rows[:, 0] >> 28).tolist() == [0, 1, 2, 3]
+    assert rows[:, 1].tolist() == [9, -1, -1, 10]
+    p0 = rows[:, 2].view(np.float32)
+    p1 = rows[:, 3].view(np.float32)
+    assert p0.tolist() == pytest.approx([0.5, 0.25, 0.7, 0.4])
+    assert p1.tolist() == pytest.approx([1e-6, 2e-6, 0.3, 0.6])
+    assert anc[1, :3].tolist() == [1, 2, 3]
+
+
+def test_jacobi_row_order_sorted_and_tiling():
+    from xpbd_amd import Solver
+    from xpbd_amd.scenes import cloth_sheet
+    from xpbd_amd.solver import choose_jacobi_tiling, jacobi_row_order, pack_rows
+
+    s, _ = cloth_sheet(Solver, res=64)
+    h = s.host_arrays()
+    order = jacobi_row_order(h["a"], h["b"])
+    key = np.where(h["b"] >= 0, np.minimum(h["a"], h["b"]), h["a"])[order]
+    assert (np.diff(key) >= 0).all()
+    assert sorted(order.tolist()) == list(range(len(order)))
+    rows, _ = pack_rows(h["type"], h["a"], h["b"], h["rest"], h["compliance"], h["anchor"],
+                        h["mu"])
+    rpt, window = choose_jacobi_tiling(rows[order])
+    assert rpt >= 1 and 256 <= window <= 4096
+
+
+def test_config2_cloth_counts():
+    from xpbd_amd import Solver
+    from xpbd_amd.scenes import cloth_sheet
+
+    s, _ = cloth_sheet(Solver, res=128)
+    h = s.host_arrays()
+    assert s.num_bodies == 16384
+    assert int((h["type"] == 0).sum()) == 97026
+    assert int((h["type"] == 2).sum()) == 16384
+    assert (h["inv_mass"] == 0).sum() == 2
+    c = h["compliance"][h["type"] == 0]
+    assert c.min() >= 1e-8 * 0.999 and c.max() <= 1e-3 * 1.001
+
+
+def _brute_pairs(p, cell):
+    """All pairs whose integer cell coordinates differ by at most one per axis."""
+    c = np.floor(p / cell).astype(np.int64)
+    out = set()
+    for i in range(len(p)):
+        for j in range(i + 1, len(p)):
+            if np.all(np.abs(c[i] - c[j]) <= 1):
+                out.add((i, j))
+    return out
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_grid_candidate_pairs_cover_neighbour_cells(seed):
+    from xpbd_amd.solver import grid_candidate_pairs
+
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-1, 1, size=(150, 3)).astype(np.float32)
+    a, b = grid_candidate_pairs(p, 0.3)
+    got = {(min(i, j), max(i, j)) for i, j in zip(a.tolist(), b.tolist())}
+    assert len(got) == len(a)                      # every unordered pair once
+    assert got == _brute_pairs(p, 0.3)
+
+
+def test_grid_candidate_pairs_enumeration_order():
+    """Order = offset by offset ((0,0,0) first), owner ascending, partners in
+    ascending index within the target cell: the CONTACT row order."""
+    from xpbd_amd.solver import grid_candidate_pairs
+
+    p = np.array([[0.05, 0.05, 0.05], [1.05, 0.05, 0.05], [0.06, 0.05, 0.05],
+                  [1.06, 0.05, 0.05]], np.float32)
+    a, b = grid_candidate_pairs(p, 1.0)
+    # same-cell pairs first (0,2) and (1,3), then the +x neighbour pairs
+    assert list(zip(a.tolist(), b.tolist())) == [(0, 2), (1, 3), (0, 1), (0, 3), (2, 1), (2, 3)]
+
+
+def test_flush_preserves_reference_numerics():
+    """inv_mass = float32(1/float32(m)); rest=None uses the float64 distance."""
+    from xpbd_amd import Solver
+
+    s = Solver()
+    a = s.add_particle((0.0, 0.0, 0.0), mass=3.0)
+    b = s.add_particle((0.1, 0.2, 0.3), mass=0.0)
+    s.add_distance(a, b)
+    h = s.host_arrays()
+    assert h["inv_mass"][0] == np.float32(1.0) / np.float32(3.0)
+    assert h["inv_mass"][1] == 0.0
+    assert h["rest"][0] == np.float32(np.sqrt(0.01 + 0.04 + 0.09))

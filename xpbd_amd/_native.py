@@ -1,0 +1,118 @@
+"""ctypes binding of ``libxpbd_hip.so`` (the C-ABI declared in include/xpbd.h).
+
+The library is loaded after ``torch`` so that both share one HIP runtime
+(torch ships ``libamdhip64.so.7`` under the same soname, and the loader
+reuses the copy already mapped).  A missing library is an error: there is
+no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+_LIB_PATH = Path(__file__).resolve().parent / "lib" / "libxpbd_hip.so"
+_lib = None
+
+c_int = ctypes.c_int
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+P_int = ctypes.POINTER(ctypes.c_int)
+
+
+class ParticlesDesc(ctypes.Structure):
+    """Mirror of ``xpbd_particles_desc`` (include/xpbd.h)."""
+
+    _fields_ = [
+        ("x", c_void_p), ("v", c_void_p), ("x_prev", c_void_p), ("acc", c_void_p),
+        ("rows", c_void_p), ("anchors", c_void_p), ("lam", c_void_p),
+        ("n_particles", c_int), ("n_rows", c_int),
+        ("color_starts", c_void_p), ("num_colors", c_int),
+        ("mode", c_int), ("substeps", c_int), ("iterations", c_int),
+        ("h", c_float), ("gx", c_float), ("gy", c_float), ("gz", c_float),
+        ("damping", c_float), ("relax", c_float), ("max_speed", c_float),
+        ("rows_per_thread", c_int), ("window", c_int),
+    ]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "xpbd_particles_pass": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                    c_float, c_float, c_float, c_float, c_float, c_float,
+                                    c_float, c_int]),
+    "xpbd_integrate": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float,
+                               c_float, c_int]),
+    "xpbd_apply_jacobi": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_int]),
+    "xpbd_finalize_velocity": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float,
+                                       c_float, c_int]),
+    "xpbd_solve_jacobi": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_float, c_int, c_int, c_int, c_int]),
+    "xpbd_solve_color": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_float, c_int, c_int, c_int]),
+    "xpbd_frame3_create": (c_int, [ctypes.POINTER(ParticlesDesc), ctypes.POINTER(c_void_p)]),
+    "xpbd_frame3_run": (c_int, [c_void_p, c_void_p, c_int, c_int]),
+    "xpbd_frame3_destroy": (c_int, [c_void_p]),
+    "xpbd_color_constraints": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "xpbd_color_ranges": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "xpbd_build_info": (ctypes.c_char_p, []),
+    "xpbd_error_string": (ctypes.c_char_p, [c_int]),
+}
+
+_EXTRA_SIGS: dict = {}
+
+
+def register(sigs: dict) -> None:
+    """Add signatures of entry points defined by other modules of the package."""
+    _EXTRA_SIGS.update(sigs)
+    if _lib is not None:
+        _bind(_lib, sigs)
+
+
+def _bind(lib, sigs):
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib_path() -> Path:
+    return _LIB_PATH
+
+
+def lib():
+    """Load (once) and return the bound library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    import torch  # noqa: F401  (shares its HIP runtime with the library)
+
+    if not _LIB_PATH.exists():
+        raise RuntimeError(
+            f"{_LIB_PATH} is missing: build it with `python -m xpbd_amd.build` "
+            "(the solver has no non-HIP path)")
+    mode = os.RTLD_NOW | os.RTLD_GLOBAL
+    lb = ctypes.CDLL(str(_LIB_PATH), mode=mode)
+    _bind(lb, _SIGS)
+    _bind(lb, _EXTRA_SIGS)
+    _lib = lb
+    return lb
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().xpbd_error_string(int(rc)).decode()
+        raise RuntimeError(f"{what or 'xpbd call'} failed: {msg} (code {rc})")
+
+
+def ptr(t) -> int:
+    """Device (or host) address of a contiguous torch tensor or numpy array."""
+    if hasattr(t, "data_ptr"):
+        assert t.is_contiguous()
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def current_stream():
+    import torch
+
+    return c_void_p(torch.cuda.current_stream().cuda_stream)

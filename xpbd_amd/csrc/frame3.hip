@@ -1,0 +1,146 @@
+// Frame driver for the 3-DOF solver: issues one frame of the substep loop
+// (src/xpbd3d/solver.py:412-467) on a stream, either eagerly or as one
+// replay of a hipGraph captured once per plan.
+//
+// Launch sequence per frame (S substeps, I iterations):
+//   integrate(s=0)
+//   for s: for it: jacobi: project(first=it==0) [+ apply if it < I-1]
+//                  gs:     for color: solve_color(first=it==0)
+//          particles_pass(apply? + finalize + cap? + integrate(s+1)?)
+// The lambda reset of Alg. 1 line 4 is folded into the first iteration's
+// projection (it reads lambda = 0 and writes every row), and apply of the
+// last iteration, finalize, cap and the next integrate share one streaming
+// pass: both are exact rewrites of the reference sequence.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.h"
+#include "xpbd.h"
+
+namespace {
+
+enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8 };
+
+struct Frame3 {
+  xpbd_particles_desc d;
+  std::vector<int> color_starts;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture_stream = nullptr;
+};
+
+int issue_frame(Frame3* f, hipStream_t s) {
+  const xpbd_particles_desc& d = f->d;
+  const int n = d.n_particles;
+  if (n <= 0) return 0;
+  const bool have_rows = d.n_rows > 0 && d.iterations > 0;
+  const bool jacobi = d.mode == XPBD_MODE_JACOBI;
+  int cap = (d.max_speed > 0.0f && std::isfinite(d.max_speed)) ? F_CAP : 0;
+  int rc = xpbd_particles_pass(s, d.x, d.v, d.x_prev, d.acc, F_INTEGRATE, d.relax, d.h, d.damping,
+                               d.max_speed, d.gx, d.gy, d.gz, n);
+  if (rc) return rc;
+  for (int sub = 0; sub < d.substeps; ++sub) {
+    if (have_rows) {
+      for (int it = 0; it < d.iterations; ++it) {
+        const int first = it == 0;
+        if (jacobi) {
+          rc = xpbd_solve_jacobi(s, d.x, d.x_prev, d.rows, d.anchors, d.lam, d.acc, d.h, first,
+                                 d.n_rows, d.rows_per_thread, d.window);
+          if (rc) return rc;
+          if (it + 1 < d.iterations) {
+            rc = xpbd_apply_jacobi(s, d.x, d.acc, d.relax, n);
+            if (rc) return rc;
+          }
+        } else {
+          for (int c = 0; c + 1 < (int)f->color_starts.size(); ++c) {
+            int start = f->color_starts[c];
+            int count = f->color_starts[c + 1] - start;
+            rc = xpbd_solve_color(s, d.x, d.x_prev, d.rows, d.anchors, d.lam, d.h, first, start,
+                                  count);
+            if (rc) return rc;
+          }
+        }
+      }
+    }
+    int flags = F_FINALIZE | cap;
+    if (have_rows && jacobi) flags |= F_APPLY;
+    if (sub + 1 < d.substeps) flags |= F_INTEGRATE;
+    rc = xpbd_particles_pass(s, d.x, d.v, d.x_prev, d.acc, flags, d.relax, d.h, d.damping,
+                             d.max_speed, d.gx, d.gy, d.gz, n);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+void drop_graph(Frame3* f) {
+  if (f->exec) (void)hipGraphExecDestroy(f->exec);
+  if (f->graph) (void)hipGraphDestroy(f->graph);
+  f->exec = nullptr;
+  f->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
+  if (!desc || !handle) return (int)hipErrorInvalidValue;
+  if (desc->mode != XPBD_MODE_GS && desc->mode != XPBD_MODE_JACOBI) return (int)hipErrorInvalidValue;
+  if (desc->mode == XPBD_MODE_GS && desc->n_rows > 0 && (!desc->color_starts || desc->num_colors < 1))
+    return (int)hipErrorInvalidValue;
+  if (desc->mode == XPBD_MODE_JACOBI && desc->n_rows > 0 && !desc->acc) return (int)hipErrorInvalidValue;
+  Frame3* f = new (std::nothrow) Frame3();
+  if (!f) return (int)hipErrorOutOfMemory;
+  f->d = *desc;
+  if (desc->mode == XPBD_MODE_GS && desc->n_rows > 0)
+    f->color_starts.assign(desc->color_starts, desc->color_starts + desc->num_colors + 1);
+  f->d.color_starts = nullptr;
+  *handle = f;
+  return 0;
+}
+
+int xpbd_frame3_run(void* handle, void* stream, int frames, int use_graph) {
+  Frame3* f = (Frame3*)handle;
+  if (!f || frames < 0) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  if (!use_graph) {
+    for (int k = 0; k < frames; ++k) {
+      int rc = issue_frame(f, s);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  if (!f->exec) {
+    if (!f->capture_stream) XPBD_CHECK(hipStreamCreateWithFlags(&f->capture_stream, hipStreamNonBlocking));
+    XPBD_CHECK(hipStreamBeginCapture(f->capture_stream, hipStreamCaptureModeThreadLocal));
+    int rc = issue_frame(f, f->capture_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(f->capture_stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    XPBD_CHECK(e);
+    f->graph = g;
+    e = hipGraphInstantiate(&f->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graph(f);
+      return (int)e;
+    }
+  }
+  for (int k = 0; k < frames; ++k) XPBD_CHECK(hipGraphLaunch(f->exec, s));
+  return 0;
+}
+
+int xpbd_frame3_destroy(void* handle) {
+  Frame3* f = (Frame3*)handle;
+  if (!f) return 0;
+  drop_graph(f);
+  if (f->capture_stream) (void)hipStreamDestroy(f->capture_stream);
+  delete f;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,485 @@
+// 3-DOF particle XPBD kernels for gfx950 and their C-ABI entry points.
+//
+// Device layout (all buffers owned by the caller):
+//   X   float4[N]  (x, y, z, inv_mass)      position + inverse mass
+//   V   float4[N]  (vx, vy, vz, 0)          velocity
+//   XP  float4[N]  (x, y, z, 0)             substep-start position (x_prev)
+//   R   int4[M]    packed constraint rows   (layout in common.h)
+//   ANC float4[M]  ATTACH world anchors     (only read for ATTACH rows)
+//   LAM float[M]   accumulated multipliers
+//   ACC float4[N]  Jacobi accumulators      (sum dx.xyz, count in .w)
+//
+// Reference counterparts: src/xpbd3d/kernels.py (integrate 49-73,
+// friction_delta 80-98, solve_constraints_color 105-208,
+// solve_constraints_jacobi 219-322, apply_jacobi 326-339, finalize_velocity
+// 346-358, cap_velocity 362-371) and the frame loop of Solver.step
+// (src/xpbd3d/solver.py:412-467).
+#include <cmath>
+
+#include "common.h"
+#include "xpbd.h"
+
+namespace {
+
+// kernels.py:80-98
+__device__ inline f3 friction_delta(f3 dp_rel, f3 n_hat, float penetration, float mu) {
+  f3 dp_t = dp_rel - n_hat * dot(dp_rel, n_hat);
+  float lt = length(dp_t);
+  if (lt < XPBD_EPS) return mk3(0.0f, 0.0f, 0.0f);
+  float scale = 1.0f;
+  float cone = mu * penetration;
+  if (lt >= cone) scale = cone / lt;
+  return dp_t * (-scale);
+}
+
+// Result of projecting one row against positions that are constant for the
+// whole Jacobi launch: up to two body corrections (kernels.py:219-322).
+struct Proj {
+  int na;       // number of corrections (0, 1 or 2)
+  int a, b;
+  f3 ca, cb;
+  bool upd;     // lambda was updated
+  float lam;
+};
+
+__device__ inline Proj project_jacobi(const float4* __restrict__ X, const float4* __restrict__ XP,
+                                      int4 row, float4 anchor_or_zero, float lam, float inv_dt2) {
+  Proj p;
+  p.na = 0;
+  p.upd = false;
+  p.lam = lam;
+  const int t = row_type(row);
+  const int a = row_a(row);
+  const int bb = row.y;
+  const float p0 = __int_as_float(row.z);
+  const float p1 = __int_as_float(row.w);
+  p.a = a;
+  p.b = bb;
+  if (t == T_DISTANCE || t == T_ATTACH) {
+    float4 xa4 = X[a];
+    float wa = xa4.w, wb;
+    f3 pb;
+    if (t == T_ATTACH) {
+      pb = xyz(anchor_or_zero);
+      wb = 0.0f;
+    } else {
+      float4 xb4 = X[bb];
+      pb = xyz(xb4);
+      wb = xb4.w;
+    }
+    float wsum = wa + wb;
+    if (wsum == 0.0f) return p;
+    f3 d = xyz(xa4) - pb;
+    float L = length(d);
+    if (L < XPBD_EPS) return p;
+    f3 n_hat = d / L;
+    float C = L - p0;
+    float alpha = p1 * inv_dt2;
+    float dlam = (-C - alpha * lam) / (wsum + alpha);
+    p.lam = lam + dlam;
+    p.upd = true;
+    p.ca = n_hat * (wa * dlam);
+    p.na = 1;
+    if (t == T_DISTANCE) {
+      p.cb = (-n_hat) * (wb * dlam);
+      p.na = 2;
+    }
+    return p;
+  }
+  if (t == T_FLOOR) {
+    float4 xa4 = X[a];
+    float wa = xa4.w;
+    if (wa == 0.0f) return p;
+    float C = xa4.y - p0;
+    if (C >= 0.0f) return p;
+    float dlam = -C / wa;
+    p.lam = lam + dlam;
+    p.upd = true;
+    f3 corr = mk3(0.0f, wa * dlam, 0.0f);
+    float mu = p1;
+    if (mu > 0.0f) {
+      f3 dp_rel = (xyz(xa4) + corr) - xyz(XP[a]);
+      corr = corr + friction_delta(dp_rel, mk3(0.0f, 1.0f, 0.0f), -C, mu);
+    }
+    p.ca = corr;
+    p.na = 1;
+    return p;
+  }
+  if (t == T_CONTACT) {
+    float4 xa4 = X[a], xb4 = X[bb];
+    float wa = xa4.w, wb = xb4.w;
+    float wsum = wa + wb;
+    if (wsum == 0.0f) return p;
+    f3 d = xyz(xa4) - xyz(xb4);
+    float L = length(d);
+    if (L < XPBD_EPS) return p;
+    f3 n_hat = d / L;
+    float C = L - p0;
+    if (C >= 0.0f) return p;
+    float dlam = -C / wsum;
+    p.lam = lam + dlam;
+    p.upd = true;
+    f3 ca = n_hat * (wa * dlam);
+    f3 cb = (-n_hat) * (wb * dlam);
+    float mu = p1;
+    if (mu > 0.0f) {
+      f3 dp_rel = ((xyz(xa4) + ca) - xyz(XP[a])) - ((xyz(xb4) + cb) - xyz(XP[bb]));
+      f3 fcorr = friction_delta(dp_rel, n_hat, -C, mu);
+      ca = ca + fcorr * (wa / wsum);
+      cb = cb - fcorr * (wb / wsum);
+    }
+    p.ca = ca;
+    p.cb = cb;
+    p.na = 2;
+    return p;
+  }
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi projection with LDS-staged accumulation.
+//
+// Rows arrive sorted by their smallest endpoint, so the first row of a tile
+// holds the tile's smallest particle index `base`.  Each block keeps a window
+// of `window` accumulators [base, base+window) in LDS (SoA: x, y, z, count),
+// adds every endpoint that falls inside it with LDS float atomics, and sends
+// the rest straight to global atomics.  After the tile, the non-empty window
+// entries are flushed to ACC with one global atomic per component.
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_project(
+    const float4* __restrict__ X, const float4* __restrict__ XP, const int4* __restrict__ R,
+    const float4* __restrict__ ANC, float* __restrict__ LAM, float4* __restrict__ ACC, float h,
+    int first, int n_rows, int rows_per_thread, int window) {
+  extern __shared__ float lds[];
+  float* sx = lds;
+  float* sy = lds + window;
+  float* sz = lds + 2 * window;
+  float* sn = lds + 3 * window;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile_rows = BLOCK * rows_per_thread;
+  const int row0 = tile * tile_rows;
+  if (row0 >= n_rows) return;
+  const int4 r0 = R[row0];
+  const int base = (r0.y >= 0 && r0.y < row_a(r0)) ? r0.y : row_a(r0);
+  for (int k = threadIdx.x; k < 4 * window; k += BLOCK) lds[k] = 0.0f;
+  __syncthreads();
+
+  const float inv_dt2 = 1.0f / (h * h);
+  const int row_end = min(row0 + tile_rows, n_rows);
+  for (int j = row0 + threadIdx.x; j < row_end; j += BLOCK) {
+    int4 row = R[j];
+    float lam = first ? 0.0f : LAM[j];
+    float4 anc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (row_type(row) == T_ATTACH) anc = ANC[j];
+    Proj p = project_jacobi(X, XP, row, anc, lam, inv_dt2);
+    if (p.upd || first) LAM[j] = p.lam;
+    if (p.na >= 1) {
+      unsigned off = (unsigned)(p.a - base);
+      if (off < (unsigned)window) {
+        atomicAdd(&sx[off], p.ca.x);
+        atomicAdd(&sy[off], p.ca.y);
+        atomicAdd(&sz[off], p.ca.z);
+        atomicAdd(&sn[off], 1.0f);
+      } else {
+        atomicAdd(&ACC[p.a].x, p.ca.x);
+        atomicAdd(&ACC[p.a].y, p.ca.y);
+        atomicAdd(&ACC[p.a].z, p.ca.z);
+        atomicAdd(&ACC[p.a].w, 1.0f);
+      }
+    }
+    if (p.na == 2) {
+      unsigned off = (unsigned)(p.b - base);
+      if (off < (unsigned)window) {
+        atomicAdd(&sx[off], p.cb.x);
+        atomicAdd(&sy[off], p.cb.y);
+        atomicAdd(&sz[off], p.cb.z);
+        atomicAdd(&sn[off], 1.0f);
+      } else {
+        atomicAdd(&ACC[p.b].x, p.cb.x);
+        atomicAdd(&ACC[p.b].y, p.cb.y);
+        atomicAdd(&ACC[p.b].z, p.cb.z);
+        atomicAdd(&ACC[p.b].w, 1.0f);
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < window; k += BLOCK) {
+    float c = sn[k];
+    if (c != 0.0f) {
+      float4* dst = &ACC[base + k];
+      atomicAdd(&dst->x, sx[k]);
+      atomicAdd(&dst->y, sy[k]);
+      atomicAdd(&dst->z, sz[k]);
+      atomicAdd(&dst->w, c);
+    }
+  }
+}
+
+// Plain per-row global-atomic Jacobi projection (no LDS window); kept as the
+// reference point the LDS-staged variant is measured against.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_project_global(
+    const float4* __restrict__ X, const float4* __restrict__ XP, const int4* __restrict__ R,
+    const float4* __restrict__ ANC, float* __restrict__ LAM, float4* __restrict__ ACC, float h,
+    int first, int n_rows) {
+  const int j = blockIdx.x * BLOCK + threadIdx.x;
+  if (j >= n_rows) return;
+  const float inv_dt2 = 1.0f / (h * h);
+  int4 row = R[j];
+  float lam = first ? 0.0f : LAM[j];
+  float4 anc = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (row_type(row) == T_ATTACH) anc = ANC[j];
+  Proj p = project_jacobi(X, XP, row, anc, lam, inv_dt2);
+  if (p.upd || first) LAM[j] = p.lam;
+  if (p.na >= 1) {
+    atomicAdd(&ACC[p.a].x, p.ca.x);
+    atomicAdd(&ACC[p.a].y, p.ca.y);
+    atomicAdd(&ACC[p.a].z, p.ca.z);
+    atomicAdd(&ACC[p.a].w, 1.0f);
+  }
+  if (p.na == 2) {
+    atomicAdd(&ACC[p.b].x, p.cb.x);
+    atomicAdd(&ACC[p.b].y, p.cb.y);
+    atomicAdd(&ACC[p.b].z, p.cb.z);
+    atomicAdd(&ACC[p.b].w, 1.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Colored Gauss-Seidel: one launch per color over a contiguous slice of the
+// color-sorted rows; rows of one color touch disjoint particles, so the
+// in-place updates are race-free (kernels.py:105-208).
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_gs_color(float4* __restrict__ X,
+                                                    const float4* __restrict__ XP,
+                                                    const int4* __restrict__ R,
+                                                    const float4* __restrict__ ANC,
+                                                    float* __restrict__ LAM, float h, int first,
+                                                    int start, int count) {
+  const int tid = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (tid >= count) return;
+  const int j = start + tid;
+  const int4 row = R[j];
+  const int t = row_type(row);
+  const int a = row_a(row);
+  const int bb = row.y;
+  const float p0 = __int_as_float(row.z);
+  const float p1 = __int_as_float(row.w);
+  float lam = first ? 0.0f : LAM[j];
+  if (first) LAM[j] = 0.0f;
+  const float inv_dt2 = 1.0f / (h * h);
+
+  if (t == T_DISTANCE || t == T_ATTACH) {
+    float4 xa4 = X[a];
+    float wa = xa4.w, wb;
+    f3 pb;
+    float4 xb4;
+    if (t == T_ATTACH) {
+      pb = xyz(ANC[j]);
+      wb = 0.0f;
+    } else {
+      xb4 = X[bb];
+      pb = xyz(xb4);
+      wb = xb4.w;
+    }
+    float wsum = wa + wb;
+    if (wsum == 0.0f) return;
+    f3 d = xyz(xa4) - pb;
+    float L = length(d);
+    if (L < XPBD_EPS) return;
+    f3 n_hat = d / L;
+    float C = L - p0;
+    float alpha = p1 * inv_dt2;
+    float dlam = (-C - alpha * lam) / (wsum + alpha);
+    LAM[j] = lam + dlam;
+    X[a] = with_xyz(xa4, xyz(xa4) + n_hat * (wa * dlam));
+    if (t == T_DISTANCE) X[bb] = with_xyz(xb4, xyz(xb4) - n_hat * (wb * dlam));
+    return;
+  }
+  if (t == T_FLOOR) {
+    float4 xa4 = X[a];
+    float wa = xa4.w;
+    if (wa == 0.0f) return;
+    float C = xa4.y - p0;
+    if (C >= 0.0f) return;
+    float dlam = -C / wa;
+    LAM[j] = lam + dlam;
+    f3 xa = xyz(xa4) + mk3(0.0f, wa * dlam, 0.0f);
+    float mu = p1;
+    if (mu > 0.0f) {
+      f3 dp_rel = xa - xyz(XP[a]);
+      f3 corr = friction_delta(dp_rel, mk3(0.0f, 1.0f, 0.0f), -C, mu);
+      xa = xa + corr;
+    }
+    X[a] = with_xyz(xa4, xa);
+    return;
+  }
+  if (t == T_CONTACT) {
+    float4 xa4 = X[a], xb4 = X[bb];
+    float wa = xa4.w, wb = xb4.w;
+    float wsum = wa + wb;
+    if (wsum == 0.0f) return;
+    f3 d = xyz(xa4) - xyz(xb4);
+    float L = length(d);
+    if (L < XPBD_EPS) return;
+    f3 n_hat = d / L;
+    float C = L - p0;
+    if (C >= 0.0f) return;
+    float dlam = -C / wsum;
+    LAM[j] = lam + dlam;
+    f3 xa = xyz(xa4) + n_hat * (wa * dlam);
+    f3 xb = xyz(xb4) - n_hat * (wb * dlam);
+    float mu = p1;
+    if (mu > 0.0f) {
+      f3 dp_rel = (xa - xyz(XP[a])) - (xb - xyz(XP[bb]));
+      f3 corr = friction_delta(dp_rel, n_hat, -C, mu);
+      xa = xa + corr * (wa / wsum);
+      xb = xb - corr * (wb / wsum);
+    }
+    X[a] = with_xyz(xa4, xa);
+    X[bb] = with_xyz(xb4, xb);
+    return;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-particle streaming pass.  One kernel covers, in this order and each
+// behind a flag: apply_jacobi (kernels.py:326-339), finalize_velocity
+// (346-358), cap_velocity (362-371) and the next substep's integrate (49-73).
+// Fusing them is exact: each is a pure per-particle map.
+// ---------------------------------------------------------------------------
+enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8 };
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_particles(float4* __restrict__ X, float4* __restrict__ V,
+                                                     float4* __restrict__ XP,
+                                                     float4* __restrict__ ACC, int flags,
+                                                     float relax, float h, float damping,
+                                                     float max_speed, float gx, float gy, float gz,
+                                                     int n) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  float4 x4 = X[i];
+  const float w = x4.w;
+  f3 x = xyz(x4);
+  bool x_dirty = false;
+  if (flags & F_APPLY) {
+    float4 acc = ACC[i];
+    float c = acc.w;
+    if (c > 0.0f && w > 0.0f) {
+      x = x + xyz(acc) * (relax / c);
+      x_dirty = true;
+    }
+    ACC[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  f3 v;
+  bool v_loaded = false;
+  f3 xp;
+  if (flags & F_FINALIZE) {
+    xp = xyz(XP[i]);
+    if (w == 0.0f) {
+      v = mk3(0.0f, 0.0f, 0.0f);
+    } else {
+      v = ((x - xp) / h) * (1.0f - damping);
+    }
+    v_loaded = true;
+    if (flags & F_CAP) {
+      float speed = length(v);
+      if (speed > max_speed) v = v * (max_speed / speed);
+    }
+  }
+  if (flags & F_INTEGRATE) {
+    XP[i] = make_float4(x.x, x.y, x.z, 0.0f);
+    if (w != 0.0f) {
+      if (!v_loaded) {
+        v = xyz(V[i]);
+        v_loaded = true;
+      }
+      v = v + mk3(gx, gy, gz) * h;
+      x = x + v * h;
+      x_dirty = true;
+    }
+  }
+  if (v_loaded) V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+  if (x_dirty) X[i] = make_float4(x.x, x.y, x.z, w);
+}
+
+constexpr int kBlock = 256;
+
+inline int blocks_for(long long n, int per_block) { return (int)((n + per_block - 1) / per_block); }
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI entry points
+// ===========================================================================
+extern "C" {
+
+int xpbd_particles_pass(void* stream, float* X, float* V, float* XP, float* ACC, int flags,
+                        float relax, float h, float damping, float max_speed, float gx, float gy,
+                        float gz, int n) {
+  if (n <= 0) return 0;
+  if ((flags & F_APPLY) && ACC == nullptr) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_particles<kBlock>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                     (float4*)X, (float4*)V, (float4*)XP, (float4*)ACC, flags, relax, h, damping,
+                     max_speed, gx, gy, gz, n);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_integrate(void* stream, float* X, float* V, float* XP, float gx, float gy, float gz,
+                   float h, int n) {
+  return xpbd_particles_pass(stream, X, V, XP, nullptr, F_INTEGRATE, 1.0f, h, 0.0f, 0.0f, gx, gy,
+                             gz, n);
+}
+
+int xpbd_apply_jacobi(void* stream, float* X, float* ACC, float relax, int n) {
+  return xpbd_particles_pass(stream, X, nullptr, nullptr, ACC, F_APPLY, relax, 1.0f, 0.0f, 0.0f,
+                             0.0f, 0.0f, 0.0f, n);
+}
+
+int xpbd_finalize_velocity(void* stream, const float* X, const float* XP, float* V, float h,
+                           float damping, float max_speed, int n) {
+  int flags = F_FINALIZE;
+  if (max_speed > 0.0f && std::isfinite(max_speed)) flags |= F_CAP;
+  return xpbd_particles_pass(stream, (float*)X, V, (float*)XP, nullptr, flags, 1.0f, h, damping,
+                             max_speed, 0.0f, 0.0f, 0.0f, n);
+}
+
+int xpbd_solve_jacobi(void* stream, const float* X, const float* XP, const int* R,
+                      const float* ANC, float* LAM, float* ACC, float h, int first, int n_rows,
+                      int rows_per_thread, int window) {
+  if (n_rows <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  if (window <= 0) {
+    hipLaunchKernelGGL(k_jacobi_project_global<kBlock>, dim3(blocks_for(n_rows, kBlock)),
+                       dim3(kBlock), 0, s, (const float4*)X, (const float4*)XP, (const int4*)R,
+                       (const float4*)ANC, LAM, (float4*)ACC, h, first, n_rows);
+  } else {
+    if (rows_per_thread < 1) rows_per_thread = 1;
+    if (window > XPBD_MAX_WINDOW) return (int)hipErrorInvalidValue;
+    int nb = blocks_for(n_rows, kBlock * rows_per_thread);
+    size_t shmem = sizeof(float) * 4 * (size_t)window;
+    hipLaunchKernelGGL(k_jacobi_project<kBlock>, dim3(nb), dim3(kBlock), shmem, s,
+                       (const float4*)X, (const float4*)XP, (const int4*)R, (const float4*)ANC,
+                       LAM, (float4*)ACC, h, first, n_rows, rows_per_thread, window);
+  }
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_solve_color(void* stream, float* X, const float* XP, const int* R, const float* ANC,
+                     float* LAM, float h, int first, int start, int count) {
+  if (count <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_gs_color<kBlock>, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s,
+                     (float4*)X, (const float4*)XP, (const int4*)R, (const float4*)ANC, LAM, h,
+                     first, start, count);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,188 @@
+"""Synthetic scene generators for the benchmark configurations and tests.
+
+Each builder drives the public API (``add_particle``/``add_distance``/...)
+the way the reference's example scripts do (examples/hanging_chain.py:18-30,
+examples/cloth_drop.py:18-46, examples/benchmark.py:71-107), with the sizes of
+the workloads below:
+
+* ``hanging_chain``  — 64 particles of 1 kg spaced 0.05 m along +x, particle 0
+  static, 63 DISTANCE rows (compliance 1e-6), jitter N(0, 1e-4) m, GS.
+* ``cloth_sheet``    — res x res grid at 1 cm spacing, 1e-3 kg particles, two
+  pinned corners, structural + both shear diagonals + bend DISTANCE rows
+  (compliance log-uniform in [1e-8, 1e-3]), a FLOOR row per particle at y=0
+  with mu=0.4, jitter N(0, 1e-4) m.  res=128 gives 97 026 DISTANCE rows +
+  16 384 FLOOR rows.
+* ``cloth_bench``    — the reference benchmark cloth (structural + one shear
+  diagonal, examples/benchmark.py:71-90).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .scene import Body
+
+
+def hanging_chain(solver_cls, n: int = 64, spacing: float = 0.05, compliance: float = 1e-6,
+                  seed: int = 0, jitter: float = 1e-4, **kw):
+    kw.setdefault("dt", 1.0 / 60.0)
+    kw.setdefault("substeps", 15)
+    kw.setdefault("iterations", 2)
+    kw.setdefault("solve_mode", "gs")
+    s = solver_cls(**kw)
+    rng = np.random.default_rng(seed)
+    jit = rng.normal(0.0, jitter, size=(n, 3))
+    prev = None
+    for i in range(n):
+        p = (i * spacing + jit[i, 0], jit[i, 1], jit[i, 2])
+        b = s.add_particle(p, mass=0.0 if i == 0 else 1.0, collide=False)
+        if prev is not None:
+            s.add_distance(prev, b, rest=spacing, compliance=compliance)
+        prev = b
+    return s
+
+
+def cloth_edges(res: int) -> tuple[np.ndarray, np.ndarray]:
+    """Edge list of the config-2 cloth in emission order: per grid cell
+    (iy, ix) row-major, structural-x, structural-y, shear, shear, bend-x,
+    bend-y (the edges examples/cloth_drop.py:35-45 builds)."""
+    k = np.arange(res * res, dtype=np.int64)
+    ix = k % res
+    iy = k // res
+    ea = np.empty((res * res, 6), np.int64)
+    eb = np.empty((res * res, 6), np.int64)
+    valid = np.empty((res * res, 6), bool)
+    ea[:, 0], eb[:, 0], valid[:, 0] = k, k + 1, ix + 1 < res
+    ea[:, 1], eb[:, 1], valid[:, 1] = k, k + res, iy + 1 < res
+    ea[:, 2], eb[:, 2], valid[:, 2] = k, k + res + 1, (ix + 1 < res) & (iy + 1 < res)
+    ea[:, 3], eb[:, 3], valid[:, 3] = k + 1, k + res, (ix + 1 < res) & (iy + 1 < res)
+    ea[:, 4], eb[:, 4], valid[:, 4] = k, k + 2, ix + 2 < res
+    ea[:, 5], eb[:, 5], valid[:, 5] = k, k + 2 * res, iy + 2 < res
+    return ea[valid], eb[valid]
+
+
+def cloth_sheet(solver_cls, res: int = 128, spacing: float = 0.01, mass: float = 1e-3,
+                height: float = 0.5, seed: int = 0, jitter: float = 1e-4,
+                compliance_range=(1e-8, 1e-3), floor_mu: float = 0.4, with_floor: bool = True,
+                n_sheets: int = 1, **kw):
+    """Cloth of config 2 (``n_sheets`` independent copies side by side, for
+    the HBM-roofline scene): particles row-major per sheet, DISTANCE rows per
+    ``cloth_edges`` with rest = current separation, then one FLOOR row per
+    particle."""
+    kw.setdefault("dt", 1.0 / 60.0)
+    kw.setdefault("substeps", 15)
+    kw.setdefault("iterations", 2)
+    s = solver_cls(**kw)
+    rng = np.random.default_rng(seed)
+    iy, ix = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
+    base_pos = np.stack([(ix.ravel() - (res - 1) / 2) * spacing,
+                         np.full(res * res, float(height)),
+                         (iy.ravel() - (res - 1) / 2) * spacing], axis=1)
+    pinned = np.zeros(res * res, bool)
+    pinned[0] = pinned[res - 1] = True
+    masses = np.where(pinned, 0.0, mass)
+    ea, eb = cloth_edges(res)
+    lo, hi = np.log(compliance_range[0]), np.log(compliance_range[1])
+    for sheet in range(n_sheets):
+        pos = base_pos + rng.normal(0.0, jitter, size=base_pos.shape)
+        pos[:, 0] += sheet * (res + 4) * spacing
+        first = s.add_particles(pos, masses)[0]
+    grid = np.arange(res * res, dtype=np.int64).reshape(res, res)
+    for sheet in range(n_sheets):
+        off = sheet * res * res
+        comp = np.exp(rng.uniform(lo, hi, size=ea.shape[0]))
+        s.add_distances(ea + off, eb + off, rest=None, compliance=comp)
+    if with_floor:
+        s.add_floor_contacts(np.arange(n_sheets * res * res), floor_y=0.0, friction=floor_mu)
+    del first
+    return s, grid
+
+
+def cloth_bench(solver_cls, res: int, mode: str, substeps: int = 15, iterations: int = 2, **kw):
+    """examples/benchmark.py:71-90 cloth: structural edges + one shear diagonal."""
+    s = solver_cls(dt=1 / 60, substeps=substeps, iterations=iterations, solve_mode=mode,
+                   damping=0.01, **kw)
+    h = 2.4 / (res - 1)
+    grid = np.empty((res, res), np.int64)
+    for iy in range(res):
+        for ix in range(res):
+            pinned = iy == 0 and ix in (0, res - 1)
+            b = s.add_particle(((ix - res / 2) * h, 2.5, (iy - res / 2) * h),
+                               mass=0.0 if pinned else 0.02, collide=False)
+            grid[iy, ix] = b.index
+    for iy in range(res):
+        for ix in range(res):
+            if ix + 1 < res:
+                s.add_distance(Body(int(grid[iy, ix])), Body(int(grid[iy, ix + 1])), compliance=1e-6)
+            if iy + 1 < res:
+                s.add_distance(Body(int(grid[iy, ix])), Body(int(grid[iy + 1, ix])), compliance=1e-6)
+            if ix + 1 < res and iy + 1 < res:
+                s.add_distance(Body(int(grid[iy, ix])), Body(int(grid[iy + 1, ix + 1])),
+                               compliance=2e-6)
+    return s
+
+
+def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing: float = 0.01,
+                        mass: float = 1e-3, height: float = 0.5, floor_mu: float = 0.4,
+                        compliance_range=(1e-8, 1e-3)):
+    """The config-2 cloth generator scaled to ``n_sheets`` sheets of res^2,
+    built directly on the device in the packed layout of include/xpbd.h
+    (rows already in smallest-endpoint order).  Used for the HBM-roofline
+    scene (8 x 1024^2 = 58.6M rows), where host-side scene lists would cost
+    minutes.  Per particle k the rows are, when valid: (k,k+1), (k,k+res),
+    (k,k+res+1), (k,k+res-1), (k,k+2), (k,k+2res), FLOOR(k) — the same edge
+    set as ``cloth_edges``.
+
+    Returns dict of torch tensors X, V, XP, ACC (float4 per particle), R
+    (int4 rows), ANC, LAM, and counts n_particles, n_rows, n_distance."""
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    n1 = res * res
+    n = n1 * n_sheets
+    k = torch.arange(n, device=device, dtype=torch.int64)
+    kl = k % n1
+    ix = kl % res
+    iy = kl // res
+    sheet = k // n1
+    X = torch.empty((n, 4), device=device, dtype=torch.float32)
+    X[:, 0] = (ix.float() - (res - 1) / 2) * spacing + sheet.float() * (res + 4) * spacing
+    X[:, 1] = height
+    X[:, 2] = (iy.float() - (res - 1) / 2) * spacing
+    X[:, :3] += torch.randn((n, 3), device=device, generator=g) * 1e-4
+    pinned = (iy == 0) & ((ix == 0) | (ix == res - 1))
+    X[:, 3] = torch.where(pinned, torch.zeros_like(X[:, 3]),
+                          torch.full_like(X[:, 3], float(torch.tensor(1.0 / mass,
+                                                                       dtype=torch.float32))))
+    nb = torch.stack([k + 1, k + res, k + res + 1, k + res - 1, k + 2, k + 2 * res], dim=1)
+    valid = torch.stack([ix + 1 < res, iy + 1 < res, (ix + 1 < res) & (iy + 1 < res),
+                         (ix >= 1) & (iy + 1 < res), ix + 2 < res, iy + 2 < res], dim=1)
+    kinds = torch.cat([valid, torch.ones((n, 1), dtype=torch.bool, device=device)], dim=1)
+    bcol = torch.cat([nb, torch.full((n, 1), -1, device=device, dtype=torch.int64)], dim=1)
+    acol = k[:, None].expand(n, 7)
+    types = torch.zeros((n, 7), dtype=torch.int64, device=device)
+    types[:, 6] = 2
+    a = acol[kinds]
+    b = bcol[kinds]
+    t = types[kinds]
+    m = a.shape[0]
+    is_d = t == 0
+    bb = torch.where(is_d, b, a)
+    rest = torch.linalg.vector_norm(X[a, :3] - X[bb, :3], dim=1)
+    lo, hi = torch.log(torch.tensor(compliance_range, dtype=torch.float64))
+    comp = torch.exp(torch.rand(m, device=device, generator=g, dtype=torch.float64)
+                     * (hi - lo) + lo).float()
+    p0 = torch.where(is_d, rest, torch.zeros_like(rest))
+    p1 = torch.where(is_d, comp, torch.full_like(comp, floor_mu))
+    R = torch.empty((m, 4), device=device, dtype=torch.int32)
+    R[:, 0] = (a | (t << 28)).to(torch.int32)
+    R[:, 1] = b.to(torch.int32)
+    R[:, 2] = p0.view(torch.int32)
+    R[:, 3] = p1.view(torch.int32)
+    del a, b, t, bb, rest, comp, p0, p1, nb, valid, kinds, bcol, acol, types
+    return dict(
+        X=X, V=torch.zeros((n, 4), device=device), XP=torch.zeros((n, 4), device=device),
+        ACC=torch.zeros((n, 4), device=device), R=R,
+        # no ATTACH rows: the anchor array is never read, one slot suffices
+        ANC=torch.zeros((1, 4), device=device), LAM=torch.zeros(m, device=device),
+        n_particles=n, n_rows=m, n_distance=int(is_d.sum()))

@@ -1,0 +1,683 @@
+"""3-DOF XPBD particle solver — the drop-in for ``xpbd3d.Solver``
+(src/xpbd3d/solver.py:57-508), running on hand-written gfx950 HIP kernels.
+
+Same constructor, scene-building methods, knobs and read-back as the
+reference.  The host keeps the scene as growable columns and ``_flush``
+packs them into device buffers (torch tensors used purely as HIP
+memory) in the layout of include/xpbd.h:
+
+* particles: ``X`` float4 (x, y, z, inv_mass), ``V``/``XP``/``ACC`` float4;
+* rows: one int4 per constraint (body_a | type << 28, body_b, p0, p1),
+  physically reordered — by color for Gauss-Seidel (each color is one
+  contiguous launch), by smallest endpoint for Jacobi (so a tile's endpoints
+  fall in one LDS window).  ``_perm`` maps device rows back to API order.
+
+``step()`` runs one frame through ``xpbd_frame3_run``: the whole substep loop
+is captured once into a hipGraph and replayed per frame (the reference
+launches every kernel from Python).  There is no CPU execution path;
+``device="cpu"`` is rejected.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native as N
+from .coloring import color_constraints, color_ranges, color_summary
+from .scene import Body, ConstraintHandle, Shape
+
+# Constraint type codes (solver.py:39-43)
+DISTANCE = 0
+ATTACH = 1
+FLOOR = 2
+CONTACT = 3
+
+MODE_GS = 0
+MODE_JACOBI = 1
+
+
+class _Col:
+    """Growable host column: O(1) scalar appends, bulk numpy extends kept as
+    chunks and joined once on demand.  Floats are held as float64 (the
+    reference keeps Python floats in lists and rounds them to float32 at
+    flush, solver.py:358-392)."""
+
+    def __init__(self, dtype, width: int = 0):
+        self.dtype = dtype
+        self.width = width
+        self._chunks: list = []
+        self._n_chunks = 0
+        self._tail: list = []
+
+    def _shape(self):
+        return (-1, self.width) if self.width else (-1,)
+
+    def __len__(self):
+        return self._n_chunks + len(self._tail)
+
+    def append(self, v):
+        self._tail.append(v)
+
+    def extend(self, arr):
+        self._spill()
+        a = np.asarray(arr, self.dtype).reshape(self._shape())
+        self._chunks.append(a)
+        self._n_chunks += a.shape[0]
+
+    def _spill(self):
+        if self._tail:
+            t = np.asarray(self._tail, self.dtype).reshape(self._shape())
+            self._chunks.append(t)
+            self._n_chunks += t.shape[0]
+            self._tail = []
+
+    def array(self) -> np.ndarray:
+        self._spill()
+        if len(self._chunks) != 1:
+            empty = np.zeros((0, self.width) if self.width else (0,), self.dtype)
+            self._chunks = [np.concatenate(self._chunks) if self._chunks else empty]
+        return self._chunks[0]
+
+    def set(self, arr) -> None:
+        a = np.array(arr, self.dtype).reshape(self._shape())
+        self._chunks = [a]
+        self._n_chunks = a.shape[0]
+        self._tail = []
+
+    def __getitem__(self, i):
+        if i >= self._n_chunks:
+            return self._tail[i - self._n_chunks]
+        return self.array()[i]
+
+
+def _resolve_device(device) -> str:
+    d = str(device)
+    if d.startswith("hip"):
+        d = "cuda" + d[3:]
+    if not d.startswith("cuda"):
+        raise ValueError(
+            f"device {device!r}: this solver runs only on the HIP device path "
+            "(use 'cuda:0' / 'hip:0'); there is no CPU implementation")
+    return d
+
+
+def pack_rows(types, a, b, rest, compliance, anchor, mu) -> tuple[np.ndarray, np.ndarray]:
+    """Reference-layout row arrays -> (int32[M,4] packed rows, float32[M,4] anchors)."""
+    m = len(types)
+    types = np.asarray(types, np.int32)
+    a = np.asarray(a, np.int32)
+    b = np.asarray(b, np.int32)
+    rest = np.asarray(rest, np.float32)
+    compliance = np.asarray(compliance, np.float32)
+    anchor = np.asarray(anchor, np.float32).reshape(-1, 3)
+    mu = np.asarray(mu, np.float32)
+    if m and int(a.max()) >= (1 << 28):
+        raise ValueError("too many particles for the packed row format (2^28)")
+    p0 = np.where(types == FLOOR, anchor[:, 1] if m else rest, rest).astype(np.float32)
+    p1 = np.where((types == FLOOR) | (types == CONTACT), mu, compliance).astype(np.float32)
+    rows = np.empty((m, 4), np.int32)
+    rows[:, 0] = a | (types << 28)
+    rows[:, 1] = b
+    rows[:, 2] = p0.view(np.int32)
+    rows[:, 3] = p1.view(np.int32)
+    anc = np.zeros((m, 4), np.float32)
+    anc[:, :3] = anchor
+    return rows, anc
+
+
+def jacobi_row_order(a, b) -> np.ndarray:
+    """Stable order of rows by smallest endpoint (the LDS-window tiling key)."""
+    a = np.asarray(a, np.int64)
+    b = np.asarray(b, np.int64)
+    key = np.where(b >= 0, np.minimum(a, b), a)
+    return np.argsort(key, kind="stable")
+
+
+def choose_jacobi_tiling(rows_sorted: np.ndarray, block: int = 256) -> tuple[int, int]:
+    """(rows_per_thread, window) for the LDS-staged Jacobi projection: the
+    smallest power-of-two window that holds the endpoint span of 90 % of the
+    tiles (the others spill to global atomics)."""
+    m = rows_sorted.shape[0]
+    if m == 0:
+        return 1, 0
+    rpt = 2 if m >= 64 * 1024 else 1
+    tile = block * rpt
+    a = rows_sorted[:, 0] & 0x0FFFFFFF
+    b = rows_sorted[:, 1]
+    lo = np.where(b >= 0, np.minimum(a, b), a)
+    hi = np.maximum(a, b)
+    nt = (m + tile - 1) // tile
+    pad = nt * tile - m
+    lo_t = np.pad(lo, (0, pad), mode="edge").reshape(nt, tile)[:, 0]
+    hi_t = np.pad(hi, (0, pad), mode="edge").reshape(nt, tile).max(axis=1)
+    span = (hi_t - lo_t + 1).astype(np.int64)
+    want = int(np.percentile(span, 90))
+    window = 256
+    while window < want and window < 4096:
+        window *= 2
+    return rpt, window
+
+
+class Solver:
+    """3D XPBD particle solver (solver.py:57-126).
+
+    Args mirror the reference: ``dt`` frame step, ``substeps``,
+    ``iterations`` per substep, ``gravity``, per-substep ``damping``,
+    ``solve_mode`` ("gs", "jacobi" or "auto" = jacobi), ``jacobi_relax``,
+    ``device`` ("cuda:N"/"hip:N"), ``max_speed`` (0 disables the clamp).
+    """
+
+    def __init__(
+        self,
+        dt: float = 1.0 / 60.0,
+        substeps: int = 10,
+        iterations: int = 1,
+        gravity: tuple = (0.0, -9.81, 0.0),
+        damping: float = 0.0,
+        solve_mode: str = "auto",
+        jacobi_relax: float = 1.0,
+        device: str = "cuda:0",
+        max_speed: float = 40.0,
+    ):
+        self.device = _resolve_device(device)
+        self.dt = float(dt)
+        self.substeps = int(substeps)
+        self.iterations = int(iterations)
+        self.gravity = tuple(float(g) for g in gravity)
+        self.damping = float(damping)
+        self.jacobi_relax = float(jacobi_relax)
+        self.max_speed = float(max_speed)
+        if solve_mode == "auto":
+            solve_mode = "jacobi"
+        assert solve_mode in ("gs", "jacobi")
+        self.solve_mode = solve_mode
+        self.use_graph = True
+
+        # host scene, one column per field (float64 as the reference's
+        # Python floats; rounded to float32 once, at flush)
+        self._x = _Col(np.float64, 3)
+        self._v = _Col(np.float64, 3)
+        self._mass = _Col(np.float64)
+        self._radius = _Col(np.float64)
+        self._collide = _Col(np.bool_)
+        self._friction = _Col(np.float64)
+        self._rt = _Col(np.int32)          # row type
+        self._ra = _Col(np.int32)          # body_a
+        self._rb = _Col(np.int32)          # body_b (-1 = none)
+        # row scalars are rounded to float32 on entry: the reference stores
+        # Python floats and rounds each once at flush, which is the same value
+        self._rrest = _Col(np.float32)
+        self._rcomp = _Col(np.float32)
+        self._ranch = _Col(np.float32, 3)
+        self._rmu = _Col(np.float32)
+
+        self._self_collision = False
+        self._contact_margin = 0.0
+        self._default_friction = 0.0
+        self._contact_pool_start = None
+        self._dirty = True
+
+        # device buffers (torch tensors), built in _flush
+        self.X = self.V = self.XP = self.ACC = None
+        self.R = self.ANC = self.LAM = None
+        self._perm = np.zeros(0, np.int64)
+        self._color_starts = np.zeros(1, dtype=np.int32)
+        self.num_colors = 0
+        self.color_counts: dict = {}
+        self._tiling = (1, 0)
+        self._plan = None
+        self._plan_sig = None
+        self._gen = 0
+
+    # ---- scene building (solver.py:130-228) ------------------------------
+    def add_particle(self, position, mass, velocity=(0.0, 0.0, 0.0), shape=None, collide=True,
+                     radius=None, friction=None) -> Body:
+        idx = len(self._x)
+        self._x.append(tuple(float(v) for v in position))
+        self._v.append(tuple(float(v) for v in velocity))
+        self._mass.append(float(mass))
+        sh = shape or Shape()
+        r = float(radius) if radius is not None else _shape_radius(sh)
+        self._radius.append(r)
+        self._collide.append(bool(collide) and r > 0.0 and mass > 0.0)
+        mu = float(friction) if friction is not None else self._default_friction
+        self._friction.append(max(0.0, mu))
+        self._dirty = True
+        return Body(index=idx, shape=sh)
+
+    def add_particles(self, positions, masses, velocities=None, radius=0.0, collide=False,
+                      friction=None) -> np.ndarray:
+        """Bulk ``add_particle`` (an extension for million-particle scenes):
+        appends len(positions) particles, returns their indices."""
+        pos = np.asarray(positions, np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        m = np.broadcast_to(np.asarray(masses, np.float64), (n,))
+        vel = (np.zeros((n, 3)) if velocities is None
+               else np.asarray(velocities, np.float64).reshape(-1, 3))
+        r = np.broadcast_to(np.asarray(radius, np.float64), (n,))
+        mu = self._default_friction if friction is None else friction
+        mu = np.maximum(0.0, np.broadcast_to(np.asarray(mu, np.float64), (n,)))
+        base = len(self._x)
+        self._x.extend(pos)
+        self._v.extend(vel)
+        self._mass.extend(m)
+        self._radius.extend(r)
+        self._collide.extend(np.asarray(collide, bool) & (r > 0.0) & (m > 0.0))
+        self._friction.extend(mu)
+        self._dirty = True
+        return np.arange(base, base + n, dtype=np.int64)
+
+    def _add_row(self, t, a, b, rest=0.0, comp=0.0, anchor=(0.0, 0.0, 0.0), mu=0.0) -> int:
+        idx = len(self._rt)
+        self._rt.append(t)
+        self._ra.append(a)
+        self._rb.append(b)
+        self._rrest.append(rest)
+        self._rcomp.append(comp)
+        self._ranch.append(anchor)
+        self._rmu.append(mu)
+        self._dirty = True
+        return idx
+
+    def add_distance(self, body_a, body_b, rest=None, compliance=0.0) -> ConstraintHandle:
+        """Compliant distance ``|x_a - x_b| = rest`` (rest None = current separation)."""
+        if rest is None:
+            pa = np.asarray(self._x[body_a.index], np.float64)
+            pb = np.asarray(self._x[body_b.index], np.float64)
+            rest = float(np.linalg.norm(pa - pb))
+        idx = self._add_row(DISTANCE, body_a.index, body_b.index, float(rest), float(compliance))
+        return ConstraintHandle(index=idx)
+
+    def add_distances(self, a, b, rest=None, compliance=0.0) -> np.ndarray:
+        """Bulk ``add_distance`` over index arrays (an extension); ``rest=None``
+        takes each pair's current separation."""
+        a = np.asarray(a, np.int64).ravel()
+        b = np.asarray(b, np.int64).ravel()
+        n = a.shape[0]
+        if rest is None:
+            x = self._x.array()
+            rest = np.linalg.norm(x[a] - x[b], axis=1)
+        base = len(self._rt)
+        self._rt.extend(np.full(n, DISTANCE, np.int32))
+        self._ra.extend(a)
+        self._rb.extend(b)
+        self._rrest.extend(np.broadcast_to(np.asarray(rest, np.float64), (n,)))
+        self._rcomp.extend(np.broadcast_to(np.asarray(compliance, np.float64), (n,)))
+        self._ranch.extend(np.zeros((n, 3), np.float32))
+        self._rmu.extend(np.zeros(n, np.float32))
+        self._dirty = True
+        return np.arange(base, base + n, dtype=np.int64)
+
+    def add_attach(self, body, world_point, compliance=0.0, rest=0.0) -> ConstraintHandle:
+        """Compliant pin of a particle to a world point."""
+        idx = self._add_row(ATTACH, body.index, -1, float(rest), float(compliance),
+                            tuple(float(p) for p in world_point))
+        return ConstraintHandle(index=idx)
+
+    def add_floor_contact(self, body, floor_y=0.0, friction=None) -> ConstraintHandle:
+        """One-sided floor at ``y = floor_y`` with Coulomb friction."""
+        mu = float(friction) if friction is not None else float(self._friction[body.index])
+        idx = self._add_row(FLOOR, body.index, -1, anchor=(0.0, float(floor_y), 0.0),
+                            mu=max(0.0, mu))
+        return ConstraintHandle(index=idx)
+
+    def add_floor_contacts(self, bodies, floor_y=0.0, friction=None) -> np.ndarray:
+        """Bulk ``add_floor_contact`` over an index array (an extension)."""
+        a = np.asarray(bodies, np.int64).ravel()
+        n = a.shape[0]
+        mu = (self._friction.array()[a] if friction is None
+              else np.broadcast_to(np.asarray(friction, np.float64), (n,)))
+        anch = np.zeros((n, 3))
+        anch[:, 1] = float(floor_y)
+        base = len(self._rt)
+        self._rt.extend(np.full(n, FLOOR))
+        self._ra.extend(a)
+        self._rb.extend(np.full(n, -1))
+        self._rrest.extend(np.zeros(n))
+        self._rcomp.extend(np.zeros(n))
+        self._ranch.extend(anch)
+        self._rmu.extend(np.maximum(0.0, mu))
+        self._dirty = True
+        return np.arange(base, base + n, dtype=np.int64)
+
+    def enable_self_collision(self, enabled=True, margin=0.0, default_friction=None) -> None:
+        """Per-frame sphere-sphere CONTACT generation between ``collide`` bodies."""
+        self._self_collision = bool(enabled)
+        self._contact_margin = float(margin)
+        if default_friction is not None:
+            self._default_friction = max(0.0, float(default_friction))
+            fr = self._friction.array().copy()
+            fr[fr == 0.0] = self._default_friction
+            self._friction.set(fr)
+        if enabled and self._contact_pool_start is None:
+            self._contact_pool_start = len(self._rt)
+
+    # ---- runtime perturbations (solver.py:232-271); in place on device ----
+    def set_position(self, body, p) -> None:
+        self._flush()
+        self.X[body.index, :3] = self._t(np.asarray(p, np.float32))
+
+    def set_velocity(self, body, vel) -> None:
+        self._flush()
+        self.V[body.index, :3] = self._t(np.asarray(vel, np.float32))
+
+    def add_impulse(self, body, dv) -> None:
+        self._flush()
+        self.V[body.index, :3] += self._t(np.asarray(dv, np.float32))
+
+    def set_all_compliance(self, value: float, only_distance: bool = True) -> None:
+        """Live compliance update; rewrites the device rows in place (no
+        reallocation, so a captured graph stays valid)."""
+        self._flush()
+        comp = self._rcomp.array().copy()
+        sel = (self._rt.array() == DISTANCE) if only_distance else slice(None)
+        comp[sel] = float(value)
+        self._rcomp.set(comp)
+        self._repack_rows_inplace()
+
+    def set_all_friction(self, mu: float) -> None:
+        self._flush()
+        self._default_friction = float(mu)
+        self._friction.set(np.full(len(self._friction), float(mu)))
+        rmu = self._rmu.array().copy()
+        t = self._rt.array()
+        rmu[(t == FLOOR) | (t == CONTACT)] = float(mu)
+        self._rmu.set(rmu)
+        self._repack_rows_inplace()
+
+    # ---- host broad phase (solver.py:275-342) ------------------------------
+    def _rebuild_contacts(self) -> None:
+        """Drop last frame's CONTACT rows and regenerate them from the current
+        positions, skipping pairs already joined by a constraint."""
+        if not self._self_collision:
+            return
+        t = self._rt.array()
+        keep_rows = t != CONTACT
+        for col in (self._rt, self._ra, self._rb, self._rrest, self._rcomp, self._ranch,
+                    self._rmu):
+            col.set(col.array()[keep_rows])
+        self._contact_pool_start = len(self._rt)
+        ra, rb = self._ra.array(), self._rb.array()
+        two = rb >= 0
+        existing = set(zip(np.minimum(ra[two], rb[two]).tolist(),
+                           np.maximum(ra[two], rb[two]).tolist()))
+        pos = (self.positions() if self.X is not None
+               else self._x.array().astype(np.float32))
+        collide = self._collide.array().astype(bool)
+        idx = np.nonzero(collide)[0]
+        if len(idx) < 2:
+            self._dirty = True
+            return
+        radii = self._radius.array().astype(np.float32)
+        friction = self._friction.array().astype(np.float32)
+        vmax = float(np.abs(self.velocities()).max()) if self.V is not None else 0.0
+        rmax = float(radii[idx].max())
+        margin = min(self._contact_margin + 3.0 * self.dt * vmax, rmax)
+        ga, gb = grid_candidate_pairs(pos[idx], float(radii[idx].max()) * 2.0 + margin)
+        if len(ga) == 0:
+            self._dirty = True
+            return
+        a_glob = idx[ga]
+        b_glob = idx[gb]
+        d = pos[a_glob] - pos[b_glob]
+        d2 = np.einsum("ij,ij->i", d, d)
+        rs = radii[a_glob] + radii[b_glob] + margin
+        keep = d2 <= rs * rs
+        a_glob = a_glob[keep]
+        b_glob = b_glob[keep]
+        mus = np.sqrt(friction[a_glob] * friction[b_glob])
+        rest = radii[a_glob] + radii[b_glob]
+        for a, b, mu, rst in zip(a_glob.tolist(), b_glob.tolist(), mus.tolist(), rest.tolist()):
+            if (min(a, b), max(a, b)) in existing:
+                continue
+            self._add_row(CONTACT, a, b, rest=float(rst), mu=float(mu))
+        self._dirty = True
+
+    # ---- flush: host lists -> device buffers (solver.py:346-408) ----------
+    def host_arrays(self) -> dict:
+        """The scene in the reference's own array layout (solver.py:358-392):
+        float32 x/v (n,3), inv_mass (n,), row type/a/b (int32), rest,
+        compliance, mu (float32) and anchor (m,3)."""
+        x_np = self._x.array().astype(np.float32)
+        v_np = self._v.array().astype(np.float32)
+        mass = self._mass.array().astype(np.float32)
+        inv_mass = np.where(mass > 0.0, 1.0 / np.maximum(mass, 1e-20), 0.0).astype(np.float32)
+        return dict(
+            x=x_np, v=v_np, inv_mass=inv_mass,
+            type=self._rt.array().astype(np.int32),
+            a=self._ra.array().astype(np.int32),
+            b=self._rb.array().astype(np.int32),
+            rest=self._rrest.array().astype(np.float32),
+            compliance=self._rcomp.array().astype(np.float32),
+            anchor=self._ranch.array().astype(np.float32),
+            mu=self._rmu.array().astype(np.float32),
+        )
+
+    def _t(self, arr):
+        import torch
+
+        return torch.as_tensor(arr).to(self.device)
+
+    def _flush(self) -> None:
+        if not self._dirty:
+            return
+        import torch
+
+        N.lib()
+        h = self.host_arrays()
+        n_b = h["x"].shape[0]
+        n_c = h["type"].shape[0]
+        x_np, v_np = h["x"], h["v"]
+        if self.X is not None:
+            prev = self.X.shape[0]
+            if 0 < prev <= n_b:
+                x_np[:prev] = self.positions()
+                v_np[:prev] = self.velocities()
+        X = np.zeros((n_b, 4), np.float32)
+        X[:, :3] = x_np
+        X[:, 3] = h["inv_mass"]
+        V = np.zeros((n_b, 4), np.float32)
+        V[:, :3] = v_np
+        rows, anc = pack_rows(h["type"], h["a"], h["b"], h["rest"], h["compliance"],
+                              h["anchor"], h["mu"])
+        if self.solve_mode == "gs" and n_c > 0:
+            color = color_constraints(n_b, h["a"], h["b"])
+            order, starts, num_colors = color_ranges(color)
+            perm = order.astype(np.int64)
+            self._color_starts = starts
+            self.num_colors = num_colors
+            self.color_counts = color_summary(color)
+        else:
+            perm = jacobi_row_order(h["a"], h["b"]) if n_c else np.zeros(0, np.int64)
+            self._color_starts = np.array([0, n_c], dtype=np.int32)
+            self.num_colors = 1 if n_c else 0
+            self.color_counts = {}
+        rows = np.ascontiguousarray(rows[perm])
+        anc = np.ascontiguousarray(anc[perm])
+        self._tiling = choose_jacobi_tiling(rows) if n_c else (1, 0)
+        dev = self.device
+        self.X = torch.from_numpy(X).to(dev)
+        self.V = torch.from_numpy(V).to(dev)
+        self.XP = torch.zeros((n_b, 4), dtype=torch.float32, device=dev)
+        self.ACC = torch.zeros((max(n_b, 1), 4), dtype=torch.float32, device=dev)
+        self.R = torch.from_numpy(rows).to(dev)
+        self.ANC = torch.from_numpy(anc).to(dev)
+        self.LAM = torch.zeros(max(n_c, 1), dtype=torch.float32, device=dev)
+        self._perm = perm
+        self._gen += 1
+        self._drop_plan()
+        self._dirty = False
+
+    def _repack_rows_inplace(self) -> None:
+        h = self.host_arrays()
+        rows, anc = pack_rows(h["type"], h["a"], h["b"], h["rest"], h["compliance"],
+                              h["anchor"], h["mu"])
+        if len(rows):
+            self.R.copy_(self._t(np.ascontiguousarray(rows[self._perm])))
+            self.ANC.copy_(self._t(np.ascontiguousarray(anc[self._perm])))
+
+    # ---- the step (solver.py:412-467) --------------------------------------
+    def _signature(self):
+        return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
+                self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling)
+
+    def _drop_plan(self):
+        if self._plan is not None:
+            N.lib().xpbd_frame3_destroy(self._plan)
+        self._plan = None
+        self._plan_sig = None
+
+    def _get_plan(self):
+        sig = self._signature()
+        if self._plan is not None and self._plan_sig == sig:
+            return self._plan
+        self._drop_plan()
+        n_b = int(self.X.shape[0])
+        n_c = len(self._rt)
+        d = N.ParticlesDesc()
+        d.x, d.v, d.x_prev, d.acc = (N.ptr(self.X), N.ptr(self.V), N.ptr(self.XP),
+                                     N.ptr(self.ACC))
+        d.rows, d.anchors, d.lam = N.ptr(self.R), N.ptr(self.ANC), N.ptr(self.LAM)
+        d.n_particles, d.n_rows = n_b, n_c
+        self._starts_buf = np.ascontiguousarray(self._color_starts, dtype=np.int32)
+        d.color_starts = self._starts_buf.ctypes.data
+        d.num_colors = int(len(self._starts_buf) - 1)
+        d.mode = MODE_GS if self.solve_mode == "gs" else MODE_JACOBI
+        d.substeps, d.iterations = self.substeps, self.iterations
+        d.h = float(np.float32(self.dt / float(self.substeps)))
+        d.gx, d.gy, d.gz = self.gravity
+        d.damping, d.relax, d.max_speed = self.damping, self.jacobi_relax, self.max_speed
+        d.rows_per_thread, d.window = self._tiling
+        handle = N.c_void_p()
+        N.check(N.lib().xpbd_frame3_create(N.ctypes.byref(d), N.ctypes.byref(handle)),
+                "xpbd_frame3_create")
+        self._plan = handle
+        self._plan_sig = sig
+        return handle
+
+    def step(self, frames: int = 1) -> None:
+        """Advance ``frames`` frames of ``dt`` (default one, as the reference).
+
+        Without self-collision the constraint set is static and all frames go
+        out as replays of one captured graph.  With self-collision the CONTACT
+        rows are rebuilt on the host before every frame (solver.py:413-416),
+        so each frame is issued eagerly."""
+        for _ in range(int(frames) if self._self_collision else 1):
+            if self._self_collision:
+                self._flush()
+                self._sanitize_positions()
+                self._rebuild_contacts()
+            self._flush()
+            if len(self._x) == 0:
+                return
+            plan = self._get_plan()
+            n_frames = 1 if self._self_collision else int(frames)
+            graph = int(self.use_graph and not self._self_collision)
+            N.check(N.lib().xpbd_frame3_run(plan, N.current_stream(), n_frames, graph),
+                    "xpbd_frame3_run")
+
+    def _sanitize_positions(self) -> None:
+        """Reset non-finite positions to their initial scene values and zero
+        their velocity (solver.py:469-483)."""
+        if self.X is None:
+            return
+        xs = self.positions()
+        if np.all(np.isfinite(xs)):
+            return
+        bad = np.nonzero(~np.all(np.isfinite(xs), axis=1))[0]
+        safe = self._x.array().astype(np.float32)
+        for i in bad.tolist():
+            self.X[i, :3] = self._t(safe[i])
+            self.V[i, :3] = 0.0
+
+    # ---- read-back (solver.py:487-508) ---------------------------------------
+    def positions(self) -> np.ndarray:
+        if self.X is None:
+            return self._x.array().astype(np.float32)
+        return self.X[:, :3].cpu().numpy().copy()
+
+    def velocities(self) -> np.ndarray:
+        if self.V is None:
+            return self._v.array().astype(np.float32)
+        return self.V[:, :3].cpu().numpy().copy()
+
+    def lambdas(self) -> np.ndarray:
+        """Multipliers in API row order."""
+        n_c = len(self._rt)
+        if self.LAM is None or self._dirty:
+            return np.zeros(n_c, dtype=np.float32)
+        out = np.zeros(n_c, dtype=np.float32)
+        out[self._perm] = self.LAM[:n_c].cpu().numpy()
+        return out
+
+    @property
+    def num_bodies(self) -> int:
+        return len(self._x)
+
+    @property
+    def num_constraints(self) -> int:
+        return len(self._rt)
+
+    def __del__(self):
+        try:
+            self._drop_plan()
+        except Exception:
+            pass
+
+
+# 13 forward neighbour cell offsets (half of the 26-cell shell) plus the cell
+# itself: every unordered pair of distinct or equal cells is visited once.
+# The order fixes CONTACT row order and therefore coloring and GS results
+# (solver.py:514-517).
+_FWD_OFFSETS = (
+    (1, -1, -1), (1, -1, 0), (1, -1, 1), (1, 0, -1), (1, 0, 0), (1, 0, 1),
+    (1, 1, -1), (1, 1, 0), (1, 1, 1), (0, 1, -1), (0, 1, 0), (0, 1, 1), (0, 0, 1),
+)
+
+
+def grid_candidate_pairs(p: np.ndarray, cell: float) -> tuple[np.ndarray, np.ndarray]:
+    """Candidate pairs of bodies in the same or an adjacent grid cell of side
+    ``cell`` (solver.py:520-562), in the reference's enumeration order: offset
+    by offset, body by body, partners in ascending index within a cell."""
+    m = p.shape[0]
+    if m < 2 or cell <= 0.0:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    c = np.floor(p / cell).astype(np.int64)
+    c -= c.min(axis=0)
+    S = int(c.max()) + 2
+    cid = (c[:, 0] * S + c[:, 1]) * S + c[:, 2]
+    # bucket bodies by cell id: members of a cell in ascending body index
+    by_cell = np.argsort(cid, kind="stable")
+    sorted_cid = cid[by_cell]
+    bodies = np.arange(m, dtype=np.int64)
+    out_a, out_b = [], []
+    for off in ((0, 0, 0),) + _FWD_OFFSETS:
+        target = cid + (off[0] * S + off[1]) * S + off[2]
+        first = np.searchsorted(sorted_cid, target, "left")
+        cnt = np.searchsorted(sorted_cid, target, "right") - first
+        total = int(cnt.sum())
+        if total == 0:
+            continue
+        owner = np.repeat(bodies, cnt)
+        run_start = np.repeat(np.cumsum(cnt) - cnt, cnt)
+        partner = by_cell[np.repeat(first, cnt) + (np.arange(total) - run_start)]
+        if off == (0, 0, 0):
+            sel = partner > owner
+            owner, partner = owner[sel], partner[sel]
+        out_a.append(owner)
+        out_b.append(partner)
+    if not out_a:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    return np.concatenate(out_a), np.concatenate(out_b)
+
+
+def _shape_radius(shape) -> float:
+    """Bounding-sphere radius used by the sphere broad phase (solver.py:565-575)."""
+    if shape is None:
+        return 0.0
+    if shape.kind == "sphere":
+        return float(shape.size[0])
+    if shape.kind == "cube":
+        return float(min(shape.size))
+    if shape.kind == "pillar":
+        return float(shape.size[0])
+    return 0.0

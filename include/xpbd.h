@@ -34,9 +34,14 @@ extern "C" {
 
 #define XPBD_MODE_GS 0
 #define XPBD_MODE_JACOBI 1
+/* Jacobi with one fused launch per iteration; after a frame the positions
+ * are in x and the multipliers in lam (iterations even) or lam_alt (odd). */
+#define XPBD_MODE_JACOBI_TILE 2
 
 /* Largest LDS accumulator window (particles) of the Jacobi projection. */
 #define XPBD_MAX_WINDOW 8192
+/* Largest particle tile of the fused Jacobi iteration (float4 per particle in LDS). */
+#define XPBD_MAX_TILE 8192
 
 /* ---- 3-DOF particle kernels ------------------------------------------- */
 
@@ -69,6 +74,46 @@ int xpbd_solve_jacobi(void* stream, const float* X, const float* XP, const int* 
                       const float* ANC, float* LAM, float* ACC, float h, int first, int n_rows,
                       int rows_per_thread, int window);
 
+/* One full Jacobi iteration, kernels.py:219-339 `solve_constraints_jacobi`
+ * + `apply_jacobi` fused (solver.py:458-467), over particle tiles of
+ * tile_size particles: reads Xin / LAMin, writes Xout / LAMout (distinct
+ * buffers).  Rows sorted by smallest endpoint; tile_row_start[t] is the first
+ * row whose smallest endpoint lies in tile t (n_tiles + 1 entries);
+ * halo_rows[halo_start[t] .. halo_start[t+1]) lists the rows whose larger
+ * endpoint lies in tile t and smaller endpoint in an earlier tile.
+ * finalize != 0 also applies finalize_velocity + cap_velocity
+ * (kernels.py:346-371) and writes V.  ACC (optional, float4[n]) holds sums
+ * from xpbd_solve_jacobi_dyn of this iteration; they join the average and
+ * are cleared. */
+int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* XP, float* V,
+                     const int* R, const float* ANC, const float* LAMin, float* LAMout,
+                     const int* tile_row_start, const int* halo_start, const int* halo_rows,
+                     float h, int first, int tile_size, int n_particles, float relax,
+                     int finalize, float damping, float max_speed, float* ACC);
+
+/* kernels.py:49-73 `integrate` reading Xin and writing Xout (may alias);
+ * gscale (optional, float[n]) scales gravity per particle (batched scenes). */
+int xpbd_integrate_to(void* stream, const float* Xin, float* Xout, float* V, float* XP, float gx,
+                      float gy, float gz, float h, const float* gscale, int n);
+
+/* solve_constraints_jacobi (kernels.py:219-322) over rows [0, min(*n_rows_dev,
+ * cap)) of a segment generated on the device (CONTACT rows), accumulating
+ * into ACC with global atomics. */
+int xpbd_solve_jacobi_dyn(void* stream, const float* X, const float* XP, const int* R,
+                          float* LAM, float* ACC, float h, int first, const int* n_rows_dev,
+                          int cap);
+
+/* Device CONTACT generation, Solver._rebuild_contacts (solver.py:275-342):
+ * margin = min(contact_margin + 3 dt max|v|, r_max), cell = 2 r_max + margin,
+ * pairs a < b in neighbouring cells with |xa - xb|^2 <= (ra + rb + margin)^2,
+ * rows CONTACT(a, b, rest ra + rb, mu sqrt(mu_a mu_b)) in (a, b) order into
+ * ROWS (int4, clamped to cap); the full count goes to count[0].  radius[i] = 0
+ * excludes particle i. */
+long long xpbd_contacts_workspace_bytes(int n);
+int xpbd_sphere_contacts(void* stream, void* ws, const float* X, const float* V,
+                         const float* radius, const float* mu, float contact_margin, float dt,
+                         float r_max, int n, int cap, int* ROWS, int* count);
+
 /* kernels.py:105-208 `solve_constraints_color` over rows [start, start+count)
  * of a color-sorted row array, launched per color at solver.py:444-456. */
 int xpbd_solve_color(void* stream, float* X, const float* XP, const int* R, const float* ANC,
@@ -98,6 +143,20 @@ typedef struct xpbd_particles_desc {
   float max_speed;       /* <= 0 or non-finite disables the cap */
   int rows_per_thread;   /* Jacobi tile shape */
   int window;
+  /* fused-tile Jacobi (mode XPBD_MODE_JACOBI_TILE) */
+  float* x_alt;          /* float4[n_particles], second position buffer */
+  float* lam_alt;        /* float[n_rows], second multiplier buffer */
+  const int* tile_row_start; /* int[n_tiles + 1] */
+  const int* halo_start;     /* int[n_tiles + 1] */
+  const int* halo_rows;      /* int[halo_start[n_tiles]] */
+  int tile_size;
+  /* device-generated row segment (tile mode): projected each iteration with
+   * global atomics into acc, merged by the tile kernel's apply */
+  const int* dyn_rows;   /* int4[dyn_cap] */
+  float* dyn_lam;        /* float[dyn_cap] */
+  const int* dyn_count;  /* int[1], device */
+  int dyn_cap;
+  const float* gscale;   /* float[n_particles] gravity factor, or NULL (tile mode) */
 } xpbd_particles_desc;
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);
@@ -105,6 +164,115 @@ int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);
  * into a hipGraph (replacing the per-launch loop of solver.py:426-442). */
 int xpbd_frame3_run(void* handle, void* stream, int frames, int use_graph);
 int xpbd_frame3_destroy(void* handle);
+
+/* ---- 6-DOF rigid bodies (src/xpbd3d/kernels_6dof.py) --------------------
+ * Body buffers are float4[n]: X (x, inv_mass), Q (quat xyzw), V, W (v, omega),
+ * XP, QP (substep start), IB (inverse body inertia diagonal), HE (half
+ * extents, .w = collision group as int bits); accumulators DX (dx, count),
+ * DR (d-rotation), DV (dv, count), DW (d-omega); LAMF float[8n].
+ * Pairs int2[cap] with the live count in PC[0]; manifold points are
+ * compacted: PTI int4 (pair, incident, reference, slot), PTN/PTOI/PTOR
+ * float4, LAMP float, live count PTC[0], capacity 8*cap; MC int[cap] holds
+ * the points per pair.  Joints: JI int2 (a, b), JA float4 (anchor_a, rest),
+ * JB float4 (anchor_b, compliance), LAMJ float. */
+
+/* kernels_6dof.py:67-102 `integrate_bodies` (solver_6dof.py:370-373); also
+ * zeroes LAMF and PTC[0] (the lambda resets of solver_6dof.py:374-375). */
+int xpbd6_integrate(void* stream, float* X, float* Q, float* V, float* W, float* XP, float* QP,
+                    const float* IB, float* LAMF, int* PTC, float gx, float gy, float gz,
+                    float h, int gyro, int n);
+/* kernels_6dof.py:233-382 `generate_box_manifold` (solver_6dof.py:378-382). */
+int xpbd6_box_manifold(void* stream, const float* X, const float* Q, const float* HE,
+                       const int* PAIRS, const int* PC, float margin, int cap, int* MC, int* PTI,
+                       float* PTN, float* PTOI, float* PTOR, float* LAMP, int* PTC);
+/* kernels_6dof.py:440-483 `solve_joints` (solver_6dof.py:385-390); first != 0
+ * resets lambda (solver_6dof.py:376-377). */
+int xpbd6_solve_joints(void* stream, const float* X, const float* Q, const float* IB,
+                       const int* JI, const float* JA, const float* JB, float* LAMJ, float h,
+                       int first, float* DX, float* DR, int nj);
+/* kernels_6dof.py:158-211 `solve_floor_contacts` (solver_6dof.py:391-395). */
+int xpbd6_solve_floor(void* stream, const float* X, const float* Q, const float* XP,
+                      const float* QP, const float* IB, const float* HE, float floor_y,
+                      float mu_s, float* LAMF, float* DX, float* DR, int n);
+/* kernels_6dof.py:385-430 `solve_box_manifold` (solver_6dof.py:396-401). */
+int xpbd6_solve_box(void* stream, const float* X, const float* Q, const float* IB,
+                    const int* PTI, const float* PTN, const float* PTOI, const float* PTOR,
+                    float* LAMP, const int* PTC, float* DX, float* DR, int max_points);
+/* kernels_6dof.py:486-506 `apply_jacobi_6dof` (solver_6dof.py:402-404). */
+int xpbd6_apply_jacobi(void* stream, float* X, float* Q, float* DX, float* DR, float relax,
+                       int n);
+/* kernels_6dof.py:512-535 `finalize_velocity_6dof` (solver_6dof.py:405-408). */
+int xpbd6_finalize_velocity(void* stream, const float* X, const float* Q, const float* XP,
+                            const float* QP, float* V, float* W, float h, float lin_damp,
+                            float ang_damp, int n);
+/* kernels_6dof.py:541-590 `velocity_floor` (solver_6dof.py:409-413). */
+int xpbd6_velocity_floor(void* stream, const float* X, const float* Q, const float* IB,
+                         const float* HE, float mu_d, float restitution, const float* LAMF,
+                         float h, const float* V, const float* W, float* DV, float* DW, int n);
+/* kernels_6dof.py:593-659 `velocity_box` (solver_6dof.py:414-419). */
+int xpbd6_velocity_box(void* stream, const float* X, const float* Q, const float* IB,
+                       const int* PTI, const float* PTN, const float* PTOI, const float* PTOR,
+                       const float* LAMP, const int* PTC, float mu_d, float restitution, float h,
+                       const float* V, const float* W, float* DV, float* DW, int max_points);
+/* kernels_6dof.py:662-691 `apply_velocity_6dof` + `cap_velocity_6dof`
+ * (solver_6dof.py:420-426). */
+int xpbd6_apply_velocity(void* stream, const float* X, float* V, float* W, float* DV, float* DW,
+                         float max_lin, float max_ang, int do_cap, int n);
+
+typedef struct xpbd_rigid_desc {
+  float *x, *q, *v, *w, *x_prev, *q_prev; /* float4[n] */
+  const float *inv_I, *he;                 /* float4[n] */
+  float *dx, *drot, *dv, *dw;              /* float4[n], zero on entry */
+  float* lam_floor;                        /* float[8n] */
+  int n;
+  const int* pairs;                        /* int2[cap] */
+  const int* pair_count;                   /* int[1], device */
+  int cap;
+  int* m_count;                            /* int[cap] */
+  int* pt_idx;                             /* int4[8cap] */
+  float *pt_n, *pt_oi, *pt_or;             /* float4[8cap] */
+  float* lam_pt;                           /* float[8cap] */
+  int* pt_count;                           /* int[1] */
+  const int* j_idx;                        /* int2[nj] */
+  const float *j_a, *j_b;                  /* float4[nj] */
+  float* lam_j;                            /* float[nj] */
+  int nj;
+  int substeps, iterations, gyroscopic;
+  float h, gx, gy, gz, floor_y, friction, restitution, lin_damp, ang_damp, relax;
+  float max_speed, max_omega, man_margin;
+} xpbd_rigid_desc;
+
+/* Solver6DOF._record (solver_6dof.py:357-426) as a plan; run() replays one
+ * hipGraph per frame (solver_6dof.py:519-527). */
+int xpbd_frame6_create(const xpbd_rigid_desc* desc, void** handle);
+int xpbd_frame6_run(void* handle, void* stream, int frames, int use_graph);
+int xpbd_frame6_destroy(void* handle);
+
+/* ---- 6-DOF broad phase (kernels_bvh.py, kernels_hashgrid.py) ------------
+ * Both write the candidate pairs (i < j, (i, j) ascending) into PAIRS
+ * (clamped to cap) and the full count into PC[0]; the caller reads PC[0],
+ * grows cap and calls again on overflow (solver_6dof.py:443-453). */
+
+/* kernels_bvh.py:21-48 `compute_body_aabb`: LO/HI float4[n]. */
+int xpbd6_body_aabb(void* stream, const float* X, const float* Q, const float* V, const float* W,
+                    const float* HE, float man_margin, float dt, float* LO, float* HI, int n);
+/* Bytes of the caller-owned workspace for n bodies. */
+long long xpbd6_lbvh_workspace_bytes(int n);
+long long xpbd6_hashgrid_workspace_bytes(int n);
+/* compute_body_aabb + wp.Bvh(lbvh).rebuild + emit_pairs
+ * (solver_6dof.py:428-454; kernels_bvh.py:52-85). */
+int xpbd6_broadphase_lbvh(void* stream, void* ws, const float* X, const float* Q, const float* V,
+                          const float* W, const float* HE, float man_margin, float dt, int n,
+                          int cap, int* PAIRS, int* PC);
+/* wp.HashGrid.build + emit_pairs_hashgrid (solver_6dof.py:456-479;
+ * kernels_hashgrid.py:37-88); cell = 2 rs_max + margin (solver_6dof.py:335-337). */
+int xpbd6_broadphase_hashgrid(void* stream, void* ws, const float* X, const float* Q,
+                              const float* V, const float* W, const float* HE, float man_margin,
+                              float dt, float rs_max, float cell, int n, int cap, int* PAIRS,
+                              int* PC);
+/* 30-bit Morton code of a point in [0,1)^3 (10 bits per axis, x highest);
+ * no reference counterpart (Warp builds its LBVH internally). */
+unsigned xpbd_morton3(float x, float y, float z);
 
 /* ---- host coloring (src/xpbd3d/coloring.py) ----------------------------- */
 

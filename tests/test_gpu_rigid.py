@@ -1,0 +1,411 @@
+"""GPU tests of the 6-DOF solver: parity of the HIP path with the CPU oracle
+(bit-exact candidate pair sets and per-pair contact counts; body state per
+substep from identical inputs within 1e-4, since the Jacobi accumulation
+order differs), the broad phases against brute force, and the reference's
+invariant tests (tests/test_solver_6dof.py) on the device."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+G = (0.0, -9.81, 0.0)
+
+
+def _rel(g, o):
+    g = np.asarray(g, np.float64)
+    o = np.asarray(o, np.float64)
+    return float(np.abs(g - o).max()) / max(float(np.abs(o).max()), 1e-30)
+
+
+def _oracle6(oracle, s, broadphase=None):
+    return oracle.Scene6(s.host_arrays(), dt=s.dt, substeps=s.substeps, iterations=s.iterations,
+                         gravity=s.gravity, floor_y=s.floor_y, friction=s.friction,
+                         restitution=s.restitution, lin_damp=s.lin_damp, ang_damp=s.ang_damp,
+                         gyroscopic=s.gyroscopic, relax=s.jacobi_relax, max_speed=s.max_speed,
+                         max_omega=s.max_omega, broadphase=broadphase or s.broadphase)
+
+
+def _load_state(s, orc):
+    import torch
+
+    b = s._buf
+    b["X"][:, :3] = torch.from_numpy(orc.x).to(DEV)
+    b["Q"][:] = torch.from_numpy(orc.q).to(DEV)
+    b["V"][:, :3] = torch.from_numpy(orc.v).to(DEV)
+    b["W"][:, :3] = torch.from_numpy(orc.w).to(DEV)
+
+
+def _random_boxes(n, seed, spread=3.0):
+    from xpbd_amd import Solver6DOF
+    from xpbd_amd.scenes import _random_quats
+
+    rng = np.random.default_rng(seed)
+    s = Solver6DOF(dt=1 / 60, substeps=1, iterations=1, device=DEV)
+    q = _random_quats(rng, n)
+    for i in range(n):
+        he = rng.uniform(0.05, 0.2, 3)
+        s.add_box(tuple(rng.uniform(-spread, spread, 3) + [0, spread + 1, 0]), tuple(he),
+                  mass=0.0 if i % 17 == 0 else 1.0, quaternion=tuple(q[i]),
+                  velocity=tuple(rng.normal(size=3)), omega=tuple(rng.normal(size=3)))
+    return s
+
+
+@pytest.mark.parametrize("broadphase", ["lbvh", "hashgrid"])
+@pytest.mark.parametrize("n,seed", [(2, 0), (300, 1), (3000, 2)])
+def test_broadphase_pair_set_bit_exact(oracle, broadphase, n, seed):
+    """The device pair list equals the oracle's brute-force list: same pairs,
+    same (i, j) order, same count."""
+    s = _random_boxes(n, seed, spread=3.0 if n > 100 else 0.2)
+    s.broadphase = broadphase
+    s._flush()
+    orc = _oracle6(oracle, s)
+    orc.broadphase_pairs()
+    cnt = s._run_broadphase(n, s.cap)
+    if cnt > s.cap:
+        s.cap = int(cnt * 1.5)
+        s._alloc_pair_buffers(s.cap)
+        cnt = s._run_broadphase(n, s.cap)
+    s.n_pairs = cnt
+    want = np.stack([orc.pair_a[:orc.n_pairs], orc.pair_b[:orc.n_pairs]], axis=1)
+    assert cnt == orc.n_pairs
+    assert (s.pairs() == want).all()
+
+
+def test_broadphase_overflow_regrows():
+    """A dense pile with a tiny pair capacity: the frame detects the overflow
+    from the device count, grows the buffers and re-emits (solver_6dof.py:443-453)."""
+    from xpbd_amd import Solver6DOF
+    from xpbd_amd.scenes import box_pile
+
+    s = box_pile(Solver6DOF, 6, 4, 6, substeps=4, device=DEV)
+    s._flush()
+    s.cap = 8
+    s._alloc_pair_buffers(8)
+    s.step()
+    assert s.n_pairs > 8 and s.cap >= s.n_pairs
+    assert len(s.pairs()) == s.n_pairs
+
+
+def test_morton_codes_decode(oracle):
+    """Morton codes decode to the quantised coordinates (10 bits per axis,
+    x in the highest bit of each triple)."""
+    from xpbd_amd import _native as N
+
+    lib = N.lib()
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        p = rng.random(3).astype(np.float32)
+        code = lib.xpbd_morton3(*map(float, p))
+        q = np.minimum((p * 1024).astype(np.int64), 1023)
+        dec = [0, 0, 0]
+        for bit in range(10):
+            for ax in range(3):
+                dec[ax] |= ((code >> (3 * bit + 2 - ax)) & 1) << bit
+        assert dec == q.tolist()
+    assert lib.xpbd_morton3(0.0, 0.0, 0.0) == 0
+    assert lib.xpbd_morton3(1.0, 1.0, 1.0) == (1 << 30) - 1
+
+
+def _pile_scene(seed=0):
+    from xpbd_amd import Solver6DOF
+    from xpbd_amd.scenes import _random_quats
+
+    rng = np.random.default_rng(seed)
+    s = Solver6DOF(dt=1 / 60, substeps=8, iterations=2, device=DEV, friction=0.5,
+                   restitution=0.2)
+    q = _random_quats(rng, 60)
+    k = 0
+    for iy in range(3):
+        for ix in range(5):
+            for iz in range(4):
+                he = rng.uniform(0.1, 0.25, 3)
+                s.add_box((ix * 0.55, 0.3 + iy * 0.5, iz * 0.55), tuple(he),
+                          mass=float(8000 * he.prod()), quaternion=tuple(q[k]))
+                k += 1
+    a = s.add_box((4.0, 3.0, 0.0), (0.05, 0.05, 0.05), mass=0.0, static=True)
+    prev = a
+    for i in range(5):
+        b = s.add_box((4.0 + 0.2 * (i + 1), 3.0, 0.0), (0.1, 0.03, 0.03), mass=0.5)
+        s.add_joint(prev, b, compliance=1e-5)
+        prev = b
+    return s
+
+
+def test_rigid_substep_parity_and_contact_counts(oracle):
+    """From identical inputs (the oracle's trajectory), one substep on the
+    device matches the oracle: pair lists and per-pair contact counts
+    bit-exact, x/q/v/w within 1e-4."""
+    s_full = _pile_scene()
+    orc = _oracle6(oracle, s_full)
+    h = float(np.float32(1 / 60 / 8))
+    s = _pile_scene()
+    s.dt, s.substeps = h, 1
+    s._flush()
+    one = _oracle6(oracle, s)
+    total_points = 0
+    for k in range(6):
+        orc.step(5)
+        for name in ("x", "q", "v", "w"):
+            getattr(one, name)[:] = getattr(orc, name)
+        _load_state(s, one)
+        s.step()
+        one.step(1)
+        assert s.n_pairs == one.n_pairs
+        want = np.stack([one.pair_a[:one.n_pairs], one.pair_b[:one.n_pairs]], axis=1)
+        assert (s.pairs() == want).all()
+        mc = s.contact_counts()
+        assert (mc == one.m_count[:one.n_pairs]).all(), k
+        total_points += int(mc.sum())
+        errs = [_rel(s.positions(), one.x), _rel(s.orientations(), one.q),
+                _rel(s.velocities(), one.v), _rel(s.omega.numpy(), one.w)]
+        assert max(errs) <= 1e-4, (k, errs)
+    assert total_points > 0
+
+
+def test_rigid_frames_track_oracle(oracle):
+    """Whole frames (8 substeps x 2 iterations, graph replay) over 20 frames:
+    the device trajectory stays close to the oracle's."""
+    s = _pile_scene(1)
+    orc = _oracle6(oracle, s)
+    for f in range(20):
+        s.step()
+        orc.step(1)
+    assert _rel(s.positions(), orc.x) < 2e-3
+
+
+# ---- the reference's invariant tests (tests/test_solver_6dof.py) -----------
+def test_free_fall():
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=10, iterations=1, device=DEV, floor_y=-100.0)
+    s.add_box((0, 0, 0), (0.2, 0.2, 0.2), mass=1.0)
+    for _ in range(60):
+        s.step()
+    assert abs(s.positions()[0, 1] - 0.5 * -9.81) < 0.05
+    assert abs(s.velocities()[0, 1] - (-9.81)) < 0.05
+
+
+def test_box_rests_on_floor():
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=15, iterations=1, device=DEV, floor_y=0.0, friction=0.6)
+    s.add_box((0, 1.0, 0), (0.2, 0.2, 0.2), mass=1.0)
+    for _ in range(180):
+        s.step()
+    assert abs(s.positions()[0, 1] - 0.2) < 2e-3
+    assert abs(s.velocities()[0, 1]) < 1e-2
+    assert abs(abs(s.orientations()[0, 3]) - 1.0) < 1e-2
+
+
+def test_spin_conservation():
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=10, iterations=1, device=DEV, gravity=(0, 0, 0),
+                   floor_y=-100.0)
+    s.add_box((0, 0, 0), (0.2, 0.1, 0.3), mass=1.0)
+    s._flush()
+    om = s.omega.numpy().copy()
+    om[0] = [0, 5.0, 0]
+    s.omega = om
+    s._graph = None
+    for _ in range(120):
+        s.step()
+    w = s.omega.numpy()[0]
+    assert abs(w[1] - 5.0) < 0.5
+    assert np.linalg.norm(w[[0, 2]]) < 0.5
+
+
+@pytest.mark.parametrize("n", [3, 5])
+def test_stack_settles_no_penetration(n):
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=20, iterations=1, device=DEV, floor_y=0.0, friction=0.6)
+    h = 0.2
+    for k in range(n):
+        s.add_box((0, h + k * (2 * h + 0.02), 0), (h, h, h), mass=1.0)
+    for _ in range(350):
+        s.step()
+    ys = np.sort(s.positions()[:, 1])
+    assert np.all(np.isfinite(s.positions()))
+    assert ys.max() < n * (2 * h) + 0.2
+    assert np.allclose(np.diff(ys), 2 * h, atol=8e-3)
+    assert np.abs(s.velocities()).max() < 0.1
+
+
+def test_pile_stable():
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=18, iterations=1, device=DEV, floor_y=0.0, friction=0.6)
+    h = 0.16
+    rng = np.random.default_rng(0)
+    for ly in range(3):
+        for ix in range(3):
+            for iz in range(3):
+                s.add_box((ix * 0.34 + rng.uniform(-2e-3, 2e-3), h + ly * 0.34,
+                           iz * 0.34 + rng.uniform(-2e-3, 2e-3)), (h, h, h), mass=1.0)
+    for _ in range(400):
+        s.step()
+    p = s.positions()
+    assert np.all(np.isfinite(p))
+    assert p[:, 1].max() < 1.2
+    assert np.abs(s.velocities()).max() < 0.2
+
+
+def test_box_tips_and_rests_flat():
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=20, iterations=1, device=DEV, floor_y=0.0, friction=0.7)
+    hw, hh, hd = 0.05, 0.25, 0.12
+    s.add_box((0, hh, 0), (hw, hh, hd), mass=1.0)
+    s._flush()
+    om = s.omega.numpy().copy()
+    om[0] = [0, 0, -3.0]
+    s.omega = om
+    s._graph = None
+    for _ in range(300):
+        s.step()
+    assert abs(s.positions()[0, 1] - hw) < 0.03
+    assert np.linalg.norm(s.omega.numpy()[0]) < 0.2
+
+
+def test_broadphases_agree():
+    from xpbd_amd import Solver6DOF
+
+    def run(bp):
+        s = Solver6DOF(dt=1 / 60, substeps=20, iterations=1, device=DEV, floor_y=0.0,
+                       friction=0.6, broadphase=bp)
+        h = 0.2
+        for k in range(4):
+            s.add_box((0, h + k * (2 * h + 0.02), 0), (h, h, h), mass=1.0)
+        for _ in range(300):
+            s.step()
+        return np.sort(s.positions()[:, 1])
+
+    ref = run("lbvh")
+    assert np.allclose(run("grid"), ref, atol=5e-3)
+    assert np.allclose(run("hashgrid"), ref, atol=5e-3)
+
+
+def test_joint_rigid_link():
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=20, iterations=2, device=DEV, floor_y=-100.0)
+    anchor = s.add_box((0, 2, 0), (0.05, 0.05, 0.05), mass=0.0, static=True)
+    bob = s.add_box((0.6, 2, 0), (0.1, 0.1, 0.1), mass=1.0)
+    s.add_joint(anchor, bob, compliance=0.0)
+    for _ in range(300):
+        s.step()
+    p = s.positions()
+    assert np.all(np.isfinite(p))
+    assert abs(np.linalg.norm(p[bob.index] - p[anchor.index]) - 0.6) < 0.02
+    assert p[bob.index, 1] < p[anchor.index, 1]
+
+
+def test_particle_rests_on_floor():
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=15, iterations=1, device=DEV, floor_y=0.0, friction=0.5)
+    r = 0.05
+    s.add_particle((0, 1.0, 0), mass=0.1, radius=r)
+    for _ in range(200):
+        s.step()
+    assert abs(s.positions()[0, 1] - r) < 5e-3
+    assert abs(s.velocities()[0, 1]) < 1e-2
+
+
+def test_cloth_hangs_from_pins():
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=15, iterations=2, device=DEV, floor_y=-100.0)
+    R, sp = 8, 0.1
+    idx = np.empty((R, R), np.int64)
+    for iy in range(R):
+        for ix in range(R):
+            pinned = iy == 0
+            b = s.add_particle((ix * sp, 2.0, iy * sp), mass=0.0 if pinned else 0.02,
+                               radius=0.01, group=1, static=pinned)
+            idx[iy, ix] = b.index
+    for iy in range(R):
+        for ix in range(R):
+            if ix + 1 < R:
+                s.add_joint(int(idx[iy, ix]), int(idx[iy, ix + 1]), compliance=1e-7)
+            if iy + 1 < R:
+                s.add_joint(int(idx[iy, ix]), int(idx[iy + 1, ix]), compliance=1e-7)
+    for _ in range(250):
+        s.step()
+    P = s.positions()
+    assert np.all(np.isfinite(P))
+    assert abs(P[idx[0]][:, 1].mean() - 2.0) < 1e-3
+    assert P[idx[R - 1]][:, 1].mean() < 1.85
+    assert abs(np.linalg.norm(P[idx[4, 1]] - P[idx[4, 0]]) - sp) < 0.02
+
+
+def test_unified_rigid_cloth_joints():
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=18, iterations=2, device=DEV, floor_y=0.0, friction=0.6,
+                   lin_damp=0.01)
+    for k in range(3):
+        s.add_box((0, 0.2 + k * 0.42, 0), (0.2, 0.2, 0.2), mass=1.0)
+    R, sp = 6, 0.12
+    idx = np.empty((R, R), np.int64)
+    for iy in range(R):
+        for ix in range(R):
+            pinned = iy == 0
+            b = s.add_particle((1.5 + ix * sp, 1.5, iy * sp), mass=0.0 if pinned else 0.02,
+                               radius=0.01, group=1, static=pinned)
+            idx[iy, ix] = b.index
+    for iy in range(R):
+        for ix in range(R):
+            if ix + 1 < R:
+                s.add_joint(int(idx[iy, ix]), int(idx[iy, ix + 1]), compliance=1e-6)
+            if iy + 1 < R:
+                s.add_joint(int(idx[iy, ix]), int(idx[iy + 1, ix]), compliance=1e-6)
+    assert s.num_joints > 0
+    for _ in range(300):
+        s.step()
+    P = s.positions()
+    assert np.all(np.isfinite(P))
+    ys = np.sort(P[:3, 1])
+    assert np.allclose(np.diff(ys), 0.4, atol=1e-2)
+    assert np.abs(s.velocities()).max() < 1.0
+
+
+def test_domino_cascade():
+    from xpbd_amd import Solver6DOF
+
+    s = Solver6DOF(dt=1 / 60, substeps=20, iterations=1, device=DEV, floor_y=0.0, friction=0.6)
+    hw, hh, hd = 0.04, 0.22, 0.12
+    sp = 0.18
+    n = 6
+    ang = -0.45
+    q0 = (0.0, 0.0, float(np.sin(ang / 2)), float(np.cos(ang / 2)))
+    for k in range(n):
+        q = q0 if k == 0 else (0, 0, 0, 1)
+        y = hh * 0.9 if k == 0 else hh
+        s.add_box((k * sp, y, 0.0), (hw, hh, hd), mass=1.0, quaternion=q)
+    for _ in range(420):
+        s.step()
+    fell = int(np.sum(s.positions()[:, 1] < hh * 0.6))
+    assert fell >= n - 1
+
+
+def test_unified_config3_runs_and_settles_contacts(oracle):
+    """Config 3 (1440 boxes + 32^2 cloth + 20-link chain, LBVH, 15x2): frames
+    run through the captured graph, stay finite, and the first frame's pair
+    list and contact counts equal the oracle's."""
+    from xpbd_amd import Solver6DOF
+    from xpbd_amd.scenes import unified_scene
+
+    s = unified_scene(Solver6DOF, device=DEV)
+    orc = _oracle6(oracle, s)
+    s.step()
+    orc.step(1)
+    want = np.stack([orc.pair_a[:orc.n_pairs], orc.pair_b[:orc.n_pairs]], axis=1)
+    assert s.n_pairs == orc.n_pairs and (s.pairs() == want).all()
+    for _ in range(60):
+        s.step()
+    assert np.all(np.isfinite(s.positions()))
+    print(f"config3: {s.num_bodies} bodies, {s.num_joints} joints, {s.n_pairs} pairs, "
+          f"{int(s.contact_counts().sum())} manifold points")

@@ -181,3 +181,61 @@ def test_flush_preserves_reference_numerics():
     assert h["inv_mass"][0] == np.float32(1.0) / np.float32(3.0)
     assert h["inv_mass"][1] == 0.0
     assert h["rest"][0] == np.float32(np.sqrt(0.01 + 0.04 + 0.09))
+
+
+def _header_arities():
+    """Parameter count of every xpbd_* function declared in include/*.h."""
+    out = {}
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"(xpbd\w*)\s*\(([^;{]*?)\)\s*;", text):
+            params = m.group(2).strip()
+            out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_ctypes_signatures_match_header_arity():
+    from xpbd_amd import _native as N
+
+    arity = _header_arities()
+    sigs = dict(N._SIGS)
+    sigs.update(N._EXTRA_SIGS)
+    assert sigs, "no bound signatures"
+    for name, (_, args) in sigs.items():
+        assert name in arity, f"{name} bound in ctypes but not declared in include/"
+        assert len(args) == arity[name], f"{name}: ctypes {len(args)} args, header {arity[name]}"
+
+
+@pytest.mark.parametrize("tile", [7, 64, 100000])
+def test_tile_tables_cover_every_endpoint(tile):
+    """Every (row, endpoint) pair is visited exactly once by the tile that
+    owns the endpoint: as an owned row (smallest endpoint in the tile) or as
+    a halo row (larger endpoint in the tile, smaller endpoint elsewhere)."""
+    from xpbd_amd import Solver
+    from xpbd_amd.scenes import cloth_sheet
+    from xpbd_amd.solver import jacobi_row_order, morton_order, pack_rows, tile_tables
+
+    s, _ = cloth_sheet(Solver, res=20)
+    h = s.host_arrays()
+    perm = morton_order(h["x"].astype(np.float64))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    ra = inv[h["a"]]
+    rb = np.where(h["b"] >= 0, inv[np.maximum(h["b"], 0)], -1)
+    rows, _ = pack_rows(h["type"], ra, rb, h["rest"], h["compliance"], h["anchor"], h["mu"])
+    rows = rows[jacobi_row_order(ra, rb)]
+    starts, hs, hr = tile_tables(rows, s.num_bodies, tile)
+    n_t = len(starts) - 1
+    seen = {}
+    a = rows[:, 0] & 0x0FFFFFFF
+    b = rows[:, 1]
+    for t in range(n_t):
+        visit = list(range(starts[t], starts[t + 1])) + hr[hs[t]:hs[t + 1]].tolist()
+        for j in visit:
+            for e in ([a[j]] + ([b[j]] if b[j] >= 0 else [])):
+                if e // tile == t:
+                    seen[(j, int(e))] = seen.get((j, int(e)), 0) + 1
+    want = {(j, int(a[j])) for j in range(len(rows))} | \
+           {(j, int(b[j])) for j in range(len(rows)) if b[j] >= 0}
+    assert set(seen) == want and set(seen.values()) == {1}
+    assert starts[0] == 0 and starts[-1] == len(rows)

@@ -32,8 +32,32 @@ class ParticlesDesc(ctypes.Structure):
         ("h", c_float), ("gx", c_float), ("gy", c_float), ("gz", c_float),
         ("damping", c_float), ("relax", c_float), ("max_speed", c_float),
         ("rows_per_thread", c_int), ("window", c_int),
+        ("x_alt", c_void_p), ("lam_alt", c_void_p), ("tile_row_start", c_void_p),
+        ("halo_start", c_void_p), ("halo_rows", c_void_p), ("tile_size", c_int),
+        ("dyn_rows", c_void_p), ("dyn_lam", c_void_p), ("dyn_count", c_void_p),
+        ("dyn_cap", c_int), ("gscale", c_void_p),
     ]
 
+
+class RigidDesc(ctypes.Structure):
+    """Mirror of ``xpbd_rigid_desc`` (include/xpbd.h)."""
+
+    _fields_ = (
+        [(k, c_void_p) for k in ("x", "q", "v", "w", "x_prev", "q_prev", "inv_I", "he", "dx",
+                                 "drot", "dv", "dw", "lam_floor")]
+        + [("n", c_int), ("pairs", c_void_p), ("pair_count", c_void_p), ("cap", c_int)]
+        + [(k, c_void_p) for k in ("m_count", "pt_idx", "pt_n", "pt_oi", "pt_or", "lam_pt",
+                                   "pt_count", "j_idx", "j_a", "j_b", "lam_j")]
+        + [("nj", c_int), ("substeps", c_int), ("iterations", c_int), ("gyroscopic", c_int)]
+        + [(k, c_float) for k in ("h", "gx", "gy", "gz", "floor_y", "friction", "restitution",
+                                  "lin_damp", "ang_damp", "relax", "max_speed", "max_omega",
+                                  "man_margin")]
+    )
+
+
+_P = c_void_p
+_F = c_float
+_I = c_int
 
 # name -> (restype, argtypes)
 _SIGS = {
@@ -47,6 +71,14 @@ _SIGS = {
                                        c_float, c_int]),
     "xpbd_solve_jacobi": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_float, c_int, c_int, c_int, c_int]),
+    "xpbd_jacobi_tile": (c_int, [c_void_p] * 12 + [c_float, c_int, c_int, c_int, c_float, c_int,
+                                                    c_float, c_float, c_void_p]),
+    "xpbd_integrate_to": (c_int, [c_void_p] * 5 + [c_float, c_float, c_float, c_float, c_void_p,
+                                                   c_int]),
+    "xpbd_solve_jacobi_dyn": (c_int, [c_void_p] * 6 + [c_float, c_int, c_void_p, c_int]),
+    "xpbd_contacts_workspace_bytes": (ctypes.c_longlong, [c_int]),
+    "xpbd_sphere_contacts": (c_int, [c_void_p] * 6 + [c_float, c_float, c_float, c_int, c_int,
+                                                      c_void_p, c_void_p]),
     "xpbd_solve_color": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_float, c_int, c_int, c_int]),
     "xpbd_frame3_create": (c_int, [ctypes.POINTER(ParticlesDesc), ctypes.POINTER(c_void_p)]),
@@ -54,6 +86,25 @@ _SIGS = {
     "xpbd_frame3_destroy": (c_int, [c_void_p]),
     "xpbd_color_constraints": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "xpbd_color_ranges": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "xpbd6_integrate": (_I, [_P] * 10 + [_F, _F, _F, _F, _I, _I]),
+    "xpbd6_box_manifold": (_I, [_P] * 6 + [_F, _I] + [_P] * 7),
+    "xpbd6_solve_joints": (_I, [_P] * 8 + [_F, _I, _P, _P, _I]),
+    "xpbd6_solve_floor": (_I, [_P] * 7 + [_F, _F, _P, _P, _P, _I]),
+    "xpbd6_solve_box": (_I, [_P] * 12 + [_I]),
+    "xpbd6_apply_jacobi": (_I, [_P] * 5 + [_F, _I]),
+    "xpbd6_finalize_velocity": (_I, [_P] * 7 + [_F, _F, _F, _I]),
+    "xpbd6_velocity_floor": (_I, [_P] * 5 + [_F, _F, _P, _F, _P, _P, _P, _P, _I]),
+    "xpbd6_velocity_box": (_I, [_P] * 10 + [_F, _F, _F, _P, _P, _P, _P, _I]),
+    "xpbd6_apply_velocity": (_I, [_P] * 6 + [_F, _F, _I, _I]),
+    "xpbd_frame6_create": (_I, [ctypes.POINTER(RigidDesc), ctypes.POINTER(c_void_p)]),
+    "xpbd_frame6_run": (_I, [_P, _P, _I, _I]),
+    "xpbd_frame6_destroy": (_I, [_P]),
+    "xpbd6_body_aabb": (_I, [_P] * 6 + [_F, _F, _P, _P, _I]),
+    "xpbd6_lbvh_workspace_bytes": (ctypes.c_longlong, [_I]),
+    "xpbd6_hashgrid_workspace_bytes": (ctypes.c_longlong, [_I]),
+    "xpbd6_broadphase_lbvh": (_I, [_P] * 7 + [_F, _F, _I, _I, _P, _P]),
+    "xpbd6_broadphase_hashgrid": (_I, [_P] * 7 + [_F, _F, _F, _F, _I, _I, _P, _P]),
+    "xpbd_morton3": (ctypes.c_uint, [_F, _F, _F]),
     "xpbd_build_info": (ctypes.c_char_p, []),
     "xpbd_error_string": (ctypes.c_char_p, [c_int]),
 }

@@ -1,0 +1,79 @@
+"""Launch-shape autotuner for the 3-DOF Jacobi sweep.
+
+For a flushed ``Solver`` in Jacobi mode it times whole frames (hipGraph
+replays, HIP events on the launch stream) under each candidate launch shape
+and keeps the fastest:
+
+* ``tile``   — the fused per-iteration kernel with tile sizes of 128 ... 4096
+  particles (LDS window of tile x 16 B; larger tiles re-project fewer halo
+  rows, smaller tiles spread the launch over more CUs);
+* ``window`` — projection with an LDS window + global atomics and a separate
+  apply pass, with 1 or 2 rows per thread.
+
+The simulation state is saved before and restored after each trial, so tuning
+does not advance the scene.  Results are cached per scene signature (particle
+count, row count, substeps, iterations) for the life of the process.
+"""
+from __future__ import annotations
+
+_CACHE: dict = {}
+
+
+def _signature(s):
+    return (s.num_bodies, s.num_constraints, s.substeps, s.iterations, s.solve_mode)
+
+
+def candidates(n_particles: int):
+    out = []
+    for t in (128, 256, 512, 1024, 2048, 4096):
+        if t <= max(128, 2 * n_particles):
+            out.append(("tile", t, None))
+    out.append(("window", 0, (1, None)))
+    out.append(("window", 0, (2, None)))
+    return out
+
+
+def _apply(s, cand):
+    kind, tile, tiling = cand
+    s.jacobi_kernel = kind
+    s.tile_size = tile
+    s._dirty = True
+    s._flush()
+    if kind == "window" and tiling is not None:
+        rpt, _ = tiling
+        s._tiling = (rpt, s._tiling[1])
+
+
+def tune(s, frames: int = 3, warmup: int = 1, verbose: bool = False):
+    """Pick the fastest Jacobi launch shape for solver ``s`` (in place) and
+    return ``(candidate, {candidate: ms_per_frame})``."""
+    import torch
+
+    if s.solve_mode != "jacobi":
+        return None, {}
+    s._flush()
+    sig = _signature(s)
+    if sig in _CACHE:
+        _apply(s, _CACHE[sig])
+        return _CACHE[sig], {}
+    x0, v0 = s.positions(), s.velocities()
+    times = {}
+    for cand in candidates(s.num_bodies):
+        _apply(s, cand)
+        s.set_state(x0, v0)
+        s.step(warmup)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        s.step(frames)
+        e1.record()
+        torch.cuda.synchronize()
+        times[cand] = e0.elapsed_time(e1) / frames
+        if verbose:
+            print(f"[autotune] {cand}: {times[cand]:.4f} ms/frame")
+    best = min(times, key=times.get)
+    _apply(s, best)
+    s.set_state(x0, v0)
+    _CACHE[sig] = best
+    return best, times

@@ -31,8 +31,52 @@ struct Frame3 {
   hipStream_t capture_stream = nullptr;
 };
 
+// Fused-tile Jacobi frame: per substep one integrate pass (X -> X or X_alt,
+// chosen so that after `iterations` ping-pong launches the state is back in
+// X) and one k_jacobi_tile launch per iteration, the last one also running
+// finalize + cap.
+int issue_frame_tile(Frame3* f, hipStream_t s) {
+  const xpbd_particles_desc& d = f->d;
+  const int n = d.n_particles;
+  if (n <= 0) return 0;
+  const bool have_rows = (d.n_rows > 0 || d.dyn_cap > 0) && d.iterations > 0;
+  const int it_n = have_rows ? d.iterations : 0;
+  float* bufs[2] = {d.x, d.x_alt};
+  float* lams[2] = {d.lam, d.lam_alt};
+  int rc;
+  for (int sub = 0; sub < d.substeps; ++sub) {
+    int cur = it_n & 1;
+    rc = xpbd_integrate_to(s, d.x, bufs[cur], d.v, d.x_prev, d.gx, d.gy, d.gz, d.h, d.gscale, n);
+    if (rc) return rc;
+    const bool dyn = d.dyn_cap > 0 && d.dyn_rows && d.dyn_count;
+    for (int it = 0; it < it_n; ++it) {
+      const int nxt = cur ^ 1;
+      if (dyn) {
+        rc = xpbd_solve_jacobi_dyn(s, bufs[cur], d.x_prev, d.dyn_rows, d.dyn_lam, d.acc, d.h,
+                                   it == 0, d.dyn_count, d.dyn_cap);
+        if (rc) return rc;
+      }
+      rc = xpbd_jacobi_tile(s, bufs[cur], bufs[nxt], d.x_prev, d.v, d.rows, d.anchors,
+                            lams[it & 1], lams[(it + 1) & 1], d.tile_row_start, d.halo_start,
+                            d.halo_rows, d.h, it == 0, d.tile_size, n, d.relax,
+                            it + 1 == it_n, d.damping, d.max_speed, dyn ? d.acc : nullptr);
+      if (rc) return rc;
+      cur = nxt;
+    }
+    if (it_n == 0) {
+      int flags = F_FINALIZE;
+      if (d.max_speed > 0.0f && std::isfinite(d.max_speed)) flags |= F_CAP;
+      rc = xpbd_particles_pass(s, d.x, d.v, d.x_prev, d.acc, flags, d.relax, d.h, d.damping,
+                               d.max_speed, d.gx, d.gy, d.gz, n);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
 int issue_frame(Frame3* f, hipStream_t s) {
   const xpbd_particles_desc& d = f->d;
+  if (d.mode == XPBD_MODE_JACOBI_TILE) return issue_frame_tile(f, s);
   const int n = d.n_particles;
   if (n <= 0) return 0;
   const bool have_rows = d.n_rows > 0 && d.iterations > 0;
@@ -87,7 +131,18 @@ extern "C" {
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
   if (!desc || !handle) return (int)hipErrorInvalidValue;
-  if (desc->mode != XPBD_MODE_GS && desc->mode != XPBD_MODE_JACOBI) return (int)hipErrorInvalidValue;
+  if (desc->mode != XPBD_MODE_GS && desc->mode != XPBD_MODE_JACOBI &&
+      desc->mode != XPBD_MODE_JACOBI_TILE)
+    return (int)hipErrorInvalidValue;
+  if (desc->mode == XPBD_MODE_JACOBI_TILE && (desc->n_rows > 0 || desc->dyn_cap > 0) &&
+      (!desc->x_alt || !desc->lam_alt || !desc->tile_row_start || !desc->halo_start ||
+       desc->tile_size < 1 || desc->tile_size > XPBD_MAX_TILE))
+    return (int)hipErrorInvalidValue;
+  if (desc->mode != XPBD_MODE_JACOBI_TILE && (desc->dyn_cap > 0 || desc->gscale))
+    return (int)hipErrorInvalidValue;  // device-counted rows / gravity scale: tile mode only
+  if (desc->mode == XPBD_MODE_JACOBI_TILE && desc->dyn_cap > 0 &&
+      (!desc->acc || !desc->dyn_rows || !desc->dyn_lam || !desc->dyn_count))
+    return (int)hipErrorInvalidValue;
   if (desc->mode == XPBD_MODE_GS && desc->n_rows > 0 && (!desc->color_starts || desc->num_colors < 1))
     return (int)hipErrorInvalidValue;
   if (desc->mode == XPBD_MODE_JACOBI && desc->n_rows > 0 && !desc->acc) return (int)hipErrorInvalidValue;

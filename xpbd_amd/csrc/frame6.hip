@@ -1,0 +1,128 @@
+// Frame driver of the 6-DOF solver: the fixed per-substep kernel sequence of
+// Solver6DOF._record (src/xpbd3d/solver_6dof.py:357-426), issued eagerly or
+// replayed from one hipGraph captured per plan (the reference captures a
+// CUDA graph, solver_6dof.py:519-529).  Box kernels launch at fixed capacity
+// and read the live pair / point counts from device memory, so the graph
+// topology does not depend on the frame.
+#include <new>
+
+#include "common.h"
+#include "xpbd.h"
+
+namespace {
+
+struct Frame6 {
+  xpbd_rigid_desc d;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture_stream = nullptr;
+};
+
+int issue(Frame6* f, hipStream_t s) {
+  const xpbd_rigid_desc& d = f->d;
+  const int n = d.n;
+  if (n <= 0) return 0;
+  const int maxp = 8 * d.cap;
+  int rc;
+  for (int sub = 0; sub < d.substeps; ++sub) {
+    rc = xpbd6_integrate(s, d.x, d.q, d.v, d.w, d.x_prev, d.q_prev, d.inv_I, d.lam_floor,
+                         d.pt_count, d.gx, d.gy, d.gz, d.h, d.gyroscopic, n);
+    if (rc) return rc;
+    rc = xpbd6_box_manifold(s, d.x, d.q, d.he, d.pairs, d.pair_count, d.man_margin, d.cap,
+                            d.m_count, d.pt_idx, d.pt_n, d.pt_oi, d.pt_or, d.lam_pt, d.pt_count);
+    if (rc) return rc;
+    for (int it = 0; it < d.iterations; ++it) {
+      if (d.nj > 0) {
+        rc = xpbd6_solve_joints(s, d.x, d.q, d.inv_I, d.j_idx, d.j_a, d.j_b, d.lam_j, d.h,
+                                it == 0, d.dx, d.drot, d.nj);
+        if (rc) return rc;
+      }
+      rc = xpbd6_solve_floor(s, d.x, d.q, d.x_prev, d.q_prev, d.inv_I, d.he, d.floor_y,
+                             d.friction, d.lam_floor, d.dx, d.drot, n);
+      if (rc) return rc;
+      rc = xpbd6_solve_box(s, d.x, d.q, d.inv_I, d.pt_idx, d.pt_n, d.pt_oi, d.pt_or, d.lam_pt,
+                           d.pt_count, d.dx, d.drot, maxp);
+      if (rc) return rc;
+      rc = xpbd6_apply_jacobi(s, d.x, d.q, d.dx, d.drot, d.relax, n);
+      if (rc) return rc;
+    }
+    rc = xpbd6_finalize_velocity(s, d.x, d.q, d.x_prev, d.q_prev, d.v, d.w, d.h, d.lin_damp,
+                                 d.ang_damp, n);
+    if (rc) return rc;
+    rc = xpbd6_velocity_floor(s, d.x, d.q, d.inv_I, d.he, d.friction, d.restitution,
+                              d.lam_floor, d.h, d.v, d.w, d.dv, d.dw, n);
+    if (rc) return rc;
+    rc = xpbd6_velocity_box(s, d.x, d.q, d.inv_I, d.pt_idx, d.pt_n, d.pt_oi, d.pt_or, d.lam_pt,
+                            d.pt_count, d.friction, d.restitution, d.h, d.v, d.w, d.dv, d.dw,
+                            maxp);
+    if (rc) return rc;
+    rc = xpbd6_apply_velocity(s, d.x, d.v, d.w, d.dv, d.dw, d.max_speed, d.max_omega,
+                              d.max_speed > 0.0f, n);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+void drop_graph(Frame6* f) {
+  if (f->exec) (void)hipGraphExecDestroy(f->exec);
+  if (f->graph) (void)hipGraphDestroy(f->graph);
+  f->exec = nullptr;
+  f->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xpbd_frame6_create(const xpbd_rigid_desc* desc, void** handle) {
+  if (!desc || !handle || desc->n < 0 || desc->cap < 0) return (int)hipErrorInvalidValue;
+  Frame6* f = new (std::nothrow) Frame6();
+  if (!f) return (int)hipErrorOutOfMemory;
+  f->d = *desc;
+  *handle = f;
+  return 0;
+}
+
+int xpbd_frame6_run(void* handle, void* stream, int frames, int use_graph) {
+  Frame6* f = (Frame6*)handle;
+  if (!f || frames < 0) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  if (!use_graph) {
+    for (int k = 0; k < frames; ++k) {
+      int rc = issue(f, s);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  if (!f->exec) {
+    if (!f->capture_stream) XPBD_CHECK(hipStreamCreateWithFlags(&f->capture_stream, hipStreamNonBlocking));
+    XPBD_CHECK(hipStreamBeginCapture(f->capture_stream, hipStreamCaptureModeThreadLocal));
+    int rc = issue(f, f->capture_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(f->capture_stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    XPBD_CHECK(e);
+    f->graph = g;
+    e = hipGraphInstantiate(&f->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graph(f);
+      return (int)e;
+    }
+  }
+  for (int k = 0; k < frames; ++k) XPBD_CHECK(hipGraphLaunch(f->exec, s));
+  return 0;
+}
+
+int xpbd_frame6_destroy(void* handle) {
+  Frame6* f = (Frame6*)handle;
+  if (!f) return 0;
+  drop_graph(f);
+  if (f->capture_stream) (void)hipStreamDestroy(f->capture_stream);
+  delete f;
+  return 0;
+}
+
+}  // extern "C"

@@ -246,6 +246,152 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_project_global(
   }
 }
 
+// Projection of a row segment whose live length is on the device (rows
+// generated on the GPU, e.g. CONTACT rows): plain global atomics into ACC.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_project_dyn(
+    const float4* __restrict__ X, const float4* __restrict__ XP, const int4* __restrict__ R,
+    float* __restrict__ LAM, float4* __restrict__ ACC, float h, int first,
+    const int* __restrict__ n_dev, int cap) {
+  const int j = blockIdx.x * BLOCK + threadIdx.x;
+  if (j >= min(*n_dev, cap)) return;
+  const float inv_dt2 = 1.0f / (h * h);
+  const int4 row = R[j];
+  const float lam = first ? 0.0f : LAM[j];
+  const Proj p = project_jacobi(X, XP, row, make_float4(0.f, 0.f, 0.f, 0.f), lam, inv_dt2);
+  if (p.upd || first) LAM[j] = p.lam;
+  if (p.na >= 1) {
+    atomicAdd(&ACC[p.a].x, p.ca.x);
+    atomicAdd(&ACC[p.a].y, p.ca.y);
+    atomicAdd(&ACC[p.a].z, p.ca.z);
+    atomicAdd(&ACC[p.a].w, 1.0f);
+  }
+  if (p.na == 2) {
+    atomicAdd(&ACC[p.b].x, p.cb.x);
+    atomicAdd(&ACC[p.b].y, p.cb.y);
+    atomicAdd(&ACC[p.b].z, p.cb.z);
+    atomicAdd(&ACC[p.b].w, 1.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused Jacobi iteration over particle tiles (projection + apply in one
+// launch, no global accumulators or atomics).
+//
+// Particles are split into tiles of P consecutive indices; a row is owned by
+// the tile of its smallest endpoint (rows are sorted by smallest endpoint, so
+// a tile owns the contiguous rows [tile_row_start[t], tile_row_start[t+1])).
+// A row whose other endpoint lies in a later tile is also listed as a halo
+// row of that tile and projected again there: both tiles read the same
+// X_in / LAM_in, so they compute the same correction; each adds only its own
+// endpoints (LDS float atomics into a P-entry float4 window) and only the
+// owner writes LAM_out.  After a barrier every thread applies the averaged
+// correction to its tile's particles and writes X_out (ping-pong buffers), so
+// one launch is one complete Jacobi iteration (kernels.py:219-339).  On the
+// last iteration of a substep the same pass also runs finalize_velocity and
+// cap_velocity (kernels.py:346-371).
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
+    const float4* __restrict__ Xin, float4* __restrict__ Xout, const float4* __restrict__ XP,
+    float4* __restrict__ V, const int4* __restrict__ R, const float4* __restrict__ ANC,
+    const float* __restrict__ LAMin, float* __restrict__ LAMout,
+    const int* __restrict__ tile_row_start, const int* __restrict__ halo_start,
+    const int* __restrict__ halo_rows, float h, int first, int P, int n, float relax,
+    int finalize, float damping, float max_speed, int do_cap, float4* __restrict__ ACC) {
+  extern __shared__ float4 acc[];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int p0 = tile * P;
+  if (p0 >= n) return;
+  const int p1 = min(p0 + P, n);
+  for (int k = threadIdx.x; k < P; k += BLOCK) acc[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+  __syncthreads();
+  const float inv_dt2 = 1.0f / (h * h);
+  const int rb = tile_row_start[tile], re = tile_row_start[tile + 1];
+  const int hb = halo_start[tile], he = halo_start[tile + 1];
+  const int owned = re - rb;
+  const int total = owned + (he - hb);
+  for (int k = threadIdx.x; k < total; k += BLOCK) {
+    const bool own = k < owned;
+    const int j = own ? rb + k : halo_rows[hb + (k - owned)];
+    const int4 row = R[j];
+    const float lam = first ? 0.0f : LAMin[j];
+    float4 anc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (row_type(row) == T_ATTACH) anc = ANC[j];
+    const Proj p = project_jacobi(Xin, XP, row, anc, lam, inv_dt2);
+    if (own) LAMout[j] = p.lam;
+    if (p.na >= 1) {
+      const unsigned off = (unsigned)(p.a - p0);
+      if (off < (unsigned)P) {
+        atomicAdd(&acc[off].x, p.ca.x);
+        atomicAdd(&acc[off].y, p.ca.y);
+        atomicAdd(&acc[off].z, p.ca.z);
+        atomicAdd(&acc[off].w, 1.0f);
+      }
+    }
+    if (p.na == 2) {
+      const unsigned off = (unsigned)(p.b - p0);
+      if (off < (unsigned)P) {
+        atomicAdd(&acc[off].x, p.cb.x);
+        atomicAdd(&acc[off].y, p.cb.y);
+        atomicAdd(&acc[off].z, p.cb.z);
+        atomicAdd(&acc[off].w, 1.0f);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = p0 + threadIdx.x; i < p1; i += BLOCK) {
+    const float4 x4 = Xin[i];
+    float4 a = acc[i - p0];
+    if (ACC) {  // corrections of the device-counted row segment
+      const float4 g = ACC[i];
+      if (g.w != 0.0f) {
+        a = make_float4(a.x + g.x, a.y + g.y, a.z + g.z, a.w + g.w);
+        ACC[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+    f3 x = xyz(x4);
+    if (a.w > 0.0f && x4.w > 0.0f) x = x + xyz(a) * (relax / a.w);
+    if (finalize) {
+      f3 v;
+      if (x4.w == 0.0f) {
+        v = mk3(0.0f, 0.0f, 0.0f);
+      } else {
+        v = ((x - xyz(XP[i])) / h) * (1.0f - damping);
+      }
+      if (do_cap) {
+        const float speed = length(v);
+        if (speed > max_speed) v = v * (max_speed / speed);
+      }
+      V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+    }
+    Xout[i] = make_float4(x.x, x.y, x.z, x4.w);
+  }
+}
+
+// integrate (kernels.py:49-73) from Xin into Xout (which may alias Xin).
+// gscale (optional): per-particle gravity factor, g_i = g * gscale[i].
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_integrate_to(const float4* Xin, float4* Xout,
+                                                        float4* __restrict__ V,
+                                                        float4* __restrict__ XP, float gx,
+                                                        float gy, float gz, float h,
+                                                        const float* __restrict__ gscale, int n) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const float4 x4 = Xin[i];
+  f3 x = xyz(x4);
+  XP[i] = make_float4(x.x, x.y, x.z, 0.0f);
+  if (x4.w != 0.0f) {
+    f3 g = mk3(gx, gy, gz);
+    if (gscale) g = g * gscale[i];
+    const f3 v = xyz(V[i]) + g * h;
+    x = x + v * h;
+    V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+  }
+  Xout[i] = make_float4(x.x, x.y, x.z, x4.w);
+}
+
 // ---------------------------------------------------------------------------
 // Colored Gauss-Seidel: one launch per color over a contiguous slice of the
 // color-sorted rows; rows of one color touch disjoint particles, so the
@@ -407,6 +553,7 @@ __global__ __launch_bounds__(BLOCK) void k_particles(float4* __restrict__ X, flo
 }
 
 constexpr int kBlock = 256;
+constexpr int kTileBlock = 512;
 
 inline int blocks_for(long long n, int per_block) { return (int)((n + per_block - 1) / per_block); }
 
@@ -467,6 +614,46 @@ int xpbd_solve_jacobi(void* stream, const float* X, const float* XP, const int* 
                        (const float4*)X, (const float4*)XP, (const int4*)R, (const float4*)ANC,
                        LAM, (float4*)ACC, h, first, n_rows, rows_per_thread, window);
   }
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* XP, float* V,
+                     const int* R, const float* ANC, const float* LAMin, float* LAMout,
+                     const int* tile_row_start, const int* halo_start, const int* halo_rows,
+                     float h, int first, int tile_size, int n_particles, float relax,
+                     int finalize, float damping, float max_speed, float* ACC) {
+  if (n_particles <= 0) return 0;
+  if (tile_size < 1 || tile_size > XPBD_MAX_TILE) return (int)hipErrorInvalidValue;
+  const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
+  const int nb = blocks_for(n_particles, tile_size);
+  const size_t shmem = sizeof(float4) * (size_t)tile_size;
+  hipLaunchKernelGGL(k_jacobi_tile<kTileBlock>, dim3(nb), dim3(kTileBlock), shmem,
+                     (hipStream_t)stream, (const float4*)Xin, (float4*)Xout, (const float4*)XP,
+                     (float4*)V, (const int4*)R, (const float4*)ANC, LAMin, LAMout,
+                     tile_row_start, halo_start, halo_rows, h, first, tile_size, n_particles,
+                     relax, finalize, damping, max_speed, do_cap, (float4*)ACC);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_integrate_to(void* stream, const float* Xin, float* Xout, float* V, float* XP, float gx,
+                      float gy, float gz, float h, const float* gscale, int n) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_integrate_to<kBlock>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0,
+                     (hipStream_t)stream, (const float4*)Xin, (float4*)Xout, (float4*)V,
+                     (float4*)XP, gx, gy, gz, h, gscale, n);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_solve_jacobi_dyn(void* stream, const float* X, const float* XP, const int* R,
+                          float* LAM, float* ACC, float h, int first, const int* n_rows_dev,
+                          int cap) {
+  if (cap <= 0) return 0;
+  hipLaunchKernelGGL(k_jacobi_project_dyn<kBlock>, dim3(blocks_for(cap, kBlock)), dim3(kBlock), 0,
+                     (hipStream_t)stream, (const float4*)X, (const float4*)XP, (const int4*)R,
+                     LAM, (float4*)ACC, h, first, n_rows_dev, cap);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

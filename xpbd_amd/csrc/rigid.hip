@@ -1,0 +1,662 @@
+// 6-DOF rigid-body XPBD kernels for gfx950 and their C-ABI entry points
+// (reference: src/xpbd3d/kernels_6dof.py; frame sequence of
+// Solver6DOF._record, src/xpbd3d/solver_6dof.py:357-426).
+//
+// Device layout (float4 per body so each gather is one 16-byte load):
+//   X  (x, y, z, inv_mass)   Q  (qx, qy, qz, qw)   V, W  (v / omega, 0)
+//   XP, QP  substep-start position / orientation
+//   IB (inverse body inertia diagonal, 0)   HE (half extents, group as int bits)
+//   DX (sum dx, count)  DR (sum d-rotation, 0)    position Jacobi accumulators
+//   DV (sum dv, count)  DW (sum d-omega, 0)       velocity-pass accumulators
+//   LAMF float[8n]  floor-corner multipliers
+// Contact manifold (generated once per substep, compacted):
+//   MC int[cap]      points per candidate pair (the reference's m_count)
+//   PTI int4[8cap]   (pair, incident body, reference body, slot)
+//   PTN/PTOI/PTOR float4[8cap]  normal, incident / reference body-frame offsets
+//   LAMP float[8cap] point multipliers;  PTC int[1] live point count
+// Joints: JI int2 (a, b), JA float4 (anchor_a, rest), JB float4 (anchor_b,
+// compliance), LAMJ float.
+#include <cmath>
+
+#include "common.h"
+#include "rigid_math.h"
+#include "xpbd.h"
+
+namespace {
+
+constexpr int kB = 256;
+
+__device__ inline void atomic_add3(float4* p, f3 v) {
+  atomicAdd(&p->x, v.x);
+  atomicAdd(&p->y, v.y);
+  atomicAdd(&p->z, v.z);
+}
+
+// kernels_6dof.py:108-130 with atomics (B may be -1 / static)
+__device__ inline void accumulate_correction(float4* DX, float4* DR, int a, int b, float ima,
+                                             float imb, q4 qa, q4 qb, f3 iIa, f3 iIb, f3 ra,
+                                             f3 rb, f3 p) {
+  if (ima > 0.0f) {
+    atomic_add3(&DX[a], p * ima);
+    atomic_add3(&DR[a], world_invI_mul(qa, iIa, cross(ra, p)));
+    atomicAdd(&DX[a].w, 1.0f);
+  }
+  if (b >= 0 && imb > 0.0f) {
+    atomic_add3(&DX[b], (-p) * imb);
+    atomic_add3(&DR[b], -world_invI_mul(qb, iIb, cross(rb, p)));
+    atomicAdd(&DX[b].w, 1.0f);
+  }
+}
+
+// kernels_6dof.py:67-102 (+ per-substep lambda_floor reset and contact
+// point counter reset, both exact rewrites of solver_6dof.py:374-375)
+__global__ __launch_bounds__(kB) void k6_integrate(float4* X, float4* Q, float4* V, float4* W,
+                                                   float4* XP, float4* QP, const float4* IB,
+                                                   float* LAMF, int* PTC, float gx, float gy,
+                                                   float gz, float h, int gyro, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i == 0 && PTC) *PTC = 0;
+  if (i >= n) return;
+  float4 x4 = X[i];
+  q4 q = ldq(Q[i]);
+  XP[i] = make_float4(x4.x, x4.y, x4.z, 0.0f);
+  QP[i] = stq(q);
+  float4* lf = (float4*)(LAMF + 8 * (size_t)i);
+  lf[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+  lf[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (x4.w == 0.0f) return;
+  f3 v = xyz(V[i]) + mk3(gx, gy, gz) * h;
+  f3 x = xyz(x4) + v * h;
+  V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+  f3 w = xyz(W[i]);
+  if (gyro != 0) {
+    f3 wb = qrot_inv(q, w);
+    f3 ii = xyz(IB[i]);
+    float ix = ii.x > 0.0f ? 1.0f / ii.x : 0.0f;
+    float iy = ii.y > 0.0f ? 1.0f / ii.y : 0.0f;
+    float iz = ii.z > 0.0f ? 1.0f / ii.z : 0.0f;
+    f3 Iw = mk3(ix * wb.x, iy * wb.y, iz * wb.z);
+    f3 tau = -cross(wb, Iw);
+    f3 dwb = mk3(ii.x * tau.x, ii.y * tau.y, ii.z * tau.z) * h;
+    w = w + qrot(q, dwb);
+    W[i] = make_float4(w.x, w.y, w.z, 0.0f);
+  }
+  q = quat_integrate(q, w, h);
+  X[i] = make_float4(x.x, x.y, x.z, x4.w);
+  Q[i] = stq(q);
+}
+
+// kernels_6dof.py:226-230
+__device__ inline float proj_radius(f3 L, const m3& R, f3 e) {
+  return fabsf(e.x * dot(L, R.c[0])) + fabsf(e.y * dot(L, R.c[1])) + fabsf(e.z * dot(L, R.c[2]));
+}
+
+// kernels_6dof.py:233-382: face-axis SAT + Sutherland-Hodgman clip of the
+// incident face against the reference face, <= 8 points per pair.  The
+// points of a wavefront are compacted into the point arrays with one atomic
+// per wave (inclusive scan of the per-lane counts over the 64 lanes).
+__global__ __launch_bounds__(kB) void k6_box_manifold(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ HE,
+    const int2* __restrict__ PAIRS, const int* __restrict__ PC, float margin, int cap,
+    int* __restrict__ MC, int4* __restrict__ PTI, float4* __restrict__ PTN,
+    float4* __restrict__ PTOI, float4* __restrict__ PTOR, float* __restrict__ LAMP,
+    int* __restrict__ PTC) {
+  const int pid = blockIdx.x * kB + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int live = min(*PC, cap);
+  int nc = 0, inc_b = 0, ref_b = 0;
+  f3 ref_face_n = mk3(0.f, 0.f, 0.f);
+  f3 off_i[8], off_r[8];
+  if (pid < live) {
+    const int2 pr = PAIRS[pid];
+    const int a = pr.x, b = pr.y;
+    const f3 cA = xyz(X[a]), cB = xyz(X[b]);
+    const f3 eA = xyz(HE[a]), eB = xyz(HE[b]);
+    const m3 RA = qmat(ldq(Q[a])), RB = qmat(ldq(Q[b]));
+    const f3 cBA = cA - cB;
+    float best_overlap = 1.0e30f;
+    f3 best_axis = mk3(0.0f, 1.0f, 0.0f);
+    int ref_is_a = 1, best_k = 0;
+    bool sep = false;
+    for (int k = 0; k < 3 && !sep; ++k) {
+      f3 L = RA.c[k];
+      float ov = comp(eA, k) + proj_radius(L, RB, eB) - fabsf(dot(L, cBA));
+      if (ov < 0.0f) sep = true;
+      else if (ov < best_overlap) { best_overlap = ov; best_axis = L; ref_is_a = 1; best_k = k; }
+    }
+    for (int k = 0; k < 3 && !sep; ++k) {
+      f3 L = RB.c[k];
+      float ov = comp(eB, k) + proj_radius(L, RA, eA) - fabsf(dot(L, cBA));
+      if (ov < 0.0f) sep = true;
+      else if (ov < best_overlap) { best_overlap = ov; best_axis = L; ref_is_a = 0; best_k = k; }
+    }
+    if (!sep) {
+      f3 nBA = best_axis;
+      if (dot(nBA, cBA) < 0.0f) nBA = -nBA;
+      f3 c_ref, e_ref, c_inc, e_inc;
+      m3 R_ref, R_inc;
+      if (ref_is_a == 1) {
+        ref_b = a; inc_b = b; c_ref = cA; R_ref = RA; e_ref = eA;
+        c_inc = cB; R_inc = RB; e_inc = eB; ref_face_n = -nBA;
+      } else {
+        ref_b = b; inc_b = a; c_ref = cB; R_ref = RB; e_ref = eB;
+        c_inc = cA; R_inc = RA; e_inc = eA; ref_face_n = nBA;
+      }
+      const int ref_axis = best_k;
+      const f3 ref_face_c = c_ref + ref_face_n * comp(e_ref, ref_axis);
+      const int ax1 = (ref_axis + 1) % 3, ax2 = (ref_axis + 2) % 3;
+      const f3 u_ref = R_ref.c[ax1], v_ref = R_ref.c[ax2];
+      const float eu = comp(e_ref, ax1), ev = comp(e_ref, ax2);
+      int inc_axis = 0;
+      float inc_sign = 1.0f, best_dot = -1.0e30f;
+      for (int k = 0; k < 3; ++k) {
+        float dp = -dot(R_inc.c[k], ref_face_n);
+        if (dp > best_dot) { best_dot = dp; inc_axis = k; inc_sign = 1.0f; }
+        if (-dp > best_dot) { best_dot = -dp; inc_axis = k; inc_sign = -1.0f; }
+      }
+      const f3 inc_n = R_inc.c[inc_axis] * inc_sign;
+      const f3 inc_face_c = c_inc + inc_n * comp(e_inc, inc_axis);
+      const int iax1 = (inc_axis + 1) % 3, iax2 = (inc_axis + 2) % 3;
+      const f3 iu = R_inc.c[iax1], iv = R_inc.c[iax2];
+      const float ieu = comp(e_inc, iax1), iev = comp(e_inc, iax2);
+      f3 poly[2][8];
+      poly[0][0] = (inc_face_c + iu * ieu) + iv * iev;
+      poly[0][1] = (inc_face_c - iu * ieu) + iv * iev;
+      poly[0][2] = (inc_face_c - iu * ieu) - iv * iev;
+      poly[0][3] = (inc_face_c + iu * ieu) - iv * iev;
+      int cur = 0, plen = 4;
+      bool dead = false;
+      for (int pi = 0; pi < 4; ++pi) {
+        f3 pp, pn;
+        if (pi == 0) { pp = ref_face_c + u_ref * eu; pn = u_ref; }
+        else if (pi == 1) { pp = ref_face_c - u_ref * eu; pn = -u_ref; }
+        else if (pi == 2) { pp = ref_face_c + v_ref * ev; pn = v_ref; }
+        else { pp = ref_face_c - v_ref * ev; pn = -v_ref; }
+        if (plen == 0) { dead = true; break; }
+        int olen = 0;
+        const int nx = cur ^ 1;
+        for (int k = 0; k < plen; ++k) {
+          f3 ap = poly[cur][k];
+          f3 bp = poly[cur][(k + 1) % plen];
+          float da = dot(ap - pp, pn), db = dot(bp - pp, pn);
+          if (da <= 0.0f) {
+            if (olen < 8) poly[nx][olen++] = ap;
+            if (db > 0.0f) {
+              float t = da / (da - db);
+              if (olen < 8) poly[nx][olen++] = ap + (bp - ap) * t;
+            }
+          } else if (db <= 0.0f) {
+            float t = da / (da - db);
+            if (olen < 8) poly[nx][olen++] = ap + (bp - ap) * t;
+          }
+        }
+        cur = nx;
+        plen = olen;
+      }
+      if (!dead && plen > 0) {
+        for (int k = 0; k < plen; ++k) {
+          if (nc >= 8) break;
+          f3 p_inc = poly[cur][k];
+          float d_signed = dot(p_inc - ref_face_c, ref_face_n);
+          if (d_signed < margin) {
+            f3 p_ref = p_inc - ref_face_n * d_signed;
+            off_i[nc] = mtv(R_inc, p_inc - c_inc);
+            off_r[nc] = mtv(R_ref, p_ref - c_ref);
+            nc += 1;
+          }
+        }
+      }
+    }
+  }
+  // wave-level compaction of the points
+  int incl = nc;
+  for (int d = 1; d < 64; d <<= 1) {
+    int t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += t;
+  }
+  const int total = __shfl(incl, 63, 64);
+  int base = 0;
+  if (lane == 63 && total > 0) base = atomicAdd(PTC, total);
+  base = __shfl(base, 63, 64);
+  const int first = base + incl - nc;
+  for (int c = 0; c < nc; ++c) {
+    const int k = first + c;
+    PTI[k] = make_int4(pid, inc_b, ref_b, c);
+    PTN[k] = make_float4(ref_face_n.x, ref_face_n.y, ref_face_n.z, 0.0f);
+    PTOI[k] = make_float4(off_i[c].x, off_i[c].y, off_i[c].z, 0.0f);
+    PTOR[k] = make_float4(off_r[c].x, off_r[c].y, off_r[c].z, 0.0f);
+    LAMP[k] = 0.0f;
+  }
+  if (pid < cap) MC[pid] = nc;
+}
+
+// kernels_6dof.py:440-483 (lambda reset folded into the first iteration)
+__global__ __launch_bounds__(kB) void k6_solve_joints(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const int2* __restrict__ JI, const float4* __restrict__ JA, const float4* __restrict__ JB,
+    float* __restrict__ LAMJ, float h, int first, float4* DX, float4* DR, int nj) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= nj) return;
+  const int2 ab = JI[k];
+  const int a = ab.x, b = ab.y;
+  const float4 ja = JA[k], jb = JB[k];
+  const float4 xa4 = X[a], xb4 = X[b];
+  const q4 qa = ldq(Q[a]), qb = ldq(Q[b]);
+  float lam = first ? 0.0f : LAMJ[k];
+  if (first) LAMJ[k] = 0.0f;
+  f3 pa = xyz(xa4) + qrot(qa, xyz(ja));
+  f3 pb = xyz(xb4) + qrot(qb, xyz(jb));
+  f3 d = pa - pb;
+  float dist = length(d);
+  if (dist < XPBD_EPS) return;
+  f3 nn = d / dist;
+  float C = dist - ja.w;
+  f3 ra = pa - xyz(xa4), rb = pb - xyz(xb4);
+  f3 iIa = xyz(IB[a]), iIb = xyz(IB[b]);
+  float w_a = gen_inv_mass(xa4.w, qa, iIa, ra, nn);
+  float w_b = gen_inv_mass(xb4.w, qb, iIb, rb, nn);
+  float wsum = w_a + w_b;
+  if (wsum <= 0.0f) return;
+  float alpha = jb.w / (h * h);
+  float dlam = (-C - alpha * lam) / (wsum + alpha);
+  LAMJ[k] = lam + dlam;
+  f3 p = nn * dlam;
+  accumulate_correction(DX, DR, a, b, xa4.w, xb4.w, qa, qb, iIa, iIb, ra, rb, p);
+}
+
+// kernels_6dof.py:158-211.  One thread owns one body, so its corners add to
+// DX/DR in corner order with plain loads and stores.
+__global__ __launch_bounds__(kB) void k6_solve_floor(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ XP,
+    const float4* __restrict__ QP, const float4* __restrict__ IB, const float4* __restrict__ HE,
+    float floor_y, float mu_s, float* __restrict__ LAMF, float4* DX, float4* DR, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  const float4 x4 = X[i];
+  const float im = x4.w;
+  if (im == 0.0f) return;
+  const f3 x = xyz(x4);
+  const q4 q = ldq(Q[i]);
+  const f3 iI = xyz(IB[i]);
+  const f3 he = xyz(HE[i]);
+  const f3 s = mk3(0.0f, 1.0f, 0.0f);
+  float4 dx = DX[i], dr = DR[i];
+  bool touched = false;
+  float lam[8];
+  for (int c = 0; c < 8; ++c) {
+    lam[c] = 0.0f;
+    const f3 corner = box_corner(he, c);
+    const f3 p1 = x + qrot(q, corner);
+    const float d = floor_y - p1.y;
+    if (d <= 0.0f) continue;
+    const f3 r = p1 - x;
+    const float w = gen_inv_mass(im, q, iI, r, s);
+    if (w <= 0.0f) continue;
+    const float dlam = d / w;
+    lam[c] = dlam;
+    const f3 p = s * dlam;
+    const f3 lin = p * im, ang = world_invI_mul(q, iI, cross(r, p));
+    dx.x += lin.x; dx.y += lin.y; dx.z += lin.z;
+    dr.x += ang.x; dr.y += ang.y; dr.z += ang.z;
+    dx.w += 1.0f;
+    touched = true;
+    if (mu_s > 0.0f) {
+      const f3 p1b = xyz(XP[i]) + qrot(ldq(QP[i]), corner);
+      const f3 dp = p1 - p1b;
+      const f3 dpt = dp - s * dot(dp, s);
+      const float lt = length(dpt);
+      if (lt > XPBD_EPS && lt < mu_s * d) {
+        const float w2 = gen_inv_mass(im, q, iI, r, dpt / lt);
+        if (w2 > 0.0f) {
+          const f3 pf = (-dpt) / w2;
+          const f3 lf = pf * im, af = world_invI_mul(q, iI, cross(r, pf));
+          dx.x += lf.x; dx.y += lf.y; dx.z += lf.z;
+          dr.x += af.x; dr.y += af.y; dr.z += af.z;
+        }
+      }
+    }
+  }
+  float4* lf = (float4*)(LAMF + 8 * (size_t)i);
+  lf[0] = make_float4(lam[0], lam[1], lam[2], lam[3]);
+  lf[1] = make_float4(lam[4], lam[5], lam[6], lam[7]);
+  if (touched) {
+    DX[i] = dx;
+    DR[i] = dr;
+  }
+}
+
+// kernels_6dof.py:385-430, one thread per compacted manifold point
+__global__ __launch_bounds__(kB) void k6_solve_box(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const int4* __restrict__ PTI, const float4* __restrict__ PTN,
+    const float4* __restrict__ PTOI, const float4* __restrict__ PTOR, float* __restrict__ LAMP,
+    const int* __restrict__ PTC, float4* DX, float4* DR, int max_points) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= min(*PTC, max_points)) return;
+  const int4 id = PTI[k];
+  const int inc = id.y, ref = id.z;
+  const f3 s = xyz(PTN[k]);
+  const float4 xi4 = X[inc], xr4 = X[ref];
+  const q4 qi = ldq(Q[inc]), qr = ldq(Q[ref]);
+  const f3 p_inc = xyz(xi4) + qrot(qi, xyz(PTOI[k]));
+  const f3 p_ref = xyz(xr4) + qrot(qr, xyz(PTOR[k]));
+  const float C = dot(p_inc - p_ref, s);
+  if (C >= 0.0f) {
+    LAMP[k] = 0.0f;
+    return;
+  }
+  const float d = -C;
+  const f3 r_inc = p_inc - xyz(xi4), r_ref = p_ref - xyz(xr4);
+  const f3 iIi = xyz(IB[inc]), iIr = xyz(IB[ref]);
+  const float w_inc = gen_inv_mass(xi4.w, qi, iIi, r_inc, s);
+  const float w_ref = gen_inv_mass(xr4.w, qr, iIr, r_ref, s);
+  const float wsum = w_inc + w_ref;
+  if (wsum <= 0.0f) return;
+  const float dlam = d / wsum;
+  LAMP[k] = LAMP[k] + dlam;
+  const f3 p = s * dlam;
+  accumulate_correction(DX, DR, inc, ref, xi4.w, xr4.w, qi, qr, iIi, iIr, r_inc, r_ref, p);
+}
+
+// kernels_6dof.py:486-506
+__global__ __launch_bounds__(kB) void k6_apply_jacobi(float4* X, float4* Q, float4* DX,
+                                                      float4* DR, float relax, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  const float4 acc = DX[i];
+  const float4 x4 = X[i];
+  if (acc.w > 0.0f && x4.w > 0.0f) {
+    const float inv_n = relax / acc.w;
+    const f3 x = xyz(x4) + xyz(acc) * inv_n;
+    const f3 dw = xyz(DR[i]) * inv_n;
+    const q4 wq{dw.x, dw.y, dw.z, 0.0f};
+    const q4 q = ldq(Q[i]);
+    X[i] = make_float4(x.x, x.y, x.z, x4.w);
+    Q[i] = stq(qnormalize(qadd(q, qscale(qmul(wq, q), 0.5f))));
+  }
+  if (acc.w != 0.0f || acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f) {
+    DX[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    DR[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// kernels_6dof.py:512-535
+__global__ __launch_bounds__(kB) void k6_finalize(const float4* __restrict__ X,
+                                                  const float4* __restrict__ Q,
+                                                  const float4* __restrict__ XP,
+                                                  const float4* __restrict__ QP, float4* V,
+                                                  float4* W, float h, float lin_damp,
+                                                  float ang_damp, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  const float4 x4 = X[i];
+  if (x4.w == 0.0f) {
+    V[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    W[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const f3 v = ((xyz(x4) - xyz(XP[i])) / h) * (1.0f - lin_damp);
+  const q4 dq = qmul(ldq(Q[i]), qconj(ldq(QP[i])));
+  f3 w = mk3(dq.x, dq.y, dq.z) * (2.0f / h);
+  if (dq.w < 0.0f) w = -w;
+  w = w * (1.0f - ang_damp);
+  V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+  W[i] = make_float4(w.x, w.y, w.z, 0.0f);
+}
+
+// Shared tail of velocity_floor / velocity_box (kernels_6dof.py:575-583):
+// dynamic friction Eq. 30 and restitution.
+__device__ inline f3 velocity_impulse(f3 vrel, f3 s, float lam_n, float mu_d, float restitution,
+                                      float h) {
+  const float vn = dot(vrel, s);
+  const f3 vt = vrel - s * vn;
+  const float ltan = length(vt);
+  f3 impulse = mk3(0.0f, 0.0f, 0.0f);
+  if (mu_d > 0.0f && ltan > XPBD_EPS) {
+    const float fn = lam_n / (h * h);
+    const float dvm = fminf(h * mu_d * fabsf(fn), ltan);
+    impulse = impulse - (vt / ltan) * dvm;
+  }
+  if (vn < 0.0f) impulse = impulse + s * (-vn + fmaxf(-restitution * vn, 0.0f));
+  return impulse;
+}
+
+// kernels_6dof.py:541-590; one thread owns one body (plain RMW of DV/DW)
+__global__ __launch_bounds__(kB) void k6_velocity_floor(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const float4* __restrict__ HE, float mu_d, float restitution,
+    const float* __restrict__ LAMF, float h, const float4* __restrict__ V,
+    const float4* __restrict__ W, float4* DV, float4* DW, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  const float im = X[i].w;
+  if (im == 0.0f) return;
+  const float4* lf4 = (const float4*)(LAMF + 8 * (size_t)i);
+  const float4 l0 = lf4[0], l1 = lf4[1];
+  const float lam[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+  const q4 q = ldq(Q[i]);
+  const f3 iI = xyz(IB[i]), he = xyz(HE[i]);
+  const f3 v = xyz(V[i]), w = xyz(W[i]);
+  const f3 s = mk3(0.0f, 1.0f, 0.0f);
+  float4 dv = DV[i], dw = DW[i];
+  bool touched = false;
+  for (int c = 0; c < 8; ++c) {
+    const float lam_n = lam[c];
+    if (lam_n == 0.0f) continue;
+    const f3 r = qrot(q, box_corner(he, c));
+    const f3 vc = v + cross(w, r);
+    const f3 impulse = velocity_impulse(vc, s, lam_n, mu_d, restitution, h);
+    if (length(impulse) > XPBD_EPS) {
+      const float wg = gen_inv_mass(im, q, iI, r, impulse / length(impulse));
+      if (wg > 0.0f) {
+        const f3 p = impulse / wg;
+        const f3 lin = p * im, ang = world_invI_mul(q, iI, cross(r, p));
+        dv.x += lin.x; dv.y += lin.y; dv.z += lin.z; dv.w += 1.0f;
+        dw.x += ang.x; dw.y += ang.y; dw.z += ang.z;
+        touched = true;
+      }
+    }
+  }
+  if (touched) {
+    DV[i] = dv;
+    DW[i] = dw;
+  }
+}
+
+// kernels_6dof.py:593-659, one thread per compacted manifold point
+__global__ __launch_bounds__(kB) void k6_velocity_box(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const int4* __restrict__ PTI, const float4* __restrict__ PTN,
+    const float4* __restrict__ PTOI, const float4* __restrict__ PTOR,
+    const float* __restrict__ LAMP, const int* __restrict__ PTC, float mu_d, float restitution,
+    float h, const float4* __restrict__ V, const float4* __restrict__ W, float4* DV, float4* DW,
+    int max_points) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= min(*PTC, max_points)) return;
+  const float lam_n = LAMP[k];
+  if (lam_n == 0.0f) return;
+  const int4 id = PTI[k];
+  const int ci = id.y, cj = id.z;
+  const f3 s = xyz(PTN[k]);
+  const float4 xi4 = X[ci], xj4 = X[cj];
+  const q4 qi = ldq(Q[ci]), qj = ldq(Q[cj]);
+  const f3 p_inc = xyz(xi4) + qrot(qi, xyz(PTOI[k]));
+  const f3 p_ref = xyz(xj4) + qrot(qj, xyz(PTOR[k]));
+  const f3 ri = p_inc - xyz(xi4), rj = p_ref - xyz(xj4);
+  const f3 vci = xyz(V[ci]) + cross(xyz(W[ci]), ri);
+  const f3 vcj = xyz(V[cj]) + cross(xyz(W[cj]), rj);
+  const f3 impulse = velocity_impulse(vci - vcj, s, lam_n, mu_d, restitution, h);
+  const float li = length(impulse);
+  if (li > XPBD_EPS) {
+    const f3 ndir = impulse / li;
+    const f3 iIi = xyz(IB[ci]), iIj = xyz(IB[cj]);
+    const float wi = gen_inv_mass(xi4.w, qi, iIi, ri, ndir);
+    const float wj = gen_inv_mass(xj4.w, qj, iIj, rj, ndir);
+    const float wsum = wi + wj;
+    if (wsum > 0.0f) {
+      const f3 p = impulse / wsum;
+      if (xi4.w > 0.0f) {
+        atomic_add3(&DV[ci], p * xi4.w);
+        atomic_add3(&DW[ci], world_invI_mul(qi, iIi, cross(ri, p)));
+        atomicAdd(&DV[ci].w, 1.0f);
+      }
+      if (xj4.w > 0.0f) {
+        atomic_add3(&DV[cj], (-p) * xj4.w);
+        atomic_add3(&DW[cj], -world_invI_mul(qj, iIj, cross(rj, p)));
+        atomicAdd(&DV[cj].w, 1.0f);
+      }
+    }
+  }
+}
+
+// kernels_6dof.py:662-691: apply_velocity_6dof then cap_velocity_6dof
+__global__ __launch_bounds__(kB) void k6_apply_velocity(const float4* __restrict__ X, float4* V,
+                                                        float4* W, float4* DV, float4* DW,
+                                                        float max_lin, float max_ang, int do_cap,
+                                                        int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  const float4 acc = DV[i];
+  f3 v = xyz(V[i]), w = xyz(W[i]);
+  bool dirty = false;
+  if (acc.w > 0.0f && X[i].w > 0.0f) {
+    const float inv_n = 1.0f / acc.w;
+    v = v + xyz(acc) * inv_n;
+    w = w + xyz(DW[i]) * inv_n;
+    dirty = true;
+  }
+  if (acc.w != 0.0f || acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f) {
+    DV[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    DW[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if (do_cap) {
+    const float sp = length(v);
+    if (max_lin > 0.0f && sp > max_lin) { v = v * (max_lin / sp); dirty = true; }
+    const float sa = length(w);
+    if (max_ang > 0.0f && sa > max_ang) { w = w * (max_ang / sa); dirty = true; }
+  }
+  if (dirty) {
+    V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+    W[i] = make_float4(w.x, w.y, w.z, 0.0f);
+  }
+}
+
+inline int nblk(long long n) { return (int)((n + kB - 1) / kB); }
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+#define F4(p) ((float4*)(p))
+#define CF4(p) ((const float4*)(p))
+
+extern "C" {
+
+int xpbd6_integrate(void* stream, float* X, float* Q, float* V, float* W, float* XP, float* QP,
+                    const float* IB, float* LAMF, int* PTC, float gx, float gy, float gz,
+                    float h, int gyro, int n) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k6_integrate, dim3(nblk(n)), dim3(kB), 0, (hipStream_t)stream, F4(X), F4(Q),
+                     F4(V), F4(W), F4(XP), F4(QP), CF4(IB), LAMF, PTC, gx, gy, gz, h, gyro, n);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_box_manifold(void* stream, const float* X, const float* Q, const float* HE,
+                       const int* PAIRS, const int* PC, float margin, int cap, int* MC, int* PTI,
+                       float* PTN, float* PTOI, float* PTOR, float* LAMP, int* PTC) {
+  if (cap <= 0) return 0;
+  hipLaunchKernelGGL(k6_box_manifold, dim3(nblk(cap)), dim3(kB), 0, (hipStream_t)stream, CF4(X),
+                     CF4(Q), CF4(HE), (const int2*)PAIRS, PC, margin, cap, MC, (int4*)PTI,
+                     F4(PTN), F4(PTOI), F4(PTOR), LAMP, PTC);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_solve_joints(void* stream, const float* X, const float* Q, const float* IB,
+                       const int* JI, const float* JA, const float* JB, float* LAMJ, float h,
+                       int first, float* DX, float* DR, int nj) {
+  if (nj <= 0) return 0;
+  hipLaunchKernelGGL(k6_solve_joints, dim3(nblk(nj)), dim3(kB), 0, (hipStream_t)stream, CF4(X),
+                     CF4(Q), CF4(IB), (const int2*)JI, CF4(JA), CF4(JB), LAMJ, h, first, F4(DX),
+                     F4(DR), nj);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_solve_floor(void* stream, const float* X, const float* Q, const float* XP,
+                      const float* QP, const float* IB, const float* HE, float floor_y,
+                      float mu_s, float* LAMF, float* DX, float* DR, int n) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k6_solve_floor, dim3(nblk(n)), dim3(kB), 0, (hipStream_t)stream, CF4(X),
+                     CF4(Q), CF4(XP), CF4(QP), CF4(IB), CF4(HE), floor_y, mu_s, LAMF, F4(DX),
+                     F4(DR), n);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_solve_box(void* stream, const float* X, const float* Q, const float* IB,
+                    const int* PTI, const float* PTN, const float* PTOI, const float* PTOR,
+                    float* LAMP, const int* PTC, float* DX, float* DR, int max_points) {
+  if (max_points <= 0) return 0;
+  hipLaunchKernelGGL(k6_solve_box, dim3(nblk(max_points)), dim3(kB), 0, (hipStream_t)stream,
+                     CF4(X), CF4(Q), CF4(IB), (const int4*)PTI, CF4(PTN), CF4(PTOI), CF4(PTOR),
+                     LAMP, PTC, F4(DX), F4(DR), max_points);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_apply_jacobi(void* stream, float* X, float* Q, float* DX, float* DR, float relax,
+                       int n) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k6_apply_jacobi, dim3(nblk(n)), dim3(kB), 0, (hipStream_t)stream, F4(X),
+                     F4(Q), F4(DX), F4(DR), relax, n);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_finalize_velocity(void* stream, const float* X, const float* Q, const float* XP,
+                            const float* QP, float* V, float* W, float h, float lin_damp,
+                            float ang_damp, int n) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k6_finalize, dim3(nblk(n)), dim3(kB), 0, (hipStream_t)stream, CF4(X),
+                     CF4(Q), CF4(XP), CF4(QP), F4(V), F4(W), h, lin_damp, ang_damp, n);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_velocity_floor(void* stream, const float* X, const float* Q, const float* IB,
+                         const float* HE, float mu_d, float restitution, const float* LAMF,
+                         float h, const float* V, const float* W, float* DV, float* DW, int n) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k6_velocity_floor, dim3(nblk(n)), dim3(kB), 0, (hipStream_t)stream, CF4(X),
+                     CF4(Q), CF4(IB), CF4(HE), mu_d, restitution, LAMF, h, CF4(V), CF4(W),
+                     F4(DV), F4(DW), n);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_velocity_box(void* stream, const float* X, const float* Q, const float* IB,
+                       const int* PTI, const float* PTN, const float* PTOI, const float* PTOR,
+                       const float* LAMP, const int* PTC, float mu_d, float restitution, float h,
+                       const float* V, const float* W, float* DV, float* DW, int max_points) {
+  if (max_points <= 0) return 0;
+  hipLaunchKernelGGL(k6_velocity_box, dim3(nblk(max_points)), dim3(kB), 0, (hipStream_t)stream,
+                     CF4(X), CF4(Q), CF4(IB), (const int4*)PTI, CF4(PTN), CF4(PTOI), CF4(PTOR),
+                     LAMP, PTC, mu_d, restitution, h, CF4(V), CF4(W), F4(DV), F4(DW),
+                     max_points);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_apply_velocity(void* stream, const float* X, float* V, float* W, float* DV, float* DW,
+                         float max_lin, float max_ang, int do_cap, int n) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k6_apply_velocity, dim3(nblk(n)), dim3(kB), 0, (hipStream_t)stream, CF4(X),
+                     F4(V), F4(W), F4(DV), F4(DW), max_lin, max_ang, do_cap, n);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"

@@ -186,3 +186,84 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
         # no ATTACH rows: the anchor array is never read, one slot suffices
         ANC=torch.zeros((1, 4), device=device), LAM=torch.zeros(m, device=device),
         n_particles=n, n_rows=m, n_distance=int(is_d.sum()))
+
+
+def _random_quats(rng, n):
+    """Uniform random unit quaternions (Shoemake 1992), xyzw."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1 - u1), np.sqrt(u1)
+    return np.stack([a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2),
+                     b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)], axis=1)
+
+
+def box_pile(solver_cls, nx, ny, nz, seed=0, **kw):
+    """examples/benchmark.py:23-34: an nx x ny x nz pile of 0.16 m cubes of 1 kg."""
+    kw.setdefault("dt", 1 / 60)
+    kw.setdefault("friction", 0.6)
+    s = solver_cls(**kw)
+    h = 0.16
+    rng = np.random.default_rng(seed)
+    sp = 2.25 * h
+    for ly in range(ny):
+        for ix in range(nx):
+            for iz in range(nz):
+                s.add_box((ix * sp + rng.uniform(-1e-3, 1e-3), h + ly * sp,
+                           iz * sp + rng.uniform(-1e-3, 1e-3)), (h, h, h), mass=1.0)
+    return s
+
+
+def unified_scene(solver_cls, seed=0, lattice=(12, 10, 12), cloth_res=32, chain_links=20,
+                  **kw):
+    """Config 3: 1440 boxes with half extents U(0.1, 0.3) m and density
+    1000 kg/m^3 on a jittered 12 x 12 x 10 lattice above the floor with
+    uniform random orientations (mu 0.5, restitution 0.2), plus a 32 x 32
+    particle cloth hung from its top row by compliant joints and a 20-link
+    joint chain (compliance 1e-5) hung from a static anchor."""
+    kw.setdefault("dt", 1 / 60)
+    kw.setdefault("substeps", 15)
+    kw.setdefault("iterations", 2)
+    kw.setdefault("friction", 0.5)
+    kw.setdefault("restitution", 0.2)
+    s = solver_cls(**kw)
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = lattice
+    sp = 1.1
+    quats = _random_quats(rng, nx * ny * nz)
+    k = 0
+    for iy in range(ny):
+        for ix in range(nx):
+            for iz in range(nz):
+                he = rng.uniform(0.1, 0.3, 3)
+                m = 8.0 * 1000.0 * he[0] * he[1] * he[2]
+                p = (ix * sp + rng.uniform(-0.05, 0.05), 0.6 + iy * sp,
+                     iz * sp + rng.uniform(-0.05, 0.05))
+                s.add_box(p, tuple(he), mass=float(m), quaternion=tuple(quats[k]))
+                k += 1
+    # cloth curtain beside the pile
+    R = cloth_res
+    cs = 1.6 / (R - 1)
+    idx = np.empty((R, R), np.int64)
+    for iy in range(R):
+        for ix in range(R):
+            pinned = iy == 0
+            b = s.add_particle((-3.0, 4.0 - iy * cs, ix * cs), mass=0.0 if pinned else 0.02,
+                               radius=0.012, group=1, static=pinned)
+            idx[iy, ix] = b.index
+    for iy in range(R):
+        for ix in range(R):
+            if ix + 1 < R:
+                s.add_joint(int(idx[iy, ix]), int(idx[iy, ix + 1]), compliance=1e-7)
+            if iy + 1 < R:
+                s.add_joint(int(idx[iy, ix]), int(idx[iy + 1, ix]), compliance=1e-7)
+            if ix + 1 < R and iy + 1 < R:
+                s.add_joint(int(idx[iy, ix]), int(idx[iy + 1, ix + 1]), compliance=2e-7)
+    # compliant link chain from a static anchor
+    hl = 0.08
+    prev = s.add_box((-6.0, 5.0, 0.0), (0.05, 0.05, 0.05), mass=0.0, static=True)
+    for i in range(chain_links):
+        b = s.add_box((-6.0 + (i + 1) * 2 * hl, 5.0, 0.0), (hl, 0.03, 0.03), mass=0.5)
+        anchor_prev = (0.0, 0.0, 0.0) if i == 0 else (hl, 0.0, 0.0)
+        s.add_joint(prev, b, compliance=1e-5, rest_length=0.0, anchor_a=anchor_prev,
+                    anchor_b=(-hl, 0.0, 0.0))
+        prev = b
+    return s

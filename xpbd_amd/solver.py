@@ -33,6 +33,17 @@ CONTACT = 3
 
 MODE_GS = 0
 MODE_JACOBI = 1
+MODE_JACOBI_TILE = 2
+
+
+def default_tile_size(n_particles: int) -> int:
+    """Particles per tile of the fused Jacobi kernel: large tiles keep the
+    share of re-projected halo rows small; small scenes get smaller tiles so
+    that the launch still spreads over the 256 CUs."""
+    for t in (2048, 1024, 512, 256):
+        if n_particles >= 256 * t:
+            return t
+    return 128
 
 
 class _Col:
@@ -157,6 +168,43 @@ def choose_jacobi_tiling(rows_sorted: np.ndarray, block: int = 256) -> tuple[int
     return rpt, window
 
 
+def morton_order(pos: np.ndarray) -> np.ndarray:
+    """Particle order along a 3-D Z-order curve (21 bits per axis over the
+    bounding box of ``pos``): a permutation that keeps spatial neighbours,
+    and therefore the endpoints of a constraint, close in memory."""
+    n = pos.shape[0]
+    if n < 2:
+        return np.arange(n, dtype=np.int64)
+    lo = pos.min(axis=0)
+    ext = np.maximum(pos.max(axis=0) - lo, 1e-30)
+    q = np.clip(((pos - lo) / ext * (1 << 21)).astype(np.int64), 0, (1 << 21) - 1)
+    code = np.zeros(n, dtype=np.uint64)
+    for bit in range(21):
+        for ax in range(3):
+            code |= ((q[:, ax] >> bit) & 1).astype(np.uint64) << np.uint64(3 * bit + 2 - ax)
+    return np.argsort(code, kind="stable")
+
+
+def tile_tables(rows_sorted: np.ndarray, n_particles: int, tile: int):
+    """Tile tables of the fused Jacobi kernel for rows sorted by smallest
+    endpoint: ``tile_row_start`` (first owned row of each tile, n_tiles + 1
+    entries) and the halo rows (larger endpoint in a later tile than the
+    smaller one) grouped by the tile of the larger endpoint."""
+    n_tiles = max(1, (n_particles + tile - 1) // tile)
+    a = (rows_sorted[:, 0] & 0x0FFFFFFF).astype(np.int64)
+    b = rows_sorted[:, 1].astype(np.int64)
+    lo = np.where(b >= 0, np.minimum(a, b), a)
+    hi = np.where(b >= 0, np.maximum(a, b), a)
+    starts = np.searchsorted(lo, np.arange(n_tiles + 1, dtype=np.int64) * tile, "left")
+    starts[-1] = len(lo)
+    cross = np.nonzero((hi // tile) != (lo // tile))[0]
+    ht = hi[cross] // tile
+    order = np.argsort(ht, kind="stable")
+    halo_rows = cross[order].astype(np.int32)
+    halo_start = np.searchsorted(ht[order], np.arange(n_tiles + 1), "left").astype(np.int32)
+    return starts.astype(np.int32), halo_start, halo_rows
+
+
 class Solver:
     """3D XPBD particle solver (solver.py:57-126).
 
@@ -191,6 +239,11 @@ class Solver:
         assert solve_mode in ("gs", "jacobi")
         self.solve_mode = solve_mode
         self.use_graph = True
+        # Jacobi kernel: "tile" (one fused launch per iteration) or "window"
+        # (projection with LDS-window atomics + a separate apply pass)
+        self.jacobi_kernel = "tile"
+        self.tile_size = 0            # 0 = chosen from the scene size
+        self.reorder = True           # Z-order particle layout for Jacobi
 
         # host scene, one column per field (float64 as the reference's
         # Python floats; rounded to float32 once, at flush)
@@ -224,6 +277,10 @@ class Solver:
         self.num_colors = 0
         self.color_counts: dict = {}
         self._tiling = (1, 0)
+        self._pperm = None            # device particle k <- API particle _pperm[k]
+        self._pinv = None             # API particle i -> device particle _pinv[i]
+        self.X_B = self.LAM_B = None
+        self._tile_tabs = None
         self._plan = None
         self._plan_sig = None
         self._gen = 0
@@ -352,17 +409,31 @@ class Solver:
             self._contact_pool_start = len(self._rt)
 
     # ---- runtime perturbations (solver.py:232-271); in place on device ----
+    def _dev(self, i: int) -> int:
+        return int(i) if self._pinv is None else int(self._pinv[int(i)])
+
     def set_position(self, body, p) -> None:
         self._flush()
-        self.X[body.index, :3] = self._t(np.asarray(p, np.float32))
+        self.X[self._dev(body.index), :3] = self._t(np.asarray(p, np.float32))
 
     def set_velocity(self, body, vel) -> None:
         self._flush()
-        self.V[body.index, :3] = self._t(np.asarray(vel, np.float32))
+        self.V[self._dev(body.index), :3] = self._t(np.asarray(vel, np.float32))
 
     def add_impulse(self, body, dv) -> None:
         self._flush()
-        self.V[body.index, :3] += self._t(np.asarray(dv, np.float32))
+        self.V[self._dev(body.index), :3] += self._t(np.asarray(dv, np.float32))
+
+    def set_state(self, x, v) -> None:
+        """Overwrite all positions and velocities ((n, 3) arrays in API
+        particle order) on the device."""
+        self._flush()
+        x = np.asarray(x, np.float32).reshape(-1, 3)
+        v = np.asarray(v, np.float32).reshape(-1, 3)
+        if self._pperm is not None:
+            x, v = x[self._pperm], v[self._pperm]
+        self.X[:, :3] = self._t(x)
+        self.V[:, :3] = self._t(v)
 
     def set_all_compliance(self, value: float, only_distance: bool = True) -> None:
         """Live compliance update; rewrites the device rows in place (no
@@ -472,12 +543,25 @@ class Solver:
             if 0 < prev <= n_b:
                 x_np[:prev] = self.positions()
                 v_np[:prev] = self.velocities()
+        tile_mode = self.solve_mode == "jacobi" and self.jacobi_kernel == "tile"
+        if tile_mode and self.reorder and n_b > 1:
+            pperm = morton_order(x_np.astype(np.float64))
+            pinv = np.empty_like(pperm)
+            pinv[pperm] = np.arange(n_b, dtype=np.int64)
+        else:
+            pperm = pinv = None
+        inv_mass = h["inv_mass"]
+        ra, rb = h["a"], h["b"]
+        if pperm is not None:
+            x_np, v_np, inv_mass = x_np[pperm], v_np[pperm], inv_mass[pperm]
+            ra = pinv[ra].astype(np.int32)
+            rb = np.where(rb >= 0, pinv[np.maximum(rb, 0)], -1).astype(np.int32)
         X = np.zeros((n_b, 4), np.float32)
         X[:, :3] = x_np
-        X[:, 3] = h["inv_mass"]
+        X[:, 3] = inv_mass
         V = np.zeros((n_b, 4), np.float32)
         V[:, :3] = v_np
-        rows, anc = pack_rows(h["type"], h["a"], h["b"], h["rest"], h["compliance"],
+        rows, anc = pack_rows(h["type"], ra, rb, h["rest"], h["compliance"],
                               h["anchor"], h["mu"])
         if self.solve_mode == "gs" and n_c > 0:
             color = color_constraints(n_b, h["a"], h["b"])
@@ -487,13 +571,19 @@ class Solver:
             self.num_colors = num_colors
             self.color_counts = color_summary(color)
         else:
-            perm = jacobi_row_order(h["a"], h["b"]) if n_c else np.zeros(0, np.int64)
+            perm = jacobi_row_order(ra, rb) if n_c else np.zeros(0, np.int64)
             self._color_starts = np.array([0, n_c], dtype=np.int32)
             self.num_colors = 1 if n_c else 0
             self.color_counts = {}
         rows = np.ascontiguousarray(rows[perm])
         anc = np.ascontiguousarray(anc[perm])
         self._tiling = choose_jacobi_tiling(rows) if n_c else (1, 0)
+        self._tile = 0
+        self._tile_tabs = None
+        if tile_mode and n_c:
+            self._tile = self.tile_size or default_tile_size(n_b)
+            self._tile_tabs = tile_tables(rows, n_b, self._tile)
+        self._pperm, self._pinv = pperm, pinv
         dev = self.device
         self.X = torch.from_numpy(X).to(dev)
         self.V = torch.from_numpy(V).to(dev)
@@ -502,6 +592,16 @@ class Solver:
         self.R = torch.from_numpy(rows).to(dev)
         self.ANC = torch.from_numpy(anc).to(dev)
         self.LAM = torch.zeros(max(n_c, 1), dtype=torch.float32, device=dev)
+        if self._tile_tabs is not None:
+            self.X_B = self.X.clone()
+            self.LAM_B = torch.zeros_like(self.LAM)
+            ts, hs, hr = self._tile_tabs
+            self._tile_dev = (torch.from_numpy(ts).to(dev), torch.from_numpy(hs).to(dev),
+                              torch.from_numpy(np.ascontiguousarray(
+                                  hr if len(hr) else np.zeros(1, np.int32))).to(dev))
+        else:
+            self.X_B = self.LAM_B = None
+            self._tile_dev = None
         self._perm = perm
         self._gen += 1
         self._drop_plan()
@@ -509,7 +609,11 @@ class Solver:
 
     def _repack_rows_inplace(self) -> None:
         h = self.host_arrays()
-        rows, anc = pack_rows(h["type"], h["a"], h["b"], h["rest"], h["compliance"],
+        ra, rb = h["a"], h["b"]
+        if self._pinv is not None:
+            ra = self._pinv[ra].astype(np.int32)
+            rb = np.where(rb >= 0, self._pinv[np.maximum(rb, 0)], -1).astype(np.int32)
+        rows, anc = pack_rows(h["type"], ra, rb, h["rest"], h["compliance"],
                               h["anchor"], h["mu"])
         if len(rows):
             self.R.copy_(self._t(np.ascontiguousarray(rows[self._perm])))
@@ -518,7 +622,8 @@ class Solver:
     # ---- the step (solver.py:412-467) --------------------------------------
     def _signature(self):
         return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
-                self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling)
+                self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling,
+                self._tile)
 
     def _drop_plan(self):
         if self._plan is not None:
@@ -542,6 +647,11 @@ class Solver:
         d.color_starts = self._starts_buf.ctypes.data
         d.num_colors = int(len(self._starts_buf) - 1)
         d.mode = MODE_GS if self.solve_mode == "gs" else MODE_JACOBI
+        if self._tile_dev is not None:
+            d.mode = MODE_JACOBI_TILE
+            d.x_alt, d.lam_alt = N.ptr(self.X_B), N.ptr(self.LAM_B)
+            d.tile_row_start, d.halo_start, d.halo_rows = (N.ptr(t) for t in self._tile_dev)
+            d.tile_size = self._tile
         d.substeps, d.iterations = self.substeps, self.iterations
         d.h = float(np.float32(self.dt / float(self.substeps)))
         d.gx, d.gy, d.gz = self.gravity
@@ -586,27 +696,38 @@ class Solver:
         bad = np.nonzero(~np.all(np.isfinite(xs), axis=1))[0]
         safe = self._x.array().astype(np.float32)
         for i in bad.tolist():
-            self.X[i, :3] = self._t(safe[i])
-            self.V[i, :3] = 0.0
+            k = self._dev(i)
+            self.X[k, :3] = self._t(safe[i])
+            self.V[k, :3] = 0.0
 
     # ---- read-back (solver.py:487-508) ---------------------------------------
+    def _api_order(self, dev_arr: np.ndarray) -> np.ndarray:
+        if self._pperm is None:
+            return dev_arr
+        out = np.empty_like(dev_arr)
+        out[self._pperm] = dev_arr
+        return out
+
     def positions(self) -> np.ndarray:
         if self.X is None:
             return self._x.array().astype(np.float32)
-        return self.X[:, :3].cpu().numpy().copy()
+        return self._api_order(self.X[:, :3].cpu().numpy())
 
     def velocities(self) -> np.ndarray:
         if self.V is None:
             return self._v.array().astype(np.float32)
-        return self.V[:, :3].cpu().numpy().copy()
+        return self._api_order(self.V[:, :3].cpu().numpy())
 
     def lambdas(self) -> np.ndarray:
-        """Multipliers in API row order."""
+        """Multipliers in API row order (after the last iteration)."""
         n_c = len(self._rt)
         if self.LAM is None or self._dirty:
             return np.zeros(n_c, dtype=np.float32)
+        lam = self.LAM
+        if self._tile_dev is not None and self.iterations % 2 == 1:
+            lam = self.LAM_B
         out = np.zeros(n_c, dtype=np.float32)
-        out[self._perm] = self.LAM[:n_c].cpu().numpy()
+        out[self._perm] = lam[:n_c].cpu().numpy()
         return out
 
     @property

@@ -11,9 +11,9 @@ Metric: constraint projections per second = rows x substeps x iterations /
 frame time (the reference's definition, examples/benchmark.py:123-130).
 
 Also reported on the same JSON line:
-* ``roofline``       — the dominant kernel (the Jacobi projection) at this
-  workload: algorithmic bytes per launch / average launch time, timed with
-  HIP events on the launch stream, against 8 TB/s.
+* ``roofline``       — the dominant kernel (k_jacobi_tile: one fused Jacobi
+  iteration) at this workload: algorithmic bytes per launch / average launch
+  time, timed with HIP events on the launch stream, against 8 TB/s.
 * ``roofline_scene`` — the same kernel and a full frame on 8 x 1024^2 cloths
   (58.6M rows, a working set far above the caches), built on the device.
 * ``cpu_baseline``   — the serial CPU restatement of the reference kernels
@@ -38,13 +38,13 @@ METRIC = "constraint proj/s (1 GPU); substeps/s at fixed residual, learned vs pl
 HBM_PEAK_GBS = 8000.0
 
 
-def proj_kernel_bytes(n_distance: int, n_floor: int, n_particles: int) -> int:
-    """Algorithmic HBM bytes of one Jacobi projection launch: 24 B per
-    DISTANCE row (a, b, rest, alpha, lambda read + write), 20 B per FLOOR row
-    (a, floor_y, mu, lambda read + write), and per particle 16 B (x + inverse
-    mass read) + 12 B (x_prev read for friction).  The x write (12 B/particle)
-    belongs to the apply pass."""
-    return 24 * n_distance + 20 * n_floor + 28 * n_particles
+def iter_bytes(n_distance: int, n_floor: int, n_particles: int) -> int:
+    """Algorithmic HBM bytes of one Jacobi iteration (the unit of one
+    k_jacobi_tile launch): 24 B per DISTANCE row (a, b, rest, alpha, lambda
+    read + write), 20 B per FLOOR row (a, floor_y, mu, lambda read + write),
+    and per particle 28 B (x read + write, inverse mass read) + 12 B (x_prev
+    read for friction)."""
+    return 24 * n_distance + 20 * n_floor + 40 * n_particles
 
 
 def time_kernel(fn, reps: int) -> float:
@@ -64,13 +64,14 @@ def time_kernel(fn, reps: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def jacobi_launcher(N, X, XP, R, ANC, LAM, ACC, h, n_rows, tiling):
+def tile_launcher(N, X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR, h, tile, n):
+    """One k_jacobi_tile launch (a full Jacobi iteration) on fixed buffers."""
     lib = N.lib()
-    args = (N.ptr(X), N.ptr(XP), N.ptr(R), N.ptr(ANC), N.ptr(LAM), N.ptr(ACC))
+    args = tuple(N.ptr(t) for t in (X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR))
 
     def fn():
-        N.check(lib.xpbd_solve_jacobi(N.current_stream(), *args, h, 0, n_rows, tiling[0],
-                                      tiling[1]), "xpbd_solve_jacobi")
+        N.check(lib.xpbd_jacobi_tile(N.current_stream(), *args, h, 0, tile, n, 1.0, 0, 0.0,
+                                     40.0, None), "xpbd_jacobi_tile")
     return fn
 
 
@@ -101,29 +102,29 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     import torch
 
     from xpbd_amd.scenes import packed_cloth_device
-    from xpbd_amd.solver import MODE_JACOBI
+    from xpbd_amd.solver import MODE_JACOBI_TILE
 
     res, sheets = 1024, 8
     sc = packed_cloth_device(res, sheets, "cuda:0")
     torch.cuda.synchronize()
     n, m, nd = sc["n_particles"], sc["n_rows"], sc["n_distance"]
-    tiling = (2, 4096)
     h = float(torch.tensor(1.0 / 60.0 / 15.0, dtype=torch.float32))
-    launch = jacobi_launcher(N, sc["X"], sc["XP"], sc["R"], sc["ANC"], sc["LAM"], sc["ACC"], h, m,
-                             tiling)
+    launch = tile_launcher(N, sc["X"], sc["X_B"], sc["XP"], sc["V"], sc["R"], sc["ANC"],
+                           sc["LAM"], sc["LAM_B"], sc["TRS"], sc["HS"], sc["HR"], h, sc["tile"], n)
     ms = time_kernel(launch, reps)
-    sc["ACC"].zero_()
-    nbytes = proj_kernel_bytes(nd, m - nd, n)
+    nbytes = iter_bytes(nd, m - nd, n)
     d = N.ParticlesDesc()
     d.x, d.v, d.x_prev, d.acc = (N.ptr(sc["X"]), N.ptr(sc["V"]), N.ptr(sc["XP"]),
                                  N.ptr(sc["ACC"]))
     d.rows, d.anchors, d.lam = N.ptr(sc["R"]), N.ptr(sc["ANC"]), N.ptr(sc["LAM"])
     d.n_particles, d.n_rows = n, m
     d.color_starts, d.num_colors = None, 0
-    d.mode, d.substeps, d.iterations = MODE_JACOBI, 15, 2
+    d.mode, d.substeps, d.iterations = MODE_JACOBI_TILE, 15, 2
     d.h, d.gx, d.gy, d.gz = h, 0.0, -9.81, 0.0
     d.damping, d.relax, d.max_speed = 0.0, 1.0, 40.0
-    d.rows_per_thread, d.window = tiling
+    d.x_alt, d.lam_alt = N.ptr(sc["X_B"]), N.ptr(sc["LAM_B"])
+    d.tile_row_start, d.halo_start, d.halo_rows = N.ptr(sc["TRS"]), N.ptr(sc["HS"]), N.ptr(sc["HR"])
+    d.tile_size = sc["tile"]
     plan = N.c_void_p()
     N.check(N.lib().xpbd_frame3_create(N.ctypes.byref(d), N.ctypes.byref(plan)), "create")
     run = lambda: N.check(N.lib().xpbd_frame3_run(plan, N.current_stream(), 1, 1), "run")  # noqa
@@ -132,13 +133,58 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     achieved = nbytes / (ms * 1e-3) / 1e9
     return {
         "workload": f"{sheets} x {res}^2 cloth: {n} particles, {nd} DISTANCE + {m - nd} FLOOR "
-                    "rows, jacobi 15x2",
-        "kernel": "k_jacobi_project",
+                    f"rows ({sc['n_halo']} halo rows re-projected), jacobi 15x2",
+        "kernel": "k_jacobi_tile",
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": nbytes,
         "kernel_ms": round(ms, 4), "frame_ms": round(frame_ms, 3),
         "proj_per_s": m * 30 / (frame_ms * 1e-3),
     }
+
+
+def time_frames(step, frames: int, warmup: int) -> float:
+    """ms per frame of ``step()`` over ``frames`` frames after ``warmup``."""
+    import torch
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / frames * 1e3
+
+
+def other_workloads(frames: int) -> dict:
+    """Configurations 1, 3 and 4 of the benchmark set, timed the same way
+    (secondary lines; the headline is the cloth)."""
+    from xpbd_amd import Solver, Solver6DOF
+    from xpbd_amd.batch import BatchedWorld, draw
+    from xpbd_amd.scenes import hanging_chain, unified_scene
+
+    out = {}
+    c = hanging_chain(Solver, device="cuda:0")
+    ms = time_frames(c.step, frames, 10)
+    out["chain64_gs"] = {"ms_per_frame": round(ms, 4), "rows": c.num_constraints,
+                         "proj_per_s": c.num_constraints * 30 / (ms * 1e-3),
+                         "substeps_per_s": 15 / (ms * 1e-3)}
+    u = unified_scene(Solver6DOF, device="cuda:0")
+    ms = time_frames(u.step, max(frames // 4, 10), 5)
+    rows = u.num_joints + int(u.contact_counts().sum()) + 8 * int(
+        sum(1 for m in u._inv_mass if m > 0))
+    out["unified_1440_boxes"] = {"ms_per_frame": round(ms, 4), "bodies": u.num_bodies,
+                                 "joints": u.num_joints, "pairs": u.n_pairs,
+                                 "manifold_points": int(u.contact_counts().sum()),
+                                 "rows": rows, "proj_per_s": rows * 30 / (ms * 1e-3)}
+    spec = draw(1024, 0)
+    w = BatchedWorld(spec, device="cuda:0")
+    ms = time_frames(w.step, max(frames // 10, 5), 3)
+    rows = w.rows()
+    out["batched_1024_scenes"] = {"ms_per_frame": round(ms, 4), **spec.totals(), "rows": rows,
+                                  "proj_per_s": rows * 30 / (ms * 1e-3),
+                                  "substeps_per_s": 15 / (ms * 1e-3), "mix": spec.mix}
+    return out
 
 
 def load_traffic(kernel: str):
@@ -164,6 +210,8 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline-scene", action="store_true")
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the chain / unified / batched secondary lines")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run plain 512 MiB device copies (PMC calibration)")
     args = ap.parse_args()
@@ -198,19 +246,21 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     roof = None
-    if args.mode == "jacobi":
+    if args.mode == "jacobi" and s._tile_dev is not None:
         h = float(torch.tensor(s.dt / s.substeps, dtype=torch.float32))
-        launch = jacobi_launcher(N, s.X, s.XP, s.R, s.ANC, s.LAM, s.ACC, h, s.num_constraints,
-                                 s._tiling)
+        ts, hs, hr = s._tile_dev
+        launch = tile_launcher(N, s.X, s.X_B, s.XP, s.V, s.R, s.ANC, s.LAM, s.LAM_B, ts, hs, hr,
+                               h, s._tile, s.num_bodies)
+        x_keep, v_keep = s.positions(), s.velocities()
         kms = time_kernel(launch, args.kernel_reps)
-        s.ACC.zero_()
-        nbytes = proj_kernel_bytes(n_dist, n_floor, s.num_bodies)
+        s.set_state(x_keep, v_keep)
+        nbytes = iter_bytes(n_dist, n_floor, s.num_bodies)
         ach = nbytes / (kms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic("k_jacobi_project"),
-                "kernel": "k_jacobi_project", "kernel_ms": round(kms, 5),
-                "bytes_per_launch": nbytes, "tiling": list(s._tiling)}
-        print(f"[bench] jacobi kernel {kms * 1e3:.2f} us, {ach:.1f} GB/s", file=sys.stderr,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic("k_jacobi_tile"),
+                "kernel": "k_jacobi_tile", "kernel_ms": round(kms, 5),
+                "bytes_per_launch": nbytes, "tile": s._tile}
+        print(f"[bench] jacobi tile kernel {kms * 1e3:.2f} us, {ach:.1f} GB/s", file=sys.stderr,
               flush=True)
 
     if args.calib_copy:
@@ -227,6 +277,12 @@ def main() -> int:
         print(f"[bench] roofline scene: kernel {scene['kernel_ms']:.3f} ms "
               f"{scene['achieved']} GB/s, frame {scene['frame_ms']} ms", file=sys.stderr,
               flush=True)
+        torch.cuda.empty_cache()
+
+    others = None
+    if not args.no_others:
+        others = other_workloads(max(args.steps // 3, 30))
+        print(f"[bench] others: {json.dumps(others)}", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 
     cpu = None
@@ -248,6 +304,7 @@ def main() -> int:
         "roofline": roof,
         "roofline_scene": scene,
         "cpu_baseline": cpu,
+        "other_workloads": others,
     }
     print(json.dumps(line), flush=True)
     return 0

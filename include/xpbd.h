@@ -177,10 +177,11 @@ int xpbd_frame3_destroy(void* handle);
  * JB float4 (anchor_b, compliance), LAMJ float. */
 
 /* kernels_6dof.py:67-102 `integrate_bodies` (solver_6dof.py:370-373); also
- * zeroes LAMF and PTC[0] (the lambda resets of solver_6dof.py:374-375). */
+ * zeroes LAMF and PTC[0] (the lambda resets of solver_6dof.py:374-375).
+ * gscale (optional, float[n]) scales gravity per body (batched scenes). */
 int xpbd6_integrate(void* stream, float* X, float* Q, float* V, float* W, float* XP, float* QP,
                     const float* IB, float* LAMF, int* PTC, float gx, float gy, float gz,
-                    float h, int gyro, int n);
+                    float h, int gyro, const float* gscale, int n);
 /* kernels_6dof.py:233-382 `generate_box_manifold` (solver_6dof.py:378-382). */
 int xpbd6_box_manifold(void* stream, const float* X, const float* Q, const float* HE,
                        const int* PAIRS, const int* PC, float margin, int cap, int* MC, int* PTI,
@@ -240,6 +241,7 @@ typedef struct xpbd_rigid_desc {
   int substeps, iterations, gyroscopic;
   float h, gx, gy, gz, floor_y, friction, restitution, lin_damp, ang_damp, relax;
   float max_speed, max_omega, man_margin;
+  const float* gscale;                     /* float[n] gravity factor, or NULL */
 } xpbd_rigid_desc;
 
 /* Solver6DOF._record (solver_6dof.py:357-426) as a plan; run() replays one

@@ -52,6 +52,7 @@ class RigidDesc(ctypes.Structure):
         + [(k, c_float) for k in ("h", "gx", "gy", "gz", "floor_y", "friction", "restitution",
                                   "lin_damp", "ang_damp", "relax", "max_speed", "max_omega",
                                   "man_margin")]
+        + [("gscale", c_void_p)]
     )
 
 
@@ -86,7 +87,7 @@ _SIGS = {
     "xpbd_frame3_destroy": (c_int, [c_void_p]),
     "xpbd_color_constraints": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "xpbd_color_ranges": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
-    "xpbd6_integrate": (_I, [_P] * 10 + [_F, _F, _F, _F, _I, _I]),
+    "xpbd6_integrate": (_I, [_P] * 10 + [_F, _F, _F, _F, _I, _P, _I]),
     "xpbd6_box_manifold": (_I, [_P] * 6 + [_F, _I] + [_P] * 7),
     "xpbd6_solve_joints": (_I, [_P] * 8 + [_F, _I, _P, _P, _I]),
     "xpbd6_solve_floor": (_I, [_P] * 7 + [_F, _F, _P, _P, _P, _I]),

@@ -26,7 +26,7 @@ int issue(Frame6* f, hipStream_t s) {
   int rc;
   for (int sub = 0; sub < d.substeps; ++sub) {
     rc = xpbd6_integrate(s, d.x, d.q, d.v, d.w, d.x_prev, d.q_prev, d.inv_I, d.lam_floor,
-                         d.pt_count, d.gx, d.gy, d.gz, d.h, d.gyroscopic, n);
+                         d.pt_count, d.gx, d.gy, d.gz, d.h, d.gyroscopic, d.gscale, n);
     if (rc) return rc;
     rc = xpbd6_box_manifold(s, d.x, d.q, d.he, d.pairs, d.pair_count, d.man_margin, d.cap,
                             d.m_count, d.pt_idx, d.pt_n, d.pt_oi, d.pt_or, d.lam_pt, d.pt_count);

@@ -50,10 +50,12 @@ __device__ inline void accumulate_correction(float4* DX, float4* DR, int a, int 
 
 // kernels_6dof.py:67-102 (+ per-substep lambda_floor reset and contact
 // point counter reset, both exact rewrites of solver_6dof.py:374-375)
+// gscale (optional): per-body gravity factor (batched independent scenes)
 __global__ __launch_bounds__(kB) void k6_integrate(float4* X, float4* Q, float4* V, float4* W,
                                                    float4* XP, float4* QP, const float4* IB,
                                                    float* LAMF, int* PTC, float gx, float gy,
-                                                   float gz, float h, int gyro, int n) {
+                                                   float gz, float h, int gyro,
+                                                   const float* __restrict__ gscale, int n) {
   const int i = blockIdx.x * kB + threadIdx.x;
   if (i == 0 && PTC) *PTC = 0;
   if (i >= n) return;
@@ -65,7 +67,9 @@ __global__ __launch_bounds__(kB) void k6_integrate(float4* X, float4* Q, float4*
   lf[0] = make_float4(0.f, 0.f, 0.f, 0.f);
   lf[1] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (x4.w == 0.0f) return;
-  f3 v = xyz(V[i]) + mk3(gx, gy, gz) * h;
+  f3 g = mk3(gx, gy, gz);
+  if (gscale) g = g * gscale[i];
+  f3 v = xyz(V[i]) + g * h;
   f3 x = xyz(x4) + v * h;
   V[i] = make_float4(v.x, v.y, v.z, 0.0f);
   f3 w = xyz(W[i]);
@@ -555,10 +559,11 @@ extern "C" {
 
 int xpbd6_integrate(void* stream, float* X, float* Q, float* V, float* W, float* XP, float* QP,
                     const float* IB, float* LAMF, int* PTC, float gx, float gy, float gz,
-                    float h, int gyro, int n) {
+                    float h, int gyro, const float* gscale, int n) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(k6_integrate, dim3(nblk(n)), dim3(kB), 0, (hipStream_t)stream, F4(X), F4(Q),
-                     F4(V), F4(W), F4(XP), F4(QP), CF4(IB), LAMF, PTC, gx, gy, gz, h, gyro, n);
+                     F4(V), F4(W), F4(XP), F4(QP), CF4(IB), LAMF, PTC, gx, gy, gz, h, gyro,
+                     gscale, n);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

@@ -121,19 +121,33 @@ def cloth_bench(solver_cls, res: int, mode: str, substeps: int = 15, iterations:
     return s
 
 
+def _part1by1(v):
+    """Spread the low 16 bits of an int64 tensor to even bit positions."""
+    v = v & 0xFFFF
+    v = (v | (v << 8)) & 0x00FF00FF
+    v = (v | (v << 4)) & 0x0F0F0F0F
+    v = (v | (v << 2)) & 0x33333333
+    v = (v | (v << 1)) & 0x55555555
+    return v
+
+
 def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing: float = 0.01,
                         mass: float = 1e-3, height: float = 0.5, floor_mu: float = 0.4,
-                        compliance_range=(1e-8, 1e-3)):
+                        compliance_range=(1e-8, 1e-3), tile: int = 2048):
     """The config-2 cloth generator scaled to ``n_sheets`` sheets of res^2,
-    built directly on the device in the packed layout of include/xpbd.h
-    (rows already in smallest-endpoint order).  Used for the HBM-roofline
-    scene (8 x 1024^2 = 58.6M rows), where host-side scene lists would cost
-    minutes.  Per particle k the rows are, when valid: (k,k+1), (k,k+res),
-    (k,k+res+1), (k,k+res-1), (k,k+2), (k,k+2res), FLOOR(k) — the same edge
-    set as ``cloth_edges``.
+    built directly on the device in the packed layout of include/xpbd.h, for
+    the HBM-roofline scene (8 x 1024^2 = 58.6M rows), where host-side scene
+    lists would cost minutes.  The edge set per particle k (grid (ix, iy)) is
+    that of ``cloth_edges``: (k,+x), (k,+y), (k,+x+y), (k+x,+y-x) as
+    (k,k+res-1) from its right neighbour, (k,+2x), (k,+2y), plus FLOOR(k).
 
-    Returns dict of torch tensors X, V, XP, ACC (float4 per particle), R
-    (int4 rows), ANC, LAM, and counts n_particles, n_rows, n_distance."""
+    Particles are numbered in 2-D Z-order within each sheet (the locality
+    layout the solver's Jacobi path uses), rows are sorted by smallest
+    endpoint, and the fused-kernel tile tables for ``tile`` particles per tile
+    are built on the device.
+
+    Returns dict of torch tensors X, X_B, V, XP, ACC (float4 per particle), R
+    (int4 rows), ANC, LAM, LAM_B, TRS, HS, HR (tile tables) and counts."""
     import torch
 
     g = torch.Generator(device=device)
@@ -145,33 +159,44 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
     ix = kl % res
     iy = kl // res
     sheet = k // n1
-    X = torch.empty((n, 4), device=device, dtype=torch.float32)
-    X[:, 0] = (ix.float() - (res - 1) / 2) * spacing + sheet.float() * (res + 4) * spacing
-    X[:, 1] = height
-    X[:, 2] = (iy.float() - (res - 1) / 2) * spacing
-    X[:, :3] += torch.randn((n, 3), device=device, generator=g) * 1e-4
+    # Z-order numbering: new index of row-major particle k
+    code = sheet * (1 << 40) + (_part1by1(iy) << 1 | _part1by1(ix))
+    order = torch.argsort(code, stable=True)          # order[new] = old
+    inv = torch.empty_like(order)
+    inv[order] = k
+    Xo = torch.empty((n, 4), device=device, dtype=torch.float32)
+    Xo[:, 0] = (ix.float() - (res - 1) / 2) * spacing + sheet.float() * (res + 4) * spacing
+    Xo[:, 1] = height
+    Xo[:, 2] = (iy.float() - (res - 1) / 2) * spacing
+    Xo[:, :3] += torch.randn((n, 3), device=device, generator=g) * 1e-4
     pinned = (iy == 0) & ((ix == 0) | (ix == res - 1))
-    X[:, 3] = torch.where(pinned, torch.zeros_like(X[:, 3]),
-                          torch.full_like(X[:, 3], float(torch.tensor(1.0 / mass,
-                                                                       dtype=torch.float32))))
+    w = float(torch.tensor(1.0 / mass, dtype=torch.float32))
+    Xo[:, 3] = torch.where(pinned, torch.zeros_like(Xo[:, 3]), torch.full_like(Xo[:, 3], w))
     nb = torch.stack([k + 1, k + res, k + res + 1, k + res - 1, k + 2, k + 2 * res], dim=1)
     valid = torch.stack([ix + 1 < res, iy + 1 < res, (ix + 1 < res) & (iy + 1 < res),
                          (ix >= 1) & (iy + 1 < res), ix + 2 < res, iy + 2 < res], dim=1)
     kinds = torch.cat([valid, torch.ones((n, 1), dtype=torch.bool, device=device)], dim=1)
     bcol = torch.cat([nb, torch.full((n, 1), -1, device=device, dtype=torch.int64)], dim=1)
-    acol = k[:, None].expand(n, 7)
+    a = k[:, None].expand(n, 7)[kinds]
+    b = bcol[kinds]
     types = torch.zeros((n, 7), dtype=torch.int64, device=device)
     types[:, 6] = 2
-    a = acol[kinds]
-    b = bcol[kinds]
     t = types[kinds]
+    del nb, valid, kinds, bcol, types
     m = a.shape[0]
     is_d = t == 0
     bb = torch.where(is_d, b, a)
-    rest = torch.linalg.vector_norm(X[a, :3] - X[bb, :3], dim=1)
-    lo, hi = torch.log(torch.tensor(compliance_range, dtype=torch.float64))
+    rest = torch.linalg.vector_norm(Xo[a, :3] - Xo[bb, :3], dim=1)
+    lo_c, hi_c = torch.log(torch.tensor(compliance_range, dtype=torch.float64))
     comp = torch.exp(torch.rand(m, device=device, generator=g, dtype=torch.float64)
-                     * (hi - lo) + lo).float()
+                     * (hi_c - lo_c) + lo_c).float()
+    # renumber endpoints, sort rows by smallest endpoint
+    a = inv[a]
+    b = torch.where(is_d, inv[torch.clamp(b, min=0)], b)
+    lo = torch.where(is_d, torch.minimum(a, b), a)
+    hi = torch.where(is_d, torch.maximum(a, b), a)
+    ro = torch.argsort(lo, stable=True)
+    a, b, t, lo, hi, rest, comp, is_d = (v[ro] for v in (a, b, t, lo, hi, rest, comp, is_d))
     p0 = torch.where(is_d, rest, torch.zeros_like(rest))
     p1 = torch.where(is_d, comp, torch.full_like(comp, floor_mu))
     R = torch.empty((m, 4), device=device, dtype=torch.int32)
@@ -179,13 +204,27 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
     R[:, 1] = b.to(torch.int32)
     R[:, 2] = p0.view(torch.int32)
     R[:, 3] = p1.view(torch.int32)
-    del a, b, t, bb, rest, comp, p0, p1, nb, valid, kinds, bcol, acol, types
+    n_tiles = (n + tile - 1) // tile
+    bounds = torch.arange(n_tiles + 1, device=device, dtype=torch.int64) * tile
+    TRS = torch.searchsorted(lo, bounds).to(torch.int32)
+    TRS[-1] = m
+    cross = torch.nonzero((hi // tile) != (lo // tile)).flatten()
+    ht = hi[cross] // tile
+    so = torch.argsort(ht, stable=True)
+    HR = cross[so].to(torch.int32)
+    HS = torch.searchsorted(ht[so].contiguous(), torch.arange(n_tiles + 1, device=device)
+                            ).to(torch.int32)
+    n_dist = int(is_d.sum())
+    X = Xo[order].contiguous()
+    del a, b, t, lo, hi, rest, comp, p0, p1, is_d, Xo, ro, cross, ht, so
     return dict(
-        X=X, V=torch.zeros((n, 4), device=device), XP=torch.zeros((n, 4), device=device),
-        ACC=torch.zeros((n, 4), device=device), R=R,
+        X=X, X_B=X.clone(), V=torch.zeros((n, 4), device=device),
+        XP=torch.zeros((n, 4), device=device), ACC=torch.zeros((n, 4), device=device), R=R,
         # no ATTACH rows: the anchor array is never read, one slot suffices
         ANC=torch.zeros((1, 4), device=device), LAM=torch.zeros(m, device=device),
-        n_particles=n, n_rows=m, n_distance=int(is_d.sum()))
+        LAM_B=torch.zeros(m, device=device), TRS=TRS, HS=HS,
+        HR=HR if HR.numel() else torch.zeros(1, dtype=torch.int32, device=device),
+        n_particles=n, n_rows=m, n_distance=n_dist, tile=tile, n_halo=int(HR.numel()))
 
 
 def _random_quats(rng, n):

@@ -244,6 +244,10 @@ class Solver:
         self.jacobi_kernel = "tile"
         self.tile_size = 0            # 0 = chosen from the scene size
         self.reorder = True           # Z-order particle layout for Jacobi
+        # device-side CONTACT generation (Jacobi tile mode): an extension
+        # that replaces the per-frame host broad phase; see enable_device_contacts
+        self._device_contacts = False
+        self._gscale = None           # per-particle gravity factor (API order)
 
         # host scene, one column per field (float64 as the reference's
         # Python floats; rounded to float32 once, at flush)
@@ -407,6 +411,24 @@ class Solver:
             self._friction.set(fr)
         if enabled and self._contact_pool_start is None:
             self._contact_pool_start = len(self._rt)
+
+    def set_gravity_scale(self, scale) -> None:
+        """Per-particle gravity factor g_i = g * scale[i] (an extension used
+        for batches of independent scenes; Jacobi tile mode only)."""
+        self._gscale = np.asarray(scale, np.float32).reshape(-1).copy()
+        self._dirty = True
+
+    def enable_device_contacts(self, enabled=True, margin=0.0) -> None:
+        """Sphere-sphere CONTACT rows generated on the GPU every frame (an
+        extension; the reference rebuilds them on the host, solver.py:275-342).
+        Same margin, cell and filter as the host path; pairs already joined
+        by a constraint are not excluded, so use it for scenes whose
+        colliding particles carry no two-body rows.  Jacobi tile mode only."""
+        if enabled and (self.solve_mode != "jacobi" or self.jacobi_kernel != "tile"):
+            raise ValueError("device contacts need solve_mode='jacobi' with the tile kernel")
+        self._device_contacts = bool(enabled)
+        self._contact_margin = float(margin)
+        self._dirty = True
 
     # ---- runtime perturbations (solver.py:232-271); in place on device ----
     def _dev(self, i: int) -> int:
@@ -603,9 +625,67 @@ class Solver:
             self.X_B = self.LAM_B = None
             self._tile_dev = None
         self._perm = perm
+        self.GS = None
+        if self._gscale is not None:
+            if not tile_mode:
+                raise ValueError("a gravity scale needs solve_mode='jacobi' with the tile kernel")
+            if len(self._gscale) != n_b:
+                raise ValueError("gravity scale length != number of particles")
+            g = self._gscale if pperm is None else self._gscale[pperm]
+            self.GS = torch.from_numpy(np.ascontiguousarray(g)).to(dev)
+        self.CROWS = None
+        if self._device_contacts:
+            rad = self._radius.array().astype(np.float32)
+            col = self._collide.array().astype(bool)
+            rad = np.where(col, rad, 0.0).astype(np.float32)
+            fr = self._friction.array().astype(np.float32)
+            if pperm is not None:
+                rad, fr = rad[pperm], fr[pperm]
+            self._crmax = float(rad.max()) if n_b else 0.0
+            self.CRAD = torch.from_numpy(np.ascontiguousarray(rad)).to(dev)
+            self.CMU = torch.from_numpy(np.ascontiguousarray(fr)).to(dev)
+            self.CWS = torch.empty(int(N.lib().xpbd_contacts_workspace_bytes(n_b)),
+                                   dtype=torch.uint8, device=dev)
+            self.CCOUNT = torch.zeros(1, dtype=torch.int32, device=dev)
+            self._alloc_contacts(max(1024, 4 * int(col.sum())))
+            if self.ACC is None or self.ACC.shape[0] < n_b:
+                self.ACC = torch.zeros((max(n_b, 1), 4), dtype=torch.float32, device=dev)
+            if self._tile_dev is None:      # no static rows: tables for an empty row set
+                self._tile = self.tile_size or default_tile_size(n_b)
+                self._tile_tabs = tile_tables(np.zeros((0, 4), np.int32), n_b, self._tile)
+                ts, hs, hr = self._tile_tabs
+                self._tile_dev = (torch.from_numpy(ts).to(dev), torch.from_numpy(hs).to(dev),
+                                  torch.zeros(1, dtype=torch.int32, device=dev))
+                self.X_B = self.X.clone()
+                self.LAM_B = torch.zeros_like(self.LAM)
+        self.n_contacts = 0
         self._gen += 1
         self._drop_plan()
         self._dirty = False
+
+    def _alloc_contacts(self, cap: int) -> None:
+        import torch
+
+        self._ccap = int(cap)
+        self.CROWS = torch.zeros((self._ccap, 4), dtype=torch.int32, device=self.device)
+        self.CLAM = torch.zeros(self._ccap, dtype=torch.float32, device=self.device)
+
+    def _device_contact_pass(self) -> None:
+        """Generate this frame's CONTACT rows on the device; grow the buffer
+        (and drop the captured graph) if the count exceeds the capacity."""
+        lib = N.lib()
+        for _ in range(2):
+            N.check(lib.xpbd_sphere_contacts(
+                N.current_stream(), N.ptr(self.CWS), N.ptr(self.X), N.ptr(self.V),
+                N.ptr(self.CRAD), N.ptr(self.CMU), self._contact_margin, self.dt, self._crmax,
+                int(self.X.shape[0]), self._ccap, N.ptr(self.CROWS), N.ptr(self.CCOUNT)),
+                "xpbd_sphere_contacts")
+            cnt = int(self.CCOUNT.item())
+            if cnt <= self._ccap:
+                break
+            self._alloc_contacts(int(cnt * 1.5))
+            self._drop_plan()
+        self.n_contacts = cnt
 
     def _repack_rows_inplace(self) -> None:
         h = self.host_arrays()
@@ -623,7 +703,7 @@ class Solver:
     def _signature(self):
         return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
                 self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling,
-                self._tile)
+                self._tile, getattr(self, "_ccap", 0))
 
     def _drop_plan(self):
         if self._plan is not None:
@@ -652,6 +732,11 @@ class Solver:
             d.x_alt, d.lam_alt = N.ptr(self.X_B), N.ptr(self.LAM_B)
             d.tile_row_start, d.halo_start, d.halo_rows = (N.ptr(t) for t in self._tile_dev)
             d.tile_size = self._tile
+            if self.GS is not None:
+                d.gscale = N.ptr(self.GS)
+            if self.CROWS is not None:
+                d.dyn_rows, d.dyn_lam = N.ptr(self.CROWS), N.ptr(self.CLAM)
+                d.dyn_count, d.dyn_cap = N.ptr(self.CCOUNT), self._ccap
         d.substeps, d.iterations = self.substeps, self.iterations
         d.h = float(np.float32(self.dt / float(self.substeps)))
         d.gx, d.gy, d.gz = self.gravity
@@ -671,7 +756,8 @@ class Solver:
         out as replays of one captured graph.  With self-collision the CONTACT
         rows are rebuilt on the host before every frame (solver.py:413-416),
         so each frame is issued eagerly."""
-        for _ in range(int(frames) if self._self_collision else 1):
+        per_frame = self._self_collision or self._device_contacts
+        for _ in range(int(frames) if per_frame else 1):
             if self._self_collision:
                 self._flush()
                 self._sanitize_positions()
@@ -679,8 +765,10 @@ class Solver:
             self._flush()
             if len(self._x) == 0:
                 return
+            if self._device_contacts:
+                self._device_contact_pass()
             plan = self._get_plan()
-            n_frames = 1 if self._self_collision else int(frames)
+            n_frames = 1 if per_frame else int(frames)
             graph = int(self.use_graph and not self._self_collision)
             N.check(N.lib().xpbd_frame3_run(plan, N.current_stream(), n_frames, graph),
                     "xpbd_frame3_run")

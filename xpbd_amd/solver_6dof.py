@@ -107,6 +107,7 @@ class Solver6DOF:
         self._plan = None
         self._plan_sig = None
         self._graph = None
+        self._gscale = None
         self.n_pairs = 0
         self.n_joints = 0
         self.cap = 0
@@ -175,6 +176,12 @@ class Solver6DOF:
         self._dirty = True
         return len(self._j_a) - 1
 
+    def set_gravity_scale(self, scale) -> None:
+        """Per-body gravity factor g_i = g * scale[i] (an extension used for
+        batches of independent scenes)."""
+        self._gscale = np.asarray(scale, np.float32).reshape(-1).copy()
+        self._dirty = True
+
     # ---- host arrays in the reference layout --------------------------------
     def host_arrays(self) -> dict:
         n = len(self._x)
@@ -239,6 +246,10 @@ class Solver6DOF:
         b.update(JI=torch.from_numpy(ji).to(dev), JA=torch.from_numpy(ja).to(dev),
                  JB=torch.from_numpy(jb).to(dev),
                  LAMJ=torch.zeros(max(nj, 1), dtype=torch.float32, device=dev))
+        if self._gscale is not None:
+            if len(self._gscale) != n:
+                raise ValueError("gravity scale length != number of bodies")
+            b["GS"] = torch.from_numpy(self._gscale).to(dev)
         self._buf = b
         self.cap = max(64, n * 4)
         self._alloc_pair_buffers(self.cap)
@@ -302,6 +313,8 @@ class Solver6DOF:
         d.floor_y, d.friction, d.restitution = self.floor_y, self.friction, self.restitution
         d.lin_damp, d.ang_damp, d.relax = self.lin_damp, self.ang_damp, self.jacobi_relax
         d.max_speed, d.max_omega, d.man_margin = self.max_speed, self.max_omega, self._man_margin
+        if "GS" in b:
+            d.gscale = P(b["GS"])
         handle = N.c_void_p()
         N.check(N.lib().xpbd_frame6_create(N.ctypes.byref(d), N.ctypes.byref(handle)),
                 "xpbd_frame6_create")

@@ -46,7 +46,8 @@ extern "C" {
 /* ---- 3-DOF particle kernels ------------------------------------------- */
 
 /* Fused per-particle pass; flags: 1 apply_jacobi, 2 finalize_velocity,
- * 4 cap_velocity, 8 integrate (applied in that order).
+ * 4 cap_velocity, 8 integrate (applied in that order); 16 instead of 1 adds
+ * the summed corrections without averaging (the warm-start position update).
  * Replaces wp.launch(apply_jacobi / finalize_velocity / cap_velocity /
  * integrate) at src/xpbd3d/solver.py:465, 437, 441, 427. */
 int xpbd_particles_pass(void* stream, float* X, float* V, float* XP, float* ACC, int flags,
@@ -157,6 +158,11 @@ typedef struct xpbd_particles_desc {
   const int* dyn_count;  /* int[1], device */
   int dyn_cap;
   const float* gscale;   /* float[n_particles] gravity factor, or NULL (tile mode) */
+  /* learned warm start (XPBD_MODE_JACOBI): MLP parameters, or NULL for the
+   * reference's zero start; feature and output scratch */
+  const float* ws_theta; /* float[5057] */
+  float* ws_feat;        /* float[n_rows * 12] */
+  float* ws_y;           /* float[n_rows] */
 } xpbd_particles_desc;
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);
@@ -275,6 +281,33 @@ int xpbd6_broadphase_hashgrid(void* stream, void* ws, const float* X, const floa
 /* 30-bit Morton code of a point in [0,1)^3 (10 bits per axis, x highest);
  * no reference counterpart (Warp builds its LBVH internally). */
 unsigned xpbd_morton3(float x, float y, float z);
+
+/* ---- learned lambda warm start (no reference counterpart) -----------------
+ * MLP 12 -> 64 -> 64 -> 1 (exact-erf GELU), 5057 float32 parameters
+ * W1[12][64] b1[64] W2[64][64] b2[64] W3[64] b3; BF16 MFMA products with
+ * FP32 accumulation.  It predicts the multipliers that replace the zero reset
+ * of solver.py:431.  F is float[m][12] (features), Y/T float[m]. */
+int xpbd_mlp_param_count(void);
+int xpbd_mlp_forward(void* stream, const float* theta, const float* F, float* Y, int m);
+long long xpbd_mlp_train_workspace_bytes(int nblocks);
+/* Gradient of the mean squared error over m rows (grad float[5057]) and the
+ * loss (loss float[1]); nblocks partial sums reduced in block order. */
+int xpbd_mlp_grad(void* stream, const float* theta, const float* F, const float* T, int m,
+                  int nblocks, void* ws, float* grad, float* loss);
+/* One Adam step (step >= 1) on theta with moment buffers mom/vel. */
+int xpbd_mlp_adam(void* stream, float* theta, float* mom, float* vel, const float* grad,
+                  float lr, float beta1, float beta2, float eps, int step);
+
+/* Warm-start rows at the start of a substep (features of the MLP, float[m][12]),
+ * lambda0 from the MLP output Y written to LAM with M^-1 grad C^T lambda0
+ * summed into ACC, and the squared-residual sum (r = C + alpha~ lambda for
+ * two-sided rows, max(0, -C) for one-sided rows) added to *out_sum. */
+int xpbd_ws_features(void* stream, const float* X, const float* V, const int* R, const float* ANC,
+                     const float* LAM, float h, float* F, int n_rows);
+int xpbd_ws_apply(void* stream, const float* X, const float* V, const int* R, const float* ANC,
+                  const float* Y, float h, float* LAM, float* ACC, int n_rows);
+int xpbd_ws_residual(void* stream, const float* X, const int* R, const float* ANC,
+                     const float* LAM, float h, double* out_sum, int n_rows);
 
 /* ---- host coloring (src/xpbd3d/coloring.py) ----------------------------- */
 

@@ -21,7 +21,21 @@
 
 namespace {
 
-enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8 };
+enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8, F_APPLY_SUM = 16 };
+
+// Learned warm start of one substep (window-Jacobi mode): features ->
+// MLP -> lambda0 into LAM and its position correction summed into ACC ->
+// applied; the iterations then continue from lambda0 (first = 0).
+int warm_start(const xpbd_particles_desc& d, hipStream_t s) {
+  int rc = xpbd_ws_features(s, d.x, d.v, d.rows, d.anchors, d.lam, d.h, d.ws_feat, d.n_rows);
+  if (rc) return rc;
+  rc = xpbd_mlp_forward(s, d.ws_theta, d.ws_feat, d.ws_y, d.n_rows);
+  if (rc) return rc;
+  rc = xpbd_ws_apply(s, d.x, d.v, d.rows, d.anchors, d.ws_y, d.h, d.lam, d.acc, d.n_rows);
+  if (rc) return rc;
+  return xpbd_particles_pass(s, d.x, d.v, d.x_prev, d.acc, F_APPLY_SUM, d.relax, d.h, d.damping,
+                             d.max_speed, d.gx, d.gy, d.gz, d.n_particles);
+}
 
 struct Frame3 {
   xpbd_particles_desc d;
@@ -85,10 +99,15 @@ int issue_frame(Frame3* f, hipStream_t s) {
   int rc = xpbd_particles_pass(s, d.x, d.v, d.x_prev, d.acc, F_INTEGRATE, d.relax, d.h, d.damping,
                                d.max_speed, d.gx, d.gy, d.gz, n);
   if (rc) return rc;
+  const bool warm = jacobi && have_rows && d.ws_theta != nullptr;
   for (int sub = 0; sub < d.substeps; ++sub) {
+    if (warm) {
+      rc = warm_start(d, s);
+      if (rc) return rc;
+    }
     if (have_rows) {
       for (int it = 0; it < d.iterations; ++it) {
-        const int first = it == 0;
+        const int first = (it == 0) && !warm;
         if (jacobi) {
           rc = xpbd_solve_jacobi(s, d.x, d.x_prev, d.rows, d.anchors, d.lam, d.acc, d.h, first,
                                  d.n_rows, d.rows_per_thread, d.window);
@@ -138,6 +157,8 @@ int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
       (!desc->x_alt || !desc->lam_alt || !desc->tile_row_start || !desc->halo_start ||
        desc->tile_size < 1 || desc->tile_size > XPBD_MAX_TILE))
     return (int)hipErrorInvalidValue;
+  if (desc->ws_theta && (desc->mode != XPBD_MODE_JACOBI || !desc->ws_feat || !desc->ws_y))
+    return (int)hipErrorInvalidValue;  // learned warm start: window-Jacobi mode
   if (desc->mode != XPBD_MODE_JACOBI_TILE && (desc->dyn_cap > 0 || desc->gscale))
     return (int)hipErrorInvalidValue;  // device-counted rows / gravity scale: tile mode only
   if (desc->mode == XPBD_MODE_JACOBI_TILE && desc->dyn_cap > 0 &&

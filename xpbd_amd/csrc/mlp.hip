@@ -1,0 +1,344 @@
+// Warm-start MLP (12 -> 64 -> 64 -> 1, exact-erf GELU) on gfx950 MFMA.
+//
+// No reference counterpart: the learned lambda warm start is new; it
+// replaces the zero reset of the multipliers at the start of a substep
+// (src/xpbd3d/solver.py:431, solver_6dof.py:374-377).
+//
+// Parameters: one float32 vector of 5057 values, W1[12][64] b1[64] W2[64][64]
+// b2[64] W3[64] b3[1] (row-major, input x output).  The two 64-wide products
+// run on v_mfma_f32_16x16x16_bf16: inputs, hidden activations and W1/W2 are
+// rounded to bfloat16 (nearest even), products accumulate in float32; the
+// output layer and every elementwise op stay in float32.
+//
+// A wavefront owns 16 rows at a time.  MFMA operand layout (16x16x16):
+//   A (M x K):  lane l holds A[l % 16][4 (l / 16) + j], j = 0..3
+//   B (K x N):  lane l holds B[4 (l / 16) + j][l % 16]
+//   D (M x N):  lane l holds D[4 (l / 16) + j][l % 16]
+// so a D tile of one layer is already the A^T / B operand of the weight
+// gradient, and only the forward H1 and backward dZ2 tiles take a trip
+// through LDS to become A operands.
+//
+// Training: each block accumulates its gradient in LDS and writes one
+// partial vector; a second kernel sums the partials in block order
+// (deterministic), and an Adam kernel updates the parameters.
+#include <cmath>
+
+#include "common.h"
+#include "xpbd.h"
+
+namespace {
+
+constexpr int NIN = 12, NH = 64;
+constexpr int P_W1 = 0, P_B1 = P_W1 + NIN * NH, P_W2 = P_B1 + NH, P_B2 = P_W2 + NH * NH,
+              P_W3 = P_B2 + NH, P_B3 = P_W3 + NH, NPARAM = P_B3 + 1;
+static_assert(NPARAM == 5057, "parameter count");
+constexpr int kWaves = 4, kBlk = 64 * kWaves;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float from_bf16(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+         z * expf(-0.5f * z * z) * 0.39894228040143268f;
+}
+
+__device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+struct Lds {
+  unsigned short w1[16][NH];   // W1 padded to 16 input rows (12..15 zero)
+  unsigned short w2[NH][NH];
+  float b1[NH], b2[NH], w3[NH];
+  unsigned short tile[kWaves][16][NH];  // per-wave transposition tile
+  float grad[NPARAM];                    // block gradient (training only)
+};
+
+__device__ void load_params(Lds& s, const float* __restrict__ theta, bool zero_grad) {
+  for (int i = threadIdx.x; i < 16 * NH; i += kBlk) {
+    const int k = i / NH, n = i % NH;
+    s.w1[k][n] = k < NIN ? to_bf16(theta[P_W1 + k * NH + n]) : (unsigned short)0;
+  }
+  for (int i = threadIdx.x; i < NH * NH; i += kBlk) s.w2[i / NH][i % NH] = to_bf16(theta[P_W2 + i]);
+  for (int i = threadIdx.x; i < NH; i += kBlk) {
+    s.b1[i] = theta[P_B1 + i];
+    s.b2[i] = theta[P_B2 + i];
+    s.w3[i] = theta[P_W3 + i];
+  }
+  if (zero_grad)
+    for (int i = threadIdx.x; i < NPARAM; i += kBlk) s.grad[i] = 0.0f;
+  __syncthreads();
+}
+
+// Forward of one 16-row tile held by a wave.  On return z1/z2 hold the
+// pre-activations (D layout, 4 column tiles x 4 rows per lane) and y the
+// outputs of rows 4g+j (identical across the 16 lanes of group g).
+__device__ inline void forward_tile(Lds& s, const float* __restrict__ F, int row0, int m,
+                                    float b3, f32x4 z1[4], f32x4 z2[4], float y[4],
+                                    bf16x4& xa) {
+  const int l = threadIdx.x & 63, r16 = l & 15, g = l >> 4, wv = threadIdx.x >> 6;
+  // layer 1: A = X tile (16 rows x 16 padded features)
+  {
+    const int row = row0 + r16;
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * g + j;
+      const float v = (row < m && k < NIN) ? F[(size_t)row * NIN + k] : 0.0f;
+      xa[j] = (short)to_bf16(v);
+    }
+  }
+  for (int n = 0; n < 4; ++n) {
+    bf16x4 b;
+    for (int j = 0; j < 4; ++j) b[j] = (short)s.w1[4 * g + j][16 * n + r16];
+    z1[n] = mfma(xa, b, f32x4{0.f, 0.f, 0.f, 0.f});
+    for (int j = 0; j < 4; ++j) {
+      z1[n][j] += s.b1[16 * n + r16];
+      s.tile[wv][4 * g + j][16 * n + r16] = to_bf16(gelu(z1[n][j]));
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  // layer 2: A = H1 tile read back in A layout
+  bf16x4 a[4];
+  for (int sK = 0; sK < 4; ++sK)
+    for (int j = 0; j < 4; ++j) a[sK][j] = (short)s.tile[wv][r16][16 * sK + 4 * g + j];
+  for (int n = 0; n < 4; ++n) {
+    f32x4 acc{0.f, 0.f, 0.f, 0.f};
+    for (int sK = 0; sK < 4; ++sK) {
+      bf16x4 b;
+      for (int j = 0; j < 4; ++j) b[j] = (short)s.w2[16 * sK + 4 * g + j][16 * n + r16];
+      acc = mfma(a[sK], b, acc);
+    }
+    for (int j = 0; j < 4; ++j) acc[j] += s.b2[16 * n + r16];
+    z2[n] = acc;
+  }
+  // layer 3 (float32): y[row] = sum_c gelu(z2[row][c]) w3[c] + b3
+  for (int j = 0; j < 4; ++j) {
+    float p = 0.0f;
+    for (int n = 0; n < 4; ++n) p += gelu(z2[n][j]) * s.w3[16 * n + r16];
+    for (int o = 1; o < 16; o <<= 1) p += __shfl_xor(p, o, 64);
+    y[j] = p + b3;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(kBlk) void k_mlp_forward(const float* __restrict__ theta,
+                                                      const float* __restrict__ F,
+                                                      float* __restrict__ Y, int m) {
+  __shared__ Lds s;
+  load_params(s, theta, false);
+  const int l = threadIdx.x & 63, r16 = l & 15, g = l >> 4, wv = threadIdx.x >> 6;
+  const float b3 = theta[P_B3];
+  const int ntiles = (m + 15) / 16;
+  for (int t = blockIdx.x * kWaves + wv; t < ntiles; t += gridDim.x * kWaves) {
+    f32x4 z1[4], z2[4];
+    float y[4];
+    bf16x4 xa;
+    forward_tile(s, F, t * 16, m, b3, z1, z2, y, xa);
+    if (r16 == 0)
+      for (int j = 0; j < 4; ++j) {
+        const int row = t * 16 + 4 * g + j;
+        if (row < m) Y[row] = y[j];
+      }
+  }
+}
+
+// Forward + backward of the mean-squared error over rows [0, m); writes the
+// block's gradient (not yet divided by anything but m) and squared-error sum.
+__global__ __launch_bounds__(kBlk) void k_mlp_train_partial(const float* __restrict__ theta,
+                                                            const float* __restrict__ F,
+                                                            const float* __restrict__ T, int m,
+                                                            float* __restrict__ partial,
+                                                            float* __restrict__ loss_partial) {
+  __shared__ Lds s;
+  load_params(s, theta, true);
+  const int l = threadIdx.x & 63, r16 = l & 15, g = l >> 4, wv = threadIdx.x >> 6;
+  const float b3 = theta[P_B3];
+  const float inv_m2 = 2.0f / (float)m;
+  f32x4 gW2[4][4], gW1[4];
+  for (int a = 0; a < 4; ++a) {
+    gW1[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < 4; ++b) gW2[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gw3[4] = {0, 0, 0, 0}, gb3 = 0.0f;
+  float sq = 0.0f;
+  const int ntiles = (m + 15) / 16;
+  for (int t = blockIdx.x * kWaves + wv; t < ntiles; t += gridDim.x * kWaves) {
+    f32x4 z1[4], z2[4];
+    float y[4];
+    bf16x4 xa;
+    forward_tile(s, F, t * 16, m, b3, z1, z2, y, xa);
+    float dy[4];
+    for (int j = 0; j < 4; ++j) {
+      const int row = t * 16 + 4 * g + j;
+      const float r = row < m ? y[j] - T[row] : 0.0f;
+      dy[j] = r * inv_m2;
+      if (r16 == 0) {
+        sq += r * r;
+        gb3 += dy[j];
+      }
+    }
+    // output layer and dZ2
+    f32x4 dz2[4];
+    for (int n = 0; n < 4; ++n) {
+      const float w3 = s.w3[16 * n + r16];
+      for (int j = 0; j < 4; ++j) {
+        gw3[n] += gelu(z2[n][j]) * dy[j];
+        dz2[n][j] = dy[j] * w3 * gelu_grad(z2[n][j]);
+        gb2[n] += dz2[n][j];
+      }
+    }
+    // dW2 = H1^T dZ2: the H1 and dZ2 D tiles are the A and B operands
+    bf16x4 h1a[4], dzb[4];
+    for (int n = 0; n < 4; ++n)
+      for (int j = 0; j < 4; ++j) {
+        h1a[n][j] = (short)to_bf16(gelu(z1[n][j]));
+        dzb[n][j] = (short)to_bf16(dz2[n][j]);
+      }
+    for (int mt = 0; mt < 4; ++mt)
+      for (int nt = 0; nt < 4; ++nt) gW2[mt][nt] = mfma(h1a[mt], dzb[nt], gW2[mt][nt]);
+    // dH1 = dZ2 W2^T: dZ2 goes through LDS into the A layout
+    for (int n = 0; n < 4; ++n)
+      for (int j = 0; j < 4; ++j) s.tile[wv][4 * g + j][16 * n + r16] = (unsigned short)dzb[n][j];
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    bf16x4 a[4];
+    for (int sK = 0; sK < 4; ++sK)
+      for (int j = 0; j < 4; ++j) a[sK][j] = (short)s.tile[wv][r16][16 * sK + 4 * g + j];
+    __builtin_amdgcn_wave_barrier();
+    bf16x4 dz1b[4];
+    for (int nt = 0; nt < 4; ++nt) {
+      f32x4 acc{0.f, 0.f, 0.f, 0.f};
+      for (int sK = 0; sK < 4; ++sK) {
+        bf16x4 b;  // B[k][n] = W2[n][k]
+        for (int j = 0; j < 4; ++j) b[j] = (short)s.w2[16 * nt + r16][16 * sK + 4 * g + j];
+        acc = mfma(a[sK], b, acc);
+      }
+      for (int j = 0; j < 4; ++j) {
+        const float dz = acc[j] * gelu_grad(z1[nt][j]);
+        gb1[nt] += dz;
+        dz1b[nt][j] = (short)to_bf16(dz);
+      }
+    }
+    // dW1 = X^T dZ1: A[m][k] = X[row k][feature m]
+    bf16x4 xt;
+    {
+      const int row0 = t * 16;
+      for (int j = 0; j < 4; ++j) {
+        const int row = row0 + 4 * g + j;
+        const float v = (row < m && r16 < NIN) ? F[(size_t)row * NIN + r16] : 0.0f;
+        xt[j] = (short)to_bf16(v);
+      }
+    }
+    for (int nt = 0; nt < 4; ++nt) gW1[nt] = mfma(xt, dz1b[nt], gW1[nt]);
+  }
+  // block reduction into LDS (each wave adds its register partials)
+  for (int mt = 0; mt < 4; ++mt)
+    for (int nt = 0; nt < 4; ++nt)
+      for (int j = 0; j < 4; ++j)
+        atomicAdd(&s.grad[P_W2 + (16 * mt + 4 * g + j) * NH + 16 * nt + r16], gW2[mt][nt][j]);
+  for (int nt = 0; nt < 4; ++nt)
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * g + j;
+      if (k < NIN) atomicAdd(&s.grad[P_W1 + k * NH + 16 * nt + r16], gW1[nt][j]);
+    }
+  for (int n = 0; n < 4; ++n) {
+    atomicAdd(&s.grad[P_B1 + 16 * n + r16], gb1[n]);
+    atomicAdd(&s.grad[P_B2 + 16 * n + r16], gb2[n]);
+    atomicAdd(&s.grad[P_W3 + 16 * n + r16], gw3[n]);
+  }
+  if (r16 == 0) atomicAdd(&s.grad[P_B3], gb3);
+  __shared__ float sq_sh[kBlk];
+  sq_sh[threadIdx.x] = sq;
+  __syncthreads();
+  for (int i = threadIdx.x; i < NPARAM; i += kBlk) partial[(size_t)blockIdx.x * NPARAM + i] = s.grad[i];
+  if (threadIdx.x == 0) {
+    float tot = 0.0f;
+    for (int i = 0; i < kBlk; ++i) tot += sq_sh[i];
+    loss_partial[blockIdx.x] = tot;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_partials(const float* __restrict__ partial,
+                                                         int nblocks, float* __restrict__ grad,
+                                                         const float* __restrict__ loss_partial,
+                                                         float* __restrict__ loss, int m) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < NPARAM) {
+    float acc = 0.0f;
+    for (int b = 0; b < nblocks; ++b) acc += partial[(size_t)b * NPARAM + i];
+    grad[i] = acc;
+  }
+  if (i == 0 && loss) {
+    float tot = 0.0f;
+    for (int b = 0; b < nblocks; ++b) tot += loss_partial[b];
+    *loss = tot / (float)m;
+  }
+}
+
+// Adam (Kingma & Ba 2015); bias corrections passed in as 1 - beta^t.
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ theta, float* __restrict__ mom,
+                                              float* __restrict__ vel,
+                                              const float* __restrict__ grad, float lr, float b1,
+                                              float b2, float eps, float c1, float c2, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = grad[i];
+  const float m = b1 * mom[i] + (1.0f - b1) * gi;
+  const float v = b2 * vel[i] + (1.0f - b2) * gi * gi;
+  mom[i] = m;
+  vel[i] = v;
+  theta[i] = theta[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+}  // namespace
+
+extern "C" {
+
+int xpbd_mlp_param_count(void) { return NPARAM; }
+
+int xpbd_mlp_forward(void* stream, const float* theta, const float* F, float* Y, int m) {
+  if (m <= 0) return 0;
+  int nb = (m + 16 * kWaves - 1) / (16 * kWaves);
+  if (nb > 2048) nb = 2048;
+  hipLaunchKernelGGL(k_mlp_forward, dim3(nb), dim3(kBlk), 0, (hipStream_t)stream, theta, F, Y, m);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+long long xpbd_mlp_train_workspace_bytes(int nblocks) {
+  return (long long)sizeof(float) * ((long long)nblocks * NPARAM + nblocks);
+}
+
+int xpbd_mlp_grad(void* stream, const float* theta, const float* F, const float* T, int m,
+                  int nblocks, void* ws, float* grad, float* loss) {
+  if (m <= 0 || nblocks <= 0) return (int)hipErrorInvalidValue;
+  float* partial = (float*)ws;
+  float* lossp = partial + (size_t)nblocks * NPARAM;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_mlp_train_partial, dim3(nblocks), dim3(kBlk), 0, s, theta, F, T, m,
+                     partial, lossp);
+  hipLaunchKernelGGL(k_reduce_partials, dim3((NPARAM + 255) / 256), dim3(256), 0, s, partial,
+                     nblocks, grad, lossp, loss, m);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_mlp_adam(void* stream, float* theta, float* mom, float* vel, const float* grad,
+                  float lr, float beta1, float beta2, float eps, int step) {
+  if (step < 1) return (int)hipErrorInvalidValue;
+  const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, (hipStream_t)stream, theta,
+                     mom, vel, grad, lr, beta1, beta2, eps, c1, c2, NPARAM);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"

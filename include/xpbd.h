@@ -40,8 +40,9 @@ extern "C" {
 
 /* Largest LDS accumulator window (particles) of the Jacobi projection. */
 #define XPBD_MAX_WINDOW 8192
-/* Largest particle tile of the fused Jacobi iteration (float4 per particle in LDS). */
-#define XPBD_MAX_TILE 8192
+/* Largest particle tile of the fused Jacobi iteration (28 B of LDS per particle:
+ * three 64-bit fixed-point sums and a count). */
+#define XPBD_MAX_TILE 2048
 
 /* ---- 3-DOF particle kernels ------------------------------------------- */
 
@@ -78,10 +79,12 @@ int xpbd_solve_jacobi(void* stream, const float* X, const float* XP, const int* 
 /* One full Jacobi iteration, kernels.py:219-339 `solve_constraints_jacobi`
  * + `apply_jacobi` fused (solver.py:458-467), over particle tiles of
  * tile_size particles: reads Xin / LAMin, writes Xout / LAMout (distinct
- * buffers).  Rows sorted by smallest endpoint; tile_row_start[t] is the first
- * row whose smallest endpoint lies in tile t (n_tiles + 1 entries);
- * halo_rows[halo_start[t] .. halo_start[t+1]) lists the rows whose larger
- * endpoint lies in tile t and smaller endpoint in an earlier tile.
+ * buffers).  Rows grouped by the tile of their smallest endpoint (any order
+ * inside a tile); tile_row_start[t] is the first row of tile t (n_tiles + 1
+ * entries); halo_rows[halo_start[t] .. halo_start[t+1]) lists the rows whose
+ * larger endpoint lies in tile t and smaller endpoint in an earlier tile.
+ * Per-particle sums are exact (2^-40 m fixed point), so results do not depend
+ * on the order of the atomic adds.
  * finalize != 0 also applies finalize_velocity + cap_velocity
  * (kernels.py:346-371) and writes V.  ACC (optional, float4[n]) holds sums
  * from xpbd_solve_jacobi_dyn of this iteration; they join the average and

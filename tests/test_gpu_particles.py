@@ -140,22 +140,33 @@ def _stats(x, v, inv_mass, h):
 
 
 def test_cloth_jacobi_600_frame_statistics(oracle):
-    """Jacobi over 600 frames: RMS constraint violation and kinetic energy of
-    the GPU run stay within 1 % of the oracle's (48^2 cloth, same generator)."""
+    """Jacobi over 600 frames (48^2 cloth, same generator): the 600-frame
+    means of the RMS violation of the distance rows and of the kinetic energy
+    agree with the oracle's within 1 %, and so does the RMS violation over
+    every 100-frame window.  Instantaneous values are not compared: the
+    swinging sheet is chaotic, and a one-ulp change of the oracle's own start
+    positions moves its instantaneous kinetic energy by up to 10 % (while the
+    600-frame means move by 0.01 % and 0.4 %)."""
     from xpbd_amd import Solver
     from xpbd_amd.scenes import cloth_sheet
 
     s, _ = cloth_sheet(Solver, res=48, solve_mode="jacobi", device=DEV)
     h = s.host_arrays()
     orc = _oracle_scene(oracle, s)
+    g, o = [], []
     for f in range(600):
         s.step()
         orc.step(1)
-        if (f + 1) % 100 == 0:
-            rg, kg = _stats(s.positions(), s.velocities(), h["inv_mass"], h)
-            ro, ko = _stats(orc.x, orc.v, h["inv_mass"], h)
-            assert abs(rg - ro) <= 0.01 * ro + 1e-9, (f, rg, ro)
-            assert abs(kg - ko) <= 0.01 * ko + 1e-9, (f, kg, ko)
+        g.append(_stats(s.positions(), s.velocities(), h["inv_mass"], h))
+        o.append(_stats(orc.x, orc.v, h["inv_mass"], h))
+    g, o = np.asarray(g), np.asarray(o)
+    for w in range(6):
+        rg, ro = g[100 * w:100 * (w + 1), 0].mean(), o[100 * w:100 * (w + 1), 0].mean()
+        assert abs(rg - ro) <= 0.01 * ro, (w, rg, ro)
+    (rg, kg), (ro, ko) = g.mean(axis=0), o.mean(axis=0)
+    print(f"600-frame means: rms {rg:.4e} vs {ro:.4e}, ke {kg:.4e} vs {ko:.4e}")
+    assert abs(rg - ro) <= 0.01 * ro, (rg, ro)
+    assert abs(kg - ko) <= 0.01 * ko, (kg, ko)
 
 
 @pytest.mark.parametrize("tiling", [(1, 0), (2, 1024), (1, 256)])
@@ -196,6 +207,25 @@ def test_jacobi_fused_tile_variants(oracle, tile, reorder, iters):
         return s
 
     _substep_check(oracle, build, n_checks=3, frames_between=2, tol=1e-4)
+
+
+@pytest.mark.parametrize("tile", [64, 1024])
+def test_jacobi_tile_is_bitwise_reproducible(tile):
+    """The fused tile kernel sums corrections in exact fixed point, so two
+    runs of the same scene agree bit for bit although the order of the
+    atomic adds differs from run to run."""
+    from xpbd_amd import Solver
+    from xpbd_amd.scenes import cloth_sheet
+
+    out = []
+    for _ in range(2):
+        s, _ = cloth_sheet(Solver, res=48, solve_mode="jacobi", device=DEV)
+        s.tile_size = tile
+        for _ in range(30):
+            s.step()
+        out.append((s.positions(), s.velocities(), s.lambdas()))
+    for g, e in zip(out[0], out[1]):
+        assert np.array_equal(g, e)
 
 
 def test_graph_replay_equals_eager_launches():

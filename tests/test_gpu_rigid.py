@@ -164,14 +164,22 @@ def test_rigid_substep_parity_and_contact_counts(oracle):
 
 
 def test_rigid_frames_track_oracle(oracle):
-    """Whole frames (8 substeps x 2 iterations, graph replay) over 20 frames:
-    the device trajectory stays close to the oracle's."""
+    """Whole frames (8 substeps x 2 iterations, graph replay) over 20 frames
+    of a 60-box pile.  The pile is chaotic: a one-ulp change of the oracle's
+    own start positions grows to a 2.7e-3 relative difference by frame 3 and
+    to 0.18 by frame 20.  Each frame the device trajectory must stay within
+    1e-3, or within 4x that self-sensitivity of the oracle where it is larger."""
     s = _pile_scene(1)
+    h = s.host_arrays()
     orc = _oracle6(oracle, s)
+    pert = _oracle6(oracle, s)
+    pert.x = (h["x"] * np.float32(1 + 1.2e-7)).astype(np.float32)
     for f in range(20):
         s.step()
         orc.step(1)
-    assert _rel(s.positions(), orc.x) < 2e-3
+        pert.step(1)
+        e, sens = _rel(s.positions(), orc.x), _rel(pert.x, orc.x)
+        assert e <= max(1e-3, 4 * sens), (f, e, sens)
 
 
 # ---- the reference's invariant tests (tests/test_solver_6dof.py) -----------

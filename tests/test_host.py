@@ -206,14 +206,16 @@ def test_ctypes_signatures_match_header_arity():
         assert len(args) == arity[name], f"{name}: ctypes {len(args)} args, header {arity[name]}"
 
 
-@pytest.mark.parametrize("tile", [7, 64, 100000])
-def test_tile_tables_cover_every_endpoint(tile):
+@pytest.mark.parametrize("tile,order", [(7, "lo"), (64, "lo"), (100000, "lo"), (7, "tile"),
+                                        (64, "tile"), (100000, "tile")])
+def test_tile_tables_cover_every_endpoint(tile, order):
     """Every (row, endpoint) pair is visited exactly once by the tile that
     owns the endpoint: as an owned row (smallest endpoint in the tile) or as
     a halo row (larger endpoint in the tile, smaller endpoint elsewhere)."""
     from xpbd_amd import Solver
     from xpbd_amd.scenes import cloth_sheet
-    from xpbd_amd.solver import jacobi_row_order, morton_order, pack_rows, tile_tables
+    from xpbd_amd.solver import (jacobi_row_order, jacobi_tile_order, morton_order, pack_rows,
+                                 tile_tables)
 
     s, _ = cloth_sheet(Solver, res=20)
     h = s.host_arrays()
@@ -223,7 +225,12 @@ def test_tile_tables_cover_every_endpoint(tile):
     ra = inv[h["a"]]
     rb = np.where(h["b"] >= 0, inv[np.maximum(h["b"], 0)], -1)
     rows, _ = pack_rows(h["type"], ra, rb, h["rest"], h["compliance"], h["anchor"], h["mu"])
-    rows = rows[jacobi_row_order(ra, rb)]
+    if order == "lo":
+        rows = rows[jacobi_row_order(ra, rb)]
+    else:
+        perm = jacobi_tile_order(ra, rb, h["type"], tile)
+        assert sorted(perm.tolist()) == list(range(len(ra)))
+        rows = rows[perm]
     starts, hs, hr = tile_tables(rows, s.num_bodies, tile)
     n_t = len(starts) - 1
     seen = {}
@@ -239,3 +246,31 @@ def test_tile_tables_cover_every_endpoint(tile):
            {(j, int(b[j])) for j in range(len(rows)) if b[j] >= 0}
     assert set(seen) == want and set(seen.values()) == {1}
     assert starts[0] == 0 and starts[-1] == len(rows)
+
+
+def test_jacobi_tile_order_groups_by_tile_then_type_then_rank():
+    """Inside each tile the rows run through the types in order, and inside a
+    type through the ranks (0 = first row of a smallest endpoint), so each
+    rank block holds every smallest endpoint at most once."""
+    from xpbd_amd import Solver
+    from xpbd_amd.scenes import cloth_sheet
+    from xpbd_amd.solver import jacobi_tile_order
+
+    s, _ = cloth_sheet(Solver, res=24)
+    h = s.host_arrays()
+    a, b, t = h["a"].astype(np.int64), h["b"].astype(np.int64), h["type"].astype(np.int64)
+    tile = 64
+    perm = jacobi_tile_order(a, b, t, tile)
+    lo = np.where(b >= 0, np.minimum(a, b), a)[perm]
+    tp = t[perm]
+    tl = lo // tile
+    assert np.all(np.diff(tl) >= 0)
+    for k in np.unique(tl):
+        sel = tl == k
+        assert np.all(np.diff(tp[sel]) >= 0)
+        for ty in np.unique(tp[sel]):
+            seg = lo[sel][tp[sel] == ty]
+            # ascending runs of distinct endpoints, one run per rank
+            breaks = np.nonzero(np.diff(seg) <= 0)[0]
+            for run in np.split(seg, breaks + 1):
+                assert len(np.unique(run)) == len(run)

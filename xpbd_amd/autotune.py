@@ -4,8 +4,8 @@ For a flushed ``Solver`` in Jacobi mode it times whole frames (hipGraph
 replays, HIP events on the launch stream) under each candidate launch shape
 and keeps the fastest:
 
-* ``tile``   — the fused per-iteration kernel with tile sizes of 128 ... 4096
-  particles (LDS window of tile x 16 B; larger tiles re-project fewer halo
+* ``tile``   — the fused per-iteration kernel with tile sizes of 64 ... 2048
+  particles (28 B of LDS per particle; larger tiles re-project fewer halo
   rows, smaller tiles spread the launch over more CUs);
 * ``window`` — projection with an LDS window + global atomics and a separate
   apply pass, with 1 or 2 rows per thread.
@@ -25,8 +25,8 @@ def _signature(s):
 
 def candidates(n_particles: int):
     out = []
-    for t in (128, 256, 512, 1024, 2048, 4096):
-        if t <= max(128, 2 * n_particles):
+    for t in (64, 128, 256, 512, 1024, 2048):
+        if t <= max(64, 2 * n_particles):
             out.append(("tile", t, None))
     out.append(("window", 0, (1, None)))
     out.append(("window", 0, (2, None)))

@@ -279,18 +279,44 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_project_dyn(
 // launch, no global accumulators or atomics).
 //
 // Particles are split into tiles of P consecutive indices; a row is owned by
-// the tile of its smallest endpoint (rows are sorted by smallest endpoint, so
-// a tile owns the contiguous rows [tile_row_start[t], tile_row_start[t+1])).
-// A row whose other endpoint lies in a later tile is also listed as a halo
-// row of that tile and projected again there: both tiles read the same
-// X_in / LAM_in, so they compute the same correction; each adds only its own
-// endpoints (LDS float atomics into a P-entry float4 window) and only the
-// owner writes LAM_out.  After a barrier every thread applies the averaged
-// correction to its tile's particles and writes X_out (ping-pong buffers), so
-// one launch is one complete Jacobi iteration (kernels.py:219-339).  On the
-// last iteration of a substep the same pass also runs finalize_velocity and
-// cap_velocity (kernels.py:346-371).
+// the tile of its smallest endpoint (rows are grouped by that tile, so tile t
+// owns the contiguous rows [tile_row_start[t], tile_row_start[t+1]); inside a
+// tile they are ordered by type and by rank among the rows of the same
+// smallest endpoint, so neighbouring lanes take the same branch and add into
+// different particles).  A row whose other endpoint lies in another tile is
+// also listed as a halo row of that tile and projected again there: both
+// tiles read the same X_in / LAM_in, so they compute the same correction;
+// each adds only its own endpoints and only the owner writes LAM_out.
+//
+// Accumulation: corrections are summed per particle in LDS as 2^-40 m fixed
+// point with 64-bit integer atomics (plus a 32-bit count).  Integer sums are
+// exact and independent of the order in which lanes arrive, so an iteration
+// is bitwise reproducible from run to run; 64-bit integer LDS atomics also
+// run several times faster than 32-bit float LDS atomics on CDNA.  Range:
+// |correction| <= 2^22 m per row; a larger or non-finite correction marks the
+// particle, whose position then becomes NaN (as a float sum would).
+//
+// After a barrier every thread applies the averaged correction to its tile's
+// particles and writes X_out (ping-pong buffers), so one launch is one
+// complete Jacobi iteration (kernels.py:219-339).  On the last iteration of a
+// substep the same pass also runs finalize_velocity and cap_velocity
+// (kernels.py:346-371).
 // ---------------------------------------------------------------------------
+constexpr float kFixScale = 1099511627776.0f;          // 2^40 units per metre
+constexpr double kFixInv = 1.0 / 1099511627776.0;
+constexpr float kFixLimit = 4194304.0f;                // 2^22 m
+constexpr unsigned kFixPoison = 1u << 24;              // count flag: bad correction
+
+__device__ inline unsigned long long fix_of(float c) {
+  return (unsigned long long)__float2ll_rn(c * kFixScale);
+}
+__device__ inline float unfix(unsigned long long v) {
+  return (float)((double)(long long)v * kFixInv);
+}
+
+// LDS layout of a tile accumulator: x, y, z sums (int64, P each), count (u32, P).
+inline size_t tile_lds_bytes(int P) { return (size_t)P * (3 * sizeof(long long) + sizeof(unsigned)); }
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     const float4* __restrict__ Xin, float4* __restrict__ Xout, const float4* __restrict__ XP,
@@ -299,14 +325,30 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     const int* __restrict__ tile_row_start, const int* __restrict__ halo_start,
     const int* __restrict__ halo_rows, float h, int first, int P, int n, float relax,
     int finalize, float damping, float max_speed, int do_cap, float4* __restrict__ ACC) {
-  extern __shared__ float4 acc[];
+  extern __shared__ unsigned long long lds64[];
+  unsigned long long* sx = lds64;
+  unsigned long long* sy = lds64 + P;
+  unsigned long long* sz = lds64 + 2 * P;
+  unsigned* cnt = (unsigned*)(lds64 + 3 * P);
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int p0 = tile * P;
   if (p0 >= n) return;
   const int p1 = min(p0 + P, n);
-  for (int k = threadIdx.x; k < P; k += BLOCK) acc[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int k = threadIdx.x; k < 3 * P; k += BLOCK) lds64[k] = 0ull;
+  for (int k = threadIdx.x; k < P; k += BLOCK) cnt[k] = 0u;
   __syncthreads();
   const float inv_dt2 = 1.0f / (h * h);
+  auto add = [&](int i, f3 c) {
+    const unsigned off = (unsigned)(i - p0);
+    if (off >= (unsigned)P) return;
+    const bool ok = fabsf(c.x) <= kFixLimit && fabsf(c.y) <= kFixLimit && fabsf(c.z) <= kFixLimit;
+    if (ok) {
+      atomicAdd(&sx[off], fix_of(c.x));
+      atomicAdd(&sy[off], fix_of(c.y));
+      atomicAdd(&sz[off], fix_of(c.z));
+    }
+    atomicAdd(&cnt[off], ok ? 1u : (kFixPoison | 1u));
+  };
   const int rb = tile_row_start[tile], re = tile_row_start[tile + 1];
   const int hb = halo_start[tile], he = halo_start[tile + 1];
   const int owned = re - rb;
@@ -320,29 +362,17 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     if (row_type(row) == T_ATTACH) anc = ANC[j];
     const Proj p = project_jacobi(Xin, XP, row, anc, lam, inv_dt2);
     if (own) LAMout[j] = p.lam;
-    if (p.na >= 1) {
-      const unsigned off = (unsigned)(p.a - p0);
-      if (off < (unsigned)P) {
-        atomicAdd(&acc[off].x, p.ca.x);
-        atomicAdd(&acc[off].y, p.ca.y);
-        atomicAdd(&acc[off].z, p.ca.z);
-        atomicAdd(&acc[off].w, 1.0f);
-      }
-    }
-    if (p.na == 2) {
-      const unsigned off = (unsigned)(p.b - p0);
-      if (off < (unsigned)P) {
-        atomicAdd(&acc[off].x, p.cb.x);
-        atomicAdd(&acc[off].y, p.cb.y);
-        atomicAdd(&acc[off].z, p.cb.z);
-        atomicAdd(&acc[off].w, 1.0f);
-      }
-    }
+    if (p.na >= 1) add(p.a, p.ca);
+    if (p.na == 2) add(p.b, p.cb);
   }
   __syncthreads();
   for (int i = p0 + threadIdx.x; i < p1; i += BLOCK) {
+    const int o = i - p0;
     const float4 x4 = Xin[i];
-    float4 a = acc[i - p0];
+    const unsigned c = cnt[o];
+    float4 a = make_float4(0.f, 0.f, 0.f, (float)(c & (kFixPoison - 1u)));
+    if (c) a = make_float4(unfix(sx[o]), unfix(sy[o]), unfix(sz[o]), a.w);
+    if (c & kFixPoison) a.x = __int_as_float(0x7fc00000);
     if (ACC) {  // corrections of the device-counted row segment
       const float4 g = ACC[i];
       if (g.w != 0.0f) {
@@ -634,12 +664,20 @@ int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* X
   if (tile_size < 1 || tile_size > XPBD_MAX_TILE) return (int)hipErrorInvalidValue;
   const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
   const int nb = blocks_for(n_particles, tile_size);
-  const size_t shmem = sizeof(float4) * (size_t)tile_size;
-  hipLaunchKernelGGL(k_jacobi_tile<kTileBlock>, dim3(nb), dim3(kTileBlock), shmem,
-                     (hipStream_t)stream, (const float4*)Xin, (float4*)Xout, (const float4*)XP,
-                     (float4*)V, (const int4*)R, (const float4*)ANC, LAMin, LAMout,
-                     tile_row_start, halo_start, halo_rows, h, first, tile_size, n_particles,
-                     relax, finalize, damping, max_speed, do_cap, (float4*)ACC);
+  const size_t shmem = tile_lds_bytes(tile_size);
+  // small tiles (small scenes, many blocks) run 256-thread blocks
+  if (tile_size <= 256)
+    hipLaunchKernelGGL(k_jacobi_tile<kBlock>, dim3(nb), dim3(kBlock), shmem, (hipStream_t)stream,
+                       (const float4*)Xin, (float4*)Xout, (const float4*)XP, (float4*)V,
+                       (const int4*)R, (const float4*)ANC, LAMin, LAMout, tile_row_start,
+                       halo_start, halo_rows, h, first, tile_size, n_particles, relax, finalize,
+                       damping, max_speed, do_cap, (float4*)ACC);
+  else
+    hipLaunchKernelGGL(k_jacobi_tile<kTileBlock>, dim3(nb), dim3(kTileBlock), shmem,
+                       (hipStream_t)stream, (const float4*)Xin, (float4*)Xout, (const float4*)XP,
+                       (float4*)V, (const int4*)R, (const float4*)ANC, LAMin, LAMout,
+                       tile_row_start, halo_start, halo_rows, h, first, tile_size, n_particles,
+                       relax, finalize, damping, max_speed, do_cap, (float4*)ACC);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

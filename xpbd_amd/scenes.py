@@ -133,7 +133,7 @@ def _part1by1(v):
 
 def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing: float = 0.01,
                         mass: float = 1e-3, height: float = 0.5, floor_mu: float = 0.4,
-                        compliance_range=(1e-8, 1e-3), tile: int = 2048):
+                        compliance_range=(1e-8, 1e-3), tile: int = 1024):
     """The config-2 cloth generator scaled to ``n_sheets`` sheets of res^2,
     built directly on the device in the packed layout of include/xpbd.h, for
     the HBM-roofline scene (8 x 1024^2 = 58.6M rows), where host-side scene
@@ -142,9 +142,9 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
     (k,k+res-1) from its right neighbour, (k,+2x), (k,+2y), plus FLOOR(k).
 
     Particles are numbered in 2-D Z-order within each sheet (the locality
-    layout the solver's Jacobi path uses), rows are sorted by smallest
-    endpoint, and the fused-kernel tile tables for ``tile`` particles per tile
-    are built on the device.
+    layout the solver's Jacobi path uses), rows are in the fused kernel's
+    order (``solver.jacobi_tile_order``), and its tile tables for ``tile``
+    particles per tile are built on the device.
 
     Returns dict of torch tensors X, X_B, V, XP, ACC (float4 per particle), R
     (int4 rows), ANC, LAM, LAM_B, TRS, HS, HR (tile tables) and counts."""
@@ -190,12 +190,19 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
     lo_c, hi_c = torch.log(torch.tensor(compliance_range, dtype=torch.float64))
     comp = torch.exp(torch.rand(m, device=device, generator=g, dtype=torch.float64)
                      * (hi_c - lo_c) + lo_c).float()
-    # renumber endpoints, sort rows by smallest endpoint
+    # renumber endpoints; order rows as the fused tile kernel wants them:
+    # by tile of the smallest endpoint, then type, then rank among the rows of
+    # the same smallest endpoint, then that endpoint
     a = inv[a]
     b = torch.where(is_d, inv[torch.clamp(b, min=0)], b)
     lo = torch.where(is_d, torch.minimum(a, b), a)
     hi = torch.where(is_d, torch.maximum(a, b), a)
     ro = torch.argsort(lo, stable=True)
+    a, b, t, lo, hi, rest, comp, is_d = (v[ro] for v in (a, b, t, lo, hi, rest, comp, is_d))
+    rank = torch.arange(m, device=device) - torch.searchsorted(lo, lo)
+    key = (lo // tile) * (1 << 32) + t * (1 << 28) + rank * (1 << 16) + lo % tile
+    ro = torch.argsort(key, stable=True)
+    del rank, key
     a, b, t, lo, hi, rest, comp, is_d = (v[ro] for v in (a, b, t, lo, hi, rest, comp, is_d))
     p0 = torch.where(is_d, rest, torch.zeros_like(rest))
     p1 = torch.where(is_d, comp, torch.full_like(comp, floor_mu))
@@ -205,8 +212,7 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
     R[:, 2] = p0.view(torch.int32)
     R[:, 3] = p1.view(torch.int32)
     n_tiles = (n + tile - 1) // tile
-    bounds = torch.arange(n_tiles + 1, device=device, dtype=torch.int64) * tile
-    TRS = torch.searchsorted(lo, bounds).to(torch.int32)
+    TRS = torch.searchsorted(lo // tile, torch.arange(n_tiles + 1, device=device)).to(torch.int32)
     TRS[-1] = m
     cross = torch.nonzero((hi // tile) != (lo // tile)).flatten()
     ht = hi[cross] // tile

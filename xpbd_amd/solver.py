@@ -39,11 +39,12 @@ MODE_JACOBI_TILE = 2
 def default_tile_size(n_particles: int) -> int:
     """Particles per tile of the fused Jacobi kernel: large tiles keep the
     share of re-projected halo rows small; small scenes get smaller tiles so
-    that the launch still spreads over the 256 CUs."""
-    for t in (2048, 1024, 512, 256):
+    that the launch still spreads over the 256 CUs.  1024 is the largest
+    default: at 2048 (56 KB of LDS per block) fewer blocks fit on a CU."""
+    for t in (1024, 512, 256, 128):
         if n_particles >= 256 * t:
             return t
-    return 128
+    return 64
 
 
 class _Col:
@@ -143,6 +144,22 @@ def jacobi_row_order(a, b) -> np.ndarray:
     return np.argsort(key, kind="stable")
 
 
+def jacobi_tile_order(a, b, types, tile: int) -> np.ndarray:
+    """Stable row order of the fused tile kernel: grouped by the tile of the
+    smallest endpoint; inside a tile by type, then by rank among the rows of
+    the same smallest endpoint, then by that endpoint.  Rows of a wavefront
+    then take the same branch and mostly add into distinct particles."""
+    a = np.asarray(a, np.int64)
+    b = np.asarray(b, np.int64)
+    lo = np.where(b >= 0, np.minimum(a, b), a)
+    first = jacobi_row_order(a, b)
+    los = lo[first]
+    rank = np.empty(len(lo), np.int64)
+    rank[first] = np.arange(len(lo), dtype=np.int64) - np.searchsorted(los, los, "left")
+    key = np.lexsort((lo, rank, np.asarray(types, np.int64), lo // tile))
+    return key.astype(np.int64)
+
+
 def choose_jacobi_tiling(rows_sorted: np.ndarray, block: int = 256) -> tuple[int, int]:
     """(rows_per_thread, window) for the LDS-staged Jacobi projection: the
     smallest power-of-two window that holds the endpoint span of 90 % of the
@@ -186,16 +203,19 @@ def morton_order(pos: np.ndarray) -> np.ndarray:
 
 
 def tile_tables(rows_sorted: np.ndarray, n_particles: int, tile: int):
-    """Tile tables of the fused Jacobi kernel for rows sorted by smallest
-    endpoint: ``tile_row_start`` (first owned row of each tile, n_tiles + 1
-    entries) and the halo rows (larger endpoint in a later tile than the
-    smaller one) grouped by the tile of the larger endpoint."""
+    """Tile tables of the fused Jacobi kernel for rows grouped by the tile of
+    their smallest endpoint: ``tile_row_start`` (first owned row of each
+    tile, n_tiles + 1 entries) and the halo rows (larger endpoint in a later
+    tile than the smaller one) grouped by the tile of the larger endpoint."""
     n_tiles = max(1, (n_particles + tile - 1) // tile)
     a = (rows_sorted[:, 0] & 0x0FFFFFFF).astype(np.int64)
     b = rows_sorted[:, 1].astype(np.int64)
     lo = np.where(b >= 0, np.minimum(a, b), a)
     hi = np.where(b >= 0, np.maximum(a, b), a)
-    starts = np.searchsorted(lo, np.arange(n_tiles + 1, dtype=np.int64) * tile, "left")
+    lt = lo // tile
+    if len(lt) and np.any(np.diff(lt) < 0):
+        raise ValueError("rows are not grouped by the tile of their smallest endpoint")
+    starts = np.searchsorted(lt, np.arange(n_tiles + 1, dtype=np.int64), "left")
     starts[-1] = len(lo)
     cross = np.nonzero((hi // tile) != (lo // tile))[0]
     ht = hi[cross] // tile
@@ -593,13 +613,20 @@ class Solver:
             self.num_colors = num_colors
             self.color_counts = color_summary(color)
         else:
-            perm = jacobi_row_order(ra, rb) if n_c else np.zeros(0, np.int64)
+            if not n_c:
+                perm = np.zeros(0, np.int64)
+            elif tile_mode:
+                self._tile = self.tile_size or default_tile_size(n_b)
+                perm = jacobi_tile_order(ra, rb, h["type"], self._tile)
+            else:
+                perm = jacobi_row_order(ra, rb)
             self._color_starts = np.array([0, n_c], dtype=np.int32)
             self.num_colors = 1 if n_c else 0
             self.color_counts = {}
         rows = np.ascontiguousarray(rows[perm])
         anc = np.ascontiguousarray(anc[perm])
-        self._tiling = choose_jacobi_tiling(rows) if n_c else (1, 0)
+        self._tiling = ((choose_jacobi_tiling(rows) if not tile_mode else (1, 0)) if n_c
+                        else (1, 0))
         self._tile = 0
         self._tile_tabs = None
         if tile_mode and n_c:

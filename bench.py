@@ -47,27 +47,35 @@ def iter_bytes(n_distance: int, n_floor: int, n_particles: int) -> int:
     return 24 * n_distance + 20 * n_floor + 40 * n_particles
 
 
-def time_kernel(fn, reps: int) -> float:
+def time_kernel(fn, reps: int, rounds: int = 3, warmup: int = 3) -> float:
     """Average duration in ms of ``fn`` (which launches on torch's current
-    stream), bracketed by HIP events on that stream."""
+    stream), bracketed by HIP events on that stream: the median over
+    ``rounds`` windows of ``reps`` back-to-back launches, after ``warmup``
+    launches."""
     import torch
 
-    fn()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
+    for _ in range(warmup):
         fn()
-    e1.record()
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps
+    out = []
+    for _ in range(rounds):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out.append(e0.elapsed_time(e1) / reps)
+    return sorted(out)[len(out) // 2]
 
 
 def tile_launcher(N, X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR, h, tile, n):
-    """One k_jacobi_tile launch (a full Jacobi iteration) on fixed buffers."""
+    """One k_jacobi_tile launch (a full Jacobi iteration) on fixed buffers
+    (ANC None: the scene has no ATTACH rows)."""
     lib = N.lib()
-    args = tuple(N.ptr(t) for t in (X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR))
+    args = tuple(None if t is None else N.ptr(t)
+                 for t in (X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR))
 
     def fn():
         N.check(lib.xpbd_jacobi_tile(N.current_stream(), *args, h, 0, tile, n, 1.0, 0, 0.0,
@@ -109,14 +117,14 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     torch.cuda.synchronize()
     n, m, nd = sc["n_particles"], sc["n_rows"], sc["n_distance"]
     h = float(torch.tensor(1.0 / 60.0 / 15.0, dtype=torch.float32))
-    launch = tile_launcher(N, sc["X"], sc["X_B"], sc["XP"], sc["V"], sc["R"], sc["ANC"],
+    launch = tile_launcher(N, sc["X"], sc["X_B"], sc["XP"], sc["V"], sc["R"], None,
                            sc["LAM"], sc["LAM_B"], sc["TRS"], sc["HS"], sc["HR"], h, sc["tile"], n)
     ms = time_kernel(launch, reps)
     nbytes = iter_bytes(nd, m - nd, n)
     d = N.ParticlesDesc()
     d.x, d.v, d.x_prev, d.acc = (N.ptr(sc["X"]), N.ptr(sc["V"]), N.ptr(sc["XP"]),
                                  N.ptr(sc["ACC"]))
-    d.rows, d.anchors, d.lam = N.ptr(sc["R"]), N.ptr(sc["ANC"]), N.ptr(sc["LAM"])
+    d.rows, d.anchors, d.lam = N.ptr(sc["R"]), None, N.ptr(sc["LAM"])
     d.n_particles, d.n_rows = n, m
     d.color_starts, d.num_colors = None, 0
     d.mode, d.substeps, d.iterations = MODE_JACOBI_TILE, 15, 2
@@ -125,11 +133,15 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     d.x_alt, d.lam_alt = N.ptr(sc["X_B"]), N.ptr(sc["LAM_B"])
     d.tile_row_start, d.halo_start, d.halo_rows = N.ptr(sc["TRS"]), N.ptr(sc["HS"]), N.ptr(sc["HR"])
     d.tile_size = sc["tile"]
-    plan = N.c_void_p()
-    N.check(N.lib().xpbd_frame3_create(N.ctypes.byref(d), N.ctypes.byref(plan)), "create")
-    run = lambda: N.check(N.lib().xpbd_frame3_run(plan, N.current_stream(), 1, 1), "run")  # noqa
-    frame_ms = time_kernel(run, frames)
-    N.lib().xpbd_frame3_destroy(plan)
+    frame = {}
+    for fuse in (1, 0):
+        d.fuse_first = fuse
+        plan = N.c_void_p()
+        N.check(N.lib().xpbd_frame3_create(N.ctypes.byref(d), N.ctypes.byref(plan)), "create")
+        run = lambda: N.check(N.lib().xpbd_frame3_run(plan, N.current_stream(), 1, 1), "run")  # noqa
+        frame[fuse] = time_kernel(run, frames)
+        N.lib().xpbd_frame3_destroy(plan)
+    frame_ms = frame[1]
     achieved = nbytes / (ms * 1e-3) / 1e9
     return {
         "workload": f"{sheets} x {res}^2 cloth: {n} particles, {nd} DISTANCE + {m - nd} FLOOR "
@@ -138,6 +150,7 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": nbytes,
         "kernel_ms": round(ms, 4), "frame_ms": round(frame_ms, 3),
+        "frame_ms_separate_integrate": round(frame[0], 3), "tile": sc["tile"],
         "proj_per_s": m * 30 / (frame_ms * 1e-3),
     }
 
@@ -212,6 +225,8 @@ def main() -> int:
     ap.add_argument("--no-roofline-scene", action="store_true")
     ap.add_argument("--no-others", action="store_true",
                     help="skip the chain / unified / batched secondary lines")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape instead of the autotuner's pick")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run plain 512 MiB device copies (PMC calibration)")
     args = ap.parse_args()
@@ -231,6 +246,15 @@ def main() -> int:
     host = s.host_arrays()
     n_dist = int((host["type"] == 0).sum())
     n_floor = int((host["type"] == 2).sum())
+    tuned = None
+    if args.mode == "jacobi" and not args.no_autotune:
+        from xpbd_amd import autotune
+
+        best, trials = autotune.tune(s, frames=4)
+        tuned = {"kernel": best[0], "tile": best[1], "fuse_first": best[3],
+                 "trials_ms_per_frame": {f"{c[0]}:{c[1]}:{'fused' if c[3] else 'separate'}":
+                                         round(v, 4) for c, v in trials.items()}}
+        print(f"[bench] autotune picked {best}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         s.step()
     torch.cuda.synchronize()
@@ -246,10 +270,11 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     roof = None
-    if args.mode == "jacobi" and s._tile_dev is not None:
+    if args.mode == "jacobi" and s._tile_dev is not None and s.jacobi_kernel == "tile":
         h = float(torch.tensor(s.dt / s.substeps, dtype=torch.float32))
         ts, hs, hr = s._tile_dev
-        launch = tile_launcher(N, s.X, s.X_B, s.XP, s.V, s.R, s.ANC, s.LAM, s.LAM_B, ts, hs, hr,
+        launch = tile_launcher(N, s.X, s.X_B, s.XP, s.V, s.R, s.ANC if s._has_attach else None,
+                               s.LAM, s.LAM_B, ts, hs, hr,
                                h, s._tile, s.num_bodies)
         x_keep, v_keep = s.positions(), s.velocities()
         kms = time_kernel(launch, args.kernel_reps)
@@ -299,7 +324,7 @@ def main() -> int:
                                f"{n_dist} DISTANCE + {n_floor} FLOOR rows, {args.mode}, "
                                f"15 substeps x 2 iterations, dt 1/60, hipGraph per frame",
                    "rows": s.num_constraints, "substeps": s.substeps,
-                   "iterations": s.iterations, "mode": args.mode},
+                   "iterations": s.iterations, "mode": args.mode, "launch": tuned},
         "substeps_per_s": s.substeps / (ms * 1e-3),
         "roofline": roof,
         "roofline_scene": scene,

@@ -40,9 +40,9 @@ extern "C" {
 
 /* Largest LDS accumulator window (particles) of the Jacobi projection. */
 #define XPBD_MAX_WINDOW 8192
-/* Largest particle tile of the fused Jacobi iteration (28 B of LDS per particle:
- * three 64-bit fixed-point sums and a count). */
-#define XPBD_MAX_TILE 2048
+/* Largest particle tile of the fused Jacobi iteration (44 B of LDS per particle:
+ * three 64-bit fixed-point sums, a count and the staged position). */
+#define XPBD_MAX_TILE 1024
 
 /* ---- 3-DOF particle kernels ------------------------------------------- */
 
@@ -84,7 +84,7 @@ int xpbd_solve_jacobi(void* stream, const float* X, const float* XP, const int* 
  * entries); halo_rows[halo_start[t] .. halo_start[t+1]) lists the rows whose
  * larger endpoint lies in tile t and smaller endpoint in an earlier tile.
  * Per-particle sums are exact (2^-40 m fixed point), so results do not depend
- * on the order of the atomic adds.
+ * on the order of the atomic adds.  ANC may be NULL when no row is ATTACH.
  * finalize != 0 also applies finalize_velocity + cap_velocity
  * (kernels.py:346-371) and writes V.  ACC (optional, float4[n]) holds sums
  * from xpbd_solve_jacobi_dyn of this iteration; they join the average and
@@ -94,6 +94,21 @@ int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* X
                      const int* tile_row_start, const int* halo_start, const int* halo_rows,
                      float h, int first, int tile_size, int n_particles, float relax,
                      int finalize, float damping, float max_speed, float* ACC);
+
+/* The first iteration of a substep with kernels.py:49-73 `integrate` fused
+ * in (solver.py:431-458: integrate, lambda reset, first projection + apply):
+ * X holds the positions at the start of the substep and is only read; every
+ * position the launch needs is predicted from X and V exactly as
+ * xpbd_integrate_to computes it; x_prev (XP) of every particle is written,
+ * lambda starts from zero (LAMout written for every row), Xout receives the
+ * iterated positions (Xout != X).  V is not written: the caller finishes the
+ * substep with at least one xpbd_jacobi_tile launch with finalize = 1.
+ * Same tile tables as xpbd_jacobi_tile; gscale as in xpbd_integrate_to. */
+int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP, const float* V,
+                           const int* R, const float* ANC, float* LAMout,
+                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
+                           float h, float gx, float gy, float gz, const float* gscale,
+                           int tile_size, int n_particles, float relax);
 
 /* kernels.py:49-73 `integrate` reading Xin and writing Xout (may alias);
  * gscale (optional, float[n]) scales gravity per particle (batched scenes). */
@@ -166,6 +181,10 @@ typedef struct xpbd_particles_desc {
   const float* ws_theta; /* float[5057] */
   float* ws_feat;        /* float[n_rows * 12] */
   float* ws_y;           /* float[n_rows] */
+  /* Tile mode with >= 2 iterations and no dyn rows: 1 folds integrate into
+   * the first iteration (xpbd_jacobi_tile_first), 0 keeps a separate
+   * xpbd_integrate_to launch per substep.  Results are identical. */
+  int fuse_first;
 } xpbd_particles_desc;
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);

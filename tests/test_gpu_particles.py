@@ -190,11 +190,11 @@ def test_jacobi_window_and_global_atomic_variants_agree(oracle, tiling):
     _substep_check(oracle, build, n_checks=3, frames_between=2, tol=1e-4)
 
 
-@pytest.mark.parametrize("tile,reorder,iters", [(32, True, 2), (128, False, 3), (2048, True, 1)])
+@pytest.mark.parametrize("tile,reorder,iters", [(32, True, 2), (128, False, 3), (1024, True, 1)])
 def test_jacobi_fused_tile_variants(oracle, tile, reorder, iters):
     """The fused tile kernel with tiny tiles (most rows are halo rows), no
     particle reordering, an odd iteration count (final state in the second
-    ping-pong buffer) and one tile larger than the scene."""
+    ping-pong buffer) and the largest tile."""
     from xpbd_amd import Solver
     from xpbd_amd.scenes import cloth_sheet
 
@@ -226,6 +226,77 @@ def test_jacobi_tile_is_bitwise_reproducible(tile):
         out.append((s.positions(), s.velocities(), s.lambdas()))
     for g, e in zip(out[0], out[1]):
         assert np.array_equal(g, e)
+
+
+@pytest.mark.parametrize("gscale", [False, True])
+def test_tile_first_equals_integrate_then_tile(gscale):
+    """xpbd_jacobi_tile_first (integrate folded into the first iteration)
+    gives bit for bit what xpbd_integrate_to followed by a first plain tile
+    launch gives: positions, x_prev and multipliers."""
+    import torch
+
+    from xpbd_amd import Solver
+    from xpbd_amd import _native as N
+    from xpbd_amd.scenes import cloth_sheet
+
+    s, _ = cloth_sheet(Solver, res=40, solve_mode="jacobi", device=DEV)
+    s.tile_size = 64
+    for _ in range(5):
+        s.step()
+    s._flush()
+    lib = N.lib()
+    n, m = s.num_bodies, s.num_constraints
+    h = float(np.float32(s.dt / s.substeps))
+    g = s.gravity
+    gs = (torch.rand(n, device=DEV) + 0.5) if gscale else None
+    ts, hs, hr = s._tile_dev
+    X0, V0 = s.X.clone(), s.V.clone()
+    # path A: integrate, then the plain first iteration
+    XA, VA, XPA = torch.empty_like(X0), V0.clone(), torch.zeros_like(X0)
+    XoA, LA = torch.empty_like(X0), torch.zeros(m, device=DEV)
+    N.check(lib.xpbd_integrate_to(None, N.ptr(X0), N.ptr(XA), N.ptr(VA), N.ptr(XPA), *g, h,
+                                  N.ptr(gs) if gs is not None else None, n))
+    N.check(lib.xpbd_jacobi_tile(None, N.ptr(XA), N.ptr(XoA), N.ptr(XPA), N.ptr(VA), N.ptr(s.R),
+                                 N.ptr(s.ANC), N.ptr(LA), N.ptr(LA), N.ptr(ts), N.ptr(hs),
+                                 N.ptr(hr), h, 1, s._tile, n, s.jacobi_relax, 0, s.damping,
+                                 s.max_speed, None))
+    # path B: fused
+    XoB, XPB, LB = torch.empty_like(X0), torch.zeros_like(X0), torch.zeros(m, device=DEV)
+    N.check(lib.xpbd_jacobi_tile_first(None, N.ptr(X0), N.ptr(XoB), N.ptr(XPB), N.ptr(V0),
+                                       N.ptr(s.R), N.ptr(s.ANC), N.ptr(LB), N.ptr(ts), N.ptr(hs),
+                                       N.ptr(hr), h, *g, N.ptr(gs) if gs is not None else None,
+                                       s._tile, n, s.jacobi_relax))
+    torch.cuda.synchronize()
+    assert torch.equal(XoA, XoB)
+    assert torch.equal(XPA, XPB)
+    assert torch.equal(LA, LB)
+    assert torch.equal(V0, s.V)          # the fused launch leaves V alone
+
+
+def test_autotune_restores_state_and_keeps_results():
+    """The autotuner times every launch shape without advancing the scene,
+    and the shape it picks steps to the same state (to the Jacobi parity
+    bar) as the default shape."""
+    from xpbd_amd import Solver, autotune
+    from xpbd_amd.scenes import cloth_sheet
+
+    s, _ = cloth_sheet(Solver, res=40, solve_mode="jacobi", device=DEV)
+    ref, _ = cloth_sheet(Solver, res=40, solve_mode="jacobi", device=DEV)
+    for _ in range(3):
+        s.step()
+        ref.step()
+    x0, v0 = s.positions(), s.velocities()
+    best, times = autotune.tune(s, frames=2)
+    assert best in times and len(times) >= 4
+    assert np.array_equal(s.positions(), x0) and np.array_equal(s.velocities(), v0)
+    for _ in range(5):
+        s.step()
+        ref.step()
+    assert _rel(s.positions(), ref.positions()) < 1e-4
+    assert _rel(s.velocities(), ref.velocities()) < 2e-3
+    # cached per scene signature
+    again, t2 = autotune.tune(s, frames=2)
+    assert again == best and t2 == {}
 
 
 def test_graph_replay_equals_eager_launches():

@@ -37,6 +37,7 @@ class ParticlesDesc(ctypes.Structure):
         ("dyn_rows", c_void_p), ("dyn_lam", c_void_p), ("dyn_count", c_void_p),
         ("dyn_cap", c_int), ("gscale", c_void_p),
         ("ws_theta", c_void_p), ("ws_feat", c_void_p), ("ws_y", c_void_p),
+        ("fuse_first", c_int),
     ]
 
 
@@ -75,6 +76,8 @@ _SIGS = {
                                   c_void_p, c_float, c_int, c_int, c_int, c_int]),
     "xpbd_jacobi_tile": (c_int, [c_void_p] * 12 + [c_float, c_int, c_int, c_int, c_float, c_int,
                                                     c_float, c_float, c_void_p]),
+    "xpbd_jacobi_tile_first": (c_int, [c_void_p] * 11 + [c_float, c_float, c_float, c_float,
+                                                         c_void_p, c_int, c_int, c_float]),
     "xpbd_integrate_to": (c_int, [c_void_p] * 5 + [c_float, c_float, c_float, c_float, c_void_p,
                                                    c_int]),
     "xpbd_solve_jacobi_dyn": (c_int, [c_void_p] * 6 + [c_float, c_int, c_void_p, c_int]),

@@ -4,9 +4,10 @@ For a flushed ``Solver`` in Jacobi mode it times whole frames (hipGraph
 replays, HIP events on the launch stream) under each candidate launch shape
 and keeps the fastest:
 
-* ``tile``   — the fused per-iteration kernel with tile sizes of 64 ... 2048
-  particles (28 B of LDS per particle; larger tiles re-project fewer halo
-  rows, smaller tiles spread the launch over more CUs);
+* ``tile``   — the fused per-iteration kernel with tile sizes of 64 ... 1024
+  particles (44 B of LDS per particle; larger tiles re-project fewer halo
+  rows, smaller tiles spread the launch over more CUs), each with integrate
+  folded into the first iteration or run as its own launch;
 * ``window`` — projection with an LDS window + global atomics and a separate
   apply pass, with 1 or 2 rows per thread.
 
@@ -25,18 +26,20 @@ def _signature(s):
 
 def candidates(n_particles: int):
     out = []
-    for t in (64, 128, 256, 512, 1024, 2048):
+    for t in (64, 128, 256, 512, 1024):
         if t <= max(64, 2 * n_particles):
-            out.append(("tile", t, None))
-    out.append(("window", 0, (1, None)))
-    out.append(("window", 0, (2, None)))
+            out.append(("tile", t, None, True))
+            out.append(("tile", t, None, False))
+    out.append(("window", 0, (1, None), False))
+    out.append(("window", 0, (2, None), False))
     return out
 
 
 def _apply(s, cand):
-    kind, tile, tiling = cand
+    kind, tile, tiling, fuse = cand
     s.jacobi_kernel = kind
     s.tile_size = tile
+    s.fuse_first = fuse
     s._dirty = True
     s._flush()
     if kind == "window" and tiling is not None:

@@ -45,24 +45,55 @@ struct Frame3 {
   hipStream_t capture_stream = nullptr;
 };
 
-// Fused-tile Jacobi frame: per substep one integrate pass (X -> X or X_alt,
-// chosen so that after `iterations` ping-pong launches the state is back in
-// X) and one k_jacobi_tile launch per iteration, the last one also running
-// finalize + cap.
+// Fused-tile Jacobi frame.
+//
+// With fuse_first, two or more iterations and no device-counted rows, each
+// substep is `iterations` k_jacobi_tile launches only: the first has
+// integrate folded in (xpbd_jacobi_tile_first), the last runs finalize + cap.  Launches
+// ping-pong between X and X_alt; `loc` tracks which one holds the state, and
+// a frame that ends in X_alt copies it back.
+//
+// Otherwise (one iteration, or contact rows projected by a separate launch
+// before each iteration) a substep is one integrate pass (X -> X or X_alt,
+// chosen so that after `iterations` launches the state is back in X) and the
+// plain launches.
 int issue_frame_tile(Frame3* f, hipStream_t s) {
   const xpbd_particles_desc& d = f->d;
   const int n = d.n_particles;
   if (n <= 0) return 0;
   const bool have_rows = (d.n_rows > 0 || d.dyn_cap > 0) && d.iterations > 0;
   const int it_n = have_rows ? d.iterations : 0;
+  const bool dyn = d.dyn_cap > 0 && d.dyn_rows && d.dyn_count;
   float* bufs[2] = {d.x, d.x_alt};
   float* lams[2] = {d.lam, d.lam_alt};
   int rc;
+  if (d.fuse_first && it_n >= 2 && !dyn && d.n_rows > 0) {
+    int loc = 0;
+    for (int sub = 0; sub < d.substeps; ++sub) {
+      rc = xpbd_jacobi_tile_first(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
+                                  lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.h,
+                                  d.gx, d.gy, d.gz, d.gscale, d.tile_size, n, d.relax);
+      if (rc) return rc;
+      loc ^= 1;
+      for (int it = 1; it < it_n; ++it) {
+        rc = xpbd_jacobi_tile(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
+                              lams[it & 1], lams[(it + 1) & 1], d.tile_row_start, d.halo_start,
+                              d.halo_rows, d.h, 0, d.tile_size, n, d.relax, it + 1 == it_n,
+                              d.damping, d.max_speed, nullptr);
+        if (rc) return rc;
+        loc ^= 1;
+      }
+    }
+    if (loc) {
+      XPBD_CHECK(hipMemcpyAsync(d.x, d.x_alt, sizeof(float) * 4 * (size_t)n,
+                                hipMemcpyDeviceToDevice, s));
+    }
+    return 0;
+  }
   for (int sub = 0; sub < d.substeps; ++sub) {
     int cur = it_n & 1;
     rc = xpbd_integrate_to(s, d.x, bufs[cur], d.v, d.x_prev, d.gx, d.gy, d.gz, d.h, d.gscale, n);
     if (rc) return rc;
-    const bool dyn = d.dyn_cap > 0 && d.dyn_rows && d.dyn_count;
     for (int it = 0; it < it_n; ++it) {
       const int nxt = cur ^ 1;
       if (dyn) {

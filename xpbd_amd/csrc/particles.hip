@@ -42,8 +42,12 @@ struct Proj {
   float lam;
 };
 
-__device__ inline Proj project_jacobi(const float4* __restrict__ X, const float4* __restrict__ XP,
-                                      int4 row, float4 anchor_or_zero, float lam, float inv_dt2) {
+// `pos(i)` returns the (x, y, z, inverse mass) the launch sees for particle
+// i, `prev(i)` its x_prev and `anchor()` the row's ATTACH anchor; the loads
+// happen only on the branches that use them.
+template <class Pos, class Prev, class Anchor>
+__device__ inline Proj project_row(Pos pos, Prev prev, Anchor anchor, int4 row, float lam,
+                                   float inv_dt2) {
   Proj p;
   p.na = 0;
   p.upd = false;
@@ -56,14 +60,14 @@ __device__ inline Proj project_jacobi(const float4* __restrict__ X, const float4
   p.a = a;
   p.b = bb;
   if (t == T_DISTANCE || t == T_ATTACH) {
-    float4 xa4 = X[a];
+    float4 xa4 = pos(a);
     float wa = xa4.w, wb;
     f3 pb;
     if (t == T_ATTACH) {
-      pb = xyz(anchor_or_zero);
+      pb = xyz(anchor());
       wb = 0.0f;
     } else {
-      float4 xb4 = X[bb];
+      float4 xb4 = pos(bb);
       pb = xyz(xb4);
       wb = xb4.w;
     }
@@ -87,7 +91,7 @@ __device__ inline Proj project_jacobi(const float4* __restrict__ X, const float4
     return p;
   }
   if (t == T_FLOOR) {
-    float4 xa4 = X[a];
+    float4 xa4 = pos(a);
     float wa = xa4.w;
     if (wa == 0.0f) return p;
     float C = xa4.y - p0;
@@ -98,7 +102,7 @@ __device__ inline Proj project_jacobi(const float4* __restrict__ X, const float4
     f3 corr = mk3(0.0f, wa * dlam, 0.0f);
     float mu = p1;
     if (mu > 0.0f) {
-      f3 dp_rel = (xyz(xa4) + corr) - xyz(XP[a]);
+      f3 dp_rel = (xyz(xa4) + corr) - prev(a);
       corr = corr + friction_delta(dp_rel, mk3(0.0f, 1.0f, 0.0f), -C, mu);
     }
     p.ca = corr;
@@ -106,7 +110,7 @@ __device__ inline Proj project_jacobi(const float4* __restrict__ X, const float4
     return p;
   }
   if (t == T_CONTACT) {
-    float4 xa4 = X[a], xb4 = X[bb];
+    float4 xa4 = pos(a), xb4 = pos(bb);
     float wa = xa4.w, wb = xb4.w;
     float wsum = wa + wb;
     if (wsum == 0.0f) return p;
@@ -123,7 +127,7 @@ __device__ inline Proj project_jacobi(const float4* __restrict__ X, const float4
     f3 cb = (-n_hat) * (wb * dlam);
     float mu = p1;
     if (mu > 0.0f) {
-      f3 dp_rel = ((xyz(xa4) + ca) - xyz(XP[a])) - ((xyz(xb4) + cb) - xyz(XP[bb]));
+      f3 dp_rel = ((xyz(xa4) + ca) - prev(a)) - ((xyz(xb4) + cb) - prev(bb));
       f3 fcorr = friction_delta(dp_rel, n_hat, -C, mu);
       ca = ca + fcorr * (wa / wsum);
       cb = cb - fcorr * (wb / wsum);
@@ -134,6 +138,12 @@ __device__ inline Proj project_jacobi(const float4* __restrict__ X, const float4
     return p;
   }
   return p;
+}
+
+__device__ inline Proj project_jacobi(const float4* __restrict__ X, const float4* __restrict__ XP,
+                                      int4 row, float4 anchor_or_zero, float lam, float inv_dt2) {
+  return project_row([&](int i) { return X[i]; }, [&](int i) { return xyz(XP[i]); },
+                     [&]() { return anchor_or_zero; }, row, lam, inv_dt2);
 }
 
 // ---------------------------------------------------------------------------
@@ -314,66 +324,127 @@ __device__ inline float unfix(unsigned long long v) {
   return (float)((double)(long long)v * kFixInv);
 }
 
-// LDS layout of a tile accumulator: x, y, z sums (int64, P each), count (u32, P).
-inline size_t tile_lds_bytes(int P) { return (size_t)P * (3 * sizeof(long long) + sizeof(unsigned)); }
+// LDS of a tile: x, y, z sums (int64, P each), count (u32, P), then the
+// tile's positions (float4, P) at a 16-byte aligned offset.
+__host__ __device__ inline size_t tile_lds_pos_offset(int P) {
+  return ((size_t)P * (3 * sizeof(long long) + sizeof(unsigned)) + 15) & ~(size_t)15;
+}
+inline size_t tile_lds_bytes(int P) { return tile_lds_pos_offset(P) + (size_t)P * sizeof(float4); }
 
-template <int BLOCK>
+//
+// PREDICT: the first iteration of a substep with `integrate` (kernels.py:49-73)
+// folded in.  Xin then holds the positions at the start of the substep; every
+// position the launch reads is predicted on the fly, x~ = x + (v + g h) h,
+// from Xin and V (the same arithmetic as k_integrate_to, so the result is
+// identical to an integrate launch followed by the plain iteration), x_prev
+// is written for the tile's own particles, and lambda starts from zero.  V
+// is left as it was: finalize, in a later launch of the same substep,
+// rewrites it, so a predict launch must never also finalize.
+//
+// The launch mode is a template argument (bits of TileMode) so that a
+// launch pays only for what it uses: every optional branch left in the
+// kernel costs registers and scheduling freedom in the row loop (a runtime
+// finalize / ACC branch alone measured ~10 % on 8 M-particle cloths).
+enum TileMode : int { TM_PREDICT = 1, TM_FINALIZE = 2, TM_ACC = 4, TM_ANCHOR = 8 };
+
+template <int BLOCK, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
-    const float4* __restrict__ Xin, float4* __restrict__ Xout, const float4* __restrict__ XP,
+    const float4* __restrict__ Xin, float4* __restrict__ Xout, float4* __restrict__ XP,
     float4* __restrict__ V, const int4* __restrict__ R, const float4* __restrict__ ANC,
     const float* __restrict__ LAMin, float* __restrict__ LAMout,
     const int* __restrict__ tile_row_start, const int* __restrict__ halo_start,
     const int* __restrict__ halo_rows, float h, int first, int P, int n, float relax,
-    int finalize, float damping, float max_speed, int do_cap, float4* __restrict__ ACC) {
+    int finalize, float damping, float max_speed, int do_cap, float4* __restrict__ ACC, float gx,
+    float gy, float gz, const float* __restrict__ gscale) {
+  constexpr bool PREDICT = MODE & TM_PREDICT, FINALIZE = MODE & TM_FINALIZE;
+  constexpr bool HAS_ACC = MODE & TM_ACC, HAS_ANCHOR = MODE & TM_ANCHOR;
   extern __shared__ unsigned long long lds64[];
   unsigned long long* sx = lds64;
   unsigned long long* sy = lds64 + P;
   unsigned long long* sz = lds64 + 2 * P;
   unsigned* cnt = (unsigned*)(lds64 + 3 * P);
+  float4* spos = (float4*)(lds64 + tile_lds_pos_offset(P) / sizeof(unsigned long long));
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int p0 = tile * P;
   if (p0 >= n) return;
   const int p1 = min(p0 + P, n);
+  const float inv_dt2 = 1.0f / (h * h);
+  auto predict = [&](float4 x4, int i) -> float4 {
+    if (!PREDICT || x4.w == 0.0f) return x4;
+    f3 g = mk3(gx, gy, gz);
+    if (gscale) g = g * gscale[i];
+    const f3 v = xyz(V[i]) + g * h;
+    const f3 x = xyz(x4) + v * h;
+    return make_float4(x.x, x.y, x.z, x4.w);
+  };
   for (int k = threadIdx.x; k < 3 * P; k += BLOCK) lds64[k] = 0ull;
   for (int k = threadIdx.x; k < P; k += BLOCK) cnt[k] = 0u;
+  // the tile's own positions (predicted in a PREDICT launch) go to LDS once;
+  // rows read endpoints of this tile from there and others from HBM / L2
+  for (int i = p0 + threadIdx.x; i < p1; i += BLOCK) {
+    const float4 x0 = Xin[i];
+    spos[i - p0] = predict(x0, i);
+    if (PREDICT) XP[i] = make_float4(x0.x, x0.y, x0.z, 0.0f);
+  }
   __syncthreads();
-  const float inv_dt2 = 1.0f / (h * h);
+  auto pos = [&](int i) -> float4 {
+    const unsigned off = (unsigned)(i - p0);
+    if (off < (unsigned)(p1 - p0)) return spos[off];
+    return predict(Xin[i], i);
+  };
+  auto prev = [&](int i) { return PREDICT ? xyz(Xin[i]) : xyz(XP[i]); };
   auto add = [&](int i, f3 c) {
     const unsigned off = (unsigned)(i - p0);
     if (off >= (unsigned)P) return;
+    // branch-free: out-of-range or NaN components are clamped for the
+    // conversion and flag the particle through the count
     const bool ok = fabsf(c.x) <= kFixLimit && fabsf(c.y) <= kFixLimit && fabsf(c.z) <= kFixLimit;
-    if (ok) {
-      atomicAdd(&sx[off], fix_of(c.x));
-      atomicAdd(&sy[off], fix_of(c.y));
-      atomicAdd(&sz[off], fix_of(c.z));
-    }
+    atomicAdd(&sx[off], fix_of(fminf(fmaxf(c.x, -kFixLimit), kFixLimit)));
+    atomicAdd(&sy[off], fix_of(fminf(fmaxf(c.y, -kFixLimit), kFixLimit)));
+    atomicAdd(&sz[off], fix_of(fminf(fmaxf(c.z, -kFixLimit), kFixLimit)));
     atomicAdd(&cnt[off], ok ? 1u : (kFixPoison | 1u));
   };
   const int rb = tile_row_start[tile], re = tile_row_start[tile + 1];
   const int hb = halo_start[tile], he = halo_start[tile + 1];
   const int owned = re - rb;
   const int total = owned + (he - hb);
-  for (int k = threadIdx.x; k < total; k += BLOCK) {
-    const bool own = k < owned;
-    const int j = own ? rb + k : halo_rows[hb + (k - owned)];
-    const int4 row = R[j];
-    const float lam = first ? 0.0f : LAMin[j];
-    float4 anc = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (row_type(row) == T_ATTACH) anc = ANC[j];
-    const Proj p = project_jacobi(Xin, XP, row, anc, lam, inv_dt2);
-    if (own) LAMout[j] = p.lam;
-    if (p.na >= 1) add(p.a, p.ca);
-    if (p.na == 2) add(p.b, p.cb);
+  // two rows per thread per pass, their loads issued together
+  constexpr int U = 2;
+  for (int k0 = threadIdx.x; k0 < total; k0 += BLOCK * U) {
+    int4 row[U];
+    float lam[U];
+    int jj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * BLOCK;
+      jj[u] = -1;
+      if (k < total) {
+        jj[u] = k < owned ? rb + k : halo_rows[hb + (k - owned)];
+        row[u] = R[jj[u]];
+        lam[u] = (PREDICT || first) ? 0.0f : LAMin[jj[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (jj[u] < 0) continue;
+      const int j = jj[u];
+      const Proj p = project_row(
+          pos, prev, [&]() { return HAS_ANCHOR ? ANC[j] : make_float4(0.f, 0.f, 0.f, 0.f); },
+          row[u], lam[u], inv_dt2);
+      if (k0 + u * BLOCK < owned) LAMout[jj[u]] = p.lam;
+      if (p.na >= 1) add(p.a, p.ca);
+      if (p.na == 2) add(p.b, p.cb);
+    }
   }
   __syncthreads();
   for (int i = p0 + threadIdx.x; i < p1; i += BLOCK) {
     const int o = i - p0;
-    const float4 x4 = Xin[i];
+    const float4 x4 = spos[o];
     const unsigned c = cnt[o];
     float4 a = make_float4(0.f, 0.f, 0.f, (float)(c & (kFixPoison - 1u)));
     if (c) a = make_float4(unfix(sx[o]), unfix(sy[o]), unfix(sz[o]), a.w);
     if (c & kFixPoison) a.x = __int_as_float(0x7fc00000);
-    if (ACC) {  // corrections of the device-counted row segment
+    if (HAS_ACC) {  // corrections of the device-counted row segment
       const float4 g = ACC[i];
       if (g.w != 0.0f) {
         a = make_float4(a.x + g.x, a.y + g.y, a.z + g.z, a.w + g.w);
@@ -382,7 +453,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     }
     f3 x = xyz(x4);
     if (a.w > 0.0f && x4.w > 0.0f) x = x + xyz(a) * (relax / a.w);
-    if (finalize) {
+    if (FINALIZE) {
       f3 v;
       if (x4.w == 0.0f) {
         v = mk3(0.0f, 0.0f, 0.0f);
@@ -594,6 +665,80 @@ constexpr int kTileBlock = 512;
 
 inline int blocks_for(long long n, int per_block) { return (int)((n + per_block - 1) / per_block); }
 
+template <int BLOCK, int MODE>
+void launch_tile_mode(hipStream_t s, int nb, size_t shmem, const float* Xin, float* Xout,
+                      float* XP, float* V, const int* R, const float* ANC, const float* LAMin,
+                      float* LAMout, const int* trs, const int* hs, const int* hr, float h,
+                      int first, int tile_size, int n, float relax, float damping,
+                      float max_speed, int do_cap, float* ACC, float gx, float gy, float gz,
+                      const float* gscale) {
+  hipLaunchKernelGGL((k_jacobi_tile<BLOCK, MODE>), dim3(nb), dim3(BLOCK), shmem, s,
+                     (const float4*)Xin, (float4*)Xout, (float4*)XP, (float4*)V, (const int4*)R,
+                     (const float4*)ANC, LAMin, LAMout, trs, hs, hr, h, first, tile_size, n, relax,
+                     (MODE & TM_FINALIZE) ? 1 : 0, damping, max_speed, do_cap, (float4*)ACC, gx,
+                     gy, gz, gscale);
+}
+
+template <int BLOCK>
+int launch_tile_block(int mode, hipStream_t s, int nb, size_t shmem, const float* Xin,
+                      float* Xout, float* XP, float* V, const int* R, const float* ANC,
+                      const float* LAMin, float* LAMout, const int* trs, const int* hs,
+                      const int* hr, float h, int first, int tile_size, int n, float relax,
+                      float damping, float max_speed, int do_cap, float* ACC, float gx, float gy,
+                      float gz, const float* gscale) {
+#define XPBD_TILE_MODE(M)                                                                      \
+  case M:                                                                                      \
+    launch_tile_mode<BLOCK, M>(s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin, LAMout, trs, hs, \
+                               hr, h, first, tile_size, n, relax, damping, max_speed, do_cap,  \
+                               ACC, gx, gy, gz, gscale);                                      \
+    return 0;
+  switch (mode) {
+    XPBD_TILE_MODE(0)
+    XPBD_TILE_MODE(TM_PREDICT)
+    XPBD_TILE_MODE(TM_FINALIZE)
+    XPBD_TILE_MODE(TM_ACC)
+    XPBD_TILE_MODE(TM_FINALIZE | TM_ACC)
+    XPBD_TILE_MODE(TM_ANCHOR)
+    XPBD_TILE_MODE(TM_PREDICT | TM_ANCHOR)
+    XPBD_TILE_MODE(TM_FINALIZE | TM_ANCHOR)
+    XPBD_TILE_MODE(TM_ACC | TM_ANCHOR)
+    XPBD_TILE_MODE(TM_FINALIZE | TM_ACC | TM_ANCHOR)
+  }
+#undef XPBD_TILE_MODE
+  return (int)hipErrorInvalidValue;  // predict never combines with finalize or ACC
+}
+
+// ANC may be NULL when no row is ATTACH, ACC when no device-counted rows
+// join the iteration.
+int launch_tile(bool predict, hipStream_t s, const float* Xin, float* Xout, float* XP, float* V,
+                const int* R, const float* ANC, const float* LAMin, float* LAMout,
+                const int* tile_row_start, const int* halo_start, const int* halo_rows, float h,
+                int first, int tile_size, int n, float relax, int finalize, float damping,
+                float max_speed, float* ACC, float gx, float gy, float gz, const float* gscale) {
+  if (n <= 0) return 0;
+  if (tile_size < 1 || tile_size > XPBD_MAX_TILE) return (int)hipErrorInvalidValue;
+  const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
+  const int nb = blocks_for(n, tile_size);
+  const size_t shmem = tile_lds_bytes(tile_size);
+  const int mode = (predict ? TM_PREDICT : 0) | (finalize ? TM_FINALIZE : 0) |
+                   (ACC ? TM_ACC : 0) | (ANC ? TM_ANCHOR : 0);
+  int rc;
+  // small tiles (small scenes, many blocks) run 256-thread blocks
+  if (tile_size <= 256)
+    rc = launch_tile_block<kBlock>(mode, s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin, LAMout,
+                                   tile_row_start, halo_start, halo_rows, h, first, tile_size, n,
+                                   relax, damping, max_speed, do_cap, ACC, gx, gy, gz, gscale);
+  else
+    rc = launch_tile_block<kTileBlock>(mode, s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin,
+                                       LAMout, tile_row_start, halo_start, halo_rows, h, first,
+                                       tile_size, n, relax, damping, max_speed, do_cap, ACC, gx,
+                                       gy, gz, gscale);
+  if (rc) return rc;
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+
 }  // namespace
 
 // ===========================================================================
@@ -660,26 +805,22 @@ int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* X
                      const int* tile_row_start, const int* halo_start, const int* halo_rows,
                      float h, int first, int tile_size, int n_particles, float relax,
                      int finalize, float damping, float max_speed, float* ACC) {
-  if (n_particles <= 0) return 0;
-  if (tile_size < 1 || tile_size > XPBD_MAX_TILE) return (int)hipErrorInvalidValue;
-  const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
-  const int nb = blocks_for(n_particles, tile_size);
-  const size_t shmem = tile_lds_bytes(tile_size);
-  // small tiles (small scenes, many blocks) run 256-thread blocks
-  if (tile_size <= 256)
-    hipLaunchKernelGGL(k_jacobi_tile<kBlock>, dim3(nb), dim3(kBlock), shmem, (hipStream_t)stream,
-                       (const float4*)Xin, (float4*)Xout, (const float4*)XP, (float4*)V,
-                       (const int4*)R, (const float4*)ANC, LAMin, LAMout, tile_row_start,
-                       halo_start, halo_rows, h, first, tile_size, n_particles, relax, finalize,
-                       damping, max_speed, do_cap, (float4*)ACC);
-  else
-    hipLaunchKernelGGL(k_jacobi_tile<kTileBlock>, dim3(nb), dim3(kTileBlock), shmem,
-                       (hipStream_t)stream, (const float4*)Xin, (float4*)Xout, (const float4*)XP,
-                       (float4*)V, (const int4*)R, (const float4*)ANC, LAMin, LAMout,
-                       tile_row_start, halo_start, halo_rows, h, first, tile_size, n_particles,
-                       relax, finalize, damping, max_speed, do_cap, (float4*)ACC);
-  XPBD_LAUNCH_CHECK();
-  return 0;
+  // the plain launch only reads XP (friction, finalize)
+  return launch_tile(false, (hipStream_t)stream, Xin, Xout, const_cast<float*>(XP), V, R, ANC,
+                     LAMin, LAMout, tile_row_start, halo_start, halo_rows, h, first, tile_size,
+                     n_particles, relax, finalize, damping, max_speed, ACC, 0.0f, 0.0f, 0.0f,
+                     nullptr);
+}
+
+int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP, const float* V,
+                           const int* R, const float* ANC, float* LAMout,
+                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
+                           float h, float gx, float gy, float gz, const float* gscale,
+                           int tile_size, int n_particles, float relax) {
+  if (X == Xout) return (int)hipErrorInvalidValue;  // neighbours read X while tiles write
+  return launch_tile(true, (hipStream_t)stream, X, Xout, XP, const_cast<float*>(V), R, ANC,
+                     nullptr, LAMout, tile_row_start, halo_start, halo_rows, h, 1, tile_size,
+                     n_particles, relax, 0, 0.0f, 0.0f, nullptr, gx, gy, gz, gscale);
 }
 
 int xpbd_integrate_to(void* stream, const float* Xin, float* Xout, float* V, float* XP, float gx,

@@ -133,7 +133,7 @@ def _part1by1(v):
 
 def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing: float = 0.01,
                         mass: float = 1e-3, height: float = 0.5, floor_mu: float = 0.4,
-                        compliance_range=(1e-8, 1e-3), tile: int = 1024):
+                        compliance_range=(1e-8, 1e-3), tile: int = 512):
     """The config-2 cloth generator scaled to ``n_sheets`` sheets of res^2,
     built directly on the device in the packed layout of include/xpbd.h, for
     the HBM-roofline scene (8 x 1024^2 = 58.6M rows), where host-side scene

@@ -39,9 +39,10 @@ MODE_JACOBI_TILE = 2
 def default_tile_size(n_particles: int) -> int:
     """Particles per tile of the fused Jacobi kernel: large tiles keep the
     share of re-projected halo rows small; small scenes get smaller tiles so
-    that the launch still spreads over the 256 CUs.  1024 is the largest
-    default: at 2048 (56 KB of LDS per block) fewer blocks fit on a CU."""
-    for t in (1024, 512, 256, 128):
+    that the launch still spreads over the 256 CUs.  512 is the largest
+    default: on 8 M-particle cloths it beats 1024 (22 KB against 45 KB of LDS
+    per block, so more blocks share a CU)."""
+    for t in (512, 256, 128):
         if n_particles >= 256 * t:
             return t
     return 64
@@ -264,6 +265,7 @@ class Solver:
         self.jacobi_kernel = "tile"
         self.tile_size = 0            # 0 = chosen from the scene size
         self.reorder = True           # Z-order particle layout for Jacobi
+        self.fuse_first = True        # integrate folded into the first tile launch
         # device-side CONTACT generation (Jacobi tile mode): an extension
         # that replaces the per-frame host broad phase; see enable_device_contacts
         self._device_contacts = False
@@ -640,6 +642,7 @@ class Solver:
         self.ACC = torch.zeros((max(n_b, 1), 4), dtype=torch.float32, device=dev)
         self.R = torch.from_numpy(rows).to(dev)
         self.ANC = torch.from_numpy(anc).to(dev)
+        self._has_attach = bool(np.any(h["type"] == ATTACH)) if n_c else False
         self.LAM = torch.zeros(max(n_c, 1), dtype=torch.float32, device=dev)
         if self._tile_tabs is not None:
             self.X_B = self.X.clone()
@@ -730,7 +733,7 @@ class Solver:
     def _signature(self):
         return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
                 self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling,
-                self._tile, getattr(self, "_ccap", 0))
+                self._tile, getattr(self, "_ccap", 0), bool(self.fuse_first))
 
     def _drop_plan(self):
         if self._plan is not None:
@@ -749,6 +752,8 @@ class Solver:
         d.x, d.v, d.x_prev, d.acc = (N.ptr(self.X), N.ptr(self.V), N.ptr(self.XP),
                                      N.ptr(self.ACC))
         d.rows, d.anchors, d.lam = N.ptr(self.R), N.ptr(self.ANC), N.ptr(self.LAM)
+        if self._tile_dev is not None and not self._has_attach:
+            d.anchors = None        # the tile kernel variant without anchor loads
         d.n_particles, d.n_rows = n_b, n_c
         self._starts_buf = np.ascontiguousarray(self._color_starts, dtype=np.int32)
         d.color_starts = self._starts_buf.ctypes.data
@@ -761,6 +766,7 @@ class Solver:
             d.tile_size = self._tile
             if self.GS is not None:
                 d.gscale = N.ptr(self.GS)
+            d.fuse_first = int(bool(self.fuse_first))
             if self.CROWS is not None:
                 d.dyn_rows, d.dyn_lam = N.ptr(self.CROWS), N.ptr(self.CLAM)
                 d.dyn_count, d.dyn_cap = N.ptr(self.CCOUNT), self._ccap

@@ -45,7 +45,8 @@ struct Proj {
 // `pos(i)` returns the (x, y, z, inverse mass) the launch sees for particle
 // i, `prev(i)` its x_prev and `anchor()` the row's ATTACH anchor; the loads
 // happen only on the branches that use them.
-template <class Pos, class Prev, class Anchor>
+// FAST_NORMAL: n = d * (1 / L) instead of d / L (one ulp apart at most).
+template <bool FAST_NORMAL = false, class Pos, class Prev, class Anchor>
 __device__ inline Proj project_row(Pos pos, Prev prev, Anchor anchor, int4 row, float lam,
                                    float inv_dt2) {
   Proj p;
@@ -76,7 +77,7 @@ __device__ inline Proj project_row(Pos pos, Prev prev, Anchor anchor, int4 row, 
     f3 d = xyz(xa4) - pb;
     float L = length(d);
     if (L < XPBD_EPS) return p;
-    f3 n_hat = d / L;
+    f3 n_hat = FAST_NORMAL ? d * (1.0f / L) : d / L;
     float C = L - p0;
     float alpha = p1 * inv_dt2;
     float dlam = (-C - alpha * lam) / (wsum + alpha);
@@ -302,9 +303,18 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_project_dyn(
 // point with 64-bit integer atomics (plus a 32-bit count).  Integer sums are
 // exact and independent of the order in which lanes arrive, so an iteration
 // is bitwise reproducible from run to run; 64-bit integer LDS atomics also
-// run several times faster than 32-bit float LDS atomics on CDNA.  Range:
-// |correction| <= 2^22 m per row; a larger or non-finite correction marks the
+// run several times faster than 32-bit float LDS atomics on CDNA.
+// Conversion in two instructions: (double)c + 6144.0 lies in [4096, 8192),
+// a binade whose ulp is 2^-40, so the add rounds c to the nearest multiple
+// of 2^-40 (ties to even) and the bit pattern of the sum is
+// bits(6144.0) + round(c * 2^40).  Those bit patterns are added (modulo
+// 2^64) and the count x bits(6144.0) is subtracted at apply.  Range:
+// |correction| < 2^11 m per row; a larger or non-finite correction marks the
 // particle, whose position then becomes NaN (as a float sum would).
+//
+// The unit normal of the tile kernel's rows is d * (1 / L) rather than the
+// reference's three divides d / L (one ulp apart at most); C, the lambda
+// update and the correction magnitude keep the reference's arithmetic.
 //
 // After a barrier every thread applies the averaged correction to its tile's
 // particles and writes X_out (ping-pong buffers), so one launch is one
@@ -312,16 +322,18 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_project_dyn(
 // substep the same pass also runs finalize_velocity and cap_velocity
 // (kernels.py:346-371).
 // ---------------------------------------------------------------------------
-constexpr float kFixScale = 1099511627776.0f;          // 2^40 units per metre
-constexpr double kFixInv = 1.0 / 1099511627776.0;
-constexpr float kFixLimit = 4194304.0f;                // 2^22 m
-constexpr unsigned kFixPoison = 1u << 24;              // count flag: bad correction
+constexpr double kFixBias = 6144.0;                    // 1.5 * 2^12, ulp 2^-40
+constexpr unsigned long long kFixBiasBits = 0x40B8000000000000ull;  // bits of 6144.0
+constexpr double kFixInv = 1.0 / 1099511627776.0;      // 2^-40
+constexpr float kFixLimit = 2047.0f;                   // |correction| per row, metres
+constexpr unsigned kFixPoison = 1u << 24;              // count unit of a bad correction
 
-__device__ inline unsigned long long fix_of(float c) {
-  return (unsigned long long)__float2ll_rn(c * kFixScale);
+__device__ inline unsigned long long fix_bits(float c) {
+  return (unsigned long long)__double_as_longlong((double)c + kFixBias);
 }
-__device__ inline float unfix(unsigned long long v) {
-  return (float)((double)(long long)v * kFixInv);
+// Sum of `adds` fix_bits values -> metres.
+__device__ inline float unfix(unsigned long long s, unsigned adds) {
+  return (float)((double)(long long)(s - (unsigned long long)adds * kFixBiasBits) * kFixInv);
 }
 
 // LDS of a tile: x, y, z sums (int64, P each), count (u32, P), then the
@@ -379,6 +391,27 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
   };
   for (int k = threadIdx.x; k < 3 * P; k += BLOCK) lds64[k] = 0ull;
   for (int k = threadIdx.x; k < P; k += BLOCK) cnt[k] = 0u;
+  const int rb = tile_row_start[tile], re = tile_row_start[tile + 1];
+  const int hb = halo_start[tile], he = halo_start[tile + 1];
+  const int owned = re - rb;
+  const int total = owned + (he - hb);
+  constexpr int U = 2;
+  int4 row[U];
+  float lam[U];
+  int jj[U];
+  auto load_pass = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * BLOCK;
+      jj[u] = -1;
+      if (k < total) {
+        jj[u] = k < owned ? rb + k : halo_rows[hb + (k - owned)];
+        row[u] = R[jj[u]];
+        lam[u] = (PREDICT || first) ? 0.0f : LAMin[jj[u]];
+      }
+    }
+  };
+  load_pass(threadIdx.x);
   // the tile's own positions (predicted in a PREDICT launch) go to LDS once;
   // rows read endpoints of this tile from there and others from HBM / L2
   for (int i = p0 + threadIdx.x; i < p1; i += BLOCK) {
@@ -399,36 +432,21 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     // branch-free: out-of-range or NaN components are clamped for the
     // conversion and flag the particle through the count
     const bool ok = fabsf(c.x) <= kFixLimit && fabsf(c.y) <= kFixLimit && fabsf(c.z) <= kFixLimit;
-    atomicAdd(&sx[off], fix_of(fminf(fmaxf(c.x, -kFixLimit), kFixLimit)));
-    atomicAdd(&sy[off], fix_of(fminf(fmaxf(c.y, -kFixLimit), kFixLimit)));
-    atomicAdd(&sz[off], fix_of(fminf(fmaxf(c.z, -kFixLimit), kFixLimit)));
+    atomicAdd(&sx[off], fix_bits(fminf(fmaxf(c.x, -kFixLimit), kFixLimit)));
+    atomicAdd(&sy[off], fix_bits(fminf(fmaxf(c.y, -kFixLimit), kFixLimit)));
+    atomicAdd(&sz[off], fix_bits(fminf(fmaxf(c.z, -kFixLimit), kFixLimit)));
     atomicAdd(&cnt[off], ok ? 1u : (kFixPoison | 1u));
   };
-  const int rb = tile_row_start[tile], re = tile_row_start[tile + 1];
-  const int hb = halo_start[tile], he = halo_start[tile + 1];
-  const int owned = re - rb;
-  const int total = owned + (he - hb);
-  // two rows per thread per pass, their loads issued together
-  constexpr int U = 2;
+  // two rows per thread per pass, their loads issued together; the loads of
+  // the first pass are issued before the staging barrier so that they
+  // overlap the position staging
   for (int k0 = threadIdx.x; k0 < total; k0 += BLOCK * U) {
-    int4 row[U];
-    float lam[U];
-    int jj[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = k0 + u * BLOCK;
-      jj[u] = -1;
-      if (k < total) {
-        jj[u] = k < owned ? rb + k : halo_rows[hb + (k - owned)];
-        row[u] = R[jj[u]];
-        lam[u] = (PREDICT || first) ? 0.0f : LAMin[jj[u]];
-      }
-    }
+    if (k0 != (int)threadIdx.x) load_pass(k0);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (jj[u] < 0) continue;
       const int j = jj[u];
-      const Proj p = project_row(
+      const Proj p = project_row<true>(
           pos, prev, [&]() { return HAS_ANCHOR ? ANC[j] : make_float4(0.f, 0.f, 0.f, 0.f); },
           row[u], lam[u], inv_dt2);
       if (k0 + u * BLOCK < owned) LAMout[jj[u]] = p.lam;
@@ -441,9 +459,10 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     const int o = i - p0;
     const float4 x4 = spos[o];
     const unsigned c = cnt[o];
-    float4 a = make_float4(0.f, 0.f, 0.f, (float)(c & (kFixPoison - 1u)));
-    if (c) a = make_float4(unfix(sx[o]), unfix(sy[o]), unfix(sz[o]), a.w);
-    if (c & kFixPoison) a.x = __int_as_float(0x7fc00000);
+    const unsigned adds = c & (kFixPoison - 1u);
+    float4 a = make_float4(0.f, 0.f, 0.f, (float)adds);
+    if (c) a = make_float4(unfix(sx[o], adds), unfix(sy[o], adds), unfix(sz[o], adds), a.w);
+    if (c >= kFixPoison) a.x = __int_as_float(0x7fc00000);
     if (HAS_ACC) {  // corrections of the device-counted row segment
       const float4 g = ACC[i];
       if (g.w != 0.0f) {

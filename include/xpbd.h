@@ -47,8 +47,7 @@ extern "C" {
 /* ---- 3-DOF particle kernels ------------------------------------------- */
 
 /* Fused per-particle pass; flags: 1 apply_jacobi, 2 finalize_velocity,
- * 4 cap_velocity, 8 integrate (applied in that order); 16 instead of 1 adds
- * the summed corrections without averaging (the warm-start position update).
+ * 4 cap_velocity, 8 integrate (applied in that order).
  * Replaces wp.launch(apply_jacobi / finalize_velocity / cap_velocity /
  * integrate) at src/xpbd3d/solver.py:465, 437, 441, 427. */
 int xpbd_particles_pass(void* stream, float* X, float* V, float* XP, float* ACC, int flags,
@@ -320,10 +319,13 @@ int xpbd_mlp_grad(void* stream, const float* theta, const float* F, const float*
 int xpbd_mlp_adam(void* stream, float* theta, float* mom, float* vel, const float* grad,
                   float lr, float beta1, float beta2, float eps, int step);
 
-/* Warm-start rows at the start of a substep (features of the MLP, float[m][12]),
- * lambda0 from the MLP output Y written to LAM with M^-1 grad C^T lambda0
- * summed into ACC, and the squared-residual sum (r = C + alpha~ lambda for
- * two-sided rows, max(0, -C) for one-sided rows) added to *out_sum. */
+/* Warm-start rows at the start of a substep (features of the MLP,
+ * float[m][12]; LAM may be NULL, the previous-substep multiplier feature is
+ * then zero), lambda0 from the MLP output Y written to LAM with w grad C
+ * lambda0 and a count of one per endpoint summed into ACC (averaged by
+ * xpbd_particles_pass flag 1, as Jacobi averages), and the squared-residual
+ * sum (r = C + alpha~ lambda for two-sided rows, max(0, -C) for one-sided
+ * rows) added to *out_sum. */
 int xpbd_ws_features(void* stream, const float* X, const float* V, const int* R, const float* ANC,
                      const float* LAM, float h, float* F, int n_rows);
 int xpbd_ws_apply(void* stream, const float* X, const float* V, const int* R, const float* ANC,

@@ -21,19 +21,21 @@
 
 namespace {
 
-enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8, F_APPLY_SUM = 16 };
+enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8 };
 
 // Learned warm start of one substep (window-Jacobi mode): features ->
 // MLP -> lambda0 into LAM and its position correction summed into ACC ->
 // applied; the iterations then continue from lambda0 (first = 0).
 int warm_start(const xpbd_particles_desc& d, hipStream_t s) {
-  int rc = xpbd_ws_features(s, d.x, d.v, d.rows, d.anchors, d.lam, d.h, d.ws_feat, d.n_rows);
+  // no previous-substep lambda feature: fed back from warm-started substeps
+  // it drives the prediction upward from substep to substep
+  int rc = xpbd_ws_features(s, d.x, d.v, d.rows, d.anchors, nullptr, d.h, d.ws_feat, d.n_rows);
   if (rc) return rc;
   rc = xpbd_mlp_forward(s, d.ws_theta, d.ws_feat, d.ws_y, d.n_rows);
   if (rc) return rc;
   rc = xpbd_ws_apply(s, d.x, d.v, d.rows, d.anchors, d.ws_y, d.h, d.lam, d.acc, d.n_rows);
   if (rc) return rc;
-  return xpbd_particles_pass(s, d.x, d.v, d.x_prev, d.acc, F_APPLY_SUM, d.relax, d.h, d.damping,
+  return xpbd_particles_pass(s, d.x, d.v, d.x_prev, d.acc, F_APPLY, d.relax, d.h, d.damping,
                              d.max_speed, d.gx, d.gy, d.gz, d.n_particles);
 }
 

@@ -616,7 +616,7 @@ __global__ __launch_bounds__(BLOCK) void k_gs_color(float4* __restrict__ X,
 // (346-358), cap_velocity (362-371) and the next substep's integrate (49-73).
 // Fusing them is exact: each is a pure per-particle map.
 // ---------------------------------------------------------------------------
-enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8, F_APPLY_SUM = 16 };
+enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8 };
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_particles(float4* __restrict__ X, float4* __restrict__ V,
@@ -636,13 +636,6 @@ __global__ __launch_bounds__(BLOCK) void k_particles(float4* __restrict__ X, flo
     float c = acc.w;
     if (c > 0.0f && w > 0.0f) {
       x = x + xyz(acc) * (relax / c);
-      x_dirty = true;
-    }
-    ACC[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  } else if (flags & F_APPLY_SUM) {  // warm start: x += sum of the corrections
-    float4 acc = ACC[i];
-    if (w > 0.0f && (acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f)) {
-      x = x + xyz(acc);
       x_dirty = true;
     }
     ACC[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -769,7 +762,7 @@ int xpbd_particles_pass(void* stream, float* X, float* V, float* XP, float* ACC,
                         float relax, float h, float damping, float max_speed, float gx, float gy,
                         float gz, int n) {
   if (n <= 0) return 0;
-  if ((flags & (F_APPLY | F_APPLY_SUM)) && ACC == nullptr) return (int)hipErrorInvalidValue;
+  if ((flags & F_APPLY) && ACC == nullptr) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_particles<kBlock>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
                      (float4*)X, (float4*)V, (float4*)XP, (float4*)ACC, flags, relax, h, damping,

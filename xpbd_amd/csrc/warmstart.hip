@@ -3,7 +3,8 @@
 //
 // At the start of a substep (after integrate), for every row j:
 //   features F[j][12]  one-hot type (4), slog(C/l), log10(w)/4,
-//                      alpha~/(w + alpha~), slog(lambda_prev (w + alpha~)/l),
+//                      alpha~/(w + alpha~), slog(lambda_prev (w + alpha~)/l)
+//                      (zero when LAM is NULL),
 //                      slog(v_n h / l), slog(|v_t| h / l), slog(pen / l),
 //                      log10(1 + alpha~)/8
 //   with w = w_a + w_b (|grad C|^2 weighted by inverse mass), alpha~ = alpha/h^2,
@@ -11,8 +12,11 @@
 // The MLP predicts y = slog(lambda* (w + alpha~) / l) (lambda* = the
 // multiplier after many plain iterations); the warm start sets
 // lambda0 = slog^-1(y) l / (w + alpha~) (>= 0 for the one-sided rows) and
-// moves the particles to x = x~ + M^-1 grad C^T lambda0 (sum over rows), the
-// state XPBD's Eq. 18 assumes for a nonzero lambda.
+// moves every particle by the average over its rows of w grad C lambda0
+// (times the relaxation), the position the Jacobi sweep itself reaches when
+// its multipliers add up to lambda0: Jacobi applies the mean, not the sum,
+// of a particle's corrections (kernels.py:326-339), so the summed offset
+// x~ + M^-1 grad C^T lambda0 of Eq. 18 would overshoot by the row count.
 #include <cmath>
 
 #include "common.h"
@@ -93,7 +97,7 @@ __global__ __launch_bounds__(kB) void k_ws_features(const float4* __restrict__ X
   f[4] = slog(g.C / kLen);
   f[5] = log10f(fmaxf(g.w, 1e-30f)) * 0.25f;
   f[6] = s > 0.0f ? g.alpha / s : 0.0f;
-  f[7] = slog(LAM[j] * s / kLen);
+  f[7] = LAM ? slog(LAM[j] * s / kLen) : 0.0f;
   f[8] = slog(g.vn * h / kLen);
   f[9] = slog(g.vt * h / kLen);
   f[10] = slog(g.pen / kLen);
@@ -104,8 +108,8 @@ __global__ __launch_bounds__(kB) void k_ws_features(const float4* __restrict__ X
   o[2] = make_float4(f[8], f[9], f[10], f[11]);
 }
 
-// y -> lambda0 per row, written to LAM, and M^-1 grad C^T lambda0 summed
-// into ACC (applied by the particle pass with F_APPLY_SUM).
+// y -> lambda0 per row, written to LAM; w grad C lambda0 and a count of one
+// per endpoint summed into ACC (averaged by the particle pass's apply).
 __global__ __launch_bounds__(kB) void k_ws_apply(const float4* __restrict__ X,
                                                  const float4* __restrict__ V,
                                                  const int4* __restrict__ R,
@@ -128,12 +132,14 @@ __global__ __launch_bounds__(kB) void k_ws_apply(const float4* __restrict__ X,
   atomicAdd(&ACC[g.a].x, ca.x);
   atomicAdd(&ACC[g.a].y, ca.y);
   atomicAdd(&ACC[g.a].z, ca.z);
+  atomicAdd(&ACC[g.a].w, 1.0f);
   if (g.t == T_DISTANCE || g.t == T_CONTACT) {
     const float wb = X[g.b].w;
     const f3 cb = g.n * (-wb * lam0);
     atomicAdd(&ACC[g.b].x, cb.x);
     atomicAdd(&ACC[g.b].y, cb.y);
     atomicAdd(&ACC[g.b].z, cb.z);
+    atomicAdd(&ACC[g.b].w, 1.0f);
   }
 }
 

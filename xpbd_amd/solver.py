@@ -270,6 +270,8 @@ class Solver:
         # that replaces the per-frame host broad phase; see enable_device_contacts
         self._device_contacts = False
         self._gscale = None           # per-particle gravity factor (API order)
+        self._ws_theta = None         # learned lambda warm start parameters
+        self._ws_gen = 0
 
         # host scene, one column per field (float64 as the reference's
         # Python floats; rounded to float32 once, at flush)
@@ -438,6 +440,30 @@ class Solver:
         """Per-particle gravity factor g_i = g * scale[i] (an extension used
         for batches of independent scenes; Jacobi tile mode only)."""
         self._gscale = np.asarray(scale, np.float32).reshape(-1).copy()
+        self._dirty = True
+
+    def set_warm_start(self, theta) -> None:
+        """Learned lambda warm start (an extension; the reference resets
+        lambda to zero every substep, solver.py:431): ``theta`` holds the
+        5057 parameters of the 12-64-64-1 network (``warmstart.train``);
+        every substep then starts from the predicted multipliers and the
+        matching position offset.  ``None`` restores the zero start.  Runs
+        the Jacobi window kernel; ``warmstart.Guard`` adds the plain-start
+        fallback."""
+        import torch
+
+        if theta is None:
+            self._ws_theta = None
+        else:
+            if self.solve_mode != "jacobi":
+                raise ValueError("the learned warm start runs with solve_mode='jacobi'")
+            t = torch.as_tensor(theta, dtype=torch.float32).to(self.device).reshape(-1).clone()
+            n = int(N.lib().xpbd_mlp_param_count())
+            if t.numel() != n:
+                raise ValueError(f"warm-start parameters: {t.numel()} values, expected {n}")
+            self._ws_theta = t
+            self.jacobi_kernel = "window"
+        self._ws_gen += 1
         self._dirty = True
 
     def enable_device_contacts(self, enabled=True, margin=0.0) -> None:
@@ -663,6 +689,10 @@ class Solver:
                 raise ValueError("gravity scale length != number of particles")
             g = self._gscale if pperm is None else self._gscale[pperm]
             self.GS = torch.from_numpy(np.ascontiguousarray(g)).to(dev)
+        self.WSF = self.WSY = None
+        if self._ws_theta is not None:
+            self.WSF = torch.empty((max(n_c, 1), 12), dtype=torch.float32, device=dev)
+            self.WSY = torch.empty(max(n_c, 1), dtype=torch.float32, device=dev)
         self.CROWS = None
         if self._device_contacts:
             rad = self._radius.array().astype(np.float32)
@@ -733,7 +763,7 @@ class Solver:
     def _signature(self):
         return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
                 self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling,
-                self._tile, getattr(self, "_ccap", 0), bool(self.fuse_first))
+                self._tile, getattr(self, "_ccap", 0), bool(self.fuse_first), self._ws_gen)
 
     def _drop_plan(self):
         if self._plan is not None:
@@ -770,6 +800,9 @@ class Solver:
             if self.CROWS is not None:
                 d.dyn_rows, d.dyn_lam = N.ptr(self.CROWS), N.ptr(self.CLAM)
                 d.dyn_count, d.dyn_cap = N.ptr(self.CCOUNT), self._ccap
+        if self._ws_theta is not None and d.mode == MODE_JACOBI:
+            d.ws_theta, d.ws_feat, d.ws_y = (N.ptr(self._ws_theta), N.ptr(self.WSF),
+                                             N.ptr(self.WSY))
         d.substeps, d.iterations = self.substeps, self.iterations
         d.h = float(np.float32(self.dt / float(self.substeps)))
         d.gx, d.gy, d.gz = self.gravity

@@ -2,19 +2,24 @@
 counterpart; it replaces the per-substep zero reset of the multipliers,
 src/xpbd3d/solver.py:431).
 
-* ``collect``  runs the plain solver and, at sampled substeps, records the
-  12 row features (kernel ``xpbd_ws_features``) and the target
-  y* = slog(lambda* (w + alpha~) / 1 mm), lambda* = the multipliers after
-  ``target_iters`` plain Jacobi iterations of that substep from lambda = 0
-  (run on a copy of the state, so the trajectory itself is the plain one).
+* ``collect``  runs the solver and, at sampled substeps, records the row
+  features (kernel ``xpbd_ws_features``) and the target
+  y* = slog(beta lambda* (w + alpha~) / 1 mm), lambda* = the multipliers
+  after ``target_iters`` plain Jacobi iterations of that substep from
+  lambda = 0 (run on a copy of the state), beta = 0.5;
+* ``fit``      two rounds of collect + train: on the plain trajectory, then on
+  the learned start's own trajectory;
 * ``train``    fits the 12-64-64-1 MLP on the GPU with the library's own BF16
-  MFMA forward/backward (``xpbd_mlp_grad``) and Adam (``xpbd_mlp_adam``).
-* ``Evaluator`` steps one substep at a time with the plain zero start or the
-  learned start (features -> MLP -> lambda0 and the matching position
-  offset -> iterations continuing from lambda0) and measures the RMS
-  constraint residual after the last iteration; the plain solver is the
-  judge, and the fallback keeps the plain result whenever the learned start
-  ends with a larger residual.
+  MFMA forward/backward (``xpbd_mlp_grad``) and Adam (``xpbd_mlp_adam``);
+* ``Evaluator`` runs the plain zero start and the learned start (features ->
+  MLP -> lambda0 and the matching position offset -> iterations continuing
+  from lambda0) on their own trajectories and measures the RMS constraint
+  residual after each substep's iterations;
+* ``Guard``    keeps the plain solver as fallback and judge in deployment.
+
+The previous-substep lambda feature slot is left at zero: measured on
+cloth, carrying it in feeds every warm-started substep's multipliers into
+the next prediction and the scene blows up within a few frames.
 """
 from __future__ import annotations
 
@@ -95,9 +100,12 @@ class _Eager:
                                              self.h, s.damping, s.max_speed, *s.gravity, self.n))
 
     def features(self, F):
+        """Row features; the previous-substep lambda slot stays zero (fed
+        back from warm-started substeps it drives the predictions upward
+        from substep to substep until the scene blows up)."""
         s = self.s
         N.check(self.lib.xpbd_ws_features(N.current_stream(), N.ptr(s.X), N.ptr(s.V), N.ptr(s.R),
-                                          N.ptr(s.ANC), N.ptr(s.LAM), self.h, N.ptr(F), self.m))
+                                          N.ptr(s.ANC), None, self.h, N.ptr(F), self.m))
 
     def warm(self, theta, F, Y):
         s = self.s
@@ -108,7 +116,7 @@ class _Eager:
                                        N.ptr(s.ANC), N.ptr(Y), self.h, N.ptr(s.LAM), N.ptr(s.ACC),
                                        self.m))
         N.check(self.lib.xpbd_particles_pass(N.current_stream(), N.ptr(s.X), N.ptr(s.V),
-                                             N.ptr(s.XP), N.ptr(s.ACC), 16, s.jacobi_relax,
+                                             N.ptr(s.XP), N.ptr(s.ACC), 1, s.jacobi_relax,
                                              self.h, s.damping, s.max_speed, *s.gravity, self.n))
 
     def residual(self, X=None, LAM=None) -> float:
@@ -131,18 +139,35 @@ def _prepare(s):
     s._flush()
 
 
-def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int | None = None):
-    """Run the plain solver for ``substeps`` substeps; every ``every``-th
-    substep append (features, target) for all rows.  Returns (F, T) device
-    tensors of shape (k m, 12) and (k m,)."""
+TARGET_SCALE = 0.5
+
+
+def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int | None = None,
+            theta=None, target_scale: float = TARGET_SCALE):
+    """Run the solver for ``substeps`` substeps; every ``every``-th substep
+    append (features, target) for all rows.  Returns (F, T) device tensors of
+    shape (k m, 12) and (k m,).
+
+    Target: slog(beta lambda* (w + alpha~) / 1 mm), lambda* = the multipliers
+    after ``target_iters`` plain iterations from zero at that substep (run on
+    a copy of the state), beta = ``target_scale``.  A full lambda* start
+    overshoots: carried from substep to substep it pumps energy into the
+    scene (measured on cloth: the reuse of the previous substep's lambda
+    explodes within a few frames, half of it matches the plain solver with
+    twice the iterations), so the network learns the damped start.
+
+    ``theta`` None: the trajectory is the plain solver's; ``theta`` given: the
+    trajectory runs the learned start (an on-policy round, whose states are
+    the ones the deployed network meets)."""
     import torch
 
     _prepare(s)
     e = _Eager(s)
     iters = s.iterations if iters is None else iters
-    scale = row_scale(s)
+    scale = row_scale(s) * float(target_scale)
     F_all, T_all = [], []
     F = torch.empty((e.m, 12), device=s.device)
+    Y = torch.empty(e.m, device=s.device)
     for k in range(substeps):
         e.integrate()
         if k % every == 0:
@@ -153,9 +178,40 @@ def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int
             e.iterate(Xc, Lc, Ac, target_iters, first=True)
             F_all.append(F.clone())
             T_all.append(_slog(Lc[:e.m] * scale / LEN))
-        e.iterate(s.X, s.LAM, s.ACC, iters, first=True)
+        if theta is None:
+            e.iterate(s.X, s.LAM, s.ACC, iters, first=True)
+        else:
+            e.warm(theta, F, Y)
+            e.iterate(s.X, s.LAM, s.ACC, iters, first=False)
         e.finalize()
     return torch.cat(F_all), torch.cat(T_all)
+
+
+def fit(make_scene, substeps: int = 120, settle_frames: int = 2, steps: int = 800,
+        batch: int = 16384, seed: int = 0, log: list | None = None):
+    """Two rounds: data from the plain solver's trajectory, training, data
+    from the learned start's own trajectory, training on both.
+    ``make_scene()`` returns a fresh Jacobi ``Solver``; returns theta."""
+    import torch
+
+    def fresh():
+        s = make_scene()
+        s.step(settle_frames)
+        return s
+
+    F, T = collect(fresh(), substeps)
+    theta, h1 = train(F, T, steps=steps, batch=batch, lr=3e-3, seed=seed, log_every=steps)
+    F2, T2 = collect(fresh(), substeps, theta=theta)
+    ok = bool(torch.isfinite(F2).all() and torch.isfinite(T2).all())
+    if ok:
+        theta, h2 = train(torch.cat([F, F2]), torch.cat([T, T2]), steps=steps, batch=batch,
+                          lr=1e-3, seed=seed + 1, theta=theta, log_every=steps)
+    else:
+        h2 = []
+    if log is not None:
+        log.append({"round1_loss": h1, "round2_loss": h2, "round2_data_finite": ok,
+                    "samples": int(F.shape[0] + (F2.shape[0] if ok else 0))})
+    return theta
 
 
 def train(F, T, steps: int = 2000, batch: int = 65536, lr: float = 3e-3, seed: int = 0,
@@ -200,10 +256,14 @@ def predict(theta, F):
 
 
 class Evaluator:
-    """Plain vs learned start on the same trajectory: at each substep both
-    start from the plain run's state; residuals after k iterations are
-    compared, and the fallback keeps the plain result when the learned one is
-    worse."""
+    """Plain zero start against the learned start, each on its own trajectory
+    from the same initial state (what a deployed solver sees, including its
+    own previous-substep lambda).  Per substep the RMS residual after the
+    iterations is recorded; ``run`` returns the RMS over the substeps per
+    configuration and, per learned iteration count k, the number of substeps
+    whose residual exceeded the plain run's at the same k and substep (the
+    substeps on which the fallback would take over).  The state is restored
+    afterwards."""
 
     def __init__(self, s, theta):
         import torch
@@ -217,36 +277,90 @@ class Evaluator:
 
     def run(self, substeps: int, plain_iters=(1, 2, 4), learned_iters=(1, 2)):
         s, e = self.s, self.e
-        res = {("plain", k): [] for k in plain_iters}
-        res.update({("learned", k): [] for k in learned_iters})
-        fallback = {k: 0 for k in learned_iters}
-        for _ in range(substeps):
-            e.integrate()
-            x0, v0, lam_prev = s.X.clone(), s.V.clone(), s.LAM.clone()
-            plain_k = {}
-            for k in plain_iters:
-                s.X.copy_(x0)
-                s.V.copy_(v0)
-                s.ACC.zero_()
-                e.iterate(s.X, s.LAM, s.ACC, k, first=True)
-                plain_k[k] = e.residual()
-                res[("plain", k)].append(plain_k[k])
-            for k in learned_iters:
-                s.X.copy_(x0)
-                s.V.copy_(v0)
-                s.LAM.copy_(lam_prev)
-                s.ACC.zero_()
-                e.warm(self.theta, self.F, self.Y)
-                e.iterate(s.X, s.LAM, s.ACC, k, first=False)
-                r = e.residual()
-                res[("learned", k)].append(r)
-                if k in plain_k and r > plain_k[k]:
-                    fallback[k] += 1
-            # continue the trajectory with the reference's plain 2-iteration substep
-            s.X.copy_(x0)
-            s.V.copy_(v0)
+        keep = [t.clone() for t in (s.X, s.V, s.XP, s.LAM)]
+
+        def restore():
+            for dst, src in zip((s.X, s.V, s.XP, s.LAM), keep):
+                dst.copy_(src)
             s.ACC.zero_()
-            e.iterate(s.X, s.LAM, s.ACC, max(plain_iters[0], s.iterations), first=True)
-            e.finalize()
-        return ({f"{a}_{k}": float(np.sqrt(np.mean(np.square(v)))) for (a, k), v in res.items()},
-                fallback)
+
+        runs = {}
+        for kind, its in [("plain", k) for k in plain_iters] + [("learned", k)
+                                                                 for k in learned_iters]:
+            restore()
+            r = []
+            for _ in range(substeps):
+                e.integrate()
+                if kind == "learned":
+                    e.warm(self.theta, self.F, self.Y)
+                    e.iterate(s.X, s.LAM, s.ACC, its, first=False)
+                else:
+                    e.iterate(s.X, s.LAM, s.ACC, its, first=True)
+                r.append(e.residual())
+                e.finalize()
+            runs[(kind, its)] = np.asarray(r)
+        restore()
+        out = {f"{kind}_{k}": float(np.sqrt(np.mean(np.square(v)))) for (kind, k), v in runs.items()}
+        fallback = {k: int(np.sum(runs[("learned", k)] > runs[("plain", k)]))
+                    for k in learned_iters if ("plain", k) in runs}
+        return out, fallback
+
+
+class Guard:
+    """The learned start with the plain zero start as fallback and judge.
+
+    Every ``check_every`` frames one substep is replayed both ways from the
+    current state (``iters`` Jacobi iterations each, eager launches, state
+    restored afterwards).  The learned start stays on while its RMS residual
+    is no larger than the plain one; otherwise the solver runs the zero start
+    until the next check.  ``history`` keeps (frame, plain, learned, on)."""
+
+    def __init__(self, s, theta, check_every: int = 30, iters: int | None = None):
+        import torch
+
+        _prepare(s)
+        self.s = s
+        self.theta = torch.as_tensor(theta, dtype=torch.float32).to(s.device).contiguous()
+        self.every = max(1, int(check_every))
+        self.iters = s.iterations if iters is None else int(iters)
+        self.frame = 0
+        self.on = False
+        self.history: list = []
+
+    def compare(self) -> tuple[float, float]:
+        """(plain, learned) RMS residual of one substep from the current state."""
+        import torch
+
+        s = self.s
+        e = _Eager(s)
+        keep = [t.clone() for t in (s.X, s.V, s.XP, s.LAM)]
+        F = torch.empty((e.m, 12), device=s.device)
+        Y = torch.empty(e.m, device=s.device)
+
+        def restore():
+            for dst, src in zip((s.X, s.V, s.XP, s.LAM), keep):
+                dst.copy_(src)
+            s.ACC.zero_()
+
+        out = []
+        for learned in (False, True):
+            restore()
+            e.integrate()
+            if learned:
+                e.warm(self.theta, F, Y)
+            e.iterate(s.X, s.LAM, s.ACC, self.iters, first=not learned)
+            out.append(e.residual())
+        restore()
+        return out[0], out[1]
+
+    def step(self, frames: int = 1) -> None:
+        for _ in range(frames):
+            if self.frame % self.every == 0:
+                rp, rl = self.compare()
+                on = rl <= rp
+                if on != self.on:
+                    self.s.set_warm_start(self.theta if on else None)
+                    self.on = on
+                self.history.append((self.frame, rp, rl, on))
+            self.s.step()
+            self.frame += 1

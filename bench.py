@@ -133,8 +133,8 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     d.x_alt, d.lam_alt = N.ptr(sc["X_B"]), N.ptr(sc["LAM_B"])
     d.tile_row_start, d.halo_start, d.halo_rows = N.ptr(sc["TRS"]), N.ptr(sc["HS"]), N.ptr(sc["HR"])
     d.tile_size = sc["tile"]
-    frame = {}
-    for fuse in (1, 0):
+    frame = {1: None, 0: None}
+    for fuse in ((1, 0) if frames > 0 else ()):
         d.fuse_first = fuse
         plan = N.c_void_p()
         N.check(N.lib().xpbd_frame3_create(N.ctypes.byref(d), N.ctypes.byref(plan)), "create")
@@ -149,9 +149,10 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
         "kernel": "k_jacobi_tile",
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": nbytes,
-        "kernel_ms": round(ms, 4), "frame_ms": round(frame_ms, 3),
-        "frame_ms_separate_integrate": round(frame[0], 3), "tile": sc["tile"],
-        "proj_per_s": m * 30 / (frame_ms * 1e-3),
+        "kernel_ms": round(ms, 4), "frame_ms": None if frame_ms is None else round(frame_ms, 3),
+        "frame_ms_separate_integrate": None if frame[0] is None else round(frame[0], 3),
+        "tile": sc["tile"],
+        "proj_per_s": m * 30 / (frame_ms * 1e-3) if frames > 0 else None,
     }
 
 
@@ -200,16 +201,66 @@ def other_workloads(frames: int) -> dict:
     return out
 
 
+def warm_start_section(res: int, frames: int) -> dict:
+    """Learned lambda warm start on the cloth: fit (two rounds, on the GPU),
+    RMS residual of each start on its own trajectory, and whole-frame
+    substeps/s of the captured frame plans (plain zero start with the window
+    and the tile kernel, learned start with the window kernel)."""
+    import torch
+
+    from xpbd_amd import Solver
+    from xpbd_amd import warmstart as ws
+    from xpbd_amd.scenes import cloth_sheet
+
+    def cloth(iters=2):
+        return cloth_sheet(Solver, res=res, solve_mode="jacobi", device="cuda:0",
+                           iterations=iters)[0]
+
+    t0 = time.perf_counter()
+    log = []
+    theta = ws.fit(cloth, substeps=120, settle_frames=2, steps=800, log=log)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    s = cloth()
+    s.step(20)
+    resid, fallback = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(1, 2, 4),
+                                                 learned_iters=(1, 2))
+    sps = {}
+    for name, iters, kernel, learned in [("plain_window_4", 4, "window", False),
+                                         ("plain_window_2", 2, "window", False),
+                                         ("plain_tile_4", 4, "tile", False),
+                                         ("plain_tile_2", 2, "tile", False),
+                                         ("learned_window_2", 2, "window", True),
+                                         ("learned_window_1", 1, "window", True)]:
+        c = cloth(iters)
+        c.jacobi_kernel = kernel
+        if learned:
+            c.set_warm_start(theta)
+        ms = time_frames(c.step, frames, 5)
+        sps[name] = round(c.substeps / (ms * 1e-3), 1)
+    return {
+        "scene": f"cloth {res}x{res}, 15 substeps, Jacobi",
+        "fit_seconds": round(fit_s, 2), "fit": log[0] if log else None,
+        "rms_residual": {k: float(f"{v:.4e}") for k, v in resid.items()},
+        "fallback_substeps_of_60": fallback,
+        "substeps_per_s": sps,
+        "learned2_vs_plain4_window": round(sps["learned_window_2"] / sps["plain_window_4"], 3),
+        "learned2_vs_plain4_tile": round(sps["learned_window_2"] / sps["plain_tile_4"], 3),
+    }
+
+
 def load_traffic(kernel: str):
-    """HBM traffic per launch measured by rocprofv3 --pmc runs, committed
-    under profiles/ (None when no measurement for this kernel exists)."""
+    """HBM bytes per launch of ``kernel`` measured by rocprofv3 --pmc runs
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain copy of known size),
+    committed under profiles/; None when no measurement exists."""
     p = ROOT / "profiles" / "traffic.json"
     if not p.exists():
         return None
     try:
-        return json.loads(p.read_text()).get(kernel)
+        e = json.loads(p.read_text()).get(kernel)
     except (ValueError, OSError):
         return None
+    return None if e is None else e.get("bytes_per_launch")
 
 
 def main() -> int:
@@ -223,8 +274,12 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline-scene", action="store_true")
+    ap.add_argument("--roofline-frames", type=int, default=5,
+                    help="frames timed on the roofline scene (0: kernel only)")
     ap.add_argument("--no-others", action="store_true",
                     help="skip the chain / unified / batched secondary lines")
+    ap.add_argument("--no-warm-start", action="store_true",
+                    help="skip the learned lambda warm-start section")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape instead of the autotuner's pick")
     ap.add_argument("--calib-copy", action="store_true",
@@ -298,16 +353,23 @@ def main() -> int:
 
     scene = None
     if not args.no_roofline_scene:
-        scene = roofline_scene(N, reps=20, frames=5)
+        scene = roofline_scene(N, reps=20, frames=args.roofline_frames)
         print(f"[bench] roofline scene: kernel {scene['kernel_ms']:.3f} ms "
               f"{scene['achieved']} GB/s, frame {scene['frame_ms']} ms", file=sys.stderr,
               flush=True)
+        scene["traffic"] = load_traffic("k_jacobi_tile@8x1024^2")
         torch.cuda.empty_cache()
 
     others = None
     if not args.no_others:
         others = other_workloads(max(args.steps // 3, 30))
         print(f"[bench] others: {json.dumps(others)}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+
+    warm = None
+    if not args.no_warm_start and args.mode == "jacobi":
+        warm = warm_start_section(args.res, max(args.steps // 3, 30))
+        print(f"[bench] warm start: {json.dumps(warm)}", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 
     cpu = None
@@ -330,6 +392,7 @@ def main() -> int:
         "roofline_scene": scene,
         "cpu_baseline": cpu,
         "other_workloads": others,
+        "learned_warm_start": warm,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -123,7 +123,10 @@ __global__ __launch_bounds__(kB) void k_ws_apply(const float4* __restrict__ X,
   const RowGeo g = row_geo(X, V, ANC, row, j, 1.0f / (h * h));
   const float s = g.w + g.alpha;
   float lam0 = (s > 0.0f && g.ok) ? slog_inv(Y[j]) * kLen / s : 0.0f;
-  if (g.t == T_FLOOR || g.t == T_CONTACT) lam0 = fmaxf(lam0, 0.0f);
+  // one-sided rows push only while they are violated (the projection itself
+  // leaves a separated row alone): a small positive prediction on a free
+  // floor row would lift the particle every substep
+  if (g.t == T_FLOOR || g.t == T_CONTACT) lam0 = g.C < 0.0f ? fmaxf(lam0, 0.0f) : 0.0f;
   if (!isfinite(lam0)) lam0 = 0.0f;
   LAM[j] = lam0;
   if (lam0 == 0.0f) return;

@@ -274,3 +274,39 @@ def test_jacobi_tile_order_groups_by_tile_then_type_then_rank():
             breaks = np.nonzero(np.diff(seg) <= 0)[0]
             for run in np.split(seg, breaks + 1):
                 assert len(np.unique(run)) == len(run)
+
+
+def test_ws_oracle_one_row_closed_form():
+    """One stretched DISTANCE row between two particles: C, the normal, the
+    first plain step -C/(w + alpha~), lambda0 = 8 sigmoid(y) step and the
+    mean offsets w grad C lambda0 worked out by hand; a separated FLOOR row
+    gives no push; the residual of the row."""
+    import struct
+
+    from oracle import ws_ref
+
+    h = 1.0 / 600.0
+    X = np.array([[0.0, 0.0, 0.0, 2.0], [0.3, 0.4, 0.0, 1.0]])     # |d| = 0.5
+    V = np.zeros((2, 4))
+    comp = 1e-6
+    rest = 0.4
+    bits = lambda v: struct.unpack("<i", struct.pack("<f", v))[0]   # noqa: E731
+    rows = np.array([[0 | (0 << 28), 1, bits(rest), bits(comp)],
+                     [1 | (2 << 28), -1, bits(0.0), bits(0.5)]], np.int64).astype(np.int32)
+    anc = np.zeros((2, 4))
+    F = ws_ref.features(X, V, rows, anc, h)
+    C = 0.5 - np.float32(rest)
+    alpha = np.float32(comp) / (h * h)
+    assert F[0, 0] == 1.0 and F[1, 2] == 1.0
+    assert np.isclose(F[0, 4], np.log1p(C / 1e-3))
+    assert np.isclose(F[0, 6], alpha / (3.0 + alpha))
+    assert np.isclose(F[1, 4], np.log1p(0.4 / 1e-3))            # floor row: C = y = 0.4
+    lam0, acc = ws_ref.apply(X, V, rows, anc, np.array([0.0, 5.0]), h)
+    step = -C / (3.0 + alpha)
+    assert np.isclose(lam0[0], 4.0 * step)                      # sigmoid(0) = 1/2
+    assert lam0[1] == 0.0                                       # separated floor row
+    n = np.array([-0.6, -0.8, 0.0])                             # (x_a - x_b) / |d|
+    assert np.allclose(acc[0, :3], n * 2.0 * lam0[0]) and acc[0, 3] == 1
+    assert np.allclose(acc[1, :3], -n * 1.0 * lam0[0]) and acc[1, 3] == 1
+    r2 = ws_ref.residual_sum(X, rows, anc, np.array([lam0[0], 0.0]), h)
+    assert np.isclose(r2, (C + alpha * lam0[0]) ** 2)

@@ -9,9 +9,10 @@
 //                      log10(1 + alpha~)/8
 //   with w = w_a + w_b (|grad C|^2 weighted by inverse mass), alpha~ = alpha/h^2,
 //   l = 1 mm and slog(x) = sign(x) ln(1 + |x|).
-// The MLP predicts y = slog(lambda* (w + alpha~) / l) (lambda* = the
-// multiplier after many plain iterations); the warm start sets
-// lambda0 = slog^-1(y) l / (w + alpha~) (>= 0 for the one-sided rows) and
+// The MLP predicts y = logit(omega / 8), omega = beta lambda* / lambda1
+// (lambda* = the multiplier after many plain iterations, beta = 0.5, lambda1
+// = -C / (w + alpha~) the first plain step from zero); the warm start sets
+// lambda0 = 8 sigmoid(y) lambda1 (zero for separated one-sided rows) and
 // moves every particle by the average over its rows of w grad C lambda0
 // (times the relaxation), the position the Jacobi sweep itself reaches when
 // its multipliers add up to lambda0: Jacobi applies the mean, not the sum,
@@ -26,9 +27,9 @@ namespace {
 
 constexpr int kB = 256;
 constexpr float kLen = 1.0e-3f;
+constexpr float kOmegaMax = 8.0f;  // largest multiple of the first plain step
 
 __device__ inline float slog(float x) { return copysignf(log1pf(fabsf(x)), x); }
-__device__ inline float slog_inv(float y) { return copysignf(expm1f(fabsf(y)), y); }
 
 struct RowGeo {
   int t, a, b;
@@ -122,11 +123,14 @@ __global__ __launch_bounds__(kB) void k_ws_apply(const float4* __restrict__ X,
   const int4 row = R[j];
   const RowGeo g = row_geo(X, V, ANC, row, j, 1.0f / (h * h));
   const float s = g.w + g.alpha;
-  float lam0 = (s > 0.0f && g.ok) ? slog_inv(Y[j]) * kLen / s : 0.0f;
-  // one-sided rows push only while they are violated (the projection itself
-  // leaves a separated row alone): a small positive prediction on a free
-  // floor row would lift the particle every substep
-  if (g.t == T_FLOOR || g.t == T_CONTACT) lam0 = g.C < 0.0f ? fmaxf(lam0, 0.0f) : 0.0f;
+  // lambda0 = omega * (the first plain step from zero, -C / (w + alpha~)),
+  // omega = kOmegaMax * sigmoid(y): bounded by the row's own violation now,
+  // so nothing from earlier substeps is fed back.  One-sided rows push only
+  // while violated (the projection leaves a separated row alone).
+  float step = (s > 0.0f && g.ok) ? -g.C / s : 0.0f;
+  if (g.t == T_FLOOR || g.t == T_CONTACT) step = g.C < 0.0f ? step : 0.0f;
+  const float omega = kOmegaMax / (1.0f + expf(-Y[j]));
+  float lam0 = omega * step;
   if (!isfinite(lam0)) lam0 = 0.0f;
   LAM[j] = lam0;
   if (lam0 == 0.0f) return;

@@ -3,10 +3,12 @@ counterpart; it replaces the per-substep zero reset of the multipliers,
 src/xpbd3d/solver.py:431).
 
 * ``collect``  runs the solver and, at sampled substeps, records the row
-  features (kernel ``xpbd_ws_features``) and the target
-  y* = slog(beta lambda* (w + alpha~) / 1 mm), lambda* = the multipliers
-  after ``target_iters`` plain Jacobi iterations of that substep from
-  lambda = 0 (run on a copy of the state), beta = 0.5;
+  features (kernel ``xpbd_ws_features``) and the target y* =
+  logit(omega / 8), omega = beta lambda* / lambda1: lambda* = the
+  multipliers after ``target_iters`` plain Jacobi iterations of that substep
+  from lambda = 0 (run on a copy of the state), beta = 0.5, lambda1 = the
+  row's first plain step -C / (w + alpha~); the deployed start is
+  lambda0 = 8 sigmoid(y) lambda1, bounded by the row's current violation;
 * ``fit``      two rounds of collect + train: on the plain trajectory, then on
   the learned start's own trajectory;
 * ``train``    fits the 12-64-64-1 MLP on the GPU with the library's own BF16
@@ -29,6 +31,7 @@ from . import _native as N
 
 LEN = 1e-3
 N_PARAMS = 5057
+OMEGA_MAX = 8.0         # kOmegaMax of csrc/warmstart.hip
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -45,6 +48,31 @@ def _slog(x):
     import torch
 
     return torch.sign(x) * torch.log1p(torch.abs(x))
+
+
+def _slog_inv(y):
+    import torch
+
+    return torch.sign(y) * torch.expm1(torch.abs(y))
+
+
+def target_logit(F, lam_star, scale, beta: float):
+    """Network target y* = logit(omega / OMEGA_MAX), omega = beta lambda* /
+    lambda1 clipped to (0, OMEGA_MAX), lambda1 = -C / (w + alpha~) the first
+    plain step from zero (C recovered from feature 4); rows without a step
+    (separated one-sided rows, w + alpha~ = 0) get y* = 0."""
+    import torch
+
+    C = _slog_inv(F[:, 4]) * LEN
+    one_sided = (F[:, 2] > 0.5) | (F[:, 3] > 0.5)
+    lam1 = torch.where(scale > 0, -C / torch.clamp(scale, min=1e-30), torch.zeros_like(C))
+    lam1 = torch.where(one_sided & (C >= 0), torch.zeros_like(lam1), lam1)
+    valid = lam1.abs() > 1e-12
+    ratio = torch.where(valid, beta * lam_star / torch.where(valid, lam1, torch.ones_like(lam1)),
+                        torch.zeros_like(lam1))
+    om = torch.clamp(ratio, 0.01, OMEGA_MAX - 0.01)
+    y = torch.log(om / (OMEGA_MAX - om))
+    return torch.where(valid, y, torch.zeros_like(y))
 
 
 def row_scale(s) -> "torch.Tensor":
@@ -148,13 +176,13 @@ def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int
     append (features, target) for all rows.  Returns (F, T) device tensors of
     shape (k m, 12) and (k m,).
 
-    Target: slog(beta lambda* (w + alpha~) / 1 mm), lambda* = the multipliers
-    after ``target_iters`` plain iterations from zero at that substep (run on
-    a copy of the state), beta = ``target_scale``.  A full lambda* start
-    overshoots: carried from substep to substep it pumps energy into the
-    scene (measured on cloth: the reuse of the previous substep's lambda
-    explodes within a few frames, half of it matches the plain solver with
-    twice the iterations), so the network learns the damped start.
+    Target: ``target_logit`` of beta lambda*, lambda* = the multipliers after
+    ``target_iters`` plain iterations from zero at that substep (run on a
+    copy of the state), beta = ``target_scale``: the network predicts the
+    multiple omega of the row's first plain step that the start should take.
+    A full lambda* start overshoots (measured on cloth: reusing the previous
+    substep's lambda explodes within a few frames, half of it matches the
+    plain solver with twice the iterations), so the target is damped.
 
     ``theta`` None: the trajectory is the plain solver's; ``theta`` given: the
     trajectory runs the learned start (an on-policy round, whose states are
@@ -164,7 +192,7 @@ def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int
     _prepare(s)
     e = _Eager(s)
     iters = s.iterations if iters is None else iters
-    scale = row_scale(s) * float(target_scale)
+    scale = row_scale(s)
     F_all, T_all = [], []
     F = torch.empty((e.m, 12), device=s.device)
     Y = torch.empty(e.m, device=s.device)
@@ -177,7 +205,7 @@ def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int
             Ac = torch.zeros_like(s.ACC)
             e.iterate(Xc, Lc, Ac, target_iters, first=True)
             F_all.append(F.clone())
-            T_all.append(_slog(Lc[:e.m] * scale / LEN))
+            T_all.append(target_logit(F, Lc[:e.m], scale, float(target_scale)))
         if theta is None:
             e.iterate(s.X, s.LAM, s.ACC, iters, first=True)
         else:

@@ -437,11 +437,24 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     atomicAdd(&sz[off], fix_bits(fminf(fmaxf(c.z, -kFixLimit), kFixLimit)));
     atomicAdd(&cnt[off], ok ? 1u : (kFixPoison | 1u));
   };
-  // two rows per thread per pass, their loads issued together; the loads of
-  // the first pass are issued before the staging barrier so that they
-  // overlap the position staging
+  // two rows per thread per pass, their loads issued together: those of the
+  // first pass before the staging barrier (they overlap the position
+  // staging), those of each next pass before the current pass is projected
+  // (61 VGPRs: still 8 waves per SIMD)
   for (int k0 = threadIdx.x; k0 < total; k0 += BLOCK * U) {
-    if (k0 != (int)threadIdx.x) load_pass(k0);
+    int4 nrow[U];
+    float nlam[U];
+    int njj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + (U + u) * BLOCK;
+      njj[u] = -1;
+      if (k < total) {
+        njj[u] = k < owned ? rb + k : halo_rows[hb + (k - owned)];
+        nrow[u] = R[njj[u]];
+        nlam[u] = (PREDICT || first) ? 0.0f : LAMin[njj[u]];
+      }
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (jj[u] < 0) continue;
@@ -452,6 +465,12 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
       if (k0 + u * BLOCK < owned) LAMout[jj[u]] = p.lam;
       if (p.na >= 1) add(p.a, p.ca);
       if (p.na == 2) add(p.b, p.cb);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      row[u] = nrow[u];
+      lam[u] = nlam[u];
+      jj[u] = njj[u];
     }
   }
   __syncthreads();

@@ -201,6 +201,52 @@ def other_workloads(frames: int) -> dict:
     return out
 
 
+# The reference's own published rows (README.md:108-115, 186-191; Warp on an
+# RTX 3060 Laptop GPU, ms per frame), for the same scenes built here.
+PUBLISHED_MS = {
+    ("cloth16", "gs"): 6.6, ("cloth16", "jacobi"): 2.4, ("cloth32", "gs"): 6.5,
+    ("cloth32", "jacobi"): 2.3, ("cloth48", "gs"): 6.5, ("cloth48", "jacobi"): 2.3,
+    ("stack8x8x5", "gs"): 7.8, ("stack8x8x5", "jacobi"): 5.2,
+    ("boxes27", "lbvh"): 1.33, ("boxes100", "lbvh"): 1.38, ("boxes384", "lbvh"): 1.41,
+    ("boxes1440", "lbvh"): 1.84,
+}
+
+
+def reference_rows(frames: int = 120, warmup: int = 20) -> dict:
+    """The reference benchmark's scenes (examples/benchmark.py:71-107 and the
+    6-DOF box piles of README.md:186-191), timed like its time_steps
+    (warm-up frames, then frames between device syncs), with the published
+    RTX 3060 Laptop figures beside them."""
+    from xpbd_amd import Solver, Solver6DOF
+    from xpbd_amd.scenes import box_pile, cloth_bench, sphere_stack
+
+    rows = {}
+
+    def put(name, mode, s, ms):
+        proj = s.num_constraints * s.substeps * s.iterations
+        pub = PUBLISHED_MS.get((name, mode))
+        rows[f"{name}_{mode}"] = {
+            "ms_per_frame": round(ms, 4), "bodies": s.num_bodies, "rows": s.num_constraints,
+            "proj_per_s": proj / (ms * 1e-3), "published_rtx3060_ms": pub,
+            "speedup_vs_published": None if pub is None else round(pub / ms, 1)}
+
+    for mode in ("gs", "jacobi"):
+        for res in (16, 32, 48):
+            s = cloth_bench(Solver, res, mode, device="cuda:0")
+            put(f"cloth{res}", mode, s, time_frames(s.step, frames, warmup))
+        s = sphere_stack(Solver, 8, 5, mode, device="cuda:0")
+        put("stack8x8x5", mode, s, time_frames(s.step, frames, warmup))
+    for dims in ((3, 3, 3), (5, 4, 5), (8, 6, 8), (12, 10, 12)):
+        s = box_pile(Solver6DOF, *dims, device="cuda:0", broadphase="lbvh")
+        n = dims[0] * dims[1] * dims[2]
+        ms = time_frames(s.step, frames, warmup)
+        pub = PUBLISHED_MS.get((f"boxes{n}", "lbvh"))
+        rows[f"boxes{n}_lbvh"] = {"ms_per_frame": round(ms, 4), "bodies": s.num_bodies,
+                                  "published_rtx3060_ms": pub,
+                                  "speedup_vs_published": round(pub / ms, 1)}
+    return rows
+
+
 def warm_start_section(res: int, frames: int) -> dict:
     """Learned lambda warm start on the cloth: fit (two rounds, on the GPU),
     RMS residual of each start on its own trajectory, and whole-frame
@@ -278,6 +324,8 @@ def main() -> int:
                     help="frames timed on the roofline scene (0: kernel only)")
     ap.add_argument("--no-others", action="store_true",
                     help="skip the chain / unified / batched secondary lines")
+    ap.add_argument("--no-reference-rows", action="store_true",
+                    help="skip the reference benchmark's own scenes")
     ap.add_argument("--no-warm-start", action="store_true",
                     help="skip the learned lambda warm-start section")
     ap.add_argument("--no-autotune", action="store_true",
@@ -366,6 +414,12 @@ def main() -> int:
         print(f"[bench] others: {json.dumps(others)}", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 
+    ref_rows = None
+    if not args.no_reference_rows:
+        ref_rows = reference_rows()
+        print(f"[bench] reference rows: {json.dumps(ref_rows)}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+
     warm = None
     if not args.no_warm_start and args.mode == "jacobi":
         warm = warm_start_section(args.res, max(args.steps // 3, 30))
@@ -393,6 +447,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "other_workloads": others,
         "learned_warm_start": warm,
+        "reference_benchmark_rows": ref_rows,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -132,6 +132,21 @@ int xpbd_sphere_contacts(void* stream, void* ws, const float* X, const float* V,
                          const float* radius, const float* mu, float contact_margin, float dt,
                          float r_max, int n, int cap, int* ROWS, int* count);
 
+/* Whole frames of a small scene in one workgroup (solver.py:412-467): the
+ * positions stay in LDS, phases are separated by workgroup barriers.
+ * gs = 1: Gauss-Seidel over the color slices color_starts[0..num_colors]
+ * (device array; rows sorted by color), bit-identical to xpbd_solve_color
+ * launches; gs = 0: Jacobi with the fused tile kernel's arithmetic and exact
+ * sums (identical to xpbd_jacobi_tile).  Runs `frames` frames of `substeps`
+ * substeps x `iterations` iterations; n_particles <=
+ * xpbd_frame_block_capacity(gs). */
+int xpbd_frame_block(void* stream, float* X, float* V, float* XP, const int* R, const float* ANC,
+                     float* LAM, const int* color_starts, int num_colors, int n_particles,
+                     int n_rows, int substeps, int iterations, float h, float gx, float gy,
+                     float gz, float damping, float relax, float max_speed, int gs, int frames);
+/* Largest particle count of xpbd_frame_block (LDS-bound). */
+int xpbd_frame_block_capacity(int gs);
+
 /* kernels.py:105-208 `solve_constraints_color` over rows [start, start+count)
  * of a color-sorted row array, launched per color at solver.py:444-456. */
 int xpbd_solve_color(void* stream, float* X, const float* XP, const int* R, const float* ANC,
@@ -184,6 +199,12 @@ typedef struct xpbd_particles_desc {
    * the first iteration (xpbd_jacobi_tile_first), 0 keeps a separate
    * xpbd_integrate_to launch per substep.  Results are identical. */
   int fuse_first;
+  /* Small scenes (n_particles <= xpbd_frame_block_capacity): 1 runs whole
+   * frames in one workgroup (xpbd_frame_block, GS or window-Jacobi mode; one
+   * launch per xpbd_frame3_run call); color_starts_dev is the device copy of
+   * color_starts (GS). */
+  int block;
+  const int* color_starts_dev;
 } xpbd_particles_desc;
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);
@@ -246,6 +267,35 @@ int xpbd6_velocity_box(void* stream, const float* X, const float* Q, const float
  * (solver_6dof.py:420-426). */
 int xpbd6_apply_velocity(void* stream, const float* X, float* V, float* W, float* DV, float* DW,
                          float max_lin, float max_ang, int do_cap, int n);
+
+/* Fused launches of the same kernels, one launch per phase of a substep
+ * (solver_6dof.py:357-426 records the phases as separate launches).
+ * begin_substep: apply_velocity of the previous substep when apply != 0
+ * (solver_6dof.py:420-426), then integrate_bodies and the lambda resets
+ * (solver_6dof.py:370-375).  solve_all: one thread per joint, per body's
+ * floor corners and per live manifold point (solver_6dof.py:385-401); every
+ * correction is added atomically into DX/DR.  apply_finalize: apply_jacobi
+ * and, when finalize != 0, finalize_velocity (solver_6dof.py:402-408).
+ * velocity_all: velocity_floor and velocity_box (solver_6dof.py:409-419),
+ * added atomically into DV/DW. */
+int xpbd6_begin_substep(void* stream, float* X, float* Q, float* V, float* W, float* XP,
+                        float* QP, const float* IB, float* LAMF, int* PTC, float* DV, float* DW,
+                        float max_lin, float max_ang, int do_cap, int apply, float gx, float gy,
+                        float gz, float h, int gyro, const float* gscale, int n);
+int xpbd6_solve_all(void* stream, const float* X, const float* Q, const float* XP,
+                    const float* QP, const float* IB, const float* HE, const int* JI,
+                    const float* JA, const float* JB, float* LAMJ, float h, int first, int nj,
+                    float floor_y, float mu_s, float* LAMF, int n, const int* PTI,
+                    const float* PTN, const float* PTOI, const float* PTOR, float* LAMP,
+                    const int* PTC, int max_points, float* DX, float* DR);
+int xpbd6_apply_finalize(void* stream, float* X, float* Q, float* DX, float* DR, float relax,
+                         int finalize, const float* XP, const float* QP, float* V, float* W,
+                         float h, float lin_damp, float ang_damp, int n);
+int xpbd6_velocity_all(void* stream, const float* X, const float* Q, const float* IB,
+                       const float* HE, float mu_d, float restitution, const float* LAMF,
+                       float h, const float* V, const float* W, int n, const int* PTI,
+                       const float* PTN, const float* PTOI, const float* PTOR, const float* LAMP,
+                       const int* PTC, int max_points, float* DV, float* DW);
 
 typedef struct xpbd_rigid_desc {
   float *x, *q, *v, *w, *x_prev, *q_prev; /* float4[n] */

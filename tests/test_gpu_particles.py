@@ -299,6 +299,43 @@ def test_autotune_restores_state_and_keeps_results():
     assert again == best and t2 == {}
 
 
+@pytest.mark.parametrize("scene,mode", [("chain", "gs"), ("cloth", "gs"), ("cloth", "jacobi")])
+def test_block_frames_equal_launch_path(scene, mode):
+    """Whole frames in one workgroup (small scenes) give bit for bit what
+    the per-phase launches give: GS (the same row arithmetic in the same
+    color order) and Jacobi (the fused tile kernel's arithmetic with exact
+    sums)."""
+    from xpbd_amd import Solver
+    from xpbd_amd.scenes import cloth_sheet, hanging_chain
+
+    out = []
+    for block in (True, False):
+        if scene == "chain":
+            s = hanging_chain(Solver, device=DEV)
+        else:
+            s = cloth_sheet(Solver, res=24, solve_mode=mode, device=DEV)[0]
+        s.block_frames = block
+        s.step(5)
+        s.step()
+        assert s._block_active == block
+        out.append((s.positions(), s.velocities(), s.lambdas()))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_block_frames_capacity_falls_back():
+    """Above the workgroup's LDS capacity the solver keeps the launch path."""
+    from xpbd_amd import Solver
+    from xpbd_amd import _native as N
+    from xpbd_amd.scenes import cloth_sheet
+
+    cap = int(N.lib().xpbd_frame_block_capacity(0))
+    s = cloth_sheet(Solver, res=int(np.ceil(np.sqrt(cap + 1))), solve_mode="jacobi",
+                    device=DEV)[0]
+    s.step()
+    assert not s._block_active and s.num_bodies > cap
+
+
 def test_graph_replay_equals_eager_launches():
     from xpbd_amd import Solver
     from xpbd_amd.scenes import cloth_sheet
@@ -307,6 +344,7 @@ def test_graph_replay_equals_eager_launches():
     for use_graph in (True, False):
         s, _ = cloth_sheet(Solver, res=32, solve_mode="gs", device=DEV)
         s.use_graph = use_graph
+        s.block_frames = False       # the per-phase launch path, graph or eager
         for _ in range(20):
             s.step()
         out.append((s.positions(), s.velocities(), s.lambdas()))

@@ -37,7 +37,7 @@ class ParticlesDesc(ctypes.Structure):
         ("dyn_rows", c_void_p), ("dyn_lam", c_void_p), ("dyn_count", c_void_p),
         ("dyn_cap", c_int), ("gscale", c_void_p),
         ("ws_theta", c_void_p), ("ws_feat", c_void_p), ("ws_y", c_void_p),
-        ("fuse_first", c_int),
+        ("fuse_first", c_int), ("block", c_int), ("color_starts_dev", c_void_p),
     ]
 
 
@@ -84,6 +84,8 @@ _SIGS = {
     "xpbd_contacts_workspace_bytes": (ctypes.c_longlong, [c_int]),
     "xpbd_sphere_contacts": (c_int, [c_void_p] * 6 + [c_float, c_float, c_float, c_int, c_int,
                                                       c_void_p, c_void_p]),
+    "xpbd_frame_block": (c_int, [c_void_p] * 8 + [c_int] * 5 + [c_float] * 7 + [c_int, c_int]),
+    "xpbd_frame_block_capacity": (c_int, [c_int]),
     "xpbd_solve_color": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_float, c_int, c_int, c_int]),
     "xpbd_frame3_create": (c_int, [ctypes.POINTER(ParticlesDesc), ctypes.POINTER(c_void_p)]),
@@ -101,6 +103,10 @@ _SIGS = {
     "xpbd6_velocity_floor": (_I, [_P] * 5 + [_F, _F, _P, _F, _P, _P, _P, _P, _I]),
     "xpbd6_velocity_box": (_I, [_P] * 10 + [_F, _F, _F, _P, _P, _P, _P, _I]),
     "xpbd6_apply_velocity": (_I, [_P] * 6 + [_F, _F, _I, _I]),
+    "xpbd6_begin_substep": (_I, [_P] * 12 + [_F, _F, _I, _I, _F, _F, _F, _F, _I, _P, _I]),
+    "xpbd6_solve_all": (_I, [_P] * 11 + [_F, _I, _I, _F, _F, _P, _I] + [_P] * 6 + [_I, _P, _P]),
+    "xpbd6_apply_finalize": (_I, [_P] * 5 + [_F, _I] + [_P] * 4 + [_F, _F, _F, _I]),
+    "xpbd6_velocity_all": (_I, [_P] * 5 + [_F, _F, _P, _F, _P, _P, _I] + [_P] * 6 + [_I, _P, _P]),
     "xpbd_frame6_create": (_I, [ctypes.POINTER(RigidDesc), ctypes.POINTER(c_void_p)]),
     "xpbd_frame6_run": (_I, [_P, _P, _I, _I]),
     "xpbd_frame6_destroy": (_I, [_P]),

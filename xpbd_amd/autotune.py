@@ -9,7 +9,8 @@ and keeps the fastest:
   rows, smaller tiles spread the launch over more CUs), each with integrate
   folded into the first iteration or run as its own launch;
 * ``window`` — projection with an LDS window + global atomics and a separate
-  apply pass, with 1 or 2 rows per thread.
+  apply pass, with 1 or 2 rows per thread;
+* ``block``  — whole frames in one workgroup (scenes that fit its LDS).
 
 The simulation state is saved before and restored after each trial, so tuning
 does not advance the scene.  Results are cached per scene signature (particle
@@ -32,12 +33,21 @@ def candidates(n_particles: int):
             out.append(("tile", t, None, False))
     out.append(("window", 0, (1, None), False))
     out.append(("window", 0, (2, None), False))
+    if n_particles <= _block_capacity():
+        out.append(("block", 0, None, False))
     return out
+
+
+def _block_capacity() -> int:
+    from . import _native as N
+
+    return int(N.lib().xpbd_frame_block_capacity(0))
 
 
 def _apply(s, cand):
     kind, tile, tiling, fuse = cand
-    s.jacobi_kernel = kind
+    s.block_frames = kind == "block"
+    s.jacobi_kernel = "tile" if kind == "block" else kind
     s.tile_size = tile
     s.fuse_first = fuse
     s._dirty = True

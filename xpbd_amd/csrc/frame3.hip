@@ -200,6 +200,11 @@ int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
   if (desc->mode == XPBD_MODE_GS && desc->n_rows > 0 && (!desc->color_starts || desc->num_colors < 1))
     return (int)hipErrorInvalidValue;
   if (desc->mode == XPBD_MODE_JACOBI && desc->n_rows > 0 && !desc->acc) return (int)hipErrorInvalidValue;
+  if (desc->block &&
+      (desc->mode == XPBD_MODE_JACOBI_TILE || desc->ws_theta ||
+       desc->n_particles > xpbd_frame_block_capacity(desc->mode == XPBD_MODE_GS) ||
+       (desc->mode == XPBD_MODE_GS && desc->n_rows > 0 && !desc->color_starts_dev)))
+    return (int)hipErrorInvalidValue;
   Frame3* f = new (std::nothrow) Frame3();
   if (!f) return (int)hipErrorOutOfMemory;
   f->d = *desc;
@@ -214,6 +219,13 @@ int xpbd_frame3_run(void* handle, void* stream, int frames, int use_graph) {
   Frame3* f = (Frame3*)handle;
   if (!f || frames < 0) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
+  if (f->d.block) {  // whole frames in one workgroup: one launch, no graph needed
+    const xpbd_particles_desc& d = f->d;
+    return xpbd_frame_block(s, d.x, d.v, d.x_prev, d.rows, d.anchors, d.lam, d.color_starts_dev,
+                            d.num_colors, d.n_particles, d.n_rows, d.substeps, d.iterations, d.h,
+                            d.gx, d.gy, d.gz, d.damping, d.relax, d.max_speed,
+                            d.mode == XPBD_MODE_GS ? 1 : 0, frames);
+  }
   if (!use_graph) {
     for (int k = 0; k < frames; ++k) {
       int rc = issue_frame(f, s);

@@ -18,49 +18,48 @@ struct Frame6 {
   hipStream_t capture_stream = nullptr;
 };
 
+// Per substep five launches: apply_velocity of the previous substep fused
+// with integrate; the manifold; per iteration one launch over every row kind
+// (joints, floor corners, box points) and one apply (with finalize on the
+// last iteration); one velocity pass over floor bodies and box points.  The
+// frame's last apply_velocity is its own launch.  Order of operations as
+// Solver6DOF._record (solver_6dof.py:357-426).
 int issue(Frame6* f, hipStream_t s) {
   const xpbd_rigid_desc& d = f->d;
   const int n = d.n;
   if (n <= 0) return 0;
   const int maxp = 8 * d.cap;
+  const int cap_v = d.max_speed > 0.0f;
   int rc;
   for (int sub = 0; sub < d.substeps; ++sub) {
-    rc = xpbd6_integrate(s, d.x, d.q, d.v, d.w, d.x_prev, d.q_prev, d.inv_I, d.lam_floor,
-                         d.pt_count, d.gx, d.gy, d.gz, d.h, d.gyroscopic, d.gscale, n);
+    rc = xpbd6_begin_substep(s, d.x, d.q, d.v, d.w, d.x_prev, d.q_prev, d.inv_I, d.lam_floor,
+                             d.pt_count, d.dv, d.dw, d.max_speed, d.max_omega, cap_v, sub > 0,
+                             d.gx, d.gy, d.gz, d.h, d.gyroscopic, d.gscale, n);
     if (rc) return rc;
     rc = xpbd6_box_manifold(s, d.x, d.q, d.he, d.pairs, d.pair_count, d.man_margin, d.cap,
                             d.m_count, d.pt_idx, d.pt_n, d.pt_oi, d.pt_or, d.lam_pt, d.pt_count);
     if (rc) return rc;
     for (int it = 0; it < d.iterations; ++it) {
-      if (d.nj > 0) {
-        rc = xpbd6_solve_joints(s, d.x, d.q, d.inv_I, d.j_idx, d.j_a, d.j_b, d.lam_j, d.h,
-                                it == 0, d.dx, d.drot, d.nj);
-        if (rc) return rc;
-      }
-      rc = xpbd6_solve_floor(s, d.x, d.q, d.x_prev, d.q_prev, d.inv_I, d.he, d.floor_y,
-                             d.friction, d.lam_floor, d.dx, d.drot, n);
+      rc = xpbd6_solve_all(s, d.x, d.q, d.x_prev, d.q_prev, d.inv_I, d.he, d.j_idx, d.j_a, d.j_b,
+                           d.lam_j, d.h, it == 0, d.nj, d.floor_y, d.friction, d.lam_floor, n,
+                           d.pt_idx, d.pt_n, d.pt_oi, d.pt_or, d.lam_pt, d.pt_count, maxp, d.dx,
+                           d.drot);
       if (rc) return rc;
-      rc = xpbd6_solve_box(s, d.x, d.q, d.inv_I, d.pt_idx, d.pt_n, d.pt_oi, d.pt_or, d.lam_pt,
-                           d.pt_count, d.dx, d.drot, maxp);
-      if (rc) return rc;
-      rc = xpbd6_apply_jacobi(s, d.x, d.q, d.dx, d.drot, d.relax, n);
+      rc = xpbd6_apply_finalize(s, d.x, d.q, d.dx, d.drot, d.relax, it + 1 == d.iterations,
+                                d.x_prev, d.q_prev, d.v, d.w, d.h, d.lin_damp, d.ang_damp, n);
       if (rc) return rc;
     }
-    rc = xpbd6_finalize_velocity(s, d.x, d.q, d.x_prev, d.q_prev, d.v, d.w, d.h, d.lin_damp,
-                                 d.ang_damp, n);
-    if (rc) return rc;
-    rc = xpbd6_velocity_floor(s, d.x, d.q, d.inv_I, d.he, d.friction, d.restitution,
-                              d.lam_floor, d.h, d.v, d.w, d.dv, d.dw, n);
-    if (rc) return rc;
-    rc = xpbd6_velocity_box(s, d.x, d.q, d.inv_I, d.pt_idx, d.pt_n, d.pt_oi, d.pt_or, d.lam_pt,
-                            d.pt_count, d.friction, d.restitution, d.h, d.v, d.w, d.dv, d.dw,
-                            maxp);
-    if (rc) return rc;
-    rc = xpbd6_apply_velocity(s, d.x, d.v, d.w, d.dv, d.dw, d.max_speed, d.max_omega,
-                              d.max_speed > 0.0f, n);
+    if (d.iterations == 0) {
+      rc = xpbd6_finalize_velocity(s, d.x, d.q, d.x_prev, d.q_prev, d.v, d.w, d.h, d.lin_damp,
+                                   d.ang_damp, n);
+      if (rc) return rc;
+    }
+    rc = xpbd6_velocity_all(s, d.x, d.q, d.inv_I, d.he, d.friction, d.restitution, d.lam_floor,
+                            d.h, d.v, d.w, n, d.pt_idx, d.pt_n, d.pt_oi, d.pt_or, d.lam_pt,
+                            d.pt_count, maxp, d.dv, d.dw);
     if (rc) return rc;
   }
-  return 0;
+  return xpbd6_apply_velocity(s, d.x, d.v, d.w, d.dv, d.dw, d.max_speed, d.max_omega, cap_v, n);
 }
 
 void drop_graph(Frame6* f) {

@@ -536,16 +536,11 @@ __global__ __launch_bounds__(BLOCK) void k_integrate_to(const float4* Xin, float
 // color-sorted rows; rows of one color touch disjoint particles, so the
 // in-place updates are race-free (kernels.py:105-208).
 // ---------------------------------------------------------------------------
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_gs_color(float4* __restrict__ X,
-                                                    const float4* __restrict__ XP,
-                                                    const int4* __restrict__ R,
-                                                    const float4* __restrict__ ANC,
-                                                    float* __restrict__ LAM, float h, int first,
-                                                    int start, int count) {
-  const int tid = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  if (tid >= count) return;
-  const int j = start + tid;
+// One Gauss-Seidel row, in place on X (global memory, or the LDS copy of
+// the single-workgroup frame kernel): kernels.py:105-208.
+__device__ inline void gs_row(float4* __restrict__ X, const float4* __restrict__ XP,
+                              const int4* __restrict__ R, const float4* __restrict__ ANC,
+                              float* __restrict__ LAM, float inv_dt2, int first, int j) {
   const int4 row = R[j];
   const int t = row_type(row);
   const int a = row_a(row);
@@ -554,7 +549,6 @@ __global__ __launch_bounds__(BLOCK) void k_gs_color(float4* __restrict__ X,
   const float p1 = __int_as_float(row.w);
   float lam = first ? 0.0f : LAM[j];
   if (first) LAM[j] = 0.0f;
-  const float inv_dt2 = 1.0f / (h * h);
 
   if (t == T_DISTANCE || t == T_ATTACH) {
     float4 xa4 = X[a];
@@ -627,6 +621,136 @@ __global__ __launch_bounds__(BLOCK) void k_gs_color(float4* __restrict__ X,
     X[bb] = with_xyz(xb4, xb);
     return;
   }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_gs_color(float4* __restrict__ X,
+                                                    const float4* __restrict__ XP,
+                                                    const int4* __restrict__ R,
+                                                    const float4* __restrict__ ANC,
+                                                    float* __restrict__ LAM, float h, int first,
+                                                    int start, int count) {
+  const int tid = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (tid >= count) return;
+  gs_row(X, XP, R, ANC, LAM, 1.0f / (h * h), first, start + tid);
+}
+
+// ---------------------------------------------------------------------------
+// Whole frames of a small scene in one workgroup (solver.py:412-467).
+//
+// A scene of a few thousand particles leaves a one-launch-per-phase frame
+// bound by launch latency (a 64-link chain in GS mode is 90 launches a
+// frame).  Here one workgroup keeps the positions in LDS for the whole run
+// and separates the phases with workgroup barriers: integrate, then per
+// iteration either the color slices (GS, in place, gs_row) or a Jacobi pass
+// (the fused tile kernel's row arithmetic and exact fixed-point sums, then
+// the averaged apply), then finalize + cap.  Velocities, x_prev and lambda
+// stay in global memory (each is touched by one thread per phase).  The
+// arithmetic is the launch path's, so GS results are identical bit for bit
+// and Jacobi results equal the tile kernel's.
+// ---------------------------------------------------------------------------
+template <bool GS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_frame_block(
+    float4* __restrict__ X, float4* __restrict__ V, float4* __restrict__ XP,
+    const int4* __restrict__ R, const float4* __restrict__ ANC, float* __restrict__ LAM,
+    const int* __restrict__ color_starts, int num_colors, int n, int m, int substeps,
+    int iterations, float h, float gx, float gy, float gz, float damping, float relax,
+    float max_speed, int do_cap, int frames) {
+  extern __shared__ unsigned long long lds64[];
+  unsigned long long* sx = lds64;
+  unsigned long long* sy = lds64 + n;
+  unsigned long long* sz = lds64 + 2 * n;
+  unsigned* cnt = (unsigned*)(lds64 + 3 * n);
+  float4* spos = GS ? (float4*)lds64
+                    : (float4*)(lds64 + tile_lds_pos_offset(n) / sizeof(unsigned long long));
+  const int tid = threadIdx.x;
+  const float inv_dt2 = 1.0f / (h * h);
+  for (int i = tid; i < n; i += BLOCK) spos[i] = X[i];
+  __syncthreads();
+  auto pos = [&](int i) { return spos[i]; };
+  auto prev = [&](int i) { return xyz(XP[i]); };
+  for (int f = 0; f < frames; ++f) {
+    for (int sub = 0; sub < substeps; ++sub) {
+      // integrate (kernels.py:49-73)
+      for (int i = tid; i < n; i += BLOCK) {
+        const float4 x4 = spos[i];
+        f3 x = xyz(x4);
+        XP[i] = make_float4(x.x, x.y, x.z, 0.0f);
+        if (x4.w != 0.0f) {
+          const f3 v = xyz(V[i]) + mk3(gx, gy, gz) * h;
+          x = x + v * h;
+          V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+          spos[i] = make_float4(x.x, x.y, x.z, x4.w);
+        }
+      }
+      __syncthreads();
+      for (int it = 0; it < iterations && m > 0; ++it) {
+        const int first = it == 0;
+        if (GS) {
+          for (int c = 0; c < num_colors; ++c) {
+            const int c0 = color_starts[c], c1 = color_starts[c + 1];
+            for (int j = c0 + tid; j < c1; j += BLOCK) gs_row(spos, XP, R, ANC, LAM, inv_dt2, first, j);
+            __syncthreads();
+          }
+        } else {
+          for (int i = tid; i < n; i += BLOCK) {
+            sx[i] = 0ull;
+            sy[i] = 0ull;
+            sz[i] = 0ull;
+            cnt[i] = 0u;
+          }
+          __syncthreads();
+          for (int j = tid; j < m; j += BLOCK) {
+            const float lam = first ? 0.0f : LAM[j];
+            const Proj p = project_row<true>(pos, prev, [&]() { return ANC[j]; }, R[j], lam,
+                                             inv_dt2);
+            LAM[j] = p.lam;
+            auto add = [&](int i, f3 c) {
+              const bool ok =
+                  fabsf(c.x) <= kFixLimit && fabsf(c.y) <= kFixLimit && fabsf(c.z) <= kFixLimit;
+              atomicAdd(&sx[i], fix_bits(fminf(fmaxf(c.x, -kFixLimit), kFixLimit)));
+              atomicAdd(&sy[i], fix_bits(fminf(fmaxf(c.y, -kFixLimit), kFixLimit)));
+              atomicAdd(&sz[i], fix_bits(fminf(fmaxf(c.z, -kFixLimit), kFixLimit)));
+              atomicAdd(&cnt[i], ok ? 1u : (kFixPoison | 1u));
+            };
+            if (p.na >= 1) add(p.a, p.ca);
+            if (p.na == 2) add(p.b, p.cb);
+          }
+          __syncthreads();
+          for (int i = tid; i < n; i += BLOCK) {
+            const float4 x4 = spos[i];
+            const unsigned c = cnt[i];
+            const unsigned adds = c & (kFixPoison - 1u);
+            float4 a = make_float4(0.f, 0.f, 0.f, (float)adds);
+            if (c) a = make_float4(unfix(sx[i], adds), unfix(sy[i], adds), unfix(sz[i], adds), a.w);
+            if (c >= kFixPoison) a.x = __int_as_float(0x7fc00000);
+            if (a.w > 0.0f && x4.w > 0.0f) {
+              const f3 x = xyz(x4) + xyz(a) * (relax / a.w);
+              spos[i] = make_float4(x.x, x.y, x.z, x4.w);
+            }
+          }
+          __syncthreads();
+        }
+      }
+      // finalize_velocity + cap_velocity (kernels.py:346-371), per particle
+      for (int i = tid; i < n; i += BLOCK) {
+        const float4 x4 = spos[i];
+        f3 v;
+        if (x4.w == 0.0f) {
+          v = mk3(0.0f, 0.0f, 0.0f);
+        } else {
+          v = ((xyz(x4) - xyz(XP[i])) / h) * (1.0f - damping);
+        }
+        if (do_cap) {
+          const float speed = length(v);
+          if (speed > max_speed) v = v * (max_speed / speed);
+        }
+        V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < n; i += BLOCK) X[i] = spos[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -871,6 +995,39 @@ int xpbd_solve_jacobi_dyn(void* stream, const float* X, const float* XP, const i
   hipLaunchKernelGGL(k_jacobi_project_dyn<kBlock>, dim3(blocks_for(cap, kBlock)), dim3(kBlock), 0,
                      (hipStream_t)stream, (const float4*)X, (const float4*)XP, (const int4*)R,
                      LAM, (float4*)ACC, h, first, n_rows_dev, cap);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_frame_block_capacity(int gs) {
+  // dynamic LDS within the default 64 KB: 16 B (GS) or 44 B (Jacobi) per particle
+  return gs ? 4096 : (int)((65536 - 16) / (3 * sizeof(long long) + sizeof(unsigned) +
+                                          sizeof(float4)));
+}
+
+int xpbd_frame_block(void* stream, float* X, float* V, float* XP, const int* R, const float* ANC,
+                     float* LAM, const int* color_starts, int num_colors, int n_particles,
+                     int n_rows, int substeps, int iterations, float h, float gx, float gy,
+                     float gz, float damping, float relax, float max_speed, int gs, int frames) {
+  if (n_particles <= 0 || frames <= 0) return 0;
+  if (n_particles > xpbd_frame_block_capacity(gs) || substeps < 0 || iterations < 0 ||
+      (gs && n_rows > 0 && (!color_starts || num_colors < 1)))
+    return (int)hipErrorInvalidValue;
+  constexpr int kWG = 1024;
+  const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
+  hipStream_t s = (hipStream_t)stream;
+  if (gs)
+    hipLaunchKernelGGL((k_frame_block<true, kWG>), dim3(1), dim3(kWG),
+                       sizeof(float4) * (size_t)n_particles, s, (float4*)X, (float4*)V,
+                       (float4*)XP, (const int4*)R, (const float4*)ANC, LAM, color_starts,
+                       num_colors, n_particles, n_rows, substeps, iterations, h, gx, gy, gz,
+                       damping, relax, max_speed, do_cap, frames);
+  else
+    hipLaunchKernelGGL((k_frame_block<false, kWG>), dim3(1), dim3(kWG),
+                       tile_lds_bytes(n_particles), s, (float4*)X, (float4*)V, (float4*)XP,
+                       (const int4*)R, (const float4*)ANC, LAM, color_starts, num_colors,
+                       n_particles, n_rows, substeps, iterations, h, gx, gy, gz, damping, relax,
+                       max_speed, do_cap, frames);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

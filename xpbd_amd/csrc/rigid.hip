@@ -51,14 +51,11 @@ __device__ inline void accumulate_correction(float4* DX, float4* DR, int a, int 
 // kernels_6dof.py:67-102 (+ per-substep lambda_floor reset and contact
 // point counter reset, both exact rewrites of solver_6dof.py:374-375)
 // gscale (optional): per-body gravity factor (batched independent scenes)
-__global__ __launch_bounds__(kB) void k6_integrate(float4* X, float4* Q, float4* V, float4* W,
+__device__ inline void integrate_elem(int i, float4* X, float4* Q, float4* V, float4* W,
                                                    float4* XP, float4* QP, const float4* IB,
-                                                   float* LAMF, int* PTC, float gx, float gy,
+                                                   float* LAMF, float gx, float gy,
                                                    float gz, float h, int gyro,
                                                    const float* __restrict__ gscale, int n) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i == 0 && PTC) *PTC = 0;
-  if (i >= n) return;
   float4 x4 = X[i];
   q4 q = ldq(Q[i]);
   XP[i] = make_float4(x4.x, x4.y, x4.z, 0.0f);
@@ -88,6 +85,17 @@ __global__ __launch_bounds__(kB) void k6_integrate(float4* X, float4* Q, float4*
   q = quat_integrate(q, w, h);
   X[i] = make_float4(x.x, x.y, x.z, x4.w);
   Q[i] = stq(q);
+}
+
+__global__ __launch_bounds__(kB) void k6_integrate(float4* X, float4* Q, float4* V, float4* W,
+                                                   float4* XP, float4* QP, const float4* IB,
+                                                   float* LAMF, int* PTC, float gx, float gy,
+                                                   float gz, float h, int gyro,
+                                                   const float* __restrict__ gscale, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i == 0 && PTC) *PTC = 0;
+  if (i >= n) return;
+  integrate_elem(i, X, Q, V, W, XP, QP, IB, LAMF, gx, gy, gz, h, gyro, gscale, n);
 }
 
 // kernels_6dof.py:226-230
@@ -235,12 +243,10 @@ __global__ __launch_bounds__(kB) void k6_box_manifold(
 }
 
 // kernels_6dof.py:440-483 (lambda reset folded into the first iteration)
-__global__ __launch_bounds__(kB) void k6_solve_joints(
+__device__ inline void solve_joints_elem(int k, 
     const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
     const int2* __restrict__ JI, const float4* __restrict__ JA, const float4* __restrict__ JB,
     float* __restrict__ LAMJ, float h, int first, float4* DX, float4* DR, int nj) {
-  const int k = blockIdx.x * kB + threadIdx.x;
-  if (k >= nj) return;
   const int2 ab = JI[k];
   const int a = ab.x, b = ab.y;
   const float4 ja = JA[k], jb = JB[k];
@@ -268,14 +274,22 @@ __global__ __launch_bounds__(kB) void k6_solve_joints(
   accumulate_correction(DX, DR, a, b, xa4.w, xb4.w, qa, qb, iIa, iIb, ra, rb, p);
 }
 
+__global__ __launch_bounds__(kB) void k6_solve_joints(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const int2* __restrict__ JI, const float4* __restrict__ JA, const float4* __restrict__ JB,
+    float* __restrict__ LAMJ, float h, int first, float4* DX, float4* DR, int nj) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= nj) return;
+  solve_joints_elem(k, X, Q, IB, JI, JA, JB, LAMJ, h, first, DX, DR, nj);
+}
+
 // kernels_6dof.py:158-211.  One thread owns one body, so its corners add to
 // DX/DR in corner order with plain loads and stores.
-__global__ __launch_bounds__(kB) void k6_solve_floor(
+template <bool ATOMIC>
+__device__ inline void solve_floor_elem(int i, 
     const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ XP,
     const float4* __restrict__ QP, const float4* __restrict__ IB, const float4* __restrict__ HE,
     float floor_y, float mu_s, float* __restrict__ LAMF, float4* DX, float4* DR, int n) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
   const float4 x4 = X[i];
   const float im = x4.w;
   if (im == 0.0f) return;
@@ -284,7 +298,10 @@ __global__ __launch_bounds__(kB) void k6_solve_floor(
   const f3 iI = xyz(IB[i]);
   const f3 he = xyz(HE[i]);
   const f3 s = mk3(0.0f, 1.0f, 0.0f);
-  float4 dx = DX[i], dr = DR[i];
+  // ATOMIC (launches shared with other row kinds): sum this body's corners
+  // from zero and add the sums atomically; otherwise one thread owns DX[i]
+  float4 dx = ATOMIC ? make_float4(0.f, 0.f, 0.f, 0.f) : DX[i];
+  float4 dr = ATOMIC ? make_float4(0.f, 0.f, 0.f, 0.f) : DR[i];
   bool touched = false;
   float lam[8];
   for (int c = 0; c < 8; ++c) {
@@ -324,19 +341,32 @@ __global__ __launch_bounds__(kB) void k6_solve_floor(
   lf[0] = make_float4(lam[0], lam[1], lam[2], lam[3]);
   lf[1] = make_float4(lam[4], lam[5], lam[6], lam[7]);
   if (touched) {
-    DX[i] = dx;
-    DR[i] = dr;
+    if (ATOMIC) {
+      atomic_add3(&DX[i], xyz(dx));
+      atomicAdd(&DX[i].w, dx.w);
+      atomic_add3(&DR[i], xyz(dr));
+    } else {
+      DX[i] = dx;
+      DR[i] = dr;
+    }
   }
 }
 
+__global__ __launch_bounds__(kB) void k6_solve_floor(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ XP,
+    const float4* __restrict__ QP, const float4* __restrict__ IB, const float4* __restrict__ HE,
+    float floor_y, float mu_s, float* __restrict__ LAMF, float4* DX, float4* DR, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  solve_floor_elem<false>(i, X, Q, XP, QP, IB, HE, floor_y, mu_s, LAMF, DX, DR, n);
+}
+
 // kernels_6dof.py:385-430, one thread per compacted manifold point
-__global__ __launch_bounds__(kB) void k6_solve_box(
+__device__ inline void solve_box_elem(int k, 
     const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
     const int4* __restrict__ PTI, const float4* __restrict__ PTN,
     const float4* __restrict__ PTOI, const float4* __restrict__ PTOR, float* __restrict__ LAMP,
     const int* __restrict__ PTC, float4* DX, float4* DR, int max_points) {
-  const int k = blockIdx.x * kB + threadIdx.x;
-  if (k >= min(*PTC, max_points)) return;
   const int4 id = PTI[k];
   const int inc = id.y, ref = id.z;
   const f3 s = xyz(PTN[k]);
@@ -362,11 +392,19 @@ __global__ __launch_bounds__(kB) void k6_solve_box(
   accumulate_correction(DX, DR, inc, ref, xi4.w, xr4.w, qi, qr, iIi, iIr, r_inc, r_ref, p);
 }
 
+__global__ __launch_bounds__(kB) void k6_solve_box(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const int4* __restrict__ PTI, const float4* __restrict__ PTN,
+    const float4* __restrict__ PTOI, const float4* __restrict__ PTOR, float* __restrict__ LAMP,
+    const int* __restrict__ PTC, float4* DX, float4* DR, int max_points) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= min(*PTC, max_points)) return;
+  solve_box_elem(k, X, Q, IB, PTI, PTN, PTOI, PTOR, LAMP, PTC, DX, DR, max_points);
+}
+
 // kernels_6dof.py:486-506
-__global__ __launch_bounds__(kB) void k6_apply_jacobi(float4* X, float4* Q, float4* DX,
+__device__ inline void apply_jacobi_elem(int i, float4* X, float4* Q, float4* DX,
                                                       float4* DR, float relax, int n) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
   const float4 acc = DX[i];
   const float4 x4 = X[i];
   if (acc.w > 0.0f && x4.w > 0.0f) {
@@ -384,15 +422,20 @@ __global__ __launch_bounds__(kB) void k6_apply_jacobi(float4* X, float4* Q, floa
   }
 }
 
+__global__ __launch_bounds__(kB) void k6_apply_jacobi(float4* X, float4* Q, float4* DX,
+                                                      float4* DR, float relax, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  apply_jacobi_elem(i, X, Q, DX, DR, relax, n);
+}
+
 // kernels_6dof.py:512-535
-__global__ __launch_bounds__(kB) void k6_finalize(const float4* __restrict__ X,
+__device__ inline void finalize_elem(int i, const float4* __restrict__ X,
                                                   const float4* __restrict__ Q,
                                                   const float4* __restrict__ XP,
                                                   const float4* __restrict__ QP, float4* V,
                                                   float4* W, float h, float lin_damp,
                                                   float ang_damp, int n) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
   const float4 x4 = X[i];
   if (x4.w == 0.0f) {
     V[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -406,6 +449,17 @@ __global__ __launch_bounds__(kB) void k6_finalize(const float4* __restrict__ X,
   w = w * (1.0f - ang_damp);
   V[i] = make_float4(v.x, v.y, v.z, 0.0f);
   W[i] = make_float4(w.x, w.y, w.z, 0.0f);
+}
+
+__global__ __launch_bounds__(kB) void k6_finalize(const float4* __restrict__ X,
+                                                  const float4* __restrict__ Q,
+                                                  const float4* __restrict__ XP,
+                                                  const float4* __restrict__ QP, float4* V,
+                                                  float4* W, float h, float lin_damp,
+                                                  float ang_damp, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  finalize_elem(i, X, Q, XP, QP, V, W, h, lin_damp, ang_damp, n);
 }
 
 // Shared tail of velocity_floor / velocity_box (kernels_6dof.py:575-583):
@@ -426,13 +480,12 @@ __device__ inline f3 velocity_impulse(f3 vrel, f3 s, float lam_n, float mu_d, fl
 }
 
 // kernels_6dof.py:541-590; one thread owns one body (plain RMW of DV/DW)
-__global__ __launch_bounds__(kB) void k6_velocity_floor(
+template <bool ATOMIC>
+__device__ inline void velocity_floor_elem(int i, 
     const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
     const float4* __restrict__ HE, float mu_d, float restitution,
     const float* __restrict__ LAMF, float h, const float4* __restrict__ V,
     const float4* __restrict__ W, float4* DV, float4* DW, int n) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
   const float im = X[i].w;
   if (im == 0.0f) return;
   const float4* lf4 = (const float4*)(LAMF + 8 * (size_t)i);
@@ -442,7 +495,8 @@ __global__ __launch_bounds__(kB) void k6_velocity_floor(
   const f3 iI = xyz(IB[i]), he = xyz(HE[i]);
   const f3 v = xyz(V[i]), w = xyz(W[i]);
   const f3 s = mk3(0.0f, 1.0f, 0.0f);
-  float4 dv = DV[i], dw = DW[i];
+  float4 dv = ATOMIC ? make_float4(0.f, 0.f, 0.f, 0.f) : DV[i];
+  float4 dw = ATOMIC ? make_float4(0.f, 0.f, 0.f, 0.f) : DW[i];
   bool touched = false;
   for (int c = 0; c < 8; ++c) {
     const float lam_n = lam[c];
@@ -462,21 +516,35 @@ __global__ __launch_bounds__(kB) void k6_velocity_floor(
     }
   }
   if (touched) {
-    DV[i] = dv;
-    DW[i] = dw;
+    if (ATOMIC) {
+      atomic_add3(&DV[i], xyz(dv));
+      atomicAdd(&DV[i].w, dv.w);
+      atomic_add3(&DW[i], xyz(dw));
+    } else {
+      DV[i] = dv;
+      DW[i] = dw;
+    }
   }
 }
 
+__global__ __launch_bounds__(kB) void k6_velocity_floor(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const float4* __restrict__ HE, float mu_d, float restitution,
+    const float* __restrict__ LAMF, float h, const float4* __restrict__ V,
+    const float4* __restrict__ W, float4* DV, float4* DW, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  velocity_floor_elem<false>(i, X, Q, IB, HE, mu_d, restitution, LAMF, h, V, W, DV, DW, n);
+}
+
 // kernels_6dof.py:593-659, one thread per compacted manifold point
-__global__ __launch_bounds__(kB) void k6_velocity_box(
+__device__ inline void velocity_box_elem(int k, 
     const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
     const int4* __restrict__ PTI, const float4* __restrict__ PTN,
     const float4* __restrict__ PTOI, const float4* __restrict__ PTOR,
     const float* __restrict__ LAMP, const int* __restrict__ PTC, float mu_d, float restitution,
     float h, const float4* __restrict__ V, const float4* __restrict__ W, float4* DV, float4* DW,
     int max_points) {
-  const int k = blockIdx.x * kB + threadIdx.x;
-  if (k >= min(*PTC, max_points)) return;
   const float lam_n = LAMP[k];
   if (lam_n == 0.0f) return;
   const int4 id = PTI[k];
@@ -513,13 +581,23 @@ __global__ __launch_bounds__(kB) void k6_velocity_box(
   }
 }
 
+__global__ __launch_bounds__(kB) void k6_velocity_box(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const int4* __restrict__ PTI, const float4* __restrict__ PTN,
+    const float4* __restrict__ PTOI, const float4* __restrict__ PTOR,
+    const float* __restrict__ LAMP, const int* __restrict__ PTC, float mu_d, float restitution,
+    float h, const float4* __restrict__ V, const float4* __restrict__ W, float4* DV, float4* DW,
+    int max_points) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= min(*PTC, max_points)) return;
+  velocity_box_elem(k, X, Q, IB, PTI, PTN, PTOI, PTOR, LAMP, PTC, mu_d, restitution, h, V, W, DV, DW, max_points);
+}
+
 // kernels_6dof.py:662-691: apply_velocity_6dof then cap_velocity_6dof
-__global__ __launch_bounds__(kB) void k6_apply_velocity(const float4* __restrict__ X, float4* V,
+__device__ inline void apply_velocity_elem(int i, const float4* __restrict__ X, float4* V,
                                                         float4* W, float4* DV, float4* DW,
                                                         float max_lin, float max_ang, int do_cap,
                                                         int n) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
   const float4 acc = DV[i];
   f3 v = xyz(V[i]), w = xyz(W[i]);
   bool dirty = false;
@@ -542,6 +620,92 @@ __global__ __launch_bounds__(kB) void k6_apply_velocity(const float4* __restrict
   if (dirty) {
     V[i] = make_float4(v.x, v.y, v.z, 0.0f);
     W[i] = make_float4(w.x, w.y, w.z, 0.0f);
+  }
+}
+
+__global__ __launch_bounds__(kB) void k6_apply_velocity(const float4* __restrict__ X, float4* V,
+                                                        float4* W, float4* DV, float4* DW,
+                                                        float max_lin, float max_ang, int do_cap,
+                                                        int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  apply_velocity_elem(i, X, V, W, DV, DW, max_lin, max_ang, do_cap, n);
+}
+
+// ---------------------------------------------------------------------------
+// Fused launches of the substep (frame plan): consecutive per-body kernels
+// run as one pass, and the row kinds of one Jacobi sweep (joints, floor
+// corners, box points) as one launch over their concatenated index ranges.
+// Rows of different kinds then add into the same bodies concurrently, so the
+// floor rows add their per-body sums atomically too; the accumulation order
+// of a Jacobi sweep is arbitrary in the reference as well (atomic_add).
+// ---------------------------------------------------------------------------
+
+// apply_velocity + cap of the previous substep (apply != 0), then integrate
+__global__ __launch_bounds__(kB) void k6_begin_substep(
+    float4* X, float4* Q, float4* V, float4* W, float4* XP, float4* QP, const float4* IB,
+    float* LAMF, int* PTC, float4* DV, float4* DW, float max_lin, float max_ang, int do_cap,
+    int apply, float gx, float gy, float gz, float h, int gyro, const float* __restrict__ gscale,
+    int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i == 0 && PTC) *PTC = 0;
+  if (i >= n) return;
+  if (apply) apply_velocity_elem(i, X, V, W, DV, DW, max_lin, max_ang, do_cap, n);
+  integrate_elem(i, X, Q, V, W, XP, QP, IB, LAMF, gx, gy, gz, h, gyro, gscale, n);
+}
+
+// joints [0, nj), floor bodies [nj, nj + n), box points [nj + n, nj + n + max_points)
+__global__ __launch_bounds__(kB) void k6_solve_all(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ XP,
+    const float4* __restrict__ QP, const float4* __restrict__ IB, const float4* __restrict__ HE,
+    const int2* __restrict__ JI, const float4* __restrict__ JA, const float4* __restrict__ JB,
+    float* __restrict__ LAMJ, float h, int first, int nj, float floor_y, float mu_s,
+    float* __restrict__ LAMF, int n, const int4* __restrict__ PTI,
+    const float4* __restrict__ PTN, const float4* __restrict__ PTOI,
+    const float4* __restrict__ PTOR, float* __restrict__ LAMP, const int* __restrict__ PTC,
+    int max_points, float4* DX, float4* DR) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k < nj) {
+    solve_joints_elem(k, X, Q, IB, JI, JA, JB, LAMJ, h, first, DX, DR, nj);
+  } else if (k < nj + n) {
+    solve_floor_elem<true>(k - nj, X, Q, XP, QP, IB, HE, floor_y, mu_s, LAMF, DX, DR, n);
+  } else {
+    const int p = k - nj - n;
+    if (p < min(*PTC, max_points))
+      solve_box_elem(p, X, Q, IB, PTI, PTN, PTOI, PTOR, LAMP, PTC, DX, DR, max_points);
+  }
+}
+
+// apply_jacobi, then finalize_velocity on the last iteration
+__global__ __launch_bounds__(kB) void k6_apply_finalize(float4* X, float4* Q, float4* DX,
+                                                        float4* DR, float relax, int finalize,
+                                                        const float4* __restrict__ XP,
+                                                        const float4* __restrict__ QP, float4* V,
+                                                        float4* W, float h, float lin_damp,
+                                                        float ang_damp, int n) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  apply_jacobi_elem(i, X, Q, DX, DR, relax, n);
+  if (finalize) finalize_elem(i, X, Q, XP, QP, V, W, h, lin_damp, ang_damp, n);
+}
+
+// velocity pass: floor bodies [0, n), box points [n, n + max_points)
+__global__ __launch_bounds__(kB) void k6_velocity_all(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const float4* __restrict__ HE, float mu_d, float restitution, const float* __restrict__ LAMF,
+    float h, const float4* __restrict__ V, const float4* __restrict__ W, int n,
+    const int4* __restrict__ PTI, const float4* __restrict__ PTN,
+    const float4* __restrict__ PTOI, const float4* __restrict__ PTOR,
+    const float* __restrict__ LAMP, const int* __restrict__ PTC, int max_points, float4* DV,
+    float4* DW) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k < n) {
+    velocity_floor_elem<true>(k, X, Q, IB, HE, mu_d, restitution, LAMF, h, V, W, DV, DW, n);
+  } else {
+    const int p = k - n;
+    if (p < min(*PTC, max_points))
+      velocity_box_elem(p, X, Q, IB, PTI, PTN, PTOI, PTOR, LAMP, PTC, mu_d, restitution, h, V, W,
+                        DV, DW, max_points);
   }
 }
 
@@ -651,6 +815,60 @@ int xpbd6_velocity_box(void* stream, const float* X, const float* Q, const float
                      CF4(X), CF4(Q), CF4(IB), (const int4*)PTI, CF4(PTN), CF4(PTOI), CF4(PTOR),
                      LAMP, PTC, mu_d, restitution, h, CF4(V), CF4(W), F4(DV), F4(DW),
                      max_points);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_begin_substep(void* stream, float* X, float* Q, float* V, float* W, float* XP,
+                        float* QP, const float* IB, float* LAMF, int* PTC, float* DV, float* DW,
+                        float max_lin, float max_ang, int do_cap, int apply, float gx, float gy,
+                        float gz, float h, int gyro, const float* gscale, int n) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k6_begin_substep, dim3(nblk(n)), dim3(kB), 0, (hipStream_t)stream, F4(X),
+                     F4(Q), F4(V), F4(W), F4(XP), F4(QP), CF4(IB), LAMF, PTC, F4(DV), F4(DW),
+                     max_lin, max_ang, do_cap, apply, gx, gy, gz, h, gyro, gscale, n);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_solve_all(void* stream, const float* X, const float* Q, const float* XP,
+                    const float* QP, const float* IB, const float* HE, const int* JI,
+                    const float* JA, const float* JB, float* LAMJ, float h, int first, int nj,
+                    float floor_y, float mu_s, float* LAMF, int n, const int* PTI,
+                    const float* PTN, const float* PTOI, const float* PTOR, float* LAMP,
+                    const int* PTC, int max_points, float* DX, float* DR) {
+  const long long total = (long long)nj + n + max_points;
+  if (total <= 0) return 0;
+  hipLaunchKernelGGL(k6_solve_all, dim3(nblk(total)), dim3(kB), 0, (hipStream_t)stream, CF4(X),
+                     CF4(Q), CF4(XP), CF4(QP), CF4(IB), CF4(HE), (const int2*)JI, CF4(JA),
+                     CF4(JB), LAMJ, h, first, nj, floor_y, mu_s, LAMF, n, (const int4*)PTI,
+                     CF4(PTN), CF4(PTOI), CF4(PTOR), LAMP, PTC, max_points, F4(DX), F4(DR));
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_apply_finalize(void* stream, float* X, float* Q, float* DX, float* DR, float relax,
+                         int finalize, const float* XP, const float* QP, float* V, float* W,
+                         float h, float lin_damp, float ang_damp, int n) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k6_apply_finalize, dim3(nblk(n)), dim3(kB), 0, (hipStream_t)stream, F4(X),
+                     F4(Q), F4(DX), F4(DR), relax, finalize, CF4(XP), CF4(QP), F4(V), F4(W), h,
+                     lin_damp, ang_damp, n);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_velocity_all(void* stream, const float* X, const float* Q, const float* IB,
+                       const float* HE, float mu_d, float restitution, const float* LAMF,
+                       float h, const float* V, const float* W, int n, const int* PTI,
+                       const float* PTN, const float* PTOI, const float* PTOR, const float* LAMP,
+                       const int* PTC, int max_points, float* DV, float* DW) {
+  const long long total = (long long)n + max_points;
+  if (total <= 0) return 0;
+  hipLaunchKernelGGL(k6_velocity_all, dim3(nblk(total)), dim3(kB), 0, (hipStream_t)stream,
+                     CF4(X), CF4(Q), CF4(IB), CF4(HE), mu_d, restitution, LAMF, h, CF4(V), CF4(W),
+                     n, (const int4*)PTI, CF4(PTN), CF4(PTOI), CF4(PTOR), LAMP, PTC, max_points,
+                     F4(DV), F4(DW));
   XPBD_LAUNCH_CHECK();
   return 0;
 }

@@ -121,6 +121,27 @@ def cloth_bench(solver_cls, res: int, mode: str, substeps: int = 15, iterations:
     return s
 
 
+def sphere_stack(solver_cls, n_side: int, rows: int, mode: str, substeps: int = 15,
+                 iterations: int = 2, **kw):
+    """examples/benchmark.py:92-107: rows of n_side^2 spheres (r = 0.16 m,
+    1 kg, mu 0.4) on a floor, self-collision on."""
+    from .scene import Shape
+
+    s = solver_cls(dt=1 / 60, substeps=substeps, iterations=iterations, solve_mode=mode, **kw)
+    rng = np.random.default_rng(0)
+    r = 0.16
+    for ly in range(rows):
+        for ix in range(n_side):
+            for iz in range(n_side):
+                b = s.add_particle(((ix - n_side / 2) * 2.3 * r + rng.uniform(-2e-3, 2e-3),
+                                    r + ly * 2.3 * r,
+                                    (iz - n_side / 2) * 2.3 * r + rng.uniform(-2e-3, 2e-3)),
+                                   mass=1.0, shape=Shape("sphere", (r,)), friction=0.4)
+                s.add_floor_contact(b, floor_y=r, friction=0.4)
+    s.enable_self_collision(True, default_friction=0.4)
+    return s
+
+
 def _part1by1(v):
     """Spread the low 16 bits of an int64 tensor to even bit positions."""
     v = v & 0xFFFF

@@ -250,11 +250,12 @@ def reference_rows(frames: int = 120, warmup: int = 20) -> dict:
 def warm_start_section(res: int, frames: int) -> dict:
     """Learned lambda warm start on the cloth: fit (two rounds, on the GPU),
     RMS residual of each start on its own trajectory, and whole-frame
-    substeps/s of the captured frame plans (plain zero start with the window
-    and the tile kernel, learned start with the window kernel)."""
+    substeps/s of the captured frame plans (tile kernel; the learned start
+    runs inside each substep's first launch), each plan autotuned."""
     import torch
 
     from xpbd_amd import Solver
+    from xpbd_amd import autotune
     from xpbd_amd import warmstart as ws
     from xpbd_amd.scenes import cloth_sheet
 
@@ -271,27 +272,31 @@ def warm_start_section(res: int, frames: int) -> dict:
     s.step(20)
     resid, fallback = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(1, 2, 4),
                                                  learned_iters=(1, 2))
-    sps = {}
-    for name, iters, kernel, learned in [("plain_window_4", 4, "window", False),
-                                         ("plain_window_2", 2, "window", False),
-                                         ("plain_tile_4", 4, "tile", False),
-                                         ("plain_tile_2", 2, "tile", False),
-                                         ("learned_window_2", 2, "window", True),
-                                         ("learned_window_1", 1, "window", True)]:
+    sps, shapes = {}, {}
+    for name, iters, learned in [("plain_4", 4, False), ("plain_2", 2, False),
+                                 ("plain_1", 1, False), ("learned_2", 2, True),
+                                 ("learned_1", 1, True)]:
         c = cloth(iters)
-        c.jacobi_kernel = kernel
+        c.block_frames = False
         if learned:
             c.set_warm_start(theta)
+        best, _ = autotune.tune(c)
+        shapes[name] = list(best) if best else None
         ms = time_frames(c.step, frames, 5)
         sps[name] = round(c.substeps / (ms * 1e-3), 1)
     return {
-        "scene": f"cloth {res}x{res}, 15 substeps, Jacobi",
+        "scene": f"cloth {res}x{res}, 15 substeps, Jacobi tile kernel",
         "fit_seconds": round(fit_s, 2), "fit": log[0] if log else None,
         "rms_residual": {k: float(f"{v:.4e}") for k, v in resid.items()},
         "fallback_substeps_of_60": fallback,
         "substeps_per_s": sps,
-        "learned2_vs_plain4_window": round(sps["learned_window_2"] / sps["plain_window_4"], 3),
-        "learned2_vs_plain4_tile": round(sps["learned_window_2"] / sps["plain_tile_4"], 3),
+        "launch_shapes": shapes,
+        "learned1_vs_plain2": {"residual_ratio": round(resid["learned_1"] / resid["plain_2"], 3),
+                               "speedup": round(sps["learned_1"] / sps["plain_2"], 3)},
+        "learned2_vs_plain4": {"residual_ratio": round(resid["learned_2"] / resid["plain_4"], 3),
+                               "speedup": round(sps["learned_2"] / sps["plain_4"], 3)},
+        "learned1_vs_plain4": {"residual_ratio": round(resid["learned_1"] / resid["plain_4"], 3),
+                               "speedup": round(sps["learned_1"] / sps["plain_4"], 3)},
     }
 
 

@@ -43,6 +43,9 @@ extern "C" {
 /* Largest particle tile of the fused Jacobi iteration (44 B of LDS per particle:
  * three 64-bit fixed-point sums, a count and the staged position). */
 #define XPBD_MAX_TILE 1024
+/* Largest tile of the warm-start launch (the network's weights and per-wave
+ * tiles take another 28 KB of LDS; the total stays within 64 KB). */
+#define XPBD_MAX_WARM_TILE 512
 
 /* ---- 3-DOF particle kernels ------------------------------------------- */
 
@@ -144,6 +147,24 @@ int xpbd_frame_block(void* stream, float* X, float* V, float* XP, const int* R, 
                      float* LAM, const int* color_starts, int num_colors, int n_particles,
                      int n_rows, int substeps, int iterations, float h, float gx, float gy,
                      float gz, float damping, float relax, float max_speed, int gs, int frames);
+/* Learned warm start of a substep (no reference counterpart; it replaces the
+ * zero start of solver.py:431), launched before the substep's Jacobi
+ * iterations, which then run with first = 0 from LAMout.  Every row
+ * evaluates the MLP theta on its features (as xpbd_ws_features computes
+ * them) and starts from lambda0 = 4 sigmoid(y) (-C / (w + alpha~)) (one-sided
+ * rows only while violated), written to LAMout; each particle moves by relax
+ * x the mean over its rows of w_i grad C lambda0 (Xin -> Xout, Xin !=
+ * Xout).  predict = 1 folds integrate in as xpbd_jacobi_tile_first does (Xin
+ * = positions at the start of the substep; XP written; V read), predict = 0
+ * reads integrated positions and XP.  Y (optional, float[n_rows]) receives
+ * the network output of each row.  Tile tables as xpbd_jacobi_tile;
+ * tile_size <= XPBD_MAX_WARM_TILE. */
+int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP, const float* V,
+                          const int* R, const float* ANC, float* LAMout,
+                          const int* tile_row_start, const int* halo_start, const int* halo_rows,
+                          const float* theta, float* Y, float h, float gx, float gy, float gz,
+                          const float* gscale, int tile_size, int n_particles, float relax,
+                          int predict);
 /* Largest particle count of xpbd_frame_block (LDS-bound). */
 int xpbd_frame_block_capacity(int gs);
 
@@ -190,11 +211,12 @@ typedef struct xpbd_particles_desc {
   const int* dyn_count;  /* int[1], device */
   int dyn_cap;
   const float* gscale;   /* float[n_particles] gravity factor, or NULL (tile mode) */
-  /* learned warm start (XPBD_MODE_JACOBI): MLP parameters, or NULL for the
-   * reference's zero start; feature and output scratch */
+  /* learned warm start (XPBD_MODE_JACOBI_TILE): MLP parameters, or NULL for
+   * the reference's zero start (xpbd_jacobi_tile_warm runs the first
+   * iteration of every substep); ws_feat and ws_y are unused */
   const float* ws_theta; /* float[5057] */
-  float* ws_feat;        /* float[n_rows * 12] */
-  float* ws_y;           /* float[n_rows] */
+  float* ws_feat;
+  float* ws_y;
   /* Tile mode with >= 2 iterations and no dyn rows: 1 folds integrate into
    * the first iteration (xpbd_jacobi_tile_first), 0 keeps a separate
    * xpbd_integrate_to launch per substep.  Results are identical. */
@@ -369,17 +391,12 @@ int xpbd_mlp_grad(void* stream, const float* theta, const float* F, const float*
 int xpbd_mlp_adam(void* stream, float* theta, float* mom, float* vel, const float* grad,
                   float lr, float beta1, float beta2, float eps, int step);
 
-/* Warm-start rows at the start of a substep (features of the MLP,
- * float[m][12]; LAM may be NULL, the previous-substep multiplier feature is
- * then zero), lambda0 from the MLP output Y written to LAM with w grad C
- * lambda0 and a count of one per endpoint summed into ACC (averaged by
- * xpbd_particles_pass flag 1, as Jacobi averages), and the squared-residual
- * sum (r = C + alpha~ lambda for two-sided rows, max(0, -C) for one-sided
- * rows) added to *out_sum. */
-int xpbd_ws_features(void* stream, const float* X, const float* V, const int* R, const float* ANC,
-                     const float* LAM, float h, float* F, int n_rows);
-int xpbd_ws_apply(void* stream, const float* X, const float* V, const int* R, const float* ANC,
-                  const float* Y, float h, float* LAM, float* ACC, int n_rows);
+/* Row features of the MLP (float[m][12]) at the predicted positions X with
+ * x_prev XP (velocities (X - XP) / h), as the warm-start launch computes
+ * them; and the squared-residual sum (r = C + alpha~ lambda for two-sided
+ * rows, max(0, -C) for one-sided rows) added to *out_sum. */
+int xpbd_ws_features(void* stream, const float* X, const float* XP, const int* R,
+                     const float* ANC, float h, float* F, int n_rows);
 int xpbd_ws_residual(void* stream, const float* X, const int* R, const float* ANC,
                      const float* LAM, float h, double* out_sum, int n_rows);
 

@@ -1,7 +1,8 @@
 """GPU tests of the learned lambda warm start (no reference counterpart):
-training data generated by the solver itself, training with the library's
-own BF16 MFMA kernels, and what the trained network gains over the zero
-start at the same iteration count."""
+the warm-start Jacobi launch against the float64 oracle and against the plain
+launch, training data generated by the solver itself, training with the
+library's own BF16 MFMA kernels, and what the trained network gains over the
+zero start."""
 import numpy as np
 import pytest
 
@@ -10,11 +11,173 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
 
 
-def _cloth(res=32):
+def _cloth(res=32, **kw):
     from xpbd_amd import Solver
     from xpbd_amd.scenes import cloth_sheet
 
-    return cloth_sheet(Solver, res=res, solve_mode="jacobi", device=DEV)[0]
+    return cloth_sheet(Solver, res=res, solve_mode="jacobi", device=DEV, **kw)[0]
+
+
+def _theta(seed=0, b3=None):
+    import torch
+
+    from xpbd_amd import warmstart as ws
+
+    th = ws.init_params(seed)
+    th[-1] = 0.0 if b3 is None else b3
+    return torch.from_numpy(th).to(DEV)
+
+
+def _warm(s, src, dst, lam, theta, Y=None, predict=0):
+    from xpbd_amd import _native as N
+
+    h = float(np.float32(s.dt / s.substeps))
+    trs, hs, hr = (N.ptr(t) for t in s._tile_dev)
+    N.check(N.lib().xpbd_jacobi_tile_warm(
+        None, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), N.ptr(s.ANC),
+        N.ptr(lam), trs, hs, hr, N.ptr(theta), N.ptr(Y) if Y is not None else None, h,
+        *s.gravity, None, s._tile, s.num_bodies, s.jacobi_relax, predict))
+
+
+def _perturbed_state(s, seed, below_floor=False):
+    """Positions jittered, x_prev = x - v h with random velocities."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    h = np.float32(s.dt / s.substeps)
+    X = s.X.cpu().numpy().copy()
+    X[:, :3] += rng.normal(0, 2e-3, (len(X), 3)).astype(np.float32)
+    if below_floor:
+        X[:, 1] -= 0.5 + rng.normal(0, 2e-3, len(X)).astype(np.float32)
+    V = rng.normal(0, 0.3, (len(X), 3)).astype(np.float32)
+    XP = X.copy()
+    XP[:, :3] = X[:, :3] - V * h
+    XP[:, 3] = 0.0
+    s.X.copy_(torch.from_numpy(X))
+    s.XP.copy_(torch.from_numpy(XP))
+    s.V[:, :3] = torch.from_numpy(V).to(DEV)
+    return X, XP
+
+
+def test_warm_launch_matches_oracle():
+    """One warm-start launch on a jittered 24^2 cloth, half of it below the
+    floor, with random velocities and a random network: (1) every row's
+    network output equals xpbd_ws_features -> xpbd_mlp_forward bit for bit;
+    (2) those features match the float64 oracle; (3) lambda0 and the moved
+    positions match oracle/ws_ref.warm_launch given the same outputs."""
+    import torch
+
+    from oracle import ws_ref
+    from xpbd_amd import _native as N
+    from xpbd_amd import warmstart as ws
+
+    s = _cloth(24)
+    ws._prepare(s)
+    X, XP = _perturbed_state(s, 5, below_floor=True)
+    m = s.num_constraints
+    h = float(np.float32(s.dt / s.substeps))
+    theta = _theta(7, b3=-1.0)
+    Xout = torch.empty_like(s.X)
+    lam = torch.zeros(m, device=DEV)
+    Y = torch.zeros(m, device=DEV)
+    _warm(s, s.X, Xout, lam, theta, Y=Y)
+    F = torch.empty((m, 12), device=DEV)
+    N.check(N.lib().xpbd_ws_features(None, N.ptr(s.X), N.ptr(s.XP), N.ptr(s.R), N.ptr(s.ANC), h,
+                                     N.ptr(F), m))
+    Yf = ws.predict(theta, F)
+    y_k, y_f = Y.cpu().numpy(), Yf.cpu().numpy()
+    assert np.array_equal(y_k, y_f), np.abs(y_k - y_f).max()
+    omega = 4.0 / (1.0 + np.exp(-y_k))
+    print(f"omega range {omega.min():.2f} .. {omega.max():.2f}")
+    assert omega.max() - omega.min() > 0.25
+    rows = s.R[:m].cpu().numpy()
+    anc = s.ANC[:m].cpu().numpy()
+    Fr = ws_ref.features(X, XP, rows, anc, h)
+    assert np.allclose(F.cpu().numpy(), Fr, rtol=1e-5, atol=2e-5)
+    Xr, lam_r, _ = ws_ref.warm_launch(X, XP, rows, anc, y_k, h, s.jacobi_relax)
+    lg = lam.cpu().numpy()
+    t = (rows[:, 0].view(np.uint32) >> 28)
+    assert (lam_r[t == 2] != 0).sum() > 50                      # violated FLOOR rows pushed
+    lerr = np.abs(lg - lam_r).max()
+    print(f"lambda0: max |gpu - oracle| {lerr:.3e}, max |lambda0| {np.abs(lam_r).max():.3e}")
+    assert lerr <= 1e-4 * np.abs(lam_r).max()
+    xg = Xout.cpu().numpy()
+    err = np.abs(xg[:, :3] - Xr[:, :3]).max()
+    move = np.abs(Xr[:, :3] - X[:, :3]).max()
+    print(f"positions: max |gpu - oracle| {err:.3e}, largest move {move:.3e}")
+    assert err <= 1e-6 + 1e-4 * move
+
+
+def test_zero_start_is_the_plain_solver():
+    """With the network's output pushed to -inf (omega = 0) the warm-start
+    launch leaves the predicted positions as they are, writes zero
+    multipliers, and two frames of the learned frame plan equal two frames
+    of the plain plan bit for bit (cloth meeting the floor: active FLOOR
+    rows with friction)."""
+    import torch
+
+    from xpbd_amd import _native as N
+    from xpbd_amd import warmstart as ws
+
+    s = _cloth(24)
+    ws._prepare(s)
+    _perturbed_state(s, 6, below_floor=True)
+    m, n = s.num_constraints, s.num_bodies
+    h = float(np.float32(s.dt / s.substeps))
+    theta = _theta(1, b3=-1e4)
+    X0 = s.X.clone()
+    Xo = torch.empty_like(s.X)
+    lam = torch.ones(m, device=DEV)
+    _warm(s, X0, Xo, lam, theta, predict=1)
+    Xi = X0.clone()
+    V1 = s.V.clone()
+    XP1 = torch.empty_like(s.XP)
+    N.check(N.lib().xpbd_integrate_to(None, N.ptr(Xi), N.ptr(Xi), N.ptr(V1), N.ptr(XP1),
+                                      *s.gravity, h, None, n))
+    assert torch.equal(Xo, Xi)
+    assert torch.equal(s.XP, XP1)
+    assert not bool(lam.any())
+    for fuse in (True, False):
+        a, b = _cloth(24), _cloth(24)
+        for c in (a, b):
+            c.fuse_first = fuse
+            c.block_frames = False
+            c.step(18)                                           # cloth reaches the floor
+        a.set_warm_start(theta)
+        a.step(2)
+        b.step(2)
+        assert np.array_equal(a.positions(), b.positions()), fuse
+        assert np.array_equal(a.velocities(), b.velocities()), fuse
+
+
+def test_warm_launch_folded_integrate_equals_integrate_then_warm():
+    """The warm launch with integrate folded in equals integrate followed by
+    the warm launch bit for bit (positions, multipliers, network outputs)."""
+    import torch
+
+    from xpbd_amd import _native as N
+    from xpbd_amd import warmstart as ws
+
+    s = _cloth(24)
+    ws._prepare(s)
+    _perturbed_state(s, 8, below_floor=True)
+    m, n = s.num_constraints, s.num_bodies
+    h = float(np.float32(s.dt / s.substeps))
+    theta = _theta(3, b3=-0.5)
+    X0, V0, XP0 = s.X.clone(), s.V.clone(), s.XP.clone()
+    res = []
+    for predict in (1, 0):
+        s.X.copy_(X0), s.V.copy_(V0), s.XP.copy_(XP0)
+        if not predict:
+            N.check(N.lib().xpbd_integrate_to(None, N.ptr(s.X), N.ptr(s.X), N.ptr(s.V),
+                                              N.ptr(s.XP), *s.gravity, h, None, n))
+        Xo = torch.empty_like(s.X)
+        lam = torch.zeros(m, device=DEV)
+        Y = torch.zeros(m, device=DEV)
+        _warm(s, s.X if not predict else X0.clone(), Xo, lam, theta, Y=Y, predict=predict)
+        res.append((Xo, lam, Y, s.XP.clone()))
+    for name, a, b in zip(("X", "lambda", "Y", "x_prev"), *res):
+        assert torch.equal(a, b), name
 
 
 @pytest.fixture(scope="module")
@@ -28,7 +191,8 @@ def trained():
 
 def test_training_lowers_the_loss_and_data_stay_finite(trained):
     """Data generation (plain, then on-policy) and training on the GPU: the
-    final loss is finite and the on-policy round's states stayed finite."""
+    final loss is finite, the on-policy round's states stayed finite, and the
+    trained network fits fresh data better than its initial parameters."""
     import torch
 
     from xpbd_amd import warmstart as ws
@@ -38,8 +202,6 @@ def test_training_lowers_the_loss_and_data_stay_finite(trained):
     assert log["round2_data_finite"]
     assert bool(torch.isfinite(theta).all())
     assert all(np.isfinite(v) for _, v in log["round1_loss"] + log["round2_loss"])
-    # the trained network fits fresh plain-trajectory data better than the
-    # initial parameters do (targets are logits of the first-step multiple)
     F, T = ws.collect(_cloth(), 30, every=3)
     y0 = ws.predict(torch.from_numpy(ws.init_params(0)).to(F.device), F)
     y1 = ws.predict(theta, F)
@@ -48,28 +210,52 @@ def test_training_lowers_the_loss_and_data_stay_finite(trained):
     assert e1 < 0.9 * e0
 
 
-def test_learned_start_beats_zero_start_at_equal_iterations(trained):
+def test_learned_start_reaches_zero_start_residual_with_half_the_iterations(trained):
     """On a fresh run of the same scene, each start on its own trajectory:
-    RMS residual after k iterations from the learned start against k (and
-    2k) iterations from zero."""
+    RMS residual after the warm-start launch + k iterations against k and 2k
+    iterations from zero."""
     from xpbd_amd import warmstart as ws
 
     theta, _ = trained
     s = _cloth()
-    for _ in range(20):
-        s.step()
+    s.step(20)
     res, fallback = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(1, 2, 4),
                                                learned_iters=(1, 2))
     print("rms residual", res, "fallbacks", fallback)
     assert res["learned_1"] < res["plain_1"]
     assert res["learned_2"] < res["plain_2"]
-    # reaches the zero start's residual with half the iterations
+    assert res["learned_1"] <= res["plain_2"]
     assert res["learned_2"] <= res["plain_4"]
+
+
+@pytest.mark.parametrize("fuse", [True, False])
+def test_frame_plan_with_warm_start_matches_eager_launches(trained, fuse):
+    """Two frames of the captured plan with the warm start (integrate folded
+    into the warm-start launch, or not) equal the eager per-substep launches
+    (integrate, warm-start launch, plain launches) bit for bit."""
+    import torch
+
+    from xpbd_amd import warmstart as ws
+
+    theta, _ = trained
+    a, b = _cloth(), _cloth()
+    for s in (a, b):
+        s.fuse_first = fuse
+        s.step(3)
+    a.set_warm_start(theta)
+    a.step(2)
+    e = ws._Eager(b)
+    th = torch.as_tensor(theta).to(DEV)
+    for _ in range(2 * b.substeps):
+        e.integrate()
+        e.iterate(b.iterations, theta=th)
+    assert np.array_equal(a.positions(), b.positions())
+    assert np.array_equal(a.velocities(), b.velocities())
 
 
 def test_frame_plan_with_warm_start_and_guard(trained):
     """The warm start inside the captured frame plan runs and stays finite;
-    the guard's first check records both residuals and picks the better."""
+    the guard's checks record both residuals and pick the better."""
     from xpbd_amd import warmstart as ws
 
     theta, _ = trained
@@ -88,11 +274,9 @@ def test_frame_plan_with_warm_start_and_guard(trained):
     assert np.all(np.isfinite(s.positions()))
 
 
-def test_ws_kernels_match_numpy_oracle():
-    """xpbd_ws_features / xpbd_ws_apply / xpbd_ws_residual against the
-    float64 NumPy restatement (oracle/ws_ref.py) on a cloth with random
-    velocities, part of it pushed below the floor, and random network
-    outputs."""
+def test_ws_residual_matches_numpy_oracle():
+    """xpbd_ws_residual against the float64 restatement on a cloth half
+    below the floor with random multipliers."""
     import torch
 
     from oracle import ws_ref
@@ -101,42 +285,14 @@ def test_ws_kernels_match_numpy_oracle():
 
     s = _cloth(16)
     ws._prepare(s)
-    rng = np.random.default_rng(3)
-    X = s.X.cpu().numpy().copy()
-    X[:, 1] -= 0.5 + rng.normal(0, 2e-3, len(X)).astype(np.float32)   # about half below y = 0
-    X[:, :3] += rng.normal(0, 1e-3, (len(X), 3)).astype(np.float32)
-    V = np.zeros_like(X)
-    V[:, :3] = rng.normal(0, 0.3, (len(X), 3)).astype(np.float32)
-    s.X.copy_(torch.from_numpy(X))
-    s.V.copy_(torch.from_numpy(V))
+    X, _ = _perturbed_state(s, 3, below_floor=True)
     m = s.num_constraints
     h = float(np.float32(s.dt / s.substeps))
-    lib = N.lib()
-    F = torch.empty((m, 12), device=DEV)
-    N.check(lib.xpbd_ws_features(None, N.ptr(s.X), N.ptr(s.V), N.ptr(s.R), N.ptr(s.ANC), None, h,
-                                 N.ptr(F), m))
-    rows = s.R[:m].cpu().numpy()
-    anc = s.ANC[:m].cpu().numpy()
-    Fr = ws_ref.features(X, V, rows, anc, h)
-    assert np.allclose(F.cpu().numpy(), Fr, rtol=1e-5, atol=2e-5)
-    Y = torch.from_numpy(rng.normal(0, 2, m).astype(np.float32)).to(DEV)
-    LAM = torch.zeros(m, device=DEV)
-    ACC = torch.zeros_like(s.X)
-    N.check(lib.xpbd_ws_apply(None, N.ptr(s.X), N.ptr(s.V), N.ptr(s.R), N.ptr(s.ANC), N.ptr(Y), h,
-                              N.ptr(LAM), N.ptr(ACC), m))
-    lam_r, acc_r = ws_ref.apply(X, V, rows, anc, Y.cpu().numpy(), h)
-    lam_g = LAM.cpu().numpy()
-    assert (lam_g != 0).sum() > m // 4
-    assert np.allclose(lam_g, lam_r, rtol=1e-4, atol=1e-10)
-    acc_g = ACC.cpu().numpy()
-    assert np.array_equal(acc_g[:, 3], acc_r[:, 3])
-    # float atomic sums of terms of both signs: compare against the scale of
-    # the largest sum
-    err = np.abs(acc_g[:, :3] - acc_r[:, :3]).max()
-    print(f"offset sums: max |gpu - oracle| {err:.3e}, max |oracle| {np.abs(acc_r[:, :3]).max():.3e}")
-    assert err <= 1e-5 * np.abs(acc_r[:, :3]).max()
+    rng = np.random.default_rng(4)
+    lam = rng.normal(0, 1e-3, m).astype(np.float32)
+    LAM = torch.from_numpy(lam).to(DEV)
     out = torch.zeros(1, dtype=torch.float64, device=DEV)
-    N.check(lib.xpbd_ws_residual(None, N.ptr(s.X), N.ptr(s.R), N.ptr(s.ANC), N.ptr(LAM), h,
-                                 N.ptr(out), m))
-    r_ref = ws_ref.residual_sum(X, rows, anc, lam_g, h)
+    N.check(N.lib().xpbd_ws_residual(None, N.ptr(s.X), N.ptr(s.R), N.ptr(s.ANC), N.ptr(LAM), h,
+                                     N.ptr(out), m))
+    r_ref = ws_ref.residual_sum(X, s.R[:m].cpu().numpy(), s.ANC[:m].cpu().numpy(), lam, h)
     assert np.isclose(float(out.item()), r_ref, rtol=1e-4)

@@ -277,36 +277,52 @@ def test_jacobi_tile_order_groups_by_tile_then_type_then_rank():
 
 
 def test_ws_oracle_one_row_closed_form():
-    """One stretched DISTANCE row between two particles: C, the normal, the
-    first plain step -C/(w + alpha~), lambda0 = 8 sigmoid(y) step and the
-    mean offsets w grad C lambda0 worked out by hand; a separated FLOOR row
-    gives no push; the residual of the row."""
+    """One stretched DISTANCE row between two particles with known
+    velocities: C, the normal, the velocity features, lambda0 = 4 sigmoid(y)
+    times the first plain step, the offsets worked out by hand, and what the
+    geometry fixes: both particles move along the normal, so the new length
+    is L + w lambda0, and one plain XPBD step from (x + offset, lambda0)
+    solves the row exactly (C + alpha~ lambda = 0) whatever lambda0 was; a
+    separated FLOOR row gets no push."""
     import struct
 
     from oracle import ws_ref
 
     h = 1.0 / 600.0
     X = np.array([[0.0, 0.0, 0.0, 2.0], [0.3, 0.4, 0.0, 1.0]])     # |d| = 0.5
-    V = np.zeros((2, 4))
+    V = np.array([[0.0, 0.0, 1.0, 0.0], [0.6, 0.8, 0.0, 0.0]])
+    XP = X - V * h
     comp = 1e-6
     rest = 0.4
     bits = lambda v: struct.unpack("<i", struct.pack("<f", v))[0]   # noqa: E731
     rows = np.array([[0 | (0 << 28), 1, bits(rest), bits(comp)],
                      [1 | (2 << 28), -1, bits(0.0), bits(0.5)]], np.int64).astype(np.int32)
     anc = np.zeros((2, 4))
-    F = ws_ref.features(X, V, rows, anc, h)
-    C = 0.5 - np.float32(rest)
-    alpha = np.float32(comp) / (h * h)
+    F = ws_ref.features(X, XP, rows, anc, h)
+    rest32 = float(np.float32(rest))                           # rows hold float32
+    C = 0.5 - rest32
+    alpha = float(np.float32(comp)) / (h * h)
     assert F[0, 0] == 1.0 and F[1, 2] == 1.0
     assert np.isclose(F[0, 4], np.log1p(C / 1e-3))
     assert np.isclose(F[0, 6], alpha / (3.0 + alpha))
     assert np.isclose(F[1, 4], np.log1p(0.4 / 1e-3))            # floor row: C = y = 0.4
-    lam0, acc = ws_ref.apply(X, V, rows, anc, np.array([0.0, 5.0]), h)
+    # v_a - v_b = (-0.6, -0.8, 1): along n = (-0.6, -0.8, 0) it is 1, across 1
+    assert np.isclose(F[0, 8], np.log1p(1.0 * h / 1e-3))
+    assert np.isclose(F[0, 9], np.log1p(1.0 * h / 1e-3))
+    assert F[0, 7] == 0.0
     step = -C / (3.0 + alpha)
-    assert np.isclose(lam0[0], 4.0 * step)                      # sigmoid(0) = 1/2
-    assert lam0[1] == 0.0                                       # separated floor row
     n = np.array([-0.6, -0.8, 0.0])                             # (x_a - x_b) / |d|
-    assert np.allclose(acc[0, :3], n * 2.0 * lam0[0]) and acc[0, 3] == 1
-    assert np.allclose(acc[1, :3], -n * 1.0 * lam0[0]) and acc[1, 3] == 1
-    r2 = ws_ref.residual_sum(X, rows, anc, np.array([lam0[0], 0.0]), h)
-    assert np.isclose(r2, (C + alpha * lam0[0]) ** 2)
+    for y, omega in ((0.0, 2.0), (1.0, 4.0 / (1.0 + np.exp(-1.0)))):
+        Xn, lam0, acc = ws_ref.warm_launch(X, XP, rows, anc, np.array([y, 5.0]), h)
+        assert np.isclose(lam0[0], omega * step)
+        assert lam0[1] == 0.0                                   # separated floor row
+        assert np.allclose(acc[0, :3], n * 2.0 * lam0[0]) and acc[0, 3] == 1
+        assert np.allclose(acc[1, :3], -n * 1.0 * lam0[0]) and acc[1, 3] == 1
+        L1 = np.linalg.norm(Xn[0, :3] - Xn[1, :3])
+        assert np.isclose(L1, 0.5 + 3.0 * lam0[0])
+        # one plain step from (Xn, lambda0) (kernels.py:131-160 arithmetic)
+        dlam = (-(L1 - rest32) - alpha * lam0[0]) / (3.0 + alpha)
+        L2 = L1 + 3.0 * dlam
+        assert np.isclose((L2 - rest32) + alpha * (lam0[0] + dlam), 0.0, atol=1e-12)
+        r2 = ws_ref.residual_sum(Xn, rows, anc, lam0, h)
+        assert np.isclose(r2, ((L1 - rest32) + alpha * lam0[0]) ** 2)

@@ -121,8 +121,8 @@ _SIGS = {
     "xpbd_mlp_train_workspace_bytes": (ctypes.c_longlong, [_I]),
     "xpbd_mlp_grad": (c_int, [_P, _P, _P, _P, _I, _I, _P, _P, _P]),
     "xpbd_mlp_adam": (c_int, [_P, _P, _P, _P, _P, _F, _F, _F, _F, _I]),
-    "xpbd_ws_features": (c_int, [_P] * 6 + [_F, _P, _I]),
-    "xpbd_ws_apply": (c_int, [_P] * 6 + [_F, _P, _P, _I]),
+    "xpbd_ws_features": (c_int, [_P] * 5 + [_F, _P, _I]),
+    "xpbd_jacobi_tile_warm": (c_int, [_P] * 13 + [_F, _F, _F, _F, _P, _I, _I, _F, _I]),
     "xpbd_ws_residual": (c_int, [_P] * 5 + [_F, _P, _I]),
     "xpbd_build_info": (ctypes.c_char_p, []),
     "xpbd_error_string": (ctypes.c_char_p, [c_int]),

@@ -12,6 +12,9 @@ and keeps the fastest:
   apply pass, with 1 or 2 rows per thread;
 * ``block``  — whole frames in one workgroup (scenes that fit its LDS).
 
+With a learned warm start set (``Solver.set_warm_start``) only the tile
+candidates apply (the warm start runs inside the first tile launch).
+
 The simulation state is saved before and restored after each trial, so tuning
 does not advance the scene.  Results are cached per scene signature (particle
 count, row count, substeps, iterations) for the life of the process.
@@ -22,15 +25,20 @@ _CACHE: dict = {}
 
 
 def _signature(s):
-    return (s.num_bodies, s.num_constraints, s.substeps, s.iterations, s.solve_mode)
+    return (s.num_bodies, s.num_constraints, s.substeps, s.iterations, s.solve_mode,
+            s._ws_theta is not None)
 
 
-def candidates(n_particles: int):
+def candidates(n_particles: int, tile_only: bool = False):
+    """Launch shapes to time; ``tile_only`` (learned warm start): tiles up to
+    512 (the warm launch's LDS limit) and no window / block shapes."""
     out = []
     for t in (64, 128, 256, 512, 1024):
-        if t <= max(64, 2 * n_particles):
+        if t <= max(64, 2 * n_particles) and not (tile_only and t > 512):
             out.append(("tile", t, None, True))
             out.append(("tile", t, None, False))
+    if tile_only:
+        return out
     out.append(("window", 0, (1, None), False))
     out.append(("window", 0, (2, None), False))
     if n_particles <= _block_capacity():
@@ -71,7 +79,7 @@ def tune(s, frames: int = 3, warmup: int = 1, verbose: bool = False):
         return _CACHE[sig], {}
     x0, v0 = s.positions(), s.velocities()
     times = {}
-    for cand in candidates(s.num_bodies):
+    for cand in candidates(s.num_bodies, tile_only=s._ws_theta is not None):
         _apply(s, cand)
         s.set_state(x0, v0)
         s.step(warmup)

@@ -23,22 +23,6 @@ namespace {
 
 enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8 };
 
-// Learned warm start of one substep (window-Jacobi mode): features ->
-// MLP -> lambda0 into LAM and its position correction summed into ACC ->
-// applied; the iterations then continue from lambda0 (first = 0).
-int warm_start(const xpbd_particles_desc& d, hipStream_t s) {
-  // no previous-substep lambda feature: fed back from warm-started substeps
-  // it drives the prediction upward from substep to substep
-  int rc = xpbd_ws_features(s, d.x, d.v, d.rows, d.anchors, nullptr, d.h, d.ws_feat, d.n_rows);
-  if (rc) return rc;
-  rc = xpbd_mlp_forward(s, d.ws_theta, d.ws_feat, d.ws_y, d.n_rows);
-  if (rc) return rc;
-  rc = xpbd_ws_apply(s, d.x, d.v, d.rows, d.anchors, d.ws_y, d.h, d.lam, d.acc, d.n_rows);
-  if (rc) return rc;
-  return xpbd_particles_pass(s, d.x, d.v, d.x_prev, d.acc, F_APPLY, d.relax, d.h, d.damping,
-                             d.max_speed, d.gx, d.gy, d.gz, d.n_particles);
-}
-
 struct Frame3 {
   xpbd_particles_desc d;
   std::vector<int> color_starts;
@@ -49,16 +33,21 @@ struct Frame3 {
 
 // Fused-tile Jacobi frame.
 //
-// With fuse_first, two or more iterations and no device-counted rows, each
-// substep is `iterations` k_jacobi_tile launches only: the first has
-// integrate folded in (xpbd_jacobi_tile_first), the last runs finalize + cap.  Launches
+// With fuse_first and no device-counted rows, each substep is `iterations`
+// k_jacobi_tile launches only: the first has integrate folded in
+// (xpbd_jacobi_tile_first), the last runs finalize + cap.  Launches
 // ping-pong between X and X_alt; `loc` tracks which one holds the state, and
 // a frame that ends in X_alt copies it back.
 //
-// Otherwise (one iteration, or contact rows projected by a separate launch
-// before each iteration) a substep is one integrate pass (X -> X or X_alt,
-// chosen so that after `iterations` launches the state is back in X) and the
-// plain launches.
+// Otherwise (contact rows projected by a separate launch before each
+// iteration, or fuse_first off) a substep is one integrate pass (X -> X or
+// X_alt, chosen so that after the substep's launches the state is back in X)
+// and the plain launches.
+//
+// With the learned warm start (ws_theta) every substep first runs
+// xpbd_jacobi_tile_warm (integrate folded in on the fused path) and then all
+// `iterations` plain launches with first = 0.  Multipliers ping-pong as the
+// positions do: a substep's first launch writes lam_alt.
 int issue_frame_tile(Frame3* f, hipStream_t s) {
   const xpbd_particles_desc& d = f->d;
   const int n = d.n_particles;
@@ -66,24 +55,34 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
   const bool have_rows = (d.n_rows > 0 || d.dyn_cap > 0) && d.iterations > 0;
   const int it_n = have_rows ? d.iterations : 0;
   const bool dyn = d.dyn_cap > 0 && d.dyn_rows && d.dyn_count;
+  const bool warm = d.ws_theta != nullptr && d.n_rows > 0 && it_n > 0;
   float* bufs[2] = {d.x, d.x_alt};
   float* lams[2] = {d.lam, d.lam_alt};
   int rc;
-  if (d.fuse_first && it_n >= 2 && !dyn && d.n_rows > 0) {
+  if (d.fuse_first && !dyn && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1))) {
     int loc = 0;
     for (int sub = 0; sub < d.substeps; ++sub) {
-      rc = xpbd_jacobi_tile_first(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
-                                  lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.h,
-                                  d.gx, d.gy, d.gz, d.gscale, d.tile_size, n, d.relax);
+      if (warm)
+        rc = xpbd_jacobi_tile_warm(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
+                                   lams[1], d.tile_row_start, d.halo_start, d.halo_rows,
+                                   d.ws_theta, nullptr, d.h, d.gx, d.gy, d.gz, d.gscale,
+                                   d.tile_size, n, d.relax, 1);
+      else
+        rc = xpbd_jacobi_tile_first(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows,
+                                    d.anchors, lams[1], d.tile_row_start, d.halo_start,
+                                    d.halo_rows, d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n,
+                                    d.relax);
       if (rc) return rc;
       loc ^= 1;
-      for (int it = 1; it < it_n; ++it) {
+      int li = 1;  // lams[li] holds the current multipliers
+      for (int it = warm ? 0 : 1; it < it_n; ++it) {
         rc = xpbd_jacobi_tile(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
-                              lams[it & 1], lams[(it + 1) & 1], d.tile_row_start, d.halo_start,
+                              lams[li], lams[li ^ 1], d.tile_row_start, d.halo_start,
                               d.halo_rows, d.h, 0, d.tile_size, n, d.relax, it + 1 == it_n,
                               d.damping, d.max_speed, nullptr);
         if (rc) return rc;
         loc ^= 1;
+        li ^= 1;
       }
     }
     if (loc) {
@@ -92,10 +91,21 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
     }
     return 0;
   }
+  const int launches = it_n + (warm ? 1 : 0);
   for (int sub = 0; sub < d.substeps; ++sub) {
-    int cur = it_n & 1;
+    int cur = launches & 1;
     rc = xpbd_integrate_to(s, d.x, bufs[cur], d.v, d.x_prev, d.gx, d.gy, d.gz, d.h, d.gscale, n);
     if (rc) return rc;
+    int li = 0;
+    if (warm) {
+      rc = xpbd_jacobi_tile_warm(s, bufs[cur], bufs[cur ^ 1], d.x_prev, d.v, d.rows, d.anchors,
+                                 lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.ws_theta,
+                                 nullptr, d.h, 0.0f, 0.0f, 0.0f, nullptr, d.tile_size, n, d.relax,
+                                 0);
+      if (rc) return rc;
+      cur ^= 1;
+      li = 1;
+    }
     for (int it = 0; it < it_n; ++it) {
       const int nxt = cur ^ 1;
       if (dyn) {
@@ -103,12 +113,13 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
                                    it == 0, d.dyn_count, d.dyn_cap);
         if (rc) return rc;
       }
-      rc = xpbd_jacobi_tile(s, bufs[cur], bufs[nxt], d.x_prev, d.v, d.rows, d.anchors,
-                            lams[it & 1], lams[(it + 1) & 1], d.tile_row_start, d.halo_start,
-                            d.halo_rows, d.h, it == 0, d.tile_size, n, d.relax,
-                            it + 1 == it_n, d.damping, d.max_speed, dyn ? d.acc : nullptr);
+      rc = xpbd_jacobi_tile(s, bufs[cur], bufs[nxt], d.x_prev, d.v, d.rows, d.anchors, lams[li],
+                            lams[li ^ 1], d.tile_row_start, d.halo_start, d.halo_rows, d.h,
+                            it == 0 && !warm, d.tile_size, n, d.relax, it + 1 == it_n,
+                            d.damping, d.max_speed, dyn ? d.acc : nullptr);
       if (rc) return rc;
       cur = nxt;
+      li ^= 1;
     }
     if (it_n == 0) {
       int flags = F_FINALIZE;
@@ -132,15 +143,10 @@ int issue_frame(Frame3* f, hipStream_t s) {
   int rc = xpbd_particles_pass(s, d.x, d.v, d.x_prev, d.acc, F_INTEGRATE, d.relax, d.h, d.damping,
                                d.max_speed, d.gx, d.gy, d.gz, n);
   if (rc) return rc;
-  const bool warm = jacobi && have_rows && d.ws_theta != nullptr;
   for (int sub = 0; sub < d.substeps; ++sub) {
-    if (warm) {
-      rc = warm_start(d, s);
-      if (rc) return rc;
-    }
     if (have_rows) {
       for (int it = 0; it < d.iterations; ++it) {
-        const int first = (it == 0) && !warm;
+        const int first = it == 0;
         if (jacobi) {
           rc = xpbd_solve_jacobi(s, d.x, d.x_prev, d.rows, d.anchors, d.lam, d.acc, d.h, first,
                                  d.n_rows, d.rows_per_thread, d.window);
@@ -190,8 +196,9 @@ int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
       (!desc->x_alt || !desc->lam_alt || !desc->tile_row_start || !desc->halo_start ||
        desc->tile_size < 1 || desc->tile_size > XPBD_MAX_TILE))
     return (int)hipErrorInvalidValue;
-  if (desc->ws_theta && (desc->mode != XPBD_MODE_JACOBI || !desc->ws_feat || !desc->ws_y))
-    return (int)hipErrorInvalidValue;  // learned warm start: window-Jacobi mode
+  if (desc->ws_theta &&
+      (desc->mode != XPBD_MODE_JACOBI_TILE || desc->tile_size > XPBD_MAX_WARM_TILE))
+    return (int)hipErrorInvalidValue;  // learned warm start: tile mode, tiles <= 512
   if (desc->mode != XPBD_MODE_JACOBI_TILE && (desc->dyn_cap > 0 || desc->gscale))
     return (int)hipErrorInvalidValue;  // device-counted rows / gravity scale: tile mode only
   if (desc->mode == XPBD_MODE_JACOBI_TILE && desc->dyn_cap > 0 &&

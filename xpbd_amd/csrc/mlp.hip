@@ -4,19 +4,11 @@
 // replaces the zero reset of the multipliers at the start of a substep
 // (src/xpbd3d/solver.py:431, solver_6dof.py:374-377).
 //
-// Parameters: one float32 vector of 5057 values, W1[12][64] b1[64] W2[64][64]
-// b2[64] W3[64] b3[1] (row-major, input x output).  The two 64-wide products
-// run on v_mfma_f32_16x16x16_bf16: inputs, hidden activations and W1/W2 are
-// rounded to bfloat16 (nearest even), products accumulate in float32; the
-// output layer and every elementwise op stay in float32.
-//
-// A wavefront owns 16 rows at a time.  MFMA operand layout (16x16x16):
-//   A (M x K):  lane l holds A[l % 16][4 (l / 16) + j], j = 0..3
-//   B (K x N):  lane l holds B[4 (l / 16) + j][l % 16]
-//   D (M x N):  lane l holds D[4 (l / 16) + j][l % 16]
-// so a D tile of one layer is already the A^T / B operand of the weight
-// gradient, and only the forward H1 and backward dZ2 tiles take a trip
-// through LDS to become A operands.
+// The network layout, bfloat16 rounding and the per-wave forward live in
+// mlp_dev.h (shared with the fused warm-start Jacobi launch).  A D tile of
+// one layer is already the A^T / B operand of the weight gradient, and only
+// the forward H1 and backward dZ2 tiles take a trip through LDS to become A
+// operands.
 //
 // Training: each block accumulates its gradient in LDS and writes one
 // partial vector; a second kernel sums the partials in block order
@@ -26,107 +18,37 @@
 #include "common.h"
 #include "xpbd.h"
 
+#include "mlp_dev.h"
+
 namespace {
 
-constexpr int NIN = 12, NH = 64;
-constexpr int P_W1 = 0, P_B1 = P_W1 + NIN * NH, P_W2 = P_B1 + NH, P_B2 = P_W2 + NH * NH,
-              P_W3 = P_B2 + NH, P_B3 = P_W3 + NH, NPARAM = P_B3 + 1;
-static_assert(NPARAM == 5057, "parameter count");
+using namespace xpbd_mlp;
+
 constexpr int kWaves = 4, kBlk = 64 * kWaves;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ inline unsigned short to_bf16(float f) {
-  unsigned u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-__device__ inline float from_bf16(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
-
-__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
-__device__ inline float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-         z * expf(-0.5f * z * z) * 0.39894228040143268f;
-}
-
-__device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
-
 struct Lds {
-  unsigned short w1[16][NH];   // W1 padded to 16 input rows (12..15 zero)
-  unsigned short w2[NH][NH];
-  float b1[NH], b2[NH], w3[NH];
+  Weights w;
   unsigned short tile[kWaves][16][NH];  // per-wave transposition tile
   float grad[NPARAM];                    // block gradient (training only)
 };
 
 __device__ void load_params(Lds& s, const float* __restrict__ theta, bool zero_grad) {
-  for (int i = threadIdx.x; i < 16 * NH; i += kBlk) {
-    const int k = i / NH, n = i % NH;
-    s.w1[k][n] = k < NIN ? to_bf16(theta[P_W1 + k * NH + n]) : (unsigned short)0;
-  }
-  for (int i = threadIdx.x; i < NH * NH; i += kBlk) s.w2[i / NH][i % NH] = to_bf16(theta[P_W2 + i]);
-  for (int i = threadIdx.x; i < NH; i += kBlk) {
-    s.b1[i] = theta[P_B1 + i];
-    s.b2[i] = theta[P_B2 + i];
-    s.w3[i] = theta[P_W3 + i];
-  }
+  load_weights(s.w, theta, threadIdx.x, kBlk);
   if (zero_grad)
     for (int i = threadIdx.x; i < NPARAM; i += kBlk) s.grad[i] = 0.0f;
   __syncthreads();
 }
 
-// Forward of one 16-row tile held by a wave.  On return z1/z2 hold the
-// pre-activations (D layout, 4 column tiles x 4 rows per lane) and y the
-// outputs of rows 4g+j (identical across the 16 lanes of group g).
+// Forward of rows [row0, row0 + 16) of F (rows >= m read as zero).
 __device__ inline void forward_tile(Lds& s, const float* __restrict__ F, int row0, int m,
                                     float b3, f32x4 z1[4], f32x4 z2[4], float y[4],
                                     bf16x4& xa) {
-  const int l = threadIdx.x & 63, r16 = l & 15, g = l >> 4, wv = threadIdx.x >> 6;
-  // layer 1: A = X tile (16 rows x 16 padded features)
-  {
-    const int row = row0 + r16;
-    for (int j = 0; j < 4; ++j) {
-      const int k = 4 * g + j;
-      const float v = (row < m && k < NIN) ? F[(size_t)row * NIN + k] : 0.0f;
-      xa[j] = (short)to_bf16(v);
-    }
-  }
-  for (int n = 0; n < 4; ++n) {
-    bf16x4 b;
-    for (int j = 0; j < 4; ++j) b[j] = (short)s.w1[4 * g + j][16 * n + r16];
-    z1[n] = mfma(xa, b, f32x4{0.f, 0.f, 0.f, 0.f});
-    for (int j = 0; j < 4; ++j) {
-      z1[n][j] += s.b1[16 * n + r16];
-      s.tile[wv][4 * g + j][16 * n + r16] = to_bf16(gelu(z1[n][j]));
-    }
-  }
-  __builtin_amdgcn_s_waitcnt(0);
-  __builtin_amdgcn_wave_barrier();
-  // layer 2: A = H1 tile read back in A layout
-  bf16x4 a[4];
-  for (int sK = 0; sK < 4; ++sK)
-    for (int j = 0; j < 4; ++j) a[sK][j] = (short)s.tile[wv][r16][16 * sK + 4 * g + j];
-  for (int n = 0; n < 4; ++n) {
-    f32x4 acc{0.f, 0.f, 0.f, 0.f};
-    for (int sK = 0; sK < 4; ++sK) {
-      bf16x4 b;
-      for (int j = 0; j < 4; ++j) b[j] = (short)s.w2[16 * sK + 4 * g + j][16 * n + r16];
-      acc = mfma(a[sK], b, acc);
-    }
-    for (int j = 0; j < 4; ++j) acc[j] += s.b2[16 * n + r16];
-    z2[n] = acc;
-  }
-  // layer 3 (float32): y[row] = sum_c gelu(z2[row][c]) w3[c] + b3
-  for (int j = 0; j < 4; ++j) {
-    float p = 0.0f;
-    for (int n = 0; n < 4; ++n) p += gelu(z2[n][j]) * s.w3[16 * n + r16];
-    for (int o = 1; o < 16; o <<= 1) p += __shfl_xor(p, o, 64);
-    y[j] = p + b3;
-  }
-  __builtin_amdgcn_wave_barrier();
+  const int wv = threadIdx.x >> 6;
+  auto feat = [&](int r, int k) -> unsigned short {
+    const int row = row0 + r;
+    return (row < m && k < NIN) ? to_bf16(F[(size_t)row * NIN + k]) : (unsigned short)0;
+  };
+  forward16(s.w, s.tile[wv], feat, b3, z1, z2, y, xa);
 }
 
 __global__ __launch_bounds__(kBlk) void k_mlp_forward(const float* __restrict__ theta,
@@ -188,7 +110,7 @@ __global__ __launch_bounds__(kBlk) void k_mlp_train_partial(const float* __restr
     // output layer and dZ2
     f32x4 dz2[4];
     for (int n = 0; n < 4; ++n) {
-      const float w3 = s.w3[16 * n + r16];
+      const float w3 = s.w.w3[16 * n + r16];
       for (int j = 0; j < 4; ++j) {
         gw3[n] += gelu(z2[n][j]) * dy[j];
         dz2[n][j] = dy[j] * w3 * gelu_grad(z2[n][j]);
@@ -218,7 +140,7 @@ __global__ __launch_bounds__(kBlk) void k_mlp_train_partial(const float* __restr
       f32x4 acc{0.f, 0.f, 0.f, 0.f};
       for (int sK = 0; sK < 4; ++sK) {
         bf16x4 b;  // B[k][n] = W2[n][k]
-        for (int j = 0; j < 4; ++j) b[j] = (short)s.w2[16 * nt + r16][16 * sK + 4 * g + j];
+        for (int j = 0; j < 4; ++j) b[j] = (short)s.w.w2[16 * nt + r16][16 * sK + 4 * g + j];
         acc = mfma(a[sK], b, acc);
       }
       for (int j = 0; j < 4; ++j) {

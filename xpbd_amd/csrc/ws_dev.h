@@ -1,0 +1,48 @@
+// Row features and the bounded start of the learned lambda warm start,
+// shared by the standalone feature kernel (warmstart.hip) and the fused
+// warm-start Jacobi launch (particles.hip).  No reference counterpart.
+//
+// Features of a row (12, in this order): one-hot type (4), slog(C/l),
+// log10(w)/4, alpha~/(w + alpha~), 0 (slot of a previous-substep lambda,
+// unused), slog(v_n h / l), slog(|v_t| h / l), slog(pen / l),
+// log10(1 + alpha~)/8, with w = w_a + w_b (|grad C|^2 weighted by inverse
+// mass), alpha~ = alpha / h^2, l = 1 mm, slog(x) = sign(x) ln(1 + |x|) and
+// the relative velocity v = (x - x_prev) / h of the predicted positions.
+//
+// Start: lambda0 = 4 sigmoid(y) * (-C / (w + alpha~)), a multiple (0..4) of
+// the row's first plain step (one-sided rows only while violated).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace xpbd_ws {
+
+constexpr float kLen = 1.0e-3f;
+constexpr float kOmegaMax = 4.0f;
+
+__device__ inline float slog(float x) { return copysignf(log1pf(fabsf(x)), x); }
+
+__device__ inline void row_features(int t, float C, float w, float alpha, float vn, float vt,
+                                    float pen, float h, float f[12]) {
+  const float s = w + alpha;
+  for (int k = 0; k < 4; ++k) f[k] = (t == k) ? 1.0f : 0.0f;
+  f[4] = slog(C / kLen);
+  f[5] = log10f(fmaxf(w, 1e-30f)) * 0.25f;
+  f[6] = s > 0.0f ? alpha / s : 0.0f;
+  f[7] = 0.0f;
+  f[8] = slog(vn * h / kLen);
+  f[9] = slog(vt * h / kLen);
+  f[10] = slog(pen / kLen);
+  f[11] = log10f(1.0f + alpha) * 0.125f;
+}
+
+// lambda0 of a row from the network output y; `step` is the row's first
+// plain step -C / (w + alpha~) (zero when the row is inactive).
+__device__ inline float start_lambda(float y, float step) {
+  const float lam0 = kOmegaMax / (1.0f + expf(-y)) * step;
+  return isfinite(lam0) ? lam0 : 0.0f;
+}
+
+}  // namespace xpbd_ws

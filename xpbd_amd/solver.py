@@ -448,10 +448,11 @@ class Solver:
         """Learned lambda warm start (an extension; the reference resets
         lambda to zero every substep, solver.py:431): ``theta`` holds the
         5057 parameters of the 12-64-64-1 network (``warmstart.train``);
-        every substep then starts from the predicted multipliers and the
-        matching position offset.  ``None`` restores the zero start.  Runs
-        the Jacobi window kernel; ``warmstart.Guard`` adds the plain-start
-        fallback."""
+        the first Jacobi launch of every substep then starts each
+        DISTANCE / ATTACH row from the predicted multiplier
+        (``xpbd_jacobi_tile_warm``).  ``None`` restores the zero start.
+        Runs the Jacobi tile kernel; ``warmstart.Guard`` adds the
+        plain-start fallback."""
         import torch
 
         if theta is None:
@@ -464,7 +465,7 @@ class Solver:
             if t.numel() != n:
                 raise ValueError(f"warm-start parameters: {t.numel()} values, expected {n}")
             self._ws_theta = t
-            self.jacobi_kernel = "window"
+            self.jacobi_kernel = "tile"
         self._ws_gen += 1
         self._dirty = True
 
@@ -691,10 +692,6 @@ class Solver:
                 raise ValueError("gravity scale length != number of particles")
             g = self._gscale if pperm is None else self._gscale[pperm]
             self.GS = torch.from_numpy(np.ascontiguousarray(g)).to(dev)
-        self.WSF = self.WSY = None
-        if self._ws_theta is not None:
-            self.WSF = torch.empty((max(n_c, 1), 12), dtype=torch.float32, device=dev)
-            self.WSY = torch.empty(max(n_c, 1), dtype=torch.float32, device=dev)
         self.CROWS = None
         if self._device_contacts:
             rad = self._radius.array().astype(np.float32)
@@ -821,9 +818,8 @@ class Solver:
             if self.CROWS is not None:
                 d.dyn_rows, d.dyn_lam = N.ptr(self.CROWS), N.ptr(self.CLAM)
                 d.dyn_count, d.dyn_cap = N.ptr(self.CCOUNT), self._ccap
-        if self._ws_theta is not None and d.mode == MODE_JACOBI:
-            d.ws_theta, d.ws_feat, d.ws_y = (N.ptr(self._ws_theta), N.ptr(self.WSF),
-                                             N.ptr(self.WSY))
+        if self._ws_theta is not None and d.mode == MODE_JACOBI_TILE:
+            d.ws_theta = N.ptr(self._ws_theta)
         self._block_active = self._use_block()
         if self._block_active:
             d.block = 1
@@ -907,7 +903,8 @@ class Solver:
         if self.LAM is None or self._dirty:
             return np.zeros(n_c, dtype=np.float32)
         lam = self.LAM
-        if self._tile_dev is not None and self.iterations % 2 == 1 and not self._block_active:
+        launches = self.iterations + (1 if self._ws_theta is not None else 0)
+        if self._tile_dev is not None and launches % 2 == 1 and not self._block_active:
             lam = self.LAM_B
         out = np.zeros(n_c, dtype=np.float32)
         out[self._perm] = lam[:n_c].cpu().numpy()

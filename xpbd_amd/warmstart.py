@@ -2,26 +2,35 @@
 counterpart; it replaces the per-substep zero reset of the multipliers,
 src/xpbd3d/solver.py:431).
 
+Each substep starts with one warm-start launch (``xpbd_jacobi_tile_warm``):
+every row evaluates the network on its features and starts from lambda0 =
+4 sigmoid(y) lambda1, lambda1 = -C / (w + alpha~) the row's first plain step
+(one-sided rows only while violated), and every particle moves by the mean
+over its rows of w grad C lambda0 -- the position the Jacobi sweep reaches
+when its multipliers add up to lambda0.  The substep's Jacobi iterations
+then continue from lambda0 (first = 0).  With k iterations a learned substep
+is k + 1 launches.
+
 * ``collect``  runs the solver and, at sampled substeps, records the row
-  features (kernel ``xpbd_ws_features``) and the target y* =
-  logit(omega / 8), omega = beta lambda* / lambda1: lambda* = the
-  multipliers after ``target_iters`` plain Jacobi iterations of that substep
-  from lambda = 0 (run on a copy of the state), beta = 0.5, lambda1 = the
-  row's first plain step -C / (w + alpha~); the deployed start is
-  lambda0 = 8 sigmoid(y) lambda1, bounded by the row's current violation;
+  features (kernel ``xpbd_ws_features``, the same features the launch
+  computes) and the target y* = logit(omega / 4), omega = beta lambda* /
+  lambda1 clipped to [0.2, 3.8]: lambda* = the multipliers after
+  ``target_iters`` plain iterations of that substep from zero (run on a copy
+  of the state), beta = 0.5 (a damped target: the full lambda* overshoots);
 * ``fit``      two rounds of collect + train: on the plain trajectory, then on
   the learned start's own trajectory;
 * ``train``    fits the 12-64-64-1 MLP on the GPU with the library's own BF16
   MFMA forward/backward (``xpbd_mlp_grad``) and Adam (``xpbd_mlp_adam``);
-* ``Evaluator`` runs the plain zero start and the learned start (features ->
-  MLP -> lambda0 and the matching position offset -> iterations continuing
-  from lambda0) on their own trajectories and measures the RMS constraint
-  residual after each substep's iterations;
+* ``Evaluator`` runs the plain zero start and the learned start on their
+  own trajectories and measures the RMS constraint residual after each
+  substep's iterations;
 * ``Guard``    keeps the plain solver as fallback and judge in deployment.
 
-The previous-substep lambda feature slot is left at zero: measured on
-cloth, carrying it in feeds every warm-started substep's multipliers into
-the next prediction and the scene blows up within a few frames.
+Measured on cloth: the previous-substep lambda feature (slot 7, left at
+zero) feeds every warm-started substep's multipliers into the next
+prediction and the scene blows up within a few frames; folding the start
+into the first iteration's update instead of a launch of its own (a learned
+per-row over-relaxation) lost to the zero start at one iteration.
 """
 from __future__ import annotations
 
@@ -31,7 +40,8 @@ from . import _native as N
 
 LEN = 1e-3
 N_PARAMS = 5057
-OMEGA_MAX = 8.0         # kOmegaMax of csrc/warmstart.hip
+OMEGA_MAX = 4.0         # kOmegaMax of csrc/ws_dev.h
+TARGET_CLIP = 0.05      # targets omega / OMEGA_MAX clipped to [0.05, 0.95]
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -58,9 +68,12 @@ def _slog_inv(y):
 
 def target_logit(F, lam_star, scale, beta: float):
     """Network target y* = logit(omega / OMEGA_MAX), omega = beta lambda* /
-    lambda1 clipped to (0, OMEGA_MAX), lambda1 = -C / (w + alpha~) the first
-    plain step from zero (C recovered from feature 4); rows without a step
-    (separated one-sided rows, w + alpha~ = 0) get y* = 0."""
+    lambda1 clipped to OMEGA_MAX x [TARGET_CLIP, 1 - TARGET_CLIP] (targets
+    stay within +-3 and the many rows whose ratio lies beyond the cap do not
+    dominate the squared error), lambda1 = -C / (w + alpha~) the first
+    plain step from zero (C recovered from feature 4, w + alpha~ = ``scale``);
+    rows without a step (separated one-sided rows, w + alpha~ = 0) get
+    y* = 0."""
     import torch
 
     C = _slog_inv(F[:, 4]) * LEN
@@ -70,7 +83,7 @@ def target_logit(F, lam_star, scale, beta: float):
     valid = lam1.abs() > 1e-12
     ratio = torch.where(valid, beta * lam_star / torch.where(valid, lam1, torch.ones_like(lam1)),
                         torch.zeros_like(lam1))
-    om = torch.clamp(ratio, 0.01, OMEGA_MAX - 0.01)
+    om = torch.clamp(ratio, TARGET_CLIP * OMEGA_MAX, (1.0 - TARGET_CLIP) * OMEGA_MAX)
     y = torch.log(om / (OMEGA_MAX - om))
     return torch.where(valid, y, torch.zeros_like(y))
 
@@ -85,67 +98,70 @@ def row_scale(s) -> "torch.Tensor":
     t = (R[:, 0] >> 28) & 0xF
     w = s.X[a, 3] + torch.where(b >= 0, s.X[b.clamp(min=0), 3], torch.zeros_like(s.X[a, 3]))
     w = torch.where((t == 1) | (t == 2), s.X[a, 3], w)            # ATTACH / FLOOR: one body
-    h = float(np.float32(s.dt / s.substeps))
+    h = np.float32(s.dt / s.substeps)
+    inv_dt2 = float(np.float32(1.0) / (h * h))       # float32 arithmetic, as the kernels
     comp = R[:, 3].view(torch.float32)
-    alpha = torch.where((t == 0) | (t == 1), comp / (h * h), torch.zeros_like(comp))
+    alpha = torch.where((t == 0) | (t == 1), comp * inv_dt2, torch.zeros_like(comp))
     return w + alpha
 
 
 class _Eager:
-    """Per-substep launches of the window-Jacobi path through the C-ABI."""
+    """Per-substep tile launches through the C-ABI (scenes without
+    device-generated contact rows)."""
 
     def __init__(self, s):
-        s._flush()
+        _prepare(s)
+        if s.CROWS is not None:
+            raise ValueError("the eager warm-start driver does not run device contact rows")
         self.s = s
         self.lib = N.lib()
         self.h = float(np.float32(s.dt / s.substeps))
         self.m = s.num_constraints
         self.n = s.num_bodies
+        self.tables = tuple(N.ptr(t) for t in s._tile_dev)
 
-    def integrate(self, X=None):
+    def integrate(self):
         s = self.s
-        N.check(self.lib.xpbd_particles_pass(N.current_stream(), N.ptr(s.X if X is None else X),
-                                             N.ptr(s.V), N.ptr(s.XP), N.ptr(s.ACC), 8,
-                                             s.jacobi_relax, self.h, s.damping, s.max_speed,
-                                             *s.gravity, self.n))
+        N.check(self.lib.xpbd_integrate_to(N.current_stream(), N.ptr(s.X), N.ptr(s.X), N.ptr(s.V),
+                                           N.ptr(s.XP), *s.gravity, self.h,
+                                           N.ptr(s.GS) if s.GS is not None else None, self.n))
 
-    def iterate(self, X, LAM, ACC, iters, first):
+    def iterate(self, iters, theta=None, finalize=True, X=None, X2=None, L=None, L2=None,
+                Y=None):
+        """``iters`` tile launches from X (default the solver's state),
+        ping-ponging with X2; the result is left in X and L.  ``theta``: a
+        warm-start launch first (network output into Y), then the iterations
+        continue from its multipliers."""
         s = self.s
-        rpt, win = s._tiling
+        X = s.X if X is None else X
+        X2 = s.X_B if X2 is None else X2
+        L = s.LAM if L is None else L
+        L2 = s.LAM_B if L2 is None else L2
+        src, dst, ls, ld = X, X2, L, L2
+        st = N.current_stream()
+        trs, hs, hr = self.tables
+        if theta is not None:
+            N.check(self.lib.xpbd_jacobi_tile_warm(
+                st, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), N.ptr(s.ANC),
+                N.ptr(ld), trs, hs, hr, N.ptr(theta), N.ptr(Y) if Y is not None else None,
+                self.h, 0.0, 0.0, 0.0, None, s._tile, self.n, s.jacobi_relax, 0))
+            src, dst, ls, ld = dst, src, ld, ls
         for it in range(iters):
-            N.check(self.lib.xpbd_solve_jacobi(N.current_stream(), N.ptr(X), N.ptr(s.XP),
-                                               N.ptr(s.R), N.ptr(s.ANC), N.ptr(LAM), N.ptr(ACC),
-                                               self.h, int(first and it == 0), self.m, rpt, win))
-            N.check(self.lib.xpbd_particles_pass(N.current_stream(), N.ptr(X), N.ptr(s.V),
-                                                 N.ptr(s.XP), N.ptr(ACC), 1, s.jacobi_relax,
-                                                 self.h, s.damping, s.max_speed, *s.gravity,
-                                                 self.n))
-
-    def finalize(self):
-        s = self.s
-        N.check(self.lib.xpbd_particles_pass(N.current_stream(), N.ptr(s.X), N.ptr(s.V),
-                                             N.ptr(s.XP), N.ptr(s.ACC), 2 | 4, s.jacobi_relax,
-                                             self.h, s.damping, s.max_speed, *s.gravity, self.n))
+            fin = int(finalize and it + 1 == iters)
+            N.check(self.lib.xpbd_jacobi_tile(
+                st, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), N.ptr(s.ANC),
+                N.ptr(ls), N.ptr(ld), trs, hs, hr, self.h, int(it == 0 and theta is None),
+                s._tile, self.n, s.jacobi_relax, fin, s.damping, s.max_speed, None))
+            src, dst, ls, ld = dst, src, ld, ls
+        if src is not X:
+            X.copy_(src)
+            L.copy_(ls)
 
     def features(self, F):
-        """Row features; the previous-substep lambda slot stays zero (fed
-        back from warm-started substeps it drives the predictions upward
-        from substep to substep until the scene blows up)."""
+        """Row features at the integrated positions (velocity (x - x_prev)/h)."""
         s = self.s
-        N.check(self.lib.xpbd_ws_features(N.current_stream(), N.ptr(s.X), N.ptr(s.V), N.ptr(s.R),
-                                          N.ptr(s.ANC), None, self.h, N.ptr(F), self.m))
-
-    def warm(self, theta, F, Y):
-        s = self.s
-        self.features(F)
-        N.check(self.lib.xpbd_mlp_forward(N.current_stream(), N.ptr(theta), N.ptr(F), N.ptr(Y),
-                                          self.m))
-        N.check(self.lib.xpbd_ws_apply(N.current_stream(), N.ptr(s.X), N.ptr(s.V), N.ptr(s.R),
-                                       N.ptr(s.ANC), N.ptr(Y), self.h, N.ptr(s.LAM), N.ptr(s.ACC),
-                                       self.m))
-        N.check(self.lib.xpbd_particles_pass(N.current_stream(), N.ptr(s.X), N.ptr(s.V),
-                                             N.ptr(s.XP), N.ptr(s.ACC), 1, s.jacobi_relax,
-                                             self.h, s.damping, s.max_speed, *s.gravity, self.n))
+        N.check(self.lib.xpbd_ws_features(N.current_stream(), N.ptr(s.X), N.ptr(s.XP), N.ptr(s.R),
+                                          N.ptr(s.ANC), self.h, N.ptr(F), self.m))
 
     def residual(self, X=None, LAM=None) -> float:
         import torch
@@ -162,9 +178,12 @@ class _Eager:
 def _prepare(s):
     if s.solve_mode != "jacobi":
         raise ValueError("the learned warm start runs with solve_mode='jacobi'")
-    s.jacobi_kernel = "window"
-    s._dirty = True
+    if s.jacobi_kernel != "tile" or s._dirty:
+        s.jacobi_kernel = "tile"
+        s._dirty = True
     s._flush()
+    if s._tile_dev is None:
+        raise ValueError("the learned warm start needs the Jacobi tile kernel")
 
 
 TARGET_SCALE = 0.5
@@ -173,52 +192,40 @@ TARGET_SCALE = 0.5
 def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int | None = None,
             theta=None, target_scale: float = TARGET_SCALE):
     """Run the solver for ``substeps`` substeps; every ``every``-th substep
-    append (features, target) for all rows.  Returns (F, T) device tensors of
-    shape (k m, 12) and (k m,).
-
-    Target: ``target_logit`` of beta lambda*, lambda* = the multipliers after
-    ``target_iters`` plain iterations from zero at that substep (run on a
-    copy of the state), beta = ``target_scale``: the network predicts the
-    multiple omega of the row's first plain step that the start should take.
-    A full lambda* start overshoots (measured on cloth: reusing the previous
-    substep's lambda explodes within a few frames, half of it matches the
-    plain solver with twice the iterations), so the target is damped.
+    append (features, target) for all rows.  Returns (F, T) device tensors
+    of shape (k m, 12) and (k m,).
 
     ``theta`` None: the trajectory is the plain solver's; ``theta`` given: the
     trajectory runs the learned start (an on-policy round, whose states are
     the ones the deployed network meets)."""
     import torch
 
-    _prepare(s)
     e = _Eager(s)
     iters = s.iterations if iters is None else iters
     scale = row_scale(s)
     F_all, T_all = [], []
     F = torch.empty((e.m, 12), device=s.device)
-    Y = torch.empty(e.m, device=s.device)
+    th = None if theta is None else torch.as_tensor(theta, dtype=torch.float32).to(s.device)
     for k in range(substeps):
         e.integrate()
         if k % every == 0:
             e.features(F)
-            Xc = s.X.clone()
-            Lc = torch.zeros_like(s.LAM)
-            Ac = torch.zeros_like(s.ACC)
-            e.iterate(Xc, Lc, Ac, target_iters, first=True)
+            Xc, Xc2 = s.X.clone(), s.X.clone()
+            Lc, Lc2 = torch.zeros_like(s.LAM), torch.zeros_like(s.LAM)
+            e.iterate(target_iters, finalize=False, X=Xc, X2=Xc2, L=Lc, L2=Lc2)
             F_all.append(F.clone())
             T_all.append(target_logit(F, Lc[:e.m], scale, float(target_scale)))
-        if theta is None:
-            e.iterate(s.X, s.LAM, s.ACC, iters, first=True)
-        else:
-            e.warm(theta, F, Y)
-            e.iterate(s.X, s.LAM, s.ACC, iters, first=False)
-        e.finalize()
+        e.iterate(iters, theta=th)
     return torch.cat(F_all), torch.cat(T_all)
 
 
 def fit(make_scene, substeps: int = 120, settle_frames: int = 2, steps: int = 800,
-        batch: int = 16384, seed: int = 0, log: list | None = None):
+        batch: int = 16384, seed: int = 0, log: list | None = None,
+        onpolicy_iters=(1, 2)):
     """Two rounds: data from the plain solver's trajectory, training, data
-    from the learned start's own trajectory, training on both.
+    from the learned start's own trajectories (one per iteration count in
+    ``onpolicy_iters``: the states a start followed by one iteration meets
+    differ from those after two), training on all of it.
     ``make_scene()`` returns a fresh Jacobi ``Solver``; returns theta."""
     import torch
 
@@ -229,16 +236,22 @@ def fit(make_scene, substeps: int = 120, settle_frames: int = 2, steps: int = 80
 
     F, T = collect(fresh(), substeps)
     theta, h1 = train(F, T, steps=steps, batch=batch, lr=3e-3, seed=seed, log_every=steps)
-    F2, T2 = collect(fresh(), substeps, theta=theta)
-    ok = bool(torch.isfinite(F2).all() and torch.isfinite(T2).all())
-    if ok:
-        theta, h2 = train(torch.cat([F, F2]), torch.cat([T, T2]), steps=steps, batch=batch,
-                          lr=1e-3, seed=seed + 1, theta=theta, log_every=steps)
-    else:
-        h2 = []
+    Fs, Ts = [F], [T]
+    ok = True
+    for k in onpolicy_iters:
+        F2, T2 = collect(fresh(), substeps, theta=theta, iters=k)
+        if bool(torch.isfinite(F2).all() and torch.isfinite(T2).all()):
+            Fs.append(F2)
+            Ts.append(T2)
+        else:
+            ok = False
+    h2 = []
+    if len(Fs) > 1:
+        theta, h2 = train(torch.cat(Fs), torch.cat(Ts), steps=steps, batch=batch, lr=1e-3,
+                          seed=seed + 1, theta=theta, log_every=steps)
     if log is not None:
         log.append({"round1_loss": h1, "round2_loss": h2, "round2_data_finite": ok,
-                    "samples": int(F.shape[0] + (F2.shape[0] if ok else 0))})
+                    "samples": int(sum(f.shape[0] for f in Fs))})
     return theta
 
 
@@ -285,8 +298,7 @@ def predict(theta, F):
 
 class Evaluator:
     """Plain zero start against the learned start, each on its own trajectory
-    from the same initial state (what a deployed solver sees, including its
-    own previous-substep lambda).  Per substep the RMS residual after the
+    from the same initial state.  Per substep the RMS residual after the
     iterations is recorded; ``run`` returns the RMS over the substeps per
     configuration and, per learned iteration count k, the number of substeps
     whose residual exceeded the plain run's at the same k and substep (the
@@ -296,22 +308,22 @@ class Evaluator:
     def __init__(self, s, theta):
         import torch
 
-        _prepare(s)
         self.s = s
         self.e = _Eager(s)
-        self.theta = theta
-        self.F = torch.empty((self.e.m, 12), device=s.device)
-        self.Y = torch.empty(self.e.m, device=s.device)
+        self.theta = torch.as_tensor(theta, dtype=torch.float32).to(s.device).contiguous()
 
-    def run(self, substeps: int, plain_iters=(1, 2, 4), learned_iters=(1, 2)):
-        s, e = self.s, self.e
+    def _keep(self):
+        s = self.s
         keep = [t.clone() for t in (s.X, s.V, s.XP, s.LAM)]
 
         def restore():
             for dst, src in zip((s.X, s.V, s.XP, s.LAM), keep):
                 dst.copy_(src)
-            s.ACC.zero_()
+        return restore
 
+    def run(self, substeps: int, plain_iters=(1, 2, 4), learned_iters=(1, 2)):
+        e = self.e
+        restore = self._keep()
         runs = {}
         for kind, its in [("plain", k) for k in plain_iters] + [("learned", k)
                                                                  for k in learned_iters]:
@@ -319,13 +331,8 @@ class Evaluator:
             r = []
             for _ in range(substeps):
                 e.integrate()
-                if kind == "learned":
-                    e.warm(self.theta, self.F, self.Y)
-                    e.iterate(s.X, s.LAM, s.ACC, its, first=False)
-                else:
-                    e.iterate(s.X, s.LAM, s.ACC, its, first=True)
+                e.iterate(its, theta=self.theta if kind == "learned" else None)
                 r.append(e.residual())
-                e.finalize()
             runs[(kind, its)] = np.asarray(r)
         restore()
         out = {f"{kind}_{k}": float(np.sqrt(np.mean(np.square(v)))) for (kind, k), v in runs.items()}
@@ -357,26 +364,19 @@ class Guard:
 
     def compare(self) -> tuple[float, float]:
         """(plain, learned) RMS residual of one substep from the current state."""
-        import torch
-
         s = self.s
         e = _Eager(s)
         keep = [t.clone() for t in (s.X, s.V, s.XP, s.LAM)]
-        F = torch.empty((e.m, 12), device=s.device)
-        Y = torch.empty(e.m, device=s.device)
 
         def restore():
             for dst, src in zip((s.X, s.V, s.XP, s.LAM), keep):
                 dst.copy_(src)
-            s.ACC.zero_()
 
         out = []
         for learned in (False, True):
             restore()
             e.integrate()
-            if learned:
-                e.warm(self.theta, F, Y)
-            e.iterate(s.X, s.LAM, s.ACC, self.iters, first=not learned)
+            e.iterate(self.iters, theta=self.theta if learned else None)
             out.append(e.residual())
         restore()
         return out[0], out[1]

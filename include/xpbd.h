@@ -43,9 +43,6 @@ extern "C" {
 /* Largest particle tile of the fused Jacobi iteration (44 B of LDS per particle:
  * three 64-bit fixed-point sums, a count and the staged position). */
 #define XPBD_MAX_TILE 1024
-/* Largest tile of the warm-start launch (the network's weights and per-wave
- * tiles take another 28 KB of LDS; the total stays within 64 KB). */
-#define XPBD_MAX_WARM_TILE 512
 
 /* ---- 3-DOF particle kernels ------------------------------------------- */
 
@@ -148,21 +145,19 @@ int xpbd_frame_block(void* stream, float* X, float* V, float* XP, const int* R, 
                      int n_rows, int substeps, int iterations, float h, float gx, float gy,
                      float gz, float damping, float relax, float max_speed, int gs, int frames);
 /* Learned warm start of a substep (no reference counterpart; it replaces the
- * zero start of solver.py:431), launched before the substep's Jacobi
- * iterations, which then run with first = 0 from LAMout.  Every row
- * evaluates the MLP theta on its features (as xpbd_ws_features computes
- * them) and starts from lambda0 = 4 sigmoid(y) (-C / (w + alpha~)) (one-sided
- * rows only while violated), written to LAMout; each particle moves by relax
- * x the mean over its rows of w_i grad C lambda0 (Xin -> Xout, Xin !=
- * Xout).  predict = 1 folds integrate in as xpbd_jacobi_tile_first does (Xin
- * = positions at the start of the substep; XP written; V read), predict = 0
- * reads integrated positions and XP.  Y (optional, float[n_rows]) receives
- * the network output of each row.  Tile tables as xpbd_jacobi_tile;
- * tile_size <= XPBD_MAX_WARM_TILE. */
+ * zero start of solver.py:431), launched after xpbd_ws_net and before the
+ * substep's Jacobi iterations, which then run with first = 0 from LAMout.
+ * Every row starts from lambda0 = 4 sigmoid(Y[row]) (-C / (w + alpha~))
+ * (one-sided rows only while violated), written to LAMout; each particle
+ * moves by relax x the mean over its rows of w_i grad C lambda0 (Xin ->
+ * Xout, Xin != Xout).  predict = 1 folds integrate in as
+ * xpbd_jacobi_tile_first does (Xin = positions at the start of the substep;
+ * XP written; V read), predict = 0 reads integrated positions.  Tile tables
+ * as xpbd_jacobi_tile. */
 int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP, const float* V,
                           const int* R, const float* ANC, float* LAMout,
                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
-                          const float* theta, float* Y, float h, float gx, float gy, float gz,
+                          const float* Y, float h, float gx, float gy, float gz,
                           const float* gscale, int tile_size, int n_particles, float relax,
                           int predict);
 /* Largest particle count of xpbd_frame_block (LDS-bound). */
@@ -211,12 +206,12 @@ typedef struct xpbd_particles_desc {
   const int* dyn_count;  /* int[1], device */
   int dyn_cap;
   const float* gscale;   /* float[n_particles] gravity factor, or NULL (tile mode) */
-  /* learned warm start (XPBD_MODE_JACOBI_TILE): MLP parameters, or NULL for
-   * the reference's zero start (xpbd_jacobi_tile_warm runs the first
-   * iteration of every substep); ws_feat and ws_y are unused */
-  const float* ws_theta; /* float[5057] */
-  float* ws_feat;
-  float* ws_y;
+  /* learned warm start (XPBD_MODE_JACOBI_TILE): the network packed by
+   * xpbd_mlp_pack, or NULL for the reference's zero start; every substep
+   * then starts with xpbd_ws_net (outputs into ws_y) and
+   * xpbd_jacobi_tile_warm */
+  const void* ws_image;
+  float* ws_y;           /* float[n_rows] */
   /* Tile mode with >= 2 iterations and no dyn rows: 1 folds integrate into
    * the first iteration (xpbd_jacobi_tile_first), 0 keeps a separate
    * xpbd_integrate_to launch per substep.  Results are identical. */
@@ -376,12 +371,17 @@ int xpbd6_broadphase_hashgrid(void* stream, void* ws, const float* X, const floa
 unsigned xpbd_morton3(float x, float y, float z);
 
 /* ---- learned lambda warm start (no reference counterpart) -----------------
- * MLP 12 -> 64 -> 64 -> 1 (exact-erf GELU), 5057 float32 parameters
+ * MLP 12 -> 64 -> 64 -> 1 (GELU, tanh form), 5057 float32 parameters
  * W1[12][64] b1[64] W2[64][64] b2[64] W3[64] b3; BF16 MFMA products with
  * FP32 accumulation.  It predicts the multipliers that replace the zero reset
  * of solver.py:431.  F is float[m][12] (features), Y/T float[m]. */
 int xpbd_mlp_param_count(void);
 int xpbd_mlp_forward(void* stream, const float* theta, const float* F, float* Y, int m);
+/* The network packed for the warm-start network pass (bfloat16 weights in
+ * the LDS layout, then b3): image of xpbd_mlp_image_bytes() bytes, 16-byte
+ * aligned. */
+long long xpbd_mlp_image_bytes(void);
+int xpbd_mlp_pack(void* stream, const float* theta, void* image);
 long long xpbd_mlp_train_workspace_bytes(int nblocks);
 /* Gradient of the mean squared error over m rows (grad float[5057]) and the
  * loss (loss float[1]); nblocks partial sums reduced in block order. */
@@ -397,6 +397,14 @@ int xpbd_mlp_adam(void* stream, float* theta, float* mom, float* vel, const floa
  * rows, max(0, -C) for one-sided rows) added to *out_sum. */
 int xpbd_ws_features(void* stream, const float* X, const float* XP, const int* R,
                      const float* ANC, float h, float* F, int n_rows);
+/* The network pass of a warm-started substep: per row the features (as
+ * xpbd_ws_features; at the positions integrate would predict from Xin and V
+ * when predict = 1, x_prev = Xin; otherwise at Xin with x_prev XP) and the
+ * network's output Y[row] (as xpbd_mlp_forward on those features); image =
+ * the network packed by xpbd_mlp_pack. */
+int xpbd_ws_net(void* stream, const float* Xin, const float* XP, const float* V, const int* R,
+                const float* ANC, const void* image, float* Y, float h, float gx, float gy,
+                float gz, const float* gscale, int n_rows, int predict);
 int xpbd_ws_residual(void* stream, const float* X, const int* R, const float* ANC,
                      const float* LAM, float h, double* out_sum, int n_rows);
 

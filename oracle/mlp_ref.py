@@ -1,5 +1,5 @@
 """ORACLE — test infrastructure only.  NumPy reference of the warm-start MLP
-(12 -> 64 -> 64 -> 1, exact-erf GELU) used to check the HIP training and
+(12 -> 64 -> 64 -> 1, GELU in its tanh form) used to check the HIP training and
 inference kernels.  No reference counterpart: the learned warm start is new.
 
 Mixed precision is emulated at the points where the device kernels round:
@@ -41,14 +41,27 @@ def bf16(x: np.ndarray) -> np.ndarray:
 
 
 _erf = np.vectorize(math.erf, otypes=[np.float64])
+C0 = 2.0 * math.sqrt(2.0 / math.pi)
+C1 = 0.044715
 
 
-def gelu(z):
+def gelu_exact(z):
+    """0.5 z (1 + erf(z / sqrt 2)), the definition the tanh form approximates."""
     return 0.5 * z * (1.0 + _erf(z / math.sqrt(2.0)))
 
 
+def _sig(z):
+    return 1.0 / (1.0 + np.exp(-C0 * z * (1.0 + C1 * z * z)))
+
+
+def gelu(z):
+    """Tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))) = z sigmoid(u)."""
+    return z * _sig(z)
+
+
 def gelu_grad(z):
-    return 0.5 * (1.0 + _erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2 * math.pi)
+    s = _sig(z)
+    return s + z * s * (1.0 - s) * C0 * (1.0 + 3.0 * C1 * z * z)
 
 
 def init_params(seed: int = 0) -> np.ndarray:

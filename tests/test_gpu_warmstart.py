@@ -29,14 +29,24 @@ def _theta(seed=0, b3=None):
 
 
 def _warm(s, src, dst, lam, theta, Y=None, predict=0):
+    """Network pass + warm-start launch."""
+    import torch
+
     from xpbd_amd import _native as N
 
     h = float(np.float32(s.dt / s.substeps))
     trs, hs, hr = (N.ptr(t) for t in s._tile_dev)
+    if Y is None:
+        Y = torch.empty(s.num_constraints, device=DEV)
+    img = torch.empty(int(N.lib().xpbd_mlp_image_bytes()), dtype=torch.uint8, device=DEV)
+    N.check(N.lib().xpbd_mlp_pack(None, N.ptr(theta), N.ptr(img)))
+    N.check(N.lib().xpbd_ws_net(None, N.ptr(src), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R),
+                                N.ptr(s.ANC), N.ptr(img), N.ptr(Y), h, *s.gravity, None,
+                                s.num_constraints, predict))
     N.check(N.lib().xpbd_jacobi_tile_warm(
         None, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), N.ptr(s.ANC),
-        N.ptr(lam), trs, hs, hr, N.ptr(theta), N.ptr(Y) if Y is not None else None, h,
-        *s.gravity, None, s._tile, s.num_bodies, s.jacobi_relax, predict))
+        N.ptr(lam), trs, hs, hr, N.ptr(Y), h, *s.gravity, None, s._tile, s.num_bodies,
+        s.jacobi_relax, predict))
 
 
 def _perturbed_state(s, seed, below_floor=False):
@@ -60,9 +70,9 @@ def _perturbed_state(s, seed, below_floor=False):
 
 
 def test_warm_launch_matches_oracle():
-    """One warm-start launch on a jittered 24^2 cloth, half of it below the
-    floor, with random velocities and a random network: (1) every row's
-    network output equals xpbd_ws_features -> xpbd_mlp_forward bit for bit;
+    """Network pass + warm-start launch on a jittered 24^2 cloth, half of it
+    below the floor, with random velocities and a random network: (1) every
+    row's output equals xpbd_ws_features -> xpbd_mlp_forward bit for bit;
     (2) those features match the float64 oracle; (3) lambda0 and the moved
     positions match oracle/ws_ref.warm_launch given the same outputs."""
     import torch
@@ -151,8 +161,9 @@ def test_zero_start_is_the_plain_solver():
 
 
 def test_warm_launch_folded_integrate_equals_integrate_then_warm():
-    """The warm launch with integrate folded in equals integrate followed by
-    the warm launch bit for bit (positions, multipliers, network outputs)."""
+    """The network pass and warm-start launch with integrate folded in equal
+    integrate followed by both bit for bit (positions, multipliers, network
+    outputs)."""
     import torch
 
     from xpbd_amd import _native as N

@@ -41,6 +41,12 @@ def test_oracle_gelu_and_bf16():
     assert M.gelu(np.array([0.0]))[0] == 0.0
     assert np.allclose(M.gelu(z) - M.gelu(-z), z)
     assert M.gelu_grad(np.array([0.0]))[0] == pytest.approx(0.5)
+    # the tanh form stays within 1e-3 of the erf definition (its published
+    # worst case is ~5e-4 near |z| = 2); the derivative by central differences
+    g = np.linspace(-6.0, 6.0, 2401)
+    assert np.abs(M.gelu(g) - M.gelu_exact(g)).max() < 1e-3
+    fd = (M.gelu(g + 1e-6) - M.gelu(g - 1e-6)) / 2e-6
+    assert np.allclose(M.gelu_grad(g), fd, atol=1e-7)
     # bf16 keeps 8 significant bits, ties to even
     x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 3.14159265], np.float32)
     assert M.bf16(x).tolist() == [1.0, 1.0, 1.0 + 2 ** -6, 3.140625]

@@ -36,7 +36,7 @@ class ParticlesDesc(ctypes.Structure):
         ("halo_start", c_void_p), ("halo_rows", c_void_p), ("tile_size", c_int),
         ("dyn_rows", c_void_p), ("dyn_lam", c_void_p), ("dyn_count", c_void_p),
         ("dyn_cap", c_int), ("gscale", c_void_p),
-        ("ws_theta", c_void_p), ("ws_feat", c_void_p), ("ws_y", c_void_p),
+        ("ws_image", c_void_p), ("ws_y", c_void_p),
         ("fuse_first", c_int), ("block", c_int), ("color_starts_dev", c_void_p),
     ]
 
@@ -118,11 +118,14 @@ _SIGS = {
     "xpbd_morton3": (ctypes.c_uint, [_F, _F, _F]),
     "xpbd_mlp_param_count": (c_int, []),
     "xpbd_mlp_forward": (c_int, [_P, _P, _P, _P, _I]),
+    "xpbd_mlp_image_bytes": (ctypes.c_longlong, []),
+    "xpbd_mlp_pack": (c_int, [_P, _P, _P]),
     "xpbd_mlp_train_workspace_bytes": (ctypes.c_longlong, [_I]),
     "xpbd_mlp_grad": (c_int, [_P, _P, _P, _P, _I, _I, _P, _P, _P]),
     "xpbd_mlp_adam": (c_int, [_P, _P, _P, _P, _P, _F, _F, _F, _F, _I]),
     "xpbd_ws_features": (c_int, [_P] * 5 + [_F, _P, _I]),
-    "xpbd_jacobi_tile_warm": (c_int, [_P] * 13 + [_F, _F, _F, _F, _P, _I, _I, _F, _I]),
+    "xpbd_ws_net": (c_int, [_P] * 8 + [_F, _F, _F, _F, _P, _I, _I]),
+    "xpbd_jacobi_tile_warm": (c_int, [_P] * 12 + [_F, _F, _F, _F, _P, _I, _I, _F, _I]),
     "xpbd_ws_residual": (c_int, [_P] * 5 + [_F, _P, _I]),
     "xpbd_build_info": (ctypes.c_char_p, []),
     "xpbd_error_string": (ctypes.c_char_p, [c_int]),

@@ -30,11 +30,11 @@ def _signature(s):
 
 
 def candidates(n_particles: int, tile_only: bool = False):
-    """Launch shapes to time; ``tile_only`` (learned warm start): tiles up to
-    512 (the warm launch's LDS limit) and no window / block shapes."""
+    """Launch shapes to time; ``tile_only`` (learned warm start): no window
+    or block shapes."""
     out = []
     for t in (64, 128, 256, 512, 1024):
-        if t <= max(64, 2 * n_particles) and not (tile_only and t > 512):
+        if t <= max(64, 2 * n_particles):
             out.append(("tile", t, None, True))
             out.append(("tile", t, None, False))
     if tile_only:

@@ -44,9 +44,9 @@ struct Frame3 {
 // X_alt, chosen so that after the substep's launches the state is back in X)
 // and the plain launches.
 //
-// With the learned warm start (ws_theta) every substep first runs
-// xpbd_jacobi_tile_warm (integrate folded in on the fused path) and then all
-// `iterations` plain launches with first = 0.  Multipliers ping-pong as the
+// With the learned warm start (ws_image) every substep first runs
+// xpbd_ws_net and xpbd_jacobi_tile_warm (integrate folded into both on the
+// fused path) and then all `iterations` plain launches with first = 0.  Multipliers ping-pong as the
 // positions do: a substep's first launch writes lam_alt.
 int issue_frame_tile(Frame3* f, hipStream_t s) {
   const xpbd_particles_desc& d = f->d;
@@ -55,23 +55,26 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
   const bool have_rows = (d.n_rows > 0 || d.dyn_cap > 0) && d.iterations > 0;
   const int it_n = have_rows ? d.iterations : 0;
   const bool dyn = d.dyn_cap > 0 && d.dyn_rows && d.dyn_count;
-  const bool warm = d.ws_theta != nullptr && d.n_rows > 0 && it_n > 0;
+  const bool warm = d.ws_image != nullptr && d.n_rows > 0 && it_n > 0;
   float* bufs[2] = {d.x, d.x_alt};
   float* lams[2] = {d.lam, d.lam_alt};
   int rc;
   if (d.fuse_first && !dyn && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1))) {
     int loc = 0;
     for (int sub = 0; sub < d.substeps; ++sub) {
-      if (warm)
+      if (warm) {
+        rc = xpbd_ws_net(s, bufs[loc], d.x_prev, d.v, d.rows, d.anchors, d.ws_image, d.ws_y, d.h,
+                         d.gx, d.gy, d.gz, d.gscale, d.n_rows, 1);
+        if (rc) return rc;
         rc = xpbd_jacobi_tile_warm(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
-                                   lams[1], d.tile_row_start, d.halo_start, d.halo_rows,
-                                   d.ws_theta, nullptr, d.h, d.gx, d.gy, d.gz, d.gscale,
-                                   d.tile_size, n, d.relax, 1);
-      else
+                                   lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.ws_y,
+                                   d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n, d.relax, 1);
+      } else {
         rc = xpbd_jacobi_tile_first(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows,
                                     d.anchors, lams[1], d.tile_row_start, d.halo_start,
                                     d.halo_rows, d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n,
                                     d.relax);
+      }
       if (rc) return rc;
       loc ^= 1;
       int li = 1;  // lams[li] holds the current multipliers
@@ -98,10 +101,12 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
     if (rc) return rc;
     int li = 0;
     if (warm) {
+      rc = xpbd_ws_net(s, bufs[cur], d.x_prev, d.v, d.rows, d.anchors, d.ws_image, d.ws_y, d.h,
+                       0.0f, 0.0f, 0.0f, nullptr, d.n_rows, 0);
+      if (rc) return rc;
       rc = xpbd_jacobi_tile_warm(s, bufs[cur], bufs[cur ^ 1], d.x_prev, d.v, d.rows, d.anchors,
-                                 lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.ws_theta,
-                                 nullptr, d.h, 0.0f, 0.0f, 0.0f, nullptr, d.tile_size, n, d.relax,
-                                 0);
+                                 lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.ws_y,
+                                 d.h, 0.0f, 0.0f, 0.0f, nullptr, d.tile_size, n, d.relax, 0);
       if (rc) return rc;
       cur ^= 1;
       li = 1;
@@ -196,9 +201,9 @@ int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
       (!desc->x_alt || !desc->lam_alt || !desc->tile_row_start || !desc->halo_start ||
        desc->tile_size < 1 || desc->tile_size > XPBD_MAX_TILE))
     return (int)hipErrorInvalidValue;
-  if (desc->ws_theta &&
-      (desc->mode != XPBD_MODE_JACOBI_TILE || desc->tile_size > XPBD_MAX_WARM_TILE))
-    return (int)hipErrorInvalidValue;  // learned warm start: tile mode, tiles <= 512
+  if (desc->ws_image &&
+      (desc->mode != XPBD_MODE_JACOBI_TILE || !desc->ws_y))
+    return (int)hipErrorInvalidValue;  // learned warm start: tile mode
   if (desc->mode != XPBD_MODE_JACOBI_TILE && (desc->dyn_cap > 0 || desc->gscale))
     return (int)hipErrorInvalidValue;  // device-counted rows / gravity scale: tile mode only
   if (desc->mode == XPBD_MODE_JACOBI_TILE && desc->dyn_cap > 0 &&
@@ -208,7 +213,7 @@ int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
     return (int)hipErrorInvalidValue;
   if (desc->mode == XPBD_MODE_JACOBI && desc->n_rows > 0 && !desc->acc) return (int)hipErrorInvalidValue;
   if (desc->block &&
-      (desc->mode == XPBD_MODE_JACOBI_TILE || desc->ws_theta ||
+      (desc->mode == XPBD_MODE_JACOBI_TILE || desc->ws_image ||
        desc->n_particles > xpbd_frame_block_capacity(desc->mode == XPBD_MODE_GS) ||
        (desc->mode == XPBD_MODE_GS && desc->n_rows > 0 && !desc->color_starts_dev)))
     return (int)hipErrorInvalidValue;

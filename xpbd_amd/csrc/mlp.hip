@@ -1,4 +1,4 @@
-// Warm-start MLP (12 -> 64 -> 64 -> 1, exact-erf GELU) on gfx950 MFMA.
+// Warm-start MLP (12 -> 64 -> 64 -> 1, tanh-form GELU) on gfx950 MFMA.
 //
 // No reference counterpart: the learned lambda warm start is new; it
 // replaces the zero reset of the multipliers at the start of a substep
@@ -49,6 +49,15 @@ __device__ inline void forward_tile(Lds& s, const float* __restrict__ F, int row
     return (row < m && k < NIN) ? to_bf16(F[(size_t)row * NIN + k]) : (unsigned short)0;
   };
   forward16(s.w, s.tile[wv], feat, b3, z1, z2, y, xa);
+}
+
+__global__ __launch_bounds__(kBlk) void k_mlp_pack(const float* __restrict__ theta,
+                                                   Image* __restrict__ img) {
+  load_weights(img->w, theta, threadIdx.x, kBlk);
+  if (threadIdx.x == 0) {
+    img->b3 = theta[P_B3];
+    img->pad[0] = img->pad[1] = img->pad[2] = 0.0f;
+  }
 }
 
 __global__ __launch_bounds__(kBlk) void k_mlp_forward(const float* __restrict__ theta,
@@ -140,7 +149,7 @@ __global__ __launch_bounds__(kBlk) void k_mlp_train_partial(const float* __restr
       f32x4 acc{0.f, 0.f, 0.f, 0.f};
       for (int sK = 0; sK < 4; ++sK) {
         bf16x4 b;  // B[k][n] = W2[n][k]
-        for (int j = 0; j < 4; ++j) b[j] = (short)s.w.w2[16 * nt + r16][16 * sK + 4 * g + j];
+        for (int j = 0; j < 4; ++j) b[j] = (short)s.w.w2t[16 * sK + 4 * g + j][16 * nt + r16];
         acc = mfma(a[sK], b, acc);
       }
       for (int j = 0; j < 4; ++j) {
@@ -231,6 +240,16 @@ int xpbd_mlp_forward(void* stream, const float* theta, const float* F, float* Y,
   int nb = (m + 16 * kWaves - 1) / (16 * kWaves);
   if (nb > 2048) nb = 2048;
   hipLaunchKernelGGL(k_mlp_forward, dim3(nb), dim3(kBlk), 0, (hipStream_t)stream, theta, F, Y, m);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+long long xpbd_mlp_image_bytes(void) { return (long long)sizeof(Image); }
+
+int xpbd_mlp_pack(void* stream, const float* theta, void* image) {
+  if (!theta || !image) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(kBlk), 0, (hipStream_t)stream, theta,
+                     (Image*)image);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

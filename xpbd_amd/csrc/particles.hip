@@ -17,7 +17,6 @@
 #include <cmath>
 
 #include "common.h"
-#include "mlp_dev.h"
 #include "ws_dev.h"
 #include "xpbd.h"
 
@@ -512,61 +511,39 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
 
 // ---------------------------------------------------------------------------
 // Learned warm start launch (no reference counterpart; it replaces the zero
-// start of solver.py:431), run once per substep before the Jacobi
+// start of solver.py:431), run once per substep after the network pass
+// (xpbd_ws_net, which writes each row's output y) and before the Jacobi
 // iterations, which then continue from its multipliers (first = 0).
 //
 // Same tiles, staging (with integrate folded in when PREDICT) and exact
-// accumulation as k_jacobi_tile.  Each wave takes 64 rows per pass: every
-// lane computes its row's features (ws_dev.h) into LDS, the wave evaluates
-// the network on them (mlp_dev.h, four 16-row MFMA tiles), and every row
-// then starts from lambda0 = 4 sigmoid(y) lambda1, lambda1 = -C / (w +
-// alpha~) its first plain step (one-sided rows only while violated), writes
-// lambda0 and adds w_i grad C lambda0 to its endpoints; each particle moves
-// by relax x the mean of those offsets, the position a Jacobi sweep reaches
+// accumulation as k_jacobi_tile.  Every row (owned and halo) starts from
+// lambda0 = 4 sigmoid(y) lambda1, lambda1 = -C / (w + alpha~) its first
+// plain step (one-sided rows only while violated), writes lambda0 (owner)
+// and adds w_i grad C lambda0 to its endpoints; each particle moves by
+// relax x the mean of those offsets, the position a Jacobi sweep reaches
 // when its multipliers add up to lambda0 (Jacobi applies the mean of a
-// particle's corrections, kernels.py:326-339).  The velocity features use
-// (x - x_prev) / h.  Y (optional) receives each owned row's network output.
+// particle's corrections, kernels.py:326-339).  The normal is d / L, as the
+// feature pass computes it.
 // ---------------------------------------------------------------------------
-constexpr int kWarmBlock = 256;
-constexpr int kWarmWaves = kWarmBlock / 64;
-static_assert(XPBD_MAX_WARM_TILE * 44 + 11008 + 4 * 4352 + 16 <= 65536, "warm launch LDS");
-
-__host__ __device__ inline size_t warm_lds_net_offset(int P) {
-  return (tile_lds_bytes(P) + 15) & ~(size_t)15;
-}
-inline size_t warm_lds_bytes(int P) {
-  return warm_lds_net_offset(P) + sizeof(xpbd_mlp::Weights) +
-         (size_t)kWarmWaves * (64 * 16 * 2 + 16 * 64 * 2 + 64 * 4);
-}
-
-template <bool PREDICT>
-__global__ __launch_bounds__(kWarmBlock) void k_jacobi_tile_warm(
+template <int BLOCK, bool PREDICT>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_tile_warm(
     const float4* __restrict__ Xin, float4* __restrict__ Xout, float4* __restrict__ XP,
     const float4* __restrict__ V, const int4* __restrict__ R, const float4* __restrict__ ANC,
     float* __restrict__ LAMout, const int* __restrict__ tile_row_start,
     const int* __restrict__ halo_start, const int* __restrict__ halo_rows,
-    const float* __restrict__ theta, float* __restrict__ Yout, float h, int P, int n,
-    float relax, float gx, float gy, float gz, const float* __restrict__ gscale) {
-  using namespace xpbd_mlp;
+    const float* __restrict__ Y, float h, int P, int n, float relax, float gx, float gy,
+    float gz, const float* __restrict__ gscale) {
   extern __shared__ unsigned long long lds64[];
   unsigned long long* sx = lds64;
   unsigned long long* sy = lds64 + P;
   unsigned long long* sz = lds64 + 2 * P;
   unsigned* cnt = (unsigned*)(lds64 + 3 * P);
   float4* spos = (float4*)(lds64 + tile_lds_pos_offset(P) / sizeof(unsigned long long));
-  char* net = (char*)lds64 + warm_lds_net_offset(P);
-  Weights& wts = *(Weights*)net;
-  unsigned short(*ftile)[64][16] = (unsigned short(*)[64][16])(net + sizeof(Weights));
-  unsigned short(*htile)[16][NH] =
-      (unsigned short(*)[16][NH])(net + sizeof(Weights) + kWarmWaves * 64 * 16 * 2);
-  float(*ysh)[64] =
-      (float(*)[64])(net + sizeof(Weights) + kWarmWaves * (64 * 16 * 2 + 16 * NH * 2));
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int p0 = tile * P;
   if (p0 >= n) return;
   const int p1 = min(p0 + P, n);
   const float inv_dt2 = 1.0f / (h * h);
-  const float inv_h = 1.0f / h;
   auto predict = [&](float4 x4, int i) -> float4 {
     if (!PREDICT || x4.w == 0.0f) return x4;
     f3 g = mk3(gx, gy, gz);
@@ -575,16 +552,14 @@ __global__ __launch_bounds__(kWarmBlock) void k_jacobi_tile_warm(
     const f3 x = xyz(x4) + v * h;
     return make_float4(x.x, x.y, x.z, x4.w);
   };
-  for (int k = threadIdx.x; k < 3 * P; k += kWarmBlock) lds64[k] = 0ull;
-  for (int k = threadIdx.x; k < P; k += kWarmBlock) cnt[k] = 0u;
-  load_weights(wts, theta, threadIdx.x, kWarmBlock);
-  for (int i = p0 + threadIdx.x; i < p1; i += kWarmBlock) {
+  for (int k = threadIdx.x; k < 3 * P; k += BLOCK) lds64[k] = 0ull;
+  for (int k = threadIdx.x; k < P; k += BLOCK) cnt[k] = 0u;
+  for (int i = p0 + threadIdx.x; i < p1; i += BLOCK) {
     const float4 x0 = Xin[i];
     spos[i - p0] = predict(x0, i);
     if (PREDICT) XP[i] = make_float4(x0.x, x0.y, x0.z, 0.0f);
   }
   __syncthreads();
-  const float b3 = theta[P_B3];
   const int rb = tile_row_start[tile], re = tile_row_start[tile + 1];
   const int hb = halo_start[tile], he = halo_start[tile + 1];
   const int owned = re - rb;
@@ -594,7 +569,6 @@ __global__ __launch_bounds__(kWarmBlock) void k_jacobi_tile_warm(
     if (off < (unsigned)(p1 - p0)) return spos[off];
     return predict(Xin[i], i);
   };
-  auto prev = [&](int i) { return PREDICT ? xyz(Xin[i]) : xyz(XP[i]); };
   auto add = [&](int i, f3 c) {
     const unsigned off = (unsigned)(i - p0);
     if (off >= (unsigned)P) return;
@@ -604,94 +578,49 @@ __global__ __launch_bounds__(kWarmBlock) void k_jacobi_tile_warm(
     atomicAdd(&sz[off], fix_bits(fminf(fmaxf(c.z, -kFixLimit), kFixLimit)));
     atomicAdd(&cnt[off], ok ? 1u : (kFixPoison | 1u));
   };
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int base = 0; base < total; base += kWarmBlock) {
-    const int k = base + threadIdx.x;
-    int j = -1;
-    int4 row = make_int4(0, 0, 0, 0);
-    if (k < total) {
-      j = k < owned ? rb + k : halo_rows[hb + (k - owned)];
-      row = R[j];
-    }
+  for (int k = threadIdx.x; k < total; k += BLOCK) {
+    const int j = k < owned ? rb + k : halo_rows[hb + (k - owned)];
+    const int4 row = R[j];
+    const float yv = Y[j];
     const int t = row_type(row);
     const int a = row_a(row), bb = row.y;
-    // geometry as the feature kernel computes it (the exact normal d / L)
-    float C = 0.0f, w = 0.0f, wa = 0.0f, wb = 0.0f, alpha = 0.0f, pen = 0.0f;
+    const float p0r = __int_as_float(row.z);
+    const float4 xa4 = pos(a);
+    const float wa = xa4.w;
+    float C, w, wb = 0.0f, alpha = 0.0f;
     f3 nrm = mk3(0.0f, 1.0f, 0.0f);
     bool ok = true;
-    float f[12];
-    for (int q = 0; q < 12; ++q) f[q] = 0.0f;
-    if (j >= 0) {
-      const float p0r = __int_as_float(row.z);
-      const float4 xa4 = pos(a);
-      wa = xa4.w;
-      f3 vrel = (xyz(xa4) - prev(a)) * inv_h;
-      if (t == T_FLOOR) {
-        C = xa4.y - p0r;
-        w = wa;
-        pen = fmaxf(0.0f, -C);
+    if (t == T_FLOOR) {
+      C = xa4.y - p0r;
+      w = wa;
+    } else {
+      f3 pb;
+      if (t == T_ATTACH) {
+        pb = xyz(ANC[j]);
       } else {
-        f3 pb;
-        if (t == T_ATTACH) {
-          pb = xyz(ANC[j]);
-        } else {
-          const float4 xb4 = pos(bb);
-          pb = xyz(xb4);
-          wb = xb4.w;
-          vrel = vrel - (xyz(xb4) - prev(bb)) * inv_h;
-        }
-        const f3 d = xyz(xa4) - pb;
-        const float L = length(d);
-        ok = L >= XPBD_EPS;
-        nrm = ok ? d / L : mk3(0.0f, 1.0f, 0.0f);
-        C = L - p0r;
-        w = wa + wb;
-        if (t == T_CONTACT) pen = fmaxf(0.0f, -C);
-        else alpha = __int_as_float(row.w) * inv_dt2;
+        const float4 xb4 = pos(bb);
+        pb = xyz(xb4);
+        wb = xb4.w;
       }
-      const float vn = dot(vrel, nrm);
-      const float vt = length(vrel - nrm * vn);
-      xpbd_ws::row_features(t, C, w, alpha, vn, vt, pen, h, f);
+      const f3 d = xyz(xa4) - pb;
+      const float L = length(d);
+      ok = L >= XPBD_EPS;
+      if (ok) nrm = d / L;
+      C = L - p0r;
+      w = wa + wb;
+      if (t != T_CONTACT) alpha = __int_as_float(row.w) * inv_dt2;
     }
-    {  // this lane's features -> LDS (bfloat16, 16 per row)
-      unsigned pk[8];
-      for (int q = 0; q < 6; ++q)
-        pk[q] = (unsigned)to_bf16(f[2 * q]) | ((unsigned)to_bf16(f[2 * q + 1]) << 16);
-      pk[6] = pk[7] = 0u;
-      uint4* dst = (uint4*)&ftile[wv][lane][0];
-      dst[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
-      dst[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    for (int t16 = 0; t16 < 4; ++t16) {
-      f32x4 z1[4], z2[4];
-      float y[4];
-      bf16x4 xa;
-      auto feat = [&](int r, int q) -> unsigned short { return ftile[wv][16 * t16 + r][q]; };
-      forward16(wts, htile[wv], feat, b3, z1, z2, y, xa);
-      if ((lane & 15) == 0)
-        for (int q = 0; q < 4; ++q) ysh[wv][16 * t16 + 4 * (lane >> 4) + q] = y[q];
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    const float yv = ysh[wv][lane];
-    __builtin_amdgcn_wave_barrier();
-    if (j < 0) continue;
     const float sden = w + alpha;
     float step = (sden > 0.0f && ok) ? -C / sden : 0.0f;
     if (t == T_FLOOR || t == T_CONTACT) step = C < 0.0f ? step : 0.0f;
     const float lam0 = xpbd_ws::start_lambda(yv, step);
-    if (k < owned) {
-      LAMout[j] = lam0;
-      if (Yout) Yout[j] = yv;
-    }
+    if (k < owned) LAMout[j] = lam0;
     if (lam0 == 0.0f) continue;
     add(a, nrm * (wa * lam0));
     if (t == T_DISTANCE || t == T_CONTACT) add(bb, nrm * (-wb * lam0));
   }
   __syncthreads();
-  for (int i = p0 + threadIdx.x; i < p1; i += kWarmBlock) {
+  for (int i = p0 + threadIdx.x; i < p1; i += BLOCK) {
     const int o = i - p0;
     const float4 x4 = spos[o];
     const unsigned c = cnt[o];
@@ -1178,25 +1107,28 @@ int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP,
 int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP, const float* V,
                           const int* R, const float* ANC, float* LAMout,
                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
-                          const float* theta, float* Y, float h, float gx, float gy, float gz,
+                          const float* Y, float h, float gx, float gy, float gz,
                           const float* gscale, int tile_size, int n_particles, float relax,
                           int predict) {
   if (n_particles <= 0) return 0;
-  if (tile_size < 1 || tile_size > XPBD_MAX_WARM_TILE || !theta || Xin == Xout)
+  if (tile_size < 1 || tile_size > XPBD_MAX_TILE || !Y || Xin == Xout)
     return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   const int nb = blocks_for(n_particles, tile_size);
-  const size_t shmem = warm_lds_bytes(tile_size);
-  if (predict)
-    hipLaunchKernelGGL(k_jacobi_tile_warm<true>, dim3(nb), dim3(kWarmBlock), shmem, s,
-                       (const float4*)Xin, (float4*)Xout, (float4*)XP, (const float4*)V,
-                       (const int4*)R, (const float4*)ANC, LAMout, tile_row_start, halo_start,
-                       halo_rows, theta, Y, h, tile_size, n_particles, relax, gx, gy, gz, gscale);
-  else
-    hipLaunchKernelGGL(k_jacobi_tile_warm<false>, dim3(nb), dim3(kWarmBlock), shmem, s,
-                       (const float4*)Xin, (float4*)Xout, (float4*)XP, (const float4*)V,
-                       (const int4*)R, (const float4*)ANC, LAMout, tile_row_start, halo_start,
-                       halo_rows, theta, Y, h, tile_size, n_particles, relax, gx, gy, gz, gscale);
+  const size_t shmem = tile_lds_bytes(tile_size);
+#define XPBD_WARM_LAUNCH(B, PR)                                                                \
+  hipLaunchKernelGGL((k_jacobi_tile_warm<B, PR>), dim3(nb), dim3(B), shmem, s,                 \
+                     (const float4*)Xin, (float4*)Xout, (float4*)XP, (const float4*)V,         \
+                     (const int4*)R, (const float4*)ANC, LAMout, tile_row_start, halo_start,   \
+                     halo_rows, Y, h, tile_size, n_particles, relax, gx, gy, gz, gscale)
+  if (tile_size <= 256) {
+    if (predict) XPBD_WARM_LAUNCH(kBlock, true);
+    else XPBD_WARM_LAUNCH(kBlock, false);
+  } else {
+    if (predict) XPBD_WARM_LAUNCH(kTileBlock, true);
+    else XPBD_WARM_LAUNCH(kTileBlock, false);
+  }
+#undef XPBD_WARM_LAUNCH
   XPBD_LAUNCH_CHECK();
   return 0;
 }

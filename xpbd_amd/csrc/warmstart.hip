@@ -1,15 +1,15 @@
 // Learned lambda warm start for the 3-DOF solver (no reference
 // counterpart; it replaces the zero reset of solver.py:431 / Alg. 1 line 4):
-// the row features of the training data and the residual the warm start is
-// judged by.  The start itself runs inside the first Jacobi launch of a
-// substep (k_jacobi_tile_warm, particles.hip): lambda0 from the network, then
-// that iteration's update continues from lambda0 and moves the particles by
-// w grad C (lambda0 + dlambda).  Features and lambda0: ws_dev.h.
+// the row features, the network pass of a substep (features + MLP -> y per
+// row) and the residual the warm start is judged by.  The start itself is
+// applied by the warm-start tile launch (k_jacobi_tile_warm, particles.hip)
+// from y.  Features and lambda0: ws_dev.h; the network: mlp_dev.h.
 #include <cmath>
 
 #include "common.h"
-#include "xpbd.h"
+#include "mlp_dev.h"
 #include "ws_dev.h"
+#include "xpbd.h"
 
 namespace {
 
@@ -24,22 +24,23 @@ struct RowGeo {
   bool ok;
 };
 
-// Constraint value, gradient direction and relative velocity (x - x_prev) / h
-// of one row at the current positions (kernels.py:131-208 conventions).
-__device__ inline RowGeo row_geo(const float4* __restrict__ X, const float4* __restrict__ XP,
-                                 const float4* __restrict__ ANC, int4 row, int j, float h) {
+// Constraint value, gradient direction and relative velocity
+// (x - x_prev) / h of one row (kernels.py:131-208 conventions); pos(i) and
+// prev(i) give a particle's position (x, y, z, inverse mass) and x_prev.
+template <bool VEL, class Pos, class Prev>
+__device__ inline RowGeo row_geo(Pos pos, Prev prev, const float4* __restrict__ ANC, int4 row,
+                                 int j, float h) {
   RowGeo g;
   g.t = row_type(row);
   g.a = row_a(row);
   g.b = row.y;
   const float p0 = __int_as_float(row.z), p1 = __int_as_float(row.w);
-  const float4 xa = X[g.a];
+  const float4 xa = pos(g.a);
   const float inv_h = 1.0f / h;
-  auto vel = [&](int i, float4 x4) { return (xyz(x4) - xyz(XP[i])) * inv_h; };
   g.alpha = 0.0f;
   g.pen = 0.0f;
   g.ok = true;
-  f3 vrel = XP ? vel(g.a, xa) : mk3(0.0f, 0.0f, 0.0f);
+  f3 vrel = VEL ? (xyz(xa) - prev(g.a)) * inv_h : mk3(0.0f, 0.0f, 0.0f);
   if (g.t == T_FLOOR) {
     g.n = mk3(0.0f, 1.0f, 0.0f);
     g.C = xa.y - p0;
@@ -51,10 +52,10 @@ __device__ inline RowGeo row_geo(const float4* __restrict__ X, const float4* __r
     if (g.t == T_ATTACH) {
       pb = xyz(ANC[j]);
     } else {
-      const float4 xb = X[g.b];
+      const float4 xb = pos(g.b);
       pb = xyz(xb);
       wb = xb.w;
-      if (XP) vrel = vrel - vel(g.b, xb);
+      if (VEL) vrel = vrel - (xyz(xb) - prev(g.b)) * inv_h;
     }
     const f3 d = xyz(xa) - pb;
     const float L = length(d);
@@ -77,13 +78,84 @@ __global__ __launch_bounds__(kB) void k_ws_features(const float4* __restrict__ X
                                                     float* __restrict__ F, int m) {
   const int j = blockIdx.x * kB + threadIdx.x;
   if (j >= m) return;
-  const RowGeo g = row_geo(X, XP, ANC, R[j], j, h);
+  const RowGeo g = row_geo<true>([&](int i) { return X[i]; }, [&](int i) { return xyz(XP[i]); },
+                                 ANC, R[j], j, h);
   float f[12];
   row_features(g.t, g.C, g.w, g.alpha, g.vn, g.vt, g.pen, h, f);
   float4* o = (float4*)(F + (size_t)j * 12);
   o[0] = make_float4(f[0], f[1], f[2], f[3]);
   o[1] = make_float4(f[4], f[5], f[6], f[7]);
   o[2] = make_float4(f[8], f[9], f[10], f[11]);
+}
+
+// Network pass of a substep: every wave takes 16 consecutive rows (one
+// MFMA tile: many waves per SIMD hide the latency of the GELU and MFMA
+// chains), lanes 0..15 compute their features (as k_ws_features, at the
+// predicted positions when PREDICT: x~ = x + (v + g h) h from Xin and V,
+// x_prev = Xin, the arithmetic of integrate) into LDS, the wave evaluates
+// the MLP and writes y per row.
+constexpr int kNetWaves = 4, kNetBlock = 64 * kNetWaves;
+
+struct NetLds {
+  xpbd_mlp::Weights w;
+  unsigned short ftile[kNetWaves][16][16];
+  unsigned short htile[kNetWaves][16][xpbd_mlp::NH];
+};
+
+template <bool PREDICT>
+__global__ __launch_bounds__(kNetBlock) void k_ws_net(
+    const float4* __restrict__ Xin, const float4* __restrict__ XP, const float4* __restrict__ V,
+    const int4* __restrict__ R, const float4* __restrict__ ANC,
+    const xpbd_mlp::Image* __restrict__ img, float* __restrict__ Y, float h, float gx, float gy,
+    float gz, const float* __restrict__ gscale, int m) {
+  using namespace xpbd_mlp;
+  __shared__ NetLds s;
+  load_image(s.w, img, threadIdx.x, kNetBlock);
+  __syncthreads();
+  const float b3 = img->b3;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  auto pos = [&](int i) -> float4 {
+    const float4 x4 = Xin[i];
+    if (!PREDICT || x4.w == 0.0f) return x4;
+    f3 g = mk3(gx, gy, gz);
+    if (gscale) g = g * gscale[i];
+    const f3 v = xyz(V[i]) + g * h;
+    const f3 x = xyz(x4) + v * h;
+    return make_float4(x.x, x.y, x.z, x4.w);
+  };
+  auto prev = [&](int i) { return PREDICT ? xyz(Xin[i]) : xyz(XP[i]); };
+  const int chunks = (m + 15) / 16;
+  for (int c = blockIdx.x * kNetWaves + wv; c < chunks; c += gridDim.x * kNetWaves) {
+    if (lane < 16) {
+      const int j = c * 16 + lane;
+      float f[12];
+      for (int q = 0; q < 12; ++q) f[q] = 0.0f;
+      if (j < m) {
+        const RowGeo g = row_geo<true>(pos, prev, ANC, R[j], j, h);
+        row_features(g.t, g.C, g.w, g.alpha, g.vn, g.vt, g.pen, h, f);
+      }
+      unsigned pk[8];
+      for (int q = 0; q < 6; ++q)
+        pk[q] = (unsigned)to_bf16(f[2 * q]) | ((unsigned)to_bf16(f[2 * q + 1]) << 16);
+      pk[6] = pk[7] = 0u;
+      uint4* dst = (uint4*)&s.ftile[wv][lane][0];
+      dst[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+      dst[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    f32x4 z1[4], z2[4];
+    float y[4];
+    bf16x4 xa;
+    auto feat = [&](int r, int q) -> unsigned short { return s.ftile[wv][r][q]; };
+    forward16(s.w, s.htile[wv], feat, b3, z1, z2, y, xa);
+    if ((lane & 15) == 0)
+      for (int q = 0; q < 4; ++q) {
+        const int j = c * 16 + 4 * (lane >> 4) + q;
+        if (j < m) Y[j] = y[q];
+      }
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 // Residual: sum over rows of r^2, r = C + alpha~ lambda (two-sided rows),
@@ -98,7 +170,8 @@ __global__ __launch_bounds__(kB) void k_ws_residual(const float4* __restrict__ X
   double r2 = 0.0;
   if (j < m) {
     const int4 row = R[j];
-    const RowGeo g = row_geo(X, nullptr, ANC, row, j, h);
+    const RowGeo g = row_geo<false>([&](int i) { return X[i]; },
+                                    [&](int i) { return mk3(0.0f, 0.0f, 0.0f); }, ANC, row, j, h);
     float r;
     if (g.t == T_FLOOR || g.t == T_CONTACT) r = fmaxf(0.0f, -g.C);
     else r = g.C + g.alpha * LAM[j];
@@ -125,6 +198,28 @@ int xpbd_ws_features(void* stream, const float* X, const float* XP, const int* R
   hipLaunchKernelGGL(k_ws_features, dim3(nblk(n_rows)), dim3(kB), 0, (hipStream_t)stream,
                      (const float4*)X, (const float4*)XP, (const int4*)R, (const float4*)ANC, h, F,
                      n_rows);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_ws_net(void* stream, const float* Xin, const float* XP, const float* V, const int* R,
+                const float* ANC, const void* image, float* Y, float h, float gx, float gy,
+                float gz, const float* gscale, int n_rows, int predict) {
+  if (n_rows <= 0) return 0;
+  if (!image || !Y) return (int)hipErrorInvalidValue;
+  const xpbd_mlp::Image* img = (const xpbd_mlp::Image*)image;
+  const int chunks = (n_rows + 15) / 16;
+  int nb = (chunks + kNetWaves - 1) / kNetWaves;
+  if (nb > 2048) nb = 2048;
+  hipStream_t s = (hipStream_t)stream;
+  if (predict)
+    hipLaunchKernelGGL(k_ws_net<true>, dim3(nb), dim3(kNetBlock), 0, s, (const float4*)Xin,
+                       (const float4*)XP, (const float4*)V, (const int4*)R, (const float4*)ANC,
+                       img, Y, h, gx, gy, gz, gscale, n_rows);
+  else
+    hipLaunchKernelGGL(k_ws_net<false>, dim3(nb), dim3(kNetBlock), 0, s, (const float4*)Xin,
+                       (const float4*)XP, (const float4*)V, (const int4*)R, (const float4*)ANC,
+                       img, Y, h, gx, gy, gz, gscale, n_rows);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

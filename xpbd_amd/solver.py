@@ -448,9 +448,10 @@ class Solver:
         """Learned lambda warm start (an extension; the reference resets
         lambda to zero every substep, solver.py:431): ``theta`` holds the
         5057 parameters of the 12-64-64-1 network (``warmstart.train``);
-        the first Jacobi launch of every substep then starts each
-        DISTANCE / ATTACH row from the predicted multiplier
-        (``xpbd_jacobi_tile_warm``).  ``None`` restores the zero start.
+        every substep then starts each row from the predicted multiplier
+        and the matching position offset (``xpbd_ws_net`` +
+        ``xpbd_jacobi_tile_warm``) before its Jacobi iterations.  ``None``
+        restores the zero start.
         Runs the Jacobi tile kernel; ``warmstart.Guard`` adds the
         plain-start fallback."""
         import torch
@@ -684,6 +685,7 @@ class Solver:
             self.X_B = self.LAM_B = None
             self._tile_dev = None
         self._perm = perm
+        self.WSY = torch.empty(max(n_c, 1), dtype=torch.float32, device=dev)  # warm-start y
         self.GS = None
         if self._gscale is not None:
             if not tile_mode:
@@ -819,7 +821,13 @@ class Solver:
                 d.dyn_rows, d.dyn_lam = N.ptr(self.CROWS), N.ptr(self.CLAM)
                 d.dyn_count, d.dyn_cap = N.ptr(self.CCOUNT), self._ccap
         if self._ws_theta is not None and d.mode == MODE_JACOBI_TILE:
-            d.ws_theta = N.ptr(self._ws_theta)
+            import torch
+
+            self.WSI = torch.empty(int(N.lib().xpbd_mlp_image_bytes()), dtype=torch.uint8,
+                                   device=self.device)
+            N.check(N.lib().xpbd_mlp_pack(N.current_stream(), N.ptr(self._ws_theta),
+                                          N.ptr(self.WSI)), "xpbd_mlp_pack")
+            d.ws_image, d.ws_y = N.ptr(self.WSI), N.ptr(self.WSY)
         self._block_active = self._use_block()
         if self._block_active:
             d.block = 1
@@ -827,7 +835,7 @@ class Solver:
             d.anchors = N.ptr(self.ANC)
             self._starts_dev = self._t(self._starts_buf)
             d.color_starts_dev = N.ptr(self._starts_dev)
-            d.ws_theta = d.ws_feat = d.ws_y = None
+            d.ws_image = d.ws_y = None
         d.substeps, d.iterations = self.substeps, self.iterations
         d.h = float(np.float32(self.dt / float(self.substeps)))
         d.gx, d.gy, d.gz = self.gravity

@@ -2,14 +2,14 @@
 counterpart; it replaces the per-substep zero reset of the multipliers,
 src/xpbd3d/solver.py:431).
 
-Each substep starts with one warm-start launch (``xpbd_jacobi_tile_warm``):
-every row evaluates the network on its features and starts from lambda0 =
-4 sigmoid(y) lambda1, lambda1 = -C / (w + alpha~) the row's first plain step
-(one-sided rows only while violated), and every particle moves by the mean
-over its rows of w grad C lambda0 -- the position the Jacobi sweep reaches
-when its multipliers add up to lambda0.  The substep's Jacobi iterations
-then continue from lambda0 (first = 0).  With k iterations a learned substep
-is k + 1 launches.
+Each substep starts with the network pass (``xpbd_ws_net``: features and
+MLP output y per row) and the warm-start launch (``xpbd_jacobi_tile_warm``):
+every row starts from lambda0 = 4 sigmoid(y) lambda1, lambda1 = -C / (w +
+alpha~) the row's first plain step (one-sided rows only while violated),
+and every particle moves by the mean over its rows of w grad C lambda0 --
+the position the Jacobi sweep reaches when its multipliers add up to
+lambda0.  The substep's Jacobi iterations then continue from lambda0
+(first = 0).  With k iterations a learned substep is k + 2 launches.
 
 * ``collect``  runs the solver and, at sampled substeps, records the row
   features (kernel ``xpbd_ws_features``, the same features the launch
@@ -120,6 +120,16 @@ class _Eager:
         self.n = s.num_bodies
         self.tables = tuple(N.ptr(t) for t in s._tile_dev)
 
+    def image(self, theta):
+        """The network ``theta`` packed for the network pass (xpbd_mlp_pack)."""
+        import torch
+
+        if getattr(self, "_img", None) is None:
+            self._img = torch.empty(int(self.lib.xpbd_mlp_image_bytes()), dtype=torch.uint8,
+                                    device=self.s.device)
+        N.check(self.lib.xpbd_mlp_pack(N.current_stream(), N.ptr(theta), N.ptr(self._img)))
+        return self._img
+
     def integrate(self):
         s = self.s
         N.check(self.lib.xpbd_integrate_to(N.current_stream(), N.ptr(s.X), N.ptr(s.X), N.ptr(s.V),
@@ -129,9 +139,10 @@ class _Eager:
     def iterate(self, iters, theta=None, finalize=True, X=None, X2=None, L=None, L2=None,
                 Y=None):
         """``iters`` tile launches from X (default the solver's state),
-        ping-ponging with X2; the result is left in X and L.  ``theta``: a
-        warm-start launch first (network output into Y), then the iterations
-        continue from its multipliers."""
+        ping-ponging with X2; the result is left in X and L.  ``theta``: the
+        network pass (output into Y, default the solver's buffer) and the
+        warm-start launch first, then the iterations continue from its
+        multipliers."""
         s = self.s
         X = s.X if X is None else X
         X2 = s.X_B if X2 is None else X2
@@ -141,10 +152,15 @@ class _Eager:
         st = N.current_stream()
         trs, hs, hr = self.tables
         if theta is not None:
+            Y = s.WSY if Y is None else Y
+            img = self.image(theta)
+            N.check(self.lib.xpbd_ws_net(st, N.ptr(src), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R),
+                                         N.ptr(s.ANC), N.ptr(img), N.ptr(Y), self.h, 0.0, 0.0,
+                                         0.0, None, self.m, 0))
             N.check(self.lib.xpbd_jacobi_tile_warm(
                 st, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), N.ptr(s.ANC),
-                N.ptr(ld), trs, hs, hr, N.ptr(theta), N.ptr(Y) if Y is not None else None,
-                self.h, 0.0, 0.0, 0.0, None, s._tile, self.n, s.jacobi_relax, 0))
+                N.ptr(ld), trs, hs, hr, N.ptr(Y), self.h, 0.0, 0.0, 0.0, None, s._tile, self.n,
+                s.jacobi_relax, 0))
             src, dst, ls, ld = dst, src, ld, ls
         for it in range(iters):
             fin = int(finalize and it + 1 == iters)

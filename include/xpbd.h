@@ -340,6 +340,19 @@ typedef struct xpbd_rigid_desc {
 
 /* Solver6DOF._record (solver_6dof.py:357-426) as a plan; run() replays one
  * hipGraph per frame (solver_6dof.py:519-527). */
+/* Whole frames of a small rigid scene in one workgroup (n <=
+ * xpbd6_frame_block_capacity() = 256 bodies, state held in LDS):
+ * per frame the LBVH path's broad phase (compute_body_aabb + the emit_pairs
+ * filter of kernels_bvh.py:21-85, every pair (i, j > i) tested, pairs
+ * written in (i, j) order with the count in pair_count[0]) followed by the
+ * substep loop of xpbd_frame6_run (solver_6dof.py:357-426), `frames` times.
+ * desc->cap must be >= n (n - 1) / 2 (no overflow); LO/HI float4[n] scratch;
+ * dt = the frame length (AABB velocity margin).  pairs and pair_count are
+ * written despite their const in the descriptor. */
+int xpbd6_frame_block(void* stream, const xpbd_rigid_desc* desc, float* LO, float* HI, float dt,
+                      int frames);
+int xpbd6_frame_block_capacity(void);
+
 int xpbd_frame6_create(const xpbd_rigid_desc* desc, void** handle);
 int xpbd_frame6_run(void* handle, void* stream, int frames, int use_graph);
 int xpbd_frame6_destroy(void* handle);

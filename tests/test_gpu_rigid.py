@@ -52,10 +52,11 @@ def _random_boxes(n, seed, spread=3.0):
 
 
 @pytest.mark.parametrize("broadphase", ["lbvh", "hashgrid"])
-@pytest.mark.parametrize("n,seed", [(2, 0), (300, 1), (3000, 2)])
+@pytest.mark.parametrize("n,seed", [(2, 0), (300, 1), (3000, 2), (6000, 3)])
 def test_broadphase_pair_set_bit_exact(oracle, broadphase, n, seed):
     """The device pair list equals the oracle's brute-force list: same pairs,
-    same (i, j) order, same count."""
+    same (i, j) order, same count (LBVH: the all-pairs sweep up to 4096
+    bodies, the tree above)."""
     s = _random_boxes(n, seed, spread=3.0 if n > 100 else 0.2)
     s.broadphase = broadphase
     s._flush()
@@ -79,6 +80,7 @@ def test_broadphase_overflow_regrows():
     from xpbd_amd.scenes import box_pile
 
     s = box_pile(Solver6DOF, 6, 4, 6, substeps=4, device=DEV)
+    s.block_frames = False          # the per-launch path sizes the pair list by the count
     s._flush()
     s.cap = 8
     s._alloc_pair_buffers(8)
@@ -132,14 +134,17 @@ def _pile_scene(seed=0):
     return s
 
 
-def test_rigid_substep_parity_and_contact_counts(oracle):
+@pytest.mark.parametrize("block", [False, True])
+def test_rigid_substep_parity_and_contact_counts(oracle, block):
     """From identical inputs (the oracle's trajectory), one substep on the
     device matches the oracle: pair lists and per-pair contact counts
-    bit-exact, x/q/v/w within 1e-4."""
+    bit-exact, x/q/v/w within 1e-4; through the per-launch plan and through
+    the whole-frame workgroup kernel."""
     s_full = _pile_scene()
     orc = _oracle6(oracle, s_full)
     h = float(np.float32(1 / 60 / 8))
     s = _pile_scene()
+    s.block_frames = block
     s.dt, s.substeps = h, 1
     s._flush()
     one = _oracle6(oracle, s)
@@ -163,13 +168,16 @@ def test_rigid_substep_parity_and_contact_counts(oracle):
     assert total_points > 0
 
 
-def test_rigid_frames_track_oracle(oracle):
-    """Whole frames (8 substeps x 2 iterations, graph replay) over 20 frames
-    of a 60-box pile.  The pile is chaotic: a one-ulp change of the oracle's
-    own start positions grows to a 2.7e-3 relative difference by frame 3 and
-    to 0.18 by frame 20.  Each frame the device trajectory must stay within
-    1e-3, or within 4x that self-sensitivity of the oracle where it is larger."""
+@pytest.mark.parametrize("block", [False, True])
+def test_rigid_frames_track_oracle(oracle, block):
+    """Whole frames (8 substeps x 2 iterations; graph replay, or one
+    workgroup launch) over 20 frames of a 60-box pile.  The pile is chaotic:
+    a one-ulp change of the oracle's own start positions grows to a 2.7e-3
+    relative difference by frame 3 and to 0.18 by frame 20.  Each frame the
+    device trajectory must stay within 1e-3, or within 4x that
+    self-sensitivity of the oracle where it is larger."""
     s = _pile_scene(1)
+    s.block_frames = block
     h = s.host_arrays()
     orc = _oracle6(oracle, s)
     pert = _oracle6(oracle, s)
@@ -180,6 +188,34 @@ def test_rigid_frames_track_oracle(oracle):
         pert.step(1)
         e, sens = _rel(s.positions(), orc.x), _rel(pert.x, orc.x)
         assert e <= max(1e-3, 4 * sens), (f, e, sens)
+
+
+def test_block_frames_match_launch_plan():
+    """The whole-frame workgroup kernel against the per-launch plan on the
+    100-box pile of the reference benchmark: identical pair lists and contact
+    counts after the first frame, states within 1e-5 (float atomics sum in a
+    different order), and a settled pile after 60 frames both ways."""
+    from xpbd_amd import Solver6DOF
+    from xpbd_amd.scenes import box_pile
+
+    a = box_pile(Solver6DOF, 5, 4, 5, device=DEV, broadphase="lbvh")
+    b = box_pile(Solver6DOF, 5, 4, 5, device=DEV, broadphase="lbvh")
+    a.block_frames, b.block_frames = True, False
+    a.step()
+    b.step()
+    assert a.n_pairs == b.n_pairs > 0
+    assert (a.pairs() == b.pairs()).all()
+    assert (a.contact_counts() == b.contact_counts()).all()
+    assert _rel(a.positions(), b.positions()) <= 1e-5
+    assert _rel(a.velocities(), b.velocities()) <= 1e-3
+    for _ in range(59):
+        a.step()
+        b.step()
+    for s in (a, b):
+        x, v = s.positions(), s.velocities()
+        assert np.all(np.isfinite(x))
+        assert x[:, 1].min() > 0.16 - 0.02
+        assert np.abs(v).max() < 0.5
 
 
 # ---- the reference's invariant tests (tests/test_solver_6dof.py) -----------

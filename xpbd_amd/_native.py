@@ -102,6 +102,8 @@ _SIGS = {
     "xpbd6_finalize_velocity": (_I, [_P] * 7 + [_F, _F, _F, _I]),
     "xpbd6_velocity_floor": (_I, [_P] * 5 + [_F, _F, _P, _F, _P, _P, _P, _P, _I]),
     "xpbd6_velocity_box": (_I, [_P] * 10 + [_F, _F, _F, _P, _P, _P, _P, _I]),
+    "xpbd6_frame_block": (_I, [_P, _P, _P, _P, _F, _I]),
+    "xpbd6_frame_block_capacity": (_I, []),
     "xpbd6_apply_velocity": (_I, [_P] * 6 + [_F, _F, _I, _I]),
     "xpbd6_begin_substep": (_I, [_P] * 12 + [_F, _F, _I, _I, _F, _F, _F, _F, _I, _P, _I]),
     "xpbd6_solve_all": (_I, [_P] * 11 + [_F, _I, _I, _F, _F, _P, _I] + [_P] * 6 + [_I, _P, _P]),

@@ -123,15 +123,7 @@ __global__ __launch_bounds__(kB) void k_body_aabb(const float4* __restrict__ X,
                                                   float dt, float4* LO, float4* HI, int n) {
   const int i = blockIdx.x * kB + threadIdx.x;
   if (i >= n) return;
-  const f3 e = xyz(HE[i]);
-  const f3 half = aabb_half(ldq(Q[i]), e);
-  const float rr = length(e);
-  const float m = man_margin + dt * (length(xyz(V[i])) + length(xyz(W[i])) * rr);
-  const f3 pad = mk3(m, m, m);
-  const f3 x = xyz(X[i]);
-  const f3 lo = (x - half) - pad, hi = (x + half) + pad;
-  LO[i] = make_float4(lo.x, lo.y, lo.z, 0.0f);
-  HI[i] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+  body_aabb(X[i], Q[i], V[i], W[i], HE[i], man_margin, dt, LO[i], HI[i]);
 }
 
 __global__ __launch_bounds__(kB) void k_bounds_partial(const float4* __restrict__ LO,
@@ -365,19 +357,6 @@ __global__ __launch_bounds__(kB) void k_refit(const int* __restrict__ val,
   }
 }
 
-// kernels_bvh.py:52-85 filter: j > i, not static-static, not same group > 0
-__device__ inline bool keep_pair(int i, int j, float imi, float imj, int gi, int gj) {
-  if (j <= i) return false;
-  if (imi == 0.0f && imj == 0.0f) return false;
-  if (gi == gj && gi > 0) return false;
-  return true;
-}
-
-__device__ inline bool overlap(float4 alo, float4 ahi, float4 blo, float4 bhi) {
-  return !(alo.x > bhi.x || blo.x > ahi.x || alo.y > bhi.y || blo.y > ahi.y || alo.z > bhi.z ||
-           blo.z > ahi.z);
-}
-
 // Insertion sort of one body's partner segment (ascending j).
 __device__ inline void sort_segment(int2* seg, int m) {
   for (int a = 1; a < m; ++a) {
@@ -389,6 +368,44 @@ __device__ inline void sort_segment(int2* seg, int m) {
     }
     seg[b + 1] = v;
   }
+}
+
+// All-pairs test for small scenes: one wavefront per body i sweeps the
+// bodies j > i in 64-wide chunks against the same AABBs and filter as the
+// LBVH query.  COUNT writes the partner count of i; EMIT writes the pairs
+// at off[i] in ascending j (each lane's slot = the hits of lower lanes in
+// its chunk, from the ballot), i.e. exactly the sorted segments of the LBVH
+// path, in 6 launches instead of ~25.
+constexpr int kAllPairsMax = 4096;
+
+template <bool EMIT>
+__global__ __launch_bounds__(kB) void k_allpairs(const float4* __restrict__ X,
+                                                 const float4* __restrict__ HE,
+                                                 const float4* __restrict__ LO,
+                                                 const float4* __restrict__ HI, int n,
+                                                 int* __restrict__ cnt, const int* __restrict__ off,
+                                                 int2* __restrict__ pairs, int cap) {
+  const int i = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;  // whole wave
+  const float4 lo = LO[i], hi = HI[i];
+  const float imi = X[i].w;
+  const int gi = __float_as_int(HE[i].w);
+  int k = EMIT ? off[i] : 0;
+  for (int j0 = i + 1; j0 < n; j0 += 64) {
+    const int j = j0 + lane;
+    bool hit = false;
+    if (j < n)
+      hit = overlap(lo, hi, LO[j], HI[j]) &&
+            keep_pair(i, j, imi, X[j].w, gi, __float_as_int(HE[j].w));
+    const unsigned long long m = __ballot(hit);
+    if (EMIT && hit) {
+      const int slot = k + __popcll(m & ((1ull << lane) - 1ull));
+      if (slot < cap) pairs[slot] = make_int2(i, j);
+    }
+    k += __popcll(m);
+  }
+  if (!EMIT && lane == 0) cnt[i] = k;
 }
 
 template <bool EMIT>
@@ -569,6 +586,18 @@ int xpbd6_broadphase_lbvh(void* stream, void* ws, const float* X, const float* Q
   hipLaunchKernelGGL(k_body_aabb, dim3(nb), dim3(kB), 0, s, (const float4*)X, (const float4*)Q,
                      (const float4*)V, (const float4*)W, (const float4*)HE, man_margin, dt, w.lo,
                      w.hi, n);
+  if (n <= kAllPairsMax) {  // same pair list, fewer launches
+    const int nw = nblk(n, kB / 64);
+    hipLaunchKernelGGL(k_allpairs<false>, dim3(nw), dim3(kB), 0, s, (const float4*)X,
+                       (const float4*)HE, w.lo, w.hi, n, w.cnt, w.off, (int2*)PAIRS, cap);
+    XPBD_LAUNCH_CHECK();
+    int rc = exclusive_scan(s, w.cnt, w.off, n, w.scan, PC);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_allpairs<true>, dim3(nw), dim3(kB), 0, s, (const float4*)X,
+                       (const float4*)HE, w.lo, w.hi, n, w.cnt, w.off, (int2*)PAIRS, cap);
+    XPBD_LAUNCH_CHECK();
+    return 0;
+  }
   hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(kB), 0, s, w.lo, w.hi, w.part, n);
   hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kB), 0, s, w.part, nb, w.bounds);
   hipLaunchKernelGGL(k_morton, dim3(nb), dim3(kB), 0, s, w.lo, w.hi, w.bounds, w.key_a, w.val_a,

@@ -22,6 +22,9 @@
 #include "rigid_math.h"
 #include "xpbd.h"
 
+#define F4(p) ((float4*)(p))
+#define CF4(p) ((const float4*)(p))
+
 namespace {
 
 constexpr int kB = 256;
@@ -106,16 +109,15 @@ __device__ inline float proj_radius(f3 L, const m3& R, f3 e) {
 // kernels_6dof.py:233-382: face-axis SAT + Sutherland-Hodgman clip of the
 // incident face against the reference face, <= 8 points per pair.  The
 // points of a wavefront are compacted into the point arrays with one atomic
-// per wave (inclusive scan of the per-lane counts over the 64 lanes).
-__global__ __launch_bounds__(kB) void k6_box_manifold(
-    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ HE,
-    const int2* __restrict__ PAIRS, const int* __restrict__ PC, float margin, int cap,
+// per wave (inclusive scan of the per-lane counts over the 64 lanes); all 64
+// lanes of a wave call it together (pid >= live: no pair).
+__device__ inline void box_manifold_elem(
+    int pid, int live, const float4* __restrict__ X, const float4* __restrict__ Q,
+    const float4* __restrict__ HE, const int2* __restrict__ PAIRS, float margin, int cap,
     int* __restrict__ MC, int4* __restrict__ PTI, float4* __restrict__ PTN,
     float4* __restrict__ PTOI, float4* __restrict__ PTOR, float* __restrict__ LAMP,
     int* __restrict__ PTC) {
-  const int pid = blockIdx.x * kB + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const int live = min(*PC, cap);
   int nc = 0, inc_b = 0, ref_b = 0;
   f3 ref_face_n = mk3(0.f, 0.f, 0.f);
   f3 off_i[8], off_r[8];
@@ -240,6 +242,16 @@ __global__ __launch_bounds__(kB) void k6_box_manifold(
     LAMP[k] = 0.0f;
   }
   if (pid < cap) MC[pid] = nc;
+}
+
+__global__ __launch_bounds__(kB) void k6_box_manifold(
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ HE,
+    const int2* __restrict__ PAIRS, const int* __restrict__ PC, float margin, int cap,
+    int* __restrict__ MC, int4* __restrict__ PTI, float4* __restrict__ PTN,
+    float4* __restrict__ PTOI, float4* __restrict__ PTOR, float* __restrict__ LAMP,
+    int* __restrict__ PTC) {
+  box_manifold_elem(blockIdx.x * kB + threadIdx.x, min(*PC, cap), X, Q, HE, PAIRS, margin, cap,
+                    MC, PTI, PTN, PTOI, PTOR, LAMP, PTC);
 }
 
 // kernels_6dof.py:440-483 (lambda reset folded into the first iteration)
@@ -709,6 +721,170 @@ __global__ __launch_bounds__(kB) void k6_velocity_all(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Whole frames of a small rigid scene in one workgroup (no reference
+// counterpart as a kernel: it runs solver_6dof.py:428-454 with the LBVH
+// filter and the substep loop of solver_6dof.py:357-426 for `frames`
+// frames).  The broad phase tests all pairs (i, j > i) against the same
+// AABBs and filter as the LBVH path and emits them in (i, j) order through a
+// workgroup scan of the per-body counts, so the pair list equals the LBVH
+// path's; the substep phases are the elem functions of the fused launches,
+// separated by workgroup barriers (__syncthreads is a workgroup-scope
+// release/acquire: all waves of the workgroup share one CU's L1, so stores,
+// and the L2 atomics that sum the accumulators, are visible to every wave
+// after it without a device-scope fence).  Counters written inside the
+// launch (pair and point counts) are read back with atomic loads, never
+// through the scalar cache.  The body state (12 float4 arrays and the floor
+// multipliers, 240 B per body) lives in LDS for the whole launch: up to
+// kFBMax bodies.  cap >= n (n - 1) / 2, so the pair list cannot overflow.
+// ---------------------------------------------------------------------------
+constexpr int kFB = 1024;
+constexpr int kFBMax = 256;
+
+__device__ inline void fb_sync() { __syncthreads(); }
+
+__device__ inline int ld_count(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kFB) void k6_frame_block(xpbd_rigid_desc d, float4* __restrict__ LO,
+                                                     float4* __restrict__ HI, float dt,
+                                                     int frames) {
+  __shared__ int scnt[kFBMax + 1];
+  __shared__ float4 sb[12][kFBMax];
+  __shared__ float slf[8 * kFBMax];
+  const int tid = threadIdx.x;
+  const int n = d.n;
+  float4* const G[12] = {F4(d.x),      F4(d.q),       F4(d.v),     F4(d.w),
+                         F4(d.x_prev), F4(d.q_prev),  F4(d.inv_I), F4(d.he),
+                         F4(d.dx),     F4(d.drot),    F4(d.dv),    F4(d.dw)};
+  for (int k = 0; k < 12; ++k)
+    for (int i = tid; i < n; i += kFB) sb[k][i] = G[k][i];
+  for (int i = tid; i < 8 * n; i += kFB) slf[i] = d.lam_floor[i];
+  __syncthreads();
+  float4* X = sb[0];
+  float4* Q = sb[1];
+  float4* V = sb[2];
+  float4* W = sb[3];
+  float4* XP = sb[4];
+  float4* QP = sb[5];
+  const float4* IB = sb[6];
+  const float4* HE = sb[7];
+  float4* DX = sb[8];
+  float4* DR = sb[9];
+  float4* DV = sb[10];
+  float4* DW = sb[11];
+  float* LAMF = slf;
+  int2* PAIRS = (int2*)d.pairs;
+  int4* PTI = (int4*)d.pt_idx;
+  float4* PTN = F4(d.pt_n);
+  float4* PTOI = F4(d.pt_oi);
+  float4* PTOR = F4(d.pt_or);
+  const int2* JI = (const int2*)d.j_idx;
+  const float4* JA = CF4(d.j_a);
+  const float4* JB = CF4(d.j_b);
+  const int maxp = 8 * d.cap;
+  const int cap_v = d.max_speed > 0.0f;
+  for (int fr = 0; fr < frames; ++fr) {
+    // ---- broad phase: AABBs, per-body partner counts, scan, ordered emit
+    for (int i = tid; i < n; i += kFB)
+      body_aabb(X[i], Q[i], V[i], W[i], HE[i], d.man_margin, dt, LO[i], HI[i]);
+    fb_sync();
+    for (int i = tid; i < kFBMax; i += kFB) {
+      int c = 0;
+      if (i < n) {
+        const float4 lo = LO[i], hi = HI[i];
+        const float imi = X[i].w;
+        const int gi = __float_as_int(HE[i].w);
+        for (int j = i + 1; j < n; ++j)
+          if (overlap(lo, hi, LO[j], HI[j]) && keep_pair(i, j, imi, X[j].w, gi, __float_as_int(HE[j].w)))
+            ++c;
+      }
+      scnt[i] = c;
+    }
+    __syncthreads();
+    if (tid == 0) {  // exclusive scan (n <= 1024: serial in LDS)
+      int acc = 0;
+      for (int i = 0; i < kFBMax; ++i) {
+        const int c = scnt[i];
+        scnt[i] = acc;
+        acc += c;
+      }
+      scnt[kFBMax] = acc;
+      ((int*)d.pair_count)[0] = acc;  // this launch is the broad phase too
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += kFB) {
+      int k = scnt[i];
+      const float4 lo = LO[i], hi = HI[i];
+      const float imi = X[i].w;
+      const int gi = __float_as_int(HE[i].w);
+      for (int j = i + 1; j < n; ++j)
+        if (overlap(lo, hi, LO[j], HI[j]) && keep_pair(i, j, imi, X[j].w, gi, __float_as_int(HE[j].w))) {
+          if (k < d.cap) PAIRS[k] = make_int2(i, j);
+          ++k;
+        }
+    }
+    const int live = min(scnt[kFBMax], d.cap);
+    fb_sync();
+    // ---- substeps (the fused launches' phases)
+    for (int sub = 0; sub < d.substeps; ++sub) {
+      if (tid == 0) d.pt_count[0] = 0;
+      for (int i = tid; i < n; i += kFB) {
+        if (sub > 0) apply_velocity_elem(i, X, V, W, DV, DW, d.max_speed, d.max_omega, cap_v, n);
+        integrate_elem(i, X, Q, V, W, XP, QP, IB, LAMF, d.gx, d.gy, d.gz, d.h,
+                       d.gyroscopic, d.gscale, n);
+      }
+      fb_sync();
+      for (int base = 0; base < live; base += kFB)  // whole waves call the manifold
+        box_manifold_elem(base + tid, live, X, Q, HE, PAIRS, d.man_margin, d.cap, d.m_count, PTI,
+                          PTN, PTOI, PTOR, d.lam_pt, d.pt_count);
+      fb_sync();
+      const int npts = min(ld_count(d.pt_count), maxp);
+      for (int it = 0; it < d.iterations; ++it) {
+        for (int k = tid; k < d.nj + n + npts; k += kFB) {
+          if (k < d.nj)
+            solve_joints_elem(k, X, Q, IB, JI, JA, JB, d.lam_j, d.h, it == 0, DX, DR, d.nj);
+          else if (k < d.nj + n)
+            solve_floor_elem<true>(k - d.nj, X, Q, XP, QP, IB, HE, d.floor_y, d.friction,
+                                   LAMF, DX, DR, n);
+          else
+            solve_box_elem(k - d.nj - n, X, Q, IB, PTI, PTN, PTOI, PTOR, d.lam_pt, d.pt_count,
+                           DX, DR, maxp);
+        }
+        fb_sync();
+        for (int i = tid; i < n; i += kFB) {
+          apply_jacobi_elem(i, X, Q, DX, DR, d.relax, n);
+          if (it + 1 == d.iterations)
+            finalize_elem(i, X, Q, XP, QP, V, W, d.h, d.lin_damp, d.ang_damp, n);
+        }
+        fb_sync();
+      }
+      if (d.iterations == 0) {
+        for (int i = tid; i < n; i += kFB)
+          finalize_elem(i, X, Q, XP, QP, V, W, d.h, d.lin_damp, d.ang_damp, n);
+        fb_sync();
+      }
+      for (int k = tid; k < n + npts; k += kFB) {
+        if (k < n)
+          velocity_floor_elem<true>(k, X, Q, IB, HE, d.friction, d.restitution, LAMF, d.h,
+                                    V, W, DV, DW, n);
+        else
+          velocity_box_elem(k - n, X, Q, IB, PTI, PTN, PTOI, PTOR, d.lam_pt, d.pt_count,
+                            d.friction, d.restitution, d.h, V, W, DV, DW, maxp);
+      }
+      fb_sync();
+    }
+    for (int i = tid; i < n; i += kFB)
+      apply_velocity_elem(i, X, V, W, DV, DW, d.max_speed, d.max_omega, cap_v, n);
+    fb_sync();
+  }
+  for (int k = 0; k < 12; ++k)
+    if (k != 6 && k != 7)  // inverse inertia and half extents are read-only
+      for (int i = tid; i < n; i += kFB) G[k][i] = sb[k][i];
+  for (int i = tid; i < 8 * n; i += kFB) d.lam_floor[i] = slf[i];
+}
+
 inline int nblk(long long n) { return (int)((n + kB - 1) / kB); }
 
 }  // namespace
@@ -716,9 +892,6 @@ inline int nblk(long long n) { return (int)((n + kB - 1) / kB); }
 // ===========================================================================
 // C-ABI
 // ===========================================================================
-#define F4(p) ((float4*)(p))
-#define CF4(p) ((const float4*)(p))
-
 extern "C" {
 
 int xpbd6_integrate(void* stream, float* X, float* Q, float* V, float* W, float* XP, float* QP,
@@ -869,6 +1042,21 @@ int xpbd6_velocity_all(void* stream, const float* X, const float* Q, const float
                      CF4(X), CF4(Q), CF4(IB), CF4(HE), mu_d, restitution, LAMF, h, CF4(V), CF4(W),
                      n, (const int4*)PTI, CF4(PTN), CF4(PTOI), CF4(PTOR), LAMP, PTC, max_points,
                      F4(DV), F4(DW));
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_frame_block_capacity(void) { return kFBMax; }
+
+int xpbd6_frame_block(void* stream, const xpbd_rigid_desc* desc, float* LO, float* HI, float dt,
+                      int frames) {
+  if (!desc || frames < 0) return (int)hipErrorInvalidValue;
+  const xpbd_rigid_desc& d = *desc;
+  if (d.n <= 0 || frames == 0) return 0;
+  if (d.n > kFBMax || (long long)d.cap < (long long)d.n * (d.n - 1) / 2 || !LO || !HI)
+    return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k6_frame_block, dim3(1), dim3(kFB), 0, (hipStream_t)stream, d, F4(LO),
+                     F4(HI), dt, frames);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

@@ -88,3 +88,30 @@ __host__ __device__ inline f3 aabb_half(q4 q, f3 e) {
              fabsf(R.c[0].y) * e.x + fabsf(R.c[1].y) * e.y + fabsf(R.c[2].y) * e.z,
              fabsf(R.c[0].z) * e.x + fabsf(R.c[1].z) * e.y + fabsf(R.c[2].z) * e.z);
 }
+
+// kernels_bvh.py:21-48 `compute_body_aabb`: |R| he plus a velocity margin
+__device__ inline void body_aabb(float4 x4, float4 q4v, float4 v4, float4 w4, float4 he4,
+                                 float man_margin, float dt, float4& lo4, float4& hi4) {
+  const f3 e = xyz(he4);
+  const f3 half = aabb_half(ldq(q4v), e);
+  const float rr = length(e);
+  const float m = man_margin + dt * (length(xyz(v4)) + length(xyz(w4)) * rr);
+  const f3 pad = mk3(m, m, m);
+  const f3 x = xyz(x4);
+  const f3 lo = (x - half) - pad, hi = (x + half) + pad;
+  lo4 = make_float4(lo.x, lo.y, lo.z, 0.0f);
+  hi4 = make_float4(hi.x, hi.y, hi.z, 0.0f);
+}
+
+// kernels_bvh.py:52-85 filter: j > i, not static-static, not same group > 0
+__device__ inline bool keep_pair(int i, int j, float imi, float imj, int gi, int gj) {
+  if (j <= i) return false;
+  if (imi == 0.0f && imj == 0.0f) return false;
+  if (gi == gj && gi > 0) return false;
+  return true;
+}
+
+__device__ inline bool overlap(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+  return !(alo.x > bhi.x || blo.x > ahi.x || alo.y > bhi.y || blo.y > ahi.y || alo.z > bhi.z ||
+           blo.z > ahi.z);
+}

@@ -108,8 +108,11 @@ class Solver6DOF:
         self._plan_sig = None
         self._graph = None
         self._gscale = None
-        self.n_pairs = 0
+        self._n_pairs = 0
         self.n_joints = 0
+        # "auto": whole frames of LBVH scenes up to BLOCK_AUTO bodies in one
+        # workgroup (xpbd6_frame_block); True / False force it on / off
+        self.block_frames = "auto"
         self.cap = 0
         self._cur_margin = 0.0
 
@@ -275,6 +278,46 @@ class Solver6DOF:
         b["PTOR"] = torch.zeros((8 * cap, 4), dtype=torch.float32, device=dev)
         b["LAMP"] = torch.zeros(8 * cap, dtype=torch.float32, device=dev)
 
+    # one workgroup runs the whole frame up to this many bodies ("auto"):
+    # above it the per-launch plan's 256 CUs win the manifold and solve work
+    BLOCK_AUTO = 256
+
+    @property
+    def n_pairs(self) -> int:
+        """Candidate pairs of the last frame (read from the device after a
+        whole-frame workgroup launch)."""
+        if self._n_pairs is None:
+            self._n_pairs = int(self._buf["PC"].item())
+        return self._n_pairs
+
+    @n_pairs.setter
+    def n_pairs(self, value) -> None:
+        self._n_pairs = value
+
+    def _use_block(self, n) -> bool:
+        if self.block_frames is False or self.broadphase != "lbvh":
+            return False
+        if n > int(N.lib().xpbd6_frame_block_capacity()):
+            return False
+        return self.block_frames is True or n <= self.BLOCK_AUTO
+
+    def _step_block(self, n) -> None:
+        import torch
+
+        b = self._buf
+        need = max(64, n * (n - 1) // 2)
+        if self.cap < need:
+            self.cap = need
+            self._alloc_pair_buffers(self.cap)
+        if "BLO" not in b or b["BLO"].shape[0] != n:
+            b["BLO"] = torch.empty((n, 4), dtype=torch.float32, device=self.device)
+            b["BHI"] = torch.empty((n, 4), dtype=torch.float32, device=self.device)
+        self._get_plan(n)
+        N.check(N.lib().xpbd6_frame_block(N.current_stream(), N.ctypes.byref(self._desc),
+                                          N.ptr(b["BLO"]), N.ptr(b["BHI"]), self.dt, 1),
+                "xpbd6_frame_block")
+        self._n_pairs = None
+
     # ---- plan / graph ---------------------------------------------------------
     def _signature(self, n):
         """Graph key, as solver_6dof.py:481-487 (plus buffer identity)."""
@@ -315,6 +358,7 @@ class Solver6DOF:
         d.max_speed, d.max_omega, d.man_margin = self.max_speed, self.max_omega, self._man_margin
         if "GS" in b:
             d.gscale = P(b["GS"])
+        self._desc = d
         handle = N.c_void_p()
         N.check(N.lib().xpbd_frame6_create(N.ctypes.byref(d), N.ctypes.byref(handle)),
                 "xpbd_frame6_create")
@@ -385,6 +429,9 @@ class Solver6DOF:
         import torch
 
         b = self._buf
+        if self._use_block(n):
+            self._step_block(n)
+            return
         if self.broadphase in ("lbvh", "hashgrid"):
             cnt = self._run_broadphase(n, self.cap)
             if cnt > self.cap:                       # overflow: grow, re-emit, re-capture

@@ -4,7 +4,7 @@ For a flushed ``Solver`` in Jacobi mode it times whole frames (hipGraph
 replays, HIP events on the launch stream) under each candidate launch shape
 and keeps the fastest:
 
-* ``tile``   — the fused per-iteration kernel with tile sizes of 64 ... 1024
+* ``tile``   — the fused per-iteration kernel with tile sizes of 32 ... 1024
   particles (44 B of LDS per particle; larger tiles re-project fewer halo
   rows, smaller tiles spread the launch over more CUs), each with integrate
   folded into the first iteration or run as its own launch;
@@ -33,7 +33,7 @@ def candidates(n_particles: int, tile_only: bool = False):
     """Launch shapes to time; ``tile_only`` (learned warm start): no window
     or block shapes."""
     out = []
-    for t in (64, 128, 256, 512, 1024):
+    for t in (32, 64, 128, 256, 512, 1024):
         if t <= max(64, 2 * n_particles):
             out.append(("tile", t, None, True))
             out.append(("tile", t, None, False))

@@ -162,10 +162,11 @@ class _Eager:
                 N.ptr(ld), trs, hs, hr, N.ptr(Y), self.h, 0.0, 0.0, 0.0, None, s._tile, self.n,
                 s.jacobi_relax, 0))
             src, dst, ls, ld = dst, src, ld, ls
+        anc = N.ptr(s.ANC) if s._has_attach else None   # the variant without anchor loads
         for it in range(iters):
             fin = int(finalize and it + 1 == iters)
             N.check(self.lib.xpbd_jacobi_tile(
-                st, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), N.ptr(s.ANC),
+                st, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), anc,
                 N.ptr(ls), N.ptr(ld), trs, hs, hr, self.h, int(it == 0 and theta is None),
                 s._tile, self.n, s.jacobi_relax, fin, s.damping, s.max_speed, None))
             src, dst, ls, ld = dst, src, ld, ls

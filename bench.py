@@ -205,13 +205,20 @@ def time_frames(step, frames: int, warmup: int) -> float:
 
 
 def other_workloads(frames: int) -> dict:
-    """Configurations 1, 3 and 4 of the benchmark set, timed the same way
-    (secondary lines; the headline is the cloth)."""
+    """Configurations 1, 3 and 4 of the benchmark set and the cloth in GS
+    mode, timed the same way (secondary lines; the headline is the cloth in
+    Jacobi mode)."""
     from xpbd_amd import Solver, Solver6DOF
     from xpbd_amd.batch import BatchedWorld, draw
-    from xpbd_amd.scenes import hanging_chain, unified_scene
+    from xpbd_amd.scenes import cloth_sheet, hanging_chain, unified_scene
 
     out = {}
+    g = cloth_sheet(Solver, res=128, solve_mode="gs", device="cuda:0")[0]
+    ms = time_frames(g.step, max(frames // 4, 10), 5)
+    out["cloth128_gs"] = {"ms_per_frame": round(ms, 4), "rows": g.num_constraints,
+                          "colors": int(len(g._color_starts) - 1),
+                          "proj_per_s": g.num_constraints * 30 / (ms * 1e-3),
+                          "substeps_per_s": 15 / (ms * 1e-3)}
     c = hanging_chain(Solver, device="cuda:0")
     ms = time_frames(c.step, frames, 10)
     out["chain64_gs"] = {"ms_per_frame": round(ms, 4), "rows": c.num_constraints,

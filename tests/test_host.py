@@ -326,3 +326,36 @@ def test_ws_oracle_one_row_closed_form():
         assert np.isclose((L2 - rest32) + alpha * (lam0[0] + dlam), 0.0, atol=1e-12)
         r2 = ws_ref.residual_sum(Xn, rows, anc, lam0, h)
         assert np.isclose(r2, ((L1 - rest32) + alpha * lam0[0]) ** 2)
+
+
+def test_scene_files_round_trip_host(tmp_path):
+    """Scene files (xpbd_amd.scene_io): a cloth and a box pile with a joint
+    saved before any device work load back to identical host arrays, knobs
+    and body flags; the files hold no pickled objects."""
+    from xpbd_amd import Solver, Solver6DOF, scene_io
+    from xpbd_amd.scenes import box_pile, cloth_sheet
+
+    s = cloth_sheet(Solver, res=8, solve_mode="gs", device="cuda:0", substeps=7)[0]
+    s.enable_self_collision(True, margin=0.01)
+    p = tmp_path / "cloth.npz"
+    scene_io.save(s, p)
+    t = scene_io.load(p, device="cuda:0")
+    h0, h1 = s.host_arrays(), t.host_arrays()
+    for k in h0:
+        assert np.array_equal(h0[k], h1[k]), k
+    assert (t.substeps, t.solve_mode, t._self_collision) == (7, "gs", True)
+    assert t._contact_margin == 0.01
+    r = box_pile(Solver6DOF, 2, 2, 2, device="cuda:0", restitution=0.3)
+    q = r.add_particle((0.0, 3.0, 0.0), 0.1)
+    r.add_joint(r.bodies[0], q, compliance=1e-4)
+    p6 = tmp_path / "pile.npz"
+    scene_io.save(r, p6)
+    u = scene_io.load(p6, device="cuda:0")
+    h0, h1 = r.host_arrays(), u.host_arrays()
+    for k in h0:
+        assert np.array_equal(h0[k], h1[k]), k
+    assert [b.is_particle for b in u.bodies] == [b.is_particle for b in r.bodies]
+    assert (u.restitution, u.num_joints) == (0.3, 1)
+    for f in (p, p6):
+        with np.load(f, allow_pickle=False) as z:
+            assert all(z[k].dtype != object for k in z.files)

@@ -196,6 +196,31 @@ class BatchedWorld:
             n += self.rigid.num_joints + pts + 8 * dyn
         return n
 
+    def save(self, prefix: str) -> None:
+        """Write the batch as scene files (``xpbd_amd.scene_io``):
+        ``<prefix>_particles.npz`` and, with box piles, ``<prefix>_rigid.npz``."""
+        from . import scene_io
+
+        scene_io.save(self.particles, prefix + "_particles.npz")
+        if self.rigid is not None:
+            scene_io.save(self.rigid, prefix + "_rigid.npz")
+
+    @classmethod
+    def load(cls, prefix: str, device="cuda:0") -> "BatchedWorld":
+        """A batch from the scene files written by ``save`` (no spec: the
+        per-scene ranges are not stored)."""
+        import os
+
+        from . import scene_io
+
+        w = cls.__new__(cls)
+        w.spec = None
+        w.particles = scene_io.load(prefix + "_particles.npz", device)
+        rp = prefix + "_rigid.npz"
+        w.rigid = scene_io.load(rp, device) if os.path.exists(rp) else None
+        w._stream_b = None
+        return w
+
     def step(self) -> None:
         import torch
 

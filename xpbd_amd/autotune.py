@@ -12,8 +12,8 @@ and keeps the fastest:
   apply pass, with 1 or 2 rows per thread;
 * ``block``  — whole frames in one workgroup (scenes that fit its LDS).
 
-With a learned warm start set (``Solver.set_warm_start``) only the tile
-candidates apply (the warm start runs inside the first tile launch).
+With a learned warm start, a per-particle gravity scale or device contact
+generation (tile-mode extensions) only the tile candidates apply.
 
 The simulation state is saved before and restored after each trial, so tuning
 does not advance the scene.  Results are cached per scene signature (particle
@@ -26,7 +26,7 @@ _CACHE: dict = {}
 
 def _signature(s):
     return (s.num_bodies, s.num_constraints, s.substeps, s.iterations, s.solve_mode,
-            s._ws_theta is not None)
+            s._ws_theta is not None, s._gscale is not None, getattr(s, "_device_contacts", False))
 
 
 def candidates(n_particles: int, tile_only: bool = False):
@@ -79,7 +79,9 @@ def tune(s, frames: int = 3, warmup: int = 1, verbose: bool = False):
         return _CACHE[sig], {}
     x0, v0 = s.positions(), s.velocities()
     times = {}
-    for cand in candidates(s.num_bodies, tile_only=s._ws_theta is not None):
+    tile_only = (s._ws_theta is not None or s._gscale is not None
+                 or getattr(s, "_device_contacts", False))
+    for cand in candidates(s.num_bodies, tile_only=tile_only):
         _apply(s, cand)
         s.set_state(x0, v0)
         s.step(warmup)

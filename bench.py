@@ -313,13 +313,19 @@ def warm_start_section(res: int, frames: int) -> dict:
     s.step(20)
     resid, fallback = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(1, 2, 4),
                                                  learned_iters=(1, 2))
+    amort, _ = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(), learned_iters=(1, 2),
+                                          net_every=15)
+    resid.update({f"{k}_net_per_frame": v for k, v in amort.items()})
     sps, shapes = {}, {}
-    for name, iters, learned in [("plain_4", 4, False), ("plain_2", 2, False),
-                                 ("plain_1", 1, False), ("learned_2", 2, True),
-                                 ("learned_1", 1, True)]:
+    for name, iters, learned, every in [("plain_4", 4, False, 1), ("plain_2", 2, False, 1),
+                                        ("plain_1", 1, False, 1), ("learned_2", 2, True, 1),
+                                        ("learned_1", 1, True, 1),
+                                        ("learned_2_net_per_frame", 2, True, 15),
+                                        ("learned_1_net_per_frame", 1, True, 15)]:
         c = cloth(iters)
         c.block_frames = False
         if learned:
+            c.warm_start_every = every
             c.set_warm_start(theta)
         best, _ = autotune.tune(c)
         shapes[name] = list(best) if best else None
@@ -338,6 +344,12 @@ def warm_start_section(res: int, frames: int) -> dict:
                                "speedup": round(sps["learned_2"] / sps["plain_4"], 3)},
         "learned1_vs_plain4": {"residual_ratio": round(resid["learned_1"] / resid["plain_4"], 3),
                                "speedup": round(sps["learned_1"] / sps["plain_4"], 3)},
+        "learned1_net_per_frame_vs_plain4": {
+            "residual_ratio": round(resid["learned_1_net_per_frame"] / resid["plain_4"], 3),
+            "speedup": round(sps["learned_1_net_per_frame"] / sps["plain_4"], 3)},
+        "learned2_net_per_frame_vs_plain4": {
+            "residual_ratio": round(resid["learned_2_net_per_frame"] / resid["plain_4"], 3),
+            "speedup": round(sps["learned_2_net_per_frame"] / sps["plain_4"], 3)},
     }
 
 

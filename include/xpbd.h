@@ -222,6 +222,10 @@ typedef struct xpbd_particles_desc {
    * color_starts (GS). */
   int block;
   const int* color_starts_dev;
+  /* learned warm start: the network pass runs on substeps sub % ws_every ==
+   * 0 of each frame (<= 1: every substep); the other substeps reuse ws_y
+   * with their own first plain steps */
+  int ws_every;
 } xpbd_particles_desc;
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);

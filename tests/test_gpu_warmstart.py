@@ -237,13 +237,21 @@ def test_learned_start_reaches_zero_start_residual_with_half_the_iterations(trai
     assert res["learned_2"] < res["plain_2"]
     assert res["learned_1"] <= res["plain_2"]
     assert res["learned_2"] <= res["plain_4"]
+    # the network pass once per frame (15 substeps), its outputs reused by
+    # the other substeps with their own first plain steps
+    amort, fb = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(), learned_iters=(1, 2),
+                                           net_every=15)
+    print("network once per frame: rms residual", amort)
+    assert amort["learned_1"] <= res["plain_2"]
+    assert amort["learned_2"] <= res["plain_4"]
 
 
-@pytest.mark.parametrize("fuse", [True, False])
-def test_frame_plan_with_warm_start_matches_eager_launches(trained, fuse):
+@pytest.mark.parametrize("fuse,every", [(True, 1), (False, 1), (True, 5)])
+def test_frame_plan_with_warm_start_matches_eager_launches(trained, fuse, every):
     """Two frames of the captured plan with the warm start (integrate folded
-    into the warm-start launch, or not) equal the eager per-substep launches
-    (integrate, warm-start launch, plain launches) bit for bit."""
+    into the warm-start launch, or not; network pass on every substep or on
+    every 5th) equal the eager per-substep launches (integrate, network
+    pass, warm-start launch, plain launches) bit for bit."""
     import torch
 
     from xpbd_amd import warmstart as ws
@@ -253,13 +261,14 @@ def test_frame_plan_with_warm_start_matches_eager_launches(trained, fuse):
     for s in (a, b):
         s.fuse_first = fuse
         s.step(3)
+    a.warm_start_every = every
     a.set_warm_start(theta)
     a.step(2)
     e = ws._Eager(b)
     th = torch.as_tensor(theta).to(DEV)
-    for _ in range(2 * b.substeps):
+    for k in range(2 * b.substeps):
         e.integrate()
-        e.iterate(b.iterations, theta=th)
+        e.iterate(b.iterations, theta=th, net=(k % b.substeps) % every == 0)
     assert np.array_equal(a.positions(), b.positions())
     assert np.array_equal(a.velocities(), b.velocities())
 

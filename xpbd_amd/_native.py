@@ -38,6 +38,7 @@ class ParticlesDesc(ctypes.Structure):
         ("dyn_cap", c_int), ("gscale", c_void_p),
         ("ws_image", c_void_p), ("ws_y", c_void_p),
         ("fuse_first", c_int), ("block", c_int), ("color_starts_dev", c_void_p),
+        ("ws_every", c_int),
     ]
 
 

@@ -45,8 +45,10 @@ struct Frame3 {
 // and the plain launches.
 //
 // With the learned warm start (ws_image) every substep first runs
-// xpbd_ws_net and xpbd_jacobi_tile_warm (integrate folded into both on the
-// fused path) and then all `iterations` plain launches with first = 0.  Multipliers ping-pong as the
+// xpbd_jacobi_tile_warm (integrate folded in on the fused path), preceded
+// on every ws_every-th substep by the network pass xpbd_ws_net (the other
+// substeps reuse its outputs), and then all `iterations` plain launches with
+// first = 0.  Multipliers ping-pong as the
 // positions do: a substep's first launch writes lam_alt.
 int issue_frame_tile(Frame3* f, hipStream_t s) {
   const xpbd_particles_desc& d = f->d;
@@ -56,6 +58,7 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
   const int it_n = have_rows ? d.iterations : 0;
   const bool dyn = d.dyn_cap > 0 && d.dyn_rows && d.dyn_count;
   const bool warm = d.ws_image != nullptr && d.n_rows > 0 && it_n > 0;
+  const int ws_every = d.ws_every > 1 ? d.ws_every : 1;
   float* bufs[2] = {d.x, d.x_alt};
   float* lams[2] = {d.lam, d.lam_alt};
   int rc;
@@ -63,9 +66,11 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
     int loc = 0;
     for (int sub = 0; sub < d.substeps; ++sub) {
       if (warm) {
-        rc = xpbd_ws_net(s, bufs[loc], d.x_prev, d.v, d.rows, d.anchors, d.ws_image, d.ws_y, d.h,
-                         d.gx, d.gy, d.gz, d.gscale, d.n_rows, 1);
-        if (rc) return rc;
+        if (sub % ws_every == 0) {
+          rc = xpbd_ws_net(s, bufs[loc], d.x_prev, d.v, d.rows, d.anchors, d.ws_image, d.ws_y,
+                           d.h, d.gx, d.gy, d.gz, d.gscale, d.n_rows, 1);
+          if (rc) return rc;
+        }
         rc = xpbd_jacobi_tile_warm(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
                                    lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.ws_y,
                                    d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n, d.relax, 1);
@@ -101,9 +106,11 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
     if (rc) return rc;
     int li = 0;
     if (warm) {
-      rc = xpbd_ws_net(s, bufs[cur], d.x_prev, d.v, d.rows, d.anchors, d.ws_image, d.ws_y, d.h,
-                       0.0f, 0.0f, 0.0f, nullptr, d.n_rows, 0);
-      if (rc) return rc;
+      if (sub % ws_every == 0) {
+        rc = xpbd_ws_net(s, bufs[cur], d.x_prev, d.v, d.rows, d.anchors, d.ws_image, d.ws_y,
+                         d.h, 0.0f, 0.0f, 0.0f, nullptr, d.n_rows, 0);
+        if (rc) return rc;
+      }
       rc = xpbd_jacobi_tile_warm(s, bufs[cur], bufs[cur ^ 1], d.x_prev, d.v, d.rows, d.anchors,
                                  lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.ws_y,
                                  d.h, 0.0f, 0.0f, 0.0f, nullptr, d.tile_size, n, d.relax, 0);

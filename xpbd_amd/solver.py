@@ -274,6 +274,7 @@ class Solver:
         self._gscale = None           # per-particle gravity factor (API order)
         self._ws_theta = None         # learned lambda warm start parameters
         self._ws_gen = 0
+        self.warm_start_every = 1     # network pass every k-th substep of a frame
 
         # host scene, one column per field (float64 as the reference's
         # Python floats; rounded to float32 once, at flush)
@@ -765,7 +766,7 @@ class Solver:
         return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
                 self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling,
                 self._tile, getattr(self, "_ccap", 0), bool(self.fuse_first), self._ws_gen,
-                str(self.block_frames))
+                str(self.block_frames), int(self.warm_start_every))
 
     # "auto" limit of whole-frame workgroups in Jacobi mode: above it the
     # tile path's 256 CUs beat one CU (measured: 256 particles 0.055 against
@@ -828,6 +829,7 @@ class Solver:
             N.check(N.lib().xpbd_mlp_pack(N.current_stream(), N.ptr(self._ws_theta),
                                           N.ptr(self.WSI)), "xpbd_mlp_pack")
             d.ws_image, d.ws_y = N.ptr(self.WSI), N.ptr(self.WSY)
+            d.ws_every = int(self.warm_start_every)
         self._block_active = self._use_block()
         if self._block_active:
             d.block = 1

@@ -137,12 +137,12 @@ class _Eager:
                                            N.ptr(s.GS) if s.GS is not None else None, self.n))
 
     def iterate(self, iters, theta=None, finalize=True, X=None, X2=None, L=None, L2=None,
-                Y=None):
+                Y=None, net=True):
         """``iters`` tile launches from X (default the solver's state),
         ping-ponging with X2; the result is left in X and L.  ``theta``: the
-        network pass (output into Y, default the solver's buffer) and the
-        warm-start launch first, then the iterations continue from its
-        multipliers."""
+        network pass (output into Y, default the solver's buffer; skipped
+        with ``net`` False, reusing Y) and the warm-start launch first, then
+        the iterations continue from its multipliers."""
         s = self.s
         X = s.X if X is None else X
         X2 = s.X_B if X2 is None else X2
@@ -153,10 +153,11 @@ class _Eager:
         trs, hs, hr = self.tables
         if theta is not None:
             Y = s.WSY if Y is None else Y
-            img = self.image(theta)
-            N.check(self.lib.xpbd_ws_net(st, N.ptr(src), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R),
-                                         N.ptr(s.ANC), N.ptr(img), N.ptr(Y), self.h, 0.0, 0.0,
-                                         0.0, None, self.m, 0))
+            if net:
+                img = self.image(theta)
+                N.check(self.lib.xpbd_ws_net(st, N.ptr(src), N.ptr(s.XP), N.ptr(s.V),
+                                             N.ptr(s.R), N.ptr(s.ANC), N.ptr(img), N.ptr(Y),
+                                             self.h, 0.0, 0.0, 0.0, None, self.m, 0))
             N.check(self.lib.xpbd_jacobi_tile_warm(
                 st, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), N.ptr(s.ANC),
                 N.ptr(ld), trs, hs, hr, N.ptr(Y), self.h, 0.0, 0.0, 0.0, None, s._tile, self.n,
@@ -207,14 +208,15 @@ TARGET_SCALE = 0.5
 
 
 def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int | None = None,
-            theta=None, target_scale: float = TARGET_SCALE):
+            theta=None, target_scale: float = TARGET_SCALE, net_every: int = 1):
     """Run the solver for ``substeps`` substeps; every ``every``-th substep
     append (features, target) for all rows.  Returns (F, T) device tensors
     of shape (k m, 12) and (k m,).
 
     ``theta`` None: the trajectory is the plain solver's; ``theta`` given: the
     trajectory runs the learned start (an on-policy round, whose states are
-    the ones the deployed network meets)."""
+    the ones the deployed network meets), with the network pass on every
+    ``net_every``-th substep of each frame as deployed."""
     import torch
 
     e = _Eager(s)
@@ -232,13 +234,13 @@ def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int
             e.iterate(target_iters, finalize=False, X=Xc, X2=Xc2, L=Lc, L2=Lc2)
             F_all.append(F.clone())
             T_all.append(target_logit(F, Lc[:e.m], scale, float(target_scale)))
-        e.iterate(iters, theta=th)
+        e.iterate(iters, theta=th, net=(k % s.substeps) % max(1, net_every) == 0)
     return torch.cat(F_all), torch.cat(T_all)
 
 
 def fit(make_scene, substeps: int = 120, settle_frames: int = 2, steps: int = 800,
         batch: int = 16384, seed: int = 0, log: list | None = None,
-        onpolicy_iters=(1, 2)):
+        onpolicy_iters=(1, 2), net_every: int = 1):
     """Two rounds: data from the plain solver's trajectory, training, data
     from the learned start's own trajectories (one per iteration count in
     ``onpolicy_iters``: the states a start followed by one iteration meets
@@ -256,7 +258,7 @@ def fit(make_scene, substeps: int = 120, settle_frames: int = 2, steps: int = 80
     Fs, Ts = [F], [T]
     ok = True
     for k in onpolicy_iters:
-        F2, T2 = collect(fresh(), substeps, theta=theta, iters=k)
+        F2, T2 = collect(fresh(), substeps, theta=theta, iters=k, net_every=net_every)
         if bool(torch.isfinite(F2).all() and torch.isfinite(T2).all()):
             Fs.append(F2)
             Ts.append(T2)
@@ -338,7 +340,10 @@ class Evaluator:
                 dst.copy_(src)
         return restore
 
-    def run(self, substeps: int, plain_iters=(1, 2, 4), learned_iters=(1, 2)):
+    def run(self, substeps: int, plain_iters=(1, 2, 4), learned_iters=(1, 2), net_every: int = 1):
+        """``net_every``: the learned start's network pass runs on every
+        ``net_every``-th substep of each frame (the frame plan's
+        ``warm_start_every``)."""
         e = self.e
         restore = self._keep()
         runs = {}
@@ -346,9 +351,10 @@ class Evaluator:
                                                                  for k in learned_iters]:
             restore()
             r = []
-            for _ in range(substeps):
+            for k in range(substeps):
                 e.integrate()
-                e.iterate(its, theta=self.theta if kind == "learned" else None)
+                net = (k % self.s.substeps) % max(1, net_every) == 0
+                e.iterate(its, theta=self.theta if kind == "learned" else None, net=net)
                 r.append(e.residual())
             runs[(kind, its)] = np.asarray(r)
         restore()

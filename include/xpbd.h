@@ -102,12 +102,13 @@ int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* X
  * lambda starts from zero (LAMout written for every row), Xout receives the
  * iterated positions (Xout != X).  V is not written: the caller finishes the
  * substep with at least one xpbd_jacobi_tile launch with finalize = 1.
- * Same tile tables as xpbd_jacobi_tile; gscale as in xpbd_integrate_to. */
+ * Same tile tables as xpbd_jacobi_tile; gscale as in xpbd_integrate_to; ACC
+ * (optional) as in xpbd_jacobi_tile, filled by xpbd_solve_jacobi_dyn_first. */
 int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP, const float* V,
                            const int* R, const float* ANC, float* LAMout,
                            const int* tile_row_start, const int* halo_start, const int* halo_rows,
                            float h, float gx, float gy, float gz, const float* gscale,
-                           int tile_size, int n_particles, float relax);
+                           int tile_size, int n_particles, float relax, float* ACC);
 
 /* kernels.py:49-73 `integrate` reading Xin and writing Xout (may alias);
  * gscale (optional, float[n]) scales gravity per particle (batched scenes). */
@@ -120,6 +121,12 @@ int xpbd_integrate_to(void* stream, const float* Xin, float* Xout, float* V, flo
 int xpbd_solve_jacobi_dyn(void* stream, const float* X, const float* XP, const int* R,
                           float* LAM, float* ACC, float h, int first, const int* n_rows_dev,
                           int cap);
+/* The same for the first iteration of a substep with integrate folded in
+ * (as xpbd_jacobi_tile_first: X = positions at the start of the substep,
+ * predicted from X and V on the fly, x_prev = X; lambda from zero). */
+int xpbd_solve_jacobi_dyn_first(void* stream, const float* X, const float* V, const int* R,
+                                float* LAM, float* ACC, float h, float gx, float gy, float gz,
+                                const float* gscale, const int* n_rows_dev, int cap);
 
 /* Device CONTACT generation, Solver._rebuild_contacts (solver.py:275-342):
  * margin = min(contact_margin + 3 dt max|v|, r_max), cell = 2 r_max + margin,

@@ -33,9 +33,11 @@ struct Frame3 {
 
 // Fused-tile Jacobi frame.
 //
-// With fuse_first and no device-counted rows, each substep is `iterations`
-// k_jacobi_tile launches only: the first has integrate folded in
-// (xpbd_jacobi_tile_first), the last runs finalize + cap.  Launches
+// With fuse_first, each substep is `iterations` k_jacobi_tile launches only
+// (plus, with device-counted rows, their projection before each): the first
+// has integrate folded in (xpbd_jacobi_tile_first, and
+// xpbd_solve_jacobi_dyn_first for the device rows), the last runs finalize +
+// cap.  Launches
 // ping-pong between X and X_alt; `loc` tracks which one holds the state, and
 // a frame that ends in X_alt copies it back.
 //
@@ -62,9 +64,15 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
   float* bufs[2] = {d.x, d.x_alt};
   float* lams[2] = {d.lam, d.lam_alt};
   int rc;
-  if (d.fuse_first && !dyn && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1))) {
+  if (d.fuse_first && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1)) && !(dyn && warm)) {
     int loc = 0;
+    float* acc = dyn ? d.acc : nullptr;
     for (int sub = 0; sub < d.substeps; ++sub) {
+      if (dyn) {  // device-counted rows, first iteration on the predicted positions
+        rc = xpbd_solve_jacobi_dyn_first(s, bufs[loc], d.v, d.dyn_rows, d.dyn_lam, d.acc, d.h,
+                                         d.gx, d.gy, d.gz, d.gscale, d.dyn_count, d.dyn_cap);
+        if (rc) return rc;
+      }
       if (warm) {
         if (sub % ws_every == 0) {
           rc = xpbd_ws_net(s, bufs[loc], d.x_prev, d.v, d.rows, d.anchors, d.ws_image, d.ws_y,
@@ -78,16 +86,21 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
         rc = xpbd_jacobi_tile_first(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows,
                                     d.anchors, lams[1], d.tile_row_start, d.halo_start,
                                     d.halo_rows, d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n,
-                                    d.relax);
+                                    d.relax, acc);
       }
       if (rc) return rc;
       loc ^= 1;
       int li = 1;  // lams[li] holds the current multipliers
       for (int it = warm ? 0 : 1; it < it_n; ++it) {
+        if (dyn) {
+          rc = xpbd_solve_jacobi_dyn(s, bufs[loc], d.x_prev, d.dyn_rows, d.dyn_lam, d.acc, d.h, 0,
+                                     d.dyn_count, d.dyn_cap);
+          if (rc) return rc;
+        }
         rc = xpbd_jacobi_tile(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
                               lams[li], lams[li ^ 1], d.tile_row_start, d.halo_start,
                               d.halo_rows, d.h, 0, d.tile_size, n, d.relax, it + 1 == it_n,
-                              d.damping, d.max_speed, nullptr);
+                              d.damping, d.max_speed, acc);
         if (rc) return rc;
         loc ^= 1;
         li ^= 1;

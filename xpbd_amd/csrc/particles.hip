@@ -260,17 +260,32 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_project_global(
 
 // Projection of a row segment whose live length is on the device (rows
 // generated on the GPU, e.g. CONTACT rows): plain global atomics into ACC.
-template <int BLOCK>
+// PREDICT: the first iteration of a substep with integrate folded in, as in
+// the tile kernel: positions are predicted from X and V on the fly (the
+// arithmetic of integrate, gscale optional), x_prev is X itself.
+template <int BLOCK, bool PREDICT>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_project_dyn(
-    const float4* __restrict__ X, const float4* __restrict__ XP, const int4* __restrict__ R,
-    float* __restrict__ LAM, float4* __restrict__ ACC, float h, int first,
+    const float4* __restrict__ X, const float4* __restrict__ XP, const float4* __restrict__ V,
+    const int4* __restrict__ R, float* __restrict__ LAM, float4* __restrict__ ACC, float h,
+    int first, float gx, float gy, float gz, const float* __restrict__ gscale,
     const int* __restrict__ n_dev, int cap) {
   const int j = blockIdx.x * BLOCK + threadIdx.x;
   if (j >= min(*n_dev, cap)) return;
   const float inv_dt2 = 1.0f / (h * h);
   const int4 row = R[j];
   const float lam = first ? 0.0f : LAM[j];
-  const Proj p = project_jacobi(X, XP, row, make_float4(0.f, 0.f, 0.f, 0.f), lam, inv_dt2);
+  auto pos = [&](int i) -> float4 {
+    const float4 x4 = X[i];
+    if (!PREDICT || x4.w == 0.0f) return x4;
+    f3 g = mk3(gx, gy, gz);
+    if (gscale) g = g * gscale[i];
+    const f3 v = xyz(V[i]) + g * h;
+    const f3 x = xyz(x4) + v * h;
+    return make_float4(x.x, x.y, x.z, x4.w);
+  };
+  auto prev = [&](int i) { return PREDICT ? xyz(X[i]) : xyz(XP[i]); };
+  const Proj p = project_row(pos, prev, [&]() { return make_float4(0.f, 0.f, 0.f, 0.f); }, row,
+                             lam, inv_dt2);
   if (p.upd || first) LAM[j] = p.lam;
   if (p.na >= 1) {
     atomicAdd(&ACC[p.a].x, p.ca.x);
@@ -994,12 +1009,14 @@ int launch_tile_block(int mode, hipStream_t s, int nb, size_t shmem, const float
     XPBD_TILE_MODE(TM_FINALIZE | TM_ACC)
     XPBD_TILE_MODE(TM_ANCHOR)
     XPBD_TILE_MODE(TM_PREDICT | TM_ANCHOR)
+    XPBD_TILE_MODE(TM_PREDICT | TM_ACC)
+    XPBD_TILE_MODE(TM_PREDICT | TM_ACC | TM_ANCHOR)
     XPBD_TILE_MODE(TM_FINALIZE | TM_ANCHOR)
     XPBD_TILE_MODE(TM_ACC | TM_ANCHOR)
     XPBD_TILE_MODE(TM_FINALIZE | TM_ACC | TM_ANCHOR)
   }
 #undef XPBD_TILE_MODE
-  return (int)hipErrorInvalidValue;  // predict never combines with finalize or ACC
+  return (int)hipErrorInvalidValue;  // predict never combines with finalize
 }
 
 // ANC may be NULL when no row is ATTACH, ACC when no device-counted rows
@@ -1112,11 +1129,11 @@ int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP,
                            const int* R, const float* ANC, float* LAMout,
                            const int* tile_row_start, const int* halo_start, const int* halo_rows,
                            float h, float gx, float gy, float gz, const float* gscale,
-                           int tile_size, int n_particles, float relax) {
+                           int tile_size, int n_particles, float relax, float* ACC) {
   if (X == Xout) return (int)hipErrorInvalidValue;  // neighbours read X while tiles write
   return launch_tile(true, (hipStream_t)stream, X, Xout, XP, const_cast<float*>(V), R, ANC,
                      nullptr, LAMout, tile_row_start, halo_start, halo_rows, h, 1, tile_size,
-                     n_particles, relax, 0, 0.0f, 0.0f, nullptr, gx, gy, gz, gscale);
+                     n_particles, relax, 0, 0.0f, 0.0f, ACC, gx, gy, gz, gscale);
 }
 
 int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP, const float* V,
@@ -1162,9 +1179,22 @@ int xpbd_solve_jacobi_dyn(void* stream, const float* X, const float* XP, const i
                           float* LAM, float* ACC, float h, int first, const int* n_rows_dev,
                           int cap) {
   if (cap <= 0) return 0;
-  hipLaunchKernelGGL(k_jacobi_project_dyn<kBlock>, dim3(blocks_for(cap, kBlock)), dim3(kBlock), 0,
-                     (hipStream_t)stream, (const float4*)X, (const float4*)XP, (const int4*)R,
-                     LAM, (float4*)ACC, h, first, n_rows_dev, cap);
+  hipLaunchKernelGGL((k_jacobi_project_dyn<kBlock, false>), dim3(blocks_for(cap, kBlock)),
+                     dim3(kBlock), 0, (hipStream_t)stream, (const float4*)X, (const float4*)XP,
+                     nullptr, (const int4*)R, LAM, (float4*)ACC, h, first, 0.0f, 0.0f, 0.0f,
+                     nullptr, n_rows_dev, cap);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_solve_jacobi_dyn_first(void* stream, const float* X, const float* V, const int* R,
+                                float* LAM, float* ACC, float h, float gx, float gy, float gz,
+                                const float* gscale, const int* n_rows_dev, int cap) {
+  if (cap <= 0) return 0;
+  hipLaunchKernelGGL((k_jacobi_project_dyn<kBlock, true>), dim3(blocks_for(cap, kBlock)),
+                     dim3(kBlock), 0, (hipStream_t)stream, (const float4*)X, nullptr,
+                     (const float4*)V, (const int4*)R, LAM, (float4*)ACC, h, 1, gx, gy, gz,
+                     gscale, n_rows_dev, cap);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

@@ -161,6 +161,11 @@ class BatchedWorld:
             P.set_gravity_scale(np.concatenate(gs))
             if any(sc.kind == "pile" for sc in spec.scenes):
                 P.enable_device_contacts(True)
+                # a separate integrate pass: folding it into the first
+                # iteration measured slower on this mix (3.71 against 3.53 ms
+                # per frame: every contact and out-of-tile endpoint then
+                # re-reads V to predict its position)
+                P.fuse_first = False
         self.rigid = None
         if boxes:
             R = Solver6DOF(dt=dt, substeps=substeps, iterations=iterations, device=device,

@@ -110,6 +110,24 @@ int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP,
                            float h, float gx, float gy, float gz, const float* gscale,
                            int tile_size, int n_particles, float relax, float* ACC);
 
+/* The last iteration of a substep followed by the next substep's
+ * kernels.py:49-73 `integrate` (solver.py:431-458: apply, finalize_velocity,
+ * cap_velocity of substep k, then integrate of substep k + 1) in one launch:
+ * as xpbd_jacobi_tile with finalize = 1, except that the finalized velocity
+ * stays in registers: XP receives the iterated positions, Xout the positions
+ * predicted from them (v + g h, gscale optional, exactly as
+ * xpbd_integrate_to computes it) and V is not written, so the next substep
+ * continues with a plain first launch (first = 1) and the frame's last
+ * substep ends with xpbd_jacobi_tile (finalize = 1), which writes V.  Other
+ * tiles read XP while a tile writes it: no row may be CONTACT (their
+ * friction reads the other endpoint's x_prev); Xin != Xout. */
+int xpbd_jacobi_tile_next(void* stream, const float* Xin, float* Xout, float* XP, const int* R,
+                          const float* ANC, const float* LAMin, float* LAMout,
+                          const int* tile_row_start, const int* halo_start, const int* halo_rows,
+                          float h, int first, float gx, float gy, float gz, const float* gscale,
+                          int tile_size, int n_particles, float relax, float damping,
+                          float max_speed, float* ACC);
+
 /* kernels.py:49-73 `integrate` reading Xin and writing Xout (may alias);
  * gscale (optional, float[n]) scales gravity per particle (batched scenes). */
 int xpbd_integrate_to(void* stream, const float* Xin, float* Xout, float* V, float* XP, float gx,
@@ -219,8 +237,11 @@ typedef struct xpbd_particles_desc {
    * xpbd_jacobi_tile_warm */
   const void* ws_image;
   float* ws_y;           /* float[n_rows] */
-  /* Tile mode with >= 2 iterations and no dyn rows: 1 folds integrate into
-   * the first iteration (xpbd_jacobi_tile_first), 0 keeps a separate
+  /* Tile mode: 1 folds integrate into the first iteration of every substep
+   * (xpbd_jacobi_tile_first; >= 2 iterations or the learned warm start), 2
+   * folds it into the first substep's first iteration and into the previous
+   * substep's last iteration for the others (xpbd_jacobi_tile_next; no
+   * CONTACT rows among `rows`; any iteration count), 0 keeps a separate
    * xpbd_integrate_to launch per substep.  Results are identical. */
   int fuse_first;
   /* Small scenes (n_particles <= xpbd_frame_block_capacity): 1 runs whole

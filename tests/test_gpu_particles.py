@@ -273,6 +273,93 @@ def test_tile_first_equals_integrate_then_tile(gscale):
     assert torch.equal(V0, s.V)          # the fused launch leaves V alone
 
 
+@pytest.mark.parametrize("gscale", [False, True])
+def test_tile_next_equals_finalize_then_integrate(gscale):
+    """xpbd_jacobi_tile_next (the last iteration of a substep with the next
+    substep's integrate folded in) gives bit for bit what a finalizing
+    xpbd_jacobi_tile launch followed by xpbd_integrate_to gives: predicted
+    positions, x_prev and multipliers; V is left alone."""
+    import torch
+
+    from xpbd_amd import Solver
+    from xpbd_amd import _native as N
+    from xpbd_amd.scenes import cloth_sheet
+
+    s, _ = cloth_sheet(Solver, res=40, solve_mode="jacobi", device=DEV)
+    s.tile_size = 64
+    s.max_speed = 0.5                      # the cap takes part
+    for _ in range(12):
+        s.step()
+    s._flush()
+    lib = N.lib()
+    n, m = s.num_bodies, s.num_constraints
+    h = float(np.float32(s.dt / s.substeps))
+    g = s.gravity
+    gs = (torch.rand(n, device=DEV) + 0.5) if gscale else None
+    ts, hs, hr = s._tile_dev
+    # a mid-substep state: integrate, one plain iteration
+    X0, V0, XP0 = torch.empty_like(s.X), s.V.clone(), torch.zeros_like(s.X)
+    L0 = torch.zeros(m, device=DEV)
+    N.check(lib.xpbd_integrate_to(None, N.ptr(s.X), N.ptr(X0), N.ptr(V0), N.ptr(XP0), *g, h,
+                                  N.ptr(gs) if gs is not None else None, n))
+    X1 = torch.empty_like(X0)
+    N.check(lib.xpbd_jacobi_tile(None, N.ptr(X0), N.ptr(X1), N.ptr(XP0), N.ptr(V0), N.ptr(s.R),
+                                 N.ptr(s.ANC), N.ptr(L0), N.ptr(L0), N.ptr(ts), N.ptr(hs),
+                                 N.ptr(hr), h, 1, s._tile, n, s.jacobi_relax, 0, s.damping,
+                                 s.max_speed, None))
+    # path A: finalizing iteration, then integrate
+    XA, VA, XPA, LA = torch.empty_like(X0), V0.clone(), XP0.clone(), torch.zeros(m, device=DEV)
+    N.check(lib.xpbd_jacobi_tile(None, N.ptr(X1), N.ptr(XA), N.ptr(XPA), N.ptr(VA), N.ptr(s.R),
+                                 N.ptr(s.ANC), N.ptr(L0), N.ptr(LA), N.ptr(ts), N.ptr(hs),
+                                 N.ptr(hr), h, 0, s._tile, n, s.jacobi_relax, 1, s.damping,
+                                 s.max_speed, None))
+    N.check(lib.xpbd_integrate_to(None, N.ptr(XA), N.ptr(XA), N.ptr(VA), N.ptr(XPA), *g, h,
+                                  N.ptr(gs) if gs is not None else None, n))
+    # path B: one launch
+    XB, XPB, LB = torch.empty_like(X0), XP0.clone(), torch.zeros(m, device=DEV)
+    VB = V0.clone()
+    N.check(lib.xpbd_jacobi_tile_next(None, N.ptr(X1), N.ptr(XB), N.ptr(XPB), N.ptr(s.R),
+                                      N.ptr(s.ANC), N.ptr(L0), N.ptr(LB), N.ptr(ts), N.ptr(hs),
+                                      N.ptr(hr), h, 0, *g,
+                                      N.ptr(gs) if gs is not None else None, s._tile, n,
+                                      s.jacobi_relax, s.damping, s.max_speed, None))
+    torch.cuda.synchronize()
+    assert torch.equal(XA, XB)
+    assert torch.equal(XPA, XPB)
+    assert torch.equal(LA, LB)
+    assert torch.equal(VB, V0)
+    with pytest.raises(RuntimeError):       # Xin == Xout is refused
+        N.check(lib.xpbd_jacobi_tile_next(None, N.ptr(X1), N.ptr(X1), N.ptr(XPB), N.ptr(s.R),
+                                          None, N.ptr(L0), N.ptr(LB), N.ptr(ts), N.ptr(hs),
+                                          N.ptr(hr), h, 0, *g, None, s._tile, n,
+                                          s.jacobi_relax, s.damping, s.max_speed, None))
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3])
+def test_frame_plan_integrate_folding_is_exact(iters):
+    """The three ways of running integrate in a tile frame (a separate pass,
+    folded into each substep's first launch, folded into the previous
+    substep's last launch) give identical positions, velocities and
+    multipliers, with per-particle gravity scales and ATTACH rows."""
+    from xpbd_amd import Solver
+    from xpbd_amd.scenes import cloth_sheet
+    from xpbd_amd.solver import Body
+
+    out = {}
+    for fuse in (False, "first", True):
+        s, _ = cloth_sheet(Solver, res=36, solve_mode="jacobi", device=DEV, iterations=iters)
+        s.tile_size = 64
+        s.block_frames = False
+        s.fuse_first = fuse
+        s.set_gravity_scale(np.linspace(0.5, 1.5, s.num_bodies))
+        s.add_attach(Body(5), (0.1, 0.6, 0.0), compliance=1e-6)
+        s.step(8)
+        out[str(fuse)] = (s.positions(), s.velocities(), s.lambdas())
+    for k in ("first", "True"):
+        for g, e in zip(out[k], out["False"]):
+            assert np.array_equal(g, e), (k, iters)
+
+
 def test_autotune_restores_state_and_keeps_results():
     """The autotuner times every launch shape without advancing the scene,
     and the shape it picks steps to the same state (to the Jacobi parity

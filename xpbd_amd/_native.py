@@ -80,6 +80,9 @@ _SIGS = {
     "xpbd_jacobi_tile_first": (c_int, [c_void_p] * 11 + [c_float, c_float, c_float, c_float,
                                                          c_void_p, c_int, c_int, c_float,
                                                          c_void_p]),
+    "xpbd_jacobi_tile_next": (c_int, [c_void_p] * 11 + [c_float, c_int, c_float, c_float, c_float,
+                                                        c_void_p, c_int, c_int, c_float, c_float,
+                                                        c_float, c_void_p]),
     "xpbd_integrate_to": (c_int, [c_void_p] * 5 + [c_float, c_float, c_float, c_float, c_void_p,
                                                    c_int]),
     "xpbd_solve_jacobi_dyn": (c_int, [c_void_p] * 6 + [c_float, c_int, c_void_p, c_int]),

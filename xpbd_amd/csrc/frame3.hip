@@ -37,7 +37,10 @@ struct Frame3 {
 // (plus, with device-counted rows, their projection before each): the first
 // has integrate folded in (xpbd_jacobi_tile_first, and
 // xpbd_solve_jacobi_dyn_first for the device rows), the last runs finalize +
-// cap.  Launches
+// cap.  With fuse_first = 2 only the frame's first substep does so (or, with
+// one iteration, runs an integrate pass in place); the last launch of every
+// other substep also integrates the next one (xpbd_jacobi_tile_next), whose
+// first launch is then a plain one with first = 1.  Launches
 // ping-pong between X and X_alt; `loc` tracks which one holds the state, and
 // a frame that ends in X_alt copies it back.
 //
@@ -64,43 +67,67 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
   float* bufs[2] = {d.x, d.x_alt};
   float* lams[2] = {d.lam, d.lam_alt};
   int rc;
-  if (d.fuse_first && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1)) && !(dyn && warm)) {
+  const bool next = d.fuse_first >= 2 && it_n >= 1;
+  if (d.fuse_first && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1) || next) &&
+      !(dyn && warm)) {
     int loc = 0;
     float* acc = dyn ? d.acc : nullptr;
     for (int sub = 0; sub < d.substeps; ++sub) {
-      if (dyn) {  // device-counted rows, first iteration on the predicted positions
-        rc = xpbd_solve_jacobi_dyn_first(s, bufs[loc], d.v, d.dyn_rows, d.dyn_lam, d.acc, d.h,
-                                         d.gx, d.gy, d.gz, d.gscale, d.dyn_count, d.dyn_cap);
-        if (rc) return rc;
-      }
+      // predicted: the previous substep's last launch already integrated
+      const bool predicted = next && sub > 0;
+      const bool integrate_next = next && sub + 1 < d.substeps;
+      int li = 0;  // lams[li] holds the current multipliers
+      int it = 0;
       if (warm) {
         if (sub % ws_every == 0) {
           rc = xpbd_ws_net(s, bufs[loc], d.x_prev, d.v, d.rows, d.anchors, d.ws_image, d.ws_y,
-                           d.h, d.gx, d.gy, d.gz, d.gscale, d.n_rows, 1);
+                           d.h, d.gx, d.gy, d.gz, d.gscale, d.n_rows, predicted ? 0 : 1);
           if (rc) return rc;
         }
         rc = xpbd_jacobi_tile_warm(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
                                    lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.ws_y,
-                                   d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n, d.relax, 1);
-      } else {
+                                   d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n, d.relax,
+                                   predicted ? 0 : 1);
+        if (rc) return rc;
+        loc ^= 1;
+        li = 1;
+      } else if (!predicted && it_n >= 2) {
+        if (dyn) {  // device-counted rows, first iteration on the predicted positions
+          rc = xpbd_solve_jacobi_dyn_first(s, bufs[loc], d.v, d.dyn_rows, d.dyn_lam, d.acc, d.h,
+                                           d.gx, d.gy, d.gz, d.gscale, d.dyn_count, d.dyn_cap);
+          if (rc) return rc;
+        }
         rc = xpbd_jacobi_tile_first(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows,
                                     d.anchors, lams[1], d.tile_row_start, d.halo_start,
                                     d.halo_rows, d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n,
                                     d.relax, acc);
+        if (rc) return rc;
+        loc ^= 1;
+        li = 1;
+        it = 1;
+      } else if (!predicted) {  // one iteration: integrate in place, then the chain
+        rc = xpbd_integrate_to(s, bufs[loc], bufs[loc], d.v, d.x_prev, d.gx, d.gy, d.gz, d.h,
+                               d.gscale, n);
+        if (rc) return rc;
       }
-      if (rc) return rc;
-      loc ^= 1;
-      int li = 1;  // lams[li] holds the current multipliers
-      for (int it = warm ? 0 : 1; it < it_n; ++it) {
+      for (; it < it_n; ++it) {
+        const int first = (!warm && it == 0) ? 1 : 0;
         if (dyn) {
-          rc = xpbd_solve_jacobi_dyn(s, bufs[loc], d.x_prev, d.dyn_rows, d.dyn_lam, d.acc, d.h, 0,
-                                     d.dyn_count, d.dyn_cap);
+          rc = xpbd_solve_jacobi_dyn(s, bufs[loc], d.x_prev, d.dyn_rows, d.dyn_lam, d.acc, d.h,
+                                     first, d.dyn_count, d.dyn_cap);
           if (rc) return rc;
         }
-        rc = xpbd_jacobi_tile(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
-                              lams[li], lams[li ^ 1], d.tile_row_start, d.halo_start,
-                              d.halo_rows, d.h, 0, d.tile_size, n, d.relax, it + 1 == it_n,
-                              d.damping, d.max_speed, acc);
+        if (it + 1 == it_n && integrate_next) {
+          rc = xpbd_jacobi_tile_next(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.rows, d.anchors,
+                                     lams[li], lams[li ^ 1], d.tile_row_start, d.halo_start,
+                                     d.halo_rows, d.h, first, d.gx, d.gy, d.gz, d.gscale,
+                                     d.tile_size, n, d.relax, d.damping, d.max_speed, acc);
+        } else {
+          rc = xpbd_jacobi_tile(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
+                                lams[li], lams[li ^ 1], d.tile_row_start, d.halo_start,
+                                d.halo_rows, d.h, first, d.tile_size, n, d.relax, it + 1 == it_n,
+                                d.damping, d.max_speed, acc);
+        }
         if (rc) return rc;
         loc ^= 1;
         li ^= 1;

@@ -369,11 +369,22 @@ __host__ __device__ inline size_t tile_lds_bytes(int P) { return tile_lds_pos_of
 // is left as it was: finalize, in a later launch of the same substep,
 // rewrites it, so a predict launch must never also finalize.
 //
+// NEXT (with FINALIZE): the last iteration of a substep also runs the next
+// substep's integrate on the finalized velocity held in registers: x_prev <-
+// x, x <- x + (v + g h) h (k_integrate_to's arithmetic on the value finalize
+// would have written to V, so the result is identical to finalize, then
+// integrate), and V is not written: the next substep's finalize derives the
+// velocity from x and x_prev again, and the frame's last substep finalizes
+// without NEXT.  Other tiles must not read x_prev during such a launch, so it
+// is used only when no row of the launch reads the x_prev of a particle
+// outside its own tile (no CONTACT rows: FLOOR rows read their own
+// particle's).
+//
 // The launch mode is a template argument (bits of TileMode) so that a
 // launch pays only for what it uses: every optional branch left in the
 // kernel costs registers and scheduling freedom in the row loop (a runtime
 // finalize / ACC branch alone measured ~10 % on 8 M-particle cloths).
-enum TileMode : int { TM_PREDICT = 1, TM_FINALIZE = 2, TM_ACC = 4, TM_ANCHOR = 8 };
+enum TileMode : int { TM_PREDICT = 1, TM_FINALIZE = 2, TM_ACC = 4, TM_ANCHOR = 8, TM_NEXT = 16 };
 
 // From this many particles the new positions are written with non-temporal
 // stores: 64 MB of positions plus the rows stream through the caches anyway,
@@ -392,6 +403,8 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     float gy, float gz, const float* __restrict__ gscale) {
   constexpr bool PREDICT = MODE & TM_PREDICT, FINALIZE = MODE & TM_FINALIZE;
   constexpr bool HAS_ACC = MODE & TM_ACC, HAS_ANCHOR = MODE & TM_ANCHOR;
+  constexpr bool NEXT = MODE & TM_NEXT;
+  static_assert(!NEXT || (FINALIZE && !PREDICT), "NEXT extends a finalizing launch");
   extern __shared__ unsigned long long lds64[];
   unsigned long long* sx = lds64;
   unsigned long long* sy = lds64 + P;
@@ -525,7 +538,17 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
         const float speed = length(v);
         if (speed > max_speed) v = v * (max_speed / speed);
       }
-      V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+      if (NEXT) {  // the next substep's integrate (k_integrate_to)
+        XP[i] = make_float4(x.x, x.y, x.z, 0.0f);
+        if (x4.w != 0.0f) {
+          f3 g = mk3(gx, gy, gz);
+          if (gscale) g = g * gscale[i];
+          const f3 vn = v + g * h;
+          x = x + vn * h;
+        }
+      } else {
+        V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+      }
     }
     if (stream_out) {  // a frame's positions far exceed the caches: do not keep them
       typedef float f4v __attribute__((ext_vector_type(4)));
@@ -1014,6 +1037,10 @@ int launch_tile_block(int mode, hipStream_t s, int nb, size_t shmem, const float
     XPBD_TILE_MODE(TM_FINALIZE | TM_ANCHOR)
     XPBD_TILE_MODE(TM_ACC | TM_ANCHOR)
     XPBD_TILE_MODE(TM_FINALIZE | TM_ACC | TM_ANCHOR)
+    XPBD_TILE_MODE(TM_FINALIZE | TM_NEXT)
+    XPBD_TILE_MODE(TM_FINALIZE | TM_NEXT | TM_ACC)
+    XPBD_TILE_MODE(TM_FINALIZE | TM_NEXT | TM_ANCHOR)
+    XPBD_TILE_MODE(TM_FINALIZE | TM_NEXT | TM_ACC | TM_ANCHOR)
   }
 #undef XPBD_TILE_MODE
   return (int)hipErrorInvalidValue;  // predict never combines with finalize
@@ -1021,6 +1048,7 @@ int launch_tile_block(int mode, hipStream_t s, int nb, size_t shmem, const float
 
 // ANC may be NULL when no row is ATTACH, ACC when no device-counted rows
 // join the iteration.
+// finalize: 0 no, 1 finalize + cap, 2 also the next substep's integrate.
 int launch_tile(bool predict, hipStream_t s, const float* Xin, float* Xout, float* XP, float* V,
                 const int* R, const float* ANC, const float* LAMin, float* LAMout,
                 const int* tile_row_start, const int* halo_start, const int* halo_rows, float h,
@@ -1032,7 +1060,7 @@ int launch_tile(bool predict, hipStream_t s, const float* Xin, float* Xout, floa
   const int nb = blocks_for(n, tile_size);
   const size_t shmem = tile_lds_bytes(tile_size);
   const int mode = (predict ? TM_PREDICT : 0) | (finalize ? TM_FINALIZE : 0) |
-                   (ACC ? TM_ACC : 0) | (ANC ? TM_ANCHOR : 0);
+                   (finalize == 2 ? TM_NEXT : 0) | (ACC ? TM_ACC : 0) | (ANC ? TM_ANCHOR : 0);
   int rc;
   // 512 threads from 64-particle tiles up: a 64-particle tile of the 128^2
   // cloth has ~700 rows (owned + halo), one pass of 512 threads x 2 rows
@@ -1121,8 +1149,20 @@ int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* X
   // the plain launch only reads XP (friction, finalize)
   return launch_tile(false, (hipStream_t)stream, Xin, Xout, const_cast<float*>(XP), V, R, ANC,
                      LAMin, LAMout, tile_row_start, halo_start, halo_rows, h, first, tile_size,
-                     n_particles, relax, finalize, damping, max_speed, ACC, 0.0f, 0.0f, 0.0f,
-                     nullptr);
+                     n_particles, relax, finalize ? 1 : 0, damping, max_speed, ACC, 0.0f, 0.0f,
+                     0.0f, nullptr);
+}
+
+int xpbd_jacobi_tile_next(void* stream, const float* Xin, float* Xout, float* XP, const int* R,
+                          const float* ANC, const float* LAMin, float* LAMout,
+                          const int* tile_row_start, const int* halo_start, const int* halo_rows,
+                          float h, int first, float gx, float gy, float gz, const float* gscale,
+                          int tile_size, int n_particles, float relax, float damping,
+                          float max_speed, float* ACC) {
+  if (Xin == Xout) return (int)hipErrorInvalidValue;  // neighbours read Xin while tiles write
+  return launch_tile(false, (hipStream_t)stream, Xin, Xout, XP, nullptr, R, ANC, LAMin, LAMout,
+                     tile_row_start, halo_start, halo_rows, h, first, tile_size, n_particles,
+                     relax, 2, damping, max_speed, ACC, gx, gy, gz, gscale);
 }
 
 int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP, const float* V,

@@ -265,7 +265,11 @@ class Solver:
         self.jacobi_kernel = "tile"
         self.tile_size = 0            # 0 = chosen from the scene size
         self.reorder = True           # Z-order particle layout for Jacobi
-        self.fuse_first = True        # integrate folded into the first tile launch
+        # integrate folded into the tile launches: True = into the previous
+        # substep's last launch where the rows allow it (no CONTACT rows),
+        # else into the substep's first launch; "first" = always the latter;
+        # False = a separate integrate pass (identical results)
+        self.fuse_first = True
         self.block_frames = "auto"    # small scenes: whole frames in one workgroup
         self._block_active = False
         # device-side CONTACT generation (Jacobi tile mode): an extension
@@ -765,7 +769,7 @@ class Solver:
     def _signature(self):
         return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
                 self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling,
-                self._tile, getattr(self, "_ccap", 0), bool(self.fuse_first), self._ws_gen,
+                self._tile, getattr(self, "_ccap", 0), str(self.fuse_first), self._ws_gen,
                 str(self.block_frames), int(self.warm_start_every))
 
     # "auto" limit of whole-frame workgroups in Jacobi mode: above it the
@@ -817,7 +821,12 @@ class Solver:
             d.tile_size = self._tile
             if self.GS is not None:
                 d.gscale = N.ptr(self.GS)
-            d.fuse_first = int(bool(self.fuse_first))
+            if not self.fuse_first:
+                d.fuse_first = 0
+            elif self.fuse_first == "first" or bool(np.any(self._rt.array() == CONTACT)):
+                d.fuse_first = 1    # CONTACT friction reads other tiles' x_prev
+            else:
+                d.fuse_first = 2
             if self.CROWS is not None:
                 d.dyn_rows, d.dyn_lam = N.ptr(self.CROWS), N.ptr(self.CLAM)
                 d.dyn_count, d.dyn_cap = N.ptr(self.CCOUNT), self._ccap

@@ -118,6 +118,43 @@ def test_warm_launch_matches_oracle():
     assert err <= 1e-6 + 1e-4 * move
 
 
+@pytest.mark.parametrize("predict", [0, 1])
+def test_network_pass_in_64_row_groups_equals_features_then_forward(predict):
+    """On a row set large enough for 64-row groups (96^2 cloth, ~54 k rows,
+    a ragged last group) the network pass gives bit for bit what
+    xpbd_ws_features -> xpbd_mlp_forward gives at the same positions, with
+    and without integrate folded in."""
+    import torch
+
+    from xpbd_amd import _native as N
+    from xpbd_amd import warmstart as ws
+
+    s = _cloth(96)
+    ws._prepare(s)
+    _perturbed_state(s, 3, below_floor=True)
+    m, n = s.num_constraints, s.num_bodies
+    assert m >= 64 * 256 and m % 64 != 0
+    h = float(np.float32(s.dt / s.substeps))
+    theta = _theta(11, b3=0.5)
+    img = torch.empty(int(N.lib().xpbd_mlp_image_bytes()), dtype=torch.uint8, device=DEV)
+    N.check(N.lib().xpbd_mlp_pack(None, N.ptr(theta), N.ptr(img)))
+    Y = torch.full((m + 7,), 7.0, device=DEV)
+    N.check(N.lib().xpbd_ws_net(None, N.ptr(s.X), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R),
+                                N.ptr(s.ANC), N.ptr(img), N.ptr(Y), h, *s.gravity, None, m,
+                                predict))
+    X, XP = s.X, s.XP
+    if predict:   # features at the integrated positions
+        X, XP, V = s.X.clone(), torch.empty_like(s.XP), s.V.clone()
+        N.check(N.lib().xpbd_integrate_to(None, N.ptr(s.X), N.ptr(X), N.ptr(V), N.ptr(XP),
+                                          *s.gravity, h, None, n))
+    F = torch.empty((m, 12), device=DEV)
+    N.check(N.lib().xpbd_ws_features(None, N.ptr(X), N.ptr(XP), N.ptr(s.R), N.ptr(s.ANC), h,
+                                     N.ptr(F), m))
+    Yf = ws.predict(theta, F)
+    assert torch.equal(Y[:m], Yf)
+    assert bool((Y[m:] == 7.0).all())            # nothing written past the rows
+
+
 def test_zero_start_is_the_plain_solver():
     """With the network's output pushed to -inf (omega = 0) the warm-start
     launch leaves the predicted positions as they are, writes zero

@@ -60,24 +60,29 @@ __global__ __launch_bounds__(kBlk) void k_mlp_pack(const float* __restrict__ the
   }
 }
 
+// Inference over the rows of F (forward16_t, the order of the network pass
+// of a substep, so both give the same bits for the same features).
 __global__ __launch_bounds__(kBlk) void k_mlp_forward(const float* __restrict__ theta,
                                                       const float* __restrict__ F,
                                                       float* __restrict__ Y, int m) {
-  __shared__ Lds s;
-  load_params(s, theta, false);
+  __shared__ Weights w;
+  load_weights(w, theta, threadIdx.x, kBlk);
+  __syncthreads();
+  RegWeights rw;
+  load_reg_weights(rw, w);
   const int l = threadIdx.x & 63, r16 = l & 15, g = l >> 4, wv = threadIdx.x >> 6;
   const float b3 = theta[P_B3];
   const int ntiles = (m + 15) / 16;
   for (int t = blockIdx.x * kWaves + wv; t < ntiles; t += gridDim.x * kWaves) {
-    f32x4 z1[4], z2[4];
-    float y[4];
-    bf16x4 xa;
-    forward_tile(s, F, t * 16, m, b3, z1, z2, y, xa);
-    if (r16 == 0)
-      for (int j = 0; j < 4; ++j) {
-        const int row = t * 16 + 4 * g + j;
-        if (row < m) Y[row] = y[j];
-      }
+    const int row = t * 16 + r16;
+    bf16x4 xb;
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * g + j;
+      xb[j] = (short)((row < m && k < NIN) ? to_bf16(F[(size_t)row * NIN + k])
+                                           : (unsigned short)0);
+    }
+    const float y = forward16_t(rw, w.b1, w.b2, w.w3, xb, b3);
+    if (g == 0 && row < m) Y[row] = y;
   }
 }
 

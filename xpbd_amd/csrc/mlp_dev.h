@@ -13,6 +13,12 @@
 //   A (M x K):  lane l holds A[l % 16][4 (l / 16) + j], j = 0..3
 //   B (K x N):  lane l holds B[4 (l / 16) + j][l % 16]
 //   D (M x N):  lane l holds D[4 (l / 16) + j][l % 16]
+//
+// Two evaluation orders share these weights: forward16 (rows as M, used by
+// the training kernels, whose backward pass works in that layout) and
+// forward16_t (the transposed products Z^T = W^T X^T, used for inference:
+// a layer's output tile lands in the lanes exactly as the next layer's B
+// operand, so no LDS transposition; the weight operands stay in registers).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,10 +33,10 @@ static_assert(NPARAM == 5057, "parameter count");
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// float -> bfloat16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950;
+// for finite values the same bits as adding 0x7FFF + lsb to the pattern)
 __device__ inline unsigned short to_bf16(float f) {
-  unsigned u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
+  return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 __device__ inline float from_bf16(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
@@ -138,6 +144,56 @@ __device__ inline void forward16(const Weights& s, unsigned short (*tile)[NH], F
     y[j] = p + b3;
   }
   __builtin_amdgcn_wave_barrier();
+}
+
+// Inference weights held in registers for a wave's whole life: the W1^T and
+// W2^T tiles as MFMA A operands (40 VGPRs).  Lane (r, g) = (l % 16, l / 16)
+// holds a1[n][j] = W1[4g + j][16n + r] and a2[n][k][j] = W2[16k + 4g + j][16n + r].
+struct RegWeights {
+  bf16x4 a1[4];
+  bf16x4 a2[4][4];
+};
+
+__device__ inline void load_reg_weights(RegWeights& w, const Weights& s) {
+  const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
+  for (int n = 0; n < 4; ++n) {
+    w.a1[n] = *(const bf16x4*)&s.w1t[16 * n + r][4 * g];
+    for (int k = 0; k < 4; ++k) w.a2[n][k] = *(const bf16x4*)&s.w2t[16 * n + r][16 * k + 4 * g];
+  }
+}
+
+// Forward of one 16-row tile in the transposed order.  xb is the layer-1 B
+// operand: lane (r, g) holds the bfloat16 features 4g .. 4g+3 of tile row r
+// (zero beyond feature 11).  b1, b2, w3 point at the float vectors (LDS).
+// Returns y of row r in the four lanes (r, 0..3): layer 3 sums, per lane,
+// gelu(z2) w3 over its 16 hidden units (n outer, j inner), then over the
+// four lanes of the row ((g0 + g1) + (g2 + g3)), then adds b3.
+__device__ inline float forward16_t(const RegWeights& w, const float* b1, const float* b2,
+                                   const float* w3, bf16x4 xb, float b3) {
+  const int l = threadIdx.x & 63, g = l >> 4;
+  bf16x4 h[4];
+  for (int n = 0; n < 4; ++n) {  // Z1^T tile n: lane holds Z1[r][16n + 4g + j]
+    const f32x4 z = mfma(w.a1[n], xb, f32x4{0.f, 0.f, 0.f, 0.f});
+    const float4 b = *(const float4*)&b1[16 * n + 4 * g];
+    h[n][0] = (short)to_bf16(gelu(z[0] + b.x));
+    h[n][1] = (short)to_bf16(gelu(z[1] + b.y));
+    h[n][2] = (short)to_bf16(gelu(z[2] + b.z));
+    h[n][3] = (short)to_bf16(gelu(z[3] + b.w));
+  }
+  float p = 0.0f;
+  for (int n = 0; n < 4; ++n) {  // Z2^T tile n = sum_k W2^T(n, k) H1^T(k)
+    f32x4 acc{0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < 4; ++k) acc = mfma(w.a2[n][k], h[k], acc);
+    const float4 b = *(const float4*)&b2[16 * n + 4 * g];
+    const float4 c = *(const float4*)&w3[16 * n + 4 * g];
+    p += gelu(acc[0] + b.x) * c.x;
+    p += gelu(acc[1] + b.y) * c.y;
+    p += gelu(acc[2] + b.z) * c.z;
+    p += gelu(acc[3] + b.w) * c.w;
+  }
+  p += __shfl_xor(p, 16, 64);
+  p += __shfl_xor(p, 32, 64);
+  return p + b3;
 }
 
 }  // namespace xpbd_mlp

@@ -88,51 +88,60 @@ __global__ __launch_bounds__(kB) void k_ws_features(const float4* __restrict__ X
   o[2] = make_float4(f[8], f[9], f[10], f[11]);
 }
 
-// Network pass of a substep: every wave takes 16 consecutive rows (one
-// MFMA tile: many waves per SIMD hide the latency of the GELU and MFMA
-// chains), lanes 0..15 compute their features (as k_ws_features, at the
-// predicted positions when PREDICT: x~ = x + (v + g h) h from Xin and V,
-// x_prev = Xin, the arithmetic of integrate) into LDS, the wave evaluates
-// the MLP and writes y per row.
-constexpr int kNetWaves = 4, kNetBlock = 64 * kNetWaves;
+// Network pass of a substep: every wave takes ROWS consecutive rows; lanes
+// 0 .. ROWS-1 compute their features (as k_ws_features, at the predicted
+// positions when PREDICT: x~ = x + (v + g h) h from Xin and V, x_prev = Xin,
+// the arithmetic of integrate) into LDS as bfloat16, then the wave runs
+// forward16_t on each 16-row tile and writes y per row.  The weight operands
+// live in registers (loaded once per wave from the packed image), the
+// biases in LDS; waves loop over row groups.  ROWS = 64 (one feature pass
+// of all 64 lanes per four tiles) unless the row set is too small to give
+// every CU a wave, then 16.
+constexpr int kNetWaves = 4, kNetBlock = 64 * kNetWaves, kNetMaxBlocks = 2048;
 
+template <int ROWS>
 struct NetLds {
-  xpbd_mlp::Weights w;
-  unsigned short ftile[kNetWaves][16][16];
-  unsigned short htile[kNetWaves][16][xpbd_mlp::NH];
+  float b1[xpbd_mlp::NH], b2[xpbd_mlp::NH], w3[xpbd_mlp::NH];
+  unsigned short ftile[kNetWaves][ROWS][16];
 };
 
-template <bool PREDICT>
+template <bool PREDICT, int ROWS>
 __global__ __launch_bounds__(kNetBlock) void k_ws_net(
     const float4* __restrict__ Xin, const float4* __restrict__ XP, const float4* __restrict__ V,
     const int4* __restrict__ R, const float4* __restrict__ ANC,
     const xpbd_mlp::Image* __restrict__ img, float* __restrict__ Y, float h, float gx, float gy,
     float gz, const float* __restrict__ gscale, int m) {
   using namespace xpbd_mlp;
-  __shared__ NetLds s;
-  load_image(s.w, img, threadIdx.x, kNetBlock);
-  __syncthreads();
+  __shared__ NetLds<ROWS> s;
+  for (int i = threadIdx.x; i < NH; i += kNetBlock) {
+    s.b1[i] = img->w.b1[i];
+    s.b2[i] = img->w.b2[i];
+    s.w3[i] = img->w.w3[i];
+  }
+  RegWeights rw;
+  load_reg_weights(rw, img->w);
   const float b3 = img->b3;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r16 = lane & 15, g = lane >> 4;
   auto pos = [&](int i) -> float4 {
     const float4 x4 = Xin[i];
     if (!PREDICT || x4.w == 0.0f) return x4;
-    f3 g = mk3(gx, gy, gz);
-    if (gscale) g = g * gscale[i];
-    const f3 v = xyz(V[i]) + g * h;
+    f3 gr = mk3(gx, gy, gz);
+    if (gscale) gr = gr * gscale[i];
+    const f3 v = xyz(V[i]) + gr * h;
     const f3 x = xyz(x4) + v * h;
     return make_float4(x.x, x.y, x.z, x4.w);
   };
   auto prev = [&](int i) { return PREDICT ? xyz(Xin[i]) : xyz(XP[i]); };
-  const int chunks = (m + 15) / 16;
-  for (int c = blockIdx.x * kNetWaves + wv; c < chunks; c += gridDim.x * kNetWaves) {
-    if (lane < 16) {
-      const int j = c * 16 + lane;
+  const int groups = (m + ROWS - 1) / ROWS;
+  for (int c = blockIdx.x * kNetWaves + wv; c < groups; c += gridDim.x * kNetWaves) {
+    if (lane < ROWS) {
+      const int j = c * ROWS + lane;
       float f[12];
       for (int q = 0; q < 12; ++q) f[q] = 0.0f;
       if (j < m) {
-        const RowGeo g = row_geo<true>(pos, prev, ANC, R[j], j, h);
-        row_features(g.t, g.C, g.w, g.alpha, g.vn, g.vt, g.pen, h, f);
+        const RowGeo rg = row_geo<true>(pos, prev, ANC, R[j], j, h);
+        row_features(rg.t, rg.C, rg.w, rg.alpha, rg.vn, rg.vt, rg.pen, h, f);
       }
       unsigned pk[8];
       for (int q = 0; q < 6; ++q)
@@ -144,18 +153,32 @@ __global__ __launch_bounds__(kNetBlock) void k_ws_net(
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
-    f32x4 z1[4], z2[4];
-    float y[4];
-    bf16x4 xa;
-    auto feat = [&](int r, int q) -> unsigned short { return s.ftile[wv][r][q]; };
-    forward16(s.w, s.htile[wv], feat, b3, z1, z2, y, xa);
-    if ((lane & 15) == 0)
-      for (int q = 0; q < 4; ++q) {
-        const int j = c * 16 + 4 * (lane >> 4) + q;
-        if (j < m) Y[j] = y[q];
-      }
+    for (int t = 0; t < ROWS / 16; ++t) {
+      const int j = c * ROWS + 16 * t + r16;
+      if (c * ROWS + 16 * t >= m) break;
+      const bf16x4 xb = *(const bf16x4*)&s.ftile[wv][16 * t + r16][4 * g];
+      const float y = forward16_t(rw, s.b1, s.b2, s.w3, xb, b3);
+      if (g == 0 && j < m) Y[j] = y;
+    }
     __builtin_amdgcn_wave_barrier();
   }
+}
+
+template <int ROWS>
+void launch_net(hipStream_t s, const float* Xin, const float* XP, const float* V, const int* R,
+                const float* ANC, const xpbd_mlp::Image* img, float* Y, float h, float gx,
+                float gy, float gz, const float* gscale, int n_rows, int predict) {
+  const int groups = (n_rows + ROWS - 1) / ROWS;
+  int nb = (groups + kNetWaves - 1) / kNetWaves;
+  if (nb > kNetMaxBlocks) nb = kNetMaxBlocks;
+  if (predict)
+    hipLaunchKernelGGL((k_ws_net<true, ROWS>), dim3(nb), dim3(kNetBlock), 0, s,
+                       (const float4*)Xin, (const float4*)XP, (const float4*)V, (const int4*)R,
+                       (const float4*)ANC, img, Y, h, gx, gy, gz, gscale, n_rows);
+  else
+    hipLaunchKernelGGL((k_ws_net<false, ROWS>), dim3(nb), dim3(kNetBlock), 0, s,
+                       (const float4*)Xin, (const float4*)XP, (const float4*)V, (const int4*)R,
+                       (const float4*)ANC, img, Y, h, gx, gy, gz, gscale, n_rows);
 }
 
 // Residual: sum over rows of r^2, r = C + alpha~ lambda (two-sided rows),
@@ -208,18 +231,13 @@ int xpbd_ws_net(void* stream, const float* Xin, const float* XP, const float* V,
   if (n_rows <= 0) return 0;
   if (!image || !Y) return (int)hipErrorInvalidValue;
   const xpbd_mlp::Image* img = (const xpbd_mlp::Image*)image;
-  const int chunks = (n_rows + 15) / 16;
-  int nb = (chunks + kNetWaves - 1) / kNetWaves;
-  if (nb > 2048) nb = 2048;
   hipStream_t s = (hipStream_t)stream;
-  if (predict)
-    hipLaunchKernelGGL(k_ws_net<true>, dim3(nb), dim3(kNetBlock), 0, s, (const float4*)Xin,
-                       (const float4*)XP, (const float4*)V, (const int4*)R, (const float4*)ANC,
-                       img, Y, h, gx, gy, gz, gscale, n_rows);
+  // 64-row groups from 256 groups up (measured on cloths: 113 k rows 14.5
+  // against 21.2 us with 16-row groups, 1.8 M rows 110 against 147 us)
+  if (n_rows >= 64 * 256)
+    launch_net<64>(s, Xin, XP, V, R, ANC, img, Y, h, gx, gy, gz, gscale, n_rows, predict);
   else
-    hipLaunchKernelGGL(k_ws_net<false>, dim3(nb), dim3(kNetBlock), 0, s, (const float4*)Xin,
-                       (const float4*)XP, (const float4*)V, (const int4*)R, (const float4*)ANC,
-                       img, Y, h, gx, gy, gz, gscale, n_rows);
+    launch_net<16>(s, Xin, XP, V, R, ANC, img, Y, h, gx, gy, gz, gscale, n_rows, predict);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

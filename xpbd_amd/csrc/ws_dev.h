@@ -22,20 +22,26 @@ namespace xpbd_ws {
 constexpr float kLen = 1.0e-3f;
 constexpr float kOmegaMax = 4.0f;
 
-__device__ inline float slog(float x) { return copysignf(log1pf(fabsf(x)), x); }
+// Logarithms from the hardware log2 (v_log_f32, inputs >= 1e-30 here, all
+// normal): features are rounded to bfloat16 before use, far coarser than
+// the ~1e-7 relative error of log2(1 + |x|) ln 2.
+constexpr float kLn2 = 0.69314718055994531f, kLog10of2 = 0.30102999566398120f;
+__device__ inline float slog(float x) {
+  return copysignf(__builtin_amdgcn_logf(1.0f + fabsf(x)) * kLn2, x);
+}
 
 __device__ inline void row_features(int t, float C, float w, float alpha, float vn, float vt,
                                     float pen, float h, float f[12]) {
   const float s = w + alpha;
   for (int k = 0; k < 4; ++k) f[k] = (t == k) ? 1.0f : 0.0f;
   f[4] = slog(C / kLen);
-  f[5] = log10f(fmaxf(w, 1e-30f)) * 0.25f;
+  f[5] = __builtin_amdgcn_logf(fmaxf(w, 1e-30f)) * (kLog10of2 * 0.25f);
   f[6] = s > 0.0f ? alpha / s : 0.0f;
   f[7] = 0.0f;
   f[8] = slog(vn * h / kLen);
   f[9] = slog(vt * h / kLen);
   f[10] = slog(pen / kLen);
-  f[11] = log10f(1.0f + alpha) * 0.125f;
+  f[11] = __builtin_amdgcn_logf(1.0f + alpha) * (kLog10of2 * 0.125f);
 }
 
 // lambda0 of a row from the network output y; `step` is the row's first

@@ -294,8 +294,10 @@ def reference_rows(frames: int = 120, warmup: int = 20) -> dict:
 def warm_start_section(res: int, frames: int) -> dict:
     """Learned lambda warm start on the cloth: fit (two rounds, on the GPU),
     RMS residual of each start on its own trajectory, and whole-frame
-    substeps/s of the captured frame plans (tile kernel; the learned start
-    runs inside each substep's first launch), each plan autotuned."""
+    substeps/s of the captured frame plans (tile kernel; each substep starts
+    with the warm-start launch, after the network pass on every substep or,
+    "net_per_frame", on the first substep of each frame only, with a network
+    trained for that use), each plan autotuned."""
     import torch
 
     from xpbd_amd import Solver
@@ -312,12 +314,14 @@ def warm_start_section(res: int, frames: int) -> dict:
     theta = ws.fit(cloth, substeps=120, settle_frames=2, steps=800, log=log)
     torch.cuda.synchronize()
     fit_s = time.perf_counter() - t0
+    theta_frame = ws.fit(cloth, substeps=120, settle_frames=2, steps=800, log=log, net_every=15,
+                         pair_with_net=True)
     s = cloth()
     s.step(20)
     resid, fallback = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(1, 2, 4),
                                                  learned_iters=(1, 2))
-    amort, _ = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(), learned_iters=(1, 2),
-                                          net_every=15)
+    amort, _ = ws.Evaluator(s, theta_frame).run(substeps=60, plain_iters=(),
+                                                learned_iters=(1, 2), net_every=15)
     resid.update({f"{k}_net_per_frame": v for k, v in amort.items()})
     sps, shapes = {}, {}
     for name, iters, learned, every in [("plain_4", 4, False, 1), ("plain_2", 2, False, 1),
@@ -329,7 +333,7 @@ def warm_start_section(res: int, frames: int) -> dict:
         c.block_frames = False
         if learned:
             c.warm_start_every = every
-            c.set_warm_start(theta)
+            c.set_warm_start(theta if every == 1 else theta_frame)
         best, _ = autotune.tune(c)
         shapes[name] = list(best) if best else None
         ms = time_frames(c.step, frames, 5)
@@ -337,6 +341,7 @@ def warm_start_section(res: int, frames: int) -> dict:
     return {
         "scene": f"cloth {res}x{res}, 15 substeps, Jacobi tile kernel",
         "fit_seconds": round(fit_s, 2), "fit": log[0] if log else None,
+        "fit_net_per_frame": log[1] if len(log) > 1 else None,
         "rms_residual": {k: float(f"{v:.4e}") for k, v in resid.items()},
         "fallback_substeps_of_60": fallback,
         "substeps_per_s": sps,

@@ -208,7 +208,8 @@ TARGET_SCALE = 0.5
 
 
 def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int | None = None,
-            theta=None, target_scale: float = TARGET_SCALE, net_every: int = 1):
+            theta=None, target_scale: float = TARGET_SCALE, net_every: int = 1,
+            pair_with_net: bool = False):
     """Run the solver for ``substeps`` substeps; every ``every``-th substep
     append (features, target) for all rows.  Returns (F, T) device tensors
     of shape (k m, 12) and (k m,).
@@ -216,7 +217,10 @@ def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int
     ``theta`` None: the trajectory is the plain solver's; ``theta`` given: the
     trajectory runs the learned start (an on-policy round, whose states are
     the ones the deployed network meets), with the network pass on every
-    ``net_every``-th substep of each frame as deployed."""
+    ``net_every``-th substep of each frame as deployed.  ``pair_with_net``:
+    each target is paired with the features of the last substep on which
+    the network runs (the outputs the deployed start reuses until its next
+    network pass) instead of its own substep's."""
     import torch
 
     e = _Eager(s)
@@ -224,27 +228,33 @@ def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int
     scale = row_scale(s)
     F_all, T_all = [], []
     F = torch.empty((e.m, 12), device=s.device)
+    Fnet = torch.empty_like(F)
     th = None if theta is None else torch.as_tensor(theta, dtype=torch.float32).to(s.device)
     for k in range(substeps):
         e.integrate()
+        net = (k % s.substeps) % max(1, net_every) == 0
+        if pair_with_net and net:
+            e.features(Fnet)
         if k % every == 0:
             e.features(F)
             Xc, Xc2 = s.X.clone(), s.X.clone()
             Lc, Lc2 = torch.zeros_like(s.LAM), torch.zeros_like(s.LAM)
             e.iterate(target_iters, finalize=False, X=Xc, X2=Xc2, L=Lc, L2=Lc2)
-            F_all.append(F.clone())
+            F_all.append((Fnet if pair_with_net else F).clone())
             T_all.append(target_logit(F, Lc[:e.m], scale, float(target_scale)))
-        e.iterate(iters, theta=th, net=(k % s.substeps) % max(1, net_every) == 0)
+        e.iterate(iters, theta=th, net=net)
     return torch.cat(F_all), torch.cat(T_all)
 
 
 def fit(make_scene, substeps: int = 120, settle_frames: int = 2, steps: int = 800,
         batch: int = 16384, seed: int = 0, log: list | None = None,
-        onpolicy_iters=(1, 2), net_every: int = 1):
+        onpolicy_iters=(1, 2), net_every: int = 1, pair_with_net: bool = False):
     """Two rounds: data from the plain solver's trajectory, training, data
     from the learned start's own trajectories (one per iteration count in
     ``onpolicy_iters``: the states a start followed by one iteration meets
-    differ from those after two), training on all of it.
+    differ from those after two), training on all of it.  ``net_every`` and
+    ``pair_with_net`` (see ``collect``) train a network for a start whose
+    network pass runs on every ``net_every``-th substep only.
     ``make_scene()`` returns a fresh Jacobi ``Solver``; returns theta."""
     import torch
 
@@ -253,12 +263,13 @@ def fit(make_scene, substeps: int = 120, settle_frames: int = 2, steps: int = 80
         s.step(settle_frames)
         return s
 
-    F, T = collect(fresh(), substeps)
+    F, T = collect(fresh(), substeps, net_every=net_every, pair_with_net=pair_with_net)
     theta, h1 = train(F, T, steps=steps, batch=batch, lr=3e-3, seed=seed, log_every=steps)
     Fs, Ts = [F], [T]
     ok = True
     for k in onpolicy_iters:
-        F2, T2 = collect(fresh(), substeps, theta=theta, iters=k, net_every=net_every)
+        F2, T2 = collect(fresh(), substeps, theta=theta, iters=k, net_every=net_every,
+                         pair_with_net=pair_with_net)
         if bool(torch.isfinite(F2).all() and torch.isfinite(T2).all()):
             Fs.append(F2)
             Ts.append(T2)

@@ -42,14 +42,17 @@ __device__ inline float from_bf16(unsigned short b) { return __uint_as_float((un
 
 // GELU, tanh form (Hendrycks & Gimpel 2016): 0.5 z (1 + tanh(sqrt(2/pi) (z +
 // 0.044715 z^3))) = z sigmoid(u), u = 2 sqrt(2/pi) z (1 + 0.044715 z^2);
-// sigmoid from the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32):
-// about 7 VALU operations instead of the ~40 of an erf.
+// sigmoid from the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32) of
+// -u log2(e) = z (k0 + k1 z^2) with the constants folded: five VALU
+// operations and two transcendentals instead of the ~40 of an erf.
 constexpr float kGeluC0 = 1.5957691216057308f;  // 2 sqrt(2 / pi)
 constexpr float kGeluC1 = 0.044715f;
 constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kGeluK0 = (float)(-1.4426950408889634 * 1.5957691216057308);
+constexpr float kGeluK1 = (float)(-1.4426950408889634 * 1.5957691216057308 * 0.044715);
 __device__ inline float gelu_sigmoid(float z) {
-  const float u = kGeluC0 * z * fmaf(kGeluC1, z * z, 1.0f);
-  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u * -kLog2e));
+  const float v = z * fmaf(kGeluK1, z * z, kGeluK0);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(v));
 }
 __device__ inline float gelu(float z) { return z * gelu_sigmoid(z); }
 __device__ inline float gelu_grad(float z) {
@@ -165,9 +168,10 @@ __device__ inline void load_reg_weights(RegWeights& w, const Weights& s) {
 // Forward of one 16-row tile in the transposed order.  xb is the layer-1 B
 // operand: lane (r, g) holds the bfloat16 features 4g .. 4g+3 of tile row r
 // (zero beyond feature 11).  b1, b2, w3 point at the float vectors (LDS).
-// Returns y of row r in the four lanes (r, 0..3): layer 3 sums, per lane,
-// gelu(z2) w3 over its 16 hidden units (n outer, j inner), then over the
-// four lanes of the row ((g0 + g1) + (g2 + g3)), then adds b3.
+// Returns y of row r in the four lanes (r, 0..3): layer 3 accumulates, per
+// lane, gelu(z2) w3 over its 16 hidden units with fused multiply-adds (n
+// outer, j inner), then sums the four lanes of the row ((g0 + g1) + (g2 +
+// g3)) and adds b3.
 __device__ inline float forward16_t(const RegWeights& w, const float* b1, const float* b2,
                                    const float* w3, bf16x4 xb, float b3) {
   const int l = threadIdx.x & 63, g = l >> 4;
@@ -186,10 +190,10 @@ __device__ inline float forward16_t(const RegWeights& w, const float* b1, const 
     for (int k = 0; k < 4; ++k) acc = mfma(w.a2[n][k], h[k], acc);
     const float4 b = *(const float4*)&b2[16 * n + 4 * g];
     const float4 c = *(const float4*)&w3[16 * n + 4 * g];
-    p += gelu(acc[0] + b.x) * c.x;
-    p += gelu(acc[1] + b.y) * c.y;
-    p += gelu(acc[2] + b.z) * c.z;
-    p += gelu(acc[3] + b.w) * c.w;
+    p = fmaf(gelu(acc[0] + b.x), c.x, p);
+    p = fmaf(gelu(acc[1] + b.y), c.y, p);
+    p = fmaf(gelu(acc[2] + b.z), c.z, p);
+    p = fmaf(gelu(acc[3] + b.w), c.w, p);
   }
   p += __shfl_xor(p, 16, 64);
   p += __shfl_xor(p, 32, 64);

@@ -47,7 +47,9 @@ __device__ inline void row_features(int t, float C, float w, float alpha, float 
 // lambda0 of a row from the network output y; `step` is the row's first
 // plain step -C / (w + alpha~) (zero when the row is inactive).
 __device__ inline float start_lambda(float y, float step) {
-  const float lam0 = kOmegaMax / (1.0f + expf(-y)) * step;
+  constexpr float kLog2e = 1.4426950408889634f;
+  const float s = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y * -kLog2e));
+  const float lam0 = kOmegaMax * s * step;
   return isfinite(lam0) ? lam0 : 0.0f;
 }
 

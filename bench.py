@@ -361,6 +361,74 @@ def warm_start_section(res: int, frames: int) -> dict:
     }
 
 
+def warm_start_batched_section(frames: int) -> dict:
+    """Learned lambda warm start on the particle solver of the batched world
+    (1024 uneven scenes: chains, cloths, sphere piles with device contact
+    rows; per-scene gravity, compliance and friction): a network for one
+    pass per frame is fitted on the world itself (two rounds, on the GPU),
+    then each start runs on its own trajectory from the same state (RMS
+    residual over the static rows, 60 substeps) and whole frames are timed
+    through the captured frame plan."""
+    import torch
+
+    from xpbd_amd import warmstart as ws
+    from xpbd_amd.batch import BatchedWorld, draw
+
+    w = BatchedWorld(draw(1024, 0), device="cuda:0")
+    s = w.particles
+    s.block_frames = False
+    s.step(2)
+    torch.cuda.synchronize()
+    x0, v0 = s.positions(), s.velocities()     # API order: survives a re-layout
+
+    def restore():
+        if s._ws_theta is not None:
+            s.set_warm_start(None)
+        s.iterations = 2
+        s.set_state(x0, v0)
+        return s
+
+    t0 = time.perf_counter()
+    log = []
+    theta = ws.fit(restore, substeps=60, settle_frames=1, steps=800, log=log, net_every=15,
+                   pair_with_net=True, every=5)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    restore()
+    s.step(4)
+    resid, fallback = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(1, 2, 4),
+                                                 learned_iters=(1, 2), net_every=15)
+    torch.cuda.empty_cache()
+    sps = {}
+    for name, iters, learned in [("plain_4", 4, False), ("plain_2", 2, False),
+                                 ("plain_1", 1, False), ("learned_2_net_per_frame", 2, True),
+                                 ("learned_1_net_per_frame", 1, True)]:
+        restore()
+        s.iterations = iters
+        if learned:
+            s.warm_start_every = 15
+            s.set_warm_start(theta)
+        ms = time_frames(s.step, frames, 3)
+        sps[name] = round(s.substeps / (ms * 1e-3), 1)
+    restore()
+    resid = {(f"{k}_net_per_frame" if k.startswith("learned") else k): v for k, v in resid.items()}
+    out = {
+        "scene": f"batched world, particle solver: {s.num_bodies} particles, "
+                 f"{s.num_constraints} static rows + device CONTACT rows, 1024 scenes, "
+                 "15 substeps, Jacobi tile kernel",
+        "fit_seconds": round(fit_s, 2), "fit": log[0] if log else None,
+        "rms_residual_static_rows": {k: float(f"{v:.4e}") for k, v in resid.items()},
+        "fallback_substeps_of_60": fallback,
+        "substeps_per_s": sps,
+    }
+    for k in (1, 2):
+        out[f"learned{k}_net_per_frame_vs_plain4"] = {
+            "residual_ratio": round(resid[f"learned_{k}_net_per_frame"] / resid["plain_4"], 3),
+            "speedup": round(sps[f"learned_{k}_net_per_frame"] / sps["plain_4"], 3)}
+    del w, s
+    return out
+
+
 def load_traffic(kernel: str):
     """HBM bytes per launch of ``kernel`` measured by rocprofv3 --pmc runs
     (FETCH_SIZE and WRITE_SIZE scaled by a plain copy of known size),
@@ -492,6 +560,12 @@ def main() -> int:
         print(f"[bench] warm start: {json.dumps(warm)}", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 
+    warm_b = None
+    if not args.no_warm_start and args.mode == "jacobi":
+        warm_b = warm_start_batched_section(max(args.steps // 10, 10))
+        print(f"[bench] warm start batched: {json.dumps(warm_b)}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(host, s, args.cpu_budget)
@@ -513,6 +587,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "other_workloads": others,
         "learned_warm_start": warm,
+        "learned_warm_start_batched": warm_b,
         "reference_benchmark_rows": ref_rows,
     }
     print(json.dumps(line), flush=True)

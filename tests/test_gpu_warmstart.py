@@ -353,3 +353,46 @@ def test_ws_residual_matches_numpy_oracle():
                                      N.ptr(out), m))
     r_ref = ws_ref.residual_sum(X, s.R[:m].cpu().numpy(), s.ANC[:m].cpu().numpy(), lam, h)
     assert np.isclose(float(out.item()), r_ref, rtol=1e-4)
+
+
+@pytest.mark.parametrize("fuse", [True, False])
+def test_warm_start_with_device_contact_rows_matches_eager(trained, fuse):
+    """Particle scenes of a small batched world (cloths, chains, sphere
+    piles whose CONTACT rows are generated on the device each frame, gravity
+    scaled per scene): two frames of the captured plan with the learned
+    start (network once per frame; integrate folded in or not) agree with
+    the eager per-substep launches (contacts per frame, integrate, network
+    pass, warm-start launch, device-row projection before every iteration)
+    to float-atomic rounding, and the piles stay above the floor."""
+    import torch
+
+    from xpbd_amd import warmstart as ws
+    from xpbd_amd.batch import BatchedWorld, draw
+
+    theta, _ = trained
+    spec = draw(12, 3, mix={"cloth": 0.4, "pile": 0.4, "chain": 0.2})
+    assert any(sc.kind == "pile" for sc in spec.scenes)
+    a = BatchedWorld(spec, device=DEV).particles
+    b = BatchedWorld(spec, device=DEV).particles
+    for s in (a, b):
+        s.fuse_first = fuse
+        s.block_frames = False
+        s.step(3)
+    assert a.CROWS is not None and a.n_contacts > 0
+    a.warm_start_every = 15
+    a.set_warm_start(theta)
+    a.step(2)
+    e = ws._Eager(b)
+    th = torch.as_tensor(theta).to(DEV)
+    for k in range(2 * b.substeps):
+        if k % b.substeps == 0:
+            e.contacts()
+        e.integrate()
+        e.iterate(b.iterations, theta=th, net=(k % b.substeps) == 0)
+    xa, xb = a.positions(), b.positions()
+    assert np.all(np.isfinite(xa))
+    assert np.abs(xa - xb).max() <= 1e-4
+    assert np.abs(a.velocities() - b.velocities()).max() <= 5e-3
+    piles = [sc for sc in spec.scenes if sc.kind == "pile"]
+    for sc in piles:
+        assert xa[sc.first:sc.first + sc.count, 1].min() > 0.05 - 0.01

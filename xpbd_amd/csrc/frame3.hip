@@ -68,8 +68,7 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
   float* lams[2] = {d.lam, d.lam_alt};
   int rc;
   const bool next = d.fuse_first >= 2 && it_n >= 1;
-  if (d.fuse_first && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1) || next) &&
-      !(dyn && warm)) {
+  if (d.fuse_first && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1) || next)) {
     int loc = 0;
     float* acc = dyn ? d.acc : nullptr;
     for (int sub = 0; sub < d.substeps; ++sub) {
@@ -112,9 +111,9 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
       }
       for (; it < it_n; ++it) {
         const int first = (!warm && it == 0) ? 1 : 0;
-        if (dyn) {
+        if (dyn) {  // device rows start from zero (the warm start covers `rows` only)
           rc = xpbd_solve_jacobi_dyn(s, bufs[loc], d.x_prev, d.dyn_rows, d.dyn_lam, d.acc, d.h,
-                                     first, d.dyn_count, d.dyn_cap);
+                                     it == 0 ? 1 : 0, d.dyn_count, d.dyn_cap);
           if (rc) return rc;
         }
         if (it + 1 == it_n && integrate_next) {

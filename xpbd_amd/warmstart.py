@@ -106,13 +106,13 @@ def row_scale(s) -> "torch.Tensor":
 
 
 class _Eager:
-    """Per-substep tile launches through the C-ABI (scenes without
-    device-generated contact rows)."""
+    """Per-substep tile launches through the C-ABI.  Device-generated
+    contact rows (``enable_device_contacts``) are projected before every
+    tile launch from a zero start, as in the frame plan; ``contacts()``
+    regenerates them (the frame plan does so once per frame)."""
 
     def __init__(self, s):
         _prepare(s)
-        if s.CROWS is not None:
-            raise ValueError("the eager warm-start driver does not run device contact rows")
         self.s = s
         self.lib = N.lib()
         self.h = float(np.float32(s.dt / s.substeps))
@@ -129,6 +129,11 @@ class _Eager:
                                     device=self.s.device)
         N.check(self.lib.xpbd_mlp_pack(N.current_stream(), N.ptr(theta), N.ptr(self._img)))
         return self._img
+
+    def contacts(self):
+        """This frame's device contact rows (no-op without them)."""
+        if self.s.CROWS is not None:
+            self.s._device_contact_pass()
 
     def integrate(self):
         s = self.s
@@ -164,12 +169,18 @@ class _Eager:
                 s.jacobi_relax, 0))
             src, dst, ls, ld = dst, src, ld, ls
         anc = N.ptr(s.ANC) if s._has_attach else None   # the variant without anchor loads
+        dyn = s.CROWS is not None
+        acc = N.ptr(s.ACC) if dyn else None
         for it in range(iters):
             fin = int(finalize and it + 1 == iters)
+            if dyn:
+                N.check(self.lib.xpbd_solve_jacobi_dyn(
+                    st, N.ptr(src), N.ptr(s.XP), N.ptr(s.CROWS), N.ptr(s.CLAM), N.ptr(s.ACC),
+                    self.h, int(it == 0), N.ptr(s.CCOUNT), s._ccap))
             N.check(self.lib.xpbd_jacobi_tile(
                 st, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), anc,
                 N.ptr(ls), N.ptr(ld), trs, hs, hr, self.h, int(it == 0 and theta is None),
-                s._tile, self.n, s.jacobi_relax, fin, s.damping, s.max_speed, None))
+                s._tile, self.n, s.jacobi_relax, fin, s.damping, s.max_speed, acc))
             src, dst, ls, ld = dst, src, ld, ls
         if src is not X:
             X.copy_(src)
@@ -231,6 +242,8 @@ def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int
     Fnet = torch.empty_like(F)
     th = None if theta is None else torch.as_tensor(theta, dtype=torch.float32).to(s.device)
     for k in range(substeps):
+        if k % s.substeps == 0:
+            e.contacts()
         e.integrate()
         net = (k % s.substeps) % max(1, net_every) == 0
         if pair_with_net and net:
@@ -248,13 +261,15 @@ def collect(s, substeps: int, every: int = 1, target_iters: int = 64, iters: int
 
 def fit(make_scene, substeps: int = 120, settle_frames: int = 2, steps: int = 800,
         batch: int = 16384, seed: int = 0, log: list | None = None,
-        onpolicy_iters=(1, 2), net_every: int = 1, pair_with_net: bool = False):
+        onpolicy_iters=(1, 2), net_every: int = 1, pair_with_net: bool = False,
+        every: int = 1):
     """Two rounds: data from the plain solver's trajectory, training, data
     from the learned start's own trajectories (one per iteration count in
     ``onpolicy_iters``: the states a start followed by one iteration meets
     differ from those after two), training on all of it.  ``net_every`` and
     ``pair_with_net`` (see ``collect``) train a network for a start whose
-    network pass runs on every ``net_every``-th substep only.
+    network pass runs on every ``net_every``-th substep only; ``every``:
+    samples taken on every ``every``-th substep (large scenes).
     ``make_scene()`` returns a fresh Jacobi ``Solver``; returns theta."""
     import torch
 
@@ -263,13 +278,14 @@ def fit(make_scene, substeps: int = 120, settle_frames: int = 2, steps: int = 80
         s.step(settle_frames)
         return s
 
-    F, T = collect(fresh(), substeps, net_every=net_every, pair_with_net=pair_with_net)
+    F, T = collect(fresh(), substeps, every=every, net_every=net_every,
+                   pair_with_net=pair_with_net)
     theta, h1 = train(F, T, steps=steps, batch=batch, lr=3e-3, seed=seed, log_every=steps)
     Fs, Ts = [F], [T]
     ok = True
     for k in onpolicy_iters:
-        F2, T2 = collect(fresh(), substeps, theta=theta, iters=k, net_every=net_every,
-                         pair_with_net=pair_with_net)
+        F2, T2 = collect(fresh(), substeps, every=every, theta=theta, iters=k,
+                         net_every=net_every, pair_with_net=pair_with_net)
         if bool(torch.isfinite(F2).all() and torch.isfinite(T2).all()):
             Fs.append(F2)
             Ts.append(T2)
@@ -363,6 +379,8 @@ class Evaluator:
             restore()
             r = []
             for k in range(substeps):
+                if k % self.s.substeps == 0:
+                    e.contacts()
                 e.integrate()
                 net = (k % self.s.substeps) % max(1, net_every) == 0
                 e.iterate(its, theta=self.theta if kind == "learned" else None, net=net)

@@ -237,7 +237,7 @@ def other_workloads(frames: int) -> dict:
                                  "rows": rows, "proj_per_s": rows * 30 / (ms * 1e-3)}
     spec = draw(1024, 0)
     w = BatchedWorld(spec, device="cuda:0")
-    ms = time_frames(w.step, max(frames // 10, 5), 3)
+    ms = time_frames(w.step, max(frames // 4, 10), 5)
     rows = w.rows()
     out["batched_1024_scenes"] = {"ms_per_frame": round(ms, 4), **spec.totals(), "rows": rows,
                                   "proj_per_s": rows * 30 / (ms * 1e-3),

@@ -675,6 +675,27 @@ def test_batched_world_files_resume():
         assert np.abs(w.rigid.positions() - v.rigid.positions()).max() <= 1e-3
 
 
+def test_batched_world_step_equals_its_solvers_stepped_alone():
+    """The world's step (particle frame on the main stream, rigid frame on a
+    second stream, overlapping) gives the states that stepping its two
+    solvers one after the other on one stream gives, to float-atomic
+    rounding."""
+    from xpbd_amd.batch import BatchedWorld, draw
+
+    spec = draw(40, 5, mix={"cloth": 0.4, "pile": 0.2, "chain": 0.2, "boxes": 0.2})
+    assert any(sc.kind == "boxes" for sc in spec.scenes)
+    w = BatchedWorld(spec, device=DEV)
+    v = BatchedWorld(spec, device=DEV)
+    for _ in range(6):
+        w.step()
+        v.particles.step()
+        v.rigid.step()
+    xp_w, xp_v = w.particles.positions(), v.particles.positions()
+    assert np.all(np.isfinite(xp_w))
+    assert np.abs(xp_w - xp_v).max() <= 1e-4
+    assert np.abs(w.rigid.positions() - v.rigid.positions()).max() <= 1e-3
+
+
 def _sphere_pile(k=343, seed=5, **kw):
     from xpbd_amd import Solver
 

@@ -235,11 +235,15 @@ class BatchedWorld:
         if self._stream_b is None:
             self._stream_b = torch.cuda.Stream()
         main = torch.cuda.current_stream()
-        self._stream_b.wait_stream(main)
-        # the rigid frame first: the particles' device contact pass reads the
-        # contact count back (a wait on the main stream only), and the rigid
-        # frame already runs meanwhile
+        # the rigid solver waits for what was queued on the main stream before
+        # this step (state writes, the previous frame), not for this step's
+        # particle frame: both solvers read a count back once per frame (the
+        # contact count, the pair count), each waiting on its own stream only,
+        # so the rigid frame runs under the particle frame
+        start = torch.cuda.Event()
+        start.record(main)
+        self.particles.step()
+        self._stream_b.wait_event(start)
         with torch.cuda.stream(self._stream_b):
             self.rigid.step()
-        self.particles.step()
         main.wait_stream(self._stream_b)

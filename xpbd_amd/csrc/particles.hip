@@ -384,7 +384,11 @@ __host__ __device__ inline size_t tile_lds_bytes(int P) { return tile_lds_pos_of
 // launch pays only for what it uses: every optional branch left in the
 // kernel costs registers and scheduling freedom in the row loop (a runtime
 // finalize / ACC branch alone measured ~10 % on 8 M-particle cloths).
-enum TileMode : int { TM_PREDICT = 1, TM_FINALIZE = 2, TM_ACC = 4, TM_ANCHOR = 8, TM_NEXT = 16 };
+// FIRST: the first iteration of a substep whose positions are already
+// integrated (lambda starts from zero: no multiplier loads).
+enum TileMode : int {
+  TM_PREDICT = 1, TM_FINALIZE = 2, TM_ACC = 4, TM_ANCHOR = 8, TM_NEXT = 16, TM_FIRST = 32
+};
 
 // From this many particles the new positions are written with non-temporal
 // stores: 64 MB of positions plus the rows stream through the caches anyway,
@@ -405,6 +409,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
   constexpr bool HAS_ACC = MODE & TM_ACC, HAS_ANCHOR = MODE & TM_ANCHOR;
   constexpr bool NEXT = MODE & TM_NEXT;
   static_assert(!NEXT || (FINALIZE && !PREDICT), "NEXT extends a finalizing launch");
+  constexpr bool ZERO_LAM = PREDICT || (MODE & TM_FIRST);
   extern __shared__ unsigned long long lds64[];
   unsigned long long* sx = lds64;
   unsigned long long* sy = lds64 + P;
@@ -442,7 +447,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
       if (k < total) {
         jj[u] = k < owned ? rb + k : halo_rows[hb + (k - owned)];
         row[u] = R[jj[u]];
-        lam[u] = (PREDICT || first) ? 0.0f : LAMin[jj[u]];
+        lam[u] = (ZERO_LAM || first) ? 0.0f : LAMin[jj[u]];
       }
     }
   };
@@ -487,7 +492,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
       if (k < total) {
         njj[u] = k < owned ? rb + k : halo_rows[hb + (k - owned)];
         nrow[u] = R[njj[u]];
-        nlam[u] = (PREDICT || first) ? 0.0f : LAMin[njj[u]];
+        nlam[u] = (ZERO_LAM || first) ? 0.0f : LAMin[njj[u]];
       }
     }
 #pragma unroll
@@ -1043,6 +1048,10 @@ int launch_tile_block(int mode, hipStream_t s, int nb, size_t shmem, const float
     XPBD_TILE_MODE(TM_FINALIZE | TM_NEXT | TM_ACC)
     XPBD_TILE_MODE(TM_FINALIZE | TM_NEXT | TM_ANCHOR)
     XPBD_TILE_MODE(TM_FINALIZE | TM_NEXT | TM_ACC | TM_ANCHOR)
+    XPBD_TILE_MODE(TM_FIRST)
+    XPBD_TILE_MODE(TM_FIRST | TM_ACC)
+    XPBD_TILE_MODE(TM_FIRST | TM_ANCHOR)
+    XPBD_TILE_MODE(TM_FIRST | TM_ACC | TM_ANCHOR)
   }
 #undef XPBD_TILE_MODE
   return (int)hipErrorInvalidValue;  // predict never combines with finalize
@@ -1061,8 +1070,12 @@ int launch_tile(bool predict, hipStream_t s, const float* Xin, float* Xout, floa
   const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
   const int nb = blocks_for(n, tile_size);
   const size_t shmem = tile_lds_bytes(tile_size);
+  // a first, non-finalizing launch on integrated positions: lambda = 0 as
+  // a template argument (the finalizing one keeps the runtime flag)
+  const bool zero_lam = first && !predict && !finalize;
   const int mode = (predict ? TM_PREDICT : 0) | (finalize ? TM_FINALIZE : 0) |
-                   (finalize == 2 ? TM_NEXT : 0) | (ACC ? TM_ACC : 0) | (ANC ? TM_ANCHOR : 0);
+                   (finalize == 2 ? TM_NEXT : 0) | (ACC ? TM_ACC : 0) | (ANC ? TM_ANCHOR : 0) |
+                   (zero_lam ? TM_FIRST : 0);
   int rc;
   // 512 threads from 64-particle tiles up: a 64-particle tile of the 128^2
   // cloth has ~700 rows (owned + halo), one pass of 512 threads x 2 rows

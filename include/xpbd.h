@@ -146,15 +146,24 @@ int xpbd_solve_jacobi_dyn_first(void* stream, const float* X, const float* V, co
                                 float* LAM, float* ACC, float h, float gx, float gy, float gz,
                                 const float* gscale, const int* n_rows_dev, int cap);
 
-/* Device CONTACT generation, Solver._rebuild_contacts (solver.py:275-342):
- * margin = min(contact_margin + 3 dt max|v|, r_max), cell = 2 r_max + margin,
- * pairs a < b in neighbouring cells with |xa - xb|^2 <= (ra + rb + margin)^2,
- * rows CONTACT(a, b, rest ra + rb, mu sqrt(mu_a mu_b)) in (a, b) order into
- * ROWS (int4, clamped to cap); the full count goes to count[0].  radius[i] = 0
- * excludes particle i. */
+/* Device CONTACT generation, Solver._rebuild_contacts (solver.py:275-342)
+ * with the row order of _grid_candidate_pairs (solver.py:520-562):
+ * margin = min(contact_margin + 3 dt max|v|, r_max) (float64, as the host
+ * computes it), cell = 2 r_max + margin; for each of the 14 cell offsets
+ * (self, then the 13 forward neighbours in the reference's order), for each
+ * body k in ascending API index, the bodies p of the offset cell in
+ * ascending API index (same cell: p after k) with |xk - xp|^2 <= (rk + rp +
+ * margin)^2 and (min, max) of their API indices not in `excl` give rows
+ * CONTACT(k, p, rest rk + rp, mu sqrt(mu_k mu_p)), written with device
+ * indices into ROWS (int4, clamped to cap); the full count goes to
+ * count[0].  api[i] = API index of device particle i (NULL: identity);
+ * excl = sorted keys (lo << 32 | hi) of the API index pairs joined by a
+ * constraint (NULL with n_excl = 0: none).  radius[i] = 0 excludes
+ * particle i. */
 long long xpbd_contacts_workspace_bytes(int n);
 int xpbd_sphere_contacts(void* stream, void* ws, const float* X, const float* V,
-                         const float* radius, const float* mu, float contact_margin, float dt,
+                         const float* radius, const float* mu, const int* api,
+                         const long long* excl, int n_excl, double contact_margin, double dt,
                          float r_max, int n, int cap, int* ROWS, int* count);
 
 /* Whole frames of a small scene in one workgroup (solver.py:412-467): the

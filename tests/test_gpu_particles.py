@@ -712,35 +712,47 @@ def _sphere_pile(k=343, seed=5, **kw):
     return s
 
 
-def test_device_contacts_match_host_pairs():
-    """Device CONTACT generation (xpbd_sphere_contacts) finds the same pair
-    set as the host broad phase of the reference (solver.py:275-342) on a
-    jittered sphere pile, with the same rest lengths and friction."""
-    d = _sphere_pile()
+@pytest.mark.parametrize("moving", [False, True])
+def test_device_contacts_match_host_rows_in_order(moving):
+    """Device CONTACT generation (xpbd_sphere_contacts) gives the host broad
+    phase's rows (the reference's solver.py:275-342 with the enumeration
+    order of _grid_candidate_pairs) in the same order: same (a, b) in API
+    indices, same rest lengths and friction bit for bit, pairs joined by a
+    DISTANCE row skipped; on a jittered sphere pile in the Morton device
+    layout, at rest and with random velocities (larger search margin)."""
+    from xpbd_amd import Body
+
+    def build():
+        s = _sphere_pile()
+        for i in range(0, 60, 3):        # chain a few neighbours: those pairs are skipped
+            s.add_distance(Body(i), Body(i + 1), compliance=1e-6)
+        if moving:
+            rng = np.random.default_rng(2)
+            s._v.set(rng.normal(0, 0.4, (s.num_bodies, 3)))
+        return s
+
+    d = build()
     d.enable_device_contacts(True, margin=0.01)
     d._flush()
+    assert d._pperm is not None                   # the device layout is permuted
     d._device_contact_pass()
     cnt = d.n_contacts
     rows = d.CROWS[:cnt].cpu().numpy()
-    a = (rows[:, 0] & 0x0FFFFFFF).astype(np.int64)
-    b = rows[:, 1].astype(np.int64)
-    perm = d._pperm if d._pperm is not None else np.arange(d.num_bodies)
-    a, b = perm[a], perm[b]
-    dev = {(min(x, y), max(x, y)): (r[2], r[3]) for x, y, r in zip(a, b, rows)}
-    h = _sphere_pile()
+    perm = d._pperm
+    a = perm[(rows[:, 0] & 0x0FFFFFFF).astype(np.int64)]
+    b = perm[rows[:, 1].astype(np.int64)]
+    h = build()
     h.enable_self_collision(True, margin=0.01)
+    h._flush()                                    # velocities on the device (the margin)
     h._rebuild_contacts()
-    t = h._rt.array()
-    sel = t == 3
+    sel = h._rt.array() == 3
     ha, hb = h._ra.array()[sel], h._rb.array()[sel]
-    host = {(min(x, y), max(x, y)) for x, y in zip(ha, hb)}
-    assert cnt == len(host) > 300
-    assert set(dev) == host
-    rest = h._rrest.array()[sel]
-    mu = h._rmu.array()[sel]
-    for x, y, r0, m0 in zip(ha, hb, rest, mu):
-        r1, m1 = dev[(min(x, y), max(x, y))]
-        assert np.float32(r0).view(np.int32) == r1 and np.float32(m0).view(np.int32) == m1
+    assert cnt == len(ha) > 300
+    assert np.array_equal(a, ha) and np.array_equal(b, hb)
+    assert np.array_equal(rows[:, 2], h._rrest.array()[sel].astype(np.float32).view(np.int32))
+    assert np.array_equal(rows[:, 3], h._rmu.array()[sel].astype(np.float32).view(np.int32))
+    joined = {(i, i + 1) for i in range(0, 60, 3)}
+    assert not joined & {(min(x, y), max(x, y)) for x, y in zip(a, b)}
 
 
 def test_device_contacts_fused_integrate_matches_separate():

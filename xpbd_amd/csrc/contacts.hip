@@ -1,12 +1,19 @@
 // GPU sphere-sphere CONTACT generation for the 3-DOF solver — the device
-// counterpart of Solver._rebuild_contacts (src/xpbd3d/solver.py:275-342):
+// counterpart of Solver._rebuild_contacts (src/xpbd3d/solver.py:275-342)
+// with the row order of its grid broad phase (_grid_candidate_pairs,
+// solver.py:520-562):
 //   margin = min(contact_margin + 3 dt max|v|, r_max)   (solver.py:309-316)
 //   cell   = 2 r_max + margin                           (solver.py:324)
-//   keep pairs in the same / adjacent cells with |x_a - x_b|^2 <= (r_a + r_b + margin)^2
-//   row    = CONTACT(a, b, rest = r_a + r_b, mu = sqrt(mu_a mu_b))  (solver.py:334-341)
-// Pairs come out in (a, b) order with a < b (count, scan, emit, segment
-// sort), written as packed rows into a caller buffer; the live row count
-// stays on the device and gates the Jacobi projection of those rows.
+//   candidates: for each of the 14 cell offsets (the cell itself, then the
+//   13 forward neighbours in the reference's order), for each body k in
+//   ascending index, the bodies of cell(k) + offset in ascending index (the
+//   cell itself: only those after k)
+//   keep |x_k - x_p|^2 <= (r_k + r_p + margin)^2, skip pairs already joined
+//   by a constraint, row = CONTACT(k, p, rest = r_k + r_p, mu = sqrt(mu_k mu_p))
+// "Index" is the API particle index (`api`, when the device layout permutes
+// the particles), so the rows come out in exactly the reference's order.
+// Count per (offset, body), exclusive scan in offset-major order, emit; the
+// live row count stays on the device and gates the Jacobi projection.
 #include <algorithm>
 #include <cmath>
 
@@ -36,10 +43,10 @@ __global__ __launch_bounds__(kB) void k_vmax(const float4* __restrict__ V, int n
 
 // params[0] = margin, params[1] = cell (float32, computed as the reference
 // does in float64 host arithmetic, then rounded).
-__global__ void k_contact_params(const int* vmax_bits, float contact_margin, float dt,
+__global__ void k_contact_params(const int* vmax_bits, double contact_margin, double dt,
                                  float r_max, float* params) {
   double vmax = (double)__int_as_float(*vmax_bits);
-  double margin = fmin((double)contact_margin + 3.0 * (double)dt * vmax, (double)r_max);
+  double margin = fmin(contact_margin + 3.0 * dt * vmax, (double)r_max);
   params[0] = (float)margin;
   params[1] = (float)((double)r_max * 2.0 + margin);
 }
@@ -77,64 +84,97 @@ __global__ __launch_bounds__(kB) void k_sphere_scatter(const int4* __restrict__ 
   bucket[slot_start[s] + atomicAdd(&cursor[s], 1)] = i;
 }
 
-__device__ inline void sort_rows_by_b(int4* seg, int m) {
-  for (int a = 1; a < m; ++a) {
-    int4 v = seg[a];
-    int b = a - 1;
-    while (b >= 0 && seg[b].y > v.y) {
-      seg[b + 1] = seg[b];
-      --b;
-    }
-    seg[b + 1] = v;
-  }
+constexpr int kOffsets = 14;
+
+// Cell offset o of the reference's enumeration: the cell itself, then
+// (1,-1,-1) .. (1,1,1) in x-major order, (0,1,-1), (0,1,0), (0,1,1), (0,0,1).
+__device__ inline int3 cell_offset(int o) {
+  if (o == 0) return make_int3(0, 0, 0);
+  if (o <= 9) return make_int3(1, (o - 1) / 3 - 1, (o - 1) % 3 - 1);
+  if (o <= 12) return make_int3(0, 1, o - 11);
+  return make_int3(0, 0, 1);
 }
 
+// Pair (lo, hi) of API indices in the sorted key list (lo << 32 | hi)?
+__device__ inline bool excluded(const long long* __restrict__ excl, int n_excl, int a, int b) {
+  const int lo = min(a, b), hi = max(a, b);
+  const long long key = ((long long)lo << 32) | (long long)(unsigned)hi;
+  int l = 0, r = n_excl;
+  while (l < r) {
+    const int m = (l + r) >> 1;
+    if (excl[m] < key) l = m + 1;
+    else r = m;
+  }
+  return l < n_excl && excl[l] == key;
+}
+
+// EMIT = false: cnt[o n + api(i)] = number of rows body i contributes at
+// offset o.  EMIT = true: those rows at off[o n + api(i)], partners in
+// ascending API index.
 template <bool EMIT>
 __global__ __launch_bounds__(kB) void k_sphere_query(
     const float4* __restrict__ X, const float* __restrict__ radius,
     const float* __restrict__ mu, const float* __restrict__ params,
     const int4* __restrict__ cells, const int* __restrict__ slot_start,
-    const int* __restrict__ bucket, int T, int n, int* __restrict__ cnt,
+    const int* __restrict__ bucket, const int* __restrict__ api,
+    const long long* __restrict__ excl, int n_excl, int T, int n, int* __restrict__ cnt,
     const int* __restrict__ off, int4* __restrict__ rows, int cap) {
   const int i = blockIdx.x * kB + threadIdx.x;
   if (i >= n) return;
+  const int ai = api ? api[i] : i;
   const int4 ci = cells[i];
-  int found = 0;
-  if (ci.w >= 0) {
-    const float margin = params[0];
-    const float4 xi = X[i];
-    const float ri = radius[i];
-    const int base = EMIT ? off[i] : 0;
-    for (int dx = -1; dx <= 1; ++dx)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dz = -1; dz <= 1; ++dz) {
-          const int cx = ci.x + dx, cy = ci.y + dy, cz = ci.z + dz;
-          const int s = chash(cx, cy, cz, T);
-          for (int p = slot_start[s]; p < slot_start[s + 1]; ++p) {
-            const int j = bucket[p];
-            if (j <= i) continue;
-            const int4 cj = cells[j];
-            if (cj.x != cx || cj.y != cy || cj.z != cz) continue;
-            const float4 xj = X[j];
-            const float d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
-            const float dd = d0 * d0 + d1 * d1 + d2 * d2;
-            const float rs = ri + radius[j] + margin;
-            if (!(dd <= rs * rs)) continue;
-            if (EMIT && base + found < cap) {
-              const float rest = ri + radius[j];
-              const float m = sqrtf(mu[i] * mu[j]);
-              rows[base + found] =
-                  make_int4(i | (T_CONTACT << 28), j, __float_as_int(rest), __float_as_int(m));
-            }
-            ++found;
-          }
-        }
-    if (EMIT && found > 0) {
+  if (ci.w < 0) {
+    if (!EMIT)
+      for (int o = 0; o < kOffsets; ++o) cnt[o * n + ai] = 0;
+    return;
+  }
+  const float margin = params[0];
+  const float4 xi = X[i];
+  const float ri = radius[i];
+  for (int o = 0; o < kOffsets; ++o) {
+    const int3 d = cell_offset(o);
+    const int cx = ci.x + d.x, cy = ci.y + d.y, cz = ci.z + d.z;
+    const int s = chash(cx, cy, cz, T);
+    const int base = EMIT ? off[o * n + ai] : 0;
+    int found = 0;
+    for (int p = slot_start[s]; p < slot_start[s + 1]; ++p) {
+      const int j = bucket[p];
+      if (j == i) continue;
+      const int4 cj = cells[j];
+      if (cj.x != cx || cj.y != cy || cj.z != cz) continue;
+      const int aj = api ? api[j] : j;
+      if (o == 0 && aj <= ai) continue;
+      const float4 xj = X[j];
+      const float d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
+      const float dd = d0 * d0 + d1 * d1 + d2 * d2;
+      const float rs = ri + radius[j] + margin;
+      if (!(dd <= rs * rs)) continue;
+      if (n_excl > 0 && excluded(excl, n_excl, ai, aj)) continue;
+      if (EMIT && base + found < cap) {
+        const float rest = ri + radius[j];
+        const float m = sqrtf(mu[i] * mu[j]);
+        rows[base + found] =
+            make_int4(i | (T_CONTACT << 28), j, __float_as_int(rest), __float_as_int(m));
+      }
+      ++found;
+    }
+    if (EMIT) {  // partners in ascending API index (insertion sort of the segment)
+      int4* seg = rows + base;
       const int m = min(found, max(cap - base, 0));
-      if (m > 0) sort_rows_by_b(rows + base, m);
+      for (int q = 1; q < m; ++q) {
+        const int4 v = seg[q];
+        const int kv = api ? api[v.y] : v.y;
+        int r = q - 1;
+        while (r >= 0 && (api ? api[seg[r].y] : seg[r].y) > kv) {
+          seg[r + 1] = seg[r];
+          --r;
+        }
+        seg[r + 1] = v;
+      }
+    } else {
+      cnt[o * n + ai] = found;
     }
   }
-  if (!EMIT) cnt[i] = found;
 }
 
 __global__ __launch_bounds__(kB) void k_fill(int* p, int v, int len) {
@@ -172,9 +212,10 @@ size_t carve(char* base, Ws& w, int n) {
   w.slot_start = (int*)take(sizeof(int) * ((size_t)T + 1));
   w.cursor = (int*)take(sizeof(int) * (size_t)T);
   w.bucket = (int*)take(sizeof(int) * (size_t)n);
-  w.cnt = (int*)take(sizeof(int) * ((size_t)n + 1));
-  w.off = (int*)take(sizeof(int) * ((size_t)n + 1));
-  w.scan.block_sums = (int*)take(sizeof(int) * scan_block_sums(std::max<long long>(T + 1, n + 1)));
+  const long long m = (long long)kOffsets * n;
+  w.cnt = (int*)take(sizeof(int) * ((size_t)m + 1));
+  w.off = (int*)take(sizeof(int) * ((size_t)m + 1));
+  w.scan.block_sums = (int*)take(sizeof(int) * scan_block_sums(std::max<long long>(T + 1, m + 1)));
   return off;
 }
 
@@ -188,8 +229,11 @@ long long xpbd_contacts_workspace_bytes(int n) {
 }
 
 int xpbd_sphere_contacts(void* stream, void* ws, const float* X, const float* V,
-                         const float* radius, const float* mu, float contact_margin, float dt,
+                         const float* radius, const float* mu, const int* api,
+                         const long long* excl, int n_excl, double contact_margin, double dt,
                          float r_max, int n, int cap, int* ROWS, int* count) {
+  if (n_excl < 0 || (n_excl > 0 && !excl) || (long long)kOffsets * n >= (1ll << 31))
+    return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
   if (n <= 1 || !(r_max > 0.0f)) {
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(kB), 0, s, count, 0, 1);
@@ -213,13 +257,13 @@ int xpbd_sphere_contacts(void* stream, void* ws, const float* X, const float* V,
   hipLaunchKernelGGL(k_sphere_scatter, dim3(nb), dim3(kB), 0, s, w.cells, w.slot_start, w.cursor,
                      w.bucket, n);
   hipLaunchKernelGGL(k_sphere_query<false>, dim3(nb), dim3(kB), 0, s, (const float4*)X, radius,
-                     mu, w.params, w.cells, w.slot_start, w.bucket, T, n, w.cnt, w.off,
-                     (int4*)ROWS, cap);
-  rc = exclusive_scan(s, w.cnt, w.off, n, w.scan, count);
+                     mu, w.params, w.cells, w.slot_start, w.bucket, api, excl, n_excl, T, n,
+                     w.cnt, w.off, (int4*)ROWS, cap);
+  rc = exclusive_scan(s, w.cnt, w.off, kOffsets * n, w.scan, count);
   if (rc) return rc;
   hipLaunchKernelGGL(k_sphere_query<true>, dim3(nb), dim3(kB), 0, s, (const float4*)X, radius,
-                     mu, w.params, w.cells, w.slot_start, w.bucket, T, n, w.cnt, w.off,
-                     (int4*)ROWS, cap);
+                     mu, w.params, w.cells, w.slot_start, w.bucket, api, excl, n_excl, T, n,
+                     w.cnt, w.off, (int4*)ROWS, cap);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

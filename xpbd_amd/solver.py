@@ -708,6 +708,15 @@ class Solver:
             if pperm is not None:
                 rad, fr = rad[pperm], fr[pperm]
             self._crmax = float(rad.max()) if n_b else 0.0
+            # API index of each device particle, and the API pairs joined by
+            # a constraint (solver.py:285-290), as sorted lo << 32 | hi keys
+            self.CAPI = (torch.from_numpy(np.ascontiguousarray(pperm.astype(np.int32))).to(dev)
+                         if pperm is not None else None)
+            two = (h["b"] >= 0) & (h["type"] != CONTACT)
+            lo = np.minimum(h["a"][two], h["b"][two]).astype(np.int64)
+            hi = np.maximum(h["a"][two], h["b"][two]).astype(np.int64)
+            keys = np.unique((lo << 32) | hi)
+            self.CEXCL = torch.from_numpy(keys).to(dev) if len(keys) else None
             self.CRAD = torch.from_numpy(np.ascontiguousarray(rad)).to(dev)
             self.CMU = torch.from_numpy(np.ascontiguousarray(fr)).to(dev)
             self.CWS = torch.empty(int(N.lib().xpbd_contacts_workspace_bytes(n_b)),
@@ -743,7 +752,11 @@ class Solver:
         for _ in range(2):
             N.check(lib.xpbd_sphere_contacts(
                 N.current_stream(), N.ptr(self.CWS), N.ptr(self.X), N.ptr(self.V),
-                N.ptr(self.CRAD), N.ptr(self.CMU), self._contact_margin, self.dt, self._crmax,
+                N.ptr(self.CRAD), N.ptr(self.CMU),
+                None if self.CAPI is None else N.ptr(self.CAPI),
+                None if self.CEXCL is None else N.ptr(self.CEXCL),
+                0 if self.CEXCL is None else int(self.CEXCL.numel()),
+                self._contact_margin, self.dt, self._crmax,
                 int(self.X.shape[0]), self._ccap, N.ptr(self.CROWS), N.ptr(self.CCOUNT)),
                 "xpbd_sphere_contacts")
             cnt = int(self.CCOUNT.item())

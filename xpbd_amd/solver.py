@@ -704,6 +704,7 @@ class Solver:
             rad = self._radius.array().astype(np.float32)
             col = self._collide.array().astype(bool)
             rad = np.where(col, rad, 0.0).astype(np.float32)
+            rad_api = rad
             fr = self._friction.array().astype(np.float32)
             if pperm is not None:
                 rad, fr = rad[pperm], fr[pperm]
@@ -713,6 +714,7 @@ class Solver:
             self.CAPI = (torch.from_numpy(np.ascontiguousarray(pperm.astype(np.int32))).to(dev)
                          if pperm is not None else None)
             two = (h["b"] >= 0) & (h["type"] != CONTACT)
+            two[two] &= (rad_api[h["a"][two]] > 0) & (rad_api[h["b"][two]] > 0)  # candidates only
             lo = np.minimum(h["a"][two], h["b"][two]).astype(np.int64)
             hi = np.maximum(h["a"][two], h["b"][two]).astype(np.int64)
             keys = np.unique((lo << 32) | hi)

@@ -330,12 +330,13 @@ int xpbd6_apply_velocity(void* stream, const float* X, float* V, float* W, float
  * (solver_6dof.py:357-426 records the phases as separate launches).
  * begin_substep: apply_velocity of the previous substep when apply != 0
  * (solver_6dof.py:420-426), then integrate_bodies and the lambda resets
- * (solver_6dof.py:370-375).  solve_all: one thread per joint, per body's
- * floor corners and per live manifold point (solver_6dof.py:385-401); every
- * correction is added atomically into DX/DR.  apply_finalize: apply_jacobi
- * and, when finalize != 0, finalize_velocity (solver_6dof.py:402-408).
- * velocity_all: velocity_floor and velocity_box (solver_6dof.py:409-419),
- * added atomically into DV/DW. */
+ * (solver_6dof.py:370-375).  solve_all: one thread per joint, per box
+ * corner on the floor (8 lanes per body, summed in corner order) and per
+ * live manifold point (solver_6dof.py:385-401); every body's correction is
+ * added atomically into DX/DR.  apply_finalize: apply_jacobi and, when
+ * finalize != 0, finalize_velocity (solver_6dof.py:402-408).  velocity_all:
+ * velocity_floor (per corner, as solve_all) and velocity_box
+ * (solver_6dof.py:409-419), added atomically into DV/DW. */
 int xpbd6_begin_substep(void* stream, float* X, float* Q, float* V, float* W, float* XP,
                         float* QP, const float* IB, float* LAMF, int* PTC, float* DV, float* DW,
                         float max_lin, float max_ang, int do_cap, int apply, float gx, float gy,

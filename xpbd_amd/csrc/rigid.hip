@@ -653,6 +653,149 @@ __global__ __launch_bounds__(kB) void k6_apply_velocity(const float4* __restrict
 // of a Jacobi sweep is arbitrary in the reference as well (atomic_add).
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// Floor rows of the fused launches, one box corner per lane: 8 consecutive
+// lanes (a group starting at a lane index divisible by 8) take body i's
+// corners 0..7; each lane stages its corner's contributions in LDS and the
+// group's first lane adds them in corner order, exactly as the per-body loop
+// of solve_floor_elem / velocity_floor_elem does, then adds the body's sums
+// atomically.  Same results, a quarter of the dependent work per lane.
+// `slot` is this block's LDS staging area, 4 float4 per lane.
+// ---------------------------------------------------------------------------
+__device__ inline void stage_wait() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline void solve_floor_corner(int i, int c,
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ XP,
+    const float4* __restrict__ QP, const float4* __restrict__ IB, const float4* __restrict__ HE,
+    float floor_y, float mu_s, float* __restrict__ LAMF, float4* DX, float4* DR,
+    float4* slot) {
+  const float4 x4 = X[i];
+  const float im = x4.w;
+  f3 lin = mk3(0.f, 0.f, 0.f), ang = lin, lf = lin, af = lin;
+  float touched = 0.0f, fric = 0.0f, lam = 0.0f;
+  if (im != 0.0f) {
+    const f3 x = xyz(x4);
+    const q4 q = ldq(Q[i]);
+    const f3 iI = xyz(IB[i]);
+    const f3 s = mk3(0.0f, 1.0f, 0.0f);
+    const f3 corner = box_corner(xyz(HE[i]), c);
+    const f3 p1 = x + qrot(q, corner);
+    const float d = floor_y - p1.y;
+    if (d > 0.0f) {
+      const f3 r = p1 - x;
+      const float w = gen_inv_mass(im, q, iI, r, s);
+      if (w > 0.0f) {
+        const float dlam = d / w;
+        lam = dlam;
+        const f3 p = s * dlam;
+        lin = p * im;
+        ang = world_invI_mul(q, iI, cross(r, p));
+        touched = 1.0f;
+        if (mu_s > 0.0f) {
+          const f3 p1b = xyz(XP[i]) + qrot(ldq(QP[i]), corner);
+          const f3 dp = p1 - p1b;
+          const f3 dpt = dp - s * dot(dp, s);
+          const float lt = length(dpt);
+          if (lt > XPBD_EPS && lt < mu_s * d) {
+            const float w2 = gen_inv_mass(im, q, iI, r, dpt / lt);
+            if (w2 > 0.0f) {
+              const f3 pf = (-dpt) / w2;
+              lf = pf * im;
+              af = world_invI_mul(q, iI, cross(r, pf));
+              fric = 1.0f;
+            }
+          }
+        }
+      }
+    }
+    LAMF[8 * (size_t)i + c] = lam;
+  }
+  const int lane = threadIdx.x;
+  slot[4 * lane + 0] = make_float4(lin.x, lin.y, lin.z, touched);
+  slot[4 * lane + 1] = make_float4(ang.x, ang.y, ang.z, fric);
+  slot[4 * lane + 2] = make_float4(lf.x, lf.y, lf.z, 0.0f);
+  slot[4 * lane + 3] = make_float4(af.x, af.y, af.z, 0.0f);
+  stage_wait();
+  if (c != 0 || im == 0.0f) return;
+  float4 dx = make_float4(0.f, 0.f, 0.f, 0.f), dr = dx;
+  bool any = false;
+  for (int k = 0; k < 8; ++k) {
+    const float4 a = slot[4 * (lane + k) + 0];
+    if (a.w == 0.0f) continue;
+    const float4 b = slot[4 * (lane + k) + 1];
+    dx.x += a.x; dx.y += a.y; dx.z += a.z;
+    dr.x += b.x; dr.y += b.y; dr.z += b.z;
+    dx.w += 1.0f;
+    any = true;
+    if (b.w != 0.0f) {
+      const float4 e = slot[4 * (lane + k) + 2], g = slot[4 * (lane + k) + 3];
+      dx.x += e.x; dx.y += e.y; dx.z += e.z;
+      dr.x += g.x; dr.y += g.y; dr.z += g.z;
+    }
+  }
+  if (any) {
+    atomic_add3(&DX[i], xyz(dx));
+    atomicAdd(&DX[i].w, dx.w);
+    atomic_add3(&DR[i], xyz(dr));
+  }
+}
+
+__device__ inline void velocity_floor_corner(int i, int c,
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const float4* __restrict__ HE, float mu_d, float restitution,
+    const float* __restrict__ LAMF, float h, const float4* __restrict__ V,
+    const float4* __restrict__ W, float4* DV, float4* DW, float4* slot) {
+  const float im = X[i].w;
+  f3 lin = mk3(0.f, 0.f, 0.f), ang = lin;
+  float touched = 0.0f;
+  const float lam_n = im != 0.0f ? LAMF[8 * (size_t)i + c] : 0.0f;
+  if (lam_n != 0.0f) {
+    const q4 q = ldq(Q[i]);
+    const f3 iI = xyz(IB[i]);
+    const f3 r = qrot(q, box_corner(xyz(HE[i]), c));
+    const f3 vc = xyz(V[i]) + cross(xyz(W[i]), r);
+    const f3 s = mk3(0.0f, 1.0f, 0.0f);
+    const f3 impulse = velocity_impulse(vc, s, lam_n, mu_d, restitution, h);
+    if (length(impulse) > XPBD_EPS) {
+      const float wg = gen_inv_mass(im, q, iI, r, impulse / length(impulse));
+      if (wg > 0.0f) {
+        const f3 p = impulse / wg;
+        lin = p * im;
+        ang = world_invI_mul(q, iI, cross(r, p));
+        touched = 1.0f;
+      }
+    }
+  }
+  const int lane = threadIdx.x;
+  slot[4 * lane + 0] = make_float4(lin.x, lin.y, lin.z, touched);
+  slot[4 * lane + 1] = make_float4(ang.x, ang.y, ang.z, 0.0f);
+  stage_wait();
+  if (c != 0 || im == 0.0f) return;
+  float4 dv = make_float4(0.f, 0.f, 0.f, 0.f), dw = dv;
+  bool any = false;
+  for (int k = 0; k < 8; ++k) {
+    const float4 a = slot[4 * (lane + k) + 0];
+    if (a.w == 0.0f) continue;
+    const float4 b = slot[4 * (lane + k) + 1];
+    dv.x += a.x; dv.y += a.y; dv.z += a.z; dv.w += 1.0f;
+    dw.x += b.x; dw.y += b.y; dw.z += b.z;
+    any = true;
+  }
+  if (any) {
+    atomic_add3(&DV[i], xyz(dv));
+    atomicAdd(&DV[i].w, dv.w);
+    atomic_add3(&DW[i], xyz(dw));
+  }
+}
+
+// First lane of the floor corners in k6_solve_all: joints end at nj, the
+// corners start at the next multiple of 64, so every 8-lane body group lies
+// inside one wavefront.
+__host__ __device__ inline int floor_lane0(int nj) { return (nj + 63) & ~63; }
+
 // apply_velocity + cap of the previous substep (apply != 0), then integrate
 __global__ __launch_bounds__(kB) void k6_begin_substep(
     float4* X, float4* Q, float4* V, float4* W, float4* XP, float4* QP, const float4* IB,
@@ -666,7 +809,8 @@ __global__ __launch_bounds__(kB) void k6_begin_substep(
   integrate_elem(i, X, Q, V, W, XP, QP, IB, LAMF, gx, gy, gz, h, gyro, gscale, n);
 }
 
-// joints [0, nj), floor bodies [nj, nj + n), box points [nj + n, nj + n + max_points)
+// joints [0, nj), floor corners [f0, f0 + 8 n) with f0 = floor_lane0(nj), box
+// points [f0 + 8 n, f0 + 8 n + max_points)
 __global__ __launch_bounds__(kB) void k6_solve_all(
     const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ XP,
     const float4* __restrict__ QP, const float4* __restrict__ IB, const float4* __restrict__ HE,
@@ -676,13 +820,18 @@ __global__ __launch_bounds__(kB) void k6_solve_all(
     const float4* __restrict__ PTN, const float4* __restrict__ PTOI,
     const float4* __restrict__ PTOR, float* __restrict__ LAMP, const int* __restrict__ PTC,
     int max_points, float4* DX, float4* DR) {
+  __shared__ float4 slot[4 * kB];
   const int k = blockIdx.x * kB + threadIdx.x;
+  const int f0 = floor_lane0(nj);
   if (k < nj) {
     solve_joints_elem(k, X, Q, IB, JI, JA, JB, LAMJ, h, first, DX, DR, nj);
-  } else if (k < nj + n) {
-    solve_floor_elem<true>(k - nj, X, Q, XP, QP, IB, HE, floor_y, mu_s, LAMF, DX, DR, n);
+  } else if (k < f0) {
+    return;
+  } else if (k < f0 + 8 * n) {
+    const int t = k - f0;
+    solve_floor_corner(t >> 3, t & 7, X, Q, XP, QP, IB, HE, floor_y, mu_s, LAMF, DX, DR, slot);
   } else {
-    const int p = k - nj - n;
+    const int p = k - f0 - 8 * n;
     if (p < min(*PTC, max_points))
       solve_box_elem(p, X, Q, IB, PTI, PTN, PTOI, PTOR, LAMP, PTC, DX, DR, max_points);
   }
@@ -701,7 +850,7 @@ __global__ __launch_bounds__(kB) void k6_apply_finalize(float4* X, float4* Q, fl
   if (finalize) finalize_elem(i, X, Q, XP, QP, V, W, h, lin_damp, ang_damp, n);
 }
 
-// velocity pass: floor bodies [0, n), box points [n, n + max_points)
+// velocity pass: floor corners [0, 8 n), box points [8 n, 8 n + max_points)
 __global__ __launch_bounds__(kB) void k6_velocity_all(
     const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
     const float4* __restrict__ HE, float mu_d, float restitution, const float* __restrict__ LAMF,
@@ -710,11 +859,13 @@ __global__ __launch_bounds__(kB) void k6_velocity_all(
     const float4* __restrict__ PTOI, const float4* __restrict__ PTOR,
     const float* __restrict__ LAMP, const int* __restrict__ PTC, int max_points, float4* DV,
     float4* DW) {
+  __shared__ float4 slot[4 * kB];
   const int k = blockIdx.x * kB + threadIdx.x;
-  if (k < n) {
-    velocity_floor_elem<true>(k, X, Q, IB, HE, mu_d, restitution, LAMF, h, V, W, DV, DW, n);
+  if (k < 8 * n) {
+    velocity_floor_corner(k >> 3, k & 7, X, Q, IB, HE, mu_d, restitution, LAMF, h, V, W, DV, DW,
+                          slot);
   } else {
-    const int p = k - n;
+    const int p = k - 8 * n;
     if (p < min(*PTC, max_points))
       velocity_box_elem(p, X, Q, IB, PTI, PTN, PTOI, PTOR, LAMP, PTC, mu_d, restitution, h, V, W,
                         DV, DW, max_points);
@@ -1010,8 +1161,10 @@ int xpbd6_solve_all(void* stream, const float* X, const float* Q, const float* X
                     float floor_y, float mu_s, float* LAMF, int n, const int* PTI,
                     const float* PTN, const float* PTOI, const float* PTOR, float* LAMP,
                     const int* PTC, int max_points, float* DX, float* DR) {
-  const long long total = (long long)nj + n + max_points;
+  if (nj < 0 || n < 0 || max_points < 0) return (int)hipErrorInvalidValue;
+  const long long total = (long long)floor_lane0(nj) + 8ll * n + max_points;
   if (total <= 0) return 0;
+  if (total >= (1ll << 31)) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL(k6_solve_all, dim3(nblk(total)), dim3(kB), 0, (hipStream_t)stream, CF4(X),
                      CF4(Q), CF4(XP), CF4(QP), CF4(IB), CF4(HE), (const int2*)JI, CF4(JA),
                      CF4(JB), LAMJ, h, first, nj, floor_y, mu_s, LAMF, n, (const int4*)PTI,
@@ -1036,8 +1189,10 @@ int xpbd6_velocity_all(void* stream, const float* X, const float* Q, const float
                        float h, const float* V, const float* W, int n, const int* PTI,
                        const float* PTN, const float* PTOI, const float* PTOR, const float* LAMP,
                        const int* PTC, int max_points, float* DV, float* DW) {
-  const long long total = (long long)n + max_points;
+  if (n < 0 || max_points < 0) return (int)hipErrorInvalidValue;
+  const long long total = 8ll * n + max_points;
   if (total <= 0) return 0;
+  if (total >= (1ll << 31)) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL(k6_velocity_all, dim3(nblk(total)), dim3(kB), 0, (hipStream_t)stream,
                      CF4(X), CF4(Q), CF4(IB), CF4(HE), mu_d, restitution, LAMF, h, CF4(V), CF4(W),
                      n, (const int4*)PTI, CF4(PTN), CF4(PTOI), CF4(PTOR), LAMP, PTC, max_points,

@@ -111,35 +111,69 @@ __device__ inline float proj_radius(f3 L, const m3& R, f3 e) {
 // points of a wavefront are compacted into the point arrays with one atomic
 // per wave (inclusive scan of the per-lane counts over the 64 lanes); all 64
 // lanes of a wave call it together (pid >= live: no pair).
+// The clipping polygons of one pair (two buffers of up to 8 points).
+// LocalPoly: private arrays (scratch memory; the whole-frame workgroup
+// kernel, whose LDS holds the body state).  LdsPoly: this lane's slice of a
+// block-wide LDS array, lane-minor so the lanes of a wave hit consecutive
+// banks (the fused launch: a dependent scratch access costs several times an
+// LDS access, and the clip loop is a chain of them).
+struct LocalPoly {
+  f3 p[2][8];
+  __device__ f3 get(int b, int k) const { return p[b][k]; }
+  __device__ void set(int b, int k, f3 v) { p[b][k] = v; }
+};
+template <int STRIDE>
+struct LdsPoly {
+  float* base;  // &array[0][lane]
+  __device__ f3 get(int b, int k) const {
+    const int o = 3 * (8 * b + k) * STRIDE;
+    return mk3(base[o], base[o + STRIDE], base[o + 2 * STRIDE]);
+  }
+  __device__ void set(int b, int k, f3 v) {
+    const int o = 3 * (8 * b + k) * STRIDE;
+    base[o] = v.x;
+    base[o + STRIDE] = v.y;
+    base[o + 2 * STRIDE] = v.z;
+  }
+};
+constexpr int kPolyFloats = 2 * 8 * 3;
+
+template <class Poly>
 __device__ inline void box_manifold_elem(
     int pid, int live, const float4* __restrict__ X, const float4* __restrict__ Q,
     const float4* __restrict__ HE, const int2* __restrict__ PAIRS, float margin, int cap,
     int* __restrict__ MC, int4* __restrict__ PTI, float4* __restrict__ PTN,
     float4* __restrict__ PTOI, float4* __restrict__ PTOR, float* __restrict__ LAMP,
-    int* __restrict__ PTC) {
+    int* __restrict__ PTC, Poly& poly) {
   const int lane = threadIdx.x & 63;
   int nc = 0, inc_b = 0, ref_b = 0;
   f3 ref_face_n = mk3(0.f, 0.f, 0.f);
-  f3 off_i[8], off_r[8];
+  // the surviving polygon and the frames its points are expressed in
+  int cur = 0, plen = 0;
+  f3 ref_face_c = mk3(0.f, 0.f, 0.f), c_inc = ref_face_c, c_ref = ref_face_c;
+  f3 ri0 = c_inc, ri1 = c_inc, ri2 = c_inc, rr0 = c_inc, rr1 = c_inc, rr2 = c_inc;  // frames
   if (pid < live) {
     const int2 pr = PAIRS[pid];
     const int a = pr.x, b = pr.y;
     const f3 cA = xyz(X[a]), cB = xyz(X[b]);
     const f3 eA = xyz(HE[a]), eB = xyz(HE[b]);
-    const m3 RA = qmat(ldq(Q[a])), RB = qmat(ldq(Q[b]));
+    const q4 qa = ldq(Q[a]), qb = ldq(Q[b]);
+    const m3 RA = qmat(qa), RB = qmat(qb);
     const f3 cBA = cA - cB;
     float best_overlap = 1.0e30f;
     f3 best_axis = mk3(0.0f, 1.0f, 0.0f);
     int ref_is_a = 1, best_k = 0;
     bool sep = false;
+#pragma unroll
     for (int k = 0; k < 3 && !sep; ++k) {
-      f3 L = RA.c[k];
+      f3 L = col(RA, k);
       float ov = comp(eA, k) + proj_radius(L, RB, eB) - fabsf(dot(L, cBA));
       if (ov < 0.0f) sep = true;
       else if (ov < best_overlap) { best_overlap = ov; best_axis = L; ref_is_a = 1; best_k = k; }
     }
+#pragma unroll
     for (int k = 0; k < 3 && !sep; ++k) {
-      f3 L = RB.c[k];
+      f3 L = col(RB, k);
       float ov = comp(eB, k) + proj_radius(L, RA, eA) - fabsf(dot(L, cBA));
       if (ov < 0.0f) sep = true;
       else if (ov < best_overlap) { best_overlap = ov; best_axis = L; ref_is_a = 0; best_k = k; }
@@ -147,38 +181,42 @@ __device__ inline void box_manifold_elem(
     if (!sep) {
       f3 nBA = best_axis;
       if (dot(nBA, cBA) < 0.0f) nBA = -nBA;
-      f3 c_ref, e_ref, c_inc, e_inc;
-      m3 R_ref, R_inc;
-      if (ref_is_a == 1) {
-        ref_b = a; inc_b = b; c_ref = cA; R_ref = RA; e_ref = eA;
-        c_inc = cB; R_inc = RB; e_inc = eB; ref_face_n = -nBA;
-      } else {
-        ref_b = b; inc_b = a; c_ref = cB; R_ref = RB; e_ref = eB;
-        c_inc = cA; R_inc = RA; e_inc = eA; ref_face_n = nBA;
-      }
+      // reference and incident box by value selects; their frames are
+      // recomputed from the quaternions (qmat / qrot of a unit axis: the
+      // same operations on the same values as RA / RB), never picked out of
+      // a matrix by a runtime index, so everything stays in registers
+      const bool ra = ref_is_a == 1;
+      ref_b = ra ? a : b;
+      inc_b = ra ? b : a;
+      c_ref = sel3(ra, cA, cB);
+      c_inc = sel3(ra, cB, cA);
+      const f3 e_ref = sel3(ra, eA, eB), e_inc = sel3(ra, eB, eA);
+      const q4 q_ref = selq(ra, qa, qb), q_inc = selq(ra, qb, qa);
+      ref_face_n = ra ? -nBA : nBA;
+      const m3 R_ref = qmat(q_ref), R_inc = qmat(q_inc);
       const int ref_axis = best_k;
-      const f3 ref_face_c = c_ref + ref_face_n * comp(e_ref, ref_axis);
+      ref_face_c = c_ref + ref_face_n * comp(e_ref, ref_axis);
       const int ax1 = (ref_axis + 1) % 3, ax2 = (ref_axis + 2) % 3;
-      const f3 u_ref = R_ref.c[ax1], v_ref = R_ref.c[ax2];
+      const f3 u_ref = qrot(q_ref, unit_axis(ax1)), v_ref = qrot(q_ref, unit_axis(ax2));
       const float eu = comp(e_ref, ax1), ev = comp(e_ref, ax2);
       int inc_axis = 0;
       float inc_sign = 1.0f, best_dot = -1.0e30f;
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
-        float dp = -dot(R_inc.c[k], ref_face_n);
+        float dp = -dot(col(R_inc, k), ref_face_n);
         if (dp > best_dot) { best_dot = dp; inc_axis = k; inc_sign = 1.0f; }
         if (-dp > best_dot) { best_dot = -dp; inc_axis = k; inc_sign = -1.0f; }
       }
-      const f3 inc_n = R_inc.c[inc_axis] * inc_sign;
+      const f3 inc_n = qrot(q_inc, unit_axis(inc_axis)) * inc_sign;
       const f3 inc_face_c = c_inc + inc_n * comp(e_inc, inc_axis);
       const int iax1 = (inc_axis + 1) % 3, iax2 = (inc_axis + 2) % 3;
-      const f3 iu = R_inc.c[iax1], iv = R_inc.c[iax2];
+      const f3 iu = qrot(q_inc, unit_axis(iax1)), iv = qrot(q_inc, unit_axis(iax2));
       const float ieu = comp(e_inc, iax1), iev = comp(e_inc, iax2);
-      f3 poly[2][8];
-      poly[0][0] = (inc_face_c + iu * ieu) + iv * iev;
-      poly[0][1] = (inc_face_c - iu * ieu) + iv * iev;
-      poly[0][2] = (inc_face_c - iu * ieu) - iv * iev;
-      poly[0][3] = (inc_face_c + iu * ieu) - iv * iev;
-      int cur = 0, plen = 4;
+      poly.set(0, 0, (inc_face_c + iu * ieu) + iv * iev);
+      poly.set(0, 1, (inc_face_c - iu * ieu) + iv * iev);
+      poly.set(0, 2, (inc_face_c - iu * ieu) - iv * iev);
+      poly.set(0, 3, (inc_face_c + iu * ieu) - iv * iev);
+      plen = 4;
       bool dead = false;
       for (int pi = 0; pi < 4; ++pi) {
         f3 pp, pn;
@@ -190,35 +228,30 @@ __device__ inline void box_manifold_elem(
         int olen = 0;
         const int nx = cur ^ 1;
         for (int k = 0; k < plen; ++k) {
-          f3 ap = poly[cur][k];
-          f3 bp = poly[cur][(k + 1) % plen];
+          f3 ap = poly.get(cur, k);
+          f3 bp = poly.get(cur, (k + 1) % plen);
           float da = dot(ap - pp, pn), db = dot(bp - pp, pn);
           if (da <= 0.0f) {
-            if (olen < 8) poly[nx][olen++] = ap;
+            if (olen < 8) poly.set(nx, olen++, ap);
             if (db > 0.0f) {
               float t = da / (da - db);
-              if (olen < 8) poly[nx][olen++] = ap + (bp - ap) * t;
+              if (olen < 8) poly.set(nx, olen++, ap + (bp - ap) * t);
             }
           } else if (db <= 0.0f) {
             float t = da / (da - db);
-            if (olen < 8) poly[nx][olen++] = ap + (bp - ap) * t;
+            if (olen < 8) poly.set(nx, olen++, ap + (bp - ap) * t);
           }
         }
         cur = nx;
         plen = olen;
       }
-      if (!dead && plen > 0) {
-        for (int k = 0; k < plen; ++k) {
-          if (nc >= 8) break;
-          f3 p_inc = poly[cur][k];
-          float d_signed = dot(p_inc - ref_face_c, ref_face_n);
-          if (d_signed < margin) {
-            f3 p_ref = p_inc - ref_face_n * d_signed;
-            off_i[nc] = mtv(R_inc, p_inc - c_inc);
-            off_r[nc] = mtv(R_ref, p_ref - c_ref);
-            nc += 1;
-          }
-        }
+      if (dead) plen = 0;
+      ri0 = R_inc.c[0], ri1 = R_inc.c[1], ri2 = R_inc.c[2];
+      rr0 = R_ref.c[0], rr1 = R_ref.c[1], rr2 = R_ref.c[2];
+      // count the points below the margin (the first 8); emitted below
+      for (int k = 0; k < plen && nc < 8; ++k) {
+        const f3 p_inc = poly.get(cur, k);
+        if (dot(p_inc - ref_face_c, ref_face_n) < margin) nc += 1;
       }
     }
   }
@@ -233,13 +266,21 @@ __device__ inline void box_manifold_elem(
   if (lane == 63 && total > 0) base = atomicAdd(PTC, total);
   base = __shfl(base, 63, 64);
   const int first = base + incl - nc;
-  for (int c = 0; c < nc; ++c) {
-    const int k = first + c;
-    PTI[k] = make_int4(pid, inc_b, ref_b, c);
-    PTN[k] = make_float4(ref_face_n.x, ref_face_n.y, ref_face_n.z, 0.0f);
-    PTOI[k] = make_float4(off_i[c].x, off_i[c].y, off_i[c].z, 0.0f);
-    PTOR[k] = make_float4(off_r[c].x, off_r[c].y, off_r[c].z, 0.0f);
-    LAMP[k] = 0.0f;
+  for (int k = 0, c = 0; k < plen && c < nc; ++k) {
+    const f3 p_inc = poly.get(cur, k);
+    const float d_signed = dot(p_inc - ref_face_c, ref_face_n);
+    if (!(d_signed < margin)) continue;
+    const f3 p_ref = p_inc - ref_face_n * d_signed;
+    const f3 vi = p_inc - c_inc, vr = p_ref - c_ref;
+    const f3 oi = mk3(dot(ri0, vi), dot(ri1, vi), dot(ri2, vi));   // mtv(R_inc, vi)
+    const f3 orf = mk3(dot(rr0, vr), dot(rr1, vr), dot(rr2, vr));  // mtv(R_ref, vr)
+    const int q = first + c;
+    PTI[q] = make_int4(pid, inc_b, ref_b, c);
+    PTN[q] = make_float4(ref_face_n.x, ref_face_n.y, ref_face_n.z, 0.0f);
+    PTOI[q] = make_float4(oi.x, oi.y, oi.z, 0.0f);
+    PTOR[q] = make_float4(orf.x, orf.y, orf.z, 0.0f);
+    LAMP[q] = 0.0f;
+    ++c;
   }
   if (pid < cap) MC[pid] = nc;
 }
@@ -250,8 +291,10 @@ __global__ __launch_bounds__(kB) void k6_box_manifold(
     int* __restrict__ MC, int4* __restrict__ PTI, float4* __restrict__ PTN,
     float4* __restrict__ PTOI, float4* __restrict__ PTOR, float* __restrict__ LAMP,
     int* __restrict__ PTC) {
+  __shared__ float psh[kPolyFloats * kB];
+  LdsPoly<kB> poly{psh + threadIdx.x};
   box_manifold_elem(blockIdx.x * kB + threadIdx.x, min(*PC, cap), X, Q, HE, PAIRS, margin, cap,
-                    MC, PTI, PTN, PTOI, PTOR, LAMP, PTC);
+                    MC, PTI, PTN, PTOI, PTOR, LAMP, PTC, poly);
 }
 
 // kernels_6dof.py:440-483 (lambda reset folded into the first iteration)
@@ -987,9 +1030,11 @@ __global__ __launch_bounds__(kFB) void k6_frame_block(xpbd_rigid_desc d, float4*
                        d.gyroscopic, d.gscale, n);
       }
       fb_sync();
-      for (int base = 0; base < live; base += kFB)  // whole waves call the manifold
+      for (int base = 0; base < live; base += kFB) {  // whole waves call the manifold
+        LocalPoly poly;
         box_manifold_elem(base + tid, live, X, Q, HE, PAIRS, d.man_margin, d.cap, d.m_count, PTI,
-                          PTN, PTOI, PTOR, d.lam_pt, d.pt_count);
+                          PTN, PTOI, PTOR, d.lam_pt, d.pt_count, poly);
+      }
       fb_sync();
       const int npts = min(ld_count(d.pt_count), maxp);
       for (int it = 0; it < d.iterations; ++it) {

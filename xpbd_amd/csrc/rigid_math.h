@@ -57,6 +57,19 @@ __host__ __device__ inline f3 mtv(const m3& R, f3 v) {
   return mk3(dot(R.c[0], v), dot(R.c[1], v), dot(R.c[2], v));
 }
 __host__ __device__ inline float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+// Column k of R for a k the compiler can resolve (unrolled loops); a
+// runtime axis uses qrot(q, unit_axis(k)) instead, which computes the same
+// column from the quaternion without indexing a private array
+__host__ __device__ inline f3 col(const m3& R, int k) { return R.c[k]; }
+__host__ __device__ inline f3 unit_axis(int k) {
+  return mk3(k == 0 ? 1.0f : 0.0f, k == 1 ? 1.0f : 0.0f, k == 2 ? 1.0f : 0.0f);
+}
+__host__ __device__ inline f3 sel3(bool c, f3 a, f3 b) {
+  return mk3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
+}
+__host__ __device__ inline q4 selq(bool c, q4 a, q4 b) {
+  return q4{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w};
+}
 __host__ __device__ inline f3 mulc(f3 a, f3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
 
 // kernels_6dof.py:29-35

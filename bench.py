@@ -470,8 +470,9 @@ def main() -> int:
                     help="also run plain 512 MiB device copies (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} requested; this build measures one GPU only "
-              "(scenes shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: --gpus {args.gpus} requested; this benchmark measures one GPU only "
+              "(independent scenes would shard across GPUs without communication; the "
+              "multi-GPU path is not built)", file=sys.stderr)
         return 2
 
     import torch

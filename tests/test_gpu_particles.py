@@ -360,6 +360,31 @@ def test_frame_plan_integrate_folding_is_exact(iters):
             assert np.array_equal(g, e), (k, iters)
 
 
+@pytest.mark.parametrize("substeps,iters", [(1, 1), (1, 2), (2, 1), (3, 3)])
+def test_integrate_folding_edge_shapes(substeps, iters):
+    """The integrate folds on the edge shapes of a frame (one substep: no
+    next-substep fold at all; one iteration: the first substep integrates in
+    place and every launch finalizes) on a ragged scene (49 particles, one
+    partial 64-particle tile, 25 extra particles in a second one at tile 32):
+    identical to the separate integrate pass."""
+    from xpbd_amd import Solver
+    from xpbd_amd.scenes import cloth_sheet
+
+    for tile in (64, 32):
+        out = {}
+        for fuse in (False, "first", True):
+            s, _ = cloth_sheet(Solver, res=7, solve_mode="jacobi", device=DEV,
+                               substeps=substeps, iterations=iters)
+            s.tile_size = tile
+            s.block_frames = False
+            s.fuse_first = fuse
+            s.step(12)
+            out[str(fuse)] = (s.positions(), s.velocities(), s.lambdas())
+        for k in ("first", "True"):
+            for g, e in zip(out[k], out["False"]):
+                assert np.array_equal(g, e), (k, tile, substeps, iters)
+
+
 def test_autotune_restores_state_and_keeps_results():
     """The autotuner times every launch shape without advancing the scene,
     and the shape it picks steps to the same state (to the Jacobi parity

@@ -359,3 +359,17 @@ def test_scene_files_round_trip_host(tmp_path):
     for f in (p, p6):
         with np.load(f, allow_pickle=False) as z:
             assert all(z[k].dtype != object for k in z.files)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with a message (and
+    status 2) before touching a device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""

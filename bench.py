@@ -211,7 +211,7 @@ def other_workloads(frames: int) -> dict:
     """Configurations 1, 3 and 4 of the benchmark set and the cloth in GS
     mode, timed the same way (secondary lines; the headline is the cloth in
     Jacobi mode)."""
-    from xpbd_amd import Solver, Solver6DOF
+    from xpbd_amd import Solver, Solver6DOF, autotune
     from xpbd_amd.batch import BatchedWorld, draw
     from xpbd_amd.scenes import cloth_sheet, hanging_chain, unified_scene
 
@@ -228,6 +228,8 @@ def other_workloads(frames: int) -> dict:
                          "proj_per_s": c.num_constraints * 30 / (ms * 1e-3),
                          "substeps_per_s": 15 / (ms * 1e-3)}
     u = unified_scene(Solver6DOF, device="cuda:0")
+    u.step()
+    leaf, leaf_times = autotune.tune6(u, frames=3)   # LBVH leaf size (untimed setup)
     # 30 untimed frames: the boxes land and the pair and manifold buffers
     # reach their steady size (a growth re-captures the frame graph)
     ms = time_frames(u.step, max(frames // 4, 10), 30)
@@ -236,7 +238,10 @@ def other_workloads(frames: int) -> dict:
     out["unified_1440_boxes"] = {"ms_per_frame": round(ms, 4), "bodies": u.num_bodies,
                                  "joints": u.num_joints, "pairs": u.n_pairs,
                                  "manifold_points": int(u.contact_counts().sum()),
-                                 "rows": rows, "proj_per_s": rows * 30 / (ms * 1e-3)}
+                                 "rows": rows, "proj_per_s": rows * 30 / (ms * 1e-3),
+                                 "lbvh_leaf": leaf,
+                                 "leaf_trials_ms_per_frame": {str(k): round(v, 4)
+                                                              for k, v in leaf_times.items()}}
     spec = draw(1024, 0)
     w = BatchedWorld(spec, device="cuda:0")
     ms = time_frames(w.step, max(frames // 4, 10), 30)

@@ -415,6 +415,15 @@ long long xpbd6_hashgrid_workspace_bytes(int n);
 int xpbd6_broadphase_lbvh(void* stream, void* ws, const float* X, const float* Q, const float* V,
                           const float* W, const float* HE, float man_margin, float dt, int n,
                           int cap, int* PAIRS, int* PC);
+/* The same with the hierarchy's leaf size chosen by the caller: leaf_size
+ * bodies (consecutive in Morton order) per LBVH leaf, 1..64, or 0 for the
+ * all-pairs test; xpbd6_broadphase_lbvh uses 0 up to 4096 bodies and 1
+ * above.  The pair list is the same for every leaf size (the reference's
+ * leaf size is internal to wp.Bvh and unobservable); an autotuner picks it
+ * by timing. */
+int xpbd6_broadphase_lbvh_leaf(void* stream, void* ws, const float* X, const float* Q,
+                               const float* V, const float* W, const float* HE, float man_margin,
+                               float dt, int n, int cap, int* PAIRS, int* PC, int leaf_size);
 /* wp.HashGrid.build + emit_pairs_hashgrid (solver_6dof.py:456-479;
  * kernels_hashgrid.py:37-88); cell = 2 rs_max + margin (solver_6dof.py:335-337). */
 int xpbd6_broadphase_hashgrid(void* stream, void* ws, const float* X, const float* Q,

@@ -73,6 +73,51 @@ def test_broadphase_pair_set_bit_exact(oracle, broadphase, n, seed):
     assert (s.pairs() == want).all()
 
 
+@pytest.mark.parametrize("n,seed", [(1, 0), (7, 4), (300, 1), (3000, 2)])
+def test_lbvh_leaf_sizes_give_the_oracle_pairs(oracle, n, seed):
+    """Every LBVH leaf size (1 .. 64 bodies per leaf, leaves larger than the
+    scene included, and 0 = all pairs) emits the oracle's brute-force pair
+    list: same pairs, same (i, j) order, same count."""
+    s = _random_boxes(n, seed, spread=3.0 if n > 100 else 0.2)
+    s._flush()
+    orc = _oracle6(oracle, s)
+    orc.broadphase_pairs()
+    want = np.stack([orc.pair_a[:orc.n_pairs], orc.pair_b[:orc.n_pairs]], axis=1)
+    for leaf in (0, 1, 2, 3, 8, 16, 64):
+        s.lbvh_leaf = leaf
+        cnt = s._run_broadphase(n, s.cap)
+        if cnt > s.cap:
+            s.cap = int(cnt * 1.5)
+            s._alloc_pair_buffers(s.cap)
+            cnt = s._run_broadphase(n, s.cap)
+        s.n_pairs = cnt
+        assert cnt == orc.n_pairs, leaf
+        assert (s.pairs() == want).all(), leaf
+
+
+def test_autotune_leaf_size_keeps_state_and_pairs():
+    """The leaf-size autotuner times frames from one state, restores it, and
+    the picked leaf size steps to the same pair list as the default."""
+    from xpbd_amd import Solver6DOF, autotune
+    from xpbd_amd.scenes import box_pile
+
+    s = box_pile(Solver6DOF, 8, 5, 8, substeps=4, device=DEV)
+    r = box_pile(Solver6DOF, 8, 5, 8, substeps=4, device=DEV)
+    s.block_frames = r.block_frames = False
+    s.step()
+    r.step()
+    x0 = s.positions()
+    best, times = autotune.tune6(s, frames=2)
+    assert best in times and 0 in times and 1 in times
+    assert np.array_equal(s.positions(), x0)
+    s.step()
+    r.step()
+    assert s.n_pairs == r.n_pairs and (s.pairs() == r.pairs()).all()
+    with pytest.raises(RuntimeError):
+        s.lbvh_leaf = 65
+        s.step()
+
+
 def test_broadphase_overflow_regrows():
     """A dense pile with a tiny pair capacity: the frame detects the overflow
     from the device count, grows the buffers and re-emits (solver_6dof.py:443-453)."""

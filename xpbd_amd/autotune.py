@@ -100,3 +100,64 @@ def tune(s, frames: int = 3, warmup: int = 1, verbose: bool = False):
     s.set_state(x0, v0)
     _CACHE[sig] = best
     return best, times
+
+
+# ---------------------------------------------------------------------------
+# 6-DOF: leaf size of the LBVH broad phase
+# ---------------------------------------------------------------------------
+LEAF_SIZES = (1, 2, 4, 8, 16)
+_ALL_PAIRS_MAX = 16384       # all-pairs candidate up to this many bodies
+
+
+def leaf_candidates(n_bodies: int):
+    """LBVH leaf sizes to time (0 = the all-pairs test)."""
+    out = [0] if n_bodies <= _ALL_PAIRS_MAX else []
+    return out + [L for L in LEAF_SIZES if L < max(2, n_bodies)]
+
+
+def tune6(s, frames: int = 3, warmup: int = 1, verbose: bool = False):
+    """Pick the LBVH leaf size (bodies per leaf, or all pairs) of a
+    ``Solver6DOF`` with the LBVH broad phase by timing whole frames (HIP
+    events; each frame's broad phase is part of its step) from the same
+    state; sets ``s.lbvh_leaf`` and returns ``(leaf, {leaf: ms_per_frame})``.
+    Every leaf size gives the same pair list, so only the time differs.
+    Scenes stepped as whole-frame workgroups (their broad phase is inside
+    that kernel) are left alone."""
+    import torch
+
+    if s.broadphase != "lbvh":
+        return None, {}
+    s._flush()
+    n = s.num_bodies
+    if n == 0 or s._use_block(n):
+        return None, {}
+    sig = ("6dof", n, s.num_joints, s.substeps, s.iterations)
+    if sig in _CACHE:
+        s.lbvh_leaf = _CACHE[sig]
+        return _CACHE[sig], {}
+    b = s._buf
+    keep = {k: b[k].clone() for k in ("X", "Q", "V", "W")}
+    times = {}
+    for leaf in leaf_candidates(n):
+        s.lbvh_leaf = leaf
+        for k, t in keep.items():
+            b[k].copy_(t)
+        for _ in range(warmup):
+            s.step()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(frames):
+            s.step()
+        e1.record()
+        torch.cuda.synchronize()
+        times[leaf] = e0.elapsed_time(e1) / frames
+        if verbose:
+            print(f"[autotune] lbvh leaf {leaf}: {times[leaf]:.4f} ms/frame")
+    best = min(times, key=times.get)
+    s.lbvh_leaf = best
+    for k, t in keep.items():
+        b[k].copy_(t)
+    _CACHE[sig] = best
+    return best, times

@@ -283,32 +283,35 @@ __global__ __launch_bounds__(kB) void k_radix_scatter(const unsigned* __restrict
 // ---------------------------------------------------------------------------
 // Karras 2012 hierarchy.  Children: >= 0 internal node, < 0 leaf ~k.
 // ---------------------------------------------------------------------------
-__device__ inline int delta(const unsigned* key, int n, int i, int j) {
+// Leaves hold L consecutive bodies of the Morton order (L = the leaf size;
+// leaf i's key is that of its first body); the hierarchy is Karras's over
+// the n = ceil(bodies / L) leaves.
+__device__ inline int delta(const unsigned* key, int n, int L, int i, int j) {
   if (j < 0 || j >= n) return -1;
-  unsigned a = key[i], b = key[j];
+  unsigned a = key[i * L], b = key[j * L];
   if (a == b) return 32 + __clz((unsigned)(i ^ j));
   return __clz(a ^ b);
 }
 
-__global__ __launch_bounds__(kB) void k_karras(const unsigned* __restrict__ key, int n,
+__global__ __launch_bounds__(kB) void k_karras(const unsigned* __restrict__ key, int n, int L,
                                                int* left, int* right, int* parent,
                                                int* leaf_parent, int* flag) {
   const int i = blockIdx.x * kB + threadIdx.x;
   if (i >= n - 1) return;
-  const int d = (delta(key, n, i, i + 1) - delta(key, n, i, i - 1)) >= 0 ? 1 : -1;
-  const int dmin = delta(key, n, i, i - d);
+  const int d = (delta(key, n, L, i, i + 1) - delta(key, n, L, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(key, n, L, i, i - d);
   int lmax = 2;
-  while (delta(key, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  while (delta(key, n, L, i, i + lmax * d) > dmin) lmax <<= 1;
   int l = 0;
   for (int t = lmax >> 1; t >= 1; t >>= 1)
-    if (delta(key, n, i, i + (l + t) * d) > dmin) l += t;
+    if (delta(key, n, L, i, i + (l + t) * d) > dmin) l += t;
   const int j = i + l * d;
-  const int dnode = delta(key, n, i, j);
+  const int dnode = delta(key, n, L, i, j);
   int s = 0;
   int t = l;
   do {
     t = (t + 1) >> 1;
-    if (delta(key, n, i, i + (s + t) * d) > dnode) s += t;
+    if (delta(key, n, L, i, i + (s + t) * d) > dnode) s += t;
   } while (t > 1);
   const int gamma = i + s * d + min(d, 0);
   const int lo = min(i, j), hi = max(i, j);
@@ -329,7 +332,23 @@ __device__ inline float4 ld_coherent(const float4* p) {
                      __hip_atomic_load(f + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0.0f);
 }
 
-// Bottom-up refit: the second thread to reach a node unions its children.
+// Box of leaf k: the union of its bodies' boxes (bodies val[k L .. k L + L)
+// clipped to nb bodies).
+__device__ inline void leaf_box(const int* __restrict__ val, const float4* __restrict__ LO,
+                                const float4* __restrict__ HI, int k, int L, int nb, float4& lo,
+                                float4& hi) {
+  const int b0 = k * L, b1 = min(b0 + L, nb);
+  lo = LO[val[b0]];
+  hi = HI[val[b0]];
+  for (int t = b0 + 1; t < b1; ++t) {
+    const float4 l = LO[val[t]], h = HI[val[t]];
+    lo = make_float4(fminf(lo.x, l.x), fminf(lo.y, l.y), fminf(lo.z, l.z), 0.f);
+    hi = make_float4(fmaxf(hi.x, h.x), fmaxf(hi.y, h.y), fmaxf(hi.z, h.z), 0.f);
+  }
+}
+
+// Bottom-up refit over n leaves of L bodies (nb bodies): the second thread
+// to reach a node unions its children.
 __global__ __launch_bounds__(kB) void k_refit(const int* __restrict__ val,
                                               const float4* __restrict__ LO,
                                               const float4* __restrict__ HI,
@@ -337,7 +356,7 @@ __global__ __launch_bounds__(kB) void k_refit(const int* __restrict__ val,
                                               const int* __restrict__ right,
                                               const int* __restrict__ parent,
                                               const int* __restrict__ leaf_parent, int* flag,
-                                              float4* nlo, float4* nhi, int n) {
+                                              float4* nlo, float4* nhi, int n, int L, int nb) {
   const int k = blockIdx.x * kB + threadIdx.x;
   if (k >= n) return;
   int node = leaf_parent[k];
@@ -347,9 +366,9 @@ __global__ __launch_bounds__(kB) void k_refit(const int* __restrict__ val,
     __threadfence();
     const int c0 = left[node], c1 = right[node];
     float4 l0, h0, l1, h1;
-    if (c0 < 0) { l0 = LO[val[~c0]]; h0 = HI[val[~c0]]; }
+    if (c0 < 0) leaf_box(val, LO, HI, ~c0, L, nb, l0, h0);
     else { l0 = ld_coherent(&nlo[c0]); h0 = ld_coherent(&nhi[c0]); }
-    if (c1 < 0) { l1 = LO[val[~c1]]; h1 = HI[val[~c1]]; }
+    if (c1 < 0) leaf_box(val, LO, HI, ~c1, L, nb, l1, h1);
     else { l1 = ld_coherent(&nlo[c1]); h1 = ld_coherent(&nhi[c1]); }
     nlo[node] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
     nhi[node] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
@@ -377,6 +396,7 @@ __device__ inline void sort_segment(int2* seg, int m) {
 // its chunk, from the ballot), i.e. exactly the sorted segments of the LBVH
 // path, in 6 launches instead of ~25.
 constexpr int kAllPairsMax = 4096;
+constexpr int kMaxLeaf = 64;  // bodies per LBVH leaf
 
 template <bool EMIT>
 __global__ __launch_bounds__(kB) void k_allpairs(const float4* __restrict__ X,
@@ -413,8 +433,8 @@ __global__ __launch_bounds__(kB) void k_bvh_query(
     const float4* __restrict__ X, const float4* __restrict__ HE, const float4* __restrict__ LO,
     const float4* __restrict__ HI, const int* __restrict__ val, const int* __restrict__ left,
     const int* __restrict__ right, const float4* __restrict__ nlo,
-    const float4* __restrict__ nhi, int n, int* __restrict__ cnt, const int* __restrict__ off,
-    int2* __restrict__ pairs, int cap) {
+    const float4* __restrict__ nhi, int n, int L, int nl, int* __restrict__ cnt,
+    const int* __restrict__ off, int2* __restrict__ pairs, int cap) {
   const int i = blockIdx.x * kB + threadIdx.x;
   if (i >= n) return;
   const float4 lo = LO[i], hi = HI[i];
@@ -424,15 +444,18 @@ __global__ __launch_bounds__(kB) void k_bvh_query(
   const int base = EMIT ? off[i] : 0;
   int stack[kStack];
   int sp = 0;
-  stack[sp++] = (n == 1) ? ~0 : 0;
+  stack[sp++] = (nl == 1) ? ~0 : 0;
   while (sp > 0) {
     const int node = stack[--sp];
-    if (node < 0) {
-      const int j = val[~node];
-      if (!overlap(lo, hi, LO[j], HI[j])) continue;
-      if (!keep_pair(i, j, imi, X[j].w, gi, __float_as_int(HE[j].w))) continue;
-      if (EMIT && base + found < cap) pairs[base + found] = make_int2(i, j);
-      ++found;
+    if (node < 0) {  // the leaf's bodies
+      const int b0 = (~node) * L, b1 = min(b0 + L, n);
+      for (int t = b0; t < b1; ++t) {
+        const int j = val[t];
+        if (!overlap(lo, hi, LO[j], HI[j])) continue;
+        if (!keep_pair(i, j, imi, X[j].w, gi, __float_as_int(HE[j].w))) continue;
+        if (EMIT && base + found < cap) pairs[base + found] = make_int2(i, j);
+        ++found;
+      }
       continue;
     }
     if (!overlap(lo, hi, nlo[node], nhi[node])) continue;
@@ -573,7 +596,15 @@ int xpbd6_body_aabb(void* stream, const float* X, const float* Q, const float* V
 int xpbd6_broadphase_lbvh(void* stream, void* ws, const float* X, const float* Q, const float* V,
                           const float* W, const float* HE, float man_margin, float dt, int n,
                           int cap, int* PAIRS, int* PC) {
+  return xpbd6_broadphase_lbvh_leaf(stream, ws, X, Q, V, W, HE, man_margin, dt, n, cap, PAIRS,
+                                    PC, n <= kAllPairsMax ? 0 : 1);
+}
+
+int xpbd6_broadphase_lbvh_leaf(void* stream, void* ws, const float* X, const float* Q,
+                               const float* V, const float* W, const float* HE, float man_margin,
+                               float dt, int n, int cap, int* PAIRS, int* PC, int leaf_size) {
   hipStream_t s = (hipStream_t)stream;
+  if (leaf_size < 0 || leaf_size > kMaxLeaf) return (int)hipErrorInvalidValue;
   if (n <= 1) {
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kB), 0, s, PC, 0, 1);
     XPBD_LAUNCH_CHECK();
@@ -586,7 +617,7 @@ int xpbd6_broadphase_lbvh(void* stream, void* ws, const float* X, const float* Q
   hipLaunchKernelGGL(k_body_aabb, dim3(nb), dim3(kB), 0, s, (const float4*)X, (const float4*)Q,
                      (const float4*)V, (const float4*)W, (const float4*)HE, man_margin, dt, w.lo,
                      w.hi, n);
-  if (n <= kAllPairsMax) {  // same pair list, fewer launches
+  if (leaf_size == 0) {  // all pairs: the same pair list, fewer launches
     const int nw = nblk(n, kB / 64);
     hipLaunchKernelGGL(k_allpairs<false>, dim3(nw), dim3(kB), 0, s, (const float4*)X,
                        (const float4*)HE, w.lo, w.hi, n, w.cnt, w.off, (int2*)PAIRS, cap);
@@ -617,19 +648,23 @@ int xpbd6_broadphase_lbvh(void* stream, void* ws, const float* X, const float* Q
     std::swap(vin, vout);
   }
   // after 4 passes the sorted keys are back in key_a / val_a
-  hipLaunchKernelGGL(k_karras, dim3(nblk(n - 1, kB)), dim3(kB), 0, s, kin, n, w.left, w.right,
-                     w.parent, w.leaf_parent, w.flag);
-  hipLaunchKernelGGL(k_refit, dim3(nb), dim3(kB), 0, s, vin, w.lo, w.hi, w.left, w.right,
-                     w.parent, w.leaf_parent, w.flag, w.nlo, w.nhi, n);
+  const int L = leaf_size;
+  const int nl = (n + L - 1) / L;  // leaves
+  if (nl > 1) {
+    hipLaunchKernelGGL(k_karras, dim3(nblk(nl - 1, kB)), dim3(kB), 0, s, kin, nl, L, w.left,
+                       w.right, w.parent, w.leaf_parent, w.flag);
+    hipLaunchKernelGGL(k_refit, dim3(nblk(nl, kB)), dim3(kB), 0, s, vin, w.lo, w.hi, w.left,
+                       w.right, w.parent, w.leaf_parent, w.flag, w.nlo, w.nhi, nl, L, n);
+  }
   hipLaunchKernelGGL(k_bvh_query<false>, dim3(nb), dim3(kB), 0, s, (const float4*)X,
-                     (const float4*)HE, w.lo, w.hi, vin, w.left, w.right, w.nlo, w.nhi, n, w.cnt,
-                     w.off, (int2*)PAIRS, cap);
+                     (const float4*)HE, w.lo, w.hi, vin, w.left, w.right, w.nlo, w.nhi, n, L, nl,
+                     w.cnt, w.off, (int2*)PAIRS, cap);
   XPBD_LAUNCH_CHECK();
   int rc = exclusive_scan(s, w.cnt, w.off, n, w.scan, PC);
   if (rc) return rc;
   hipLaunchKernelGGL(k_bvh_query<true>, dim3(nb), dim3(kB), 0, s, (const float4*)X,
-                     (const float4*)HE, w.lo, w.hi, vin, w.left, w.right, w.nlo, w.nhi, n, w.cnt,
-                     w.off, (int2*)PAIRS, cap);
+                     (const float4*)HE, w.lo, w.hi, vin, w.left, w.right, w.nlo, w.nhi, n, L, nl,
+                     w.cnt, w.off, (int2*)PAIRS, cap);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

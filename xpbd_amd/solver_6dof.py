@@ -113,6 +113,10 @@ class Solver6DOF:
         # "auto": whole frames of LBVH scenes up to BLOCK_AUTO bodies in one
         # workgroup (xpbd6_frame_block); True / False force it on / off
         self.block_frames = "auto"
+        # LBVH broad phase: bodies per leaf (1..64), 0 = all pairs, None = the
+        # default (all pairs up to 4096 bodies, one body per leaf above);
+        # xpbd_amd.autotune.tune6 picks it by timing
+        self.lbvh_leaf = None
         self.cap = 0
         self._cur_margin = 0.0
 
@@ -381,7 +385,12 @@ class Solver6DOF:
         P = N.ptr
         lib = N.lib()
         s = N.current_stream()
-        if self.broadphase == "lbvh":
+        if self.broadphase == "lbvh" and self.lbvh_leaf is not None:
+            rc = lib.xpbd6_broadphase_lbvh_leaf(s, P(b["WS"]), P(b["X"]), P(b["Q"]), P(b["V"]),
+                                                P(b["W"]), P(b["HE"]), self._man_margin, self.dt,
+                                                n, cap, P(b["PAIRS"]), P(b["PC"]),
+                                                int(self.lbvh_leaf))
+        elif self.broadphase == "lbvh":
             rc = lib.xpbd6_broadphase_lbvh(s, P(b["WS"]), P(b["X"]), P(b["Q"]), P(b["V"]),
                                            P(b["W"]), P(b["HE"]), self._man_margin, self.dt, n,
                                            cap, P(b["PAIRS"]), P(b["PC"]))

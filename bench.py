@@ -246,8 +246,8 @@ def other_workloads(frames: int) -> dict:
                                                               for k, v in leaf_times.items()}}
     spec = draw(1024, 0)
     del g, c, u
-    # fresh allocations for the 1.65 M-body world: built on memory the
-    # caching allocator kept from the smaller scenes above it ran 25 % slower
+    # release the allocator's cached blocks first: inside this process (after
+    # the headline section) the world otherwise ran ~25 % slower
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     w = BatchedWorld(spec, device="cuda:0")

@@ -67,7 +67,7 @@ __device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
 // The network in LDS, bfloat16 weights stored transposed (output-major:
 // w1t[n][k] = W1[k][n] with k padded to 16, w2t[n][k] = W2[k][n]) so that a
 // lane's four B-operand values are contiguous (one 8-byte LDS read).
-struct Weights {
+struct alignas(16) Weights {
   unsigned short w1t[NH][16];
   unsigned short w2t[NH][NH];
   float b1[NH], b2[NH], w3[NH];
@@ -76,7 +76,7 @@ static_assert(sizeof(Weights) % 16 == 0, "weights copied in 16-byte pieces");
 
 // The network as one packed image (Weights followed by b3), built once from
 // theta by xpbd_mlp_pack and copied into LDS with 16-byte loads.
-struct Image {
+struct alignas(16) Image {
   Weights w;
   float b3, pad[3];
 };

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kB) void k_ws_features(const float4* __restrict__ X
 constexpr int kNetWaves = 4, kNetBlock = 64 * kNetWaves, kNetMaxBlocks = 2048;
 
 template <int ROWS>
-struct NetLds {
+struct alignas(16) NetLds {
   float b1[xpbd_mlp::NH], b2[xpbd_mlp::NH], w3[xpbd_mlp::NH];
   unsigned short ftile[kNetWaves][ROWS][16];
 };

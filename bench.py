@@ -280,11 +280,12 @@ def reference_rows(frames: int = 120, warmup: int = 20) -> dict:
 
     rows = {}
 
-    def put(name, mode, s, ms):
-        proj = s.num_constraints * s.substeps * s.iterations
+    def put(name, mode, s, ms, key=None):
+        n_rows = s.num_constraints + int(getattr(s, "n_contacts", 0) or 0)  # + device rows
+        proj = n_rows * s.substeps * s.iterations
         pub = PUBLISHED_MS.get((name, mode))
-        rows[f"{name}_{mode}"] = {
-            "ms_per_frame": round(ms, 4), "bodies": s.num_bodies, "rows": s.num_constraints,
+        rows[key or f"{name}_{mode}"] = {
+            "ms_per_frame": round(ms, 4), "bodies": s.num_bodies, "rows": n_rows,
             "proj_per_s": proj / (ms * 1e-3), "published_rtx3060_ms": pub,
             "speedup_vs_published": None if pub is None else round(pub / ms, 1)}
 
@@ -294,6 +295,14 @@ def reference_rows(frames: int = 120, warmup: int = 20) -> dict:
             put(f"cloth{res}", mode, s, time_frames(s.step, frames, warmup))
         s = sphere_stack(Solver, 8, 5, mode, device="cuda:0")
         put("stack8x8x5", mode, s, time_frames(s.step, frames, warmup))
+    # the same Jacobi stack with its CONTACT rows generated on the device
+    # (the host path's rows, in its order) instead of the per-frame host
+    # broad phase and re-upload
+    s = sphere_stack(Solver, 8, 5, "jacobi", device="cuda:0")
+    s.enable_self_collision(False)
+    s.enable_device_contacts(True, margin=0.0)
+    put("stack8x8x5", "jacobi", s, time_frames(s.step, frames, warmup),
+        key="stack8x8x5_jacobi_device_contacts")
     for dims in ((3, 3, 3), (5, 4, 5), (8, 6, 8), (12, 10, 12)):
         s = box_pile(Solver6DOF, *dims, device="cuda:0", broadphase="lbvh")
         n = dims[0] * dims[1] * dims[2]

@@ -947,6 +947,7 @@ __global__ __launch_bounds__(kFB) void k6_frame_block(xpbd_rigid_desc d, float4*
   __shared__ int scnt[kFBMax + 1];
   __shared__ float4 sb[12][kFBMax];
   __shared__ float slf[8 * kFBMax];
+  __shared__ float4 fslot[4 * kFB];  // floor-corner staging (4 float4 per lane)
   const int tid = threadIdx.x;
   const int n = d.n;
   float4* const G[12] = {F4(d.x),      F4(d.q),       F4(d.v),     F4(d.w),
@@ -1038,15 +1039,37 @@ __global__ __launch_bounds__(kFB) void k6_frame_block(xpbd_rigid_desc d, float4*
       fb_sync();
       const int npts = min(ld_count(d.pt_count), maxp);
       for (int it = 0; it < d.iterations; ++it) {
-        for (int k = tid; k < d.nj + n + npts; k += kFB) {
-          if (k < d.nj)
+        // joints, floor corners (one per lane, as k6_solve_all), box points;
+        // the stride is a multiple of 64, so corner groups stay in a wave.
+        // Corners per lane only while everything fits one pass of the
+        // workgroup (a second pass costs more than the per-body loop saves)
+        const int f0 = floor_lane0(d.nj);
+        if (f0 + 8 * n + npts > kFB) {
+          for (int k = tid; k < d.nj + n + npts; k += kFB) {
+            if (k < d.nj)
+              solve_joints_elem(k, X, Q, IB, JI, JA, JB, d.lam_j, d.h, it == 0, DX, DR, d.nj);
+            else if (k < d.nj + n)
+              solve_floor_elem<true>(k - d.nj, X, Q, XP, QP, IB, HE, d.floor_y, d.friction,
+                                     LAMF, DX, DR, n);
+            else
+              solve_box_elem(k - d.nj - n, X, Q, IB, PTI, PTN, PTOI, PTOR, d.lam_pt,
+                             d.pt_count, DX, DR, maxp);
+          }
+        } else {
+        for (int k = tid; k < f0 + 8 * n + npts; k += kFB) {
+          if (k < d.nj) {
             solve_joints_elem(k, X, Q, IB, JI, JA, JB, d.lam_j, d.h, it == 0, DX, DR, d.nj);
-          else if (k < d.nj + n)
-            solve_floor_elem<true>(k - d.nj, X, Q, XP, QP, IB, HE, d.floor_y, d.friction,
-                                   LAMF, DX, DR, n);
-          else
-            solve_box_elem(k - d.nj - n, X, Q, IB, PTI, PTN, PTOI, PTOR, d.lam_pt, d.pt_count,
-                           DX, DR, maxp);
+          } else if (k < f0) {
+            continue;
+          } else if (k < f0 + 8 * n) {
+            const int t = k - f0;
+            solve_floor_corner(t >> 3, t & 7, X, Q, XP, QP, IB, HE, d.floor_y, d.friction, LAMF,
+                               DX, DR, fslot);
+          } else {
+            solve_box_elem(k - f0 - 8 * n, X, Q, IB, PTI, PTN, PTOI, PTOR, d.lam_pt,
+                           d.pt_count, DX, DR, maxp);
+          }
+        }
         }
         fb_sync();
         for (int i = tid; i < n; i += kFB) {
@@ -1061,13 +1084,24 @@ __global__ __launch_bounds__(kFB) void k6_frame_block(xpbd_rigid_desc d, float4*
           finalize_elem(i, X, Q, XP, QP, V, W, d.h, d.lin_damp, d.ang_damp, n);
         fb_sync();
       }
-      for (int k = tid; k < n + npts; k += kFB) {
-        if (k < n)
-          velocity_floor_elem<true>(k, X, Q, IB, HE, d.friction, d.restitution, LAMF, d.h,
-                                    V, W, DV, DW, n);
-        else
-          velocity_box_elem(k - n, X, Q, IB, PTI, PTN, PTOI, PTOR, d.lam_pt, d.pt_count,
-                            d.friction, d.restitution, d.h, V, W, DV, DW, maxp);
+      if (8 * n + npts > kFB) {  // as the solve phase: corners only in one pass
+        for (int k = tid; k < n + npts; k += kFB) {
+          if (k < n)
+            velocity_floor_elem<true>(k, X, Q, IB, HE, d.friction, d.restitution, LAMF, d.h,
+                                      V, W, DV, DW, n);
+          else
+            velocity_box_elem(k - n, X, Q, IB, PTI, PTN, PTOI, PTOR, d.lam_pt, d.pt_count,
+                              d.friction, d.restitution, d.h, V, W, DV, DW, maxp);
+        }
+      } else {
+        for (int k = tid; k < 8 * n + npts; k += kFB) {
+          if (k < 8 * n)
+            velocity_floor_corner(k >> 3, k & 7, X, Q, IB, HE, d.friction, d.restitution,
+                                  LAMF, d.h, V, W, DV, DW, fslot);
+          else
+            velocity_box_elem(k - 8 * n, X, Q, IB, PTI, PTN, PTOI, PTOR, d.lam_pt,
+                              d.pt_count, d.friction, d.restitution, d.h, V, W, DV, DW, maxp);
+        }
       }
       fb_sync();
     }

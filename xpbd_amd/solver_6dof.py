@@ -284,7 +284,9 @@ class Solver6DOF:
 
     # one workgroup runs the whole frame up to this many bodies ("auto"):
     # above it the per-launch plan's 256 CUs win the manifold and solve work
-    BLOCK_AUTO = 256
+    # (box piles, ms per frame, block against launch: 100 bodies 0.50 / 0.67,
+    # 150: 0.67 / 0.71, 180: 0.78 / 0.71, 256: 1.03 / 0.71)
+    BLOCK_AUTO = 160
 
     @property
     def n_pairs(self) -> int:

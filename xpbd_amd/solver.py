@@ -788,8 +788,9 @@ class Solver:
                 str(self.block_frames), int(self.warm_start_every))
 
     # "auto" limit of whole-frame workgroups in Jacobi mode: above it the
-    # tile path's 256 CUs beat one CU (measured: 256 particles 0.055 against
-    # 0.117 ms per frame, 1024 particles 0.132 against 0.122 ms)
+    # tile path's 256 CUs beat one CU (cloths, ms per frame, workgroup
+    # against tiles: 256 particles 0.055 / 0.109, 729: 0.100 / 0.116, 1024:
+    # 0.132 / 0.113)
     BLOCK_AUTO_JACOBI = 768
 
     def _use_block(self) -> bool:

@@ -1,7 +1,7 @@
 // Learned lambda warm start for the 3-DOF solver (no reference
 // counterpart; it replaces the zero reset of solver.py:431 / Alg. 1 line 4):
 // the row features, the network pass of a substep (features + MLP -> y per
-// row) and the residual the warm start is judged by.  The start itself is
+// row) and the residual the warm start is measured by.  The start itself is
 // applied by the warm-start tile launch (k_jacobi_tile_warm, particles.hip)
 // from y.  Features and lambda0: ws_dev.h; the network: mlp_dev.h.
 #include <cmath>

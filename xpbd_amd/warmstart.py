@@ -24,7 +24,7 @@ lambda0.  The substep's Jacobi iterations then continue from lambda0
 * ``Evaluator`` runs the plain zero start and the learned start on their
   own trajectories and measures the RMS constraint residual after each
   substep's iterations;
-* ``Guard``    keeps the plain solver as fallback and judge in deployment.
+* ``Guard``    keeps the plain solver as fallback and yardstick in deployment.
 
 Measured on cloth: the previous-substep lambda feature (slot 7, left at
 zero) feeds every warm-started substep's multipliers into the next
@@ -394,7 +394,7 @@ class Evaluator:
 
 
 class Guard:
-    """The learned start with the plain zero start as fallback and judge.
+    """The learned start with the plain zero start as fallback and yardstick.
 
     Every ``check_every`` frames one substep is replayed both ways from the
     current state (``iters`` Jacobi iterations each, eager launches, state

@@ -154,21 +154,25 @@ def test_morton_codes_decode(oracle):
     assert lib.xpbd_morton3(1.0, 1.0, 1.0) == (1 << 30) - 1
 
 
-def _pile_scene(seed=0):
+def _pile_scene(seed=0, gyroscopic=False):
+    """60 random boxes in three layers (mu 0.5, e 0.2) plus a 5-link joint
+    chain; with ``gyroscopic`` the boxes start spinning at up to 4 rad/s so
+    the gyroscopic term of integrate_bodies is large."""
     from xpbd_amd import Solver6DOF
     from xpbd_amd.scenes import _random_quats
 
     rng = np.random.default_rng(seed)
     s = Solver6DOF(dt=1 / 60, substeps=8, iterations=2, device=DEV, friction=0.5,
-                   restitution=0.2)
+                   restitution=0.2, gyroscopic=gyroscopic)
     q = _random_quats(rng, 60)
     k = 0
     for iy in range(3):
         for ix in range(5):
             for iz in range(4):
                 he = rng.uniform(0.1, 0.25, 3)
+                w = tuple(rng.uniform(-4, 4, 3)) if gyroscopic else (0.0, 0.0, 0.0)
                 s.add_box((ix * 0.55, 0.3 + iy * 0.5, iz * 0.55), tuple(he),
-                          mass=float(8000 * he.prod()), quaternion=tuple(q[k]))
+                          mass=float(8000 * he.prod()), quaternion=tuple(q[k]), omega=w)
                 k += 1
     a = s.add_box((4.0, 3.0, 0.0), (0.05, 0.05, 0.05), mass=0.0, static=True)
     prev = a
@@ -179,16 +183,18 @@ def _pile_scene(seed=0):
     return s
 
 
-@pytest.mark.parametrize("block", [False, True])
-def test_rigid_substep_parity_and_contact_counts(oracle, block):
+@pytest.mark.parametrize("block,gyro", [(False, False), (True, False), (False, True),
+                                        (True, True)])
+def test_rigid_substep_parity_and_contact_counts(oracle, block, gyro):
     """From identical inputs (the oracle's trajectory), one substep on the
     device matches the oracle: pair lists and per-pair contact counts
     bit-exact, x/q/v/w within 1e-4; through the per-launch plan and through
-    the whole-frame workgroup kernel."""
-    s_full = _pile_scene()
+    the whole-frame workgroup kernel, with and without the gyroscopic term
+    (kernels_6dof.py:91-101)."""
+    s_full = _pile_scene(gyroscopic=gyro)
     orc = _oracle6(oracle, s_full)
     h = float(np.float32(1 / 60 / 8))
-    s = _pile_scene()
+    s = _pile_scene(gyroscopic=gyro)
     s.block_frames = block
     s.dt, s.substeps = h, 1
     s._flush()
@@ -481,9 +487,12 @@ def test_domino_cascade():
 
 
 def test_unified_config3_runs_and_settles_contacts(oracle):
-    """Config 3 (1440 boxes + 32^2 cloth + 20-link chain, LBVH, 15x2): frames
-    run through the captured graph, stay finite, and the first frame's pair
-    list and contact counts equal the oracle's."""
+    """Config 3 (1440 boxes + 32^2 cloth + 20-link chain, LBVH, 15x2): the
+    first frame's pair list equals the oracle's; frames run through the
+    captured graph and stay finite; then, once the boxes have landed (frames
+    40, 70 and 100), one substep from the device's state on a one-substep
+    copy of the scene and in the oracle from the same state: pair lists and
+    per-pair manifold contact counts bit-exact, x/q/v/w within 1e-4."""
     from xpbd_amd import Solver6DOF
     from xpbd_amd.scenes import unified_scene
 
@@ -493,8 +502,34 @@ def test_unified_config3_runs_and_settles_contacts(oracle):
     orc.step(1)
     want = np.stack([orc.pair_a[:orc.n_pairs], orc.pair_b[:orc.n_pairs]], axis=1)
     assert s.n_pairs == orc.n_pairs and (s.pairs() == want).all()
-    for _ in range(60):
-        s.step()
-    assert np.all(np.isfinite(s.positions()))
+    h = float(np.float32(s.dt / s.substeps))
+    one = unified_scene(Solver6DOF, device=DEV, dt=h, substeps=1)
+    one._flush()
+    ref = _oracle6(oracle, one)
+    done, points = 1, 0
+    for target in (40, 70, 100):
+        while done < target:
+            s.step()
+            done += 1
+        assert np.all(np.isfinite(s.positions()))
+        st = {k: s._buf[k].clone() for k in ("X", "Q", "V", "W")}
+        for k in ("X", "Q", "V", "W"):
+            one._buf[k].copy_(st[k])
+        ref.x[:] = st["X"][:, :3].cpu().numpy()
+        ref.q[:] = st["Q"].cpu().numpy()
+        ref.v[:] = st["V"][:, :3].cpu().numpy()
+        ref.w[:] = st["W"][:, :3].cpu().numpy()
+        one.step()
+        ref.step(1)
+        assert one.n_pairs == ref.n_pairs, target
+        want = np.stack([ref.pair_a[:ref.n_pairs], ref.pair_b[:ref.n_pairs]], axis=1)
+        assert (one.pairs() == want).all(), target
+        mc = one.contact_counts()
+        assert (mc == ref.m_count[:ref.n_pairs]).all(), target
+        points += int(mc.sum())
+        errs = [_rel(one.positions(), ref.x), _rel(one.orientations(), ref.q),
+                _rel(one.velocities(), ref.v), _rel(one.omega.numpy(), ref.w)]
+        assert max(errs) <= 1e-4, (target, errs)
+    assert points > 500
     print(f"config3: {s.num_bodies} bodies, {s.num_joints} joints, {s.n_pairs} pairs, "
-          f"{int(s.contact_counts().sum())} manifold points")
+          f"{points} manifold points over the three checks")

@@ -440,6 +440,7 @@ class Solver:
             fr = self._friction.array().copy()
             fr[fr == 0.0] = self._default_friction
             self._friction.set(fr)
+            self._refresh_contact_friction()
         if enabled and self._contact_pool_start is None:
             self._contact_pool_start = len(self._rt)
 
@@ -478,9 +479,10 @@ class Solver:
     def enable_device_contacts(self, enabled=True, margin=0.0) -> None:
         """Sphere-sphere CONTACT rows generated on the GPU every frame (an
         extension; the reference rebuilds them on the host, solver.py:275-342).
-        Same margin, cell and filter as the host path; pairs already joined
-        by a constraint are not excluded, so use it for scenes whose
-        colliding particles carry no two-body rows.  Jacobi tile mode only."""
+        Same margin, cell and filter as the host path, same rows in the same
+        order; pairs already joined by a two-body constraint are skipped, as
+        the host path skips them (solver.py:285-290, 336-337).  Jacobi tile
+        mode only."""
         if enabled and (self.solve_mode != "jacobi" or self.jacobi_kernel != "tile"):
             raise ValueError("device contacts need solve_mode='jacobi' with the tile kernel")
         self._device_contacts = bool(enabled)
@@ -525,6 +527,9 @@ class Solver:
         self._repack_rows_inplace()
 
     def set_all_friction(self, mu: float) -> None:
+        """Live friction update (solver.py:261-271): every particle's
+        friction, the FLOOR/CONTACT rows and the per-particle friction the
+        device CONTACT rows are generated from, all rewritten in place."""
         self._flush()
         self._default_friction = float(mu)
         self._friction.set(np.full(len(self._friction), float(mu)))
@@ -533,6 +538,17 @@ class Solver:
         rmu[(t == FLOOR) | (t == CONTACT)] = float(mu)
         self._rmu.set(rmu)
         self._repack_rows_inplace()
+        self._refresh_contact_friction()
+
+    def _refresh_contact_friction(self) -> None:
+        """Per-particle friction of the device CONTACT generator (device
+        order), rewritten in place so a captured plan stays valid."""
+        if getattr(self, "CMU", None) is None or self._dirty:
+            return
+        fr = self._friction.array().astype(np.float32)
+        if self._pperm is not None:
+            fr = fr[self._pperm]
+        self.CMU.copy_(self._t(np.ascontiguousarray(fr)))
 
     # ---- host broad phase (solver.py:275-342) ------------------------------
     def _rebuild_contacts(self) -> None:
@@ -699,7 +715,7 @@ class Solver:
                 raise ValueError("gravity scale length != number of particles")
             g = self._gscale if pperm is None else self._gscale[pperm]
             self.GS = torch.from_numpy(np.ascontiguousarray(g)).to(dev)
-        self.CROWS = None
+        self.CROWS = self.CMU = None
         if self._device_contacts:
             rad = self._radius.array().astype(np.float32)
             col = self._collide.array().astype(bool)

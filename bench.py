@@ -236,7 +236,7 @@ def other_workloads(frames: int) -> dict:
     # reach their steady size (a growth re-captures the frame graph)
     ms = time_frames(u.step, max(frames // 4, 10), 30)
     rows = u.num_joints + int(u.contact_counts().sum()) + 8 * int(
-        sum(1 for m in u._inv_mass if m > 0))
+        int((u.inv_masses() > 0).sum()))
     out["unified_1440_boxes"] = {"ms_per_frame": round(ms, 4), "bodies": u.num_bodies,
                                  "joints": u.num_joints, "pairs": u.n_pairs,
                                  "manifold_points": int(u.contact_counts().sum()),

@@ -373,3 +373,108 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+# ---- rigid host scene (xpbd_amd/solver_6dof.py) ---------------------------
+def _textbook_rot(q):
+    x, y, z, w = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_rigid_host_geometry_helpers():
+    """Rotation matrices against the textbook quaternion matrix; world-AABB
+    half extents |R| he; solid-box inverse inertia 12 / (m (l_j^2 + l_k^2))."""
+    from xpbd_amd.solver_6dof import rotation_matrices, solid_box_inverse_inertia, world_aabb_half
+
+    rng = np.random.default_rng(4)
+    q = rng.normal(size=(20, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    R = rotation_matrices(q)
+    for i in range(20):
+        np.testing.assert_allclose(R[i], _textbook_rot(q[i]), atol=1e-12)
+        np.testing.assert_allclose(R[i] @ R[i].T, np.eye(3), atol=1e-12)
+    he = rng.uniform(0.05, 0.3, (20, 3))
+    half = world_aabb_half(q, he)
+    np.testing.assert_allclose(half, np.einsum("nij,nj->ni", np.abs(R), he), rtol=2e-6)
+    m = rng.uniform(0.5, 3.0, 20)
+    l = 2 * he
+    want = np.stack([12 / (m * (l[:, 1] ** 2 + l[:, 2] ** 2)), 12 / (m * (l[:, 0] ** 2 + l[:, 2] ** 2)),
+                     12 / (m * (l[:, 0] ** 2 + l[:, 1] ** 2))], 1)
+    np.testing.assert_allclose(solid_box_inverse_inertia(he, m), want, rtol=1e-13)
+
+
+def test_rigid_bulk_and_single_adds_build_the_same_scene():
+    """add_boxes / add_joints (bulk) and add_box / add_particle / add_joint
+    (one at a time) give identical host arrays, handles and flags: static
+    for mass 0, no rotational inertia for point masses, rest length = the
+    anchors' current separation."""
+    from xpbd_amd import Solver6DOF
+
+    rng = np.random.default_rng(1)
+    n = 12
+    x = rng.normal(size=(n, 3))
+    he = rng.uniform(0.1, 0.3, (n, 3))
+    m = rng.uniform(0.5, 2.0, n)
+    m[3] = 0.0
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    a = Solver6DOF(device="cuda:0")
+    for i in range(n):
+        a.add_box(tuple(x[i]), tuple(he[i]), float(m[i]), quaternion=tuple(q[i]))
+    p = a.add_particle((0, 2, 0), 0.1, radius=0.02)
+    a.add_joint(a.bodies[0], p, compliance=1e-5, anchor_a=(0.1, 0, 0))
+    a.add_joint(1, 2, rest_length=0.3)
+    b = Solver6DOF(device="cuda:0")
+    b.add_boxes(x, he, m, quaternions=q)
+    b.add_boxes([(0, 2, 0)], 0.02, 0.1, particle=True, group=1, color=(0.85, 0.3, 0.35))
+    b.add_joints([0], [n], compliance=1e-5, anchor_a=(0.1, 0, 0))
+    b.add_joints([1], [2], rest_length=0.3)
+    ha, hb = a.host_arrays(), b.host_arrays()
+    for k in ha:
+        assert np.array_equal(ha[k], hb[k]), k
+    assert a.bodies == b.bodies
+    assert a.bodies[3].static and ha["inv_mass"][3] == 0 and not ha["inv_I"][3].any()
+    assert a.bodies[n].is_particle and not ha["inv_I"][n].any() and ha["group"][n] == 1
+    rest = np.linalg.norm((x[0] + [0.1, 0, 0]) - np.array([0, 2, 0]))
+    assert ha["j_rest"][0] == np.float32(rest)
+    assert (a.num_bodies, a.num_joints) == (n + 1, 2)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_host_grid_broadphase_pair_set_equals_brute_force(seed):
+    """The host ``grid`` broad phase (solver_6dof.py:232-258) finds exactly
+    the pairs a brute-force test over all pairs keeps: per-axis overlap of
+    the float32 world AABBs grown by the one global margin, not both static,
+    not the same collision group > 0; each pair once."""
+    from xpbd_amd.solver_6dof import host_grid_pairs, world_aabb_half
+
+    rng = np.random.default_rng(seed)
+    n = 400
+    x = rng.uniform(-2, 2, (n, 3)).astype(np.float32)
+    q = rng.normal(size=(n, 4))
+    q = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+    he = rng.uniform(0.05, 0.25, (n, 3)).astype(np.float32)
+    v = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    w = rng.normal(0, 2, (n, 3)).astype(np.float32)
+    inv_m = np.where(rng.random(n) < 0.1, 0.0, 1.0).astype(np.float32)
+    grp = np.where(rng.random(n) < 0.3, rng.integers(1, 3, n), 0).astype(np.int32)
+    a, b, margin = host_grid_pairs(x, q, he, v, w, inv_m, grp, 1 / 60, 0.02)
+    half = world_aabb_half(q, he)
+    want = set()
+    for i in range(n):
+        for j in range(i + 1, n):
+            if inv_m[i] == 0 and inv_m[j] == 0:
+                continue
+            if grp[i] == grp[j] and grp[i] > 0:
+                continue
+            if np.all(np.abs(x[i] - x[j]) - (half[i] + half[j] + np.float32(margin)) <= 0):
+                want.add((i, j))
+    got = [(min(i, j), max(i, j)) for i, j in zip(a.tolist(), b.tolist())]
+    assert len(got) == len(set(got))
+    assert set(got) == want and len(want) > 50
+    vmax, wmax = float(np.abs(v).max()), float(np.abs(w).max())
+    assert margin == pytest.approx(min(0.02 + 2 / 60 * (vmax + wmax * float(
+        np.linalg.norm(he, axis=1).max())), float(half.max())))

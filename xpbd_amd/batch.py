@@ -196,7 +196,7 @@ class BatchedWorld:
         per dynamic body)."""
         n = self.particles.num_constraints + getattr(self.particles, "n_contacts", 0)
         if self.rigid is not None:
-            dyn = int(sum(1 for m in self.rigid._inv_mass if m > 0))
+            dyn = int((self.rigid.inv_masses() > 0).sum())
             pts = int(self.rigid.contact_counts().sum()) if self.rigid._buf is not None else 0
             n += self.rigid.num_joints + pts + 8 * dyn
         return n

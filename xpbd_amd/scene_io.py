@@ -87,6 +87,7 @@ def save(solver, path) -> None:
             d[k] = h[k]
         d["static"] = np.asarray([b.static for b in solver.bodies], bool)
         d["particle"] = np.asarray([b.is_particle for b in solver.bodies], bool)
+        d["color"] = np.asarray([b.color for b in solver.bodies], np.float64).reshape(-1, 3)
         if solver._gscale is not None:
             d["gscale"] = np.asarray(solver._gscale, np.float32)
     else:
@@ -143,7 +144,7 @@ def _load3(d, device):
 
 
 def _load6(d, device):
-    from .solver_6dof import RigidBody, Solver6DOF
+    from .solver_6dof import _POINT, _STATIC, Solver6DOF
 
     s = Solver6DOF(dt=float(d["meta_dt"]), substeps=int(d["meta_substeps"]),
                    iterations=int(d["meta_iterations"]),
@@ -154,25 +155,15 @@ def _load6(d, device):
                    jacobi_relax=float(d["meta_jacobi_relax"]), device=device,
                    max_speed=float(d["meta_max_speed"]), max_omega=float(d["meta_max_omega"]),
                    broadphase=str(d["meta_broadphase"]))
-    n = d["x"].shape[0]
-    s._x = [tuple(map(float, r)) for r in d["x"]]
-    s._q = [tuple(map(float, r)) for r in d["q"]]
-    s._v = [tuple(map(float, r)) for r in d["v"]]
-    s._w = [tuple(map(float, r)) for r in d["w"]]
-    s._inv_mass = [float(m) for m in d["inv_mass"]]
-    s._inv_I = [tuple(map(float, r)) for r in d["inv_I"]]
-    s._he = [tuple(map(float, r)) for r in d["he"]]
-    s._group = [int(g) for g in d["group"]]
-    s.bodies = [RigidBody(index=i, half_extents=s._he[i], static=bool(d["static"][i]),
-                          is_particle=bool(d["particle"][i]), group=s._group[i])
-                for i in range(n)]
-    s._j_a = [int(v) for v in d["j_a"]]
-    s._j_b = [int(v) for v in d["j_b"]]
-    s._j_anchor_a = [tuple(map(float, r)) for r in d["j_anchor_a"]]
-    s._j_anchor_b = [tuple(map(float, r)) for r in d["j_anchor_b"]]
-    s._j_rest = [float(v) for v in d["j_rest"]]
-    s._j_compliance = [float(v) for v in d["j_compliance"]]
-    s.n_joints = len(s._j_a)
+    # bodies as stored (inverse masses and inertias taken from the file,
+    # not recomputed), then the joints with their stored rest lengths
+    kind = (np.where(d["static"], _STATIC, 0) | np.where(d["particle"], _POINT, 0))
+    color = d["color"] if "color" in d else np.full((d["x"].shape[0], 3), 0.6)
+    s._scene.append_bodies(dict(x=d["x"], q=d["q"], v=d["v"], w=d["w"],
+                                inv_mass=d["inv_mass"], inv_I=d["inv_I"], he=d["he"],
+                                color=color), d["group"], kind.astype(np.int8))
+    s.add_joints(d["j_a"], d["j_b"], compliance=d["j_compliance"], rest_length=d["j_rest"],
+                 anchor_a=d["j_anchor_a"], anchor_b=d["j_anchor_b"])
     if "gscale" in d:
         s.set_gravity_scale(d["gscale"])
     s._dirty = True

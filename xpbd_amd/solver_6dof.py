@@ -17,12 +17,14 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native as N
-from .solver import _resolve_device, grid_candidate_pairs
+from .solver import _Col, _resolve_device, grid_candidate_pairs
 
 
 @dataclass
 class RigidBody:
-    """Handle of a body (solver_6dof.py:43-50)."""
+    """Handle of one body, as ``xpbd3d.RigidBody`` (solver_6dof.py:43-50):
+    its index, half extents, display colour, and whether it is static, a
+    point mass, and its collision group."""
 
     index: int
     half_extents: tuple
@@ -32,27 +34,69 @@ class RigidBody:
     group: int = 0
 
 
-def _world_aabb_half(q, he):
-    """|R| he per box for (n,4) xyzw quaternions and (n,3) half extents
-    (solver_6dof.py:53-67)."""
-    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
-    xx, yy, zz = x * x, y * y, z * z
-    xy, xz, yz = x * y, x * z, y * z
-    wx, wy, wz = w * x, w * y, w * z
-    r = np.abs(np.stack([
-        np.stack([1 - 2 * (yy + zz), 2 * (xy - wz), 2 * (xz + wy)], 1),
-        np.stack([2 * (xy + wz), 1 - 2 * (xx + zz), 2 * (yz - wx)], 1),
-        np.stack([2 * (xz - wy), 2 * (yz + wx), 1 - 2 * (xx + yy)], 1)], 1))
-    return np.einsum("nij,nj->ni", r, he).astype(np.float32)
+# bit flags of the per-body ``kind`` column
+_STATIC = 1
+_POINT = 2
 
 
-def _box_inv_inertia(he, mass):
-    """Diagonal inverse inertia of a solid box (solver_6dof.py:70-76)."""
-    lx, ly, lz = 2.0 * he[0], 2.0 * he[1], 2.0 * he[2]
-    ix = mass / 12.0 * (ly * ly + lz * lz)
-    iy = mass / 12.0 * (lx * lx + lz * lz)
-    iz = mass / 12.0 * (lx * lx + ly * ly)
-    return (1.0 / ix, 1.0 / iy, 1.0 / iz)
+def solid_box_inverse_inertia(half_extents, mass) -> np.ndarray:
+    """Body-frame diagonal inverse inertia of solid boxes, (n, 3): with edge
+    lengths l = 2 he, I_k = m (|l|^2 - l_k^2) / 12 (the two other edges)."""
+    he = np.asarray(half_extents, np.float64).reshape(-1, 3)
+    m = np.asarray(mass, np.float64).reshape(-1, 1)
+    l2 = (2.0 * he) ** 2
+    return 1.0 / (m / 12.0 * (l2.sum(axis=1, keepdims=True) - l2))
+
+
+def rotation_matrices(q) -> np.ndarray:
+    """Rotation matrices (n, 3, 3) of unit quaternions (n, 4) stored x, y, z, w."""
+    q = np.asarray(q).reshape(-1, 4)
+    v, w = q[:, :3], q[:, 3]
+    # R = (w^2 - |v|^2) I + 2 v v^T + 2 w [v]x
+    outer = 2.0 * v[:, :, None] * v[:, None, :]
+    diag = (w * w - (v * v).sum(axis=1))[:, None, None] * np.eye(3, dtype=q.dtype)
+    x, y, z = v[:, 0], v[:, 1], v[:, 2]
+    zero = np.zeros_like(x)
+    skew = np.stack([np.stack([zero, -z, y], -1), np.stack([z, zero, -x], -1),
+                     np.stack([-y, x, zero], -1)], 1)
+    return diag + outer + 2.0 * w[:, None, None] * skew
+
+
+def world_aabb_half(q, he) -> np.ndarray:
+    """World-AABB half extents |R| he of oriented boxes, float32 (n, 3)."""
+    R = rotation_matrices(np.asarray(q, np.float32))
+    return (np.abs(R) @ np.asarray(he, np.float32).reshape(-1, 3, 1))[:, :, 0].astype(np.float32)
+
+
+def host_grid_pairs(x, q, he, v, w, inv_mass, group, dt, man_margin):
+    """Candidate pairs of the host ``grid`` broad phase (solver_6dof.py:232-258):
+    one global margin m = min(man_margin + 2 dt (max|v| + max|w| max|he|),
+    r_max) over the float32 world AABBs, a uniform grid of cell 2 r_max + m
+    enumerated as the 3-DOF contact grid is, then the per-axis overlap test
+    |dx| <= a_i + a_j + m and the filters (not both static, not the same
+    collision group > 0).  Returns (a, b, m)."""
+    x = np.asarray(x, np.float32).reshape(-1, 3)
+    n = x.shape[0]
+    none = np.zeros(0, np.int32)
+    if n < 2:
+        return none, none, 0.0
+    he = np.asarray(he, np.float32).reshape(-1, 3)
+    half = world_aabb_half(q, he)
+    r_max = float(half.max())
+    speed = float(np.abs(np.asarray(v, np.float32)).max())
+    spin = float(np.abs(np.asarray(w, np.float32)).max())
+    reach = float(np.linalg.norm(he, axis=1).max())
+    margin = min(man_margin + 2.0 * dt * (speed + spin * reach), r_max)
+    ca, cb = grid_candidate_pairs(x, 2.0 * r_max + margin)
+    if len(ca) == 0:
+        return none, none, float(margin)
+    gap = np.abs(x[ca] - x[cb]) - (half[ca] + half[cb] + margin)
+    inv_m = np.asarray(inv_mass, np.float32)
+    grp = np.asarray(group, np.int32)
+    ok = (gap <= 0.0).all(axis=1)
+    ok &= (inv_m[ca] > 0.0) | (inv_m[cb] > 0.0)
+    ok &= (grp[ca] != grp[cb]) | (grp[ca] <= 0)
+    return ca[ok].astype(np.int32), cb[ok].astype(np.int32), float(margin)
 
 
 class _Vec3View:
@@ -67,6 +111,67 @@ class _Vec3View:
 
     def __len__(self):
         return self._t.shape[0]
+
+
+class _RigidScene:
+    """The host scene as columns (float64, rounded to float32 once at
+    upload): bodies (pose, velocities, inverse mass and inertia, half
+    extents, collision group, static/point-mass flags, colour) and joints.
+    Single adds and bulk adds append to the same columns."""
+
+    _BODY = (("x", 3), ("q", 4), ("v", 3), ("w", 3), ("inv_mass", 0), ("inv_I", 3),
+             ("he", 3), ("color", 3))
+    _JOINT = (("anchor_a", 3), ("anchor_b", 3), ("rest", 0), ("compliance", 0))
+
+    def __init__(self):
+        self.body = {k: _Col(np.float64, w) for k, w in self._BODY}
+        self.group = _Col(np.int32)
+        self.kind = _Col(np.int8)
+        self.joint = {k: _Col(np.float64, w) for k, w in self._JOINT}
+        self.ja = _Col(np.int32)
+        self.jb = _Col(np.int32)
+
+    def n_bodies(self) -> int:
+        return len(self.group)
+
+    def n_joints(self) -> int:
+        return len(self.ja)
+
+    def append_bodies(self, cols: dict, group, kind) -> np.ndarray:
+        base = self.n_bodies()
+        for k, _ in self._BODY:
+            self.body[k].extend(cols[k])
+        self.group.extend(group)
+        self.kind.extend(kind)
+        return np.arange(base, self.n_bodies(), dtype=np.int64)
+
+    def append_joints(self, a, b, cols: dict) -> np.ndarray:
+        base = self.n_joints()
+        self.ja.extend(a)
+        self.jb.extend(b)
+        for k, _ in self._JOINT:
+            self.joint[k].extend(cols[k])
+        return np.arange(base, self.n_joints(), dtype=np.int64)
+
+    def arrays(self) -> dict:
+        """float32 / int32 arrays in the reference's layout."""
+        f = np.float32
+        b = {k: c.array().astype(f) for k, c in self.body.items()}
+        j = {k: c.array().astype(f) for k, c in self.joint.items()}
+        return dict(x=b["x"], q=b["q"], v=b["v"], w=b["w"], inv_mass=b["inv_mass"],
+                    inv_I=b["inv_I"], he=b["he"], group=self.group.array().astype(np.int32),
+                    j_a=self.ja.array().astype(np.int32), j_b=self.jb.array().astype(np.int32),
+                    j_anchor_a=j["anchor_a"], j_anchor_b=j["anchor_b"], j_rest=j["rest"],
+                    j_compliance=j["compliance"])
+
+
+def _rows(a, n, width):
+    """Broadcast a per-body value (scalar, row or (n, width) array) to (n, width)."""
+    return np.broadcast_to(np.asarray(a, np.float64), (n, width) if width else (n,))
+
+
+def _index(body) -> int:
+    return int(body.index) if hasattr(body, "index") else int(body)
 
 
 class Solver6DOF:
@@ -97,11 +202,8 @@ class Solver6DOF:
         self._man_margin = 0.02          # manifold clip gap (solver_6dof.py:115-119)
         self.use_graph = True
 
-        self._x, self._q, self._v, self._w = [], [], [], []
-        self._inv_mass, self._inv_I, self._he, self._group = [], [], [], []
-        self.bodies: list[RigidBody] = []
-        self._j_a, self._j_b, self._j_anchor_a, self._j_anchor_b = [], [], [], []
-        self._j_rest, self._j_compliance = [], []
+        self._scene = _RigidScene()
+        self._handles: list = []
         self._dirty = True
         self._buf = None
         self._plan = None
@@ -121,67 +223,101 @@ class Solver6DOF:
         self._cur_margin = 0.0
 
     # ---- scene building (solver_6dof.py:166-229) ----------------------------
-    def _add_body(self, position, half_extents, mass, quaternion, velocity, omega, color, static,
-                  is_particle, group):
-        idx = len(self._x)
-        he = tuple(float(h) for h in half_extents)
-        self._x.append(tuple(float(p) for p in position))
-        self._q.append(tuple(float(c) for c in quaternion))
-        self._v.append(tuple(float(c) for c in velocity))
-        self._w.append(tuple(float(c) for c in omega))
-        if static or mass <= 0.0:
-            self._inv_mass.append(0.0)
-            self._inv_I.append((0.0, 0.0, 0.0))
-            static = True
-        elif is_particle:
-            self._inv_mass.append(1.0 / float(mass))
-            self._inv_I.append((0.0, 0.0, 0.0))
-        else:
-            self._inv_mass.append(1.0 / float(mass))
-            self._inv_I.append(_box_inv_inertia(he, float(mass)))
-        self._he.append(he)
-        self._group.append(int(group))
+    def add_boxes(self, positions, half_extents, masses, quaternions=(0.0, 0.0, 0.0, 1.0),
+                  velocities=(0.0, 0.0, 0.0), omegas=(0.0, 0.0, 0.0), color=(0.6, 0.6, 0.75),
+                  static=False, particle=False, group=0) -> np.ndarray:
+        """Bulk body creation (an extension; ``add_box`` / ``add_particle``
+        add one).  Every argument is one value or one per body.  A body is
+        static (inverse mass and inertia 0) if ``static`` or its mass is not
+        positive; point masses (``particle``) get no rotational inertia;
+        other bodies the inertia of a solid box.  Returns the new indices."""
+        x = np.asarray(positions, np.float64).reshape(-1, 3)
+        n = x.shape[0]
+        he = _rows(half_extents, n, 3)
+        mass = _rows(masses, n, 0)
+        stat = _rows(static, n, 0).astype(bool) | (mass <= 0.0)
+        point = _rows(particle, n, 0).astype(bool)
+        dyn = ~stat
+        inv_mass = np.zeros(n)
+        inv_mass[dyn] = 1.0 / mass[dyn]
+        inv_I = np.zeros((n, 3))
+        spin = dyn & ~point
+        if spin.any():
+            inv_I[spin] = solid_box_inverse_inertia(he[spin], mass[spin])
+        cols = dict(x=x, q=_rows(quaternions, n, 4), v=_rows(velocities, n, 3),
+                    w=_rows(omegas, n, 3), inv_mass=inv_mass, inv_I=inv_I, he=he,
+                    color=_rows(color, n, 3))
+        kind = np.where(stat, _STATIC, 0) | np.where(point, _POINT, 0)
+        idx = self._scene.append_bodies(cols, _rows(group, n, 0).astype(np.int32),
+                                        kind.astype(np.int8))
         self._dirty = True
-        rb = RigidBody(index=idx, half_extents=he, color=color, static=static,
-                       is_particle=is_particle, group=int(group))
-        self.bodies.append(rb)
-        return rb
+        return idx
+
+    def _handle(self, i: int) -> RigidBody:
+        sc = self._scene
+        kind = int(sc.kind[i])
+        return RigidBody(index=i, half_extents=tuple(float(c) for c in sc.body["he"][i]),
+                         color=tuple(float(c) for c in sc.body["color"][i]),
+                         static=bool(kind & _STATIC), is_particle=bool(kind & _POINT),
+                         group=int(sc.group[i]))
 
     def add_box(self, position, half_extents, mass, quaternion=(0.0, 0.0, 0.0, 1.0),
                 velocity=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 0.0), color=(0.6, 0.6, 0.75),
-                static=False):
-        return self._add_body(position, half_extents, mass, quaternion, velocity, omega, color,
-                              static, is_particle=False, group=0)
+                static=False) -> RigidBody:
+        """One solid box (group 0: collides with everything)."""
+        i = int(self.add_boxes([position], half_extents, mass, quaternion, velocity, omega,
+                               color, static)[0])
+        return self._handle(i)
 
     def add_particle(self, position, mass, radius=0.015, velocity=(0.0, 0.0, 0.0),
-                     color=(0.85, 0.3, 0.35), static=False, group=1):
-        """Point mass as a small box of half-extent ``radius``; bodies sharing
-        a group > 0 do not collide with each other."""
-        return self._add_body(position, (radius, radius, radius), mass, (0.0, 0.0, 0.0, 1.0),
-                              velocity, (0.0, 0.0, 0.0), color, static, is_particle=True,
-                              group=group)
+                     color=(0.85, 0.3, 0.35), static=False, group=1) -> RigidBody:
+        """A point mass: a box of half extent ``radius`` for contacts, no
+        rotational inertia; bodies sharing a group > 0 do not collide with
+        each other (cloth nodes)."""
+        i = int(self.add_boxes([position], (radius, radius, radius), mass,
+                               velocities=velocity, color=color, static=static, particle=True,
+                               group=group)[0])
+        return self._handle(i)
+
+    def add_joints(self, a, b, compliance=0.0, rest_length=None, anchor_a=(0.0, 0.0, 0.0),
+                   anchor_b=(0.0, 0.0, 0.0)) -> np.ndarray:
+        """Bulk ``add_joint`` over index arrays (an extension)."""
+        a = np.asarray(a, np.int64).reshape(-1)
+        b = np.asarray(b, np.int64).reshape(-1)
+        m = a.shape[0]
+        aa, ab = _rows(anchor_a, m, 3), _rows(anchor_b, m, 3)
+        if rest_length is None:
+            # the anchors' separation at the bodies' current positions,
+            # anchors taken unrotated (solver_6dof.py:218-222)
+            x = self._scene.body["x"].array()
+            rest = np.linalg.norm((x[a] + aa) - (x[b] + ab), axis=1)
+        else:
+            rest = _rows(rest_length, m, 0)
+        idx = self._scene.append_joints(a, b, dict(anchor_a=aa, anchor_b=ab, rest=rest,
+                                                   compliance=_rows(compliance, m, 0)))
+        self._dirty = True
+        return idx
 
     def add_joint(self, a, b, compliance=0.0, rest_length=None, anchor_a=(0.0, 0.0, 0.0),
-                  anchor_b=(0.0, 0.0, 0.0)):
-        """Compliant distance joint between body-frame anchors (rest None =
-        current anchor separation, measured without rotating the anchors,
-        as the reference does, solver_6dof.py:218-222)."""
-        ia = a.index if hasattr(a, "index") else int(a)
-        ib = b.index if hasattr(b, "index") else int(b)
-        aa = np.asarray(anchor_a, np.float64)
-        ab = np.asarray(anchor_b, np.float64)
-        if rest_length is None:
-            xa = np.asarray(self._x[ia])
-            xb = np.asarray(self._x[ib])
-            rest_length = float(np.linalg.norm((xa + aa) - (xb + ab)))
-        self._j_a.append(ia)
-        self._j_b.append(ib)
-        self._j_anchor_a.append(tuple(float(c) for c in anchor_a))
-        self._j_anchor_b.append(tuple(float(c) for c in anchor_b))
-        self._j_rest.append(float(rest_length))
-        self._j_compliance.append(float(compliance))
-        self._dirty = True
-        return len(self._j_a) - 1
+                  anchor_b=(0.0, 0.0, 0.0)) -> int:
+        """Compliant distance joint between body-frame anchors of bodies (or
+        indices) ``a`` and ``b``; ``rest_length=None`` = their current
+        separation, ``compliance`` 0 = rigid.  Returns the joint index."""
+        return int(self.add_joints([_index(a)], [_index(b)], compliance, rest_length, anchor_a,
+                                   anchor_b)[0])
+
+    @property
+    def bodies(self) -> list:
+        """RigidBody handles of every body (built from the columns)."""
+        n = self._scene.n_bodies()
+        if len(self._handles) != n:
+            self._handles = self._handles[:n] + [self._handle(i)
+                                                 for i in range(len(self._handles), n)]
+        return self._handles
+
+    def inv_masses(self) -> np.ndarray:
+        """Inverse masses (float32, API order); 0 marks static bodies."""
+        return self._scene.body["inv_mass"].array().astype(np.float32)
 
     def set_gravity_scale(self, scale) -> None:
         """Per-body gravity factor g_i = g * scale[i] (an extension used for
@@ -191,19 +327,7 @@ class Solver6DOF:
 
     # ---- host arrays in the reference layout --------------------------------
     def host_arrays(self) -> dict:
-        n = len(self._x)
-        f = np.float32
-        return dict(
-            x=np.asarray(self._x, f).reshape(-1, 3), q=np.asarray(self._q, f).reshape(-1, 4),
-            v=np.asarray(self._v, f).reshape(-1, 3), w=np.asarray(self._w, f).reshape(-1, 3),
-            inv_mass=np.asarray(self._inv_mass, f).reshape(n),
-            inv_I=np.asarray(self._inv_I, f).reshape(-1, 3),
-            he=np.asarray(self._he, f).reshape(-1, 3),
-            group=np.asarray(self._group, np.int32).reshape(n),
-            j_a=np.asarray(self._j_a, np.int32), j_b=np.asarray(self._j_b, np.int32),
-            j_anchor_a=np.asarray(self._j_anchor_a, f).reshape(-1, 3),
-            j_anchor_b=np.asarray(self._j_anchor_b, f).reshape(-1, 3),
-            j_rest=np.asarray(self._j_rest, f), j_compliance=np.asarray(self._j_compliance, f))
+        return self._scene.arrays()
 
     # ---- flush (solver_6dof.py:261-354) ---------------------------------------
     def _flush(self):
@@ -241,7 +365,7 @@ class Solver6DOF:
             LAMF=torch.zeros(max(8 * n, 8), dtype=torch.float32, device=dev),
             PC=torch.zeros(1, dtype=torch.int32, device=dev),
             PTC=torch.zeros(1, dtype=torch.int32, device=dev))
-        self.n_joints = len(self._j_a)
+        self.n_joints = self._scene.n_joints()
         nj = self.n_joints
         ji = np.zeros((max(nj, 1), 2), np.int32)
         ja = np.zeros((max(nj, 1), 4), np.float32)
@@ -405,36 +529,18 @@ class Solver6DOF:
         return int(b["PC"].item())
 
     def _grid_pairs(self):
-        """Host NumPy grid with a global velocity margin (solver_6dof.py:232-258)."""
+        """The host ``grid`` broad phase on the device state (read back)."""
         b = self._buf
-        pos = b["X"][:, :3].cpu().numpy()
-        he = np.asarray(self._he, np.float32).reshape(-1, 3)
-        inv_m = np.asarray(self._inv_mass, np.float32)
-        n = len(pos)
-        if n < 2:
-            return np.zeros(0, np.int32), np.zeros(0, np.int32)
-        aabb = _world_aabb_half(b["Q"].cpu().numpy().reshape(-1, 4), he)
-        vmax = float(np.abs(b["V"][:, :3].cpu().numpy()).max())
-        wmax = float(np.abs(b["W"][:, :3].cpu().numpy()).max())
-        rmax = float(aabb.max())
-        maxhe = float(np.linalg.norm(he, axis=1).max())
-        margin = min(self._man_margin + 2.0 * self.dt * (vmax + wmax * maxhe), rmax)
-        self._cur_margin = float(margin)
-        ga, gb = grid_candidate_pairs(pos, 2.0 * rmax + margin)
-        if len(ga) == 0:
-            return np.zeros(0, np.int32), np.zeros(0, np.int32)
-        sep = np.abs(pos[ga] - pos[gb]) - (aabb[ga] + aabb[gb] + margin)
-        keep = np.all(sep <= 0.0, axis=1)
-        a = ga[keep].astype(np.int32)
-        bb = gb[keep].astype(np.int32)
-        grp = np.asarray(self._group, np.int32)
-        live = (inv_m[a] > 0.0) | (inv_m[bb] > 0.0)
-        live &= ~((grp[a] == grp[bb]) & (grp[a] > 0))
-        return a[live], bb[live]
+        h = self.host_arrays()
+        a, bb, self._cur_margin = host_grid_pairs(
+            b["X"][:, :3].cpu().numpy(), b["Q"].cpu().numpy(), h["he"],
+            b["V"][:, :3].cpu().numpy(), b["W"][:, :3].cpu().numpy(), h["inv_mass"],
+            h["group"], self.dt, self._man_margin)
+        return a, bb
 
     def step(self):
         self._flush()
-        n = len(self._x)
+        n = self.num_bodies
         if n == 0:
             return
         import torch
@@ -488,17 +594,17 @@ class Solver6DOF:
 
     def positions(self):
         if self._buf is None:
-            return np.asarray(self._x, np.float32).reshape(-1, 3)
+            return self.host_arrays()["x"]
         return self._buf["X"][:, :3].cpu().numpy().copy()
 
     def orientations(self):
         if self._buf is None:
-            return np.asarray(self._q, np.float32).reshape(-1, 4)
+            return self.host_arrays()["q"]
         return self._buf["Q"].cpu().numpy().copy()
 
     def velocities(self):
         if self._buf is None:
-            return np.asarray(self._v, np.float32).reshape(-1, 3)
+            return self.host_arrays()["v"]
         return self._buf["V"][:, :3].cpu().numpy().copy()
 
     def pairs(self) -> np.ndarray:
@@ -516,11 +622,11 @@ class Solver6DOF:
 
     @property
     def num_bodies(self):
-        return len(self._x)
+        return self._scene.n_bodies()
 
     @property
     def num_joints(self):
-        return len(self._j_a)
+        return self._scene.n_joints()
 
     def __del__(self):
         try:

@@ -424,6 +424,22 @@ int xpbd6_broadphase_lbvh(void* stream, void* ws, const float* X, const float* Q
 int xpbd6_broadphase_lbvh_leaf(void* stream, void* ws, const float* X, const float* Q,
                                const float* V, const float* W, const float* HE, float man_margin,
                                float dt, int n, int cap, int* PAIRS, int* PC, int leaf_size);
+/* Allocation-free variant of xpbd6_broadphase_lbvh (no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same inputs, outputs and pair list as xpbd6_broadphase_lbvh, but its
+ * scratch is a fixed 7 MiB array inside the library, one per GPU, held until
+ * the process ends; it calls no allocating routine.  n above
+ * XPBD_NOALLOC_MAX_BODIES returns XPBD_ERR_NOALLOC_TOO_LARGE before any
+ * launch.  Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (two streams at once) race; calls ordered on one stream do not.
+ * Nothing in the array needs to start at zero (each pass writes what the
+ * next reads). */
+#define XPBD_NOALLOC_MAX_BODIES 65536
+#define XPBD_ERR_NOALLOC_TOO_LARGE (-2)
+int xpbd6_broadphase_lbvh_noalloc(void* stream, const float* X, const float* Q, const float* V,
+                                  const float* W, const float* HE, float man_margin, float dt,
+                                  int n, int cap, int* PAIRS, int* PC);
 /* wp.HashGrid.build + emit_pairs_hashgrid (solver_6dof.py:456-479;
  * kernels_hashgrid.py:37-88); cell = 2 rs_max + margin (solver_6dof.py:335-337). */
 int xpbd6_broadphase_hashgrid(void* stream, void* ws, const float* X, const float* Q,

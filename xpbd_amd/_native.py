@@ -124,6 +124,7 @@ _SIGS = {
     "xpbd6_hashgrid_workspace_bytes": (ctypes.c_longlong, [_I]),
     "xpbd6_broadphase_lbvh": (_I, [_P] * 7 + [_F, _F, _I, _I, _P, _P]),
     "xpbd6_broadphase_lbvh_leaf": (_I, [_P] * 7 + [_F, _F, _I, _I, _P, _P, _I]),
+    "xpbd6_broadphase_lbvh_noalloc": (_I, [_P] * 6 + [_F, _F, _I, _I, _P, _P]),
     "xpbd6_broadphase_hashgrid": (_I, [_P] * 7 + [_F, _F, _F, _F, _I, _I, _P, _P]),
     "xpbd_morton3": (ctypes.c_uint, [_F, _F, _F]),
     "xpbd_mlp_param_count": (c_int, []),

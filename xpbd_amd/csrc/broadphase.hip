@@ -10,6 +10,7 @@
 // All scratch lives in a caller-owned workspace (xpbd6_*_workspace_bytes).
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "common.h"
 #include "rigid_math.h"
@@ -562,6 +563,38 @@ __global__ __launch_bounds__(kB) void k_grid_query(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fixed scratch of the allocation-free LBVH broad phase
+// (xpbd6_broadphase_lbvh_noalloc): one copy per GPU, held until the process
+// ends, shared by every call on that GPU.  7 MiB holds the LBVH workspace of
+// XPBD_NOALLOC_MAX_BODIES bodies (6.8 MiB at 65536).  Host code never passes
+// the symbol itself to a kernel: it uses the device address that
+// hipGetSymbolAddress returns for the current device, kept per device.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)7 << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per
+// device, with that device current).
+int scratch_address(void** out) {
+  int dev = 0;
+  XPBD_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    XPBD_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return (int)hipErrorInvalidSymbol;
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -667,6 +700,18 @@ int xpbd6_broadphase_lbvh_leaf(void* stream, void* ws, const float* X, const flo
                      w.cnt, w.off, (int2*)PAIRS, cap);
   XPBD_LAUNCH_CHECK();
   return 0;
+}
+
+int xpbd6_broadphase_lbvh_noalloc(void* stream, const float* X, const float* Q, const float* V,
+                                  const float* W, const float* HE, float man_margin, float dt,
+                                  int n, int cap, int* PAIRS, int* PC) {
+  if (n > XPBD_NOALLOC_MAX_BODIES) return XPBD_ERR_NOALLOC_TOO_LARGE;
+  if ((size_t)xpbd6_lbvh_workspace_bytes(n) > kNoAllocScratchBytes)
+    return XPBD_ERR_NOALLOC_TOO_LARGE;
+  void* ws = nullptr;
+  int rc = scratch_address(&ws);
+  if (rc) return rc;
+  return xpbd6_broadphase_lbvh(stream, ws, X, Q, V, W, HE, man_margin, dt, n, cap, PAIRS, PC);
 }
 
 int xpbd6_broadphase_hashgrid(void* stream, void* ws, const float* X, const float* Q,

@@ -13,5 +13,8 @@ extern "C" const char* xpbd_build_info(void) {
 
 extern "C" const char* xpbd_error_string(int code) {
   if (code == -1) return "invalid argument to a host routine";
+  if (code == XPBD_ERR_NOALLOC_TOO_LARGE)
+    return "more bodies than the allocation-free broad phase's fixed scratch holds "
+           "(XPBD_NOALLOC_MAX_BODIES = 65536)";
   return hipGetErrorString((hipError_t)code);
 }

@@ -38,13 +38,14 @@ METRIC = "constraint proj/s (1 GPU); substeps/s at fixed residual, learned vs pl
 HBM_PEAK_GBS = 8000.0
 
 
-def iter_bytes(n_distance: int, n_floor: int, n_particles: int) -> int:
+def iter_bytes(n_distance: int, n_floor: int, n_particles: int, x_prev: bool) -> int:
     """Algorithmic HBM bytes of one Jacobi iteration (the unit of one
     k_jacobi_tile launch): 24 B per DISTANCE row (a, b, rest, alpha, lambda
     read + write), 20 B per FLOOR row (a, floor_y, mu, lambda read + write),
-    and per particle 28 B (x read + write, inverse mass read) + 12 B (x_prev
-    read for friction)."""
-    return 24 * n_distance + 20 * n_floor + 40 * n_particles
+    and per particle 28 B (x read + write, inverse mass read), + 12 B (x_prev
+    read) when the launch reads x_prev: some FLOOR row with friction is
+    active (the particle lies below its floor) in the state it runs on."""
+    return 24 * n_distance + 20 * n_floor + (40 if x_prev else 28) * n_particles
 
 
 def time_kernel(fn, reps: int, rounds: int = 3, warmup: int = 3) -> float:
@@ -104,17 +105,43 @@ def time_kernel_graph(fn, reps: int, rounds: int = 3) -> float:
     return sorted(out)[len(out) // 2]
 
 
-def tile_launcher(N, X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR, h, tile, n):
+def tile_launcher(N, X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR, h, tile, n, R12=None,
+                  TSEG=None):
     """One k_jacobi_tile launch (a full Jacobi iteration) on fixed buffers
-    (ANC None: the scene has no ATTACH rows)."""
+    (ANC None: the scene has no ATTACH rows; R12 / TSEG: owned rows read as
+    12-byte records)."""
     lib = N.lib()
     args = tuple(None if t is None else N.ptr(t)
                  for t in (X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR))
+    compact = tuple(None if t is None else N.ptr(t) for t in (R12, TSEG))
 
     def fn():
         N.check(lib.xpbd_jacobi_tile(N.current_stream(), *args, h, 0, tile, n, 1.0, 0, 0.0,
-                                     40.0, None), "xpbd_jacobi_tile")
+                                     40.0, None, *compact), "xpbd_jacobi_tile")
     return fn
+
+
+def floor_rows_active(X, R) -> int:
+    """FLOOR rows whose particle lies below its floor (they read x_prev for
+    friction), counted on the device."""
+    import torch
+
+    t = (R[:, 0] >> 28) & 15
+    fl = t == 2
+    a = (R[fl, 0] & 0x0FFFFFFF).long()
+    y = X[a, 1]
+    return int(torch.count_nonzero(y < R[fl, 2].view(torch.float32)))
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), else the platform's."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def cpu_baseline(h, s, budget_s: float) -> dict:
@@ -134,27 +161,47 @@ def cpu_baseline(h, s, budget_s: float) -> dict:
     dt = time.perf_counter() - t
     proj = s.num_constraints * s.substeps * s.iterations * frames
     return {"value": proj / dt, "unit": "proj/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{frames} frames of the same workload after 1 warm-up frame "
                       f"({dt:.1f} s), serial C restatement (oracle/), "
-                      f"host {platform.processor() or platform.machine()}, "
+                      f"host CPU {cpu_model()}, "
                       f"os.cpu_count()={os.cpu_count()}"}
 
 
 def roofline_scene(N, reps: int, frames: int) -> dict:
+    """k_jacobi_tile on 8 x 1024^2 cloths (58.6 M rows, far beyond the
+    caches): the sheets hanging (no FLOOR row active, x_prev never read) and
+    lying on the floor (every FLOOR row active with friction); the owned rows
+    read as 12-byte records (the product path) and as 16-byte rows."""
     import torch
 
     from xpbd_amd.scenes import packed_cloth_device
     from xpbd_amd.solver import MODE_JACOBI_TILE
 
     res, sheets = 1024, 8
+    h = float(torch.tensor(1.0 / 60.0 / 15.0, dtype=torch.float32))
+
+    def measure(sc, compact):
+        launch = tile_launcher(N, sc["X"], sc["X_B"], sc["XP"], sc["V"], sc["R"], None,
+                               sc["LAM"], sc["LAM_B"], sc["TRS"], sc["HS"], sc["HR"], h,
+                               sc["tile"], sc["n_particles"], *((sc["R12"], sc["TSEG"])
+                                                               if compact else (None, None)))
+        return time_kernel_graph(launch, reps)
+
+    resting = packed_cloth_device(res, sheets, "cuda:0", height=-0.002)
+    torch.cuda.synchronize()
+    n, m, nd = resting["n_particles"], resting["n_rows"], resting["n_distance"]
+    rest_active = floor_rows_active(resting["X"], resting["R"])
+    rest_ms = measure(resting, True)
+    rest_bytes = iter_bytes(nd, m - nd, n, rest_active > 0)
+    del resting
+    torch.cuda.empty_cache()
     sc = packed_cloth_device(res, sheets, "cuda:0")
     torch.cuda.synchronize()
-    n, m, nd = sc["n_particles"], sc["n_rows"], sc["n_distance"]
-    h = float(torch.tensor(1.0 / 60.0 / 15.0, dtype=torch.float32))
-    launch = tile_launcher(N, sc["X"], sc["X_B"], sc["XP"], sc["V"], sc["R"], None,
-                           sc["LAM"], sc["LAM_B"], sc["TRS"], sc["HS"], sc["HR"], h, sc["tile"], n)
-    ms = time_kernel_graph(launch, reps)
-    nbytes = iter_bytes(nd, m - nd, n)
+    hang_active = floor_rows_active(sc["X"], sc["R"])
+    ms = measure(sc, True)
+    ms16 = measure(sc, False)
+    nbytes = iter_bytes(nd, m - nd, n, hang_active > 0)
     d = N.ParticlesDesc()
     d.x, d.v, d.x_prev, d.acc = (N.ptr(sc["X"]), N.ptr(sc["V"]), N.ptr(sc["XP"]),
                                  N.ptr(sc["ACC"]))
@@ -167,6 +214,7 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     d.x_alt, d.lam_alt = N.ptr(sc["X_B"]), N.ptr(sc["LAM_B"])
     d.tile_row_start, d.halo_start, d.halo_rows = N.ptr(sc["TRS"]), N.ptr(sc["HS"]), N.ptr(sc["HR"])
     d.tile_size = sc["tile"]
+    d.tile_rows12, d.tile_seg = N.ptr(sc["R12"]), N.ptr(sc["TSEG"])
     # fuse_first 2: integrate folded into the previous substep's last launch
     # (product path), 1: into each substep's first launch, 0: separate pass
     frame = {2: None, 1: None, 0: None}
@@ -179,13 +227,24 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
         N.lib().xpbd_frame3_destroy(plan)
     frame_ms = frame[2]
     achieved = nbytes / (ms * 1e-3) / 1e9
+    rest_ach = rest_bytes / (rest_ms * 1e-3) / 1e9
     return {
         "workload": f"{sheets} x {res}^2 cloth: {n} particles, {nd} DISTANCE + {m - nd} FLOOR "
-                    f"rows ({sc['n_halo']} halo rows re-projected), jacobi 15x2",
+                    f"rows ({sc['n_halo']} halo rows re-projected), jacobi 15x2, hanging at "
+                    "y = 0.5 m (no FLOOR row active)",
         "kernel": "k_jacobi_tile",
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": nbytes,
-        "kernel_ms": round(ms, 4), "frame_ms": None if frame_ms is None else round(frame_ms, 3),
+        "floor_rows_active": hang_active,
+        "kernel_ms": round(ms, 4),
+        "kernel_ms_rows16": round(ms16, 4),
+        "frac_rows16": round(nbytes / (ms16 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "resting": {"workload": "the same sheets lying on the floor (y = -2 mm, x_prev = x): "
+                                "every FLOOR row active with friction",
+                    "floor_rows_active": rest_active, "bytes_per_launch": rest_bytes,
+                    "kernel_ms": round(rest_ms, 4), "achieved": round(rest_ach, 1),
+                    "frac": round(rest_ach / HBM_PEAK_GBS, 4)},
+        "frame_ms": None if frame_ms is None else round(frame_ms, 3),
         "frame_ms_integrate_in_first_launch": None if frame[1] is None else round(frame[1], 3),
         "frame_ms_separate_integrate": None if frame[0] is None else round(frame[0], 3),
         "tile": sc["tile"],
@@ -536,16 +595,19 @@ def main() -> int:
         ts, hs, hr = s._tile_dev
         launch = tile_launcher(N, s.X, s.X_B, s.XP, s.V, s.R, s.ANC if s._has_attach else None,
                                s.LAM, s.LAM_B, ts, hs, hr,
-                               h, s._tile, s.num_bodies)
+                               h, s._tile, s.num_bodies,
+                               *((s.R12, s.TSEG) if s._compact_active() else (None, None)))
         x_keep, v_keep = s.positions(), s.velocities()
+        active = floor_rows_active(s.X, s.R)
         kms = time_kernel_graph(launch, args.kernel_reps)
         s.set_state(x_keep, v_keep)
-        nbytes = iter_bytes(n_dist, n_floor, s.num_bodies)
+        nbytes = iter_bytes(n_dist, n_floor, s.num_bodies, active > 0)
         ach = nbytes / (kms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic("k_jacobi_tile"),
                 "kernel": "k_jacobi_tile", "kernel_ms": round(kms, 5),
-                "bytes_per_launch": nbytes, "tile": s._tile}
+                "bytes_per_launch": nbytes, "floor_rows_active": active, "tile": s._tile,
+                "compact_rows": s._compact_active()}
         print(f"[bench] jacobi tile kernel {kms * 1e3:.2f} us, {ach:.1f} GB/s", file=sys.stderr,
               flush=True)
 

@@ -396,7 +396,29 @@ enum TileMode : int {
 // cloths (0.366 -> 0.359 ms); small scenes re-read them from L2.
 constexpr int kStreamOutParticles = 1 << 22;
 
-template <int BLOCK, int MODE>
+// Compact owned rows (C12): 12 bytes per row instead of 16.  A tile's owned
+// rows are stored as {lo - p0 | (hi - p0) << 10, p0 bits, p1 bits}: lo is
+// the row's endpoint inside the tile (< 1024), hi its other endpoint (two-body
+// rows, < p0 + 2^22; any value for one-body rows), and the type comes from
+// the tile's type segments (owned rows are grouped by type: TSEG[tile] =
+// first row of the ATTACH, FLOOR and CONTACT segments, relative to the
+// tile's first row).  A DISTANCE row is symmetric in its endpoints, bit for
+// bit (d and the normal only change sign), so storing (lo, hi) instead of
+// (a, b) gives the same corrections.  Halo rows are read from R.
+struct Row12 {
+  int w0, w1, w2;
+};
+constexpr int kRow12LoBits = 10;
+
+__device__ inline int4 decode_row12(Row12 c, int k, int4 seg, int p0) {
+  const int t = (k >= seg.x ? 1 : 0) + (k >= seg.y ? 1 : 0) + (k >= seg.z ? 1 : 0);
+  const int lo = p0 + (c.w0 & ((1 << kRow12LoBits) - 1));
+  const int hi = p0 + (int)((unsigned)c.w0 >> kRow12LoBits);
+  const bool two = t == T_DISTANCE || t == T_CONTACT;
+  return make_int4(lo | (t << 28), two ? hi : -1, c.w1, c.w2);
+}
+
+template <int BLOCK, int MODE, bool C12>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     const float4* __restrict__ Xin, float4* __restrict__ Xout, float4* __restrict__ XP,
     float4* __restrict__ V, const int4* __restrict__ R, const float4* __restrict__ ANC,
@@ -404,7 +426,8 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     const int* __restrict__ tile_row_start, const int* __restrict__ halo_start,
     const int* __restrict__ halo_rows, float h, int first, int P, int n, float relax,
     int finalize, float damping, float max_speed, int do_cap, float4* __restrict__ ACC, float gx,
-    float gy, float gz, const float* __restrict__ gscale) {
+    float gy, float gz, const float* __restrict__ gscale, const Row12* __restrict__ R12,
+    const int4* __restrict__ TSEG) {
   constexpr bool PREDICT = MODE & TM_PREDICT, FINALIZE = MODE & TM_FINALIZE;
   constexpr bool HAS_ACC = MODE & TM_ACC, HAS_ANCHOR = MODE & TM_ANCHOR;
   constexpr bool NEXT = MODE & TM_NEXT;
@@ -435,6 +458,11 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
   const int hb = halo_start[tile], he = halo_start[tile + 1];
   const int owned = re - rb;
   const int total = owned + (he - hb);
+  const int4 seg = C12 ? TSEG[tile] : make_int4(0, 0, 0, 0);
+  auto fetch = [&](int k, int j) -> int4 {
+    if (C12 && k < owned) return decode_row12(R12[j], k, seg, p0);
+    return R[j];
+  };
   constexpr int U = 2;
   int4 row[U];
   float lam[U];
@@ -446,7 +474,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
       jj[u] = -1;
       if (k < total) {
         jj[u] = k < owned ? rb + k : halo_rows[hb + (k - owned)];
-        row[u] = R[jj[u]];
+        row[u] = fetch(k, jj[u]);
         lam[u] = (ZERO_LAM || first) ? 0.0f : LAMin[jj[u]];
       }
     }
@@ -491,7 +519,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
       njj[u] = -1;
       if (k < total) {
         njj[u] = k < owned ? rb + k : halo_rows[hb + (k - owned)];
-        nrow[u] = R[njj[u]];
+        nrow[u] = fetch(k, njj[u]);
         nlam[u] = (ZERO_LAM || first) ? 0.0f : LAMin[njj[u]];
       }
     }
@@ -1010,12 +1038,18 @@ void launch_tile_mode(hipStream_t s, int nb, size_t shmem, const float* Xin, flo
                       float* LAMout, const int* trs, const int* hs, const int* hr, float h,
                       int first, int tile_size, int n, float relax, float damping,
                       float max_speed, int do_cap, float* ACC, float gx, float gy, float gz,
-                      const float* gscale) {
-  hipLaunchKernelGGL((k_jacobi_tile<BLOCK, MODE>), dim3(nb), dim3(BLOCK), shmem, s,
-                     (const float4*)Xin, (float4*)Xout, (float4*)XP, (float4*)V, (const int4*)R,
-                     (const float4*)ANC, LAMin, LAMout, trs, hs, hr, h, first, tile_size, n, relax,
-                     (MODE & TM_FINALIZE) ? 1 : 0, damping, max_speed, do_cap, (float4*)ACC, gx,
-                     gy, gz, gscale);
+                      const float* gscale, const int* R12, const int* TSEG) {
+#define XPBD_TILE_KERNEL(C)                                                                    \
+  hipLaunchKernelGGL((k_jacobi_tile<BLOCK, MODE, C>), dim3(nb), dim3(BLOCK), shmem, s,         \
+                     (const float4*)Xin, (float4*)Xout, (float4*)XP, (float4*)V, (const int4*)R, \
+                     (const float4*)ANC, LAMin, LAMout, trs, hs, hr, h, first, tile_size, n,     \
+                     relax, (MODE & TM_FINALIZE) ? 1 : 0, damping, max_speed, do_cap,           \
+                     (float4*)ACC, gx, gy, gz, gscale, (const Row12*)R12, (const int4*)TSEG)
+  if (R12)
+    XPBD_TILE_KERNEL(true);
+  else
+    XPBD_TILE_KERNEL(false);
+#undef XPBD_TILE_KERNEL
 }
 
 template <int BLOCK>
@@ -1024,12 +1058,12 @@ int launch_tile_block(int mode, hipStream_t s, int nb, size_t shmem, const float
                       const float* LAMin, float* LAMout, const int* trs, const int* hs,
                       const int* hr, float h, int first, int tile_size, int n, float relax,
                       float damping, float max_speed, int do_cap, float* ACC, float gx, float gy,
-                      float gz, const float* gscale) {
+                      float gz, const float* gscale, const int* R12, const int* TSEG) {
 #define XPBD_TILE_MODE(M)                                                                      \
   case M:                                                                                      \
     launch_tile_mode<BLOCK, M>(s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin, LAMout, trs, hs, \
                                hr, h, first, tile_size, n, relax, damping, max_speed, do_cap,  \
-                               ACC, gx, gy, gz, gscale);                                      \
+                               ACC, gx, gy, gz, gscale, R12, TSEG);                           \
     return 0;
   switch (mode) {
     XPBD_TILE_MODE(0)
@@ -1064,9 +1098,11 @@ int launch_tile(bool predict, hipStream_t s, const float* Xin, float* Xout, floa
                 const int* R, const float* ANC, const float* LAMin, float* LAMout,
                 const int* tile_row_start, const int* halo_start, const int* halo_rows, float h,
                 int first, int tile_size, int n, float relax, int finalize, float damping,
-                float max_speed, float* ACC, float gx, float gy, float gz, const float* gscale) {
+                float max_speed, float* ACC, float gx, float gy, float gz, const float* gscale,
+                const int* R12, const int* TSEG) {
   if (n <= 0) return 0;
   if (tile_size < 1 || tile_size > XPBD_MAX_TILE) return (int)hipErrorInvalidValue;
+  if (R12 && (!TSEG || tile_size > (1 << kRow12LoBits))) return (int)hipErrorInvalidValue;
   const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
   const int nb = blocks_for(n, tile_size);
   const size_t shmem = tile_lds_bytes(tile_size);
@@ -1083,12 +1119,13 @@ int launch_tile(bool predict, hipStream_t s, const float* Xin, float* Xout, floa
   if (tile_size < 64)
     rc = launch_tile_block<kBlock>(mode, s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin, LAMout,
                                    tile_row_start, halo_start, halo_rows, h, first, tile_size, n,
-                                   relax, damping, max_speed, do_cap, ACC, gx, gy, gz, gscale);
+                                   relax, damping, max_speed, do_cap, ACC, gx, gy, gz, gscale,
+                                   R12, TSEG);
   else
     rc = launch_tile_block<kTileBlock>(mode, s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin,
                                        LAMout, tile_row_start, halo_start, halo_rows, h, first,
                                        tile_size, n, relax, damping, max_speed, do_cap, ACC, gx,
-                                       gy, gz, gscale);
+                                       gy, gz, gscale, R12, TSEG);
   if (rc) return rc;
   XPBD_LAUNCH_CHECK();
   return 0;
@@ -1160,12 +1197,13 @@ int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* X
                      const int* R, const float* ANC, const float* LAMin, float* LAMout,
                      const int* tile_row_start, const int* halo_start, const int* halo_rows,
                      float h, int first, int tile_size, int n_particles, float relax,
-                     int finalize, float damping, float max_speed, float* ACC) {
+                     int finalize, float damping, float max_speed, float* ACC, const int* R12,
+                     const int* tile_seg) {
   // the plain launch only reads XP (friction, finalize)
   return launch_tile(false, (hipStream_t)stream, Xin, Xout, const_cast<float*>(XP), V, R, ANC,
                      LAMin, LAMout, tile_row_start, halo_start, halo_rows, h, first, tile_size,
                      n_particles, relax, finalize ? 1 : 0, damping, max_speed, ACC, 0.0f, 0.0f,
-                     0.0f, nullptr);
+                     0.0f, nullptr, R12, tile_seg);
 }
 
 int xpbd_jacobi_tile_next(void* stream, const float* Xin, float* Xout, float* XP, const int* R,
@@ -1173,22 +1211,23 @@ int xpbd_jacobi_tile_next(void* stream, const float* Xin, float* Xout, float* XP
                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
                           float h, int first, float gx, float gy, float gz, const float* gscale,
                           int tile_size, int n_particles, float relax, float damping,
-                          float max_speed, float* ACC) {
+                          float max_speed, float* ACC, const int* R12, const int* tile_seg) {
   if (Xin == Xout) return (int)hipErrorInvalidValue;  // neighbours read Xin while tiles write
   return launch_tile(false, (hipStream_t)stream, Xin, Xout, XP, nullptr, R, ANC, LAMin, LAMout,
                      tile_row_start, halo_start, halo_rows, h, first, tile_size, n_particles,
-                     relax, 2, damping, max_speed, ACC, gx, gy, gz, gscale);
+                     relax, 2, damping, max_speed, ACC, gx, gy, gz, gscale, R12, tile_seg);
 }
 
 int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP, const float* V,
                            const int* R, const float* ANC, float* LAMout,
                            const int* tile_row_start, const int* halo_start, const int* halo_rows,
                            float h, float gx, float gy, float gz, const float* gscale,
-                           int tile_size, int n_particles, float relax, float* ACC) {
+                           int tile_size, int n_particles, float relax, float* ACC,
+                           const int* R12, const int* tile_seg) {
   if (X == Xout) return (int)hipErrorInvalidValue;  // neighbours read X while tiles write
   return launch_tile(true, (hipStream_t)stream, X, Xout, XP, const_cast<float*>(V), R, ANC,
                      nullptr, LAMout, tile_row_start, halo_start, halo_rows, h, 1, tile_size,
-                     n_particles, relax, 0, 0.0f, 0.0f, ACC, gx, gy, gz, gscale);
+                     n_particles, relax, 0, 0.0f, 0.0f, ACC, gx, gy, gz, gscale, R12, tile_seg);
 }
 
 int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP, const float* V,

@@ -90,7 +90,8 @@ int xpbd_solve_jacobi(void* stream, const float* X, const float* XP, const int* 
  * are cleared.
  * R12 / tile_seg (optional, both or neither; tile_size <= 1024): the owned
  * rows again as 12-byte records, read instead of R for them (R is still read
- * for halo rows).  Row j of tile t (p0 = t * tile_size) is R12[3j .. 3j+2] =
+ * for halo rows); R12 holds 3 n_rows + 1 ints (one int of padding after the
+ * last record).  Row j of tile t (p0 = t * tile_size) is R12[3j .. 3j+2] =
  * {lo - p0 | (hi - p0) << 10, bits of p0-param, bits of p1-param} with lo the
  * endpoint inside the tile and hi the other one (two-body rows; hi - p0 <
  * 2^22), and tile_seg int4[n_tiles] holds the first owned row of the

@@ -405,17 +405,21 @@ constexpr int kStreamOutParticles = 1 << 22;
 // tile's first row).  A DISTANCE row is symmetric in its endpoints, bit for
 // bit (d and the normal only change sign), so storing (lo, hi) instead of
 // (a, b) gives the same corrections.  Halo rows are read from R.
-struct Row12 {
-  int w0, w1, w2;
-};
+// A row is fetched with one 16-byte load at 4-byte alignment from either
+// array (owned rows: 12-byte records, the fourth word belongs to the next
+// record and is ignored; halo rows: R), so the two kinds of rows share one
+// load instruction and one destination (no branch, no wait between them).
+// The record array carries one int of padding after its last record.
+typedef int RowLoad __attribute__((ext_vector_type(4), aligned(4)));
 constexpr int kRow12LoBits = 10;
 
-__device__ inline int4 decode_row12(Row12 c, int k, int4 seg, int p0) {
+// The row as loaded (raw.x/y/z = the record's words) -> the int4 row.
+__device__ inline int4 decode_row12(int4 raw, int k, int4 seg, int p0) {
   const int t = (k >= seg.x ? 1 : 0) + (k >= seg.y ? 1 : 0) + (k >= seg.z ? 1 : 0);
-  const int lo = p0 + (c.w0 & ((1 << kRow12LoBits) - 1));
-  const int hi = p0 + (int)((unsigned)c.w0 >> kRow12LoBits);
+  const int lo = p0 + (raw.x & ((1 << kRow12LoBits) - 1));
+  const int hi = p0 + (int)((unsigned)raw.x >> kRow12LoBits);
   const bool two = t == T_DISTANCE || t == T_CONTACT;
-  return make_int4(lo | (t << 28), two ? hi : -1, c.w1, c.w2);
+  return make_int4(lo | (t << 28), two ? hi : -1, raw.y, raw.z);
 }
 
 template <int BLOCK, int MODE, bool C12>
@@ -426,7 +430,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     const int* __restrict__ tile_row_start, const int* __restrict__ halo_start,
     const int* __restrict__ halo_rows, float h, int first, int P, int n, float relax,
     int finalize, float damping, float max_speed, int do_cap, float4* __restrict__ ACC, float gx,
-    float gy, float gz, const float* __restrict__ gscale, const Row12* __restrict__ R12,
+    float gy, float gz, const float* __restrict__ gscale, const int* __restrict__ R12,
     const int4* __restrict__ TSEG) {
   constexpr bool PREDICT = MODE & TM_PREDICT, FINALIZE = MODE & TM_FINALIZE;
   constexpr bool HAS_ACC = MODE & TM_ACC, HAS_ANCHOR = MODE & TM_ANCHOR;
@@ -459,9 +463,17 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
   const int owned = re - rb;
   const int total = owned + (he - hb);
   const int4 seg = C12 ? TSEG[tile] : make_int4(0, 0, 0, 0);
+  // rows are loaded raw and decoded only when projected, so a pass's loads
+  // stay in flight while the previous pass is projected
   auto fetch = [&](int k, int j) -> int4 {
-    if (C12 && k < owned) return decode_row12(R12[j], k, seg, p0);
-    return R[j];
+    if (!C12) return R[j];
+    const bool own = k < owned;
+    const int* base = own ? R12 : (const int*)R;
+    const RowLoad v = *(const RowLoad*)(base + (size_t)j * (own ? 3 : 4));
+    return make_int4(v.x, v.y, v.z, v.w);
+  };
+  auto decode = [&](int4 raw, int k) -> int4 {
+    return (C12 && k < owned) ? decode_row12(raw, k, seg, p0) : raw;
   };
   constexpr int U = 2;
   int4 row[U];
@@ -529,7 +541,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
       const int j = jj[u];
       const Proj p = project_row<true>(
           pos, prev, [&]() { return HAS_ANCHOR ? ANC[j] : make_float4(0.f, 0.f, 0.f, 0.f); },
-          row[u], lam[u], inv_dt2);
+          decode(row[u], k0 + u * BLOCK), lam[u], inv_dt2);
       if (k0 + u * BLOCK < owned) LAMout[jj[u]] = p.lam;
       if (p.na >= 1) add(p.a, p.ca);
       if (p.na == 2) add(p.b, p.cb);
@@ -1044,7 +1056,7 @@ void launch_tile_mode(hipStream_t s, int nb, size_t shmem, const float* Xin, flo
                      (const float4*)Xin, (float4*)Xout, (float4*)XP, (float4*)V, (const int4*)R, \
                      (const float4*)ANC, LAMin, LAMout, trs, hs, hr, h, first, tile_size, n,     \
                      relax, (MODE & TM_FINALIZE) ? 1 : 0, damping, max_speed, do_cap,           \
-                     (float4*)ACC, gx, gy, gz, gscale, (const Row12*)R12, (const int4*)TSEG)
+                     (float4*)ACC, gx, gy, gz, gscale, R12, (const int4*)TSEG)
   if (R12)
     XPBD_TILE_KERNEL(true);
   else

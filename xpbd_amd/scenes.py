@@ -253,15 +253,16 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
     assert int(hi_rel.max()) < (1 << 22) and tile <= 1024
     w0 = lo_rel | (hi_rel << 10)
     w0 = torch.where(w0 >= (1 << 31), w0 - (1 << 32), w0)
-    R12 = torch.empty((m, 3), device=device, dtype=torch.int32)
-    R12[:, 0] = w0.to(torch.int32)
-    R12[:, 1:] = R[:, 2:]
+    R12 = torch.zeros(3 * m + 1, device=device, dtype=torch.int32)   # + 1 int of padding
+    rec = R12[:3 * m].view(m, 3)
+    rec[:, 0] = w0.to(torch.int32)
+    rec[:, 1:] = R[:, 2:]
     per = torch.bincount(row_tile * 4 + t, minlength=4 * n_tiles).view(n_tiles, 4)
     TSEG = torch.zeros((n_tiles, 4), device=device, dtype=torch.int32)
     TSEG[:, :3] = torch.cumsum(per[:, :3], dim=1).to(torch.int32)
     X = Xo[order].contiguous()
     del a, b, t, lo, hi, rest, comp, p0, p1, is_d, Xo, ro, cross, ht, so
-    del row_tile, base, lo_rel, hi_rel, w0, per
+    del row_tile, base, lo_rel, hi_rel, w0, per, rec
     return dict(
         X=X, X_B=X.clone(), V=torch.zeros((n, 4), device=device),
         XP=X.clone() if height <= 0 else torch.zeros((n, 4), device=device),

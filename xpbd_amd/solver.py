@@ -231,8 +231,9 @@ ROW12_HI_LIMIT = 1 << (32 - ROW12_LO_BITS)
 
 
 def compact_tile_rows(rows_sorted: np.ndarray, tile_row_start: np.ndarray, tile: int):
-    """The fused Jacobi kernel's 12-byte owned-row records and per-tile type
-    segments (include/xpbd.h, xpbd_jacobi_tile: R12 / tile_seg), or None
+    """The fused Jacobi kernel's 12-byte owned-row records (3 m + 1 ints) and
+    per-tile type segments (include/xpbd.h, xpbd_jacobi_tile: R12 /
+    tile_seg), or None
     when a row does not fit the format (tiles over 1024 particles, a second
     endpoint 2^22 or more particles after its tile's start, or a tile whose
     rows are not grouped by type)."""
@@ -255,9 +256,11 @@ def compact_tile_rows(rows_sorted: np.ndarray, tile_row_start: np.ndarray, tile:
         return None
     if np.any((np.diff(row_tile) == 0) & (np.diff(t) < 0)):
         return None
-    r12 = np.empty((m, 3), np.int32)
-    r12[:, 0] = (lo | (hi << ROW12_LO_BITS)).astype(np.uint32).view(np.int32)
-    r12[:, 1:] = rows_sorted[:, 2:]
+    # one int of padding: the kernel reads each record with a 16-byte load
+    r12 = np.zeros(3 * m + 1, np.int32)
+    rec = r12[:3 * m].reshape(m, 3)
+    rec[:, 0] = (lo | (hi << ROW12_LO_BITS)).astype(np.uint32).view(np.int32)
+    rec[:, 1:] = rows_sorted[:, 2:]
     per = np.bincount(row_tile * 4 + t, minlength=4 * n_tiles).reshape(n_tiles, 4)
     seg = np.zeros((n_tiles, 4), np.int32)
     seg[:, :3] = np.cumsum(per[:, :3], axis=1)
@@ -845,7 +848,8 @@ class Solver:
             self.R.copy_(self._t(rows))
             self.ANC.copy_(self._t(np.ascontiguousarray(anc[self._perm])))
             if self.R12 is not None:
-                self.R12[:, 1:].copy_(self._t(np.ascontiguousarray(rows[:, 2:])))
+                rec = self.R12[:3 * len(rows)].view(len(rows), 3)
+                rec[:, 1:].copy_(self._t(np.ascontiguousarray(rows[:, 2:])))
 
     # ---- the step (solver.py:412-467) --------------------------------------
     def _signature(self):

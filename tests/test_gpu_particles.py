@@ -828,9 +828,10 @@ def test_compact_rows_are_bit_identical_to_full_rows(scene):
     """The tile kernel reading its owned rows as 12-byte records (endpoints
     as tile offsets, DISTANCE rows stored lo-hi, type from the tile's
     segments) gives the same positions, velocities and multipliers, bit for
-    bit, as reading the 16-byte rows: a cloth with FLOOR rows on the floor,
-    the mixed scene with ATTACH rows, and a sphere pile with device CONTACT
-    rows."""
+    bit, as reading the 16-byte rows: a cloth with FLOOR rows on the floor
+    and the mixed scene with ATTACH rows.  A sphere pile with device CONTACT
+    rows (summed with float atomics, whose order varies from run to run)
+    agrees to rounding."""
     from xpbd_amd import Solver
     from xpbd_amd.scenes import cloth_sheet
 
@@ -853,4 +854,7 @@ def test_compact_rows_are_bit_identical_to_full_rows(scene):
         assert s._compact_active() == compact
         out.append((s.positions(), s.velocities(), s.lambdas()))
     for g, e in zip(out[0], out[1]):
-        assert np.array_equal(g, e)
+        if scene == "pile":
+            assert np.abs(g - e).max() <= 1e-5 * max(1.0, float(np.abs(e).max()))
+        else:
+            assert np.array_equal(g, e)

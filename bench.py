@@ -105,19 +105,16 @@ def time_kernel_graph(fn, reps: int, rounds: int = 3) -> float:
     return sorted(out)[len(out) // 2]
 
 
-def tile_launcher(N, X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR, h, tile, n, R12=None,
-                  TSEG=None):
+def tile_launcher(N, X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR, h, tile, n):
     """One k_jacobi_tile launch (a full Jacobi iteration) on fixed buffers
-    (ANC None: the scene has no ATTACH rows; R12 / TSEG: owned rows read as
-    12-byte records)."""
+    (ANC None: the scene has no ATTACH rows)."""
     lib = N.lib()
     args = tuple(None if t is None else N.ptr(t)
                  for t in (X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR))
-    compact = tuple(None if t is None else N.ptr(t) for t in (R12, TSEG))
 
     def fn():
         N.check(lib.xpbd_jacobi_tile(N.current_stream(), *args, h, 0, tile, n, 1.0, 0, 0.0,
-                                     40.0, None, *compact), "xpbd_jacobi_tile")
+                                     40.0, None), "xpbd_jacobi_tile")
     return fn
 
 
@@ -171,8 +168,7 @@ def cpu_baseline(h, s, budget_s: float) -> dict:
 def roofline_scene(N, reps: int, frames: int) -> dict:
     """k_jacobi_tile on 8 x 1024^2 cloths (58.6 M rows, far beyond the
     caches): the sheets hanging (no FLOOR row active, x_prev never read) and
-    lying on the floor (every FLOOR row active with friction); the owned rows
-    read as 12-byte records (the product path) and as 16-byte rows."""
+    lying on the floor (every FLOOR row active with friction)."""
     import torch
 
     from xpbd_amd.scenes import packed_cloth_device
@@ -181,26 +177,24 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     res, sheets = 1024, 8
     h = float(torch.tensor(1.0 / 60.0 / 15.0, dtype=torch.float32))
 
-    def measure(sc, compact):
+    def measure(sc):
         launch = tile_launcher(N, sc["X"], sc["X_B"], sc["XP"], sc["V"], sc["R"], None,
                                sc["LAM"], sc["LAM_B"], sc["TRS"], sc["HS"], sc["HR"], h,
-                               sc["tile"], sc["n_particles"], *((sc["R12"], sc["TSEG"])
-                                                               if compact else (None, None)))
+                               sc["tile"], sc["n_particles"])
         return time_kernel_graph(launch, reps)
 
     resting = packed_cloth_device(res, sheets, "cuda:0", height=-0.002)
     torch.cuda.synchronize()
     n, m, nd = resting["n_particles"], resting["n_rows"], resting["n_distance"]
     rest_active = floor_rows_active(resting["X"], resting["R"])
-    rest_ms = measure(resting, True)
+    rest_ms = measure(resting)
     rest_bytes = iter_bytes(nd, m - nd, n, rest_active > 0)
     del resting
     torch.cuda.empty_cache()
     sc = packed_cloth_device(res, sheets, "cuda:0")
     torch.cuda.synchronize()
     hang_active = floor_rows_active(sc["X"], sc["R"])
-    ms = measure(sc, True)
-    ms16 = measure(sc, False)
+    ms = measure(sc)
     nbytes = iter_bytes(nd, m - nd, n, hang_active > 0)
     d = N.ParticlesDesc()
     d.x, d.v, d.x_prev, d.acc = (N.ptr(sc["X"]), N.ptr(sc["V"]), N.ptr(sc["XP"]),
@@ -214,7 +208,6 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     d.x_alt, d.lam_alt = N.ptr(sc["X_B"]), N.ptr(sc["LAM_B"])
     d.tile_row_start, d.halo_start, d.halo_rows = N.ptr(sc["TRS"]), N.ptr(sc["HS"]), N.ptr(sc["HR"])
     d.tile_size = sc["tile"]
-    d.tile_rows12, d.tile_seg = N.ptr(sc["R12"]), N.ptr(sc["TSEG"])
     # fuse_first 2: integrate folded into the previous substep's last launch
     # (product path), 1: into each substep's first launch, 0: separate pass
     frame = {2: None, 1: None, 0: None}
@@ -237,8 +230,6 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": nbytes,
         "floor_rows_active": hang_active,
         "kernel_ms": round(ms, 4),
-        "kernel_ms_rows16": round(ms16, 4),
-        "frac_rows16": round(nbytes / (ms16 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "resting": {"workload": "the same sheets lying on the floor (y = -2 mm, x_prev = x): "
                                 "every FLOOR row active with friction",
                     "floor_rows_active": rest_active, "bytes_per_launch": rest_bytes,
@@ -595,8 +586,7 @@ def main() -> int:
         ts, hs, hr = s._tile_dev
         launch = tile_launcher(N, s.X, s.X_B, s.XP, s.V, s.R, s.ANC if s._has_attach else None,
                                s.LAM, s.LAM_B, ts, hs, hr,
-                               h, s._tile, s.num_bodies,
-                               *((s.R12, s.TSEG) if s._compact_active() else (None, None)))
+                               h, s._tile, s.num_bodies)
         x_keep, v_keep = s.positions(), s.velocities()
         active = floor_rows_active(s.X, s.R)
         kms = time_kernel_graph(launch, args.kernel_reps)
@@ -606,8 +596,7 @@ def main() -> int:
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic("k_jacobi_tile"),
                 "kernel": "k_jacobi_tile", "kernel_ms": round(kms, 5),
-                "bytes_per_launch": nbytes, "floor_rows_active": active, "tile": s._tile,
-                "compact_rows": s._compact_active()}
+                "bytes_per_launch": nbytes, "floor_rows_active": active, "tile": s._tile}
         print(f"[bench] jacobi tile kernel {kms * 1e3:.2f} us, {ach:.1f} GB/s", file=sys.stderr,
               flush=True)
 

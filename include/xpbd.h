@@ -87,23 +87,12 @@ int xpbd_solve_jacobi(void* stream, const float* X, const float* XP, const int* 
  * finalize != 0 also applies finalize_velocity + cap_velocity
  * (kernels.py:346-371) and writes V.  ACC (optional, float4[n]) holds sums
  * from xpbd_solve_jacobi_dyn of this iteration; they join the average and
- * are cleared.
- * R12 / tile_seg (optional, both or neither; tile_size <= 1024): the owned
- * rows again as 12-byte records, read instead of R for them (R is still read
- * for halo rows); R12 holds 3 n_rows + 1 ints (one int of padding after the
- * last record).  Row j of tile t (p0 = t * tile_size) is R12[3j .. 3j+2] =
- * {lo - p0 | (hi - p0) << 10, bits of p0-param, bits of p1-param} with lo the
- * endpoint inside the tile and hi the other one (two-body rows; hi - p0 <
- * 2^22), and tile_seg int4[n_tiles] holds the first owned row of the
- * ATTACH, FLOOR and CONTACT segments of tile t relative to
- * tile_row_start[t] (owned rows grouped by type).  Results are bit-identical
- * to reading R. */
+ * are cleared. */
 int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* XP, float* V,
                      const int* R, const float* ANC, const float* LAMin, float* LAMout,
                      const int* tile_row_start, const int* halo_start, const int* halo_rows,
                      float h, int first, int tile_size, int n_particles, float relax,
-                     int finalize, float damping, float max_speed, float* ACC, const int* R12,
-                     const int* tile_seg);
+                     int finalize, float damping, float max_speed, float* ACC);
 
 /* The first iteration of a substep with kernels.py:49-73 `integrate` fused
  * in (solver.py:431-458: integrate, lambda reset, first projection + apply):
@@ -114,14 +103,12 @@ int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* X
  * iterated positions (Xout != X).  V is not written: the caller finishes the
  * substep with at least one xpbd_jacobi_tile launch with finalize = 1.
  * Same tile tables as xpbd_jacobi_tile; gscale as in xpbd_integrate_to; ACC
- * (optional) as in xpbd_jacobi_tile, filled by xpbd_solve_jacobi_dyn_first;
- * R12 / tile_seg as in xpbd_jacobi_tile. */
+ * (optional) as in xpbd_jacobi_tile, filled by xpbd_solve_jacobi_dyn_first. */
 int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP, const float* V,
                            const int* R, const float* ANC, float* LAMout,
                            const int* tile_row_start, const int* halo_start, const int* halo_rows,
                            float h, float gx, float gy, float gz, const float* gscale,
-                           int tile_size, int n_particles, float relax, float* ACC,
-                           const int* R12, const int* tile_seg);
+                           int tile_size, int n_particles, float relax, float* ACC);
 
 /* The last iteration of a substep followed by the next substep's
  * kernels.py:49-73 `integrate` (solver.py:431-458: apply, finalize_velocity,
@@ -133,14 +120,13 @@ int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP,
  * continues with a plain first launch (first = 1) and the frame's last
  * substep ends with xpbd_jacobi_tile (finalize = 1), which writes V.  Other
  * tiles read XP while a tile writes it: no row may be CONTACT (their
- * friction reads the other endpoint's x_prev); Xin != Xout.  R12 / tile_seg
- * as in xpbd_jacobi_tile. */
+ * friction reads the other endpoint's x_prev); Xin != Xout. */
 int xpbd_jacobi_tile_next(void* stream, const float* Xin, float* Xout, float* XP, const int* R,
                           const float* ANC, const float* LAMin, float* LAMout,
                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
                           float h, int first, float gx, float gy, float gz, const float* gscale,
                           int tile_size, int n_particles, float relax, float damping,
-                          float max_speed, float* ACC, const int* R12, const int* tile_seg);
+                          float max_speed, float* ACC);
 
 /* kernels.py:49-73 `integrate` reading Xin and writing Xout (may alias);
  * gscale (optional, float[n]) scales gravity per particle (batched scenes). */
@@ -277,10 +263,6 @@ typedef struct xpbd_particles_desc {
    * 0 of each frame (<= 1: every substep); the other substeps reuse ws_y
    * with their own first plain steps */
   int ws_every;
-  /* tile mode: the owned rows as 12-byte records and the tiles' type
-   * segments (see xpbd_jacobi_tile), or NULL to read every row from `rows` */
-  const int* tile_rows12;
-  const int* tile_seg;
 } xpbd_particles_desc;
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);

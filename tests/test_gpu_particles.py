@@ -279,14 +279,13 @@ def test_tile_first_equals_integrate_then_tile(gscale):
     N.check(lib.xpbd_jacobi_tile(None, N.ptr(XA), N.ptr(XoA), N.ptr(XPA), N.ptr(VA), N.ptr(s.R),
                                  N.ptr(s.ANC), N.ptr(LA), N.ptr(LA), N.ptr(ts), N.ptr(hs),
                                  N.ptr(hr), h, 1, s._tile, n, s.jacobi_relax, 0, s.damping,
-                                 s.max_speed, None, *s.compact_row_ptrs()))
+                                 s.max_speed, None))
     # path B: fused
     XoB, XPB, LB = torch.empty_like(X0), torch.zeros_like(X0), torch.zeros(m, device=DEV)
     N.check(lib.xpbd_jacobi_tile_first(None, N.ptr(X0), N.ptr(XoB), N.ptr(XPB), N.ptr(V0),
                                        N.ptr(s.R), N.ptr(s.ANC), N.ptr(LB), N.ptr(ts), N.ptr(hs),
                                        N.ptr(hr), h, *g, N.ptr(gs) if gs is not None else None,
-                                       s._tile, n, s.jacobi_relax, None,
-                                       *s.compact_row_ptrs()))
+                                       s._tile, n, s.jacobi_relax, None))
     torch.cuda.synchronize()
     assert torch.equal(XoA, XoB)
     assert torch.equal(XPA, XPB)
@@ -327,13 +326,13 @@ def test_tile_next_equals_finalize_then_integrate(gscale):
     N.check(lib.xpbd_jacobi_tile(None, N.ptr(X0), N.ptr(X1), N.ptr(XP0), N.ptr(V0), N.ptr(s.R),
                                  N.ptr(s.ANC), N.ptr(L0), N.ptr(L0), N.ptr(ts), N.ptr(hs),
                                  N.ptr(hr), h, 1, s._tile, n, s.jacobi_relax, 0, s.damping,
-                                 s.max_speed, None, *s.compact_row_ptrs()))
+                                 s.max_speed, None))
     # path A: finalizing iteration, then integrate
     XA, VA, XPA, LA = torch.empty_like(X0), V0.clone(), XP0.clone(), torch.zeros(m, device=DEV)
     N.check(lib.xpbd_jacobi_tile(None, N.ptr(X1), N.ptr(XA), N.ptr(XPA), N.ptr(VA), N.ptr(s.R),
                                  N.ptr(s.ANC), N.ptr(L0), N.ptr(LA), N.ptr(ts), N.ptr(hs),
                                  N.ptr(hr), h, 0, s._tile, n, s.jacobi_relax, 1, s.damping,
-                                 s.max_speed, None, *s.compact_row_ptrs()))
+                                 s.max_speed, None))
     N.check(lib.xpbd_integrate_to(None, N.ptr(XA), N.ptr(XA), N.ptr(VA), N.ptr(XPA), *g, h,
                                   N.ptr(gs) if gs is not None else None, n))
     # path B: one launch
@@ -343,8 +342,7 @@ def test_tile_next_equals_finalize_then_integrate(gscale):
                                       N.ptr(s.ANC), N.ptr(L0), N.ptr(LB), N.ptr(ts), N.ptr(hs),
                                       N.ptr(hr), h, 0, *g,
                                       N.ptr(gs) if gs is not None else None, s._tile, n,
-                                      s.jacobi_relax, s.damping, s.max_speed, None,
-                                      *s.compact_row_ptrs()))
+                                      s.jacobi_relax, s.damping, s.max_speed, None))
     torch.cuda.synchronize()
     assert torch.equal(XA, XB)
     assert torch.equal(XPA, XPB)
@@ -354,8 +352,7 @@ def test_tile_next_equals_finalize_then_integrate(gscale):
         N.check(lib.xpbd_jacobi_tile_next(None, N.ptr(X1), N.ptr(X1), N.ptr(XPB), N.ptr(s.R),
                                           None, N.ptr(L0), N.ptr(LB), N.ptr(ts), N.ptr(hs),
                                           N.ptr(hr), h, 0, *g, None, s._tile, n,
-                                          s.jacobi_relax, s.damping, s.max_speed, None,
-                                      *s.compact_row_ptrs()))
+                                          s.jacobi_relax, s.damping, s.max_speed, None))
 
 
 @pytest.mark.parametrize("iters", [1, 2, 3])
@@ -822,39 +819,3 @@ def test_device_contacts_fused_integrate_matches_separate():
     assert np.abs(v0 - v1).max() <= 5e-3
     assert x0[:, 1].min() > 0.05 - 0.01
 
-
-@pytest.mark.parametrize("scene", ["cloth", "mixed", "pile"])
-def test_compact_rows_are_bit_identical_to_full_rows(scene):
-    """The tile kernel reading its owned rows as 12-byte records (endpoints
-    as tile offsets, DISTANCE rows stored lo-hi, type from the tile's
-    segments) gives the same positions, velocities and multipliers, bit for
-    bit, as reading the 16-byte rows: a cloth with FLOOR rows on the floor
-    and the mixed scene with ATTACH rows.  A sphere pile with device CONTACT
-    rows (summed with float atomics, whose order varies from run to run)
-    agrees to rounding."""
-    from xpbd_amd import Solver
-    from xpbd_amd.scenes import cloth_sheet
-
-    def build():
-        if scene == "cloth":
-            s = cloth_sheet(Solver, res=48, solve_mode="jacobi", device=DEV)[0]
-        elif scene == "mixed":
-            s = _mixed_scene("jacobi")
-        else:
-            s = _sphere_pile()
-            s.enable_device_contacts(True, margin=0.01)
-        s.block_frames = False
-        return s
-
-    out = []
-    for compact in (True, False):
-        s = build()
-        s.compact_rows = compact
-        s.step(30)
-        assert s._compact_active() == compact
-        out.append((s.positions(), s.velocities(), s.lambdas()))
-    for g, e in zip(out[0], out[1]):
-        if scene == "pile":
-            assert np.abs(g - e).max() <= 1e-5 * max(1.0, float(np.abs(e).max()))
-        else:
-            assert np.array_equal(g, e)

@@ -38,7 +38,7 @@ class ParticlesDesc(ctypes.Structure):
         ("dyn_cap", c_int), ("gscale", c_void_p),
         ("ws_image", c_void_p), ("ws_y", c_void_p),
         ("fuse_first", c_int), ("block", c_int), ("color_starts_dev", c_void_p),
-        ("ws_every", c_int), ("tile_rows12", c_void_p), ("tile_seg", c_void_p),
+        ("ws_every", c_int),
     ]
 
 
@@ -76,14 +76,13 @@ _SIGS = {
     "xpbd_solve_jacobi": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_float, c_int, c_int, c_int, c_int]),
     "xpbd_jacobi_tile": (c_int, [c_void_p] * 12 + [c_float, c_int, c_int, c_int, c_float, c_int,
-                                                    c_float, c_float, c_void_p, c_void_p,
-                                                    c_void_p]),
+                                                    c_float, c_float, c_void_p]),
     "xpbd_jacobi_tile_first": (c_int, [c_void_p] * 11 + [c_float, c_float, c_float, c_float,
                                                          c_void_p, c_int, c_int, c_float,
-                                                         c_void_p, c_void_p, c_void_p]),
+                                                         c_void_p]),
     "xpbd_jacobi_tile_next": (c_int, [c_void_p] * 11 + [c_float, c_int, c_float, c_float, c_float,
                                                         c_void_p, c_int, c_int, c_float, c_float,
-                                                        c_float, c_void_p, c_void_p, c_void_p]),
+                                                        c_float, c_void_p]),
     "xpbd_integrate_to": (c_int, [c_void_p] * 5 + [c_float, c_float, c_float, c_float, c_void_p,
                                                    c_int]),
     "xpbd_solve_jacobi_dyn": (c_int, [c_void_p] * 6 + [c_float, c_int, c_void_p, c_int]),

@@ -99,7 +99,7 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
         rc = xpbd_jacobi_tile_first(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows,
                                     d.anchors, lams[1], d.tile_row_start, d.halo_start,
                                     d.halo_rows, d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n,
-                                    d.relax, acc, d.tile_rows12, d.tile_seg);
+                                    d.relax, acc);
         if (rc) return rc;
         loc ^= 1;
         li = 1;
@@ -120,13 +120,12 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
           rc = xpbd_jacobi_tile_next(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.rows, d.anchors,
                                      lams[li], lams[li ^ 1], d.tile_row_start, d.halo_start,
                                      d.halo_rows, d.h, first, d.gx, d.gy, d.gz, d.gscale,
-                                     d.tile_size, n, d.relax, d.damping, d.max_speed, acc,
-                                     d.tile_rows12, d.tile_seg);
+                                     d.tile_size, n, d.relax, d.damping, d.max_speed, acc);
         } else {
           rc = xpbd_jacobi_tile(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
                                 lams[li], lams[li ^ 1], d.tile_row_start, d.halo_start,
                                 d.halo_rows, d.h, first, d.tile_size, n, d.relax, it + 1 == it_n,
-                                d.damping, d.max_speed, acc, d.tile_rows12, d.tile_seg);
+                                d.damping, d.max_speed, acc);
         }
         if (rc) return rc;
         loc ^= 1;
@@ -168,8 +167,7 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
       rc = xpbd_jacobi_tile(s, bufs[cur], bufs[nxt], d.x_prev, d.v, d.rows, d.anchors, lams[li],
                             lams[li ^ 1], d.tile_row_start, d.halo_start, d.halo_rows, d.h,
                             it == 0 && !warm, d.tile_size, n, d.relax, it + 1 == it_n,
-                            d.damping, d.max_speed, dyn ? d.acc : nullptr, d.tile_rows12,
-                            d.tile_seg);
+                            d.damping, d.max_speed, dyn ? d.acc : nullptr);
       if (rc) return rc;
       cur = nxt;
       li ^= 1;

@@ -396,33 +396,7 @@ enum TileMode : int {
 // cloths (0.366 -> 0.359 ms); small scenes re-read them from L2.
 constexpr int kStreamOutParticles = 1 << 22;
 
-// Compact owned rows (C12): 12 bytes per row instead of 16.  A tile's owned
-// rows are stored as {lo - p0 | (hi - p0) << 10, p0 bits, p1 bits}: lo is
-// the row's endpoint inside the tile (< 1024), hi its other endpoint (two-body
-// rows, < p0 + 2^22; any value for one-body rows), and the type comes from
-// the tile's type segments (owned rows are grouped by type: TSEG[tile] =
-// first row of the ATTACH, FLOOR and CONTACT segments, relative to the
-// tile's first row).  A DISTANCE row is symmetric in its endpoints, bit for
-// bit (d and the normal only change sign), so storing (lo, hi) instead of
-// (a, b) gives the same corrections.  Halo rows are read from R.
-// A row is fetched with one 16-byte load at 4-byte alignment from either
-// array (owned rows: 12-byte records, the fourth word belongs to the next
-// record and is ignored; halo rows: R), so the two kinds of rows share one
-// load instruction and one destination (no branch, no wait between them).
-// The record array carries one int of padding after its last record.
-typedef int RowLoad __attribute__((ext_vector_type(4), aligned(4)));
-constexpr int kRow12LoBits = 10;
-
-// The row as loaded (raw.x/y/z = the record's words) -> the int4 row.
-__device__ inline int4 decode_row12(int4 raw, int k, int4 seg, int p0) {
-  const int t = (k >= seg.x ? 1 : 0) + (k >= seg.y ? 1 : 0) + (k >= seg.z ? 1 : 0);
-  const int lo = p0 + (raw.x & ((1 << kRow12LoBits) - 1));
-  const int hi = p0 + (int)((unsigned)raw.x >> kRow12LoBits);
-  const bool two = t == T_DISTANCE || t == T_CONTACT;
-  return make_int4(lo | (t << 28), two ? hi : -1, raw.y, raw.z);
-}
-
-template <int BLOCK, int MODE, bool C12>
+template <int BLOCK, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     const float4* __restrict__ Xin, float4* __restrict__ Xout, float4* __restrict__ XP,
     float4* __restrict__ V, const int4* __restrict__ R, const float4* __restrict__ ANC,
@@ -430,8 +404,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
     const int* __restrict__ tile_row_start, const int* __restrict__ halo_start,
     const int* __restrict__ halo_rows, float h, int first, int P, int n, float relax,
     int finalize, float damping, float max_speed, int do_cap, float4* __restrict__ ACC, float gx,
-    float gy, float gz, const float* __restrict__ gscale, const int* __restrict__ R12,
-    const int4* __restrict__ TSEG) {
+    float gy, float gz, const float* __restrict__ gscale) {
   constexpr bool PREDICT = MODE & TM_PREDICT, FINALIZE = MODE & TM_FINALIZE;
   constexpr bool HAS_ACC = MODE & TM_ACC, HAS_ANCHOR = MODE & TM_ANCHOR;
   constexpr bool NEXT = MODE & TM_NEXT;
@@ -462,19 +435,6 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
   const int hb = halo_start[tile], he = halo_start[tile + 1];
   const int owned = re - rb;
   const int total = owned + (he - hb);
-  const int4 seg = C12 ? TSEG[tile] : make_int4(0, 0, 0, 0);
-  // rows are loaded raw and decoded only when projected, so a pass's loads
-  // stay in flight while the previous pass is projected
-  auto fetch = [&](int k, int j) -> int4 {
-    if (!C12) return R[j];
-    const bool own = k < owned;
-    const int* base = own ? R12 : (const int*)R;
-    const RowLoad v = *(const RowLoad*)(base + (size_t)j * (own ? 3 : 4));
-    return make_int4(v.x, v.y, v.z, v.w);
-  };
-  auto decode = [&](int4 raw, int k) -> int4 {
-    return (C12 && k < owned) ? decode_row12(raw, k, seg, p0) : raw;
-  };
   constexpr int U = 2;
   int4 row[U];
   float lam[U];
@@ -486,7 +446,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
       jj[u] = -1;
       if (k < total) {
         jj[u] = k < owned ? rb + k : halo_rows[hb + (k - owned)];
-        row[u] = fetch(k, jj[u]);
+        row[u] = R[jj[u]];
         lam[u] = (ZERO_LAM || first) ? 0.0f : LAMin[jj[u]];
       }
     }
@@ -531,7 +491,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
       njj[u] = -1;
       if (k < total) {
         njj[u] = k < owned ? rb + k : halo_rows[hb + (k - owned)];
-        nrow[u] = fetch(k, njj[u]);
+        nrow[u] = R[njj[u]];
         nlam[u] = (ZERO_LAM || first) ? 0.0f : LAMin[njj[u]];
       }
     }
@@ -541,7 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
       const int j = jj[u];
       const Proj p = project_row<true>(
           pos, prev, [&]() { return HAS_ANCHOR ? ANC[j] : make_float4(0.f, 0.f, 0.f, 0.f); },
-          decode(row[u], k0 + u * BLOCK), lam[u], inv_dt2);
+          row[u], lam[u], inv_dt2);
       if (k0 + u * BLOCK < owned) LAMout[jj[u]] = p.lam;
       if (p.na >= 1) add(p.a, p.ca);
       if (p.na == 2) add(p.b, p.cb);
@@ -1050,18 +1010,12 @@ void launch_tile_mode(hipStream_t s, int nb, size_t shmem, const float* Xin, flo
                       float* LAMout, const int* trs, const int* hs, const int* hr, float h,
                       int first, int tile_size, int n, float relax, float damping,
                       float max_speed, int do_cap, float* ACC, float gx, float gy, float gz,
-                      const float* gscale, const int* R12, const int* TSEG) {
-#define XPBD_TILE_KERNEL(C)                                                                    \
-  hipLaunchKernelGGL((k_jacobi_tile<BLOCK, MODE, C>), dim3(nb), dim3(BLOCK), shmem, s,         \
-                     (const float4*)Xin, (float4*)Xout, (float4*)XP, (float4*)V, (const int4*)R, \
-                     (const float4*)ANC, LAMin, LAMout, trs, hs, hr, h, first, tile_size, n,     \
-                     relax, (MODE & TM_FINALIZE) ? 1 : 0, damping, max_speed, do_cap,           \
-                     (float4*)ACC, gx, gy, gz, gscale, R12, (const int4*)TSEG)
-  if (R12)
-    XPBD_TILE_KERNEL(true);
-  else
-    XPBD_TILE_KERNEL(false);
-#undef XPBD_TILE_KERNEL
+                      const float* gscale) {
+  hipLaunchKernelGGL((k_jacobi_tile<BLOCK, MODE>), dim3(nb), dim3(BLOCK), shmem, s,
+                     (const float4*)Xin, (float4*)Xout, (float4*)XP, (float4*)V, (const int4*)R,
+                     (const float4*)ANC, LAMin, LAMout, trs, hs, hr, h, first, tile_size, n, relax,
+                     (MODE & TM_FINALIZE) ? 1 : 0, damping, max_speed, do_cap, (float4*)ACC, gx,
+                     gy, gz, gscale);
 }
 
 template <int BLOCK>
@@ -1070,12 +1024,12 @@ int launch_tile_block(int mode, hipStream_t s, int nb, size_t shmem, const float
                       const float* LAMin, float* LAMout, const int* trs, const int* hs,
                       const int* hr, float h, int first, int tile_size, int n, float relax,
                       float damping, float max_speed, int do_cap, float* ACC, float gx, float gy,
-                      float gz, const float* gscale, const int* R12, const int* TSEG) {
+                      float gz, const float* gscale) {
 #define XPBD_TILE_MODE(M)                                                                      \
   case M:                                                                                      \
     launch_tile_mode<BLOCK, M>(s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin, LAMout, trs, hs, \
                                hr, h, first, tile_size, n, relax, damping, max_speed, do_cap,  \
-                               ACC, gx, gy, gz, gscale, R12, TSEG);                           \
+                               ACC, gx, gy, gz, gscale);                                      \
     return 0;
   switch (mode) {
     XPBD_TILE_MODE(0)
@@ -1110,11 +1064,9 @@ int launch_tile(bool predict, hipStream_t s, const float* Xin, float* Xout, floa
                 const int* R, const float* ANC, const float* LAMin, float* LAMout,
                 const int* tile_row_start, const int* halo_start, const int* halo_rows, float h,
                 int first, int tile_size, int n, float relax, int finalize, float damping,
-                float max_speed, float* ACC, float gx, float gy, float gz, const float* gscale,
-                const int* R12, const int* TSEG) {
+                float max_speed, float* ACC, float gx, float gy, float gz, const float* gscale) {
   if (n <= 0) return 0;
   if (tile_size < 1 || tile_size > XPBD_MAX_TILE) return (int)hipErrorInvalidValue;
-  if (R12 && (!TSEG || tile_size > (1 << kRow12LoBits))) return (int)hipErrorInvalidValue;
   const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
   const int nb = blocks_for(n, tile_size);
   const size_t shmem = tile_lds_bytes(tile_size);
@@ -1131,13 +1083,12 @@ int launch_tile(bool predict, hipStream_t s, const float* Xin, float* Xout, floa
   if (tile_size < 64)
     rc = launch_tile_block<kBlock>(mode, s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin, LAMout,
                                    tile_row_start, halo_start, halo_rows, h, first, tile_size, n,
-                                   relax, damping, max_speed, do_cap, ACC, gx, gy, gz, gscale,
-                                   R12, TSEG);
+                                   relax, damping, max_speed, do_cap, ACC, gx, gy, gz, gscale);
   else
     rc = launch_tile_block<kTileBlock>(mode, s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin,
                                        LAMout, tile_row_start, halo_start, halo_rows, h, first,
                                        tile_size, n, relax, damping, max_speed, do_cap, ACC, gx,
-                                       gy, gz, gscale, R12, TSEG);
+                                       gy, gz, gscale);
   if (rc) return rc;
   XPBD_LAUNCH_CHECK();
   return 0;
@@ -1209,13 +1160,12 @@ int xpbd_jacobi_tile(void* stream, const float* Xin, float* Xout, const float* X
                      const int* R, const float* ANC, const float* LAMin, float* LAMout,
                      const int* tile_row_start, const int* halo_start, const int* halo_rows,
                      float h, int first, int tile_size, int n_particles, float relax,
-                     int finalize, float damping, float max_speed, float* ACC, const int* R12,
-                     const int* tile_seg) {
+                     int finalize, float damping, float max_speed, float* ACC) {
   // the plain launch only reads XP (friction, finalize)
   return launch_tile(false, (hipStream_t)stream, Xin, Xout, const_cast<float*>(XP), V, R, ANC,
                      LAMin, LAMout, tile_row_start, halo_start, halo_rows, h, first, tile_size,
                      n_particles, relax, finalize ? 1 : 0, damping, max_speed, ACC, 0.0f, 0.0f,
-                     0.0f, nullptr, R12, tile_seg);
+                     0.0f, nullptr);
 }
 
 int xpbd_jacobi_tile_next(void* stream, const float* Xin, float* Xout, float* XP, const int* R,
@@ -1223,23 +1173,22 @@ int xpbd_jacobi_tile_next(void* stream, const float* Xin, float* Xout, float* XP
                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
                           float h, int first, float gx, float gy, float gz, const float* gscale,
                           int tile_size, int n_particles, float relax, float damping,
-                          float max_speed, float* ACC, const int* R12, const int* tile_seg) {
+                          float max_speed, float* ACC) {
   if (Xin == Xout) return (int)hipErrorInvalidValue;  // neighbours read Xin while tiles write
   return launch_tile(false, (hipStream_t)stream, Xin, Xout, XP, nullptr, R, ANC, LAMin, LAMout,
                      tile_row_start, halo_start, halo_rows, h, first, tile_size, n_particles,
-                     relax, 2, damping, max_speed, ACC, gx, gy, gz, gscale, R12, tile_seg);
+                     relax, 2, damping, max_speed, ACC, gx, gy, gz, gscale);
 }
 
 int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP, const float* V,
                            const int* R, const float* ANC, float* LAMout,
                            const int* tile_row_start, const int* halo_start, const int* halo_rows,
                            float h, float gx, float gy, float gz, const float* gscale,
-                           int tile_size, int n_particles, float relax, float* ACC,
-                           const int* R12, const int* tile_seg) {
+                           int tile_size, int n_particles, float relax, float* ACC) {
   if (X == Xout) return (int)hipErrorInvalidValue;  // neighbours read X while tiles write
   return launch_tile(true, (hipStream_t)stream, X, Xout, XP, const_cast<float*>(V), R, ANC,
                      nullptr, LAMout, tile_row_start, halo_start, halo_rows, h, 1, tile_size,
-                     n_particles, relax, 0, 0.0f, 0.0f, ACC, gx, gy, gz, gscale, R12, tile_seg);
+                     n_particles, relax, 0, 0.0f, 0.0f, ACC, gx, gy, gz, gscale);
 }
 
 int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP, const float* V,

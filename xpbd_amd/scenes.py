@@ -168,8 +168,7 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
     particles per tile are built on the device.
 
     Returns dict of torch tensors X, X_B, V, XP, ACC (float4 per particle), R
-    (int4 rows), R12 / TSEG (the owned rows as 12-byte records and the tiles'
-    type segments), ANC, LAM, LAM_B, TRS, HS, HR (tile tables) and counts.
+    (int4 rows), ANC, LAM, LAM_B, TRS, HS, HR (tile tables) and counts.
     ``height`` <= 0 lays the sheets on the floor (every FLOOR row active,
     x_prev = x so friction reads it)."""
     import torch
@@ -245,31 +244,15 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
     HS = torch.searchsorted(ht[so].contiguous(), torch.arange(n_tiles + 1, device=device)
                             ).to(torch.int32)
     n_dist = int(is_d.sum())
-    # the owned rows as 12-byte records and the tiles' type segments
-    # (xpbd_jacobi_tile R12 / tile_seg): lo - p0 | (hi - p0) << 10, p0, p1
-    row_tile = lo // tile
-    base = row_tile * tile
-    lo_rel, hi_rel = lo - base, hi - base
-    assert int(hi_rel.max()) < (1 << 22) and tile <= 1024
-    w0 = lo_rel | (hi_rel << 10)
-    w0 = torch.where(w0 >= (1 << 31), w0 - (1 << 32), w0)
-    R12 = torch.zeros(3 * m + 1, device=device, dtype=torch.int32)   # + 1 int of padding
-    rec = R12[:3 * m].view(m, 3)
-    rec[:, 0] = w0.to(torch.int32)
-    rec[:, 1:] = R[:, 2:]
-    per = torch.bincount(row_tile * 4 + t, minlength=4 * n_tiles).view(n_tiles, 4)
-    TSEG = torch.zeros((n_tiles, 4), device=device, dtype=torch.int32)
-    TSEG[:, :3] = torch.cumsum(per[:, :3], dim=1).to(torch.int32)
     X = Xo[order].contiguous()
     del a, b, t, lo, hi, rest, comp, p0, p1, is_d, Xo, ro, cross, ht, so
-    del row_tile, base, lo_rel, hi_rel, w0, per, rec
     return dict(
         X=X, X_B=X.clone(), V=torch.zeros((n, 4), device=device),
         XP=X.clone() if height <= 0 else torch.zeros((n, 4), device=device),
         ACC=torch.zeros((n, 4), device=device), R=R,
         # no ATTACH rows: the anchor array is never read, one slot suffices
         ANC=torch.zeros((1, 4), device=device), LAM=torch.zeros(m, device=device),
-        LAM_B=torch.zeros(m, device=device), TRS=TRS, HS=HS, R12=R12, TSEG=TSEG,
+        LAM_B=torch.zeros(m, device=device), TRS=TRS, HS=HS,
         HR=HR if HR.numel() else torch.zeros(1, dtype=torch.int32, device=device),
         n_particles=n, n_rows=m, n_distance=n_dist, tile=tile, n_halo=int(HR.numel()))
 

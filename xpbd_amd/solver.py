@@ -226,47 +226,6 @@ def tile_tables(rows_sorted: np.ndarray, n_particles: int, tile: int):
     return starts.astype(np.int32), halo_start, halo_rows
 
 
-ROW12_LO_BITS = 10
-ROW12_HI_LIMIT = 1 << (32 - ROW12_LO_BITS)
-
-
-def compact_tile_rows(rows_sorted: np.ndarray, tile_row_start: np.ndarray, tile: int):
-    """The fused Jacobi kernel's 12-byte owned-row records (3 m + 1 ints) and
-    per-tile type segments (include/xpbd.h, xpbd_jacobi_tile: R12 /
-    tile_seg), or None
-    when a row does not fit the format (tiles over 1024 particles, a second
-    endpoint 2^22 or more particles after its tile's start, or a tile whose
-    rows are not grouped by type)."""
-    m = rows_sorted.shape[0]
-    n_tiles = len(tile_row_start) - 1
-    if m == 0 or tile > (1 << ROW12_LO_BITS):
-        return None
-    counts = np.diff(tile_row_start.astype(np.int64))
-    row_tile = np.repeat(np.arange(n_tiles, dtype=np.int64), counts)
-    if len(row_tile) != m:
-        return None
-    t = (rows_sorted[:, 0].view(np.uint32) >> 28).astype(np.int64)
-    a = (rows_sorted[:, 0] & 0x0FFFFFFF).astype(np.int64)
-    b = rows_sorted[:, 1].astype(np.int64)
-    two = (t == DISTANCE) | (t == CONTACT)
-    p0 = row_tile * tile
-    lo = np.where(two, np.minimum(a, b), a) - p0
-    hi = np.where(two, np.maximum(a, b), a) - p0
-    if lo.min() < 0 or lo.max() >= tile or hi.min() < 0 or hi.max() >= ROW12_HI_LIMIT:
-        return None
-    if np.any((np.diff(row_tile) == 0) & (np.diff(t) < 0)):
-        return None
-    # one int of padding: the kernel reads each record with a 16-byte load
-    r12 = np.zeros(3 * m + 1, np.int32)
-    rec = r12[:3 * m].reshape(m, 3)
-    rec[:, 0] = (lo | (hi << ROW12_LO_BITS)).astype(np.uint32).view(np.int32)
-    rec[:, 1:] = rows_sorted[:, 2:]
-    per = np.bincount(row_tile * 4 + t, minlength=4 * n_tiles).reshape(n_tiles, 4)
-    seg = np.zeros((n_tiles, 4), np.int32)
-    seg[:, :3] = np.cumsum(per[:, :3], axis=1)
-    return r12, seg
-
-
 class Solver:
     """3D XPBD particle solver (solver.py:57-126).
 
@@ -312,9 +271,6 @@ class Solver:
         # False = a separate integrate pass (identical results)
         self.fuse_first = True
         self.block_frames = "auto"    # small scenes: whole frames in one workgroup
-        # tile mode: owned rows also as 12-byte records (same results, 25 %
-        # fewer row bytes per launch), when every row fits the format
-        self.compact_rows = True
         self._block_active = False
         # device-side CONTACT generation (Jacobi tile mode): an extension
         # that replaces the per-frame host broad phase; see enable_device_contacts
@@ -726,13 +682,9 @@ class Solver:
                         else (1, 0))
         self._tile = 0
         self._tile_tabs = None
-        self.R12 = self.TSEG = None
-        compact = None
         if tile_mode and n_c:
             self._tile = self.tile_size or default_tile_size(n_b)
             self._tile_tabs = tile_tables(rows, n_b, self._tile)
-            if self.compact_rows:
-                compact = compact_tile_rows(rows, self._tile_tabs[0], self._tile)
         self._pperm, self._pinv = pperm, pinv
         dev = self.device
         self.X = torch.from_numpy(X).to(dev)
@@ -750,9 +702,6 @@ class Solver:
             self._tile_dev = (torch.from_numpy(ts).to(dev), torch.from_numpy(hs).to(dev),
                               torch.from_numpy(np.ascontiguousarray(
                                   hr if len(hr) else np.zeros(1, np.int32))).to(dev))
-            if compact is not None:
-                self.R12 = torch.from_numpy(compact[0]).to(dev)
-                self.TSEG = torch.from_numpy(compact[1]).to(dev)
         else:
             self.X_B = self.LAM_B = None
             self._tile_dev = None
@@ -844,29 +793,15 @@ class Solver:
         rows, anc = pack_rows(h["type"], ra, rb, h["rest"], h["compliance"],
                               h["anchor"], h["mu"])
         if len(rows):
-            rows = np.ascontiguousarray(rows[self._perm])
-            self.R.copy_(self._t(rows))
+            self.R.copy_(self._t(np.ascontiguousarray(rows[self._perm])))
             self.ANC.copy_(self._t(np.ascontiguousarray(anc[self._perm])))
-            if self.R12 is not None:
-                rec = self.R12[:3 * len(rows)].view(len(rows), 3)
-                rec[:, 1:].copy_(self._t(np.ascontiguousarray(rows[:, 2:])))
 
     # ---- the step (solver.py:412-467) --------------------------------------
     def _signature(self):
         return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
                 self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling,
                 self._tile, getattr(self, "_ccap", 0), str(self.fuse_first), self._ws_gen,
-                str(self.block_frames), int(self.warm_start_every), self._compact_active())
-
-    def _compact_active(self) -> bool:
-        return bool(self.compact_rows) and getattr(self, "R12", None) is not None
-
-    def compact_row_ptrs(self) -> tuple:
-        """(R12, tile_seg) device pointers for the tile entry points, or
-        (None, None) when the launches read every row from R."""
-        if self._compact_active():
-            return N.ptr(self.R12), N.ptr(self.TSEG)
-        return None, None
+                str(self.block_frames), int(self.warm_start_every))
 
     # "auto" limit of whole-frame workgroups in Jacobi mode: above it the
     # tile path's 256 CUs beat one CU (cloths, ms per frame, workgroup
@@ -916,8 +851,6 @@ class Solver:
             d.x_alt, d.lam_alt = N.ptr(self.X_B), N.ptr(self.LAM_B)
             d.tile_row_start, d.halo_start, d.halo_rows = (N.ptr(t) for t in self._tile_dev)
             d.tile_size = self._tile
-            if self._compact_active():
-                d.tile_rows12, d.tile_seg = N.ptr(self.R12), N.ptr(self.TSEG)
             if self.GS is not None:
                 d.gscale = N.ptr(self.GS)
             if not self.fuse_first:

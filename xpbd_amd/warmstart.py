@@ -180,8 +180,7 @@ class _Eager:
             N.check(self.lib.xpbd_jacobi_tile(
                 st, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), anc,
                 N.ptr(ls), N.ptr(ld), trs, hs, hr, self.h, int(it == 0 and theta is None),
-                s._tile, self.n, s.jacobi_relax, fin, s.damping, s.max_speed, acc,
-                *s.compact_row_ptrs()))
+                s._tile, self.n, s.jacobi_relax, fin, s.damping, s.max_speed, acc))
             src, dst, ls, ld = dst, src, ld, ls
         if src is not X:
             X.copy_(src)

@@ -396,21 +396,25 @@ def warm_start_section(res: int, frames: int) -> dict:
     amort, _ = ws.Evaluator(s, theta_frame).run(substeps=60, plain_iters=(),
                                                 learned_iters=(1, 2), net_every=15)
     resid.update({f"{k}_net_per_frame": v for k, v in amort.items()})
-    sps, shapes = {}, {}
+    sps, shapes, judged = {}, {}, {}
     for name, iters, learned, every in [("plain_4", 4, False, 1), ("plain_2", 2, False, 1),
                                         ("plain_1", 1, False, 1), ("learned_2", 2, True, 1),
                                         ("learned_1", 1, True, 1),
                                         ("learned_2_net_per_frame", 2, True, 15),
-                                        ("learned_1_net_per_frame", 1, True, 15)]:
+                                        ("learned_1_net_per_frame", 1, True, 15),
+                                        ("learned_1_net_per_frame_no_judge", 1, True, 15)]:
         c = cloth(iters)
         c.block_frames = False
         if learned:
             c.warm_start_every = every
+            c.warm_start_judge = not name.endswith("no_judge")
             c.set_warm_start(theta if every == 1 else theta_frame)
         best, _ = autotune.tune(c)
         shapes[name] = list(best) if best else None
         ms = time_frames(c.step, frames, 5)
         sps[name] = round(c.substeps / (ms * 1e-3), 1)
+        if learned:
+            judged[name] = c.warm_start_stats()
     return {
         "scene": f"cloth {res}x{res}, 15 substeps, Jacobi tile kernel",
         "fit_seconds": round(fit_s, 2), "fit": log[0] if log else None,
@@ -418,6 +422,7 @@ def warm_start_section(res: int, frames: int) -> dict:
         "rms_residual": {k: float(f"{v:.4e}") for k, v in resid.items()},
         "fallback_substeps_of_60": fallback,
         "substeps_per_s": sps,
+        "judge": judged,
         "launch_shapes": shapes,
         "learned1_vs_plain2": {"residual_ratio": round(resid["learned_1"] / resid["plain_2"], 3),
                                "speedup": round(sps["learned_1"] / sps["plain_2"], 3)},
@@ -431,6 +436,8 @@ def warm_start_section(res: int, frames: int) -> dict:
         "learned2_net_per_frame_vs_plain4": {
             "residual_ratio": round(resid["learned_2_net_per_frame"] / resid["plain_4"], 3),
             "speedup": round(sps["learned_2_net_per_frame"] / sps["plain_4"], 3)},
+        "learned1_net_per_frame_no_judge_vs_plain4": {
+            "speedup": round(sps["learned_1_net_per_frame_no_judge"] / sps["plain_4"], 3)},
     }
 
 
@@ -472,17 +479,21 @@ def warm_start_batched_section(frames: int) -> dict:
     resid, fallback = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(1, 2, 4),
                                                  learned_iters=(1, 2), net_every=15)
     torch.cuda.empty_cache()
-    sps = {}
+    sps, judged = {}, {}
     for name, iters, learned in [("plain_4", 4, False), ("plain_2", 2, False),
                                  ("plain_1", 1, False), ("learned_2_net_per_frame", 2, True),
-                                 ("learned_1_net_per_frame", 1, True)]:
+                                 ("learned_1_net_per_frame", 1, True),
+                                 ("learned_1_net_per_frame_no_judge", 1, True)]:
         restore()
         s.iterations = iters
         if learned:
             s.warm_start_every = 15
+            s.warm_start_judge = not name.endswith("no_judge")
             s.set_warm_start(theta)
         ms = time_frames(s.step, frames, 3)
         sps[name] = round(s.substeps / (ms * 1e-3), 1)
+        if learned:
+            judged[name] = s.warm_start_stats()
     restore()
     resid = {(f"{k}_net_per_frame" if k.startswith("learned") else k): v for k, v in resid.items()}
     out = {
@@ -493,11 +504,14 @@ def warm_start_batched_section(frames: int) -> dict:
         "rms_residual_static_rows": {k: float(f"{v:.4e}") for k, v in resid.items()},
         "fallback_substeps_of_60": fallback,
         "substeps_per_s": sps,
+        "judge": judged,
     }
     for k in (1, 2):
         out[f"learned{k}_net_per_frame_vs_plain4"] = {
             "residual_ratio": round(resid[f"learned_{k}_net_per_frame"] / resid["plain_4"], 3),
             "speedup": round(sps[f"learned_{k}_net_per_frame"] / sps["plain_4"], 3)}
+    out["learned1_net_per_frame_no_judge_vs_plain4"] = {
+        "speedup": round(sps["learned_1_net_per_frame_no_judge"] / sps["plain_4"], 3)}
     del w, s
     return out
 

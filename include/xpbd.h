@@ -187,13 +187,16 @@ int xpbd_frame_block(void* stream, float* X, float* V, float* XP, const int* R, 
  * Xout, Xin != Xout).  predict = 1 folds integrate in as
  * xpbd_jacobi_tile_first does (Xin = positions at the start of the substep;
  * XP written; V read), predict = 0 reads integrated positions.  Tile tables
- * as xpbd_jacobi_tile. */
+ * as xpbd_jacobi_tile.  ws_on (optional device int, NULL = on) switches the
+ * start off when *ws_on == 0 (invert != 0: when *ws_on != 0): lambda0 = 0
+ * for every row and Xout = the (predicted) positions, after which the
+ * iterations run exactly as from the zero start. */
 int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP, const float* V,
                           const int* R, const float* ANC, float* LAMout,
                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
                           const float* Y, float h, float gx, float gy, float gz,
                           const float* gscale, int tile_size, int n_particles, float relax,
-                          int predict);
+                          int predict, const int* ws_on, int invert);
 /* Largest particle count of xpbd_frame_block (LDS-bound). */
 int xpbd_frame_block_capacity(int gs);
 
@@ -263,6 +266,28 @@ typedef struct xpbd_particles_desc {
    * 0 of each frame (<= 1: every substep); the other substeps reuse ws_y
    * with their own first plain steps */
   int ws_every;
+  /* The plain zero start as judge and fallback of the learned start
+   * (tile mode with ws_image, fuse_first 2, substeps >= 2): ws_on (device
+   * int, 1 = learned start) is read by every warm-start launch of a frame.
+   * With ws_judge != 0, the frame's first substep also runs with the other
+   * start on the judge buffers (ws_jx, ws_jx_alt, ws_jxp float4[n]; ws_jlam,
+   * ws_jlam_alt, ws_jy float[n_rows]; ws_jacc float4[n] and ws_jdlam
+   * float[dyn_cap] with device rows) on a second stream, both substeps'
+   * squared residuals go to ws_rsum (double[2], zero at the start), and
+   * xpbd_ws_decide sets ws_on for the next frame (stats in ws_stats,
+   * int[3], optional). */
+  int* ws_on;
+  int ws_judge;
+  float* ws_jx;
+  float* ws_jx_alt;
+  float* ws_jxp;
+  float* ws_jlam;
+  float* ws_jlam_alt;
+  float* ws_jy;
+  float* ws_jacc;
+  float* ws_jdlam;
+  double* ws_rsum;
+  int* ws_stats;
 } xpbd_particles_desc;
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);
@@ -481,12 +506,21 @@ int xpbd_ws_features(void* stream, const float* X, const float* XP, const int* R
  * xpbd_ws_features; at the positions integrate would predict from Xin and V
  * when predict = 1, x_prev = Xin; otherwise at Xin with x_prev XP) and the
  * network's output Y[row] (as xpbd_mlp_forward on those features); image =
- * the network packed by xpbd_mlp_pack. */
+ * the network packed by xpbd_mlp_pack.  ws_on / invert as in
+ * xpbd_jacobi_tile_warm: with the start off the launch does nothing. */
 int xpbd_ws_net(void* stream, const float* Xin, const float* XP, const float* V, const int* R,
                 const float* ANC, const void* image, float* Y, float h, float gx, float gy,
-                float gz, const float* gscale, int n_rows, int predict);
+                float gz, const float* gscale, int n_rows, int predict, const int* ws_on,
+                int invert);
 int xpbd_ws_residual(void* stream, const float* X, const int* R, const float* ANC,
                      const float* LAM, float h, double* out_sum, int n_rows);
+/* The plain zero start as judge and fallback of the learned start (one
+ * thread): sums[0] is the squared residual of the substep the solver ran,
+ * sums[1] that of the same substep run with the other start; *ws_on becomes
+ * 1 (learned start) when the learned start's sum is no larger, else 0 (zero
+ * start); stats (optional, int[3]) counts frames run with the learned start,
+ * frames run with the zero start, and switches; the sums are cleared. */
+int xpbd_ws_decide(void* stream, double* sums, int* ws_on, int* stats);
 
 /* ---- host coloring (src/xpbd3d/coloring.py) ----------------------------- */
 

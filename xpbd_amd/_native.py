@@ -39,6 +39,10 @@ class ParticlesDesc(ctypes.Structure):
         ("ws_image", c_void_p), ("ws_y", c_void_p),
         ("fuse_first", c_int), ("block", c_int), ("color_starts_dev", c_void_p),
         ("ws_every", c_int),
+        ("ws_on", c_void_p), ("ws_judge", c_int), ("ws_jx", c_void_p), ("ws_jx_alt", c_void_p),
+        ("ws_jxp", c_void_p), ("ws_jlam", c_void_p), ("ws_jlam_alt", c_void_p),
+        ("ws_jy", c_void_p), ("ws_jacc", c_void_p), ("ws_jdlam", c_void_p),
+        ("ws_rsum", c_void_p), ("ws_stats", c_void_p),
     ]
 
 
@@ -135,8 +139,9 @@ _SIGS = {
     "xpbd_mlp_grad": (c_int, [_P, _P, _P, _P, _I, _I, _P, _P, _P]),
     "xpbd_mlp_adam": (c_int, [_P, _P, _P, _P, _P, _F, _F, _F, _F, _I]),
     "xpbd_ws_features": (c_int, [_P] * 5 + [_F, _P, _I]),
-    "xpbd_ws_net": (c_int, [_P] * 8 + [_F, _F, _F, _F, _P, _I, _I]),
-    "xpbd_jacobi_tile_warm": (c_int, [_P] * 12 + [_F, _F, _F, _F, _P, _I, _I, _F, _I]),
+    "xpbd_ws_net": (c_int, [_P] * 8 + [_F, _F, _F, _F, _P, _I, _I, _P, _I]),
+    "xpbd_jacobi_tile_warm": (c_int, [_P] * 12 + [_F, _F, _F, _F, _P, _I, _I, _F, _I, _P, _I]),
+    "xpbd_ws_decide": (c_int, [_P, _P, _P, _P]),
     "xpbd_ws_residual": (c_int, [_P] * 5 + [_F, _P, _I]),
     "xpbd_build_info": (ctypes.c_char_p, []),
     "xpbd_error_string": (ctypes.c_char_p, [c_int]),

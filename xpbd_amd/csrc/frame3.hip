@@ -29,6 +29,9 @@ struct Frame3 {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t capture_stream = nullptr;
+  // judge of the learned start: its stream and the fork / read / join events
+  hipStream_t judge_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_read = nullptr, ev_join = nullptr;
 };
 
 // Fused-tile Jacobi frame.
@@ -55,6 +58,154 @@ struct Frame3 {
 // substeps reuse its outputs), and then all `iterations` plain launches with
 // first = 0.  Multipliers ping-pong as the
 // positions do: a substep's first launch writes lam_alt.
+// Buffers of one run of the fused substep: the position and multiplier
+// ping-pong pairs, x_prev, the network outputs and the device-row sums and
+// multipliers.  `src0` (optional) is read by the first launch instead of
+// x[loc] (the judge starts from the solver's state without writing it).
+struct TileRun {
+  float* x[2];
+  float* lam[2];
+  float* xp;
+  float* y;
+  float* acc;
+  float* dlam;
+  const float* src0;
+  int loc = 0;   // x[loc] holds the state
+  int li = 0;    // lam[li] holds the multipliers after the last launch
+};
+
+// One substep of the fused tile plan (see issue_frame_tile) on stream s.
+// ws_on / invert select the start of a warm-started substep; `after_read`
+// (optional) is recorded once no launch of the run reads src0 any more,
+// `before_write` (optional) is waited for before the first launch that
+// writes x[0] of a run without src0.
+int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r, int sub,
+                        int invert, hipEvent_t after_read, hipEvent_t before_write) {
+  const int n = d.n_particles;
+  const bool have_rows = (d.n_rows > 0 || d.dyn_cap > 0) && d.iterations > 0;
+  const int it_n = have_rows ? d.iterations : 0;
+  const bool dyn = d.dyn_cap > 0 && d.dyn_rows && d.dyn_count;
+  const bool warm = d.ws_image != nullptr && d.n_rows > 0 && it_n > 0;
+  const int ws_every = d.ws_every > 1 ? d.ws_every : 1;
+  const bool next = d.fuse_first >= 2 && it_n >= 1;
+  int rc;
+  // predicted: the previous substep's last launch already integrated
+  const bool predicted = next && sub > 0;
+  const bool integrate_next = next && sub + 1 < d.substeps;
+  float* const* bufs = r.x;
+  float* const* lams = r.lam;
+  auto in = [&]() -> const float* { return r.src0 ? r.src0 : bufs[r.loc]; };
+  bool waited = before_write == nullptr;
+  auto before_launch = [&]() -> int {  // the launch about to run writes bufs[r.loc ^ 1]
+    if (!waited && (r.loc ^ 1) == 0) {
+      XPBD_CHECK(hipStreamWaitEvent(s, before_write, 0));
+      waited = true;
+    }
+    return 0;
+  };
+  auto read_done = [&]() -> int {
+    if (r.src0) {
+      r.src0 = nullptr;
+      if (after_read) XPBD_CHECK(hipEventRecord(after_read, s));
+    }
+    return 0;
+  };
+  r.li = 0;
+  int it = 0;
+  if (warm) {
+    if (sub % ws_every == 0) {
+      rc = xpbd_ws_net(s, in(), r.xp, d.v, d.rows, d.anchors, d.ws_image, r.y, d.h, d.gx, d.gy,
+                       d.gz, d.gscale, d.n_rows, predicted ? 0 : 1, d.ws_on, invert);
+      if (rc) return rc;
+    }
+    if ((rc = before_launch())) return rc;
+    rc = xpbd_jacobi_tile_warm(s, in(), bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors, lams[1],
+                               d.tile_row_start, d.halo_start, d.halo_rows, r.y, d.h, d.gx, d.gy,
+                               d.gz, d.gscale, d.tile_size, n, d.relax, predicted ? 0 : 1,
+                               d.ws_on, invert);
+    if (rc) return rc;
+    if ((rc = read_done())) return rc;
+    r.loc ^= 1;
+    r.li = 1;
+  } else if (!predicted && it_n >= 2) {
+    if (dyn) {  // device-counted rows, first iteration on the predicted positions
+      rc = xpbd_solve_jacobi_dyn_first(s, in(), d.v, d.dyn_rows, r.dlam, r.acc, d.h, d.gx, d.gy,
+                                       d.gz, d.gscale, d.dyn_count, d.dyn_cap);
+      if (rc) return rc;
+    }
+    if ((rc = before_launch())) return rc;
+    rc = xpbd_jacobi_tile_first(s, in(), bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors, lams[1],
+                                d.tile_row_start, d.halo_start, d.halo_rows, d.h, d.gx, d.gy,
+                                d.gz, d.gscale, d.tile_size, n, d.relax, r.acc);
+    if (rc) return rc;
+    if ((rc = read_done())) return rc;
+    r.loc ^= 1;
+    r.li = 1;
+    it = 1;
+  } else if (!predicted) {  // one iteration: integrate in place, then the chain
+    rc = xpbd_integrate_to(s, bufs[r.loc], bufs[r.loc], d.v, r.xp, d.gx, d.gy, d.gz, d.h,
+                           d.gscale, n);
+    if (rc) return rc;
+  }
+  for (; it < it_n; ++it) {
+    const int first = (!warm && it == 0) ? 1 : 0;
+    if (dyn) {  // device rows start from zero (the warm start covers `rows` only)
+      rc = xpbd_solve_jacobi_dyn(s, bufs[r.loc], r.xp, d.dyn_rows, r.dlam, r.acc, d.h,
+                                 it == 0 ? 1 : 0, d.dyn_count, d.dyn_cap);
+      if (rc) return rc;
+    }
+    if ((rc = before_launch())) return rc;
+    if (it + 1 == it_n && integrate_next) {
+      rc = xpbd_jacobi_tile_next(s, bufs[r.loc], bufs[r.loc ^ 1], r.xp, d.rows, d.anchors,
+                                 lams[r.li], lams[r.li ^ 1], d.tile_row_start, d.halo_start,
+                                 d.halo_rows, d.h, first, d.gx, d.gy, d.gz, d.gscale,
+                                 d.tile_size, n, d.relax, d.damping, d.max_speed, r.acc);
+    } else {
+      rc = xpbd_jacobi_tile(s, bufs[r.loc], bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors,
+                            lams[r.li], lams[r.li ^ 1], d.tile_row_start, d.halo_start,
+                            d.halo_rows, d.h, first, d.tile_size, n, d.relax, it + 1 == it_n,
+                            d.damping, d.max_speed, r.acc);
+    }
+    if (rc) return rc;
+    r.loc ^= 1;
+    r.li ^= 1;
+  }
+  return 0;
+}
+
+// Fused-tile Jacobi frame.
+//
+// With fuse_first, each substep is `iterations` k_jacobi_tile launches only
+// (plus, with device-counted rows, their projection before each): the first
+// has integrate folded in (xpbd_jacobi_tile_first, and
+// xpbd_solve_jacobi_dyn_first for the device rows), the last runs finalize +
+// cap.  With fuse_first = 2 only the frame's first substep does so (or, with
+// one iteration, runs an integrate pass in place); the last launch of every
+// other substep also integrates the next one (xpbd_jacobi_tile_next), whose
+// first launch is then a plain one with first = 1.  Launches
+// ping-pong between X and X_alt; `loc` tracks which one holds the state, and
+// a frame that ends in X_alt copies it back.
+//
+// Otherwise (contact rows projected by a separate launch before each
+// iteration, or fuse_first off) a substep is one integrate pass (X -> X or
+// X_alt, chosen so that after the substep's launches the state is back in X)
+// and the plain launches.
+//
+// With the learned warm start (ws_image) every substep first runs
+// xpbd_jacobi_tile_warm (integrate folded in on the fused path), preceded
+// on every ws_every-th substep by the network pass xpbd_ws_net (the other
+// substeps reuse its outputs), and then all `iterations` plain launches with
+// first = 0.  Multipliers ping-pong as the positions do: a substep's first
+// launch writes lam_alt.
+//
+// Judge (ws_judge, fuse_first 2, >= 2 substeps): the frame's first substep
+// also runs on a second stream with the other start (the zero start while
+// the learned one is on, and the reverse), from the same state into the
+// judge buffers; both substeps' squared residuals (positions after the
+// iterations: x_prev, which the substep's last launch sets to them) are
+// summed, and after the frame xpbd_ws_decide keeps the learned start for the
+// next frame only if its residual was no larger.  The solver's own launches
+// that overwrite X wait until the judge has read it.
 int issue_frame_tile(Frame3* f, hipStream_t s) {
   const xpbd_particles_desc& d = f->d;
   const int n = d.n_particles;
@@ -63,81 +214,48 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
   const int it_n = have_rows ? d.iterations : 0;
   const bool dyn = d.dyn_cap > 0 && d.dyn_rows && d.dyn_count;
   const bool warm = d.ws_image != nullptr && d.n_rows > 0 && it_n > 0;
-  const int ws_every = d.ws_every > 1 ? d.ws_every : 1;
   float* bufs[2] = {d.x, d.x_alt};
   float* lams[2] = {d.lam, d.lam_alt};
   int rc;
   const bool next = d.fuse_first >= 2 && it_n >= 1;
   if (d.fuse_first && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1) || next)) {
-    int loc = 0;
-    float* acc = dyn ? d.acc : nullptr;
+    TileRun r{{d.x, d.x_alt}, {d.lam, d.lam_alt}, d.x_prev, d.ws_y, dyn ? d.acc : nullptr,
+              d.dyn_lam, nullptr};
+    const bool judge = warm && f->judge_stream != nullptr;
     for (int sub = 0; sub < d.substeps; ++sub) {
-      // predicted: the previous substep's last launch already integrated
-      const bool predicted = next && sub > 0;
-      const bool integrate_next = next && sub + 1 < d.substeps;
-      int li = 0;  // lams[li] holds the current multipliers
-      int it = 0;
-      if (warm) {
-        if (sub % ws_every == 0) {
-          rc = xpbd_ws_net(s, bufs[loc], d.x_prev, d.v, d.rows, d.anchors, d.ws_image, d.ws_y,
-                           d.h, d.gx, d.gy, d.gz, d.gscale, d.n_rows, predicted ? 0 : 1);
-          if (rc) return rc;
-        }
-        rc = xpbd_jacobi_tile_warm(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
-                                   lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.ws_y,
-                                   d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n, d.relax,
-                                   predicted ? 0 : 1);
+      if (judge && sub == 0) {
+        hipStream_t js = f->judge_stream;
+        XPBD_CHECK(hipEventRecord(f->ev_fork, s));
+        XPBD_CHECK(hipStreamWaitEvent(js, f->ev_fork, 0));
+        TileRun jr{{d.ws_jx, d.ws_jx_alt}, {d.ws_jlam, d.ws_jlam_alt}, d.ws_jxp, d.ws_jy,
+                   dyn ? d.ws_jacc : nullptr, d.ws_jdlam, d.x};
+        rc = issue_fused_substep(d, js, jr, 0, 1, f->ev_read, nullptr);
         if (rc) return rc;
-        loc ^= 1;
-        li = 1;
-      } else if (!predicted && it_n >= 2) {
-        if (dyn) {  // device-counted rows, first iteration on the predicted positions
-          rc = xpbd_solve_jacobi_dyn_first(s, bufs[loc], d.v, d.dyn_rows, d.dyn_lam, d.acc, d.h,
-                                           d.gx, d.gy, d.gz, d.gscale, d.dyn_count, d.dyn_cap);
-          if (rc) return rc;
-        }
-        rc = xpbd_jacobi_tile_first(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows,
-                                    d.anchors, lams[1], d.tile_row_start, d.halo_start,
-                                    d.halo_rows, d.h, d.gx, d.gy, d.gz, d.gscale, d.tile_size, n,
-                                    d.relax, acc);
+        rc = xpbd_ws_residual(js, jr.xp, d.rows, d.anchors, jr.lam[jr.li], d.h, d.ws_rsum + 1,
+                              d.n_rows);
         if (rc) return rc;
-        loc ^= 1;
-        li = 1;
-        it = 1;
-      } else if (!predicted) {  // one iteration: integrate in place, then the chain
-        rc = xpbd_integrate_to(s, bufs[loc], bufs[loc], d.v, d.x_prev, d.gx, d.gy, d.gz, d.h,
-                               d.gscale, n);
+        XPBD_CHECK(hipEventRecord(f->ev_join, js));
+        rc = issue_fused_substep(d, s, r, 0, 0, nullptr, f->ev_read);
         if (rc) return rc;
+        rc = xpbd_ws_residual(s, r.xp, d.rows, d.anchors, r.lam[r.li], d.h, d.ws_rsum, d.n_rows);
+        if (rc) return rc;
+        continue;
       }
-      for (; it < it_n; ++it) {
-        const int first = (!warm && it == 0) ? 1 : 0;
-        if (dyn) {  // device rows start from zero (the warm start covers `rows` only)
-          rc = xpbd_solve_jacobi_dyn(s, bufs[loc], d.x_prev, d.dyn_rows, d.dyn_lam, d.acc, d.h,
-                                     it == 0 ? 1 : 0, d.dyn_count, d.dyn_cap);
-          if (rc) return rc;
-        }
-        if (it + 1 == it_n && integrate_next) {
-          rc = xpbd_jacobi_tile_next(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.rows, d.anchors,
-                                     lams[li], lams[li ^ 1], d.tile_row_start, d.halo_start,
-                                     d.halo_rows, d.h, first, d.gx, d.gy, d.gz, d.gscale,
-                                     d.tile_size, n, d.relax, d.damping, d.max_speed, acc);
-        } else {
-          rc = xpbd_jacobi_tile(s, bufs[loc], bufs[loc ^ 1], d.x_prev, d.v, d.rows, d.anchors,
-                                lams[li], lams[li ^ 1], d.tile_row_start, d.halo_start,
-                                d.halo_rows, d.h, first, d.tile_size, n, d.relax, it + 1 == it_n,
-                                d.damping, d.max_speed, acc);
-        }
-        if (rc) return rc;
-        loc ^= 1;
-        li ^= 1;
-      }
+      rc = issue_fused_substep(d, s, r, sub, 0, nullptr, nullptr);
+      if (rc) return rc;
     }
-    if (loc) {
+    if (judge) {
+      XPBD_CHECK(hipStreamWaitEvent(s, f->ev_join, 0));
+      rc = xpbd_ws_decide(s, d.ws_rsum, d.ws_on, d.ws_stats);
+      if (rc) return rc;
+    }
+    if (r.loc) {
       XPBD_CHECK(hipMemcpyAsync(d.x, d.x_alt, sizeof(float) * 4 * (size_t)n,
                                 hipMemcpyDeviceToDevice, s));
     }
     return 0;
   }
+  const int ws_every = d.ws_every > 1 ? d.ws_every : 1;
   const int launches = it_n + (warm ? 1 : 0);
   for (int sub = 0; sub < d.substeps; ++sub) {
     int cur = launches & 1;
@@ -147,12 +265,13 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
     if (warm) {
       if (sub % ws_every == 0) {
         rc = xpbd_ws_net(s, bufs[cur], d.x_prev, d.v, d.rows, d.anchors, d.ws_image, d.ws_y,
-                         d.h, 0.0f, 0.0f, 0.0f, nullptr, d.n_rows, 0);
+                         d.h, 0.0f, 0.0f, 0.0f, nullptr, d.n_rows, 0, d.ws_on, 0);
         if (rc) return rc;
       }
       rc = xpbd_jacobi_tile_warm(s, bufs[cur], bufs[cur ^ 1], d.x_prev, d.v, d.rows, d.anchors,
                                  lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.ws_y,
-                                 d.h, 0.0f, 0.0f, 0.0f, nullptr, d.tile_size, n, d.relax, 0);
+                                 d.h, 0.0f, 0.0f, 0.0f, nullptr, d.tile_size, n, d.relax, 0,
+                                 d.ws_on, 0);
       if (rc) return rc;
       cur ^= 1;
       li = 1;
@@ -263,12 +382,30 @@ int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
        desc->n_particles > xpbd_frame_block_capacity(desc->mode == XPBD_MODE_GS) ||
        (desc->mode == XPBD_MODE_GS && desc->n_rows > 0 && !desc->color_starts_dev)))
     return (int)hipErrorInvalidValue;
+  const bool judge = desc->ws_judge != 0;
+  if (judge &&
+      (!desc->ws_image || desc->mode != XPBD_MODE_JACOBI_TILE || desc->block ||
+       desc->fuse_first < 2 || desc->substeps < 2 || desc->iterations < 1 || desc->n_rows <= 0 ||
+       !desc->ws_on || !desc->ws_rsum || !desc->ws_jx || !desc->ws_jx_alt || !desc->ws_jxp ||
+       !desc->ws_jlam || !desc->ws_jlam_alt || !desc->ws_jy ||
+       (desc->dyn_cap > 0 && (!desc->ws_jacc || !desc->ws_jdlam))))
+    return (int)hipErrorInvalidValue;  // the judge runs in the fused plan's shape only
   Frame3* f = new (std::nothrow) Frame3();
   if (!f) return (int)hipErrorOutOfMemory;
   f->d = *desc;
   if (desc->mode == XPBD_MODE_GS && desc->n_rows > 0)
     f->color_starts.assign(desc->color_starts, desc->color_starts + desc->num_colors + 1);
   f->d.color_starts = nullptr;
+  if (judge) {
+    hipError_t e = hipStreamCreateWithFlags(&f->judge_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ev_read, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ev_join, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      xpbd_frame3_destroy(f);
+      return (int)e;
+    }
+  }
   *handle = f;
   return 0;
 }
@@ -318,6 +455,9 @@ int xpbd_frame3_destroy(void* handle) {
   if (!f) return 0;
   drop_graph(f);
   if (f->capture_stream) (void)hipStreamDestroy(f->capture_stream);
+  if (f->judge_stream) (void)hipStreamDestroy(f->judge_stream);
+  for (hipEvent_t e : {f->ev_fork, f->ev_read, f->ev_join})
+    if (e) (void)hipEventDestroy(e);
   delete f;
   return 0;
 }

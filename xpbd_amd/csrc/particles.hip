@@ -590,8 +590,11 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_warm(
     float* __restrict__ LAMout, const int* __restrict__ tile_row_start,
     const int* __restrict__ halo_start, const int* __restrict__ halo_rows,
     const float* __restrict__ Y, float h, int P, int n, float relax, float gx, float gy,
-    float gz, const float* __restrict__ gscale) {
+    float gz, const float* __restrict__ gscale, const int* __restrict__ ws_on, int invert) {
   extern __shared__ unsigned long long lds64[];
+  // switched off: lambda0 = 0 for every row, positions as predicted (the
+  // following iterations then run exactly as from the zero start)
+  const bool active = xpbd_ws::start_active(ws_on, invert);
   unsigned long long* sx = lds64;
   unsigned long long* sy = lds64 + P;
   unsigned long long* sz = lds64 + 2 * P;
@@ -671,7 +674,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_warm(
     const float sden = w + alpha;
     float step = (sden > 0.0f && ok) ? -C * __builtin_amdgcn_rcpf(sden) : 0.0f;
     if (t == T_FLOOR || t == T_CONTACT) step = C < 0.0f ? step : 0.0f;
-    const float lam0 = xpbd_ws::start_lambda(yv, step);
+    const float lam0 = active ? xpbd_ws::start_lambda(yv, step) : 0.0f;
     if (k < owned) LAMout[j] = lam0;
     if (lam0 == 0.0f) continue;
     add(a, nrm * (wa * lam0));
@@ -1196,7 +1199,7 @@ int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP
                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
                           const float* Y, float h, float gx, float gy, float gz,
                           const float* gscale, int tile_size, int n_particles, float relax,
-                          int predict) {
+                          int predict, const int* ws_on, int invert) {
   if (n_particles <= 0) return 0;
   if (tile_size < 1 || tile_size > XPBD_MAX_TILE || !Y || Xin == Xout)
     return (int)hipErrorInvalidValue;
@@ -1207,7 +1210,8 @@ int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP
   hipLaunchKernelGGL((k_jacobi_tile_warm<B, PR>), dim3(nb), dim3(B), shmem, s,                 \
                      (const float4*)Xin, (float4*)Xout, (float4*)XP, (const float4*)V,         \
                      (const int4*)R, (const float4*)ANC, LAMout, tile_row_start, halo_start,   \
-                     halo_rows, Y, h, tile_size, n_particles, relax, gx, gy, gz, gscale)
+                     halo_rows, Y, h, tile_size, n_particles, relax, gx, gy, gz, gscale, ws_on,  \
+                     invert)
   if (tile_size < 64) {
     if (predict) XPBD_WARM_LAUNCH(kBlock, true);
     else XPBD_WARM_LAUNCH(kBlock, false);

@@ -110,8 +110,11 @@ __global__ __launch_bounds__(kNetBlock) void k_ws_net(
     const float4* __restrict__ Xin, const float4* __restrict__ XP, const float4* __restrict__ V,
     const int4* __restrict__ R, const float4* __restrict__ ANC,
     const xpbd_mlp::Image* __restrict__ img, float* __restrict__ Y, float h, float gx, float gy,
-    float gz, const float* __restrict__ gscale, int m) {
+    float gz, const float* __restrict__ gscale, int m, const int* __restrict__ ws_on,
+    int invert) {
   using namespace xpbd_mlp;
+  // the learned start switched off for this launch: nothing to predict
+  if (!xpbd_ws::start_active(ws_on, invert)) return;
   __shared__ NetLds<ROWS> s;
   for (int i = threadIdx.x; i < NH; i += kNetBlock) {
     s.b1[i] = img->w.b1[i];
@@ -167,18 +170,39 @@ __global__ __launch_bounds__(kNetBlock) void k_ws_net(
 template <int ROWS>
 void launch_net(hipStream_t s, const float* Xin, const float* XP, const float* V, const int* R,
                 const float* ANC, const xpbd_mlp::Image* img, float* Y, float h, float gx,
-                float gy, float gz, const float* gscale, int n_rows, int predict) {
+                float gy, float gz, const float* gscale, int n_rows, int predict,
+                const int* ws_on, int invert) {
   const int groups = (n_rows + ROWS - 1) / ROWS;
   int nb = (groups + kNetWaves - 1) / kNetWaves;
   if (nb > kNetMaxBlocks) nb = kNetMaxBlocks;
   if (predict)
     hipLaunchKernelGGL((k_ws_net<true, ROWS>), dim3(nb), dim3(kNetBlock), 0, s,
                        (const float4*)Xin, (const float4*)XP, (const float4*)V, (const int4*)R,
-                       (const float4*)ANC, img, Y, h, gx, gy, gz, gscale, n_rows);
+                       (const float4*)ANC, img, Y, h, gx, gy, gz, gscale, n_rows, ws_on, invert);
   else
     hipLaunchKernelGGL((k_ws_net<false, ROWS>), dim3(nb), dim3(kNetBlock), 0, s,
                        (const float4*)Xin, (const float4*)XP, (const float4*)V, (const int4*)R,
-                       (const float4*)ANC, img, Y, h, gx, gy, gz, gscale, n_rows);
+                       (const float4*)ANC, img, Y, h, gx, gy, gz, gscale, n_rows, ws_on, invert);
+}
+
+// The judge's decision after a frame (one thread): sums[0] = squared
+// residual of the solver's own substep, sums[1] = that of the same substep
+// run with the other start; the learned start stays (or becomes) active for
+// the next frame when its residual is no larger than the zero start's.
+// stats: frames run with the learned start, frames run with the zero start,
+// switches.  The sums are cleared for the next frame.
+__global__ void k_ws_decide(double* sums, int* ws_on, int* stats) {
+  const int active = *ws_on != 0;
+  const double learned = active ? sums[0] : sums[1];
+  const double zero = active ? sums[1] : sums[0];
+  const int next = learned <= zero ? 1 : 0;
+  if (stats) {
+    stats[active ? 0 : 1] += 1;
+    stats[2] += next != active;
+  }
+  *ws_on = next;
+  sums[0] = 0.0;
+  sums[1] = 0.0;
 }
 
 // Residual: sum over rows of r^2, r = C + alpha~ lambda (two-sided rows),
@@ -227,7 +251,8 @@ int xpbd_ws_features(void* stream, const float* X, const float* XP, const int* R
 
 int xpbd_ws_net(void* stream, const float* Xin, const float* XP, const float* V, const int* R,
                 const float* ANC, const void* image, float* Y, float h, float gx, float gy,
-                float gz, const float* gscale, int n_rows, int predict) {
+                float gz, const float* gscale, int n_rows, int predict, const int* ws_on,
+                int invert) {
   if (n_rows <= 0) return 0;
   if (!image || !Y) return (int)hipErrorInvalidValue;
   const xpbd_mlp::Image* img = (const xpbd_mlp::Image*)image;
@@ -235,9 +260,18 @@ int xpbd_ws_net(void* stream, const float* Xin, const float* XP, const float* V,
   // 64-row groups from 256 groups up (measured on cloths: 113 k rows 14.5
   // against 21.2 us with 16-row groups, 1.8 M rows 110 against 147 us)
   if (n_rows >= 64 * 256)
-    launch_net<64>(s, Xin, XP, V, R, ANC, img, Y, h, gx, gy, gz, gscale, n_rows, predict);
+    launch_net<64>(s, Xin, XP, V, R, ANC, img, Y, h, gx, gy, gz, gscale, n_rows, predict, ws_on,
+                   invert);
   else
-    launch_net<16>(s, Xin, XP, V, R, ANC, img, Y, h, gx, gy, gz, gscale, n_rows, predict);
+    launch_net<16>(s, Xin, XP, V, R, ANC, img, Y, h, gx, gy, gz, gscale, n_rows, predict, ws_on,
+                   invert);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_ws_decide(void* stream, double* sums, int* ws_on, int* stats) {
+  if (!sums || !ws_on) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ws_decide, dim3(1), dim3(1), 0, (hipStream_t)stream, sums, ws_on, stats);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

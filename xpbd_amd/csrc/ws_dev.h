@@ -44,6 +44,14 @@ __device__ inline void row_features(int t, float C, float w, float alpha, float 
   f[11] = __builtin_amdgcn_logf(1.0f + alpha) * (kLog10of2 * 0.125f);
 }
 
+// Whether the learned start is in use for a launch: ws_on (device flag of
+// the frame plan's judge; NULL = always) and `invert` (the judge runs the
+// start the solver is not using).
+__device__ inline bool start_active(const int* ws_on, int invert) {
+  const bool on = ws_on ? (*ws_on != 0) : true;
+  return on != (invert != 0);
+}
+
 // lambda0 of a row from the network output y; `step` is the row's first
 // plain step -C / (w + alpha~) (zero when the row is inactive).
 __device__ inline float start_lambda(float y, float step) {

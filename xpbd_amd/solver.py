@@ -279,6 +279,9 @@ class Solver:
         self._ws_theta = None         # learned lambda warm start parameters
         self._ws_gen = 0
         self.warm_start_every = 1     # network pass every k-th substep of a frame
+        # the zero start judges the learned one every frame (in the frame plan)
+        self.warm_start_judge = True
+        self.WSON = self.WSST = None
 
         # host scene, one column per field (float64 as the reference's
         # Python floats; rounded to float32 once, at flush)
@@ -458,8 +461,11 @@ class Solver:
         and the matching position offset (``xpbd_ws_net`` +
         ``xpbd_jacobi_tile_warm``) before its Jacobi iterations.  ``None``
         restores the zero start.
-        Runs the Jacobi tile kernel; ``warmstart.Guard`` adds the
-        plain-start fallback."""
+        Runs the Jacobi tile kernel.  The plain zero start stays the judge
+        and fallback (``warm_start_judge``, on by default): every frame's
+        first substep also runs with the other start on a second stream,
+        and the learned start stays on for the next frame only while its
+        residual is no larger (``warm_start_stats``)."""
         import torch
 
         if theta is None:
@@ -801,7 +807,7 @@ class Solver:
         return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
                 self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling,
                 self._tile, getattr(self, "_ccap", 0), str(self.fuse_first), self._ws_gen,
-                str(self.block_frames), int(self.warm_start_every))
+                str(self.block_frames), int(self.warm_start_every), bool(self.warm_start_judge))
 
     # "auto" limit of whole-frame workgroups in Jacobi mode: above it the
     # tile path's 256 CUs beat one CU (cloths, ms per frame, workgroup
@@ -871,6 +877,27 @@ class Solver:
                                           N.ptr(self.WSI)), "xpbd_mlp_pack")
             d.ws_image, d.ws_y = N.ptr(self.WSI), N.ptr(self.WSY)
             d.ws_every = int(self.warm_start_every)
+            dev = self.device
+            self.WSON = torch.ones(1, dtype=torch.int32, device=dev)
+            self.WSST = torch.zeros(3, dtype=torch.int32, device=dev)
+            d.ws_on = N.ptr(self.WSON)
+            if (self.warm_start_judge and d.fuse_first >= 2 and self.substeps >= 2
+                    and self.iterations >= 1 and n_c > 0):
+                m = max(n_c, 1)
+                f4 = lambda: torch.zeros((max(n_b, 1), 4), dtype=torch.float32, device=dev)  # noqa
+                f1 = lambda k: torch.zeros(k, dtype=torch.float32, device=dev)  # noqa
+                self._judge = dict(x=f4(), xa=f4(), xp=f4(), l=f1(m), la=f1(m), y=f1(m),
+                                   rs=torch.zeros(2, dtype=torch.float64, device=dev))
+                j = self._judge
+                d.ws_judge = 1
+                d.ws_jx, d.ws_jx_alt, d.ws_jxp = N.ptr(j["x"]), N.ptr(j["xa"]), N.ptr(j["xp"])
+                d.ws_jlam, d.ws_jlam_alt, d.ws_jy = N.ptr(j["l"]), N.ptr(j["la"]), N.ptr(j["y"])
+                d.ws_rsum, d.ws_stats = N.ptr(j["rs"]), N.ptr(self.WSST)
+                if self.CROWS is not None:
+                    j["acc"], j["dl"] = f4(), f1(self._ccap)
+                    d.ws_jacc, d.ws_jdlam = N.ptr(j["acc"]), N.ptr(j["dl"])
+            else:
+                self._judge = None
         self._block_active = self._use_block()
         if self._block_active:
             d.block = 1
@@ -890,6 +917,16 @@ class Solver:
         self._plan = handle
         self._plan_sig = sig
         return handle
+
+    def warm_start_stats(self) -> dict:
+        """The judge of the learned start since the plan was built: frames
+        run with the learned start, frames run with the zero start, switches
+        between them, and whether the learned start is on for the next frame."""
+        if self.WSST is None or self._plan is None:
+            return {"frames_learned": 0, "frames_zero": 0, "switches": 0, "learned_on": None}
+        st = self.WSST.cpu().numpy()
+        return {"frames_learned": int(st[0]), "frames_zero": int(st[1]), "switches": int(st[2]),
+                "learned_on": bool(int(self.WSON.item()))}
 
     def step(self, frames: int = 1) -> None:
         """Advance ``frames`` frames of ``dt`` (default one, as the reference).

@@ -24,7 +24,6 @@ lambda0.  The substep's Jacobi iterations then continue from lambda0
 * ``Evaluator`` runs the plain zero start and the learned start on their
   own trajectories and measures the RMS constraint residual after each
   substep's iterations;
-* ``Guard``    keeps the plain solver as fallback and yardstick in deployment.
 
 Measured on cloth: the previous-substep lambda feature (slot 7, left at
 zero) feeds every warm-started substep's multipliers into the next
@@ -162,11 +161,11 @@ class _Eager:
                 img = self.image(theta)
                 N.check(self.lib.xpbd_ws_net(st, N.ptr(src), N.ptr(s.XP), N.ptr(s.V),
                                              N.ptr(s.R), N.ptr(s.ANC), N.ptr(img), N.ptr(Y),
-                                             self.h, 0.0, 0.0, 0.0, None, self.m, 0))
+                                             self.h, 0.0, 0.0, 0.0, None, self.m, 0, None, 0))
             N.check(self.lib.xpbd_jacobi_tile_warm(
                 st, N.ptr(src), N.ptr(dst), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R), N.ptr(s.ANC),
                 N.ptr(ld), trs, hs, hr, N.ptr(Y), self.h, 0.0, 0.0, 0.0, None, s._tile, self.n,
-                s.jacobi_relax, 0))
+                s.jacobi_relax, 0, None, 0))
             src, dst, ls, ld = dst, src, ld, ls
         anc = N.ptr(s.ANC) if s._has_attach else None   # the variant without anchor loads
         dyn = s.CROWS is not None
@@ -391,56 +390,3 @@ class Evaluator:
         fallback = {k: int(np.sum(runs[("learned", k)] > runs[("plain", k)]))
                     for k in learned_iters if ("plain", k) in runs}
         return out, fallback
-
-
-class Guard:
-    """The learned start with the plain zero start as fallback and yardstick.
-
-    Every ``check_every`` frames one substep is replayed both ways from the
-    current state (``iters`` Jacobi iterations each, eager launches, state
-    restored afterwards).  The learned start stays on while its RMS residual
-    is no larger than the plain one; otherwise the solver runs the zero start
-    until the next check.  ``history`` keeps (frame, plain, learned, on)."""
-
-    def __init__(self, s, theta, check_every: int = 30, iters: int | None = None):
-        import torch
-
-        _prepare(s)
-        self.s = s
-        self.theta = torch.as_tensor(theta, dtype=torch.float32).to(s.device).contiguous()
-        self.every = max(1, int(check_every))
-        self.iters = s.iterations if iters is None else int(iters)
-        self.frame = 0
-        self.on = False
-        self.history: list = []
-
-    def compare(self) -> tuple[float, float]:
-        """(plain, learned) RMS residual of one substep from the current state."""
-        s = self.s
-        e = _Eager(s)
-        keep = [t.clone() for t in (s.X, s.V, s.XP, s.LAM)]
-
-        def restore():
-            for dst, src in zip((s.X, s.V, s.XP, s.LAM), keep):
-                dst.copy_(src)
-
-        out = []
-        for learned in (False, True):
-            restore()
-            e.integrate()
-            e.iterate(self.iters, theta=self.theta if learned else None)
-            out.append(e.residual())
-        restore()
-        return out[0], out[1]
-
-    def step(self, frames: int = 1) -> None:
-        for _ in range(frames):
-            if self.frame % self.every == 0:
-                rp, rl = self.compare()
-                on = rl <= rp
-                if on != self.on:
-                    self.s.set_warm_start(self.theta if on else None)
-                    self.on = on
-                self.history.append((self.frame, rp, rl, on))
-            self.s.step()
-            self.frame += 1

@@ -379,15 +379,15 @@ def test_judge_keeps_the_trained_start_and_drops_a_poor_one_within_a_frame(train
 
 @pytest.mark.parametrize("graph", [True, False])
 def test_start_switched_off_runs_the_plain_solver_bit_for_bit(trained, graph):
-    """With the judge's flag at 'zero start' the warm-start launches leave
-    the multipliers at zero and the positions as predicted, so the plan
-    gives, bit for bit, the states and multipliers of the plain plan (cloth
-    on the floor, friction active); the judge keeps the start off when the
-    network asks for the largest start everywhere."""
+    """With the flag at 'zero start' (and no judge to turn it back on) the
+    warm-start launches leave the multipliers at zero and the positions as
+    predicted, so the plan gives, bit for bit, the states and multipliers of
+    the plain plan (cloth on the floor, friction active)."""
     a, b = _cloth(), _cloth()
     for s in (a, b):
         s.use_graph = graph
         s.step(30)
+    a.warm_start_judge = False
     a.set_warm_start(_theta(3, b3=12.0))
     a._flush()
     a._get_plan()

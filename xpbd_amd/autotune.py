@@ -25,18 +25,22 @@ _CACHE: dict = {}
 
 
 def _signature(s):
-    return (s.num_bodies, s.num_constraints, s.substeps, s.iterations, s.solve_mode,
-            s._ws_theta is not None, s._gscale is not None, getattr(s, "_device_contacts", False))
+    warm = s._ws_theta is not None
+    return (s.num_bodies, s.num_constraints, s.substeps, s.iterations, s.solve_mode, warm,
+            int(s.warm_start_every) if warm else 0, bool(s.warm_start_judge) if warm else False,
+            s._gscale is not None, getattr(s, "_device_contacts", False))
 
 
-def candidates(n_particles: int, tile_only: bool = False):
+def candidates(n_particles: int, tile_only: bool = False, fused_only: bool = False):
     """Launch shapes to time; ``tile_only`` (learned warm start): no window
-    or block shapes."""
+    or block shapes; ``fused_only`` (the learned start with its judge, which
+    runs in the fused plan): integrate folded in."""
     out = []
     for t in (32, 64, 128, 256, 512, 1024):
         if t <= max(64, 2 * n_particles):
             out.append(("tile", t, None, True))
-            out.append(("tile", t, None, False))
+            if not fused_only:
+                out.append(("tile", t, None, False))
     if tile_only:
         return out
     out.append(("window", 0, (1, None), False))
@@ -81,7 +85,8 @@ def tune(s, frames: int = 3, warmup: int = 1, verbose: bool = False):
     times = {}
     tile_only = (s._ws_theta is not None or s._gscale is not None
                  or getattr(s, "_device_contacts", False))
-    for cand in candidates(s.num_bodies, tile_only=tile_only):
+    judged = s._ws_theta is not None and bool(s.warm_start_judge) and s.substeps >= 2
+    for cand in candidates(s.num_bodies, tile_only=tile_only, fused_only=judged):
         _apply(s, cand)
         s.set_state(x0, v0)
         s.step(warmup)

@@ -207,22 +207,26 @@ __global__ void k_ws_decide(double* sums, int* ws_on, int* stats) {
 
 // Residual: sum over rows of r^2, r = C + alpha~ lambda (two-sided rows),
 // max(0, -C) (one-sided rows).
+// Grid-stride over the rows with at most kResidualBlocks blocks, so that one
+// double atomic per block lands on the sum (one per 256 rows serialised on
+// the single address and dominated the judge's cost on 11 M rows).
+constexpr int kResidualBlocks = 1024;
+
 __global__ __launch_bounds__(kB) void k_ws_residual(const float4* __restrict__ X,
                                                     const int4* __restrict__ R,
                                                     const float4* __restrict__ ANC,
                                                     const float* __restrict__ LAM, float h,
                                                     double* __restrict__ out, int m) {
   __shared__ double sh[kB];
-  const int j = blockIdx.x * kB + threadIdx.x;
   double r2 = 0.0;
-  if (j < m) {
+  for (int j = blockIdx.x * kB + threadIdx.x; j < m; j += gridDim.x * kB) {
     const int4 row = R[j];
     const RowGeo g = row_geo<false>([&](int i) { return X[i]; },
                                     [&](int i) { return mk3(0.0f, 0.0f, 0.0f); }, ANC, row, j, h);
     float r;
     if (g.t == T_FLOOR || g.t == T_CONTACT) r = fmaxf(0.0f, -g.C);
     else r = g.C + g.alpha * LAM[j];
-    r2 = (double)r * (double)r;
+    r2 += (double)r * (double)r;
   }
   sh[threadIdx.x] = r2;
   __syncthreads();
@@ -279,7 +283,8 @@ int xpbd_ws_decide(void* stream, double* sums, int* ws_on, int* stats) {
 int xpbd_ws_residual(void* stream, const float* X, const int* R, const float* ANC,
                      const float* LAM, float h, double* out_sum, int n_rows) {
   if (n_rows <= 0) return 0;
-  hipLaunchKernelGGL(k_ws_residual, dim3(nblk(n_rows)), dim3(kB), 0, (hipStream_t)stream,
+  const int nb = nblk(n_rows) < kResidualBlocks ? nblk(n_rows) : kResidualBlocks;
+  hipLaunchKernelGGL(k_ws_residual, dim3(nb), dim3(kB), 0, (hipStream_t)stream,
                      (const float4*)X, (const int4*)R, (const float4*)ANC, LAM, h, out_sum,
                      n_rows);
   XPBD_LAUNCH_CHECK();

@@ -268,14 +268,16 @@ typedef struct xpbd_particles_desc {
   int ws_every;
   /* The plain zero start as judge and fallback of the learned start
    * (tile mode with ws_image, fuse_first 2, substeps >= 2): ws_on (device
-   * int, 1 = learned start) is read by every warm-start launch of a frame.
-   * With ws_judge != 0, the frame's first substep also runs with the other
-   * start on the judge buffers (ws_jx, ws_jx_alt, ws_jxp float4[n]; ws_jlam,
-   * ws_jlam_alt, ws_jy float[n_rows]; ws_jacc float4[n] and ws_jdlam
-   * float[dyn_cap] with device rows) on a second stream, both substeps'
-   * squared residuals go to ws_rsum (double[2], zero at the start), and
-   * xpbd_ws_decide sets ws_on for the next frame (stats in ws_stats,
-   * int[3], optional). */
+   * int, 1 = learned start) is read by every warm-start launch.  With
+   * ws_judge = k >= 1, the first substep of every k-th frame (counted by
+   * the plan from its first run) is first run with the other start from the
+   * same state into the judge buffers (ws_jx, ws_jx_alt, ws_jxp float4[n];
+   * ws_jlam, ws_jlam_alt, ws_jy float[n_rows]; ws_jacc float4[n] and
+   * ws_jdlam float[dyn_cap] with device rows), then with the solver's own;
+   * both squared residuals go to ws_rsum (double[2], zero at the start) and
+   * xpbd_ws_decide sets ws_on for the rest of the frame and the following
+   * ones (stats in ws_stats, int[3], optional).  Judged and other frames
+   * are two captured graphs. */
   int* ws_on;
   int ws_judge;
   float* ws_jx;
@@ -518,8 +520,8 @@ int xpbd_ws_residual(void* stream, const float* X, const int* R, const float* AN
  * thread): sums[0] is the squared residual of the substep the solver ran,
  * sums[1] that of the same substep run with the other start; *ws_on becomes
  * 1 (learned start) when the learned start's sum is no larger, else 0 (zero
- * start); stats (optional, int[3]) counts frames run with the learned start,
- * frames run with the zero start, and switches; the sums are cleared. */
+ * start); stats (optional, int[3]) counts the decisions for the learned
+ * start, those for the zero start, and the switches; the sums are cleared. */
 int xpbd_ws_decide(void* stream, double* sums, int* ws_on, int* stats);
 
 /* ---- host coloring (src/xpbd3d/coloring.py) ----------------------------- */

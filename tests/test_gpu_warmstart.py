@@ -332,8 +332,8 @@ def test_judge_keeps_the_trained_start_and_drops_a_poor_one_within_a_frame(train
     the compliant rows from 4 times their first step is worse than the zero
     start on some frames: every frame, the start the plan runs next is the
     better one on the frame's first substep, as the eager Evaluator measures
-    it from the same state (so a poor start is off from the next frame on),
-    and it is switched off at least once."""
+    it from the same state (a poor start is off from the frame's second
+    substep on), and it is switched off at least once."""
     from xpbd_amd import warmstart as ws
 
     theta, _ = trained
@@ -343,8 +343,8 @@ def test_judge_keeps_the_trained_start_and_drops_a_poor_one_within_a_frame(train
     s.step(12)
     st = s.warm_start_stats()
     print("trained start:", st)
-    assert st["frames_learned"] + st["frames_zero"] == 12
-    assert st["frames_learned"] >= 10 and st["learned_on"]
+    assert st["judged_learned"] + st["judged_zero"] == 12
+    assert st["judged_learned"] >= 10 and st["learned_on"]
     # y = -12 + 3 GELU(GELU(10 f6)), f6 = alpha~ / (w + alpha~): about -12
     # (lambda0 = 0) on rigid rows, +17 (lambda0 = 4 x the first step) on
     # the compliant ones
@@ -373,7 +373,7 @@ def test_judge_keeps_the_trained_start_and_drops_a_poor_one_within_a_frame(train
     st = c.warm_start_stats()
     print("poor start:", st)
     assert offs >= 1 and st["switches"] >= 1
-    assert st["frames_learned"] + st["frames_zero"] == 16
+    assert st["judged_learned"] + st["judged_zero"] == 16
     assert np.all(np.isfinite(c.positions()))
 
 

@@ -27,7 +27,7 @@ _CACHE: dict = {}
 def _signature(s):
     warm = s._ws_theta is not None
     return (s.num_bodies, s.num_constraints, s.substeps, s.iterations, s.solve_mode, warm,
-            int(s.warm_start_every) if warm else 0, bool(s.warm_start_judge) if warm else False,
+            int(s.warm_start_every) if warm else 0, int(s.warm_start_judge) if warm else 0,
             s._gscale is not None, getattr(s, "_device_contacts", False))
 
 
@@ -85,7 +85,7 @@ def tune(s, frames: int = 3, warmup: int = 1, verbose: bool = False):
     times = {}
     tile_only = (s._ws_theta is not None or s._gscale is not None
                  or getattr(s, "_device_contacts", False))
-    judged = s._ws_theta is not None and bool(s.warm_start_judge) and s.substeps >= 2
+    judged = s._ws_theta is not None and int(s.warm_start_judge) > 0 and s.substeps >= 2
     for cand in candidates(s.num_bodies, tile_only=tile_only, fused_only=judged):
         _apply(s, cand)
         s.set_state(x0, v0)

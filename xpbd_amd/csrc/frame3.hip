@@ -26,12 +26,11 @@ enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8 };
 struct Frame3 {
   xpbd_particles_desc d;
   std::vector<int> color_starts;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // captured frames: [0] plain, [1] with the judge of the learned start
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   hipStream_t capture_stream = nullptr;
-  // judge of the learned start: its stream and the fork / read / join events
-  hipStream_t judge_stream = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_read = nullptr, ev_join = nullptr;
+  long long frame_no = 0;  // frames run since the plan was created
 };
 
 // Fused-tile Jacobi frame.
@@ -74,13 +73,10 @@ struct TileRun {
   int li = 0;    // lam[li] holds the multipliers after the last launch
 };
 
-// One substep of the fused tile plan (see issue_frame_tile) on stream s.
-// ws_on / invert select the start of a warm-started substep; `after_read`
-// (optional) is recorded once no launch of the run reads src0 any more,
-// `before_write` (optional) is waited for before the first launch that
-// writes x[0] of a run without src0.
+// One substep of the fused tile plan (see issue_frame_tile) on stream s;
+// invert != 0 runs the start the solver is not using (the judge).
 int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r, int sub,
-                        int invert, hipEvent_t after_read, hipEvent_t before_write) {
+                        int invert) {
   const int n = d.n_particles;
   const bool have_rows = (d.n_rows > 0 || d.dyn_cap > 0) && d.iterations > 0;
   const int it_n = have_rows ? d.iterations : 0;
@@ -95,21 +91,6 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
   float* const* bufs = r.x;
   float* const* lams = r.lam;
   auto in = [&]() -> const float* { return r.src0 ? r.src0 : bufs[r.loc]; };
-  bool waited = before_write == nullptr;
-  auto before_launch = [&]() -> int {  // the launch about to run writes bufs[r.loc ^ 1]
-    if (!waited && (r.loc ^ 1) == 0) {
-      XPBD_CHECK(hipStreamWaitEvent(s, before_write, 0));
-      waited = true;
-    }
-    return 0;
-  };
-  auto read_done = [&]() -> int {
-    if (r.src0) {
-      r.src0 = nullptr;
-      if (after_read) XPBD_CHECK(hipEventRecord(after_read, s));
-    }
-    return 0;
-  };
   r.li = 0;
   int it = 0;
   if (warm) {
@@ -118,13 +99,12 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
                        d.gz, d.gscale, d.n_rows, predicted ? 0 : 1, d.ws_on, invert);
       if (rc) return rc;
     }
-    if ((rc = before_launch())) return rc;
     rc = xpbd_jacobi_tile_warm(s, in(), bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors, lams[1],
                                d.tile_row_start, d.halo_start, d.halo_rows, r.y, d.h, d.gx, d.gy,
                                d.gz, d.gscale, d.tile_size, n, d.relax, predicted ? 0 : 1,
                                d.ws_on, invert);
     if (rc) return rc;
-    if ((rc = read_done())) return rc;
+    r.src0 = nullptr;
     r.loc ^= 1;
     r.li = 1;
   } else if (!predicted && it_n >= 2) {
@@ -133,12 +113,11 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
                                        d.gz, d.gscale, d.dyn_count, d.dyn_cap);
       if (rc) return rc;
     }
-    if ((rc = before_launch())) return rc;
     rc = xpbd_jacobi_tile_first(s, in(), bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors, lams[1],
                                 d.tile_row_start, d.halo_start, d.halo_rows, d.h, d.gx, d.gy,
                                 d.gz, d.gscale, d.tile_size, n, d.relax, r.acc);
     if (rc) return rc;
-    if ((rc = read_done())) return rc;
+    r.src0 = nullptr;
     r.loc ^= 1;
     r.li = 1;
     it = 1;
@@ -154,7 +133,6 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
                                  it == 0 ? 1 : 0, d.dyn_count, d.dyn_cap);
       if (rc) return rc;
     }
-    if ((rc = before_launch())) return rc;
     if (it + 1 == it_n && integrate_next) {
       rc = xpbd_jacobi_tile_next(s, bufs[r.loc], bufs[r.loc ^ 1], r.xp, d.rows, d.anchors,
                                  lams[r.li], lams[r.li ^ 1], d.tile_row_start, d.halo_start,
@@ -198,15 +176,17 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
 // first = 0.  Multipliers ping-pong as the positions do: a substep's first
 // launch writes lam_alt.
 //
-// Judge (ws_judge, fuse_first 2, >= 2 substeps): the frame's first substep
-// also runs on a second stream with the other start (the zero start while
-// the learned one is on, and the reverse), from the same state into the
-// judge buffers; both substeps' squared residuals (positions after the
-// iterations: x_prev, which the substep's last launch sets to them) are
-// summed, and after the frame xpbd_ws_decide keeps the learned start for the
-// next frame only if its residual was no larger.  The solver's own launches
-// that overwrite X wait until the judge has read it.
-int issue_frame_tile(Frame3* f, hipStream_t s) {
+// Judged frames (ws_judge, fuse_first 2, >= 2 substeps): the frame's first
+// substep is first run with the other start (the zero start while the
+// learned one is on, and the reverse) from the same state into the judge
+// buffers, then as usual; both substeps' squared residuals (positions after
+// the iterations: x_prev, which the substep's last launch sets to them) are
+// summed, and xpbd_ws_decide keeps the learned start for the rest of the
+// frame and the following ones only if its residual was no larger.  The
+// judge runs on the same stream: on small scenes a second stream's
+// synchronisation cost more than the judge's launches, on large ones the
+// launches fill the GPU anyway.
+int issue_frame_tile(Frame3* f, hipStream_t s, bool judge) {
   const xpbd_particles_desc& d = f->d;
   const int n = d.n_particles;
   if (n <= 0) return 0;
@@ -221,32 +201,25 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
   if (d.fuse_first && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1) || next)) {
     TileRun r{{d.x, d.x_alt}, {d.lam, d.lam_alt}, d.x_prev, d.ws_y, dyn ? d.acc : nullptr,
               d.dyn_lam, nullptr};
-    const bool judge = warm && f->judge_stream != nullptr;
+    judge = judge && warm && d.ws_judge > 0;
     for (int sub = 0; sub < d.substeps; ++sub) {
       if (judge && sub == 0) {
-        hipStream_t js = f->judge_stream;
-        XPBD_CHECK(hipEventRecord(f->ev_fork, s));
-        XPBD_CHECK(hipStreamWaitEvent(js, f->ev_fork, 0));
         TileRun jr{{d.ws_jx, d.ws_jx_alt}, {d.ws_jlam, d.ws_jlam_alt}, d.ws_jxp, d.ws_jy,
                    dyn ? d.ws_jacc : nullptr, d.ws_jdlam, d.x};
-        rc = issue_fused_substep(d, js, jr, 0, 1, f->ev_read, nullptr);
+        rc = issue_fused_substep(d, s, jr, 0, 1);
         if (rc) return rc;
-        rc = xpbd_ws_residual(js, jr.xp, d.rows, d.anchors, jr.lam[jr.li], d.h, d.ws_rsum + 1,
+        rc = xpbd_ws_residual(s, jr.xp, d.rows, d.anchors, jr.lam[jr.li], d.h, d.ws_rsum + 1,
                               d.n_rows);
         if (rc) return rc;
-        XPBD_CHECK(hipEventRecord(f->ev_join, js));
-        rc = issue_fused_substep(d, s, r, 0, 0, nullptr, f->ev_read);
+        rc = issue_fused_substep(d, s, r, 0, 0);
         if (rc) return rc;
         rc = xpbd_ws_residual(s, r.xp, d.rows, d.anchors, r.lam[r.li], d.h, d.ws_rsum, d.n_rows);
         if (rc) return rc;
+        rc = xpbd_ws_decide(s, d.ws_rsum, d.ws_on, d.ws_stats);
+        if (rc) return rc;
         continue;
       }
-      rc = issue_fused_substep(d, s, r, sub, 0, nullptr, nullptr);
-      if (rc) return rc;
-    }
-    if (judge) {
-      XPBD_CHECK(hipStreamWaitEvent(s, f->ev_join, 0));
-      rc = xpbd_ws_decide(s, d.ws_rsum, d.ws_on, d.ws_stats);
+      rc = issue_fused_substep(d, s, r, sub, 0);
       if (rc) return rc;
     }
     if (r.loc) {
@@ -302,9 +275,9 @@ int issue_frame_tile(Frame3* f, hipStream_t s) {
   return 0;
 }
 
-int issue_frame(Frame3* f, hipStream_t s) {
+int issue_frame(Frame3* f, hipStream_t s, bool judge) {
   const xpbd_particles_desc& d = f->d;
-  if (d.mode == XPBD_MODE_JACOBI_TILE) return issue_frame_tile(f, s);
+  if (d.mode == XPBD_MODE_JACOBI_TILE) return issue_frame_tile(f, s, judge);
   const int n = d.n_particles;
   if (n <= 0) return 0;
   const bool have_rows = d.n_rows > 0 && d.iterations > 0;
@@ -347,10 +320,39 @@ int issue_frame(Frame3* f, hipStream_t s) {
 }
 
 void drop_graph(Frame3* f) {
-  if (f->exec) (void)hipGraphExecDestroy(f->exec);
-  if (f->graph) (void)hipGraphDestroy(f->graph);
-  f->exec = nullptr;
-  f->graph = nullptr;
+  for (int k = 0; k < 2; ++k) {
+    if (f->exec[k]) (void)hipGraphExecDestroy(f->exec[k]);
+    if (f->graph[k]) (void)hipGraphDestroy(f->graph[k]);
+    f->exec[k] = nullptr;
+    f->graph[k] = nullptr;
+  }
+}
+
+// Whether the plan's next frame is a judged one (every ws_judge-th frame).
+bool judged_frame(const Frame3* f) {
+  return f->d.ws_judge > 0 && f->frame_no % f->d.ws_judge == 0;
+}
+
+int capture(Frame3* f, int k) {
+  if (!f->capture_stream) XPBD_CHECK(hipStreamCreateWithFlags(&f->capture_stream, hipStreamNonBlocking));
+  XPBD_CHECK(hipStreamBeginCapture(f->capture_stream, hipStreamCaptureModeThreadLocal));
+  int rc = issue_frame(f, f->capture_stream, k == 1);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(f->capture_stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  XPBD_CHECK(e);
+  f->graph[k] = g;
+  e = hipGraphInstantiate(&f->exec[k], g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    f->graph[k] = nullptr;
+    f->exec[k] = nullptr;
+    return (int)e;
+  }
+  return 0;
 }
 
 }  // namespace
@@ -382,7 +384,8 @@ int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
        desc->n_particles > xpbd_frame_block_capacity(desc->mode == XPBD_MODE_GS) ||
        (desc->mode == XPBD_MODE_GS && desc->n_rows > 0 && !desc->color_starts_dev)))
     return (int)hipErrorInvalidValue;
-  const bool judge = desc->ws_judge != 0;
+  if (desc->ws_judge < 0) return (int)hipErrorInvalidValue;
+  const bool judge = desc->ws_judge > 0;
   if (judge &&
       (!desc->ws_image || desc->mode != XPBD_MODE_JACOBI_TILE || desc->block ||
        desc->fuse_first < 2 || desc->substeps < 2 || desc->iterations < 1 || desc->n_rows <= 0 ||
@@ -396,16 +399,6 @@ int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
   if (desc->mode == XPBD_MODE_GS && desc->n_rows > 0)
     f->color_starts.assign(desc->color_starts, desc->color_starts + desc->num_colors + 1);
   f->d.color_starts = nullptr;
-  if (judge) {
-    hipError_t e = hipStreamCreateWithFlags(&f->judge_stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ev_fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ev_read, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ev_join, hipEventDisableTiming);
-    if (e != hipSuccess) {
-      xpbd_frame3_destroy(f);
-      return (int)e;
-    }
-  }
   *handle = f;
   return 0;
 }
@@ -421,32 +414,20 @@ int xpbd_frame3_run(void* handle, void* stream, int frames, int use_graph) {
                             d.gx, d.gy, d.gz, d.damping, d.relax, d.max_speed,
                             d.mode == XPBD_MODE_GS ? 1 : 0, frames);
   }
-  if (!use_graph) {
-    for (int k = 0; k < frames; ++k) {
-      int rc = issue_frame(f, s);
+  for (int k = 0; k < frames; ++k) {
+    const int j = judged_frame(f) ? 1 : 0;
+    if (!use_graph) {
+      int rc = issue_frame(f, s, j == 1);
       if (rc) return rc;
+    } else {
+      if (!f->exec[j]) {
+        int rc = capture(f, j);
+        if (rc) return rc;
+      }
+      XPBD_CHECK(hipGraphLaunch(f->exec[j], s));
     }
-    return 0;
+    ++f->frame_no;
   }
-  if (!f->exec) {
-    if (!f->capture_stream) XPBD_CHECK(hipStreamCreateWithFlags(&f->capture_stream, hipStreamNonBlocking));
-    XPBD_CHECK(hipStreamBeginCapture(f->capture_stream, hipStreamCaptureModeThreadLocal));
-    int rc = issue_frame(f, f->capture_stream);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(f->capture_stream, &g);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    XPBD_CHECK(e);
-    f->graph = g;
-    e = hipGraphInstantiate(&f->exec, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      drop_graph(f);
-      return (int)e;
-    }
-  }
-  for (int k = 0; k < frames; ++k) XPBD_CHECK(hipGraphLaunch(f->exec, s));
   return 0;
 }
 
@@ -455,9 +436,6 @@ int xpbd_frame3_destroy(void* handle) {
   if (!f) return 0;
   drop_graph(f);
   if (f->capture_stream) (void)hipStreamDestroy(f->capture_stream);
-  if (f->judge_stream) (void)hipStreamDestroy(f->judge_stream);
-  for (hipEvent_t e : {f->ev_fork, f->ev_read, f->ev_join})
-    if (e) (void)hipEventDestroy(e);
   delete f;
   return 0;
 }

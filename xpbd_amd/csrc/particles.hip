@@ -613,6 +613,16 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_warm(
     const f3 x = xyz(x4) + v * h;
     return make_float4(x.x, x.y, x.z, x4.w);
   };
+  if (!active) {  // the zero start: positions as predicted, multipliers 0
+    for (int i = p0 + threadIdx.x; i < p1; i += BLOCK) {
+      const float4 x0 = Xin[i];
+      Xout[i] = predict(x0, i);
+      if (PREDICT) XP[i] = make_float4(x0.x, x0.y, x0.z, 0.0f);
+    }
+    for (int j = tile_row_start[tile] + threadIdx.x; j < tile_row_start[tile + 1]; j += BLOCK)
+      LAMout[j] = 0.0f;
+    return;
+  }
   for (int k = threadIdx.x; k < 3 * P; k += BLOCK) lds64[k] = 0ull;
   for (int k = threadIdx.x; k < P; k += BLOCK) cnt[k] = 0u;
   for (int i = p0 + threadIdx.x; i < p1; i += BLOCK) {

@@ -185,19 +185,18 @@ void launch_net(hipStream_t s, const float* Xin, const float* XP, const float* V
                        (const float4*)ANC, img, Y, h, gx, gy, gz, gscale, n_rows, ws_on, invert);
 }
 
-// The judge's decision after a frame (one thread): sums[0] = squared
-// residual of the solver's own substep, sums[1] = that of the same substep
-// run with the other start; the learned start stays (or becomes) active for
-// the next frame when its residual is no larger than the zero start's.
-// stats: frames run with the learned start, frames run with the zero start,
-// switches.  The sums are cleared for the next frame.
+// The judge's decision (one thread): sums[0] = squared residual of the
+// substep the solver ran, sums[1] = that of the same substep run with the
+// other start; the learned start stays (or becomes) active when its residual
+// is no larger than the zero start's.  stats: decisions for the learned
+// start, decisions for the zero start, switches.  The sums are cleared.
 __global__ void k_ws_decide(double* sums, int* ws_on, int* stats) {
   const int active = *ws_on != 0;
   const double learned = active ? sums[0] : sums[1];
   const double zero = active ? sums[1] : sums[0];
   const int next = learned <= zero ? 1 : 0;
   if (stats) {
-    stats[active ? 0 : 1] += 1;
+    stats[next ? 0 : 1] += 1;
     stats[2] += next != active;
   }
   *ws_on = next;

@@ -279,8 +279,9 @@ class Solver:
         self._ws_theta = None         # learned lambda warm start parameters
         self._ws_gen = 0
         self.warm_start_every = 1     # network pass every k-th substep of a frame
-        # the zero start judges the learned one every frame (in the frame plan)
-        self.warm_start_judge = True
+        # the zero start judges the learned one in the frame plan every k-th
+        # frame (True / 1: every frame; False / 0: never)
+        self.warm_start_judge = 1
         self.WSON = self.WSST = None
 
         # host scene, one column per field (float64 as the reference's
@@ -462,10 +463,11 @@ class Solver:
         ``xpbd_jacobi_tile_warm``) before its Jacobi iterations.  ``None``
         restores the zero start.
         Runs the Jacobi tile kernel.  The plain zero start stays the judge
-        and fallback (``warm_start_judge``, on by default): every frame's
-        first substep also runs with the other start on a second stream,
-        and the learned start stays on for the next frame only while its
-        residual is no larger (``warm_start_stats``)."""
+        and fallback (``warm_start_judge`` = k, every frame by default): the
+        first substep of every k-th frame is also run with the other start
+        from the same state, and the learned start stays on for the rest of
+        the frame and the following ones only while its residual is no
+        larger (``warm_start_stats``)."""
         import torch
 
         if theta is None:
@@ -807,7 +809,7 @@ class Solver:
         return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
                 self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling,
                 self._tile, getattr(self, "_ccap", 0), str(self.fuse_first), self._ws_gen,
-                str(self.block_frames), int(self.warm_start_every), bool(self.warm_start_judge))
+                str(self.block_frames), int(self.warm_start_every), int(self.warm_start_judge))
 
     # "auto" limit of whole-frame workgroups in Jacobi mode: above it the
     # tile path's 256 CUs beat one CU (cloths, ms per frame, workgroup
@@ -889,7 +891,7 @@ class Solver:
                 self._judge = dict(x=f4(), xa=f4(), xp=f4(), l=f1(m), la=f1(m), y=f1(m),
                                    rs=torch.zeros(2, dtype=torch.float64, device=dev))
                 j = self._judge
-                d.ws_judge = 1
+                d.ws_judge = int(self.warm_start_judge)
                 d.ws_jx, d.ws_jx_alt, d.ws_jxp = N.ptr(j["x"]), N.ptr(j["xa"]), N.ptr(j["xp"])
                 d.ws_jlam, d.ws_jlam_alt, d.ws_jy = N.ptr(j["l"]), N.ptr(j["la"]), N.ptr(j["y"])
                 d.ws_rsum, d.ws_stats = N.ptr(j["rs"]), N.ptr(self.WSST)
@@ -919,13 +921,14 @@ class Solver:
         return handle
 
     def warm_start_stats(self) -> dict:
-        """The judge of the learned start since the plan was built: frames
-        run with the learned start, frames run with the zero start, switches
-        between them, and whether the learned start is on for the next frame."""
+        """The judge of the learned start since the plan was built: judged
+        frames that kept (or chose) the learned start, those that chose the
+        zero start, switches between them, and whether the learned start is
+        on now."""
         if self.WSST is None or self._plan is None:
-            return {"frames_learned": 0, "frames_zero": 0, "switches": 0, "learned_on": None}
+            return {"judged_learned": 0, "judged_zero": 0, "switches": 0, "learned_on": None}
         st = self.WSST.cpu().numpy()
-        return {"frames_learned": int(st[0]), "frames_zero": int(st[1]), "switches": int(st[2]),
+        return {"judged_learned": int(st[0]), "judged_zero": int(st[1]), "switches": int(st[2]),
                 "learned_on": bool(int(self.WSON.item()))}
 
     def step(self, frames: int = 1) -> None:

@@ -524,6 +524,33 @@ int xpbd_ws_residual(void* stream, const float* X, const int* R, const float* AN
  * start, those for the zero start, and the switches; the sums are cleared. */
 int xpbd_ws_decide(void* stream, double* sums, int* ws_on, int* stats);
 
+/* ---- device coloring (src/xpbd3d/coloring.py:26-89) ----------------------
+ * The reference's greedy colouring (stable order on -(deg a + deg b), each
+ * row the smallest colour its earlier neighbours left free) computed on the
+ * device as Jones-Plassmann rounds with that order as the priority (the same
+ * colours), in one workgroup: rows R[0 .. n_rows_static + min(*dyn_count,
+ * dyn_cap)) (int4, API order: static rows, then the device CONTACT rows;
+ * dyn_count NULL = none) over n_bodies <= xpbd_color_rows_max_bodies()
+ * bodies.  Outputs: color[row]; order = the rows grouped by colour (any
+ * order inside a colour), color_starts[c] its offsets (<= 257 ints),
+ * *num_colors (-1 when more than 256 colours were needed).  inc: scratch of
+ * 2 (n_rows_static + dyn_cap) ints.  Replaces the host colouring of
+ * solver.py:394-401 for frames with device CONTACT rows. */
+int xpbd_color_rows_max_bodies(void);
+int xpbd_color_rows_device(void* stream, const int* R, int n_rows_static, const int* dyn_count,
+                           int dyn_cap, int n_bodies, int* inc, int* color, int* order,
+                           int* color_starts, int* num_colors);
+/* Whole Gauss-Seidel frames in one workgroup (as xpbd_frame_block with
+ * gs = 1) over rows coloured on the device: R, ANC and LAM in API order,
+ * colour c's rows at order[color_starts[c] .. color_starts[c + 1]), the
+ * colour count read from *num_colors_dev (a negative count projects no
+ * row).  solver.py:412-467 for self-collision frames without the host. */
+int xpbd_frame_block_ordered(void* stream, float* X, float* V, float* XP, const int* R,
+                             const float* ANC, float* LAM, const int* color_starts,
+                             const int* order, const int* num_colors_dev, int n_particles,
+                             int substeps, int iterations, float h, float gx, float gy, float gz,
+                             float damping, float relax, float max_speed, int frames);
+
 /* ---- host coloring (src/xpbd3d/coloring.py) ----------------------------- */
 
 /* coloring.py:26-71 `color_constraints`; returns the number of colors. */

@@ -103,6 +103,9 @@ _SIGS = {
     "xpbd_frame3_run": (c_int, [c_void_p, c_void_p, c_int, c_int]),
     "xpbd_frame3_destroy": (c_int, [c_void_p]),
     "xpbd_color_constraints": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "xpbd_color_rows_max_bodies": (c_int, []),
+    "xpbd_color_rows_device": (c_int, [_P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
+    "xpbd_frame_block_ordered": (c_int, [_P] * 10 + [_I, _I, _I] + [_F] * 7 + [_I]),
     "xpbd_color_ranges": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "xpbd6_integrate": (_I, [_P] * 10 + [_F, _F, _F, _F, _I, _P, _I]),
     "xpbd6_box_manifold": (_I, [_P] * 6 + [_F, _I] + [_P] * 7),

@@ -852,7 +852,8 @@ __global__ __launch_bounds__(BLOCK) void k_frame_block(
     const int4* __restrict__ R, const float4* __restrict__ ANC, float* __restrict__ LAM,
     const int* __restrict__ color_starts, int num_colors, int n, int m, int substeps,
     int iterations, float h, float gx, float gy, float gz, float damping, float relax,
-    float max_speed, int do_cap, int frames) {
+    float max_speed, int do_cap, int frames, const int* __restrict__ order,
+    const int* __restrict__ num_colors_dev) {
   extern __shared__ unsigned long long lds64[];
   unsigned long long* sx = lds64;
   unsigned long long* sy = lds64 + n;
@@ -862,6 +863,10 @@ __global__ __launch_bounds__(BLOCK) void k_frame_block(
                     : (float4*)(lds64 + tile_lds_pos_offset(n) / sizeof(unsigned long long));
   const int tid = threadIdx.x;
   const float inv_dt2 = 1.0f / (h * h);
+  // GS colours: a count on the device (rows coloured on the device; < 0 =
+  // colouring failed, no row is projected) and the rows of colour c at
+  // order[color_starts[c] ..] (NULL: the rows themselves, sorted by colour)
+  const int ncol = num_colors_dev ? *num_colors_dev : num_colors;
   for (int i = tid; i < n; i += BLOCK) spos[i] = X[i];
   __syncthreads();
   auto pos = [&](int i) { return spos[i]; };
@@ -884,9 +889,10 @@ __global__ __launch_bounds__(BLOCK) void k_frame_block(
       for (int it = 0; it < iterations && m > 0; ++it) {
         const int first = it == 0;
         if (GS) {
-          for (int c = 0; c < num_colors; ++c) {
+          for (int c = 0; c < ncol; ++c) {
             const int c0 = color_starts[c], c1 = color_starts[c + 1];
-            for (int j = c0 + tid; j < c1; j += BLOCK) gs_row(spos, XP, R, ANC, LAM, inv_dt2, first, j);
+            for (int k = c0 + tid; k < c1; k += BLOCK)
+              gs_row(spos, XP, R, ANC, LAM, inv_dt2, first, order ? order[k] : k);
             __syncthreads();
           }
         } else {
@@ -1290,13 +1296,33 @@ int xpbd_frame_block(void* stream, float* X, float* V, float* XP, const int* R, 
                        sizeof(float4) * (size_t)n_particles, s, (float4*)X, (float4*)V,
                        (float4*)XP, (const int4*)R, (const float4*)ANC, LAM, color_starts,
                        num_colors, n_particles, n_rows, substeps, iterations, h, gx, gy, gz,
-                       damping, relax, max_speed, do_cap, frames);
+                       damping, relax, max_speed, do_cap, frames, nullptr, nullptr);
   else
     hipLaunchKernelGGL((k_frame_block<false, kWG>), dim3(1), dim3(kWG),
                        tile_lds_bytes(n_particles), s, (float4*)X, (float4*)V, (float4*)XP,
                        (const int4*)R, (const float4*)ANC, LAM, color_starts, num_colors,
                        n_particles, n_rows, substeps, iterations, h, gx, gy, gz, damping, relax,
-                       max_speed, do_cap, frames);
+                       max_speed, do_cap, frames, nullptr, nullptr);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_frame_block_ordered(void* stream, float* X, float* V, float* XP, const int* R,
+                             const float* ANC, float* LAM, const int* color_starts,
+                             const int* order, const int* num_colors_dev, int n_particles,
+                             int substeps, int iterations, float h, float gx, float gy, float gz,
+                             float damping, float relax, float max_speed, int frames) {
+  if (n_particles <= 0 || frames <= 0) return 0;
+  if (n_particles > xpbd_frame_block_capacity(1) || substeps < 0 || iterations < 0 ||
+      !color_starts || !order || !num_colors_dev)
+    return (int)hipErrorInvalidValue;
+  constexpr int kWG = 1024;
+  const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
+  hipLaunchKernelGGL((k_frame_block<true, kWG>), dim3(1), dim3(kWG),
+                     sizeof(float4) * (size_t)n_particles, (hipStream_t)stream, (float4*)X,
+                     (float4*)V, (float4*)XP, (const int4*)R, (const float4*)ANC, LAM,
+                     color_starts, 0, n_particles, 1, substeps, iterations, h, gx, gy, gz,
+                     damping, relax, max_speed, do_cap, frames, order, num_colors_dev);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

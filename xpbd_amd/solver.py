@@ -489,10 +489,14 @@ class Solver:
         extension; the reference rebuilds them on the host, solver.py:275-342).
         Same margin, cell and filter as the host path, same rows in the same
         order; pairs already joined by a two-body constraint are skipped, as
-        the host path skips them (solver.py:285-290, 336-337).  Jacobi tile
-        mode only."""
-        if enabled and (self.solve_mode != "jacobi" or self.jacobi_kernel != "tile"):
-            raise ValueError("device contacts need solve_mode='jacobi' with the tile kernel")
+        the host path skips them (solver.py:285-290, 336-337).  Jacobi: the
+        tile kernel projects them each iteration.  Gauss-Seidel: every frame
+        the rows (static and contact) are recoloured on the device with the
+        reference's greedy colouring (``xpbd_color_rows_device``) and the
+        whole frame runs in one workgroup (``xpbd_frame_block_ordered``),
+        for scenes of up to 4096 particles."""
+        if enabled and self.solve_mode == "jacobi" and self.jacobi_kernel != "tile":
+            raise ValueError("device contacts in Jacobi mode need the tile kernel")
         self._device_contacts = bool(enabled)
         self._contact_margin = float(margin)
         self._dirty = True
@@ -666,7 +670,18 @@ class Solver:
         V[:, :3] = v_np
         rows, anc = pack_rows(h["type"], ra, rb, h["rest"], h["compliance"],
                               h["anchor"], h["mu"])
-        if self.solve_mode == "gs" and n_c > 0:
+        gs_dev = self.solve_mode == "gs" and self._device_contacts
+        if gs_dev:
+            cap = min(int(N.lib().xpbd_frame_block_capacity(1)),
+                      int(N.lib().xpbd_color_rows_max_bodies()))
+            if n_b > cap:
+                raise ValueError(f"Gauss-Seidel with device contacts runs scenes of up to {cap} "
+                                 f"particles in one workgroup ({n_b} given)")
+            perm = np.arange(n_c, dtype=np.int64)      # rows stay in API order
+            self._color_starts = np.array([0, n_c], dtype=np.int32)
+            self.num_colors = 0
+            self.color_counts = {}
+        elif self.solve_mode == "gs" and n_c > 0:
             color = color_constraints(n_b, h["a"], h["b"])
             order, starts, num_colors = color_ranges(color)
             perm = order.astype(np.int64)
@@ -751,7 +766,7 @@ class Solver:
             self._alloc_contacts(max(1024, 4 * int(col.sum())))
             if self.ACC is None or self.ACC.shape[0] < n_b:
                 self.ACC = torch.zeros((max(n_b, 1), 4), dtype=torch.float32, device=dev)
-            if self._tile_dev is None:      # no static rows: tables for an empty row set
+            if self._tile_dev is None and self.solve_mode != "gs":   # no static rows
                 self._tile = self.tile_size or default_tile_size(n_b)
                 self._tile_tabs = tile_tables(np.zeros((0, 4), np.int32), n_b, self._tile)
                 ts, hs, hr = self._tile_tabs
@@ -768,8 +783,29 @@ class Solver:
         import torch
 
         self._ccap = int(cap)
-        self.CROWS = torch.zeros((self._ccap, 4), dtype=torch.int32, device=self.device)
-        self.CLAM = torch.zeros(self._ccap, dtype=torch.float32, device=self.device)
+        dev = self.device
+        if self.solve_mode == "gs":
+            # Gauss-Seidel: the contact rows follow the static rows in one
+            # API-ordered row set (recoloured with them every frame)
+            m = len(self._rt)
+            tot = m + self._ccap
+            R = torch.zeros((tot, 4), dtype=torch.int32, device=dev)
+            ANC = torch.zeros((tot, 4), dtype=torch.float32, device=dev)
+            if m:
+                R[:m] = self.R[:m]
+                ANC[:m] = self.ANC[:m]
+            self.R, self.ANC = R, ANC
+            self.LAM = torch.zeros(max(tot, 1), dtype=torch.float32, device=dev)
+            self.CROWS = self.R[m:]
+            self.CLAM = self.LAM[m:tot]
+            self.CINC = torch.empty(max(2 * tot, 1), dtype=torch.int32, device=dev)
+            self.CCOL = torch.empty(max(tot, 1), dtype=torch.int32, device=dev)
+            self.CORD = torch.empty(max(tot, 1), dtype=torch.int32, device=dev)
+            self.CSTART = torch.zeros(257, dtype=torch.int32, device=dev)
+            self.CNC = torch.zeros(1, dtype=torch.int32, device=dev)
+            return
+        self.CROWS = torch.zeros((self._ccap, 4), dtype=torch.int32, device=dev)
+        self.CLAM = torch.zeros(self._ccap, dtype=torch.float32, device=dev)
 
     def _device_contact_pass(self) -> None:
         """Generate this frame's CONTACT rows on the device; grow the buffer
@@ -801,8 +837,9 @@ class Solver:
         rows, anc = pack_rows(h["type"], ra, rb, h["rest"], h["compliance"],
                               h["anchor"], h["mu"])
         if len(rows):
-            self.R.copy_(self._t(np.ascontiguousarray(rows[self._perm])))
-            self.ANC.copy_(self._t(np.ascontiguousarray(anc[self._perm])))
+            m = len(rows)
+            self.R[:m].copy_(self._t(np.ascontiguousarray(rows[self._perm])))
+            self.ANC[:m].copy_(self._t(np.ascontiguousarray(anc[self._perm])))
 
     # ---- the step (solver.py:412-467) --------------------------------------
     def _signature(self):
@@ -949,11 +986,53 @@ class Solver:
                 return
             if self._device_contacts:
                 self._device_contact_pass()
+                if self.solve_mode == "gs":
+                    self._gs_device_frame()
+                    continue
             plan = self._get_plan()
             n_frames = 1 if per_frame else int(frames)
             graph = int(self.use_graph and not self._self_collision)
             N.check(N.lib().xpbd_frame3_run(plan, N.current_stream(), n_frames, graph),
                     "xpbd_frame3_run")
+
+    def _gs_device_frame(self) -> None:
+        """One Gauss-Seidel frame with this frame's device CONTACT rows:
+        device colouring of every row, then the frame in one workgroup."""
+        lib = N.lib()
+        st = N.current_stream()
+        n_c = len(self._rt)
+        N.check(lib.xpbd_color_rows_device(st, N.ptr(self.R), n_c, N.ptr(self.CCOUNT),
+                                           self._ccap, int(self.X.shape[0]), N.ptr(self.CINC),
+                                           N.ptr(self.CCOL), N.ptr(self.CORD),
+                                           N.ptr(self.CSTART), N.ptr(self.CNC)),
+                "xpbd_color_rows_device")
+        nc = int(self.CNC.item())
+        if nc < 0:
+            raise RuntimeError("device colouring needed more than 256 colours")
+        self.num_colors = nc
+        h = float(np.float32(self.dt / float(self.substeps)))
+        N.check(lib.xpbd_frame_block_ordered(
+            st, N.ptr(self.X), N.ptr(self.V), N.ptr(self.XP), N.ptr(self.R), N.ptr(self.ANC),
+            N.ptr(self.LAM), N.ptr(self.CSTART), N.ptr(self.CORD), N.ptr(self.CNC),
+            int(self.X.shape[0]), self.substeps, self.iterations, h, *self.gravity,
+            self.damping, self.jacobi_relax, self.max_speed, 1), "xpbd_frame_block_ordered")
+
+    def device_contact_rows(self) -> dict:
+        """This frame's device CONTACT rows in API particle indices: a, b,
+        rest, mu (and, in Gauss-Seidel mode, their colours)."""
+        cnt = min(int(self.n_contacts), getattr(self, "_ccap", 0))
+        rows = self.CROWS[:cnt].cpu().numpy() if self.CROWS is not None else np.zeros((0, 4),
+                                                                                      np.int32)
+        a = (rows[:, 0] & 0x0FFFFFFF).astype(np.int64)
+        b = rows[:, 1].astype(np.int64)
+        if self._pperm is not None:
+            a, b = self._pperm[a], self._pperm[b]
+        out = dict(a=a, b=b, rest=rows[:, 2].copy().view(np.float32),
+                   mu=rows[:, 3].copy().view(np.float32))
+        if self.solve_mode == "gs" and self.CROWS is not None:
+            m = len(self._rt)
+            out["color"] = self.CCOL[m:m + cnt].cpu().numpy()
+        return out
 
     def _sanitize_positions(self) -> None:
         """Reset non-finite positions to their initial scene values and zero

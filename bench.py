@@ -345,14 +345,16 @@ def reference_rows(frames: int = 120, warmup: int = 20) -> dict:
             put(f"cloth{res}", mode, s, time_frames(s.step, frames, warmup))
         s = sphere_stack(Solver, 8, 5, mode, device="cuda:0")
         put("stack8x8x5", mode, s, time_frames(s.step, frames, warmup))
-    # the same Jacobi stack with its CONTACT rows generated on the device
-    # (the host path's rows, in its order) instead of the per-frame host
-    # broad phase and re-upload
-    s = sphere_stack(Solver, 8, 5, "jacobi", device="cuda:0")
-    s.enable_self_collision(False)
-    s.enable_device_contacts(True, margin=0.0)
-    put("stack8x8x5", "jacobi", s, time_frames(s.step, frames, warmup),
-        key="stack8x8x5_jacobi_device_contacts")
+    # the same stacks with their CONTACT rows generated on the device (the
+    # host path's rows, in its order) instead of the per-frame host broad
+    # phase and re-upload; in GS mode also recoloured on the device and the
+    # frame run in one workgroup
+    for mode in ("jacobi", "gs"):
+        s = sphere_stack(Solver, 8, 5, mode, device="cuda:0")
+        s.enable_self_collision(False)
+        s.enable_device_contacts(True, margin=0.0)
+        put("stack8x8x5", mode, s, time_frames(s.step, frames, warmup),
+            key=f"stack8x8x5_{mode}_device_contacts")
     for dims in ((3, 3, 3), (5, 4, 5), (8, 6, 8), (12, 10, 12)):
         s = box_pile(Solver6DOF, *dims, device="cuda:0", broadphase="lbvh")
         n = dims[0] * dims[1] * dims[2]

@@ -547,8 +547,45 @@ def load_traffic(kernel: str):
     return None if e is None else e.get("bytes_per_launch")
 
 
+def scene_only(name: str, frames: int, warmup: int) -> dict:
+    """Time one scene alone (for kernel traces of its frames): ``boxesN``
+    (the reference's N-box LBVH pile, N in 27, 100, 384, 1440), ``cloth128``
+    (the headline cloth, Jacobi), ``stack-gs-device`` (the 8 x 8 x 5 sphere
+    stack in GS mode with device CONTACT rows), ``tile-kernel`` (the
+    headline cloth's plain k_jacobi_tile launch replayed from a graph)."""
+    import torch
+
+    from xpbd_amd import Solver, Solver6DOF
+    from xpbd_amd import _native as N
+    from xpbd_amd.scenes import box_pile, cloth_sheet, sphere_stack
+
+    dims = {"boxes27": (3, 3, 3), "boxes100": (5, 4, 5), "boxes384": (8, 6, 8),
+            "boxes1440": (12, 10, 12)}
+    if name in dims:
+        s = box_pile(Solver6DOF, *dims[name], device="cuda:0", broadphase="lbvh")
+    elif name in ("cloth128", "tile-kernel"):
+        s = cloth_sheet(Solver, res=128, solve_mode="jacobi", device="cuda:0")[0]
+        s.tile_size = 64
+    elif name == "stack-gs-device":
+        s = sphere_stack(Solver, 8, 5, "gs", device="cuda:0")
+        s.enable_self_collision(False)
+        s.enable_device_contacts(True, margin=0.0)
+    else:
+        raise SystemExit(f"unknown scene {name!r}")
+    if name == "tile-kernel":
+        s.step(warmup)
+        h = float(torch.tensor(s.dt / s.substeps, dtype=torch.float32))
+        ts, hs, hr = s._tile_dev
+        launch = tile_launcher(N, s.X, s.X_B, s.XP, s.V, s.R, None, s.LAM, s.LAM_B, ts, hs, hr,
+                               h, s._tile, s.num_bodies)
+        return {"scene": name, "kernel_ms": time_kernel_graph(launch, frames)}
+    return {"scene": name, "ms_per_frame": time_frames(s.step, frames, warmup)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--scene", default=None,
+                    help="time only this scene and print its line (see scene_only)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
@@ -584,6 +621,9 @@ def main() -> int:
     from xpbd_amd.scenes import cloth_sheet
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.scene:
+        print(json.dumps(scene_only(args.scene, args.steps, args.warmup)), flush=True)
+        return 0
     s, _ = cloth_sheet(Solver, res=args.res, solve_mode=args.mode, device="cuda:0")
     host = s.host_arrays()
     n_dist = int((host["type"] == 0).sum())

@@ -396,7 +396,7 @@ typedef struct xpbd_rigid_desc {
   int* pt_idx;                             /* int4[8cap] */
   float *pt_n, *pt_oi, *pt_or;             /* float4[8cap] */
   float* lam_pt;                           /* float[8cap] */
-  int* pt_count;                           /* int[1] */
+  int* pt_count;                           /* int[1]; int[2] with aux */
   const int* j_idx;                        /* int2[nj] */
   const float *j_a, *j_b;                  /* float4[nj] */
   float* lam_j;                            /* float[nj] */
@@ -405,6 +405,7 @@ typedef struct xpbd_rigid_desc {
   float h, gx, gy, gz, floor_y, friction, restitution, lin_damp, ang_damp, relax;
   float max_speed, max_omega, man_margin;
   const float* gscale;                     /* float[n] gravity factor, or NULL */
+  float* aux;  /* float4[6n] for the two-launch substep (zero on entry), or NULL */
 } xpbd_rigid_desc;
 
 /* Solver6DOF._record (solver_6dof.py:357-426) as a plan; run() replays one
@@ -421,6 +422,30 @@ typedef struct xpbd_rigid_desc {
 int xpbd6_frame_block(void* stream, const xpbd_rigid_desc* desc, float* LO, float* HI, float dt,
                       int frames);
 int xpbd6_frame_block_capacity(void);
+
+/* The substep of solver_6dof.py:357-426 in two launches (plus one per
+ * further iteration), iterations >= 1; the frame plan's sequence when
+ * desc->aux is set (float4[6n]: predicted pose, then the parity-1 copies of
+ * dx, drot, dv, dw; pt_count int[2]; all zero on the first call).
+ * substep_contacts(sub): apply_velocity + cap of substep sub-1 (sub > 0;
+ * solver_6dof.py:420-426) and integrate_bodies (solver_6dof.py:370-373),
+ * recomputed on values by every lane that needs a body, then the first
+ * Jacobi iteration: joints, floor corners and the box manifold
+ * (solver_6dof.py:376-384) with each point solved as it is emitted
+ * (solver_6dof.py:385-401); stores the predicted pose in aux and the start
+ * pose in x_prev / q_prev.  Further iterations: xpbd6_apply_finalize
+ * (finalize 0) and xpbd6_solve_all (first 0) on the predicted pose.
+ * substep_velocity(sub): apply_jacobi of the last iteration and
+ * finalize_velocity on values (solver_6dof.py:402-408), then the velocity
+ * pass (solver_6dof.py:409-419); stores x, q, v, w.  substep_end: the
+ * frame's last apply_velocity + cap; every sum buffer and both point
+ * counters back to zero.  Sums of substep sub use parity sub & 1 of the
+ * sum buffers, so no lane reads what another lane of the same launch
+ * writes; results equal those of the separate launches up to the order of
+ * the atomic sums. */
+int xpbd6_substep_contacts(void* stream, const xpbd_rigid_desc* desc, int sub);
+int xpbd6_substep_velocity(void* stream, const xpbd_rigid_desc* desc, int sub);
+int xpbd6_substep_end(void* stream, const xpbd_rigid_desc* desc);
 
 int xpbd_frame6_create(const xpbd_rigid_desc* desc, void** handle);
 int xpbd_frame6_run(void* handle, void* stream, int frames, int use_graph);

@@ -183,19 +183,28 @@ def _pile_scene(seed=0, gyroscopic=False):
     return s
 
 
-@pytest.mark.parametrize("block,gyro", [(False, False), (True, False), (False, True),
-                                        (True, True)])
-def test_rigid_substep_parity_and_contact_counts(oracle, block, gyro):
+def _set_mode(s, mode):
+    """Frame path: the two-launch substep plan, the one-launch-per-phase
+    plan, or the whole-frame workgroup kernel."""
+    s.block_frames = mode == "block"
+    s.substep_launches = 5 if mode == "five" else 2
+
+
+MODES = ["two", "five", "block"]
+
+
+@pytest.mark.parametrize("mode,gyro", [(m, g) for g in (False, True) for m in MODES])
+def test_rigid_substep_parity_and_contact_counts(oracle, mode, gyro):
     """From identical inputs (the oracle's trajectory), one substep on the
     device matches the oracle: pair lists and per-pair contact counts
-    bit-exact, x/q/v/w within 1e-4; through the per-launch plan and through
-    the whole-frame workgroup kernel, with and without the gyroscopic term
-    (kernels_6dof.py:91-101)."""
+    bit-exact, x/q/v/w within 1e-4; through both per-launch plans and
+    through the whole-frame workgroup kernel, with and without the
+    gyroscopic term (kernels_6dof.py:91-101)."""
     s_full = _pile_scene(gyroscopic=gyro)
     orc = _oracle6(oracle, s_full)
     h = float(np.float32(1 / 60 / 8))
     s = _pile_scene(gyroscopic=gyro)
-    s.block_frames = block
+    _set_mode(s, mode)
     s.dt, s.substeps = h, 1
     s._flush()
     one = _oracle6(oracle, s)
@@ -219,8 +228,8 @@ def test_rigid_substep_parity_and_contact_counts(oracle, block, gyro):
     assert total_points > 0
 
 
-@pytest.mark.parametrize("block", [False, True])
-def test_rigid_frames_track_oracle(oracle, block):
+@pytest.mark.parametrize("mode", MODES)
+def test_rigid_frames_track_oracle(oracle, mode):
     """Whole frames (8 substeps x 2 iterations; graph replay, or one
     workgroup launch) over 20 frames of a 60-box pile.  The pile is chaotic:
     a one-ulp change of the oracle's own start positions grows to a 2.7e-3
@@ -228,7 +237,7 @@ def test_rigid_frames_track_oracle(oracle, block):
     device trajectory must stay within 1e-3, or within 4x that
     self-sensitivity of the oracle where it is larger."""
     s = _pile_scene(1)
-    s.block_frames = block
+    _set_mode(s, mode)
     h = s.host_arrays()
     orc = _oracle6(oracle, s)
     pert = _oracle6(oracle, s)
@@ -267,6 +276,38 @@ def test_block_frames_match_launch_plan():
         assert np.all(np.isfinite(x))
         assert x[:, 1].min() > 0.16 - 0.02
         assert np.abs(v).max() < 0.5
+
+
+@pytest.mark.parametrize("dims,iters", [((8, 6, 8), 1), ((5, 4, 5), 2)])
+def test_two_launch_substep_matches_five_launch_plan(dims, iters):
+    """The two-launch substep (per-body phases recomputed on values inside
+    the contact and velocity launches) against one launch per phase on a
+    reference benchmark pile: identical pair lists and contact counts after
+    the first frame, states within 1e-5 after it (float atomics sum in a
+    different order), both piles settled after 40 frames; one and two
+    Jacobi iterations (the latter runs the extra apply + solve launches on
+    the predicted pose)."""
+    from xpbd_amd import Solver6DOF
+    from xpbd_amd.scenes import box_pile
+
+    a = box_pile(Solver6DOF, *dims, device=DEV, broadphase="lbvh", iterations=iters)
+    b = box_pile(Solver6DOF, *dims, device=DEV, broadphase="lbvh", iterations=iters)
+    for s, m in ((a, "two"), (b, "five")):
+        _set_mode(s, m)
+        s.step()
+    assert a.n_pairs == b.n_pairs > 0
+    assert (a.pairs() == b.pairs()).all()
+    assert (a.contact_counts() == b.contact_counts()).all()
+    assert _rel(a.positions(), b.positions()) <= 1e-5
+    assert _rel(a.orientations(), b.orientations()) <= 1e-5
+    for _ in range(39):
+        a.step()
+        b.step()
+    for s in (a, b):
+        x, v = s.positions(), s.velocities()
+        assert np.all(np.isfinite(x))
+        assert x[:, 1].min() > 0.16 - 0.02
+        assert np.abs(v).max() < 1.0
 
 
 # ---- the reference's invariant tests (tests/test_solver_6dof.py) -----------

@@ -59,7 +59,7 @@ class RigidDesc(ctypes.Structure):
         + [(k, c_float) for k in ("h", "gx", "gy", "gz", "floor_y", "friction", "restitution",
                                   "lin_damp", "ang_damp", "relax", "max_speed", "max_omega",
                                   "man_margin")]
-        + [("gscale", c_void_p)]
+        + [("gscale", c_void_p), ("aux", c_void_p)]
     )
 
 
@@ -125,6 +125,9 @@ _SIGS = {
     "xpbd6_velocity_all": (_I, [_P] * 5 + [_F, _F, _P, _F, _P, _P, _I] + [_P] * 6 + [_I, _P, _P]),
     "xpbd_frame6_create": (_I, [ctypes.POINTER(RigidDesc), ctypes.POINTER(c_void_p)]),
     "xpbd_frame6_run": (_I, [_P, _P, _I, _I]),
+    "xpbd6_substep_contacts": (_I, [_P, _P, _I]),
+    "xpbd6_substep_velocity": (_I, [_P, _P, _I]),
+    "xpbd6_substep_end": (_I, [_P, _P]),
     "xpbd_frame6_destroy": (_I, [_P]),
     "xpbd6_body_aabb": (_I, [_P] * 6 + [_F, _F, _P, _P, _I]),
     "xpbd6_lbvh_workspace_bytes": (ctypes.c_longlong, [_I]),

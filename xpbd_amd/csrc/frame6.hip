@@ -7,6 +7,7 @@
 #include <new>
 
 #include "common.h"
+#include "rigid_sub.h"
 #include "xpbd.h"
 
 namespace {
@@ -62,6 +63,45 @@ int issue(Frame6* f, hipStream_t s) {
   return xpbd6_apply_velocity(s, d.x, d.v, d.w, d.dv, d.dw, d.max_speed, d.max_omega, cap_v, n);
 }
 
+// Two launches per substep (desc.aux set, iterations >= 1): the contacts
+// launch (apply_velocity + integrate folded in, first iteration solved with
+// the manifold), for each further iteration apply_jacobi and one solve
+// launch over every row kind on the predicted pose, then the velocity launch
+// (last apply_jacobi and finalize_velocity folded in); one launch ends the
+// frame.  Same operations in the same order as issue().
+int issue_two_launch(Frame6* f, hipStream_t s) {
+  const xpbd_rigid_desc& d = f->d;
+  const int n = d.n;
+  if (n <= 0) return 0;
+  const int maxp = 8 * d.cap;
+  int rc;
+  for (int sub = 0; sub < d.substeps; ++sub) {
+    rc = xpbd6_substep_contacts(s, &d, sub);
+    if (rc) return rc;
+    const SubBufs b = sub_bufs(d, sub);
+    float* XPR = (float*)b.XPR;
+    float* QPR = (float*)b.QPR;
+    for (int it = 1; it < d.iterations; ++it) {
+      rc = xpbd6_apply_finalize(s, XPR, QPR, (float*)b.DX, (float*)b.DR, d.relax, 0, d.x_prev,
+                                d.q_prev, d.v, d.w, d.h, d.lin_damp, d.ang_damp, n);
+      if (rc) return rc;
+      rc = xpbd6_solve_all(s, XPR, QPR, d.x_prev, d.q_prev, d.inv_I, d.he, d.j_idx, d.j_a, d.j_b,
+                           d.lam_j, d.h, 0, d.nj, d.floor_y, d.friction, d.lam_floor, n, d.pt_idx,
+                           d.pt_n, d.pt_oi, d.pt_or, d.lam_pt, b.PTC, maxp, (float*)b.DX,
+                           (float*)b.DR);
+      if (rc) return rc;
+    }
+    rc = xpbd6_substep_velocity(s, &d, sub);
+    if (rc) return rc;
+  }
+  return xpbd6_substep_end(s, &d);
+}
+
+int issue_frame(Frame6* f, hipStream_t s) {
+  if (f->d.aux && f->d.iterations >= 1) return issue_two_launch(f, s);
+  return issue(f, s);
+}
+
 void drop_graph(Frame6* f) {
   if (f->exec) (void)hipGraphExecDestroy(f->exec);
   if (f->graph) (void)hipGraphDestroy(f->graph);
@@ -88,7 +128,7 @@ int xpbd_frame6_run(void* handle, void* stream, int frames, int use_graph) {
   hipStream_t s = (hipStream_t)stream;
   if (!use_graph) {
     for (int k = 0; k < frames; ++k) {
-      int rc = issue(f, s);
+      int rc = issue_frame(f, s);
       if (rc) return rc;
     }
     return 0;
@@ -96,7 +136,7 @@ int xpbd_frame6_run(void* handle, void* stream, int frames, int use_graph) {
   if (!f->exec) {
     if (!f->capture_stream) XPBD_CHECK(hipStreamCreateWithFlags(&f->capture_stream, hipStreamNonBlocking));
     XPBD_CHECK(hipStreamBeginCapture(f->capture_stream, hipStreamCaptureModeThreadLocal));
-    int rc = issue(f, f->capture_stream);
+    int rc = issue_frame(f, f->capture_stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(f->capture_stream, &g);
     if (rc) {

@@ -20,6 +20,7 @@
 
 #include "common.h"
 #include "rigid_math.h"
+#include "rigid_sub.h"
 #include "xpbd.h"
 
 #define F4(p) ((float4*)(p))
@@ -33,6 +34,11 @@ __device__ inline void atomic_add3(float4* p, f3 v) {
   atomicAdd(&p->x, v.x);
   atomicAdd(&p->y, v.y);
   atomicAdd(&p->z, v.z);
+}
+
+// A counter written by an earlier launch (or atomically in this one).
+__device__ inline int ld_count(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // kernels_6dof.py:108-130 with atomics (B may be -1 / static)
@@ -51,6 +57,35 @@ __device__ inline void accumulate_correction(float4* DX, float4* DR, int a, int 
   }
 }
 
+// kernels_6dof.py:67-102 on values: one substep's prediction of a dynamic
+// body from its start pose (x0, q) and velocity (v, w); returns the
+// predicted position and updates q, v, w in place (w only with gyro).
+__device__ inline f3 integrate_vals(f3 x0, q4& q, f3& v, f3& w, f3 ii, f3 g, float h,
+                                    int gyro) {
+  v = v + g * h;
+  const f3 x = x0 + v * h;
+  if (gyro != 0) {
+    f3 wb = qrot_inv(q, w);
+    float ix = ii.x > 0.0f ? 1.0f / ii.x : 0.0f;
+    float iy = ii.y > 0.0f ? 1.0f / ii.y : 0.0f;
+    float iz = ii.z > 0.0f ? 1.0f / ii.z : 0.0f;
+    f3 Iw = mk3(ix * wb.x, iy * wb.y, iz * wb.z);
+    f3 tau = -cross(wb, Iw);
+    f3 dwb = mk3(ii.x * tau.x, ii.y * tau.y, ii.z * tau.z) * h;
+    w = w + qrot(q, dwb);
+  }
+  q = quat_integrate(q, w, h);
+  return x;
+}
+
+// gravity of body i (gscale: optional per-body factor, batched scenes)
+__device__ inline f3 body_gravity(int i, float gx, float gy, float gz,
+                                  const float* __restrict__ gscale) {
+  f3 g = mk3(gx, gy, gz);
+  if (gscale) g = g * gscale[i];
+  return g;
+}
+
 // kernels_6dof.py:67-102 (+ per-substep lambda_floor reset and contact
 // point counter reset, both exact rewrites of solver_6dof.py:374-375)
 // gscale (optional): per-body gravity factor (batched independent scenes)
@@ -67,25 +102,11 @@ __device__ inline void integrate_elem(int i, float4* X, float4* Q, float4* V, fl
   lf[0] = make_float4(0.f, 0.f, 0.f, 0.f);
   lf[1] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (x4.w == 0.0f) return;
-  f3 g = mk3(gx, gy, gz);
-  if (gscale) g = g * gscale[i];
-  f3 v = xyz(V[i]) + g * h;
-  f3 x = xyz(x4) + v * h;
+  f3 v = xyz(V[i]), w = xyz(W[i]);
+  const f3 x = integrate_vals(xyz(x4), q, v, w, xyz(IB[i]), body_gravity(i, gx, gy, gz, gscale),
+                              h, gyro);
   V[i] = make_float4(v.x, v.y, v.z, 0.0f);
-  f3 w = xyz(W[i]);
-  if (gyro != 0) {
-    f3 wb = qrot_inv(q, w);
-    f3 ii = xyz(IB[i]);
-    float ix = ii.x > 0.0f ? 1.0f / ii.x : 0.0f;
-    float iy = ii.y > 0.0f ? 1.0f / ii.y : 0.0f;
-    float iz = ii.z > 0.0f ? 1.0f / ii.z : 0.0f;
-    f3 Iw = mk3(ix * wb.x, iy * wb.y, iz * wb.z);
-    f3 tau = -cross(wb, Iw);
-    f3 dwb = mk3(ii.x * tau.x, ii.y * tau.y, ii.z * tau.z) * h;
-    w = w + qrot(q, dwb);
-    W[i] = make_float4(w.x, w.y, w.z, 0.0f);
-  }
-  q = quat_integrate(q, w, h);
+  if (gyro != 0) W[i] = make_float4(w.x, w.y, w.z, 0.0f);
   X[i] = make_float4(x.x, x.y, x.z, x4.w);
   Q[i] = stq(q);
 }
@@ -138,26 +159,40 @@ struct LdsPoly {
 };
 constexpr int kPolyFloats = 2 * 8 * 3;
 
-template <class Poly>
-__device__ inline void box_manifold_elem(
-    int pid, int live, const float4* __restrict__ X, const float4* __restrict__ Q,
-    const float4* __restrict__ HE, const int2* __restrict__ PAIRS, float margin, int cap,
-    int* __restrict__ MC, int4* __restrict__ PTI, float4* __restrict__ PTN,
-    float4* __restrict__ PTOI, float4* __restrict__ PTOR, float* __restrict__ LAMP,
-    int* __restrict__ PTC, Poly& poly) {
+// The two boxes of a candidate pair as the manifold sees them (pose and
+// half extents; inverse mass and inertia for the solve of its points).
+struct PairBodies {
+  int a, b;
+  f3 cA, cB, eA, eB, iIA, iIB;
+  q4 qa, qb;
+  float imA, imB;
+};
+
+// SOLVE: each emitted point is also solved at once, as solve_box_elem does
+// for it in the first iteration (the same operations on the same values:
+// the point's body-frame offsets taken back to the world through the same
+// poses), adding to DX / DR and writing its multiplier.
+template <bool SOLVE, class Poly>
+__device__ inline void box_manifold_pair(
+    int pid, bool valid, const PairBodies& P, float margin, int cap, int* __restrict__ MC,
+    int4* __restrict__ PTI, float4* __restrict__ PTN, float4* __restrict__ PTOI,
+    float4* __restrict__ PTOR, float* __restrict__ LAMP, int* __restrict__ PTC, Poly& poly,
+    float4* DX, float4* DR) {
   const int lane = threadIdx.x & 63;
   int nc = 0, inc_b = 0, ref_b = 0;
   f3 ref_face_n = mk3(0.f, 0.f, 0.f);
+  q4 q_inc_s{0.f, 0.f, 0.f, 1.f}, q_ref_s = q_inc_s;  // for SOLVE
+  float im_inc = 0.0f, im_ref = 0.0f;
+  f3 iI_inc = ref_face_n, iI_ref = ref_face_n;
   // the surviving polygon and the frames its points are expressed in
   int cur = 0, plen = 0;
   f3 ref_face_c = mk3(0.f, 0.f, 0.f), c_inc = ref_face_c, c_ref = ref_face_c;
   f3 ri0 = c_inc, ri1 = c_inc, ri2 = c_inc, rr0 = c_inc, rr1 = c_inc, rr2 = c_inc;  // frames
-  if (pid < live) {
-    const int2 pr = PAIRS[pid];
-    const int a = pr.x, b = pr.y;
-    const f3 cA = xyz(X[a]), cB = xyz(X[b]);
-    const f3 eA = xyz(HE[a]), eB = xyz(HE[b]);
-    const q4 qa = ldq(Q[a]), qb = ldq(Q[b]);
+  if (valid) {
+    const int a = P.a, b = P.b;
+    const f3 cA = P.cA, cB = P.cB;
+    const f3 eA = P.eA, eB = P.eB;
+    const q4 qa = P.qa, qb = P.qb;
     const m3 RA = qmat(qa), RB = qmat(qb);
     const f3 cBA = cA - cB;
     float best_overlap = 1.0e30f;
@@ -192,6 +227,14 @@ __device__ inline void box_manifold_elem(
       c_inc = sel3(ra, cB, cA);
       const f3 e_ref = sel3(ra, eA, eB), e_inc = sel3(ra, eB, eA);
       const q4 q_ref = selq(ra, qa, qb), q_inc = selq(ra, qb, qa);
+      if (SOLVE) {
+        q_inc_s = q_inc;
+        q_ref_s = q_ref;
+        im_inc = ra ? P.imB : P.imA;
+        im_ref = ra ? P.imA : P.imB;
+        iI_inc = sel3(ra, P.iIB, P.iIA);
+        iI_ref = sel3(ra, P.iIA, P.iIB);
+      }
       ref_face_n = ra ? -nBA : nBA;
       const m3 R_ref = qmat(q_ref), R_inc = qmat(q_inc);
       const int ref_axis = best_k;
@@ -275,14 +318,57 @@ __device__ inline void box_manifold_elem(
     const f3 oi = mk3(dot(ri0, vi), dot(ri1, vi), dot(ri2, vi));   // mtv(R_inc, vi)
     const f3 orf = mk3(dot(rr0, vr), dot(rr1, vr), dot(rr2, vr));  // mtv(R_ref, vr)
     const int q = first + c;
+    if (q >= 8 * cap) break;  // a counter left non-zero: drop, never write past
     PTI[q] = make_int4(pid, inc_b, ref_b, c);
     PTN[q] = make_float4(ref_face_n.x, ref_face_n.y, ref_face_n.z, 0.0f);
     PTOI[q] = make_float4(oi.x, oi.y, oi.z, 0.0f);
     PTOR[q] = make_float4(orf.x, orf.y, orf.z, 0.0f);
-    LAMP[q] = 0.0f;
+    float lam = 0.0f;
+    if (SOLVE) {  // solve_box_elem on this point's stored values
+      const f3 s = ref_face_n;
+      const f3 pi = c_inc + qrot(q_inc_s, oi);
+      const f3 pr = c_ref + qrot(q_ref_s, orf);
+      const float C = dot(pi - pr, s);
+      if (!(C >= 0.0f)) {
+        const f3 r_inc = pi - c_inc, r_ref = pr - c_ref;
+        const float wsum = gen_inv_mass(im_inc, q_inc_s, iI_inc, r_inc, s) +
+                           gen_inv_mass(im_ref, q_ref_s, iI_ref, r_ref, s);
+        if (!(wsum <= 0.0f)) {
+          const float dlam = -C / wsum;
+          lam = 0.0f + dlam;
+          accumulate_correction(DX, DR, inc_b, ref_b, im_inc, im_ref, q_inc_s, q_ref_s, iI_inc,
+                                iI_ref, r_inc, r_ref, s * dlam);
+        }
+      }
+    }
+    LAMP[q] = lam;
     ++c;
   }
   if (pid < cap) MC[pid] = nc;
+}
+
+template <class Poly>
+__device__ inline void box_manifold_elem(
+    int pid, int live, const float4* __restrict__ X, const float4* __restrict__ Q,
+    const float4* __restrict__ HE, const int2* __restrict__ PAIRS, float margin, int cap,
+    int* __restrict__ MC, int4* __restrict__ PTI, float4* __restrict__ PTN,
+    float4* __restrict__ PTOI, float4* __restrict__ PTOR, float* __restrict__ LAMP,
+    int* __restrict__ PTC, Poly& poly) {
+  PairBodies P;
+  const bool valid = pid < live;
+  if (valid) {
+    const int2 pr = PAIRS[pid];
+    P.a = pr.x;
+    P.b = pr.y;
+    P.cA = xyz(X[P.a]);
+    P.cB = xyz(X[P.b]);
+    P.eA = xyz(HE[P.a]);
+    P.eB = xyz(HE[P.b]);
+    P.qa = ldq(Q[P.a]);
+    P.qb = ldq(Q[P.b]);
+  }
+  box_manifold_pair<false>(pid, valid, P, margin, cap, MC, PTI, PTN, PTOI, PTOR, LAMP, PTC, poly,
+                           nullptr, nullptr);
 }
 
 __global__ __launch_bounds__(kB) void k6_box_manifold(
@@ -297,16 +383,12 @@ __global__ __launch_bounds__(kB) void k6_box_manifold(
                     MC, PTI, PTN, PTOI, PTOR, LAMP, PTC, poly);
 }
 
-// kernels_6dof.py:440-483 (lambda reset folded into the first iteration)
-__device__ inline void solve_joints_elem(int k, 
-    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
-    const int2* __restrict__ JI, const float4* __restrict__ JA, const float4* __restrict__ JB,
-    float* __restrict__ LAMJ, float h, int first, float4* DX, float4* DR, int nj) {
-  const int2 ab = JI[k];
-  const int a = ab.x, b = ab.y;
-  const float4 ja = JA[k], jb = JB[k];
-  const float4 xa4 = X[a], xb4 = X[b];
-  const q4 qa = ldq(Q[a]), qb = ldq(Q[b]);
+// kernels_6dof.py:440-483 (lambda reset folded into the first iteration) on
+// the poses of the joint's bodies: xa4 / xb4 = (position, inverse mass).
+__device__ inline void solve_joint_vals(int k, int a, int b, float4 xa4, q4 qa, f3 iIa,
+                                        float4 xb4, q4 qb, f3 iIb, float4 ja, float4 jb,
+                                        float* __restrict__ LAMJ, float h, int first,
+                                        float4* DX, float4* DR) {
   float lam = first ? 0.0f : LAMJ[k];
   if (first) LAMJ[k] = 0.0f;
   f3 pa = xyz(xa4) + qrot(qa, xyz(ja));
@@ -317,7 +399,6 @@ __device__ inline void solve_joints_elem(int k,
   f3 nn = d / dist;
   float C = dist - ja.w;
   f3 ra = pa - xyz(xa4), rb = pb - xyz(xb4);
-  f3 iIa = xyz(IB[a]), iIb = xyz(IB[b]);
   float w_a = gen_inv_mass(xa4.w, qa, iIa, ra, nn);
   float w_b = gen_inv_mass(xb4.w, qb, iIb, rb, nn);
   float wsum = w_a + w_b;
@@ -327,6 +408,16 @@ __device__ inline void solve_joints_elem(int k,
   LAMJ[k] = lam + dlam;
   f3 p = nn * dlam;
   accumulate_correction(DX, DR, a, b, xa4.w, xb4.w, qa, qb, iIa, iIb, ra, rb, p);
+}
+
+__device__ inline void solve_joints_elem(int k,
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const int2* __restrict__ JI, const float4* __restrict__ JA, const float4* __restrict__ JB,
+    float* __restrict__ LAMJ, float h, int first, float4* DX, float4* DR, int nj) {
+  const int2 ab = JI[k];
+  const int a = ab.x, b = ab.y;
+  solve_joint_vals(k, a, b, X[a], ldq(Q[a]), xyz(IB[a]), X[b], ldq(Q[b]), xyz(IB[b]), JA[k],
+                   JB[k], LAMJ, h, first, DX, DR);
 }
 
 __global__ __launch_bounds__(kB) void k6_solve_joints(
@@ -457,19 +548,28 @@ __global__ __launch_bounds__(kB) void k6_solve_box(
   solve_box_elem(k, X, Q, IB, PTI, PTN, PTOI, PTOR, LAMP, PTC, DX, DR, max_points);
 }
 
-// kernels_6dof.py:486-506
-__device__ inline void apply_jacobi_elem(int i, float4* X, float4* Q, float4* DX,
-                                                      float4* DR, float relax, int n) {
-  const float4 acc = DX[i];
-  const float4 x4 = X[i];
+// kernels_6dof.py:486-506 on values: the Jacobi sums (acc = sum dx, count;
+// drs = sum d-rotation) applied to the pose (x4 = position, inverse mass).
+__device__ inline void apply_jacobi_vals(float4 acc, f3 drs, float relax, float4& x4, q4& q) {
   if (acc.w > 0.0f && x4.w > 0.0f) {
     const float inv_n = relax / acc.w;
     const f3 x = xyz(x4) + xyz(acc) * inv_n;
-    const f3 dw = xyz(DR[i]) * inv_n;
+    const f3 dw = drs * inv_n;
     const q4 wq{dw.x, dw.y, dw.z, 0.0f};
-    const q4 q = ldq(Q[i]);
-    X[i] = make_float4(x.x, x.y, x.z, x4.w);
-    Q[i] = stq(qnormalize(qadd(q, qscale(qmul(wq, q), 0.5f))));
+    x4 = make_float4(x.x, x.y, x.z, x4.w);
+    q = qnormalize(qadd(q, qscale(qmul(wq, q), 0.5f)));
+  }
+}
+
+__device__ inline void apply_jacobi_elem(int i, float4* X, float4* Q, float4* DX,
+                                                      float4* DR, float relax, int n) {
+  const float4 acc = DX[i];
+  float4 x4 = X[i];
+  if (acc.w > 0.0f && x4.w > 0.0f) {
+    q4 q = ldq(Q[i]);
+    apply_jacobi_vals(acc, xyz(DR[i]), relax, x4, q);
+    X[i] = x4;
+    Q[i] = stq(q);
   }
   if (acc.w != 0.0f || acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f) {
     DX[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -484,7 +584,22 @@ __global__ __launch_bounds__(kB) void k6_apply_jacobi(float4* X, float4* Q, floa
   apply_jacobi_elem(i, X, Q, DX, DR, relax, n);
 }
 
-// kernels_6dof.py:512-535
+// kernels_6dof.py:512-535 on values: velocities of a body from its pose
+// (x4 = position, inverse mass) and substep-start pose; zero when static.
+__device__ inline void finalize_vals(float4 x4, q4 q, float4 xp4, q4 qp, float h, float lin_damp,
+                                     float ang_damp, f3& v, f3& w) {
+  if (x4.w == 0.0f) {
+    v = mk3(0.f, 0.f, 0.f);
+    w = v;
+    return;
+  }
+  v = ((xyz(x4) - xyz(xp4)) / h) * (1.0f - lin_damp);
+  const q4 dq = qmul(q, qconj(qp));
+  w = mk3(dq.x, dq.y, dq.z) * (2.0f / h);
+  if (dq.w < 0.0f) w = -w;
+  w = w * (1.0f - ang_damp);
+}
+
 __device__ inline void finalize_elem(int i, const float4* __restrict__ X,
                                                   const float4* __restrict__ Q,
                                                   const float4* __restrict__ XP,
@@ -492,16 +607,8 @@ __device__ inline void finalize_elem(int i, const float4* __restrict__ X,
                                                   float4* W, float h, float lin_damp,
                                                   float ang_damp, int n) {
   const float4 x4 = X[i];
-  if (x4.w == 0.0f) {
-    V[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    W[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    return;
-  }
-  const f3 v = ((xyz(x4) - xyz(XP[i])) / h) * (1.0f - lin_damp);
-  const q4 dq = qmul(ldq(Q[i]), qconj(ldq(QP[i])));
-  f3 w = mk3(dq.x, dq.y, dq.z) * (2.0f / h);
-  if (dq.w < 0.0f) w = -w;
-  w = w * (1.0f - ang_damp);
+  f3 v, w;
+  finalize_vals(x4, ldq(Q[i]), XP[i], ldq(QP[i]), h, lin_damp, ang_damp, v, w);
   V[i] = make_float4(v.x, v.y, v.z, 0.0f);
   W[i] = make_float4(w.x, w.y, w.z, 0.0f);
 }
@@ -592,34 +699,26 @@ __global__ __launch_bounds__(kB) void k6_velocity_floor(
   velocity_floor_elem<false>(i, X, Q, IB, HE, mu_d, restitution, LAMF, h, V, W, DV, DW, n);
 }
 
-// kernels_6dof.py:593-659, one thread per compacted manifold point
-__device__ inline void velocity_box_elem(int k, 
-    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
-    const int4* __restrict__ PTI, const float4* __restrict__ PTN,
-    const float4* __restrict__ PTOI, const float4* __restrict__ PTOR,
-    const float* __restrict__ LAMP, const int* __restrict__ PTC, float mu_d, float restitution,
-    float h, const float4* __restrict__ V, const float4* __restrict__ W, float4* DV, float4* DW,
-    int max_points) {
-  const float lam_n = LAMP[k];
-  if (lam_n == 0.0f) return;
-  const int4 id = PTI[k];
-  const int ci = id.y, cj = id.z;
-  const f3 s = xyz(PTN[k]);
-  const float4 xi4 = X[ci], xj4 = X[cj];
-  const q4 qi = ldq(Q[ci]), qj = ldq(Q[cj]);
-  const f3 p_inc = xyz(xi4) + qrot(qi, xyz(PTOI[k]));
-  const f3 p_ref = xyz(xj4) + qrot(qj, xyz(PTOR[k]));
+// kernels_6dof.py:593-659 on values: one manifold point (multiplier lam_n,
+// normal s, body-frame offsets oi / orf) between bodies ci and cj, given
+// their poses (xi4 / xj4 = position, inverse mass), velocities and inverse
+// inertia.
+__device__ inline void velocity_box_vals(float lam_n, int ci, int cj, f3 s, f3 oi, f3 orf,
+                                         float4 xi4, q4 qi, f3 vi, f3 wi, f3 iIi, float4 xj4,
+                                         q4 qj, f3 vj, f3 wj, f3 iIj, float mu_d,
+                                         float restitution, float h, float4* DV, float4* DW) {
+  const f3 p_inc = xyz(xi4) + qrot(qi, oi);
+  const f3 p_ref = xyz(xj4) + qrot(qj, orf);
   const f3 ri = p_inc - xyz(xi4), rj = p_ref - xyz(xj4);
-  const f3 vci = xyz(V[ci]) + cross(xyz(W[ci]), ri);
-  const f3 vcj = xyz(V[cj]) + cross(xyz(W[cj]), rj);
+  const f3 vci = vi + cross(wi, ri);
+  const f3 vcj = vj + cross(wj, rj);
   const f3 impulse = velocity_impulse(vci - vcj, s, lam_n, mu_d, restitution, h);
   const float li = length(impulse);
   if (li > XPBD_EPS) {
     const f3 ndir = impulse / li;
-    const f3 iIi = xyz(IB[ci]), iIj = xyz(IB[cj]);
-    const float wi = gen_inv_mass(xi4.w, qi, iIi, ri, ndir);
-    const float wj = gen_inv_mass(xj4.w, qj, iIj, rj, ndir);
-    const float wsum = wi + wj;
+    const float wi_ = gen_inv_mass(xi4.w, qi, iIi, ri, ndir);
+    const float wj_ = gen_inv_mass(xj4.w, qj, iIj, rj, ndir);
+    const float wsum = wi_ + wj_;
     if (wsum > 0.0f) {
       const f3 p = impulse / wsum;
       if (xi4.w > 0.0f) {
@@ -636,6 +735,23 @@ __device__ inline void velocity_box_elem(int k,
   }
 }
 
+// kernels_6dof.py:593-659, one thread per compacted manifold point
+__device__ inline void velocity_box_elem(int k,
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const int4* __restrict__ PTI, const float4* __restrict__ PTN,
+    const float4* __restrict__ PTOI, const float4* __restrict__ PTOR,
+    const float* __restrict__ LAMP, const int* __restrict__ PTC, float mu_d, float restitution,
+    float h, const float4* __restrict__ V, const float4* __restrict__ W, float4* DV, float4* DW,
+    int max_points) {
+  const float lam_n = LAMP[k];
+  if (lam_n == 0.0f) return;
+  const int4 id = PTI[k];
+  const int ci = id.y, cj = id.z;
+  velocity_box_vals(lam_n, ci, cj, xyz(PTN[k]), xyz(PTOI[k]), xyz(PTOR[k]), X[ci], ldq(Q[ci]),
+                    xyz(V[ci]), xyz(W[ci]), xyz(IB[ci]), X[cj], ldq(Q[cj]), xyz(V[cj]),
+                    xyz(W[cj]), xyz(IB[cj]), mu_d, restitution, h, DV, DW);
+}
+
 __global__ __launch_bounds__(kB) void k6_velocity_box(
     const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
     const int4* __restrict__ PTI, const float4* __restrict__ PTN,
@@ -648,6 +764,26 @@ __global__ __launch_bounds__(kB) void k6_velocity_box(
   velocity_box_elem(k, X, Q, IB, PTI, PTN, PTOI, PTOR, LAMP, PTC, mu_d, restitution, h, V, W, DV, DW, max_points);
 }
 
+// kernels_6dof.py:662-691 on values: apply_velocity_6dof (acc = sum dv,
+// count; dws = sum d-omega) then cap_velocity_6dof; true when v or w changed.
+__device__ inline bool apply_velocity_vals(float4 acc, f3 dws, float im, float max_lin,
+                                           float max_ang, int do_cap, f3& v, f3& w) {
+  bool dirty = false;
+  if (acc.w > 0.0f && im > 0.0f) {
+    const float inv_n = 1.0f / acc.w;
+    v = v + xyz(acc) * inv_n;
+    w = w + dws * inv_n;
+    dirty = true;
+  }
+  if (do_cap) {
+    const float sp = length(v);
+    if (max_lin > 0.0f && sp > max_lin) { v = v * (max_lin / sp); dirty = true; }
+    const float sa = length(w);
+    if (max_ang > 0.0f && sa > max_ang) { w = w * (max_ang / sa); dirty = true; }
+  }
+  return dirty;
+}
+
 // kernels_6dof.py:662-691: apply_velocity_6dof then cap_velocity_6dof
 __device__ inline void apply_velocity_elem(int i, const float4* __restrict__ X, float4* V,
                                                         float4* W, float4* DV, float4* DW,
@@ -655,22 +791,11 @@ __device__ inline void apply_velocity_elem(int i, const float4* __restrict__ X, 
                                                         int n) {
   const float4 acc = DV[i];
   f3 v = xyz(V[i]), w = xyz(W[i]);
-  bool dirty = false;
-  if (acc.w > 0.0f && X[i].w > 0.0f) {
-    const float inv_n = 1.0f / acc.w;
-    v = v + xyz(acc) * inv_n;
-    w = w + xyz(DW[i]) * inv_n;
-    dirty = true;
-  }
+  const bool dirty =
+      apply_velocity_vals(acc, xyz(DW[i]), X[i].w, max_lin, max_ang, do_cap, v, w);
   if (acc.w != 0.0f || acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f) {
     DV[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     DW[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-  if (do_cap) {
-    const float sp = length(v);
-    if (max_lin > 0.0f && sp > max_lin) { v = v * (max_lin / sp); dirty = true; }
-    const float sa = length(w);
-    if (max_ang > 0.0f && sa > max_ang) { w = w * (max_ang / sa); dirty = true; }
   }
   if (dirty) {
     V[i] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -710,21 +835,18 @@ __device__ inline void stage_wait() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__device__ inline void solve_floor_corner(int i, int c,
-    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ XP,
-    const float4* __restrict__ QP, const float4* __restrict__ IB, const float4* __restrict__ HE,
-    float floor_y, float mu_s, float* __restrict__ LAMF, float4* DX, float4* DR,
+// Body i's pose (x4 = position, inverse mass; q), substep-start pose (xs,
+// qs), inverse inertia and half extents come in as values.
+__device__ inline void solve_floor_corner_vals(int i, int c, float4 x4, q4 q, f3 xs, q4 qs,
+    f3 iI, f3 he, float floor_y, float mu_s, float* __restrict__ LAMF, float4* DX, float4* DR,
     float4* slot) {
-  const float4 x4 = X[i];
   const float im = x4.w;
   f3 lin = mk3(0.f, 0.f, 0.f), ang = lin, lf = lin, af = lin;
   float touched = 0.0f, fric = 0.0f, lam = 0.0f;
   if (im != 0.0f) {
     const f3 x = xyz(x4);
-    const q4 q = ldq(Q[i]);
-    const f3 iI = xyz(IB[i]);
     const f3 s = mk3(0.0f, 1.0f, 0.0f);
-    const f3 corner = box_corner(xyz(HE[i]), c);
+    const f3 corner = box_corner(he, c);
     const f3 p1 = x + qrot(q, corner);
     const float d = floor_y - p1.y;
     if (d > 0.0f) {
@@ -738,7 +860,7 @@ __device__ inline void solve_floor_corner(int i, int c,
         ang = world_invI_mul(q, iI, cross(r, p));
         touched = 1.0f;
         if (mu_s > 0.0f) {
-          const f3 p1b = xyz(XP[i]) + qrot(ldq(QP[i]), corner);
+          const f3 p1b = xs + qrot(qs, corner);
           const f3 dp = p1 - p1b;
           const f3 dpt = dp - s * dot(dp, s);
           const float lt = length(dpt);
@@ -786,20 +908,27 @@ __device__ inline void solve_floor_corner(int i, int c,
   }
 }
 
-__device__ inline void velocity_floor_corner(int i, int c,
-    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
-    const float4* __restrict__ HE, float mu_d, float restitution,
-    const float* __restrict__ LAMF, float h, const float4* __restrict__ V,
-    const float4* __restrict__ W, float4* DV, float4* DW, float4* slot) {
-  const float im = X[i].w;
+__device__ inline void solve_floor_corner(int i, int c,
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ XP,
+    const float4* __restrict__ QP, const float4* __restrict__ IB, const float4* __restrict__ HE,
+    float floor_y, float mu_s, float* __restrict__ LAMF, float4* DX, float4* DR,
+    float4* slot) {
+  solve_floor_corner_vals(i, c, X[i], ldq(Q[i]), xyz(XP[i]), ldq(QP[i]), xyz(IB[i]),
+                          xyz(HE[i]), floor_y, mu_s, LAMF, DX, DR, slot);
+}
+
+// Body i's inverse mass, orientation, inverse inertia, half extents,
+// velocities and this corner's multiplier (lam, read only when im != 0)
+// come in as values.
+__device__ inline void velocity_floor_corner_vals(int i, int c, float im, q4 q, f3 iI, f3 he,
+    f3 v, f3 w, float lam, float mu_d, float restitution, float h, float4* DV, float4* DW,
+    float4* slot) {
   f3 lin = mk3(0.f, 0.f, 0.f), ang = lin;
   float touched = 0.0f;
-  const float lam_n = im != 0.0f ? LAMF[8 * (size_t)i + c] : 0.0f;
+  const float lam_n = im != 0.0f ? lam : 0.0f;
   if (lam_n != 0.0f) {
-    const q4 q = ldq(Q[i]);
-    const f3 iI = xyz(IB[i]);
-    const f3 r = qrot(q, box_corner(xyz(HE[i]), c));
-    const f3 vc = xyz(V[i]) + cross(xyz(W[i]), r);
+    const f3 r = qrot(q, box_corner(he, c));
+    const f3 vc = v + cross(w, r);
     const f3 s = mk3(0.0f, 1.0f, 0.0f);
     const f3 impulse = velocity_impulse(vc, s, lam_n, mu_d, restitution, h);
     if (length(impulse) > XPBD_EPS) {
@@ -832,6 +961,16 @@ __device__ inline void velocity_floor_corner(int i, int c,
     atomicAdd(&DV[i].w, dv.w);
     atomic_add3(&DW[i], xyz(dw));
   }
+}
+
+__device__ inline void velocity_floor_corner(int i, int c,
+    const float4* __restrict__ X, const float4* __restrict__ Q, const float4* __restrict__ IB,
+    const float4* __restrict__ HE, float mu_d, float restitution,
+    const float* __restrict__ LAMF, float h, const float4* __restrict__ V,
+    const float4* __restrict__ W, float4* DV, float4* DW, float4* slot) {
+  velocity_floor_corner_vals(i, c, X[i].w, ldq(Q[i]), xyz(IB[i]), xyz(HE[i]), xyz(V[i]),
+                             xyz(W[i]), LAMF[8 * (size_t)i + c], mu_d, restitution, h, DV, DW,
+                             slot);
 }
 
 // First lane of the floor corners in k6_solve_all: joints end at nj, the
@@ -916,6 +1055,197 @@ __global__ __launch_bounds__(kB) void k6_velocity_all(
 }
 
 // ---------------------------------------------------------------------------
+// Two launches per substep (frame plan, iterations >= 1).  Each per-body
+// phase is folded into the launch that needs its result: a lane recomputes
+// the per-body operations it depends on from the values the previous launch
+// left (the same operations on the same values as the separate launches, so
+// the same results), and the corner-0 lane of each body stores them once.
+//   contacts (substep s): apply_velocity + cap of substep s-1 (s > 0) and
+//     integrate, then joints, floor corners and the box manifold of every
+//     candidate pair with each point solved at once: the first Jacobi
+//     iteration.  Stores the predicted pose and the substep-start pose.
+//   velocity (substep s): apply_jacobi of the last iteration and
+//     finalize_velocity, then floor corners and box points of the velocity
+//     pass.  Stores x, q, v, w.
+//   end of frame: apply_velocity + cap of the last substep.
+// Buffer parities: rigid_sub.h.  Lanes: contacts [0, nj) joints, [f0, f0 +
+// 8 n) floor corners, [p0, p0 + cap) pairs (p0 a multiple of 64: the
+// manifold compacts its points per wavefront); velocity [0, 8 n) floor
+// corners, [8 n, 8 n + 8 cap) points.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int pair_lane0(int nj, int n) {
+  return floor_lane0(nj) + ((8 * n + 63) & ~63);
+}
+
+// Body i at the end of the contacts launch's per-body phase: predicted pose
+// (x4 = position, inverse mass; q), substep-start pose, inverse inertia.
+struct Pred {
+  float4 x4;
+  q4 q;
+  f3 xs;
+  q4 qs;
+  f3 iI;
+};
+
+__device__ inline Pred predict_body(const xpbd_rigid_desc& d, const SubBufs& b, int i) {
+  const float4 x4 = CF4(d.x)[i];
+  const q4 q0 = ldq(CF4(d.q)[i]);
+  f3 v = xyz(CF4(d.v)[i]), w = xyz(CF4(d.w)[i]);
+  Pred p;
+  p.iI = xyz(CF4(d.inv_I)[i]);
+  p.xs = xyz(x4);
+  p.qs = q0;
+  p.x4 = x4;
+  p.q = q0;
+  if (b.apply)
+    (void)apply_velocity_vals(b.DVin[i], xyz(b.DWin[i]), x4.w, d.max_speed, d.max_omega,
+                              d.max_speed > 0.0f, v, w);
+  if (x4.w != 0.0f) {
+    q4 q = q0;
+    const f3 x = integrate_vals(p.xs, q, v, w, p.iI, body_gravity(i, d.gx, d.gy, d.gz, d.gscale),
+                                d.h, d.gyroscopic);
+    p.x4 = make_float4(x.x, x.y, x.z, x4.w);
+    p.q = q;
+  }
+  return p;
+}
+
+__global__ __launch_bounds__(kB) void k6_sub_contacts(xpbd_rigid_desc d, SubBufs b) {
+  __shared__ float psh[kPolyFloats * kB];
+  __shared__ float4 slot[4 * kB];
+  const int k = blockIdx.x * kB + threadIdx.x;
+  const int n = d.n, nj = d.nj;
+  const int f0 = floor_lane0(nj), p0 = pair_lane0(nj, n);
+  const float4* HE = CF4(d.he);
+  if (k < nj) {
+    const int2 ab = ((const int2*)d.j_idx)[k];
+    const Pred pa = predict_body(d, b, ab.x), pb = predict_body(d, b, ab.y);
+    solve_joint_vals(k, ab.x, ab.y, pa.x4, pa.q, pa.iI, pb.x4, pb.q, pb.iI, CF4(d.j_a)[k],
+                     CF4(d.j_b)[k], d.lam_j, d.h, 1, b.DX, b.DR);
+  } else if (k < f0) {
+    return;
+  } else if (k < f0 + 8 * n) {
+    const int t = k - f0, i = t >> 3, c = t & 7;
+    const Pred p = predict_body(d, b, i);
+    if (c == 0) {
+      b.XPR[i] = p.x4;
+      b.QPR[i] = stq(p.q);
+      F4(d.x_prev)[i] = make_float4(p.xs.x, p.xs.y, p.xs.z, 0.0f);
+      F4(d.q_prev)[i] = stq(p.qs);
+      b.DV[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      b.DW[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    solve_floor_corner_vals(i, c, p.x4, p.q, p.xs, p.qs, p.iI, xyz(HE[i]), d.floor_y, d.friction,
+                            d.lam_floor, b.DX, b.DR, slot);
+  } else if (k < p0) {
+    return;
+  } else {  // whole wavefronts from here on
+    const int pid = k - p0;
+    const int live = min(ld_count(d.pair_count), d.cap);
+    PairBodies P;
+    const bool valid = pid < live;
+    if (valid) {
+      const int2 pr = ((const int2*)d.pairs)[pid];
+      const Pred pa = predict_body(d, b, pr.x), pb = predict_body(d, b, pr.y);
+      P.a = pr.x;
+      P.b = pr.y;
+      P.cA = xyz(pa.x4);
+      P.cB = xyz(pb.x4);
+      P.qa = pa.q;
+      P.qb = pb.q;
+      P.imA = pa.x4.w;
+      P.imB = pb.x4.w;
+      P.iIA = pa.iI;
+      P.iIB = pb.iI;
+      P.eA = xyz(HE[pr.x]);
+      P.eB = xyz(HE[pr.y]);
+    }
+    LdsPoly<kB> poly{psh + threadIdx.x};
+    box_manifold_pair<true>(pid, valid, P, d.man_margin, d.cap, d.m_count, (int4*)d.pt_idx,
+                            F4(d.pt_n), F4(d.pt_oi), F4(d.pt_or), d.lam_pt, b.PTC, poly, b.DX,
+                            b.DR);
+  }
+}
+
+// Body i after the last iteration's apply_jacobi and finalize_velocity.
+struct Fin {
+  float4 x4;
+  q4 q;
+  f3 v, w, iI;
+};
+
+__device__ inline Fin finalize_body(const xpbd_rigid_desc& d, const SubBufs& b, int i) {
+  Fin f;
+  f.x4 = b.XPR[i];
+  f.q = ldq(b.QPR[i]);
+  const float4 acc = b.DX[i];
+  const f3 drs = xyz(b.DR[i]);
+  const float4 xp4 = CF4(d.x_prev)[i];
+  const q4 qp = ldq(CF4(d.q_prev)[i]);
+  f.iI = xyz(CF4(d.inv_I)[i]);
+  apply_jacobi_vals(acc, drs, d.relax, f.x4, f.q);
+  finalize_vals(f.x4, f.q, xp4, qp, d.h, d.lin_damp, d.ang_damp, f.v, f.w);
+  return f;
+}
+
+__global__ __launch_bounds__(kB) void k6_sub_velocity(xpbd_rigid_desc d, SubBufs b) {
+  __shared__ float4 slot[4 * kB];
+  const int k = blockIdx.x * kB + threadIdx.x;
+  const int n = d.n;
+  if (k < 8 * n) {
+    const int i = k >> 3, c = k & 7;
+    const Fin f = finalize_body(d, b, i);
+    const float lam = d.lam_floor[8 * (size_t)i + c];
+    if (c == 0) {
+      F4(d.x)[i] = f.x4;
+      F4(d.q)[i] = stq(f.q);
+      F4(d.v)[i] = make_float4(f.v.x, f.v.y, f.v.z, 0.0f);
+      F4(d.w)[i] = make_float4(f.w.x, f.w.y, f.w.z, 0.0f);
+      b.DXn[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      b.DRn[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (i == 0) *b.PTCn = 0;
+    }
+    velocity_floor_corner_vals(i, c, f.x4.w, f.q, f.iI, xyz(CF4(d.he)[i]), f.v, f.w, lam,
+                               d.friction, d.restitution, d.h, b.DV, b.DW, slot);
+    return;
+  }
+  const int p = k - 8 * n, maxp = 8 * d.cap;
+  if (p >= maxp) return;
+  // the point's own data and the live count in one round of loads (p is
+  // inside the point arrays), the bodies' state once the point is live
+  const int cnt = min(ld_count(b.PTC), maxp);
+  const float lam_n = d.lam_pt[p];
+  const int4 id = ((const int4*)d.pt_idx)[p];
+  const float4 sn = CF4(d.pt_n)[p], oi = CF4(d.pt_oi)[p], orf = CF4(d.pt_or)[p];
+  if (p >= cnt || lam_n == 0.0f) return;
+  const Fin fi = finalize_body(d, b, id.y), fj = finalize_body(d, b, id.z);
+  velocity_box_vals(lam_n, id.y, id.z, xyz(sn), xyz(oi), xyz(orf), fi.x4, fi.q, fi.v, fi.w, fi.iI,
+                    fj.x4, fj.q, fj.v, fj.w, fj.iI, d.friction, d.restitution, d.h, b.DV, b.DW);
+}
+
+// apply_velocity + cap of the frame's last substep (its sums in last.DV /
+// last.DW); every other sum buffer and both point counters back to zero.
+__global__ __launch_bounds__(kB) void k6_sub_end(xpbd_rigid_desc d, SubBufs last) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i == 0) {
+    d.pt_count[0] = 0;
+    d.pt_count[1] = 0;
+  }
+  if (i >= d.n) return;
+  apply_velocity_elem(i, CF4(d.x), F4(d.v), F4(d.w), last.DV, last.DW, d.max_speed, d.max_omega,
+                      d.max_speed > 0.0f, d.n);
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4* DVo = (float4*)last.DVin;  // the other parity
+  float4* DWo = (float4*)last.DWin;
+  DVo[i] = z;
+  DWo[i] = z;
+  last.DX[i] = z;
+  last.DR[i] = z;
+  last.DXn[i] = z;
+  last.DRn[i] = z;
+}
+
+// ---------------------------------------------------------------------------
 // Whole frames of a small rigid scene in one workgroup (no reference
 // counterpart as a kernel: it runs solver_6dof.py:428-454 with the LBVH
 // filter and the substep loop of solver_6dof.py:357-426 for `frames`
@@ -937,9 +1267,6 @@ constexpr int kFBMax = 256;
 
 __device__ inline void fb_sync() { __syncthreads(); }
 
-__device__ inline int ld_count(const int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 __global__ __launch_bounds__(kFB) void k6_frame_block(xpbd_rigid_desc d, float4* __restrict__ LO,
                                                      float4* __restrict__ HI, float dt,
@@ -1276,6 +1603,50 @@ int xpbd6_velocity_all(void* stream, const float* X, const float* Q, const float
                      CF4(X), CF4(Q), CF4(IB), CF4(HE), mu_d, restitution, LAMF, h, CF4(V), CF4(W),
                      n, (const int4*)PTI, CF4(PTN), CF4(PTOI), CF4(PTOR), LAMP, PTC, max_points,
                      F4(DV), F4(DW));
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+static int check_sub_desc(const xpbd_rigid_desc* desc, int sub) {
+  if (!desc || !desc->aux || !desc->pt_count || desc->n < 0 || desc->cap < 0 || desc->nj < 0 ||
+      sub < 0)
+    return (int)hipErrorInvalidValue;
+  const long long lanes = (long long)pair_lane0(desc->nj, desc->n) + desc->cap;
+  if (lanes >= (1ll << 31) || 8ll * desc->n + 8ll * desc->cap >= (1ll << 31))
+    return (int)hipErrorInvalidValue;
+  return 0;
+}
+
+int xpbd6_substep_contacts(void* stream, const xpbd_rigid_desc* desc, int sub) {
+  int rc = check_sub_desc(desc, sub);
+  if (rc) return rc;
+  const xpbd_rigid_desc& d = *desc;
+  if (d.n == 0) return 0;
+  const long long lanes = (long long)pair_lane0(d.nj, d.n) + d.cap;
+  hipLaunchKernelGGL(k6_sub_contacts, dim3(nblk(lanes)), dim3(kB), 0, (hipStream_t)stream, d,
+                     sub_bufs(d, sub));
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_substep_velocity(void* stream, const xpbd_rigid_desc* desc, int sub) {
+  int rc = check_sub_desc(desc, sub);
+  if (rc) return rc;
+  const xpbd_rigid_desc& d = *desc;
+  if (d.n == 0) return 0;
+  hipLaunchKernelGGL(k6_sub_velocity, dim3(nblk(8ll * d.n + 8ll * d.cap)), dim3(kB), 0,
+                     (hipStream_t)stream, d, sub_bufs(d, sub));
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd6_substep_end(void* stream, const xpbd_rigid_desc* desc) {
+  int rc = check_sub_desc(desc, 0);
+  if (rc) return rc;
+  const xpbd_rigid_desc& d = *desc;
+  if (d.n == 0 || d.substeps < 1) return d.substeps < 1 ? (int)hipErrorInvalidValue : 0;
+  hipLaunchKernelGGL(k6_sub_end, dim3(nblk(d.n)), dim3(kB), 0, (hipStream_t)stream, d,
+                     sub_bufs(d, d.substeps - 1));
   XPBD_LAUNCH_CHECK();
   return 0;
 }

@@ -219,6 +219,10 @@ class Solver6DOF:
         # default (all pairs up to 4096 bodies, one body per leaf above);
         # xpbd_amd.autotune.tune6 picks it by timing
         self.lbvh_leaf = None
+        # launches per substep of the frame plan: 2 (per-body phases folded
+        # into the contact and velocity launches, xpbd6_substep_*) or 5 (one
+        # launch per phase); the same results up to the order of atomic sums
+        self.substep_launches = 2
         self.cap = 0
         self._cur_margin = 0.0
 
@@ -364,7 +368,8 @@ class Solver6DOF:
             HE=torch.from_numpy(he4).to(dev), DX=z4(), DR=z4(), DV=z4(), DW=z4(),
             LAMF=torch.zeros(max(8 * n, 8), dtype=torch.float32, device=dev),
             PC=torch.zeros(1, dtype=torch.int32, device=dev),
-            PTC=torch.zeros(1, dtype=torch.int32, device=dev))
+            PTC=torch.zeros(2, dtype=torch.int32, device=dev),
+            AUX=torch.zeros((6 * max(n, 1), 4), dtype=torch.float32, device=dev))
         self.n_joints = self._scene.n_joints()
         nj = self.n_joints
         ji = np.zeros((max(nj, 1), 2), np.int32)
@@ -454,7 +459,8 @@ class Solver6DOF:
         return (n, self.cap, self.n_joints, self.substeps, self.iterations, self.gyroscopic,
                 self.dt, self.gravity, self.floor_y, self.friction, self.restitution,
                 self.lin_damp, self.ang_damp, self.max_speed, self.max_omega,
-                self._man_margin, self.jacobi_relax, id(self._buf), self._buf["PAIRS"].data_ptr())
+                self._man_margin, self.jacobi_relax, id(self._buf), self._buf["PAIRS"].data_ptr(),
+                self.substep_launches)
 
     def _drop_plan(self):
         if self._plan is not None:
@@ -488,6 +494,10 @@ class Solver6DOF:
         d.max_speed, d.max_omega, d.man_margin = self.max_speed, self.max_omega, self._man_margin
         if "GS" in b:
             d.gscale = P(b["GS"])
+        if self.substep_launches not in (2, 5):
+            raise ValueError("substep_launches must be 2 or 5")
+        if self.substep_launches == 2:
+            d.aux = P(b["AUX"])
         self._desc = d
         handle = N.c_void_p()
         N.check(N.lib().xpbd_frame6_create(N.ctypes.byref(d), N.ctypes.byref(handle)),

@@ -244,15 +244,20 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
 
 
 def time_frames(step, frames: int, warmup: int) -> float:
-    """ms per frame of ``step()`` over ``frames`` frames after ``warmup``."""
+    """ms per frame of ``step()`` over ``frames`` frames after ``warmup``.
+    A solver with ``settle()`` (frames whose overflow check runs late) is
+    settled inside the timed region, so a frame run again is timed too."""
     import torch
 
+    settle = getattr(getattr(step, "__self__", None), "settle", None) or (lambda: None)
     for _ in range(warmup):
         step()
+    settle()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(frames):
         step()
+    settle()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / frames * 1e3
 

@@ -446,10 +446,34 @@ int xpbd6_frame_block_capacity(void);
 int xpbd6_substep_contacts(void* stream, const xpbd_rigid_desc* desc, int sub);
 int xpbd6_substep_velocity(void* stream, const xpbd_rigid_desc* desc, int sub);
 int xpbd6_substep_end(void* stream, const xpbd_rigid_desc* desc);
+/* substep_end that also reports the frame: when ring (device address of
+ * host-mapped int[2 ring_len]) is set, slot frames[0] % ring_len gets
+ * (pair_count[0], frames[0] + 1) and frames[0] (device int) advances.
+ * All three substep launches write no state when pair_count[0] > cap (the
+ * broad phase overflowed): the frame is skipped, and a caller that grows
+ * the pair buffers runs it again. */
+int xpbd6_substep_end_report(void* stream, const xpbd_rigid_desc* desc, int* ring, int ring_len,
+                             int* frames);
 
 int xpbd_frame6_create(const xpbd_rigid_desc* desc, void** handle);
 int xpbd_frame6_run(void* handle, void* stream, int frames, int use_graph);
 int xpbd_frame6_destroy(void* handle);
+/* The frame's broad phase inside the plan (two-launch substep plans only;
+ * solver_6dof.py:428-454 runs it on the host's schedule and reads the count
+ * back every frame): kind 1 LBVH (xpbd6_broadphase_lbvh, or _leaf with
+ * leaf >= 0), 2 hash grid (rsmax, cell); ws = the broad-phase workspace,
+ * dt = the frame length.  The frame then needs no host round trip: its
+ * status goes to a host-mapped ring that xpbd_frame6_status reads.
+ * kind 0: the caller supplies the pairs again. */
+int xpbd_frame6_set_broadphase(void* handle, int kind, void* ws, int leaf, float rsmax,
+                               float cell, float dt);
+/* Status of the frames run so far (wait != 0: after synchronizing the device):
+ * out[0] frames not yet reported, out[1] the first frame whose pair count
+ * exceeded cap (-1: none), out[2] that count, out[3] frames run from it on,
+ * all of which were skipped (solver_6dof.py:443-453 grows the pair buffers
+ * and emits again; the caller grows them, makes a new plan and runs those
+ * frames again). */
+int xpbd_frame6_status(void* handle, int wait, long long* out);
 
 /* ---- 6-DOF broad phase (kernels_bvh.py, kernels_hashgrid.py) ------------
  * Both write the candidate pairs (i < j, (i, j) ascending) into PAIRS

@@ -134,6 +134,44 @@ def test_broadphase_overflow_regrows():
     assert len(s.pairs()) == s.n_pairs
 
 
+def test_overflowed_frames_run_again_when_read():
+    """The two-launch plan runs its broad phase inside the frame and checks
+    the pair count later: a frame that overflowed the pair buffers, and
+    every frame after it, wrote nothing; the next read of the state grows
+    the buffers and runs those frames again.  Three frames stepped with a
+    tiny capacity, then read, equal three frames with room from the start
+    (pair lists exact, state within 1e-5: atomic sums in another order); a
+    knob changed between frames settles the frames run before it with the
+    old value."""
+    from xpbd_amd import Solver6DOF
+    from xpbd_amd.scenes import box_pile
+
+    a = box_pile(Solver6DOF, 6, 4, 6, substeps=4, device=DEV)
+    b = box_pile(Solver6DOF, 6, 4, 6, substeps=4, device=DEV)
+    for s in (a, b):
+        s.block_frames = False
+        s._flush()
+    a.cap = 8
+    a._alloc_pair_buffers(8)
+    for _ in range(3):
+        a.step()
+        b.step()
+    assert a.cap >= a.n_pairs == b.n_pairs > 8
+    assert (a.pairs() == b.pairs()).all()
+    assert _rel(a.positions(), b.positions()) <= 1e-5
+    # overflow again, then a knob change before any read
+    a.cap = 8
+    a._alloc_pair_buffers(8)
+    a.step()
+    b.step()
+    a.friction = b.friction = 0.3
+    a.step()
+    b.step()
+    assert a.n_pairs == b.n_pairs
+    assert _rel(a.positions(), b.positions()) <= 1e-5
+    assert _rel(a.velocities(), b.velocities()) <= 1e-3
+
+
 def test_morton_codes_decode(oracle):
     """Morton codes decode to the quantised coordinates (10 bits per axis,
     x in the highest bit of each triple)."""

@@ -128,6 +128,9 @@ _SIGS = {
     "xpbd6_substep_contacts": (_I, [_P, _P, _I]),
     "xpbd6_substep_velocity": (_I, [_P, _P, _I]),
     "xpbd6_substep_end": (_I, [_P, _P]),
+    "xpbd6_substep_end_report": (_I, [_P, _P, _P, _I, _P]),
+    "xpbd_frame6_set_broadphase": (_I, [_P, _I, _P, _I, _F, _F, _F]),
+    "xpbd_frame6_status": (_I, [_P, _I, _P]),
     "xpbd_frame6_destroy": (_I, [_P]),
     "xpbd6_body_aabb": (_I, [_P] * 6 + [_F, _F, _P, _P, _I]),
     "xpbd6_lbvh_workspace_bytes": (ctypes.c_longlong, [_I]),

@@ -12,11 +12,27 @@
 
 namespace {
 
+constexpr int kRing = 64;  // frames whose status the host has not read yet, at most
+
 struct Frame6 {
   xpbd_rigid_desc d;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t capture_stream = nullptr;
+  // broad phase inside the frame (two-launch substep only): 0 none (the
+  // caller supplies the pairs), 1 LBVH (leaf < 0: its default), 2 hash grid
+  int bp_kind = 0;
+  void* bp_ws = nullptr;
+  int bp_leaf = -1;
+  float bp_rsmax = 0.0f, bp_cell = 0.0f, bp_dt = 0.0f;
+  // per-frame status: host-mapped ring of (pair count, frame + 1), the
+  // device's frame counter, and what the host has read of it
+  int* ring = nullptr;
+  int* ring_dev = nullptr;
+  int* frames_dev = nullptr;
+  long long launched = 0, checked = 0;
+  long long bad_frame = -1;
+  int bad_count = 0;
 };
 
 // Per substep five launches: apply_velocity of the previous substep fused
@@ -94,12 +110,64 @@ int issue_two_launch(Frame6* f, hipStream_t s) {
     rc = xpbd6_substep_velocity(s, &d, sub);
     if (rc) return rc;
   }
-  return xpbd6_substep_end(s, &d);
+  return xpbd6_substep_end_report(s, &d, f->ring_dev, kRing, f->frames_dev);
+}
+
+bool two_launch(const Frame6* f) { return f->d.aux && f->d.iterations >= 1; }
+
+// The frame's broad phase (kernels only: it records into the graph).
+int issue_broadphase(Frame6* f, hipStream_t s) {
+  const xpbd_rigid_desc& d = f->d;
+  float* X = d.x;
+  int* PAIRS = (int*)d.pairs;
+  int* PC = (int*)d.pair_count;
+  if (f->bp_kind == 1 && f->bp_leaf < 0)
+    return xpbd6_broadphase_lbvh(s, f->bp_ws, X, d.q, d.v, d.w, d.he, d.man_margin, f->bp_dt,
+                                 d.n, d.cap, PAIRS, PC);
+  if (f->bp_kind == 1)
+    return xpbd6_broadphase_lbvh_leaf(s, f->bp_ws, X, d.q, d.v, d.w, d.he, d.man_margin,
+                                      f->bp_dt, d.n, d.cap, PAIRS, PC, f->bp_leaf);
+  return xpbd6_broadphase_hashgrid(s, f->bp_ws, X, d.q, d.v, d.w, d.he, d.man_margin, f->bp_dt,
+                                   f->bp_rsmax, f->bp_cell, d.n, d.cap, PAIRS, PC);
 }
 
 int issue_frame(Frame6* f, hipStream_t s) {
-  if (f->d.aux && f->d.iterations >= 1) return issue_two_launch(f, s);
-  return issue(f, s);
+  if (!two_launch(f)) return issue(f, s);
+  if (f->bp_kind) {
+    int rc = issue_broadphase(f, s);
+    if (rc) return rc;
+  }
+  return issue_two_launch(f, s);
+}
+
+// Reads the status of the frames the device has reported, in order; the
+// first frame whose pair count exceeded cap is kept (it and every later
+// frame wrote nothing: their broad phase saw the same state).
+void scan_status(Frame6* f) {
+  if (!f->ring) {
+    f->checked = f->launched;
+    return;
+  }
+  while (f->checked < f->launched) {
+    const int* slot = f->ring + 2 * (f->checked % kRing);
+    const int mark = __atomic_load_n(slot + 1, __ATOMIC_ACQUIRE);
+    if (mark != (int)((f->checked + 1) & 0x7fffffff)) break;
+    const int count = __atomic_load_n(slot, __ATOMIC_RELAXED);
+    if (count > f->d.cap && f->bad_frame < 0) {
+      f->bad_frame = f->checked;
+      f->bad_count = count;
+    }
+    ++f->checked;
+  }
+}
+
+void free_status(Frame6* f) {
+  if (f->ring) {
+    (void)hipDeviceSynchronize();  // no frame in flight still reports into the ring
+    (void)hipHostFree(f->ring);
+  }
+  if (f->frames_dev) (void)hipFree(f->frames_dev);
+  f->ring = f->ring_dev = f->frames_dev = nullptr;
 }
 
 void drop_graph(Frame6* f) {
@@ -122,14 +190,67 @@ int xpbd_frame6_create(const xpbd_rigid_desc* desc, void** handle) {
   return 0;
 }
 
+int xpbd_frame6_set_broadphase(void* handle, int kind, void* ws, int leaf, float rsmax,
+                               float cell, float dt) {
+  Frame6* f = (Frame6*)handle;
+  if (!f || kind < 0 || kind > 2 || (kind && !ws) || !two_launch(f))
+    return (int)hipErrorInvalidValue;
+  if (f->launched != f->checked) return (int)hipErrorNotReady;
+  drop_graph(f);
+  f->bp_kind = kind;
+  f->bp_ws = ws;
+  f->bp_leaf = leaf;
+  f->bp_rsmax = rsmax;
+  f->bp_cell = cell;
+  f->bp_dt = dt;
+  if (kind && !f->ring) {
+    void* ring = nullptr;
+    XPBD_CHECK(hipHostMalloc(&ring, 2 * kRing * sizeof(int),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+    f->ring = (int*)ring;
+    for (int k = 0; k < 2 * kRing; ++k) f->ring[k] = 0;
+    void* dev = nullptr;
+    hipError_t e = hipHostGetDevicePointer(&dev, ring, 0);
+    if (e == hipSuccess) e = hipMalloc((void**)&f->frames_dev, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(f->frames_dev, 0, sizeof(int));
+    if (e != hipSuccess) {
+      free_status(f);
+      return (int)e;
+    }
+    f->ring_dev = (int*)dev;
+  }
+  return 0;
+}
+
+int xpbd_frame6_status(void* handle, int wait, long long* out) {
+  Frame6* f = (Frame6*)handle;
+  if (!f || !out) return (int)hipErrorInvalidValue;
+  // the frames may have run on any stream (a caller's side stream, say)
+  if (wait) XPBD_CHECK(hipDeviceSynchronize());
+  scan_status(f);
+  if (wait && f->checked != f->launched) return (int)hipErrorUnknown;  // a frame never reported
+  out[0] = f->launched - f->checked;
+  out[1] = f->bad_frame;
+  out[2] = f->bad_count;
+  out[3] = f->bad_frame < 0 ? 0 : f->launched - f->bad_frame;
+  return 0;
+}
+
 int xpbd_frame6_run(void* handle, void* stream, int frames, int use_graph) {
   Frame6* f = (Frame6*)handle;
   if (!f || frames < 0) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
+  if (f->ring && frames > kRing - 1) return (int)hipErrorInvalidValue;
+  if (f->ring && f->launched + frames - f->checked > kRing - 1) {
+    // the ring would overwrite a status the host has not read: wait for it
+    XPBD_CHECK(hipStreamSynchronize(s));
+    scan_status(f);
+  }
   if (!use_graph) {
     for (int k = 0; k < frames; ++k) {
       int rc = issue_frame(f, s);
       if (rc) return rc;
+      ++f->launched;
     }
     return 0;
   }
@@ -151,7 +272,10 @@ int xpbd_frame6_run(void* handle, void* stream, int frames, int use_graph) {
       return (int)e;
     }
   }
-  for (int k = 0; k < frames; ++k) XPBD_CHECK(hipGraphLaunch(f->exec, s));
+  for (int k = 0; k < frames; ++k) {
+    XPBD_CHECK(hipGraphLaunch(f->exec, s));
+    ++f->launched;
+  }
   return 0;
 }
 
@@ -160,6 +284,7 @@ int xpbd_frame6_destroy(void* handle) {
   if (!f) return 0;
   drop_graph(f);
   if (f->capture_stream) (void)hipStreamDestroy(f->capture_stream);
+  free_status(f);
   delete f;
   return 0;
 }

@@ -699,39 +699,50 @@ __global__ __launch_bounds__(kB) void k6_velocity_floor(
   velocity_floor_elem<false>(i, X, Q, IB, HE, mu_d, restitution, LAMF, h, V, W, DV, DW, n);
 }
 
-// kernels_6dof.py:593-659 on values: one manifold point (multiplier lam_n,
-// normal s, body-frame offsets oi / orf) between bodies ci and cj, given
-// their poses (xi4 / xj4 = position, inverse mass), velocities and inverse
-// inertia.
-__device__ inline void velocity_box_vals(float lam_n, int ci, int cj, f3 s, f3 oi, f3 orf,
-                                         float4 xi4, q4 qi, f3 vi, f3 wi, f3 iIi, float4 xj4,
-                                         q4 qj, f3 vj, f3 wj, f3 iIj, float mu_d,
-                                         float restitution, float h, float4* DV, float4* DW) {
+// kernels_6dof.py:593-659 on values: the impulse p of one manifold point
+// (multiplier lam_n, normal s, body-frame offsets oi / orf) between bodies
+// i and j, given their poses (xi4 / xj4 = position, inverse mass),
+// velocities and inverse inertia; false when it applies none.  ri / rj:
+// the point's arms.
+__device__ inline bool velocity_box_impulse(float lam_n, f3 s, f3 oi, f3 orf, float4 xi4, q4 qi,
+                                            f3 vi, f3 wi, f3 iIi, float4 xj4, q4 qj, f3 vj,
+                                            f3 wj, f3 iIj, float mu_d, float restitution,
+                                            float h, f3& p, f3& ri, f3& rj) {
   const f3 p_inc = xyz(xi4) + qrot(qi, oi);
   const f3 p_ref = xyz(xj4) + qrot(qj, orf);
-  const f3 ri = p_inc - xyz(xi4), rj = p_ref - xyz(xj4);
+  ri = p_inc - xyz(xi4);
+  rj = p_ref - xyz(xj4);
   const f3 vci = vi + cross(wi, ri);
   const f3 vcj = vj + cross(wj, rj);
   const f3 impulse = velocity_impulse(vci - vcj, s, lam_n, mu_d, restitution, h);
   const float li = length(impulse);
-  if (li > XPBD_EPS) {
-    const f3 ndir = impulse / li;
-    const float wi_ = gen_inv_mass(xi4.w, qi, iIi, ri, ndir);
-    const float wj_ = gen_inv_mass(xj4.w, qj, iIj, rj, ndir);
-    const float wsum = wi_ + wj_;
-    if (wsum > 0.0f) {
-      const f3 p = impulse / wsum;
-      if (xi4.w > 0.0f) {
-        atomic_add3(&DV[ci], p * xi4.w);
-        atomic_add3(&DW[ci], world_invI_mul(qi, iIi, cross(ri, p)));
-        atomicAdd(&DV[ci].w, 1.0f);
-      }
-      if (xj4.w > 0.0f) {
-        atomic_add3(&DV[cj], (-p) * xj4.w);
-        atomic_add3(&DW[cj], -world_invI_mul(qj, iIj, cross(rj, p)));
-        atomicAdd(&DV[cj].w, 1.0f);
-      }
-    }
+  if (!(li > XPBD_EPS)) return false;
+  const f3 ndir = impulse / li;
+  const float wi_ = gen_inv_mass(xi4.w, qi, iIi, ri, ndir);
+  const float wj_ = gen_inv_mass(xj4.w, qj, iIj, rj, ndir);
+  const float wsum = wi_ + wj_;
+  if (!(wsum > 0.0f)) return false;
+  p = impulse / wsum;
+  return true;
+}
+
+__device__ inline void velocity_box_vals(float lam_n, int ci, int cj, f3 s, f3 oi, f3 orf,
+                                         float4 xi4, q4 qi, f3 vi, f3 wi, f3 iIi, float4 xj4,
+                                         q4 qj, f3 vj, f3 wj, f3 iIj, float mu_d,
+                                         float restitution, float h, float4* DV, float4* DW) {
+  f3 p, ri, rj;
+  if (!velocity_box_impulse(lam_n, s, oi, orf, xi4, qi, vi, wi, iIi, xj4, qj, vj, wj, iIj, mu_d,
+                            restitution, h, p, ri, rj))
+    return;
+  if (xi4.w > 0.0f) {
+    atomic_add3(&DV[ci], p * xi4.w);
+    atomic_add3(&DW[ci], world_invI_mul(qi, iIi, cross(ri, p)));
+    atomicAdd(&DV[ci].w, 1.0f);
+  }
+  if (xj4.w > 0.0f) {
+    atomic_add3(&DV[cj], (-p) * xj4.w);
+    atomic_add3(&DW[cj], -world_invI_mul(qj, iIj, cross(rj, p)));
+    atomicAdd(&DV[cj].w, 1.0f);
   }
 }
 
@@ -1117,9 +1128,12 @@ __global__ __launch_bounds__(kB) void k6_sub_contacts(xpbd_rigid_desc d, SubBufs
   const int n = d.n, nj = d.nj;
   const int f0 = floor_lane0(nj), p0 = pair_lane0(nj, n);
   const float4* HE = CF4(d.he);
+  const int pc = ld_count(d.pair_count);
+  const bool skip = pc > d.cap;  // broad-phase overflow: the frame writes nothing
   if (k < nj) {
     const int2 ab = ((const int2*)d.j_idx)[k];
     const Pred pa = predict_body(d, b, ab.x), pb = predict_body(d, b, ab.y);
+    if (skip) return;
     solve_joint_vals(k, ab.x, ab.y, pa.x4, pa.q, pa.iI, pb.x4, pb.q, pb.iI, CF4(d.j_a)[k],
                      CF4(d.j_b)[k], d.lam_j, d.h, 1, b.DX, b.DR);
   } else if (k < f0) {
@@ -1127,6 +1141,7 @@ __global__ __launch_bounds__(kB) void k6_sub_contacts(xpbd_rigid_desc d, SubBufs
   } else if (k < f0 + 8 * n) {
     const int t = k - f0, i = t >> 3, c = t & 7;
     const Pred p = predict_body(d, b, i);
+    if (skip) return;
     if (c == 0) {
       b.XPR[i] = p.x4;
       b.QPR[i] = stq(p.q);
@@ -1140,8 +1155,9 @@ __global__ __launch_bounds__(kB) void k6_sub_contacts(xpbd_rigid_desc d, SubBufs
   } else if (k < p0) {
     return;
   } else {  // whole wavefronts from here on
+    if (skip) return;  // uniform over the grid: whole wavefronts leave
     const int pid = k - p0;
-    const int live = min(ld_count(d.pair_count), d.cap);
+    const int live = min(pc, d.cap);
     PairBodies P;
     const bool valid = pid < live;
     if (valid) {
@@ -1192,10 +1208,12 @@ __global__ __launch_bounds__(kB) void k6_sub_velocity(xpbd_rigid_desc d, SubBufs
   __shared__ float4 slot[4 * kB];
   const int k = blockIdx.x * kB + threadIdx.x;
   const int n = d.n;
+  const bool skip = ld_count(d.pair_count) > d.cap;  // as the contacts launch
   if (k < 8 * n) {
     const int i = k >> 3, c = k & 7;
     const Fin f = finalize_body(d, b, i);
     const float lam = d.lam_floor[8 * (size_t)i + c];
+    if (skip) return;
     if (c == 0) {
       F4(d.x)[i] = f.x4;
       F4(d.q)[i] = stq(f.q);
@@ -1217,24 +1235,43 @@ __global__ __launch_bounds__(kB) void k6_sub_velocity(xpbd_rigid_desc d, SubBufs
   const float lam_n = d.lam_pt[p];
   const int4 id = ((const int4*)d.pt_idx)[p];
   const float4 sn = CF4(d.pt_n)[p], oi = CF4(d.pt_oi)[p], orf = CF4(d.pt_or)[p];
-  if (p >= cnt || lam_n == 0.0f) return;
+  if (skip || p >= cnt || lam_n == 0.0f) return;
   const Fin fi = finalize_body(d, b, id.y), fj = finalize_body(d, b, id.z);
   velocity_box_vals(lam_n, id.y, id.z, xyz(sn), xyz(oi), xyz(orf), fi.x4, fi.q, fi.v, fi.w, fi.iI,
                     fj.x4, fj.q, fj.v, fj.w, fj.iI, d.friction, d.restitution, d.h, b.DV, b.DW);
 }
 
 // apply_velocity + cap of the frame's last substep (its sums in last.DV /
-// last.DW); every other sum buffer and both point counters back to zero.
-__global__ __launch_bounds__(kB) void k6_sub_end(xpbd_rigid_desc d, SubBufs last) {
+// last.DW) unless the frame was skipped (broad-phase overflow); every sum
+// buffer and both point counters back to zero either way.  With a status
+// ring (host-mapped int[2 * ring_len]) thread 0 also reports the frame:
+// (pair count, frame number + 1) in slot frame % ring_len, the number
+// stored last and with release order, so a host that reads it sees the count.
+__global__ __launch_bounds__(kB) void k6_sub_end(xpbd_rigid_desc d, SubBufs last, int* ring,
+                                                 int ring_len, int* frames) {
   const int i = blockIdx.x * kB + threadIdx.x;
+  const int pc = ld_count(d.pair_count);
   if (i == 0) {
     d.pt_count[0] = 0;
     d.pt_count[1] = 0;
+    if (ring) {
+      const int f = frames[0];
+      frames[0] = f + 1;
+      int* slot = ring + 2 * (f % ring_len);
+      __hip_atomic_store(slot, pc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(slot + 1, (f + 1) & 0x7fffffff, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
   if (i >= d.n) return;
-  apply_velocity_elem(i, CF4(d.x), F4(d.v), F4(d.w), last.DV, last.DW, d.max_speed, d.max_omega,
-                      d.max_speed > 0.0f, d.n);
   const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (pc <= d.cap) {
+    apply_velocity_elem(i, CF4(d.x), F4(d.v), F4(d.w), last.DV, last.DW, d.max_speed,
+                        d.max_omega, d.max_speed > 0.0f, d.n);
+  } else {
+    last.DV[i] = z;
+    last.DW[i] = z;
+  }
   float4* DVo = (float4*)last.DVin;  // the other parity
   float4* DWo = (float4*)last.DWin;
   DVo[i] = z;
@@ -1640,15 +1677,22 @@ int xpbd6_substep_velocity(void* stream, const xpbd_rigid_desc* desc, int sub) {
   return 0;
 }
 
-int xpbd6_substep_end(void* stream, const xpbd_rigid_desc* desc) {
+int xpbd6_substep_end_report(void* stream, const xpbd_rigid_desc* desc, int* ring,
+                             int ring_len, int* frames) {
   int rc = check_sub_desc(desc, 0);
   if (rc) return rc;
   const xpbd_rigid_desc& d = *desc;
-  if (d.n == 0 || d.substeps < 1) return d.substeps < 1 ? (int)hipErrorInvalidValue : 0;
-  hipLaunchKernelGGL(k6_sub_end, dim3(nblk(d.n)), dim3(kB), 0, (hipStream_t)stream, d,
-                     sub_bufs(d, d.substeps - 1));
+  if (d.substeps < 1 || (ring && (ring_len < 1 || !frames))) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k6_sub_end, dim3(nblk(d.n > 0 ? d.n : 1)), dim3(kB), 0,
+                     (hipStream_t)stream, d, sub_bufs(d, d.substeps - 1), ring, ring_len,
+                     frames);
   XPBD_LAUNCH_CHECK();
   return 0;
+}
+
+int xpbd6_substep_end(void* stream, const xpbd_rigid_desc* desc) {
+  if (desc && desc->n == 0) return 0;
+  return xpbd6_substep_end_report(stream, desc, nullptr, 0, nullptr);
 }
 
 int xpbd6_frame_block_capacity(void) { return kFBMax; }

@@ -205,7 +205,8 @@ class Solver6DOF:
         self._scene = _RigidScene()
         self._handles: list = []
         self._dirty = True
-        self._buf = None
+        self._dev = None                 # device buffers (read them through _buf)
+        self._bp_in_plan = False         # the plan runs the broad phase itself
         self._plan = None
         self._plan_sig = None
         self._graph = None
@@ -386,7 +387,7 @@ class Solver6DOF:
             if len(self._gscale) != n:
                 raise ValueError("gravity scale length != number of bodies")
             b["GS"] = torch.from_numpy(self._gscale).to(dev)
-        self._buf = b
+        self._dev = b
         self.cap = max(64, n * 4)
         self._alloc_pair_buffers(self.cap)
         # broad-phase workspace and hash-grid sizing (solver_6dof.py:330-338)
@@ -402,7 +403,7 @@ class Solver6DOF:
         import torch
 
         dev = self.device
-        b = self._buf
+        b = self._dev
         b["PAIRS"] = torch.zeros((cap, 2), dtype=torch.int32, device=dev)
         b["MC"] = torch.zeros(cap, dtype=torch.int32, device=dev)
         b["PTI"] = torch.zeros((8 * cap, 4), dtype=torch.int32, device=dev)
@@ -439,7 +440,7 @@ class Solver6DOF:
     def _step_block(self, n) -> None:
         import torch
 
-        b = self._buf
+        b = self._dev
         need = max(64, n * (n - 1) // 2)
         if self.cap < need:
             self.cap = need
@@ -459,21 +460,86 @@ class Solver6DOF:
         return (n, self.cap, self.n_joints, self.substeps, self.iterations, self.gyroscopic,
                 self.dt, self.gravity, self.floor_y, self.friction, self.restitution,
                 self.lin_damp, self.ang_damp, self.max_speed, self.max_omega,
-                self._man_margin, self.jacobi_relax, id(self._buf), self._buf["PAIRS"].data_ptr(),
-                self.substep_launches)
+                self._man_margin, self.jacobi_relax, id(self._dev), self._dev["PAIRS"].data_ptr(),
+                self.substep_launches, self.broadphase, self.lbvh_leaf)
 
     def _drop_plan(self):
         if self._plan is not None:
+            self._settle_frames()
             N.lib().xpbd_frame6_destroy(self._plan)
         self._plan = None
         self._plan_sig = None
+        self._bp_in_plan = False
+
+    # ---- broad phase inside the frame plan -----------------------------------
+    def _plan_runs_broadphase(self) -> bool:
+        """LBVH / hash-grid frames of the two-launch plan run their broad
+        phase inside the plan: no host round trip per frame."""
+        return (self.broadphase in ("lbvh", "hashgrid") and self.substep_launches == 2
+                and self.iterations >= 1)
+
+    def _status(self, wait: bool):
+        out = (N.ctypes.c_longlong * 4)()
+        N.check(N.lib().xpbd_frame6_status(self._plan, int(wait), out),
+                "xpbd_frame6_status")
+        return [int(v) for v in out]
+
+    def _settle_frames(self, wait: bool = True) -> None:
+        """Reads the status of the frames the plan has run (all of them when
+        ``wait``).  A frame whose broad phase found more pairs than the pair
+        buffers hold wrote nothing, and neither did any frame after it (the
+        state did not change, so neither did their pairs): grow the buffers
+        as solver_6dof.py:443-453 does and run those frames again, each
+        checked before the next."""
+        if self._plan is None or not self._bp_in_plan:
+            return
+        _, bad, count, redo = self._status(wait)
+        if bad < 0:
+            return
+        self._status(True)                 # every skipped frame has ended
+        plan = self._plan
+        self._plan = None                  # its frames are accounted for here
+        N.lib().xpbd_frame6_destroy(plan)
+        self._plan_sig = None
+        self._bp_in_plan = False
+        n = self.num_bodies
+        k = 0
+        while k < redo:
+            if count > self.cap:
+                self.cap = int(count * 1.5)
+                self._alloc_pair_buffers(self.cap)
+            plan = self._get_plan(n)
+            N.check(N.lib().xpbd_frame6_run(plan, N.current_stream(), 1, int(self.use_graph)),
+                    "xpbd_frame6_run")
+            _, bad, count, _ = self._status(True)
+            if bad < 0:
+                k += 1
+            else:                          # grew again: drop the plan, same frame again
+                p = self._plan
+                self._plan = None
+                N.lib().xpbd_frame6_destroy(p)
+                self._plan_sig = None
+                self._bp_in_plan = False
+
+    def settle(self) -> None:
+        """Waits for the frames run so far and makes sure every one of them
+        ran (see _settle_frames); reads of the device state do this first."""
+        self._settle_frames(True)
+
+    @property
+    def _buf(self):
+        """The device buffers, after every frame run so far has ended and
+        been checked (None before the first flush)."""
+        if self.__dict__.get("_plan") is not None and self.__dict__.get("_bp_in_plan"):
+            self._settle_frames(True)
+        return self.__dict__.get("_dev")
 
     def _get_plan(self, n):
         sig = self._signature(n)
         if self._plan is not None and self._plan_sig == sig and self._graph is not False:
             return self._plan
         self._drop_plan()
-        b = self._buf
+        b = self._dev
         P = N.ptr
         d = N.RigidDesc()
         d.x, d.q, d.v, d.w = P(b["X"]), P(b["Q"]), P(b["V"]), P(b["W"])
@@ -505,7 +571,22 @@ class Solver6DOF:
         self._plan = handle
         self._plan_sig = sig
         self._graph = True
+        if self._plan_runs_broadphase() and not self._use_block(n):
+            kind = 1 if self.broadphase == "lbvh" else 2
+            leaf = -1 if self.lbvh_leaf is None else int(self.lbvh_leaf)
+            N.check(N.lib().xpbd_frame6_set_broadphase(handle, kind, P(b["WS"]), leaf,
+                                                        self._hg_rsmax, self._hg_cell, self.dt),
+                    "xpbd_frame6_set_broadphase")
+            self._bp_in_plan = True
         return handle
+
+    # knobs the frame plan is built from: a change first settles the frames
+    # run with the old values (a skipped frame runs again with them)
+    _PLAN_KNOBS = frozenset((
+        "dt", "substeps", "iterations", "gravity", "floor_y", "friction", "restitution",
+        "lin_damp", "ang_damp", "gyroscopic", "jacobi_relax", "max_speed", "max_omega",
+        "broadphase", "lbvh_leaf", "substep_launches", "use_graph", "block_frames",
+        "_man_margin", "_gscale"))
 
     def __setattr__(self, name, value):
         # the reference drops its captured graph with `s._graph = None`
@@ -513,11 +594,13 @@ class Solver6DOF:
         if name == "_graph" and value is None and getattr(self, "_plan", None) is not None:
             object.__setattr__(self, "_graph", False)
             return
+        if name in self._PLAN_KNOBS and self.__dict__.get("_bp_in_plan"):
+            self._settle_frames(True)
         object.__setattr__(self, name, value)
 
     # ---- broad phase (solver_6dof.py:232-258, 428-479) -------------------------
     def _run_broadphase(self, n, cap):
-        b = self._buf
+        b = self._dev
         P = N.ptr
         lib = N.lib()
         s = N.current_stream()
@@ -555,9 +638,16 @@ class Solver6DOF:
             return
         import torch
 
-        b = self._buf
+        b = self._dev
         if self._use_block(n):
             self._step_block(n)
+            return
+        if self._plan_runs_broadphase():
+            self._settle_frames(False)     # only what the device has reported
+            plan = self._get_plan(n)
+            N.check(N.lib().xpbd_frame6_run(plan, N.current_stream(), 1, int(self.use_graph)),
+                    "xpbd_frame6_run")
+            self._n_pairs = None
             return
         if self.broadphase in ("lbvh", "hashgrid"):
             cnt = self._run_broadphase(n, self.cap)

@@ -177,7 +177,7 @@ __device__ inline void box_manifold_pair(
     int pid, bool valid, const PairBodies& P, float margin, int cap, int* __restrict__ MC,
     int4* __restrict__ PTI, float4* __restrict__ PTN, float4* __restrict__ PTOI,
     float4* __restrict__ PTOR, float* __restrict__ LAMP, int* __restrict__ PTC, Poly& poly,
-    float4* DX, float4* DR) {
+    float4* DX, float4* DR, int* __restrict__ PF) {
   const int lane = threadIdx.x & 63;
   int nc = 0, inc_b = 0, ref_b = 0;
   f3 ref_face_n = mk3(0.f, 0.f, 0.f);
@@ -309,6 +309,10 @@ __device__ inline void box_manifold_pair(
   if (lane == 63 && total > 0) base = atomicAdd(PTC, total);
   base = __shfl(base, 63, 64);
   const int first = base + incl - nc;
+  // SOLVE: the pair's corrections of each body summed here and added once
+  // (one chain of atomics per body and pair, not per point)
+  f3 lin_i = mk3(0.f, 0.f, 0.f), ang_i = lin_i, lin_r = lin_i, ang_r = lin_i;
+  float cnt_i = 0.0f, cnt_r = 0.0f;
   for (int k = 0, c = 0; k < plen && c < nc; ++k) {
     const f3 p_inc = poly.get(cur, k);
     const float d_signed = dot(p_inc - ref_face_c, ref_face_n);
@@ -336,8 +340,17 @@ __device__ inline void box_manifold_pair(
         if (!(wsum <= 0.0f)) {
           const float dlam = -C / wsum;
           lam = 0.0f + dlam;
-          accumulate_correction(DX, DR, inc_b, ref_b, im_inc, im_ref, q_inc_s, q_ref_s, iI_inc,
-                                iI_ref, r_inc, r_ref, s * dlam);
+          const f3 p = s * dlam;  // accumulate_correction's terms
+          if (im_inc > 0.0f) {
+            lin_i = lin_i + p * im_inc;
+            ang_i = ang_i + world_invI_mul(q_inc_s, iI_inc, cross(r_inc, p));
+            cnt_i += 1.0f;
+          }
+          if (im_ref > 0.0f) {
+            lin_r = lin_r + (-p) * im_ref;
+            ang_r = ang_r + -world_invI_mul(q_ref_s, iI_ref, cross(r_ref, p));
+            cnt_r += 1.0f;
+          }
         }
       }
     }
@@ -345,6 +358,19 @@ __device__ inline void box_manifold_pair(
     ++c;
   }
   if (pid < cap) MC[pid] = nc;
+  if (SOLVE) {
+    if (pid < cap) PF[pid] = first;
+    if (cnt_i > 0.0f) {
+      atomic_add3(&DX[inc_b], lin_i);
+      atomic_add3(&DR[inc_b], ang_i);
+      atomicAdd(&DX[inc_b].w, cnt_i);
+    }
+    if (cnt_r > 0.0f) {
+      atomic_add3(&DX[ref_b], lin_r);
+      atomic_add3(&DR[ref_b], ang_r);
+      atomicAdd(&DX[ref_b].w, cnt_r);
+    }
+  }
 }
 
 template <class Poly>
@@ -368,7 +394,7 @@ __device__ inline void box_manifold_elem(
     P.qb = ldq(Q[P.b]);
   }
   box_manifold_pair<false>(pid, valid, P, margin, cap, MC, PTI, PTN, PTOI, PTOR, LAMP, PTC, poly,
-                           nullptr, nullptr);
+                           nullptr, nullptr, nullptr);
 }
 
 __global__ __launch_bounds__(kB) void k6_box_manifold(
@@ -1179,7 +1205,7 @@ __global__ __launch_bounds__(kB) void k6_sub_contacts(xpbd_rigid_desc d, SubBufs
     LdsPoly<kB> poly{psh + threadIdx.x};
     box_manifold_pair<true>(pid, valid, P, d.man_margin, d.cap, d.m_count, (int4*)d.pt_idx,
                             F4(d.pt_n), F4(d.pt_oi), F4(d.pt_or), d.lam_pt, b.PTC, poly, b.DX,
-                            b.DR);
+                            b.DR, d.pt_first);
   }
 }
 
@@ -1204,12 +1230,67 @@ __device__ inline Fin finalize_body(const xpbd_rigid_desc& d, const SubBufs& b, 
   return f;
 }
 
+// First lane of the box points in the velocity launch (and of the pairs in
+// the contacts launch): a workgroup boundary, so the point workgroups hold
+// nothing else and every lane of them reaches their barrier.
+__host__ __device__ inline int block_lane0(int lanes) { return (lanes + kB - 1) / kB * kB; }
+
+// One point's contributions to the two bodies of its pair (the terms
+// accumulate_correction / velocity_box_elem add atomically).
+struct PointSums {
+  f3 li, ai, lj, aj;  // linear / angular terms on body i and body j
+  float ni, nj;       // 1 when the point adds to that body
+};
+
+// Block-wide: the contributions of consecutive lanes with the same key (the
+// points of one pair, at most 8, in point order) are summed in the first
+// lane of each run and added once to its bodies bi / bj (A: sum of linear
+// terms and count, B: sum of angular terms).  One chain of atomics per body
+// and pair instead of one per point.  key < 0: no contribution.  Every lane
+// of the block calls it.
+__device__ inline void add_pair_runs(int key, const PointSums& c, int bi, int bj, float4* A,
+                                     float4* B, float4* sh, int* skey) {
+  const int t = threadIdx.x;
+  sh[4 * t + 0] = make_float4(c.li.x, c.li.y, c.li.z, c.ni);
+  sh[4 * t + 1] = make_float4(c.ai.x, c.ai.y, c.ai.z, 0.0f);
+  sh[4 * t + 2] = make_float4(c.lj.x, c.lj.y, c.lj.z, c.nj);
+  sh[4 * t + 3] = make_float4(c.aj.x, c.aj.y, c.aj.z, 0.0f);
+  skey[t] = key;
+  __syncthreads();
+  if (key < 0 || (t > 0 && skey[t - 1] == key)) return;
+  float4 si = make_float4(0.f, 0.f, 0.f, 0.f), ri = si, sj = si, rj = si;
+  for (int u = t; u < kB && u < t + 8 && skey[u] == key; ++u) {
+    const float4 e0 = sh[4 * u + 0], e1 = sh[4 * u + 1], e2 = sh[4 * u + 2], e3 = sh[4 * u + 3];
+    if (e0.w != 0.0f) {
+      si.x += e0.x; si.y += e0.y; si.z += e0.z; si.w += e0.w;
+      ri.x += e1.x; ri.y += e1.y; ri.z += e1.z;
+    }
+    if (e2.w != 0.0f) {
+      sj.x += e2.x; sj.y += e2.y; sj.z += e2.z; sj.w += e2.w;
+      rj.x += e3.x; rj.y += e3.y; rj.z += e3.z;
+    }
+  }
+  if (si.w > 0.0f) {
+    atomic_add3(&A[bi], xyz(si));
+    atomic_add3(&B[bi], xyz(ri));
+    atomicAdd(&A[bi].w, si.w);
+  }
+  if (sj.w > 0.0f) {
+    atomic_add3(&A[bj], xyz(sj));
+    atomic_add3(&B[bj], xyz(rj));
+    atomicAdd(&A[bj].w, sj.w);
+  }
+}
+
 __global__ __launch_bounds__(kB) void k6_sub_velocity(xpbd_rigid_desc d, SubBufs b) {
   __shared__ float4 slot[4 * kB];
+  __shared__ int skey[kB];
   const int k = blockIdx.x * kB + threadIdx.x;
   const int n = d.n;
   const bool skip = ld_count(d.pair_count) > d.cap;  // as the contacts launch
-  if (k < 8 * n) {
+  const int pv0 = block_lane0(8 * n);
+  if (k < pv0) {
+    if (k >= 8 * n) return;
     const int i = k >> 3, c = k & 7;
     const Fin f = finalize_body(d, b, i);
     const float lam = d.lam_floor[8 * (size_t)i + c];
@@ -1227,18 +1308,43 @@ __global__ __launch_bounds__(kB) void k6_sub_velocity(xpbd_rigid_desc d, SubBufs
                                d.friction, d.restitution, d.h, b.DV, b.DW, slot);
     return;
   }
-  const int p = k - 8 * n, maxp = 8 * d.cap;
-  if (p >= maxp) return;
-  // the point's own data and the live count in one round of loads (p is
-  // inside the point arrays), the bodies' state once the point is live
+  // box points, one per lane; whole workgroups from here on (the block
+  // barrier of add_pair_runs): a workgroup leaves together or not at all
+  const int p = k - pv0, maxp = 8 * d.cap;
   const int cnt = min(ld_count(b.PTC), maxp);
-  const float lam_n = d.lam_pt[p];
-  const int4 id = ((const int4*)d.pt_idx)[p];
-  const float4 sn = CF4(d.pt_n)[p], oi = CF4(d.pt_oi)[p], orf = CF4(d.pt_or)[p];
-  if (skip || p >= cnt || lam_n == 0.0f) return;
-  const Fin fi = finalize_body(d, b, id.y), fj = finalize_body(d, b, id.z);
-  velocity_box_vals(lam_n, id.y, id.z, xyz(sn), xyz(oi), xyz(orf), fi.x4, fi.q, fi.v, fi.w, fi.iI,
-                    fj.x4, fj.q, fj.v, fj.w, fj.iI, d.friction, d.restitution, d.h, b.DV, b.DW);
+  if (skip || (p - (int)threadIdx.x) >= cnt) return;  // uniform per workgroup
+  float lam_n = 0.0f;
+  int4 id = make_int4(-1, 0, 0, 0);
+  float4 sn = make_float4(0.f, 0.f, 0.f, 0.f), oi = sn, orf = sn;
+  if (p < cnt) {
+    lam_n = d.lam_pt[p];
+    id = ((const int4*)d.pt_idx)[p];
+    sn = CF4(d.pt_n)[p];
+    oi = CF4(d.pt_oi)[p];
+    orf = CF4(d.pt_or)[p];
+  }
+  PointSums c;
+  c.li = c.ai = c.lj = c.aj = mk3(0.f, 0.f, 0.f);
+  c.ni = c.nj = 0.0f;
+  if (p < cnt && lam_n != 0.0f) {  // velocity_box_elem
+    const Fin fi = finalize_body(d, b, id.y), fj = finalize_body(d, b, id.z);
+    f3 pp, ri, rj;
+    if (velocity_box_impulse(lam_n, xyz(sn), xyz(oi), xyz(orf), fi.x4, fi.q, fi.v, fi.w, fi.iI,
+                             fj.x4, fj.q, fj.v, fj.w, fj.iI, d.friction, d.restitution, d.h, pp,
+                             ri, rj)) {
+      if (fi.x4.w > 0.0f) {
+        c.li = pp * fi.x4.w;
+        c.ai = world_invI_mul(fi.q, fi.iI, cross(ri, pp));
+        c.ni = 1.0f;
+      }
+      if (fj.x4.w > 0.0f) {
+        c.lj = (-pp) * fj.x4.w;
+        c.aj = -world_invI_mul(fj.q, fj.iI, cross(rj, pp));
+        c.nj = 1.0f;
+      }
+    }
+  }
+  add_pair_runs(p < cnt ? id.x : -1, c, id.y, id.z, b.DV, b.DW, slot, skey);
 }
 
 // apply_velocity + cap of the frame's last substep (its sums in last.DV /
@@ -1659,6 +1765,7 @@ int xpbd6_substep_contacts(void* stream, const xpbd_rigid_desc* desc, int sub) {
   if (rc) return rc;
   const xpbd_rigid_desc& d = *desc;
   if (d.n == 0) return 0;
+  if (!d.pt_first) return (int)hipErrorInvalidValue;
   const long long lanes = (long long)pair_lane0(d.nj, d.n) + d.cap;
   hipLaunchKernelGGL(k6_sub_contacts, dim3(nblk(lanes)), dim3(kB), 0, (hipStream_t)stream, d,
                      sub_bufs(d, sub));
@@ -1671,7 +1778,8 @@ int xpbd6_substep_velocity(void* stream, const xpbd_rigid_desc* desc, int sub) {
   if (rc) return rc;
   const xpbd_rigid_desc& d = *desc;
   if (d.n == 0) return 0;
-  hipLaunchKernelGGL(k6_sub_velocity, dim3(nblk(8ll * d.n + 8ll * d.cap)), dim3(kB), 0,
+  hipLaunchKernelGGL(k6_sub_velocity, dim3(nblk((long long)block_lane0(8 * d.n) + 8ll * d.cap)),
+                     dim3(kB), 0,
                      (hipStream_t)stream, d, sub_bufs(d, sub));
   XPBD_LAUNCH_CHECK();
   return 0;

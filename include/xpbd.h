@@ -406,7 +406,6 @@ typedef struct xpbd_rigid_desc {
   float max_speed, max_omega, man_margin;
   const float* gscale;                     /* float[n] gravity factor, or NULL */
   float* aux;  /* float4[6n] for the two-launch substep (zero on entry), or NULL */
-  int* pt_first;  /* int[cap] first point of each pair (two-launch substep) */
 } xpbd_rigid_desc;
 
 /* Solver6DOF._record (solver_6dof.py:357-426) as a plan; run() replays one

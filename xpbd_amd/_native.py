@@ -59,7 +59,7 @@ class RigidDesc(ctypes.Structure):
         + [(k, c_float) for k in ("h", "gx", "gy", "gz", "floor_y", "friction", "restitution",
                                   "lin_damp", "ang_damp", "relax", "max_speed", "max_omega",
                                   "man_margin")]
-        + [("gscale", c_void_p), ("aux", c_void_p), ("pt_first", c_void_p)]
+        + [("gscale", c_void_p), ("aux", c_void_p)]
     )
 
 

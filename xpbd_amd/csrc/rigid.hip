@@ -168,22 +168,17 @@ struct PairBodies {
   float imA, imB;
 };
 
-// SOLVE: each emitted point is also solved at once, as solve_box_elem does
-// for it in the first iteration (the same operations on the same values:
-// the point's body-frame offsets taken back to the world through the same
-// poses), adding to DX / DR and writing its multiplier.
-template <bool SOLVE, class Poly>
+// wbase / wtotal: the wavefront's points are [wbase, wbase + wtotal) (the
+// slots from 8 cap on are not written).
+template <class Poly>
 __device__ inline void box_manifold_pair(
     int pid, bool valid, const PairBodies& P, float margin, int cap, int* __restrict__ MC,
     int4* __restrict__ PTI, float4* __restrict__ PTN, float4* __restrict__ PTOI,
     float4* __restrict__ PTOR, float* __restrict__ LAMP, int* __restrict__ PTC, Poly& poly,
-    float4* DX, float4* DR, int* __restrict__ PF) {
+    int& wbase, int& wtotal) {
   const int lane = threadIdx.x & 63;
   int nc = 0, inc_b = 0, ref_b = 0;
   f3 ref_face_n = mk3(0.f, 0.f, 0.f);
-  q4 q_inc_s{0.f, 0.f, 0.f, 1.f}, q_ref_s = q_inc_s;  // for SOLVE
-  float im_inc = 0.0f, im_ref = 0.0f;
-  f3 iI_inc = ref_face_n, iI_ref = ref_face_n;
   // the surviving polygon and the frames its points are expressed in
   int cur = 0, plen = 0;
   f3 ref_face_c = mk3(0.f, 0.f, 0.f), c_inc = ref_face_c, c_ref = ref_face_c;
@@ -227,14 +222,6 @@ __device__ inline void box_manifold_pair(
       c_inc = sel3(ra, cB, cA);
       const f3 e_ref = sel3(ra, eA, eB), e_inc = sel3(ra, eB, eA);
       const q4 q_ref = selq(ra, qa, qb), q_inc = selq(ra, qb, qa);
-      if (SOLVE) {
-        q_inc_s = q_inc;
-        q_ref_s = q_ref;
-        im_inc = ra ? P.imB : P.imA;
-        im_ref = ra ? P.imA : P.imB;
-        iI_inc = sel3(ra, P.iIB, P.iIA);
-        iI_ref = sel3(ra, P.iIA, P.iIB);
-      }
       ref_face_n = ra ? -nBA : nBA;
       const m3 R_ref = qmat(q_ref), R_inc = qmat(q_inc);
       const int ref_axis = best_k;
@@ -309,10 +296,6 @@ __device__ inline void box_manifold_pair(
   if (lane == 63 && total > 0) base = atomicAdd(PTC, total);
   base = __shfl(base, 63, 64);
   const int first = base + incl - nc;
-  // SOLVE: the pair's corrections of each body summed here and added once
-  // (one chain of atomics per body and pair, not per point)
-  f3 lin_i = mk3(0.f, 0.f, 0.f), ang_i = lin_i, lin_r = lin_i, ang_r = lin_i;
-  float cnt_i = 0.0f, cnt_r = 0.0f;
   for (int k = 0, c = 0; k < plen && c < nc; ++k) {
     const f3 p_inc = poly.get(cur, k);
     const float d_signed = dot(p_inc - ref_face_c, ref_face_n);
@@ -327,50 +310,12 @@ __device__ inline void box_manifold_pair(
     PTN[q] = make_float4(ref_face_n.x, ref_face_n.y, ref_face_n.z, 0.0f);
     PTOI[q] = make_float4(oi.x, oi.y, oi.z, 0.0f);
     PTOR[q] = make_float4(orf.x, orf.y, orf.z, 0.0f);
-    float lam = 0.0f;
-    if (SOLVE) {  // solve_box_elem on this point's stored values
-      const f3 s = ref_face_n;
-      const f3 pi = c_inc + qrot(q_inc_s, oi);
-      const f3 pr = c_ref + qrot(q_ref_s, orf);
-      const float C = dot(pi - pr, s);
-      if (!(C >= 0.0f)) {
-        const f3 r_inc = pi - c_inc, r_ref = pr - c_ref;
-        const float wsum = gen_inv_mass(im_inc, q_inc_s, iI_inc, r_inc, s) +
-                           gen_inv_mass(im_ref, q_ref_s, iI_ref, r_ref, s);
-        if (!(wsum <= 0.0f)) {
-          const float dlam = -C / wsum;
-          lam = 0.0f + dlam;
-          const f3 p = s * dlam;  // accumulate_correction's terms
-          if (im_inc > 0.0f) {
-            lin_i = lin_i + p * im_inc;
-            ang_i = ang_i + world_invI_mul(q_inc_s, iI_inc, cross(r_inc, p));
-            cnt_i += 1.0f;
-          }
-          if (im_ref > 0.0f) {
-            lin_r = lin_r + (-p) * im_ref;
-            ang_r = ang_r + -world_invI_mul(q_ref_s, iI_ref, cross(r_ref, p));
-            cnt_r += 1.0f;
-          }
-        }
-      }
-    }
-    LAMP[q] = lam;
+    LAMP[q] = 0.0f;
     ++c;
   }
   if (pid < cap) MC[pid] = nc;
-  if (SOLVE) {
-    if (pid < cap) PF[pid] = first;
-    if (cnt_i > 0.0f) {
-      atomic_add3(&DX[inc_b], lin_i);
-      atomic_add3(&DR[inc_b], ang_i);
-      atomicAdd(&DX[inc_b].w, cnt_i);
-    }
-    if (cnt_r > 0.0f) {
-      atomic_add3(&DX[ref_b], lin_r);
-      atomic_add3(&DR[ref_b], ang_r);
-      atomicAdd(&DX[ref_b].w, cnt_r);
-    }
-  }
+  wbase = base;
+  wtotal = total;
 }
 
 template <class Poly>
@@ -393,8 +338,9 @@ __device__ inline void box_manifold_elem(
     P.qa = ldq(Q[P.a]);
     P.qb = ldq(Q[P.b]);
   }
-  box_manifold_pair<false>(pid, valid, P, margin, cap, MC, PTI, PTN, PTOI, PTOR, LAMP, PTC, poly,
-                           nullptr, nullptr, nullptr);
+  int wbase, wtotal;
+  box_manifold_pair(pid, valid, P, margin, cap, MC, PTI, PTN, PTOI, PTOR, LAMP, PTC, poly, wbase,
+                    wtotal);
 }
 
 __global__ __launch_bounds__(kB) void k6_box_manifold(
@@ -1106,13 +1052,12 @@ __global__ __launch_bounds__(kB) void k6_velocity_all(
 //     pass.  Stores x, q, v, w.
 //   end of frame: apply_velocity + cap of the last substep.
 // Buffer parities: rigid_sub.h.  Lanes: contacts [0, nj) joints, [f0, f0 +
-// 8 n) floor corners, [p0, p0 + cap) pairs (p0 a multiple of 64: the
-// manifold compacts its points per wavefront); velocity [0, 8 n) floor
-// corners, [8 n, 8 n + 8 cap) points.
+// 8 n) floor corners, [p0, p0 + cap) pairs; velocity [0, 8 n) floor
+// corners, [pv0, pv0 + 8 cap) points; p0 and pv0 start a workgroup, so the
+// pair and point workgroups hold nothing else (they meet at block barriers).
+// A pair workgroup runs the manifold of its pairs, one per lane, then the
+// first Jacobi iteration of the points they emitted, one per lane.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline int pair_lane0(int nj, int n) {
-  return floor_lane0(nj) + ((8 * n + 63) & ~63);
-}
 
 // Body i at the end of the contacts launch's per-body phase: predicted pose
 // (x4 = position, inverse mass; q), substep-start pose, inverse inertia.
@@ -1145,89 +1090,6 @@ __device__ inline Pred predict_body(const xpbd_rigid_desc& d, const SubBufs& b, 
     p.q = q;
   }
   return p;
-}
-
-__global__ __launch_bounds__(kB) void k6_sub_contacts(xpbd_rigid_desc d, SubBufs b) {
-  __shared__ float psh[kPolyFloats * kB];
-  __shared__ float4 slot[4 * kB];
-  const int k = blockIdx.x * kB + threadIdx.x;
-  const int n = d.n, nj = d.nj;
-  const int f0 = floor_lane0(nj), p0 = pair_lane0(nj, n);
-  const float4* HE = CF4(d.he);
-  const int pc = ld_count(d.pair_count);
-  const bool skip = pc > d.cap;  // broad-phase overflow: the frame writes nothing
-  if (k < nj) {
-    const int2 ab = ((const int2*)d.j_idx)[k];
-    const Pred pa = predict_body(d, b, ab.x), pb = predict_body(d, b, ab.y);
-    if (skip) return;
-    solve_joint_vals(k, ab.x, ab.y, pa.x4, pa.q, pa.iI, pb.x4, pb.q, pb.iI, CF4(d.j_a)[k],
-                     CF4(d.j_b)[k], d.lam_j, d.h, 1, b.DX, b.DR);
-  } else if (k < f0) {
-    return;
-  } else if (k < f0 + 8 * n) {
-    const int t = k - f0, i = t >> 3, c = t & 7;
-    const Pred p = predict_body(d, b, i);
-    if (skip) return;
-    if (c == 0) {
-      b.XPR[i] = p.x4;
-      b.QPR[i] = stq(p.q);
-      F4(d.x_prev)[i] = make_float4(p.xs.x, p.xs.y, p.xs.z, 0.0f);
-      F4(d.q_prev)[i] = stq(p.qs);
-      b.DV[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-      b.DW[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    solve_floor_corner_vals(i, c, p.x4, p.q, p.xs, p.qs, p.iI, xyz(HE[i]), d.floor_y, d.friction,
-                            d.lam_floor, b.DX, b.DR, slot);
-  } else if (k < p0) {
-    return;
-  } else {  // whole wavefronts from here on
-    if (skip) return;  // uniform over the grid: whole wavefronts leave
-    const int pid = k - p0;
-    const int live = min(pc, d.cap);
-    PairBodies P;
-    const bool valid = pid < live;
-    if (valid) {
-      const int2 pr = ((const int2*)d.pairs)[pid];
-      const Pred pa = predict_body(d, b, pr.x), pb = predict_body(d, b, pr.y);
-      P.a = pr.x;
-      P.b = pr.y;
-      P.cA = xyz(pa.x4);
-      P.cB = xyz(pb.x4);
-      P.qa = pa.q;
-      P.qb = pb.q;
-      P.imA = pa.x4.w;
-      P.imB = pb.x4.w;
-      P.iIA = pa.iI;
-      P.iIB = pb.iI;
-      P.eA = xyz(HE[pr.x]);
-      P.eB = xyz(HE[pr.y]);
-    }
-    LdsPoly<kB> poly{psh + threadIdx.x};
-    box_manifold_pair<true>(pid, valid, P, d.man_margin, d.cap, d.m_count, (int4*)d.pt_idx,
-                            F4(d.pt_n), F4(d.pt_oi), F4(d.pt_or), d.lam_pt, b.PTC, poly, b.DX,
-                            b.DR, d.pt_first);
-  }
-}
-
-// Body i after the last iteration's apply_jacobi and finalize_velocity.
-struct Fin {
-  float4 x4;
-  q4 q;
-  f3 v, w, iI;
-};
-
-__device__ inline Fin finalize_body(const xpbd_rigid_desc& d, const SubBufs& b, int i) {
-  Fin f;
-  f.x4 = b.XPR[i];
-  f.q = ldq(b.QPR[i]);
-  const float4 acc = b.DX[i];
-  const f3 drs = xyz(b.DR[i]);
-  const float4 xp4 = CF4(d.x_prev)[i];
-  const q4 qp = ldq(CF4(d.q_prev)[i]);
-  f.iI = xyz(CF4(d.inv_I)[i]);
-  apply_jacobi_vals(acc, drs, d.relax, f.x4, f.q);
-  finalize_vals(f.x4, f.q, xp4, qp, d.h, d.lin_damp, d.ang_damp, f.v, f.w);
-  return f;
 }
 
 // First lane of the box points in the velocity launch (and of the pairs in
@@ -1280,6 +1142,170 @@ __device__ inline void add_pair_runs(int key, const PointSums& c, int bi, int bj
     atomic_add3(&B[bj], xyz(rj));
     atomicAdd(&A[bj].w, sj.w);
   }
+}
+
+// Workgroup-local: the first point of the solve phase's point r (of the
+// points the block's four wavefronts emitted) and its global slot.
+__device__ inline int block_point_slot(int r, const int* wbase, const int* wtot) {
+  for (int w = 0; w < kB / 64; ++w) {
+    if (r < wtot[w]) return wbase[w] + r;
+    r -= wtot[w];
+  }
+  return -1;
+}
+
+__global__ __launch_bounds__(kB) void k6_sub_contacts(xpbd_rigid_desc d, SubBufs b) {
+  __shared__ float psh[kPolyFloats * kB];
+  __shared__ float4 slot[4 * kB];
+  __shared__ float4 pose[6 * kB];  // per pair: (c, im), q, iI of its bodies a and b
+  __shared__ int skey[kB];
+  __shared__ int wbase[kB / 64], wtot[kB / 64];
+  const int k = blockIdx.x * kB + threadIdx.x;
+  const int n = d.n, nj = d.nj;
+  const int f0 = floor_lane0(nj), p0 = block_lane0(f0 + 8 * n);
+  const float4* HE = CF4(d.he);
+  const int pc = ld_count(d.pair_count);
+  const bool skip = pc > d.cap;  // broad-phase overflow: the frame writes nothing
+  if (k < nj) {
+    const int2 ab = ((const int2*)d.j_idx)[k];
+    const Pred pa = predict_body(d, b, ab.x), pb = predict_body(d, b, ab.y);
+    if (skip) return;
+    solve_joint_vals(k, ab.x, ab.y, pa.x4, pa.q, pa.iI, pb.x4, pb.q, pb.iI, CF4(d.j_a)[k],
+                     CF4(d.j_b)[k], d.lam_j, d.h, 1, b.DX, b.DR);
+    return;
+  }
+  if (k < p0) {
+    if (k < f0 || k >= f0 + 8 * n) return;
+    const int t = k - f0, i = t >> 3, c = t & 7;
+    const Pred p = predict_body(d, b, i);
+    if (skip) return;
+    if (c == 0) {
+      b.XPR[i] = p.x4;
+      b.QPR[i] = stq(p.q);
+      F4(d.x_prev)[i] = make_float4(p.xs.x, p.xs.y, p.xs.z, 0.0f);
+      F4(d.q_prev)[i] = stq(p.qs);
+      b.DV[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      b.DW[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    solve_floor_corner_vals(i, c, p.x4, p.q, p.xs, p.qs, p.iI, xyz(HE[i]), d.floor_y, d.friction,
+                            d.lam_floor, b.DX, b.DR, slot);
+    return;
+  }
+  // candidate pairs: whole workgroups from here on (block barriers below);
+  // skip and the live count are uniform over the grid / the workgroup
+  const int t = threadIdx.x;
+  const int pid0 = blockIdx.x * kB - p0;  // this workgroup's first pair
+  const int live = min(pc, d.cap);
+  if (skip || pid0 >= live) return;
+  const int pid = pid0 + t;
+  PairBodies P;
+  const bool valid = pid < live;
+  if (valid) {
+    const int2 pr = ((const int2*)d.pairs)[pid];
+    const Pred pa = predict_body(d, b, pr.x), pb = predict_body(d, b, pr.y);
+    P.a = pr.x;
+    P.b = pr.y;
+    P.cA = xyz(pa.x4);
+    P.cB = xyz(pb.x4);
+    P.qa = pa.q;
+    P.qb = pb.q;
+    P.imA = pa.x4.w;
+    P.imB = pb.x4.w;
+    P.iIA = pa.iI;
+    P.iIB = pb.iI;
+    P.eA = xyz(HE[pr.x]);
+    P.eB = xyz(HE[pr.y]);
+    pose[6 * t + 0] = pa.x4;
+    pose[6 * t + 1] = stq(pa.q);
+    pose[6 * t + 2] = make_float4(pa.iI.x, pa.iI.y, pa.iI.z, __int_as_float(pr.x));
+    pose[6 * t + 3] = pb.x4;
+    pose[6 * t + 4] = stq(pb.q);
+    pose[6 * t + 5] = make_float4(pb.iI.x, pb.iI.y, pb.iI.z, __int_as_float(pr.y));
+  }
+  // the box manifold (solver_6dof.py:376-384), one pair per lane
+  LdsPoly<kB> poly{psh + t};
+  int wb, wt;
+  box_manifold_pair(pid, valid, P, d.man_margin, d.cap, d.m_count, (int4*)d.pt_idx, F4(d.pt_n),
+                    F4(d.pt_oi), F4(d.pt_or), d.lam_pt, b.PTC, poly, wb, wt);
+  const int maxp = 8 * d.cap;
+  if ((t & 63) == 0) {
+    wbase[t >> 6] = wb;
+    wtot[t >> 6] = max(0, min(wt, maxp - wb));  // slots past 8 cap were not written
+  }
+  __syncthreads();  // the block's points and their pairs' poses are visible
+  int total = 0;
+  for (int w = 0; w < kB / 64; ++w) total += wtot[w];
+  // the first Jacobi iteration over the block's points, one per lane
+  // (solve_box_elem on the stored values: the body-frame offsets taken back
+  // to the world through the same predicted poses); each pair's corrections
+  // summed in LDS and added once (solver_6dof.py:385-401)
+  for (int r0 = 0; r0 < total; r0 += kB) {
+    const int q = r0 + t < total ? block_point_slot(r0 + t, wbase, wtot) : -1;
+    PointSums c;
+    c.li = c.ai = c.lj = c.aj = mk3(0.f, 0.f, 0.f);
+    c.ni = c.nj = 0.0f;
+    int key = -1, bi = 0, bj = 0;
+    if (q >= 0) {
+      const int4 id = ((const int4*)d.pt_idx)[q];
+      const f3 s = xyz(CF4(d.pt_n)[q]);
+      const f3 oi = xyz(CF4(d.pt_oi)[q]), orf = xyz(CF4(d.pt_or)[q]);
+      const int lp = id.x - pid0;  // the pair's lane in this block
+      const bool inc_a = __float_as_int(pose[6 * lp + 2].w) == id.y;
+      const int ii = 6 * lp + (inc_a ? 0 : 3), ir = 6 * lp + (inc_a ? 3 : 0);
+      const float4 xi4 = pose[ii], xr4 = pose[ir];
+      const q4 qi = ldq(pose[ii + 1]), qr = ldq(pose[ir + 1]);
+      const f3 iIi = xyz(pose[ii + 2]), iIr = xyz(pose[ir + 2]);
+      key = id.x;
+      bi = id.y;
+      bj = id.z;
+      const f3 pi = xyz(xi4) + qrot(qi, oi);
+      const f3 pr = xyz(xr4) + qrot(qr, orf);
+      const float C = dot(pi - pr, s);
+      if (!(C >= 0.0f)) {
+        const f3 r_inc = pi - xyz(xi4), r_ref = pr - xyz(xr4);
+        const float wsum =
+            gen_inv_mass(xi4.w, qi, iIi, r_inc, s) + gen_inv_mass(xr4.w, qr, iIr, r_ref, s);
+        if (!(wsum <= 0.0f)) {
+          const float dlam = -C / wsum;
+          d.lam_pt[q] = 0.0f + dlam;
+          const f3 pp = s * dlam;  // accumulate_correction's terms
+          if (xi4.w > 0.0f) {
+            c.li = pp * xi4.w;
+            c.ai = world_invI_mul(qi, iIi, cross(r_inc, pp));
+            c.ni = 1.0f;
+          }
+          if (xr4.w > 0.0f) {
+            c.lj = (-pp) * xr4.w;
+            c.aj = -world_invI_mul(qr, iIr, cross(r_ref, pp));
+            c.nj = 1.0f;
+          }
+        }
+      }
+    }
+    add_pair_runs(key, c, bi, bj, b.DX, b.DR, slot, skey);
+    __syncthreads();  // slot / skey are reused by the next round
+  }
+}
+
+// Body i after the last iteration's apply_jacobi and finalize_velocity.
+struct Fin {
+  float4 x4;
+  q4 q;
+  f3 v, w, iI;
+};
+
+__device__ inline Fin finalize_body(const xpbd_rigid_desc& d, const SubBufs& b, int i) {
+  Fin f;
+  f.x4 = b.XPR[i];
+  f.q = ldq(b.QPR[i]);
+  const float4 acc = b.DX[i];
+  const f3 drs = xyz(b.DR[i]);
+  const float4 xp4 = CF4(d.x_prev)[i];
+  const q4 qp = ldq(CF4(d.q_prev)[i]);
+  f.iI = xyz(CF4(d.inv_I)[i]);
+  apply_jacobi_vals(acc, drs, d.relax, f.x4, f.q);
+  finalize_vals(f.x4, f.q, xp4, qp, d.h, d.lin_damp, d.ang_damp, f.v, f.w);
+  return f;
 }
 
 __global__ __launch_bounds__(kB) void k6_sub_velocity(xpbd_rigid_desc d, SubBufs b) {
@@ -1754,8 +1780,8 @@ static int check_sub_desc(const xpbd_rigid_desc* desc, int sub) {
   if (!desc || !desc->aux || !desc->pt_count || desc->n < 0 || desc->cap < 0 || desc->nj < 0 ||
       sub < 0)
     return (int)hipErrorInvalidValue;
-  const long long lanes = (long long)pair_lane0(desc->nj, desc->n) + desc->cap;
-  if (lanes >= (1ll << 31) || 8ll * desc->n + 8ll * desc->cap >= (1ll << 31))
+  const long long lanes = (long long)floor_lane0(desc->nj) + 8ll * desc->n + kB + desc->cap;
+  if (lanes >= (1ll << 31) || 8ll * desc->n + kB + 8ll * desc->cap >= (1ll << 31))
     return (int)hipErrorInvalidValue;
   return 0;
 }
@@ -1765,8 +1791,7 @@ int xpbd6_substep_contacts(void* stream, const xpbd_rigid_desc* desc, int sub) {
   if (rc) return rc;
   const xpbd_rigid_desc& d = *desc;
   if (d.n == 0) return 0;
-  if (!d.pt_first) return (int)hipErrorInvalidValue;
-  const long long lanes = (long long)pair_lane0(d.nj, d.n) + d.cap;
+  const long long lanes = (long long)block_lane0(floor_lane0(d.nj) + 8 * d.n) + d.cap;
   hipLaunchKernelGGL(k6_sub_contacts, dim3(nblk(lanes)), dim3(kB), 0, (hipStream_t)stream, d,
                      sub_bufs(d, sub));
   XPBD_LAUNCH_CHECK();

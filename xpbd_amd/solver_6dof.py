@@ -406,7 +406,6 @@ class Solver6DOF:
         b = self._dev
         b["PAIRS"] = torch.zeros((cap, 2), dtype=torch.int32, device=dev)
         b["MC"] = torch.zeros(cap, dtype=torch.int32, device=dev)
-        b["PF"] = torch.zeros(cap, dtype=torch.int32, device=dev)
         b["PTI"] = torch.zeros((8 * cap, 4), dtype=torch.int32, device=dev)
         b["PTN"] = torch.zeros((8 * cap, 4), dtype=torch.float32, device=dev)
         b["PTOI"] = torch.zeros((8 * cap, 4), dtype=torch.float32, device=dev)
@@ -564,7 +563,7 @@ class Solver6DOF:
         if self.substep_launches not in (2, 5):
             raise ValueError("substep_launches must be 2 or 5")
         if self.substep_launches == 2:
-            d.aux, d.pt_first = P(b["AUX"]), P(b["PF"])
+            d.aux = P(b["AUX"])
         self._desc = d
         handle = N.c_void_p()
         N.check(N.lib().xpbd_frame6_create(N.ctypes.byref(d), N.ctypes.byref(handle)),

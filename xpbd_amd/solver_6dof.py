@@ -413,10 +413,10 @@ class Solver6DOF:
         b["LAMP"] = torch.zeros(8 * cap, dtype=torch.float32, device=dev)
 
     # one workgroup runs the whole frame up to this many bodies ("auto"):
-    # above it the per-launch plan's 256 CUs win the manifold and solve work
-    # (box piles, ms per frame, block against launch: 100 bodies 0.50 / 0.67,
-    # 150: 0.67 / 0.71, 180: 0.78 / 0.71, 256: 1.03 / 0.71)
-    BLOCK_AUTO = 160
+    # above it the two-launch plan's 256 CUs win the manifold and solve work
+    # (box piles, ms per frame, block against two-launch plan: 27 bodies
+    # 0.308 / 0.318, 100: 0.498 / 0.365, 144: 0.624 / 0.389, 256: 1.03 / 0.41)
+    BLOCK_AUTO = 32
 
     @property
     def n_pairs(self) -> int:

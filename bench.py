@@ -371,14 +371,15 @@ def reference_rows(frames: int = 120, warmup: int = 20) -> dict:
     return rows
 
 
-def judge_interval(name: str) -> int:
+def judge_interval(name: str):
     """Frames between the judge's checks of the learned start for a timed
-    configuration: every frame unless the name says otherwise."""
+    configuration: the solver's default (every 8th frame) unless the name
+    says otherwise (None: leave the default)."""
     if name.endswith("no_judge"):
         return 0
-    if name.endswith("judge8"):
-        return 8
-    return 1
+    if name.endswith("judge1"):
+        return 1
+    return None
 
 
 def warm_start_section(res: int, frames: int) -> dict:
@@ -419,13 +420,14 @@ def warm_start_section(res: int, frames: int) -> dict:
                                         ("learned_1", 1, True, 1),
                                         ("learned_2_net_per_frame", 2, True, 15),
                                         ("learned_1_net_per_frame", 1, True, 15),
-                                        ("learned_1_net_per_frame_judge8", 1, True, 15),
+                                        ("learned_1_net_per_frame_judge1", 1, True, 15),
                                         ("learned_1_net_per_frame_no_judge", 1, True, 15)]:
         c = cloth(iters)
         c.block_frames = False
         if learned:
             c.warm_start_every = every
-            c.warm_start_judge = judge_interval(name)
+            if judge_interval(name) is not None:
+                c.warm_start_judge = judge_interval(name)
             c.set_warm_start(theta if every == 1 else theta_frame)
         best, _ = autotune.tune(c)
         shapes[name] = list(best) if best else None
@@ -454,8 +456,8 @@ def warm_start_section(res: int, frames: int) -> dict:
         "learned2_net_per_frame_vs_plain4": {
             "residual_ratio": round(resid["learned_2_net_per_frame"] / resid["plain_4"], 3),
             "speedup": round(sps["learned_2_net_per_frame"] / sps["plain_4"], 3)},
-        "learned1_net_per_frame_judge8_vs_plain4": {
-            "speedup": round(sps["learned_1_net_per_frame_judge8"] / sps["plain_4"], 3)},
+        "learned1_net_per_frame_judge1_vs_plain4": {
+            "speedup": round(sps["learned_1_net_per_frame_judge1"] / sps["plain_4"], 3)},
         "learned1_net_per_frame_no_judge_vs_plain4": {
             "speedup": round(sps["learned_1_net_per_frame_no_judge"] / sps["plain_4"], 3)},
     }
@@ -503,13 +505,14 @@ def warm_start_batched_section(frames: int) -> dict:
     for name, iters, learned in [("plain_4", 4, False), ("plain_2", 2, False),
                                  ("plain_1", 1, False), ("learned_2_net_per_frame", 2, True),
                                  ("learned_1_net_per_frame", 1, True),
-                                 ("learned_1_net_per_frame_judge8", 1, True),
+                                 ("learned_1_net_per_frame_judge1", 1, True),
                                  ("learned_1_net_per_frame_no_judge", 1, True)]:
         restore()
         s.iterations = iters
         if learned:
             s.warm_start_every = 15
-            s.warm_start_judge = judge_interval(name)
+            if judge_interval(name) is not None:
+                s.warm_start_judge = judge_interval(name)
             s.set_warm_start(theta)
         ms = time_frames(s.step, frames, 3)
         sps[name] = round(s.substeps / (ms * 1e-3), 1)
@@ -531,7 +534,7 @@ def warm_start_batched_section(frames: int) -> dict:
         out[f"learned{k}_net_per_frame_vs_plain4"] = {
             "residual_ratio": round(resid[f"learned_{k}_net_per_frame"] / resid["plain_4"], 3),
             "speedup": round(sps[f"learned_{k}_net_per_frame"] / sps["plain_4"], 3)}
-    for v in ("judge8", "no_judge"):
+    for v in ("judge1", "no_judge"):
         out[f"learned1_net_per_frame_{v}_vs_plain4"] = {
             "speedup": round(sps[f"learned_1_net_per_frame_{v}"] / sps["plain_4"], 3)}
     del w, s

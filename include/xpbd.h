@@ -430,14 +430,15 @@ int xpbd6_frame_block_capacity(void);
  * substep_contacts(sub): apply_velocity + cap of substep sub-1 (sub > 0;
  * solver_6dof.py:420-426) and integrate_bodies (solver_6dof.py:370-373),
  * recomputed on values by every lane that needs a body, then the first
- * Jacobi iteration: joints, floor corners and the box manifold
- * (solver_6dof.py:376-384) with each point solved as it is emitted
- * (solver_6dof.py:385-401); stores the predicted pose in aux and the start
- * pose in x_prev / q_prev.  Further iterations: xpbd6_apply_finalize
+ * Jacobi iteration: joints, floor corners and, in workgroups of their own,
+ * the box manifold (solver_6dof.py:376-384) one pair per lane followed by
+ * the points those pairs emitted, one per lane (solver_6dof.py:385-401);
+ * stores the predicted pose in aux and the start pose in x_prev / q_prev.  Further iterations: xpbd6_apply_finalize
  * (finalize 0) and xpbd6_solve_all (first 0) on the predicted pose.
  * substep_velocity(sub): apply_jacobi of the last iteration and
  * finalize_velocity on values (solver_6dof.py:402-408), then the velocity
- * pass (solver_6dof.py:409-419); stores x, q, v, w.  substep_end: the
+ * pass (solver_6dof.py:409-419), one point per lane, each pair's terms
+ * summed before they are added; stores x, q, v, w.  substep_end: the
  * frame's last apply_velocity + cap; every sum buffer and both point
  * counters back to zero.  Sums of substep sub use parity sub & 1 of the
  * sum buffers, so no lane reads what another lane of the same launch

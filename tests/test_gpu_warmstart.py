@@ -338,6 +338,7 @@ def test_judge_keeps_the_trained_start_and_drops_a_poor_one_within_a_frame(train
 
     theta, _ = trained
     s = _cloth()
+    s.warm_start_judge = 1
     s.step(10)
     s.set_warm_start(theta)
     s.step(12)
@@ -357,6 +358,7 @@ def test_judge_keeps_the_trained_start_and_drops_a_poor_one_within_a_frame(train
 
     bad = torch.from_numpy(th).to(DEV)
     c, probe = _stretched_chain(), _stretched_chain()
+    c.warm_start_judge = 1
     c.set_warm_start(bad)
     offs = 0
     for f in range(16):

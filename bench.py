@@ -555,6 +555,14 @@ def load_traffic(kernel: str):
     return None if e is None else e.get("bytes_per_launch")
 
 
+def traffic_frac(traffic, kernel_ms):
+    """Counter-measured HBM bytes per launch (profiles/traffic.json) over the
+    launch time, as a fraction of the HBM peak (None without counters)."""
+    if not traffic or not kernel_ms:
+        return None
+    return round(traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+
+
 def scene_only(name: str, frames: int, warmup: int) -> dict:
     """Time one scene alone (for kernel traces of its frames): ``boxesN``
     (the reference's N-box LBVH pile, N in 27, 100, 384, 1440), ``cloth128``
@@ -672,8 +680,10 @@ def main() -> int:
         s.set_state(x_keep, v_keep)
         nbytes = iter_bytes(n_dist, n_floor, s.num_bodies, active > 0)
         ach = nbytes / (kms * 1e-3) / 1e9
+        traffic = load_traffic("k_jacobi_tile")
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic("k_jacobi_tile"),
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_frac": traffic_frac(traffic, kms),
                 "kernel": "k_jacobi_tile", "kernel_ms": round(kms, 5),
                 "bytes_per_launch": nbytes, "floor_rows_active": active, "tile": s._tile}
         print(f"[bench] jacobi tile kernel {kms * 1e3:.2f} us, {ach:.1f} GB/s", file=sys.stderr,
@@ -694,6 +704,7 @@ def main() -> int:
               f"{scene['achieved']} GB/s, frame {scene['frame_ms']} ms", file=sys.stderr,
               flush=True)
         scene["traffic"] = load_traffic("k_jacobi_tile@8x1024^2")
+        scene["traffic_frac"] = traffic_frac(scene["traffic"], scene["kernel_ms"])
         torch.cuda.empty_cache()
 
     others = None

@@ -134,7 +134,34 @@ def test_broadphase_overflow_regrows():
     assert len(s.pairs()) == s.n_pairs
 
 
-def test_overflowed_frames_run_again_when_read():
+@pytest.mark.parametrize("broadphase", ["lbvh", "hashgrid"])
+def test_two_launch_plan_graph_equals_eager_launches(broadphase):
+    """The frame plan with its broad phase inside, replayed from the
+    captured graph and issued eagerly: the same pair lists and contact
+    counts frame after frame, states within 1e-5 (float atomics sum in a
+    different order)."""
+    from xpbd_amd import Solver6DOF
+    from xpbd_amd.scenes import box_pile
+
+    a = box_pile(Solver6DOF, 5, 4, 5, device=DEV, broadphase=broadphase)
+    b = box_pile(Solver6DOF, 5, 4, 5, device=DEV, broadphase=broadphase)
+    for s, g in ((a, True), (b, False)):
+        s.block_frames = False
+        s.use_graph = g
+    seen = 0
+    for f in range(15):  # the pile's boxes start 4 cm apart: pairs come as it lands
+        a.step()
+        b.step()
+        assert a.n_pairs == b.n_pairs, f
+        assert (a.pairs() == b.pairs()).all(), f
+        assert (a.contact_counts() == b.contact_counts()).all(), f
+        assert _rel(a.positions(), b.positions()) <= 1e-5, f
+        seen = max(seen, a.n_pairs)
+    assert seen > 0
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_overflowed_frames_run_again_when_read(use_graph):
     """The two-launch plan runs its broad phase inside the frame and checks
     the pair count later: a frame that overflowed the pair buffers, and
     every frame after it, wrote nothing; the next read of the state grows
@@ -150,6 +177,7 @@ def test_overflowed_frames_run_again_when_read():
     b = box_pile(Solver6DOF, 6, 4, 6, substeps=4, device=DEV)
     for s in (a, b):
         s.block_frames = False
+        s.use_graph = use_graph
         s._flush()
     a.cap = 8
     a._alloc_pair_buffers(8)

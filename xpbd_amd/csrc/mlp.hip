@@ -145,17 +145,26 @@ __global__ __launch_bounds__(kBlk) void k_mlp_train_partial(const float* __restr
       for (int j = 0; j < 4; ++j) s.tile[wv][4 * g + j][16 * n + r16] = (unsigned short)dzb[n][j];
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
-    bf16x4 a[4];
-    for (int sK = 0; sK < 4; ++sK)
-      for (int j = 0; j < 4; ++j) a[sK][j] = (short)s.tile[wv][r16][16 * sK + 4 * g + j];
+    bf16x8 a[2];  // two 32-wide K slices over the layer-2 units (mlp_dev.h)
+    for (int m = 0; m < 2; ++m) {
+      bf16x4 lo, hi;
+      for (int j = 0; j < 4; ++j) {
+        lo[j] = (short)s.tile[wv][r16][32 * m + 4 * g + j];
+        hi[j] = (short)s.tile[wv][r16][32 * m + 16 + 4 * g + j];
+      }
+      a[m] = cat8(lo, hi);
+    }
     __builtin_amdgcn_wave_barrier();
     bf16x4 dz1b[4];
     for (int nt = 0; nt < 4; ++nt) {
       f32x4 acc{0.f, 0.f, 0.f, 0.f};
-      for (int sK = 0; sK < 4; ++sK) {
-        bf16x4 b;  // B[k][n] = W2[n][k]
-        for (int j = 0; j < 4; ++j) b[j] = (short)s.w.w2t[16 * sK + 4 * g + j][16 * nt + r16];
-        acc = mfma(a[sK], b, acc);
+      for (int m = 0; m < 2; ++m) {
+        bf16x4 lo, hi;  // B[k][n] = W2[n][k]
+        for (int j = 0; j < 4; ++j) {
+          lo[j] = (short)s.w.w2t[32 * m + 4 * g + j][16 * nt + r16];
+          hi[j] = (short)s.w.w2t[32 * m + 16 + 4 * g + j][16 * nt + r16];
+        }
+        acc = mfma32(a[m], cat8(lo, hi), acc);
       }
       for (int j = 0; j < 4; ++j) {
         const float dz = acc[j] * gelu_grad(z1[nt][j]);

@@ -4,15 +4,24 @@
 // operations in the same order.
 //
 // Parameters: one float32 vector of 5057 values, W1[12][64] b1[64] W2[64][64]
-// b2[64] W3[64] b3[1] (row-major, input x output).  The two 64-wide products
-// run on v_mfma_f32_16x16x16_bf16: inputs, hidden activations and W1/W2 are
-// rounded to bfloat16 (nearest even), products accumulate in float32; the
-// output layer and every elementwise op stay in float32.
+// b2[64] W3[64] b3[1] (row-major, input x output).  Inputs, hidden
+// activations and W1/W2 are rounded to bfloat16 (nearest even), products
+// accumulate in float32 on the matrix cores; the output layer and every
+// elementwise op stay in float32.  Contractions over the 64 hidden units
+// (layer 2 and its backward) run on gfx950's v_mfma_f32_16x16x32_bf16, two
+// per 16 x 16 tile; those over 16 (the 12 features padded, or the 16 rows of
+// a tile in the weight gradients) on v_mfma_f32_16x16x16_bf16.
 //
-// A wavefront evaluates 16 rows at a time.  MFMA operand layout (16x16x16):
-//   A (M x K):  lane l holds A[l % 16][4 (l / 16) + j], j = 0..3
-//   B (K x N):  lane l holds B[4 (l / 16) + j][l % 16]
-//   D (M x N):  lane l holds D[4 (l / 16) + j][l % 16]
+// A wavefront evaluates 16 rows at a time.  MFMA operand layouts (lane l,
+// r = l % 16, g = l / 16; checked against a CPU product on the device):
+//   16x16x16:  A[r][4g + j], B[4g + j][r], j = 0..3
+//   16x16x32:  A[r][8g + j], B[8g + j][r], j = 0..7
+//   D (both):  D[4g + j][r], j = 0..3
+// In a 32-wide contraction, K slice m (m = 0, 1) puts hidden unit
+// 32m + 4g + j at position 8g + j and 32m + 16 + 4g + j at 8g + 4 + j, so
+// the two 16x16 D tiles 2m and 2m + 1 of the previous layer are, lane for
+// lane, the B operand (h[2m], h[2m + 1]) with no data movement; the weight
+// operands are read in the same order.
 //
 // Two evaluation orders share these weights: forward16 (rows as M, used by
 // the training kernels, whose backward pass works in that layout) and
@@ -31,6 +40,7 @@ constexpr int P_W1 = 0, P_B1 = P_W1 + NIN * NH, P_W2 = P_B1 + NH, P_B2 = P_W2 + 
 static_assert(NPARAM == 5057, "parameter count");
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bfloat16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950;
@@ -62,6 +72,21 @@ __device__ inline float gelu_grad(float z) {
 
 __device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Two bfloat16 quadruples as one 8-element operand (positions 8g .. 8g+3,
+// then 8g+4 .. 8g+7 of a 32-wide K slice): one shuffle of the 16-bit
+// patterns and one bit cast of the whole vector.  (Inserting bit-cast
+// __bf16 values element by element gave wrong operands with this compiler;
+// checked against a CPU product on the device.)
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+__device__ inline bf16x8 cat8(bf16x4 lo, bf16x4 hi) {
+  const s16x8 t = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8, t);
+}
+
+__device__ inline f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
 // The network in LDS, bfloat16 weights stored transposed (output-major:
@@ -127,14 +152,17 @@ __device__ inline void forward16(const Weights& s, unsigned short (*tile)[NH], F
   }
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
-  // layer 2: A = H1 tile read back in A layout
-  bf16x4 a[4];
-  for (int sK = 0; sK < 4; ++sK) a[sK] = *(const bf16x4*)&tile[r16][16 * sK + 4 * g];
+  // layer 2: A = H1 tile read back in A layout, two 32-wide K slices
+  bf16x8 a[2];
+  for (int m = 0; m < 2; ++m)
+    a[m] = cat8(*(const bf16x4*)&tile[r16][32 * m + 4 * g],
+                *(const bf16x4*)&tile[r16][32 * m + 16 + 4 * g]);
   for (int n = 0; n < 4; ++n) {
     f32x4 acc{0.f, 0.f, 0.f, 0.f};
-    for (int sK = 0; sK < 4; ++sK) {
-      const bf16x4 b = *(const bf16x4*)&s.w2t[16 * n + r16][16 * sK + 4 * g];
-      acc = mfma(a[sK], b, acc);
+    for (int m = 0; m < 2; ++m) {
+      const bf16x8 b = cat8(*(const bf16x4*)&s.w2t[16 * n + r16][32 * m + 4 * g],
+                            *(const bf16x4*)&s.w2t[16 * n + r16][32 * m + 16 + 4 * g]);
+      acc = mfma32(a[m], b, acc);
     }
     for (int j = 0; j < 4; ++j) acc[j] += s.b2[16 * n + r16];
     z2[n] = acc;
@@ -151,17 +179,20 @@ __device__ inline void forward16(const Weights& s, unsigned short (*tile)[NH], F
 
 // Inference weights held in registers for a wave's whole life: the W1^T and
 // W2^T tiles as MFMA A operands (40 VGPRs).  Lane (r, g) = (l % 16, l / 16)
-// holds a1[n][j] = W1[4g + j][16n + r] and a2[n][k][j] = W2[16k + 4g + j][16n + r].
+// holds a1[n][j] = W1[4g + j][16n + r] and a2[n][m] = (W2[32m + 4g + j][16n + r],
+// W2[32m + 16 + 4g + j][16n + r]), j = 0..3.
 struct RegWeights {
   bf16x4 a1[4];
-  bf16x4 a2[4][4];
+  bf16x8 a2[4][2];
 };
 
 __device__ inline void load_reg_weights(RegWeights& w, const Weights& s) {
   const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
   for (int n = 0; n < 4; ++n) {
     w.a1[n] = *(const bf16x4*)&s.w1t[16 * n + r][4 * g];
-    for (int k = 0; k < 4; ++k) w.a2[n][k] = *(const bf16x4*)&s.w2t[16 * n + r][16 * k + 4 * g];
+    for (int m = 0; m < 2; ++m)
+      w.a2[n][m] = cat8(*(const bf16x4*)&s.w2t[16 * n + r][32 * m + 4 * g],
+                        *(const bf16x4*)&s.w2t[16 * n + r][32 * m + 16 + 4 * g]);
   }
 }
 
@@ -185,9 +216,10 @@ __device__ inline float forward16_t(const RegWeights& w, const float* b1, const 
     h[n][3] = (short)to_bf16(gelu(z[3] + b.w));
   }
   float p = 0.0f;
-  for (int n = 0; n < 4; ++n) {  // Z2^T tile n = sum_k W2^T(n, k) H1^T(k)
+  const bf16x8 hb[2] = {cat8(h[0], h[1]), cat8(h[2], h[3])};  // B slices of H1^T
+  for (int n = 0; n < 4; ++n) {  // Z2^T tile n = sum_m W2^T(n, m) H1^T(m)
     f32x4 acc{0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < 4; ++k) acc = mfma(w.a2[n][k], h[k], acc);
+    for (int m = 0; m < 2; ++m) acc = mfma32(w.a2[n][m], hb[m], acc);
     const float4 b = *(const float4*)&b2[16 * n + 4 * g];
     const float4 c = *(const float4*)&w3[16 * n + 4 * g];
     p = fmaf(gelu(acc[0] + b.x), c.x, p);

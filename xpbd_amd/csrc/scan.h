@@ -72,7 +72,45 @@ __global__ __launch_bounds__(kB) void k_scan_add(int* out, const int* sums, int 
     if (base + k < len) out[base + k] += add;
 }
 
+// Up to kScanOneMax items: one workgroup of kScanOneT threads, kScanItems
+// items each, scans everything in one launch (in may equal out: each thread
+// reads its items before the first barrier and writes only them).
+constexpr int kScanOneT = 1024;
+constexpr int kScanOneMax = kScanOneT * kScanItems;  // 4096
+
+__global__ __launch_bounds__(kScanOneT) void k_scan_one(const int* in, int* out, int len,
+                                                         int* total_out) {
+  __shared__ int sh[kScanOneT];
+  const int t = threadIdx.x;
+  const int base = t * kScanItems;
+  int v[kScanItems];
+  int local = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < len) ? in[base + k] : 0;
+    local += v[k];
+  }
+  sh[t] = local;
+  __syncthreads();
+  for (int d = 1; d < kScanOneT; d <<= 1) {
+    const int add = t >= d ? sh[t - d] : 0;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  int ex = sh[t] - local;
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < len) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (t == kScanOneT - 1 && total_out) *total_out = sh[kScanOneT - 1];
+}
+
 int exclusive_scan(hipStream_t s, const int* in, int* out, int len, ScanWs& ws, int* total) {
+  if (len <= kScanOneMax) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneT), 0, s, in, out, len, total);
+    XPBD_LAUNCH_CHECK();
+    return 0;
+  }
   int nb = nblk(len, kScanTile);
   hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(kB), 0, s, in, out, ws.block_sums, len);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kB), 0, s, ws.block_sums, nb, total);

@@ -395,7 +395,7 @@ __device__ inline void sort_segment(int2* seg, int m) {
 // LBVH query.  COUNT writes the partner count of i; EMIT writes the pairs
 // at off[i] in ascending j (each lane's slot = the hits of lower lanes in
 // its chunk, from the ballot), i.e. exactly the sorted segments of the LBVH
-// path, in 6 launches instead of ~25.
+// path, in 4 launches (AABBs, count, scan, emit) instead of ~25.
 constexpr int kAllPairsMax = 4096;
 constexpr int kMaxLeaf = 64;  // bodies per LBVH leaf
 

@@ -290,7 +290,44 @@ typedef struct xpbd_particles_desc {
   float* ws_jdlam;
   double* ws_rsum;
   int* ws_stats;
+  /* Two-iteration substeps in one launch (tile mode, iterations = 2,
+   * fuse_first = 2, no learned start, no device rows; see
+   * xpbd_jacobi_tile_pair): when pair_p1 is set, every substep after a
+   * frame's first runs as one xpbd_jacobi_tile_pair launch with these
+   * tables, its x_prev ping-ponging between x_prev and x_prev_alt
+   * (float4[n_particles]); the last substep leaves it in x_prev. */
+  float* x_prev_alt;
+  const int* pair_ring_start;
+  const int* pair_ring1_n;
+  const int* pair_ring_idx;
+  const int* pair_p1_start;
+  const int* pair_p1;
+  const int* pair_p2_start;
+  const int* pair_p2;
+  int pair_lmax, pair_amax, pair_k1max;
 } xpbd_particles_desc;
+
+/* A whole two-iteration substep of the fused tile plan in one launch, for
+ * small scenes where launches cost latency: the first iteration
+ * (xpbd_jacobi_tile, first = 1, on predicted positions) and the second with
+ * finalize + cap and, unless `last`, the next substep's integrate
+ * (xpbd_jacobi_tile_next; with `last`, xpbd_jacobi_tile with finalize),
+ * results bit for bit those of the two launches.  Each tile also runs the
+ * first iteration of the particles one row away (ring 1), from every row
+ * incident to them: tables of xpbd_amd/solver.py tile_pair_tables (rings,
+ * phase-1 rows with local endpoints, phase-2 rows).  x_prev is read from
+ * XPin and written to XPout (!= XPin unless last): the next substep's start
+ * positions, or with `last` a copy of XPin.  Multipliers of the second
+ * iteration go to LAMout (owner tiles).  Dynamic LDS:
+ * xpbd_jacobi_tile_pair_lds_bytes(lmax, amax, k1max) <= 64 KiB. */
+int xpbd_jacobi_tile_pair(void* stream, const float* Xin, float* Xout, const float* XPin,
+                          float* XPout, float* V, const int* R, const float* ANC, float* LAMout,
+                          const int* ring_start, const int* ring1_n, const int* ring_idx,
+                          const int* p1_start, const int* p1, const int* p2_start, const int* p2,
+                          float h, int tile_size, int n, float relax, float damping,
+                          float max_speed, float gx, float gy, float gz, const float* gscale,
+                          int lmax, int amax, int k1max, int last);
+size_t xpbd_jacobi_tile_pair_lds_bytes(int lmax, int amax, int k1max);
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);
 /* Runs `frames` frames on `stream`; use_graph != 0 captures the frame once

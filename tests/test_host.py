@@ -478,3 +478,70 @@ def test_host_grid_broadphase_pair_set_equals_brute_force(seed):
     vmax, wmax = float(np.abs(v).max()), float(np.abs(w).max())
     assert margin == pytest.approx(min(0.02 + 2 / 60 * (vmax + wmax * float(
         np.linalg.norm(he, axis=1).max())), float(half.max())))
+
+
+@pytest.mark.parametrize("tile", [7, 64, 100000])
+def test_tile_pair_tables_match_brute_force(tile):
+    """The two-iteration launch's tables against set arithmetic per tile:
+    ring 1 = other endpoints of the tile's rows, ring 2 = other endpoints of
+    ring 1's rows outside tile and ring 1, phase-1 rows = rows incident to
+    tile or ring 1 with local indices naming the right particles, phase-2
+    rows = rows incident to the tile with the owner bit where the smallest
+    endpoint is in the tile; LDS maxima are the per-tile maxima."""
+    from xpbd_amd import Solver
+    from xpbd_amd.scenes import cloth_sheet
+    from xpbd_amd.solver import jacobi_tile_order, morton_order, pack_rows, tile_pair_tables
+
+    s, _ = cloth_sheet(Solver, res=12)
+    h = s.host_arrays()
+    n = s.num_bodies
+    perm = morton_order(h["x"].astype(np.float64))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    ra = inv[h["a"]]
+    rb = np.where(h["b"] >= 0, inv[np.maximum(h["b"], 0)], -1)
+    rows, _ = pack_rows(h["type"], ra, rb, h["rest"], h["compliance"], h["anchor"], h["mu"])
+    single = rows[::5].copy()  # one-endpoint rows (floor / attach shaped)
+    single[:, 1] = -1
+    rows = np.concatenate([rows, single])
+    a = (rows[:, 0] & 0x0FFFFFFF).astype(np.int64)
+    b = rows[:, 1].astype(np.int64)
+    rows = rows[jacobi_tile_order(a, b, np.zeros(len(a), np.int64), tile)]
+    a = (rows[:, 0] & 0x0FFFFFFF).astype(np.int64)
+    b = rows[:, 1].astype(np.int64)
+    t = tile_pair_tables(rows, n, tile)
+    ends = [{int(a[j])} | ({int(b[j])} if b[j] >= 0 else set()) for j in range(len(rows))]
+
+    def incident(ps):
+        return {j for j in range(len(rows)) if ends[j] & ps}
+
+    n_t = (n + tile - 1) // tile
+    lmax = amax = k1max = 0
+    for k in range(n_t):
+        T = set(range(k * tile, min((k + 1) * tile, n)))
+        rows_t = incident(T)
+        r1 = set().union(*(ends[j] for j in rows_t)) - T if rows_t else set()
+        rows_r1 = incident(r1)
+        r2 = (set().union(*(ends[j] for j in rows_r1)) if rows_r1 else set()) - T - r1
+        rb0, re0 = t["ring_start"][k], t["ring_start"][k + 1]
+        ring = t["ring_idx"][rb0:re0].tolist()
+        n1 = int(t["ring1_n"][k])
+        assert set(ring[:n1]) == r1 and set(ring[n1:]) == r2 and len(ring) == len(r1) + len(r2)
+
+        def glob(l):
+            return k * tile + l if l < tile else ring[l - tile]
+
+        p1 = t["p1"][t["p1_start"][k]:t["p1_start"][k + 1]]
+        assert sorted(p1[:, 0].tolist()) == sorted(rows_t | rows_r1)
+        for j, la, lb, _ in p1.tolist():
+            assert glob(la) == a[j] and (lb == -1 if b[j] < 0 else glob(lb) == b[j])
+        p2 = t["p2"][t["p2_start"][k]:t["p2_start"][k + 1]].astype(np.int64)
+        pos = p2 & 0x7FFFFFFF
+        own = p2 < 0
+        assert sorted(p1[pos, 0].tolist()) == sorted(rows_t)
+        lo = np.where(b >= 0, np.minimum(a, b), a)
+        assert (own == (lo[p1[pos, 0]] // tile == k)).all()
+        lmax = max(lmax, tile + len(r1) + len(r2))
+        amax = max(amax, tile + len(r1))
+        k1max = max(k1max, len(rows_t | rows_r1))
+    assert (t["lmax"], t["amax"], t["k1max"]) == (lmax, amax, k1max)

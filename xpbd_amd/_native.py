@@ -43,6 +43,10 @@ class ParticlesDesc(ctypes.Structure):
         ("ws_jxp", c_void_p), ("ws_jlam", c_void_p), ("ws_jlam_alt", c_void_p),
         ("ws_jy", c_void_p), ("ws_jacc", c_void_p), ("ws_jdlam", c_void_p),
         ("ws_rsum", c_void_p), ("ws_stats", c_void_p),
+        ("x_prev_alt", c_void_p), ("pair_ring_start", c_void_p), ("pair_ring1_n", c_void_p),
+        ("pair_ring_idx", c_void_p), ("pair_p1_start", c_void_p), ("pair_p1", c_void_p),
+        ("pair_p2_start", c_void_p), ("pair_p2", c_void_p),
+        ("pair_lmax", c_int), ("pair_amax", c_int), ("pair_k1max", c_int),
     ]
 
 
@@ -125,6 +129,8 @@ _SIGS = {
     "xpbd6_velocity_all": (_I, [_P] * 5 + [_F, _F, _P, _F, _P, _P, _I] + [_P] * 6 + [_I, _P, _P]),
     "xpbd_frame6_create": (_I, [ctypes.POINTER(RigidDesc), ctypes.POINTER(c_void_p)]),
     "xpbd_frame6_run": (_I, [_P, _P, _I, _I]),
+    "xpbd_jacobi_tile_pair": (_I, [_P] * 16 + [_F, _I, _I, _F, _F, _F, _F, _F, _F, _P, _I, _I, _I, _I]),
+    "xpbd_jacobi_tile_pair_lds_bytes": (ctypes.c_size_t, [_I, _I, _I]),
     "xpbd6_substep_contacts": (_I, [_P, _P, _I]),
     "xpbd6_substep_velocity": (_I, [_P, _P, _I]),
     "xpbd6_substep_end": (_I, [_P, _P]),

@@ -7,7 +7,9 @@ and keeps the fastest:
 * ``tile``   — the fused per-iteration kernel with tile sizes of 32 ... 1024
   particles (44 B of LDS per particle; larger tiles re-project fewer halo
   rows, smaller tiles spread the launch over more CUs), each with integrate
-  folded into the first iteration or run as its own launch;
+  folded into the first iteration or run as its own launch, and with two
+  iterations per substep also as two launches instead of one
+  (``xpbd_jacobi_tile_pair`` re-projects the ring rows);
 * ``window`` — projection with an LDS window + global atomics and a separate
   apply pass, with 1 or 2 rows per thread;
 * ``block``  — whole frames in one workgroup (scenes that fit its LDS).
@@ -31,14 +33,19 @@ def _signature(s):
             s._gscale is not None, getattr(s, "_device_contacts", False))
 
 
-def candidates(n_particles: int, tile_only: bool = False, fused_only: bool = False):
+def candidates(n_particles: int, tile_only: bool = False, fused_only: bool = False,
+               pair: bool = False):
     """Launch shapes to time; ``tile_only`` (learned warm start): no window
     or block shapes; ``fused_only`` (the learned start with its judge, which
-    runs in the fused plan): integrate folded in."""
+    runs in the fused plan): integrate folded in; ``pair`` (two iterations,
+    no warm start): the fused plan also without the one-launch substep
+    (fuse "two")."""
     out = []
     for t in (32, 64, 128, 256, 512, 1024):
         if t <= max(64, 2 * n_particles):
             out.append(("tile", t, None, True))
+            if pair:
+                out.append(("tile", t, None, "two"))
             if not fused_only:
                 out.append(("tile", t, None, False))
     if tile_only:
@@ -61,7 +68,8 @@ def _apply(s, cand):
     s.block_frames = kind == "block"
     s.jacobi_kernel = "tile" if kind == "block" else kind
     s.tile_size = tile
-    s.fuse_first = fuse
+    s.fuse_first = True if fuse == "two" else fuse
+    s.tile_pair = False if fuse == "two" else "auto"
     s._dirty = True
     s._flush()
     if kind == "window" and tiling is not None:
@@ -86,7 +94,8 @@ def tune(s, frames: int = 3, warmup: int = 1, verbose: bool = False):
     tile_only = (s._ws_theta is not None or s._gscale is not None
                  or getattr(s, "_device_contacts", False))
     judged = s._ws_theta is not None and int(s.warm_start_judge) > 0 and s.substeps >= 2
-    for cand in candidates(s.num_bodies, tile_only=tile_only, fused_only=judged):
+    pair = s.iterations == 2 and s._ws_theta is None
+    for cand in candidates(s.num_bodies, tile_only=tile_only, fused_only=judged, pair=pair):
         _apply(s, cand)
         s.set_state(x0, v0)
         s.step(warmup)

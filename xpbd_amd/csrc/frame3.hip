@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -64,7 +65,8 @@ struct Frame3 {
 struct TileRun {
   float* x[2];
   float* lam[2];
-  float* xp;
+  float* xp;      // x_prev the substep reads (and, without pair launches, writes)
+  float* xp_alt;  // the other x_prev buffer of the pair launches
   float* y;
   float* acc;
   float* dlam;
@@ -125,6 +127,23 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
     rc = xpbd_integrate_to(s, bufs[r.loc], bufs[r.loc], d.v, r.xp, d.gx, d.gy, d.gz, d.h,
                            d.gscale, n);
     if (rc) return rc;
+  } else if (it_n == 2 && d.pair_p1 && r.xp_alt && !dyn) {
+    // both iterations (and the finalize / next integrate) in one launch;
+    // x_prev goes to the other buffer (ring rows read this one), and the
+    // last substep copies it back into d.x_prev
+    const bool last = !integrate_next;
+    float* xp_out = last ? d.x_prev : r.xp_alt;
+    rc = xpbd_jacobi_tile_pair(s, bufs[r.loc], bufs[r.loc ^ 1], r.xp, xp_out, d.v, d.rows,
+                               d.anchors, lams[0], d.pair_ring_start, d.pair_ring1_n,
+                               d.pair_ring_idx, d.pair_p1_start, d.pair_p1, d.pair_p2_start,
+                               d.pair_p2, d.h, d.tile_size, n, d.relax, d.damping, d.max_speed,
+                               d.gx, d.gy, d.gz, d.gscale, d.pair_lmax, d.pair_amax,
+                               d.pair_k1max, last ? 1 : 0);
+    if (rc) return rc;
+    r.loc ^= 1;
+    r.li = 0;
+    if (!last) std::swap(r.xp, r.xp_alt);
+    return 0;
   }
   for (; it < it_n; ++it) {
     const int first = (!warm && it == 0) ? 1 : 0;
@@ -199,13 +218,13 @@ int issue_frame_tile(Frame3* f, hipStream_t s, bool judge) {
   int rc;
   const bool next = d.fuse_first >= 2 && it_n >= 1;
   if (d.fuse_first && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1) || next)) {
-    TileRun r{{d.x, d.x_alt}, {d.lam, d.lam_alt}, d.x_prev, d.ws_y, dyn ? d.acc : nullptr,
-              d.dyn_lam, nullptr};
+    TileRun r{{d.x, d.x_alt}, {d.lam, d.lam_alt}, d.x_prev,
+              warm ? nullptr : d.x_prev_alt, d.ws_y, dyn ? d.acc : nullptr, d.dyn_lam, nullptr};
     judge = judge && warm && d.ws_judge > 0;
     for (int sub = 0; sub < d.substeps; ++sub) {
       if (judge && sub == 0) {
-        TileRun jr{{d.ws_jx, d.ws_jx_alt}, {d.ws_jlam, d.ws_jlam_alt}, d.ws_jxp, d.ws_jy,
-                   dyn ? d.ws_jacc : nullptr, d.ws_jdlam, d.x};
+        TileRun jr{{d.ws_jx, d.ws_jx_alt}, {d.ws_jlam, d.ws_jlam_alt}, d.ws_jxp, nullptr,
+                   d.ws_jy, dyn ? d.ws_jacc : nullptr, d.ws_jdlam, d.x};
         rc = issue_fused_substep(d, s, jr, 0, 1);
         if (rc) return rc;
         rc = xpbd_ws_residual(s, jr.xp, d.rows, d.anchors, jr.lam[jr.li], d.h, d.ws_rsum + 1,

@@ -566,6 +566,229 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
 }
 
 // ---------------------------------------------------------------------------
+// A whole two-iteration substep in one launch (small scenes, where every
+// launch costs its latency rather than its bytes): the first iteration
+// (xpbd_jacobi_tile with first = 1) followed by the second with finalize
+// and, unless LAST, the next substep's integrate (xpbd_jacobi_tile_next), or
+// finalize + cap (xpbd_jacobi_tile with finalize) when LAST.
+//
+// A tile's second iteration reads the first iteration's positions of the
+// particles one row away (its ring 1), so the block also runs the first
+// iteration for them: every row incident to the tile or its ring 1
+// (phase 1, endpoints in the tile, ring 1 or ring 2), projected from the
+// launch's input positions with lambda = 0, summed with the same exact
+// fixed-point accumulation and applied the same way: the ring's first
+// iteration positions are bit for bit those of the separate launch.  Phase
+// 2 then projects every row incident to the tile from those positions and
+// the phase-1 multipliers, and applies, finalizes and integrates the tile.
+// Rows and multipliers in LDS between the phases; the owner tile writes a
+// row's multiplier (as the separate launches do).  x_prev is read from XPin
+// (rows of ring particles read their x_prev) and written to XPout, a second
+// buffer: the next substep's start positions, or (LAST) a copy of XPin, so
+// that the last substep's x_prev ends in the buffer the plan names.
+//
+// Host tables per tile t (xpbd_amd/solver.py: tile_pair_tables):
+//   ring_start[t] .. ring_start[t+1]: global indices of ring 1, then ring 2
+//     (ring1_n[t] of them in ring 1); local index = P + position.
+//   p1_start[t] .. p1_start[t+1]: phase-1 entries int4 (row j, local a,
+//     local b or -1, 0).
+//   p2_start[t] .. p2_start[t+1]: phase-2 entries, the phase-1 position of
+//     the row | (1 << 31) when the tile owns the row.
+// Dynamic LDS (tile_pair_lds_bytes): positions, x_prev and global indices
+// of the local particles (lmax), fixed-point sums of the tile and ring 1
+// (amax), phase-1 rows, row indices and multipliers (k1max).
+//
+// The launch is latency-bound (a 128^2 cloth has 256 tiles of 64): every
+// global load whose address the tables give (the thread's first phase-1
+// rows through their row index, its first phase-2 entries, the local
+// particles' positions and x_prev through ring_idx) is issued before the
+// staging barrier, so the block waits for two load latencies instead of one
+// per phase and pass.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline size_t tile_pair_lds_bytes(int lmax, int amax, int k1max) {
+  return (size_t)amax * 3 * sizeof(unsigned long long) + (size_t)lmax * 2 * sizeof(float4) +
+         (size_t)k1max * sizeof(int4) + (size_t)amax * sizeof(unsigned) +
+         (size_t)lmax * sizeof(int) + (size_t)k1max * (sizeof(int) + sizeof(float));
+}
+
+template <int BLOCK, bool LAST>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
+    const float4* __restrict__ Xin, float4* __restrict__ Xout, const float4* XPin,
+    float4* XPout, float4* __restrict__ V, const int4* __restrict__ R,
+    const float4* __restrict__ ANC, float* __restrict__ LAMout,
+    const int* __restrict__ ring_start, const int* __restrict__ ring1_n,
+    const int* __restrict__ ring_idx, const int* __restrict__ p1_start,
+    const int4* __restrict__ p1, const int* __restrict__ p2_start, const int* __restrict__ p2,
+    float h, int P, int n, float relax, float damping, float max_speed, int do_cap, float gx,
+    float gy, float gz, const float* __restrict__ gscale, int lmax, int amax, int k1max) {
+  constexpr int U1 = 3;  // phase-1 rows per thread loaded ahead (1536 per tile)
+  constexpr int U2 = 2;  // phase-2 entries per thread loaded ahead
+  extern __shared__ unsigned long long lds64[];
+  unsigned long long* sx = lds64;
+  unsigned long long* sy = sx + amax;
+  unsigned long long* sz = sy + amax;
+  float4* spos = (float4*)(sz + amax);
+  float4* sprev = spos + lmax;
+  int4* srow = (int4*)(sprev + lmax);
+  unsigned* cnt = (unsigned*)(srow + k1max);
+  int* gidx = (int*)(cnt + amax);
+  int* sj = gidx + lmax;
+  float* slam = (float*)(sj + k1max);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int p0 = tile * P;
+  if (p0 >= n) return;
+  const int np = min(p0 + P, n) - p0;
+  const int rb = ring_start[tile], re = ring_start[tile + 1];
+  const int nr1 = ring1_n[tile];
+  const int nloc = P + (re - rb);
+  const int nacc = P + nr1;  // tile and ring 1 get sums
+  const int k1b = p1_start[tile], n1 = p1_start[tile + 1] - k1b;
+  const int k2b = p2_start[tile], n2 = p2_start[tile + 1] - k2b;
+  const float inv_dt2 = 1.0f / (h * h);
+  // rows of phase 1 in local form: body index -> local index of endpoint a,
+  // b -> local index of endpoint b (or -1)
+  auto local_row = [&](int k, int& j) {
+    const int4 e = p1[k1b + k];
+    const int4 row = R[e.x];
+    j = e.x;
+    return make_int4((row.x & ~ROW_BODY_MASK) | e.y, e.z, row.z, row.w);
+  };
+  int4 prow[U1];
+  int pj[U1];
+#pragma unroll
+  for (int u = 0; u < U1; ++u) {
+    const int k = threadIdx.x + u * BLOCK;
+    pj[u] = -1;
+    if (k < n1) prow[u] = local_row(k, pj[u]);
+  }
+  int pe[U2];
+#pragma unroll
+  for (int u = 0; u < U2; ++u) {
+    const int k = threadIdx.x + u * BLOCK;
+    pe[u] = k < n2 ? p2[k2b + k] : 0;
+  }
+  for (int k = threadIdx.x; k < nacc; k += BLOCK) {
+    sx[k] = sy[k] = sz[k] = 0ull;
+    cnt[k] = 0u;
+  }
+  for (int l = threadIdx.x; l < nloc; l += BLOCK) {
+    int g = -1;
+    if (l < P) {
+      if (l < np) g = p0 + l;
+    } else {
+      g = ring_idx[rb + (l - P)];
+    }
+    gidx[l] = g;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    spos[l] = g >= 0 ? Xin[g] : z;
+    sprev[l] = g >= 0 ? XPin[g] : z;
+  }
+  __syncthreads();
+  auto pos = [&](int l) { return spos[l]; };
+  auto prev = [&](int l) { return xyz(sprev[l]); };
+  auto add = [&](int l, f3 c, int lim) {
+    if ((unsigned)l >= (unsigned)lim) return;
+    const bool ok = fabsf(c.x) <= kFixLimit && fabsf(c.y) <= kFixLimit && fabsf(c.z) <= kFixLimit;
+    atomicAdd(&sx[l], fix_bits(fminf(fmaxf(c.x, -kFixLimit), kFixLimit)));
+    atomicAdd(&sy[l], fix_bits(fminf(fmaxf(c.y, -kFixLimit), kFixLimit)));
+    atomicAdd(&sz[l], fix_bits(fminf(fmaxf(c.z, -kFixLimit), kFixLimit)));
+    atomicAdd(&cnt[l], ok ? 1u : (kFixPoison | 1u));
+  };
+  // phase 1: the first iteration of the tile and ring 1 (lambda from zero)
+  auto phase1 = [&](int k, int4 lrow, int j) {
+    srow[k] = lrow;
+    sj[k] = j;
+    const Proj p = project_row<true>(
+        pos, prev, [&]() { return ANC ? ANC[j] : make_float4(0.f, 0.f, 0.f, 0.f); }, lrow,
+        0.0f, inv_dt2);
+    slam[k] = p.lam;
+    if (p.na >= 1) add(p.a, p.ca, nacc);
+    if (p.na == 2) add(p.b, p.cb, nacc);
+  };
+#pragma unroll
+  for (int u = 0; u < U1; ++u)
+    if (pj[u] >= 0) phase1(threadIdx.x + u * BLOCK, prow[u], pj[u]);
+  for (int k = threadIdx.x + U1 * BLOCK; k < n1; k += BLOCK) {
+    int j;
+    const int4 lrow = local_row(k, j);
+    phase1(k, lrow, j);
+  }
+  __syncthreads();
+  // the first iteration's apply (k_jacobi_tile without finalize), in LDS
+  auto sums = [&](int o) {
+    const unsigned c = cnt[o];
+    const unsigned adds = c & (kFixPoison - 1u);
+    float4 a = make_float4(0.f, 0.f, 0.f, (float)adds);
+    if (c) a = make_float4(unfix(sx[o], adds), unfix(sy[o], adds), unfix(sz[o], adds), a.w);
+    if (c >= kFixPoison) a.x = __int_as_float(0x7fc00000);
+    return a;
+  };
+  for (int o = threadIdx.x; o < nacc; o += BLOCK) {
+    if (gidx[o] < 0) continue;
+    const float4 x4 = spos[o];
+    const float4 a = sums(o);
+    f3 x = xyz(x4);
+    if (a.w > 0.0f && x4.w > 0.0f) x = x + xyz(a) * (relax / a.w);
+    spos[o] = make_float4(x.x, x.y, x.z, x4.w);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < P; k += BLOCK) {
+    sx[k] = sy[k] = sz[k] = 0ull;
+    cnt[k] = 0u;
+  }
+  __syncthreads();
+  // phase 2: the second iteration of the tile's rows
+  auto phase2 = [&](int e) {
+    const int k1 = e & 0x7fffffff;
+    const int j = sj[k1];
+    const Proj p = project_row<true>(
+        pos, prev, [&]() { return ANC ? ANC[j] : make_float4(0.f, 0.f, 0.f, 0.f); }, srow[k1],
+        slam[k1], inv_dt2);
+    if (e < 0) LAMout[j] = p.lam;
+    if (p.na >= 1) add(p.a, p.ca, P);
+    if (p.na == 2) add(p.b, p.cb, P);
+  };
+#pragma unroll
+  for (int u = 0; u < U2; ++u)
+    if (threadIdx.x + u * BLOCK < n2) phase2(pe[u]);
+  for (int k = threadIdx.x + U2 * BLOCK; k < n2; k += BLOCK) phase2(p2[k2b + k]);
+  __syncthreads();
+  // apply + finalize + cap, then the next substep's integrate
+  // (k_jacobi_tile with FINALIZE and NEXT) or the velocities (LAST)
+  for (int o = threadIdx.x; o < np; o += BLOCK) {
+    const int i = p0 + o;
+    const float4 x4 = spos[o];
+    const float4 a = sums(o);
+    f3 x = xyz(x4);
+    if (a.w > 0.0f && x4.w > 0.0f) x = x + xyz(a) * (relax / a.w);
+    const float4 xp4 = sprev[o];
+    f3 v;
+    if (x4.w == 0.0f) {
+      v = mk3(0.0f, 0.0f, 0.0f);
+    } else {
+      v = ((x - xyz(xp4)) / h) * (1.0f - damping);
+    }
+    if (do_cap) {
+      const float speed = length(v);
+      if (speed > max_speed) v = v * (max_speed / speed);
+    }
+    if (!LAST) {
+      XPout[i] = make_float4(x.x, x.y, x.z, 0.0f);
+      if (x4.w != 0.0f) {
+        f3 g = mk3(gx, gy, gz);
+        if (gscale) g = g * gscale[i];
+        const f3 vn = v + g * h;
+        x = x + vn * h;
+      }
+    } else {
+      if (XPout != XPin) XPout[i] = xp4;
+      V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+    }
+    Xout[i] = make_float4(x.x, x.y, x.z, x4.w);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Learned warm start launch (no reference counterpart; it replaces the zero
 // start of solver.py:431), run once per substep after the network pass
 // (xpbd_ws_net, which writes each row's output y) and before the Jacobi
@@ -1120,6 +1343,46 @@ int launch_tile(bool predict, hipStream_t s, const float* Xin, float* Xout, floa
 // C-ABI entry points
 // ===========================================================================
 extern "C" {
+
+constexpr size_t kPairLdsMax = 64 * 1024;  // dynamic LDS of one pair launch, at most
+
+size_t xpbd_jacobi_tile_pair_lds_bytes(int lmax, int amax, int k1max) {
+  return tile_pair_lds_bytes(lmax, amax, k1max);
+}
+
+int xpbd_jacobi_tile_pair(void* stream, const float* Xin, float* Xout, const float* XPin,
+                          float* XPout, float* V, const int* R, const float* ANC, float* LAMout,
+                          const int* ring_start, const int* ring1_n, const int* ring_idx,
+                          const int* p1_start, const int* p1, const int* p2_start, const int* p2,
+                          float h, int tile_size, int n, float relax, float damping,
+                          float max_speed, float gx, float gy, float gz, const float* gscale,
+                          int lmax, int amax, int k1max, int last) {
+  if (n <= 0) return 0;
+  if (tile_size < 1 || tile_size > XPBD_MAX_TILE || amax < tile_size || lmax < amax ||
+      k1max < 0 || !Xin || !Xout || !XPin || !XPout || !R || !LAMout || !ring_start ||
+      !ring1_n || !ring_idx || !p1_start || !p1 || !p2_start || !p2 || (last && !V) ||
+      Xin == Xout || (XPin == XPout && !last))
+    return (int)hipErrorInvalidValue;
+  const size_t shmem = tile_pair_lds_bytes(lmax, amax, k1max);
+  if (shmem > kPairLdsMax) return (int)hipErrorInvalidValue;
+  const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
+  const int nb = blocks_for(n, tile_size);
+  hipStream_t s = (hipStream_t)stream;
+  if (last)
+    hipLaunchKernelGGL((k_jacobi_tile_pair<kTileBlock, true>), dim3(nb), dim3(kTileBlock), shmem,
+                       s, (const float4*)Xin, (float4*)Xout, (const float4*)XPin, (float4*)XPout, (float4*)V, (const int4*)R,
+                       (const float4*)ANC, LAMout, ring_start, ring1_n, ring_idx, p1_start,
+                       (const int4*)p1, p2_start, p2, h, tile_size, n, relax, damping, max_speed,
+                       do_cap, gx, gy, gz, gscale, lmax, amax, k1max);
+  else
+    hipLaunchKernelGGL((k_jacobi_tile_pair<kTileBlock, false>), dim3(nb), dim3(kTileBlock),
+                       shmem, s, (const float4*)Xin, (float4*)Xout, (const float4*)XPin, (float4*)XPout, (float4*)V,
+                       (const int4*)R, (const float4*)ANC, LAMout, ring_start, ring1_n, ring_idx,
+                       p1_start, (const int4*)p1, p2_start, p2, h, tile_size, n, relax, damping,
+                       max_speed, do_cap, gx, gy, gz, gscale, lmax, amax, k1max);
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
 
 int xpbd_particles_pass(void* stream, float* X, float* V, float* XP, float* ACC, int flags,
                         float relax, float h, float damping, float max_speed, float gx, float gy,

@@ -188,13 +188,17 @@ def choose_jacobi_tiling(rows_sorted: np.ndarray, block: int = 256) -> tuple[int
 
 def morton_order(pos: np.ndarray) -> np.ndarray:
     """Particle order along a 3-D Z-order curve (21 bits per axis over the
-    bounding box of ``pos``): a permutation that keeps spatial neighbours,
-    and therefore the endpoints of a constraint, close in memory."""
+    bounding cube of ``pos``): a permutation that keeps spatial neighbours,
+    and therefore the endpoints of a constraint, close in memory.  One scale
+    for all axes: a flat sheet with 1e-3 of jitter across it keeps that
+    axis in the low bits (scaled per axis, the jitter would fill all 21 bits
+    and order the sheet by noise: 67 % of a 128^2 cloth's rows would cross
+    64-particle tiles instead of 16 %)."""
     n = pos.shape[0]
     if n < 2:
         return np.arange(n, dtype=np.int64)
     lo = pos.min(axis=0)
-    ext = np.maximum(pos.max(axis=0) - lo, 1e-30)
+    ext = max(float((pos.max(axis=0) - lo).max()), 1e-30)
     q = np.clip(((pos - lo) / ext * (1 << 21)).astype(np.int64), 0, (1 << 21) - 1)
     code = np.zeros(n, dtype=np.uint64)
     for bit in range(21):
@@ -224,6 +228,99 @@ def tile_tables(rows_sorted: np.ndarray, n_particles: int, tile: int):
     halo_rows = cross[order].astype(np.int32)
     halo_start = np.searchsorted(ht[order], np.arange(n_tiles + 1), "left").astype(np.int32)
     return starts.astype(np.int32), halo_start, halo_rows
+
+
+def _ranges(starts, lens) -> np.ndarray:
+    """Concatenation of arange(s, s + l) for each (s, l)."""
+    starts = np.asarray(starts, np.int64)
+    lens = np.asarray(lens, np.int64)
+    total = int(lens.sum())
+    if total == 0:
+        return np.zeros(0, np.int64)
+    off = np.repeat(np.cumsum(lens) - lens, lens)
+    return np.repeat(starts, lens) + (np.arange(total, dtype=np.int64) - off)
+
+
+def tile_pair_tables(rows_sorted: np.ndarray, n_particles: int, tile: int):
+    """Tables of the two-iteration tile launch (xpbd_jacobi_tile_pair) for
+    rows in the tile kernel's order.  Per tile: ring 1 (particles one row
+    away from the tile) and ring 2 (one row away from ring 1), the phase-1
+    rows (every row incident to the tile or ring 1, with local endpoint
+    indices: tile particle p -> p - p0, ring k -> tile + its position) and
+    the phase-2 rows (those incident to the tile, as positions in the
+    phase-1 list, bit 31 set where the tile owns the row: its smallest
+    endpoint lies in the tile).  Returns a dict of int32 arrays and the
+    maxima (local particles, tile + ring 1, phase-1 rows) that size the
+    launch's LDS."""
+    m = rows_sorted.shape[0]
+    n_tiles = max(1, (n_particles + tile - 1) // tile)
+    a = (rows_sorted[:, 0] & 0x0FFFFFFF).astype(np.int64)
+    b = rows_sorted[:, 1].astype(np.int64)
+    two = b >= 0
+    ends = np.concatenate([a, b[two]])
+    rid = np.concatenate([np.arange(m, dtype=np.int64), np.nonzero(two)[0]])
+    o = np.argsort(ends, kind="stable")
+    inc_rows = rid[o]
+    inc_start = np.searchsorted(ends[o], np.arange(n_particles + 1, dtype=np.int64), "left")
+    lo = np.where(two, np.minimum(a, b), a)
+
+    def incident(parts):
+        rr = inc_rows[_ranges(inc_start[parts], inc_start[parts + 1] - inc_start[parts])]
+        return np.unique(rr)
+
+    def endpoints(rr):
+        e = np.concatenate([a[rr], b[rr][two[rr]]])
+        return np.unique(e)
+
+    ring_start, ring1_n, ring_idx = [0], [], []
+    p1_start, p1, p2_start, p2 = [0], [], [0], []
+    lmax = amax = k1max = 0
+    for t in range(n_tiles):
+        p0, p1e = t * tile, min((t + 1) * tile, n_particles)
+        T = np.arange(p0, p1e, dtype=np.int64)
+        rows_t = np.unique(inc_rows[inc_start[p0]:inc_start[p1e]])
+        r1 = endpoints(rows_t)
+        r1 = r1[(r1 < p0) | (r1 >= p1e)]
+        rows_r1 = incident(r1)
+        rows_1 = np.union1d(rows_t, rows_r1)
+        r2 = endpoints(rows_r1)
+        r2 = r2[((r2 < p0) | (r2 >= p1e)) & ~np.isin(r2, r1)]
+        loc_g = np.concatenate([T, r1, r2])
+        loc_l = np.concatenate([T - p0, tile + np.arange(len(r1) + len(r2), dtype=np.int64)])
+        so = np.argsort(loc_g)
+        sg, sl = loc_g[so], loc_l[so]
+
+        def local(g):
+            return sl[np.searchsorted(sg, g)]
+
+        la = local(a[rows_1])
+        lb = np.where(two[rows_1], local(np.where(two[rows_1], b[rows_1], p0)), -1)
+        ent = np.zeros((len(rows_1), 4), np.int32)
+        ent[:, 0], ent[:, 1], ent[:, 2] = rows_1, la, lb
+        pos_t = np.searchsorted(rows_1, rows_t)
+        own = (lo[rows_t] // tile) == t
+        e2 = pos_t.astype(np.int64) | (own.astype(np.int64) << 31)
+        ring_idx.append(np.concatenate([r1, r2]))
+        ring1_n.append(len(r1))
+        ring_start.append(ring_start[-1] + len(r1) + len(r2))
+        p1.append(ent)
+        p1_start.append(p1_start[-1] + len(rows_1))
+        p2.append(e2)
+        p2_start.append(p2_start[-1] + len(rows_t))
+        lmax = max(lmax, tile + len(r1) + len(r2))
+        amax = max(amax, tile + len(r1))
+        k1max = max(k1max, len(rows_1))
+    cat = lambda xs, dt: (np.concatenate(xs) if xs else np.zeros(0, dt))  # noqa: E731
+    return {
+        "ring_start": np.asarray(ring_start, np.int32),
+        "ring1_n": np.asarray(ring1_n, np.int32),
+        "ring_idx": cat(ring_idx, np.int64).astype(np.int32),
+        "p1_start": np.asarray(p1_start, np.int32),
+        "p1": cat(p1, np.int32).reshape(-1, 4).astype(np.int32),
+        "p2_start": np.asarray(p2_start, np.int32),
+        "p2": cat(p2, np.int64).astype(np.uint32).view(np.int32),
+        "lmax": lmax, "amax": amax, "k1max": k1max,
+    }
 
 
 class Solver:
@@ -270,6 +367,10 @@ class Solver:
         # else into the substep's first launch; "first" = always the latter;
         # False = a separate integrate pass (identical results)
         self.fuse_first = True
+        # both iterations of a substep in one launch (xpbd_jacobi_tile_pair;
+        # tile mode, 2 iterations): True, False or "auto" (scenes up to
+        # PAIR_AUTO_ROWS rows, where a launch costs latency, not bytes)
+        self.tile_pair = "auto"
         self.block_frames = "auto"    # small scenes: whole frames in one workgroup
         self._block_active = False
         # device-side CONTACT generation (Jacobi tile mode): an extension
@@ -707,6 +808,8 @@ class Solver:
                         else (1, 0))
         self._tile = 0
         self._tile_tabs = None
+        self._rows_dev_order = rows if tile_mode else None   # for tile_pair_tables
+        self._pair_dev = None
         if tile_mode and n_c:
             self._tile = self.tile_size or default_tile_size(n_b)
             self._tile_tabs = tile_tables(rows, n_b, self._tile)
@@ -848,7 +951,37 @@ class Solver:
         return (self._gen, self.solve_mode, self.substeps, self.iterations, self.dt,
                 self.gravity, self.damping, self.jacobi_relax, self.max_speed, self._tiling,
                 self._tile, getattr(self, "_ccap", 0), str(self.fuse_first), self._ws_gen,
-                str(self.block_frames), int(self.warm_start_every), int(self.warm_start_judge))
+                str(self.block_frames), int(self.warm_start_every), int(self.warm_start_judge),
+                str(self.tile_pair))
+
+    # "auto" limit of the two-iteration tile launch: up to this many rows a
+    # launch is latency-bound (the 128^2 cloth, 113 k rows); large scenes
+    # are bandwidth-bound and the ring's extra rows would cost bytes
+    PAIR_AUTO_ROWS = 400_000
+
+    def _pair_tables(self, d, n_b, n_c):
+        """Device tables of the two-iteration launch (tile_pair_tables) for
+        the descriptor, when the plan can use it; None otherwise."""
+        import torch
+
+        if self.tile_pair is False or d.fuse_first != 2 or self.iterations != 2:
+            return None
+        if self.CROWS is not None or self._ws_theta is not None or n_c == 0:
+            return None
+        if self.tile_pair == "auto" and n_c > self.PAIR_AUTO_ROWS:
+            return None
+        if self._pair_dev is None or self._pair_dev[0] != self._tile:
+            t = tile_pair_tables(self._rows_dev_order, n_b, self._tile)
+            dev = self.device
+            up = lambda a: torch.from_numpy(np.ascontiguousarray(a if a.size else np.zeros(4, np.int32))).to(dev)  # noqa: E731
+            tabs = {k: up(t[k]) for k in ("ring_start", "ring1_n", "ring_idx", "p1_start", "p1",
+                                          "p2_start", "p2")}
+            self._pair_dev = (self._tile, tabs, (t["lmax"], t["amax"], t["k1max"]),
+                              torch.empty_like(self.XP))
+        _, tabs, mx, xp_alt = self._pair_dev
+        if int(N.lib().xpbd_jacobi_tile_pair_lds_bytes(*mx)) > 64 * 1024:
+            return None
+        return tabs, mx, xp_alt
 
     # "auto" limit of whole-frame workgroups in Jacobi mode: above it the
     # tile path's 256 CUs beat one CU (cloths, ms per frame, workgroup
@@ -909,6 +1042,15 @@ class Solver:
             if self.CROWS is not None:
                 d.dyn_rows, d.dyn_lam = N.ptr(self.CROWS), N.ptr(self.CLAM)
                 d.dyn_count, d.dyn_cap = N.ptr(self.CCOUNT), self._ccap
+            pair = self._pair_tables(d, n_b, n_c)
+            if pair is not None:
+                tabs, (lmax, amax, k1max), xp_alt = pair
+                d.x_prev_alt = N.ptr(xp_alt)
+                d.pair_ring_start, d.pair_ring1_n = N.ptr(tabs["ring_start"]), N.ptr(tabs["ring1_n"])
+                d.pair_ring_idx = N.ptr(tabs["ring_idx"])
+                d.pair_p1_start, d.pair_p1 = N.ptr(tabs["p1_start"]), N.ptr(tabs["p1"])
+                d.pair_p2_start, d.pair_p2 = N.ptr(tabs["p2_start"]), N.ptr(tabs["p2"])
+                d.pair_lmax, d.pair_amax, d.pair_k1max = lmax, amax, k1max
         if self._ws_theta is not None and d.mode == MODE_JACOBI_TILE:
             import torch
 

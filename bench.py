@@ -105,17 +105,63 @@ def time_kernel_graph(fn, reps: int, rounds: int = 3) -> float:
     return sorted(out)[len(out) // 2]
 
 
-def tile_launcher(N, X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR, h, tile, n):
+def compact_struct(N, cx: dict, pf: bool = True):
+    """xpbd_tile_compact over the tensors of ``cx`` (per-particle floors
+    unless ``pf`` is False; ``cx["general"]``: some row is read whole)."""
+    cs = N.TileCompact()
+    for k in ("dist_end", "gen_end", "ci", "cc", "halo2"):
+        setattr(cs, k, N.ptr(cx[k]))
+    if pf:
+        cs.pf, cs.pflam = N.ptr(cx["pf"]), N.ptr(cx["pflam"])
+    cs.general = int(cx.get("general", 1) or not pf)
+    return cs
+
+
+def tile_launcher(N, X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR, h, tile, n, cx=None):
     """One k_jacobi_tile launch (a full Jacobi iteration) on fixed buffers
-    (ANC None: the scene has no ATTACH rows)."""
+    (ANC None: the scene has no ATTACH rows), over the compact tables when
+    ``cx`` (an xpbd_tile_compact) is given, else over the int4 rows."""
     lib = N.lib()
     args = tuple(None if t is None else N.ptr(t)
-                 for t in (X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS, HR))
+                 for t in (X, X_B, XP, V, R, ANC, LAM, LAM_B, TRS, HS))
+    hr = N.ptr(HR)
 
     def fn():
-        N.check(lib.xpbd_jacobi_tile(N.current_stream(), *args, h, 0, tile, n, 1.0, 0, 0.0,
-                                     40.0, None), "xpbd_jacobi_tile")
+        if cx is not None:
+            N.check(lib.xpbd_jacobi_tile_cx(N.current_stream(), *args, N.ctypes.addressof(cx), h,
+                                            0, tile, n, 1.0, 0, 0.0, 40.0, None),
+                    "xpbd_jacobi_tile_cx")
+        else:
+            N.check(lib.xpbd_jacobi_tile(N.current_stream(), *args, hr, h, 0, tile, n, 1.0, 0,
+                                         0.0, 40.0, None), "xpbd_jacobi_tile")
     return fn
+
+
+def headline_launch(N, s):
+    """The launch that dominates the solver's frame plan, for timing alone:
+    the two-iteration pair launch when the plan uses it (every substep after
+    a frame's first), else one plain iteration (over the plan's compact
+    tables when it has them).  Returns (fn, kernel name, iterations per
+    launch); the launch rewrites the solver's second buffers only."""
+    import numpy as np
+
+    lib = N.lib()
+    h = float(np.float32(s.dt / s.substeps))
+    ts, hs, hr = s._tile_dev
+    if s._plan_pair:
+        _, tabs, (lmax, amax, k1max), xp_alt = s._pair_dev
+        ptrs = [N.ptr(t) for t in (s.X, s.X_B, s.XP, xp_alt, s.V, s.R, s.ANC, s.LAM)]
+        ptrs += [N.ptr(tabs[k]) for k in ("ring_start", "ring1_n", "ring_idx", "p1_start", "p1",
+                                          "p2_start", "p2")]
+
+        def fn():
+            N.check(lib.xpbd_jacobi_tile_pair(
+                N.current_stream(), *ptrs, h, s._tile, s.num_bodies, s.jacobi_relax, s.damping,
+                s.max_speed, *s.gravity, None, lmax, amax, k1max, 0), "xpbd_jacobi_tile_pair")
+        return fn, "k_jacobi_tile_pair", 2
+    fn = tile_launcher(N, s.X, s.X_B, s.XP, s.V, s.R, s.ANC if s._has_attach else None, s.LAM,
+                       s.LAM_B, ts, hs, hr, h, s._tile, s.num_bodies, cx=s._cx_struct)
+    return fn, "k_jacobi_tile", 1
 
 
 def floor_rows_active(X, R) -> int:
@@ -168,7 +214,9 @@ def cpu_baseline(h, s, budget_s: float) -> dict:
 def roofline_scene(N, reps: int, frames: int) -> dict:
     """k_jacobi_tile on 8 x 1024^2 cloths (58.6 M rows, far beyond the
     caches): the sheets hanging (no FLOOR row active, x_prev never read) and
-    lying on the floor (every FLOOR row active with friction)."""
+    lying on the floor (every FLOOR row active with friction).  The launch
+    is the product's, over the compact tables; the same launch over the
+    int4 rows and halo list is timed beside it."""
     import torch
 
     from xpbd_amd.scenes import packed_cloth_device
@@ -177,10 +225,11 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     res, sheets = 1024, 8
     h = float(torch.tensor(1.0 / 60.0 / 15.0, dtype=torch.float32))
 
-    def measure(sc):
+    def measure(sc, compact=True):
+        cs = compact_struct(N, sc["CX"]) if compact else None
         launch = tile_launcher(N, sc["X"], sc["X_B"], sc["XP"], sc["V"], sc["R"], None,
                                sc["LAM"], sc["LAM_B"], sc["TRS"], sc["HS"], sc["HR"], h,
-                               sc["tile"], sc["n_particles"])
+                               sc["tile"], sc["n_particles"], cx=cs)
         return time_kernel_graph(launch, reps)
 
     resting = packed_cloth_device(res, sheets, "cuda:0", height=-0.002)
@@ -188,6 +237,7 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     n, m, nd = resting["n_particles"], resting["n_rows"], resting["n_distance"]
     rest_active = floor_rows_active(resting["X"], resting["R"])
     rest_ms = measure(resting)
+    rest_int4_ms = measure(resting, compact=False)
     rest_bytes = iter_bytes(nd, m - nd, n, rest_active > 0)
     del resting
     torch.cuda.empty_cache()
@@ -195,6 +245,7 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     torch.cuda.synchronize()
     hang_active = floor_rows_active(sc["X"], sc["R"])
     ms = measure(sc)
+    int4_ms = measure(sc, compact=False)
     nbytes = iter_bytes(nd, m - nd, n, hang_active > 0)
     d = N.ParticlesDesc()
     d.x, d.v, d.x_prev, d.acc = (N.ptr(sc["X"]), N.ptr(sc["V"]), N.ptr(sc["XP"]),
@@ -208,6 +259,8 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
     d.x_alt, d.lam_alt = N.ptr(sc["X_B"]), N.ptr(sc["LAM_B"])
     d.tile_row_start, d.halo_start, d.halo_rows = N.ptr(sc["TRS"]), N.ptr(sc["HS"]), N.ptr(sc["HR"])
     d.tile_size = sc["tile"]
+    cs = compact_struct(N, sc["CX"])
+    d.compact = N.ctypes.addressof(cs)
     # fuse_first 2: integrate folded into the previous substep's last launch
     # (product path), 1: into each substep's first launch, 0: separate pass
     frame = {2: None, 1: None, 0: None}
@@ -230,11 +283,17 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": nbytes,
         "floor_rows_active": hang_active,
         "kernel_ms": round(ms, 4),
+        "layout": "compact tables (12-byte DISTANCE rows, per-particle FLOOR rows)",
+        "int4_rows": {"kernel_ms": round(int4_ms, 4),
+                      "frac": round(nbytes / (int4_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "resting": {"workload": "the same sheets lying on the floor (y = -2 mm, x_prev = x): "
                                 "every FLOOR row active with friction",
                     "floor_rows_active": rest_active, "bytes_per_launch": rest_bytes,
                     "kernel_ms": round(rest_ms, 4), "achieved": round(rest_ach, 1),
-                    "frac": round(rest_ach / HBM_PEAK_GBS, 4)},
+                    "frac": round(rest_ach / HBM_PEAK_GBS, 4),
+                    "int4_rows": {"kernel_ms": round(rest_int4_ms, 4),
+                                  "frac": round(rest_bytes / (rest_int4_ms * 1e-3) / 1e9
+                                                / HBM_PEAK_GBS, 4)}},
         "frame_ms": None if frame_ms is None else round(frame_ms, 3),
         "frame_ms_integrate_in_first_launch": None if frame[1] is None else round(frame[1], 3),
         "frame_ms_separate_integrate": None if frame[0] is None else round(frame[0], 3),
@@ -590,11 +649,9 @@ def scene_only(name: str, frames: int, warmup: int) -> dict:
         raise SystemExit(f"unknown scene {name!r}")
     if name == "tile-kernel":
         s.step(warmup)
-        h = float(torch.tensor(s.dt / s.substeps, dtype=torch.float32))
-        ts, hs, hr = s._tile_dev
-        launch = tile_launcher(N, s.X, s.X_B, s.XP, s.V, s.R, None, s.LAM, s.LAM_B, ts, hs, hr,
-                               h, s._tile, s.num_bodies)
-        return {"scene": name, "kernel_ms": time_kernel_graph(launch, frames)}
+        launch, kname, iters = headline_launch(N, s)
+        return {"scene": name, "kernel": kname, "iterations_per_launch": iters,
+                "kernel_ms": time_kernel_graph(launch, frames)}
     return {"scene": name, "ms_per_frame": time_frames(s.step, frames, warmup)}
 
 
@@ -669,24 +726,22 @@ def main() -> int:
 
     roof = None
     if args.mode == "jacobi" and s._tile_dev is not None and s.jacobi_kernel == "tile":
-        h = float(torch.tensor(s.dt / s.substeps, dtype=torch.float32))
-        ts, hs, hr = s._tile_dev
-        launch = tile_launcher(N, s.X, s.X_B, s.XP, s.V, s.R, s.ANC if s._has_attach else None,
-                               s.LAM, s.LAM_B, ts, hs, hr,
-                               h, s._tile, s.num_bodies)
+        launch, kname, iters = headline_launch(N, s)
         x_keep, v_keep = s.positions(), s.velocities()
         active = floor_rows_active(s.X, s.R)
         kms = time_kernel_graph(launch, args.kernel_reps)
         s.set_state(x_keep, v_keep)
-        nbytes = iter_bytes(n_dist, n_floor, s.num_bodies, active > 0)
+        # a pair launch runs two iterations (its fused finalize and next
+        # integrate are not counted: a lower bound on its bytes)
+        nbytes = iters * iter_bytes(n_dist, n_floor, s.num_bodies, active > 0)
         ach = nbytes / (kms * 1e-3) / 1e9
-        traffic = load_traffic("k_jacobi_tile")
+        traffic = load_traffic(kname)
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_frac": traffic_frac(traffic, kms),
-                "kernel": "k_jacobi_tile", "kernel_ms": round(kms, 5),
+                "kernel": kname, "iterations_per_launch": iters, "kernel_ms": round(kms, 5),
                 "bytes_per_launch": nbytes, "floor_rows_active": active, "tile": s._tile}
-        print(f"[bench] jacobi tile kernel {kms * 1e3:.2f} us, {ach:.1f} GB/s", file=sys.stderr,
+        print(f"[bench] {kname} {kms * 1e3:.2f} us, {ach:.1f} GB/s", file=sys.stderr,
               flush=True)
 
     if args.calib_copy:

@@ -128,6 +128,65 @@ int xpbd_jacobi_tile_next(void* stream, const float* Xin, float* Xout, float* XP
                           int tile_size, int n_particles, float relax, float damping,
                           float max_speed, float* ACC);
 
+/* Compact description of the rows of the fused tile launches (built by
+ * xpbd_amd/solver.py compact_tile_tables; no reference counterpart: it is
+ * the device layout of the same rows).  The owned rows of tile t,
+ * [tile_row_start[t], tile_row_start[t+1]), are ordered in three segments:
+ *   [tile_row_start[t], dist_end[t])  DISTANCE rows stored compactly: ci
+ *       int2 {(lo - t*tile_size) | (hi - t*tile_size) << 10, bits of rest}
+ *       and cc float compliance, lo / hi the smaller / larger endpoint
+ *       (hi - t*tile_size < 2^22; tile_size <= 1024).  Projecting (lo, hi)
+ *       gives bit for bit the corrections of (a, b) in either order.
+ *   [dist_end[t], gen_end[t])  every other row, read from R (int4).
+ *   [gen_end[t], tile_row_start[t+1])  FLOOR rows of particles with exactly
+ *       one FLOOR row, in particle order.  With pf set they are projected per
+ *       particle by the apply pass from pf (float2 {floor_y, mu}, floor_y =
+ *       -inf for a particle without one), their multipliers kept in place in
+ *       pflam (float[n_particles]; the LAM entries of these rows are then
+ *       neither read nor written); with pf NULL they are read from R like
+ *       the general rows.
+ * halo2 (int2, one per halo row, same order as halo_rows) holds {row, base}
+ * with base = owner tile * tile_size for a compact row and -1 for a row read
+ * from R.  ci and cc hold entries for every row index (only the compact
+ * ones are read).  general must be nonzero when any row is read from R: a
+ * nonempty [dist_end[t], gen_end[t]) segment, a halo entry with base -1, or,
+ * with pf NULL, a nonempty FLOOR segment (with general = 0 such rows are
+ * skipped).  tile_size >= 64. */
+typedef struct xpbd_tile_compact {
+  const int* dist_end;
+  const int* gen_end;
+  const int* ci;
+  const float* cc;
+  const int* halo2;
+  const float* pf;
+  float* pflam;
+  int general;
+} xpbd_tile_compact;
+
+/* xpbd_jacobi_tile, xpbd_jacobi_tile_first and xpbd_jacobi_tile_next over
+ * the compact tables (cx replaces halo_rows): same results bit for bit, the
+ * FLOOR multipliers in cx->pflam when cx->pf is set.  The launch of the
+ * plain zero-start Jacobi plan (solver.py:427-467). */
+int xpbd_jacobi_tile_cx(void* stream, const float* Xin, float* Xout, const float* XP, float* V,
+                        const int* R, const float* ANC, const float* LAMin, float* LAMout,
+                        const int* tile_row_start, const int* halo_start,
+                        const xpbd_tile_compact* cx, float h, int first, int tile_size,
+                        int n_particles, float relax, int finalize, float damping,
+                        float max_speed, float* ACC);
+int xpbd_jacobi_tile_first_cx(void* stream, const float* X, float* Xout, float* XP,
+                              const float* V, const int* R, const float* ANC, float* LAMout,
+                              const int* tile_row_start, const int* halo_start,
+                              const xpbd_tile_compact* cx, float h, float gx, float gy, float gz,
+                              const float* gscale, int tile_size, int n_particles, float relax,
+                              float* ACC);
+int xpbd_jacobi_tile_next_cx(void* stream, const float* Xin, float* Xout, float* XP,
+                             const int* R, const float* ANC, const float* LAMin, float* LAMout,
+                             const int* tile_row_start, const int* halo_start,
+                             const xpbd_tile_compact* cx, float h, int first, float gx,
+                             float gy, float gz, const float* gscale, int tile_size,
+                             int n_particles, float relax, float damping, float max_speed,
+                             float* ACC);
+
 /* kernels.py:49-73 `integrate` reading Xin and writing Xout (may alias);
  * gscale (optional, float[n]) scales gravity per particle (batched scenes). */
 int xpbd_integrate_to(void* stream, const float* Xin, float* Xout, float* V, float* XP, float gx,
@@ -197,6 +256,17 @@ int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP
                           const float* Y, float h, float gx, float gy, float gz,
                           const float* gscale, int tile_size, int n_particles, float relax,
                           int predict, const int* ws_on, int invert);
+/* xpbd_jacobi_tile_warm over the compact tables (cx replaces halo_rows):
+ * the same lambda0 and positions; with cx->pf the single FLOOR rows' lambda0
+ * goes to cx->pflam (zero when the start is off), where the following
+ * xpbd_jacobi_tile_cx launches read it. */
+int xpbd_jacobi_tile_warm_cx(void* stream, const float* Xin, float* Xout, float* XP,
+                             const float* V, const int* R, const float* ANC, float* LAMout,
+                             const int* tile_row_start, const int* halo_start,
+                             const xpbd_tile_compact* cx, const float* Y, float h, float gx,
+                             float gy, float gz, const float* gscale, int tile_size,
+                             int n_particles, float relax, int predict, const int* ws_on,
+                             int invert);
 /* Largest particle count of xpbd_frame_block (LDS-bound). */
 int xpbd_frame_block_capacity(int gs);
 
@@ -305,6 +375,11 @@ typedef struct xpbd_particles_desc {
   const int* pair_p2_start;
   const int* pair_p2;
   int pair_lmax, pair_amax, pair_k1max;
+  /* Compact tile tables (tile mode, tile_size >= 64, optional; copied at
+   * create): the plain and warm-start tile launches become their _cx forms.
+   * pf may be set only without the pair launch, which reads FLOOR
+   * multipliers from lam. */
+  const xpbd_tile_compact* compact;
 } xpbd_particles_desc;
 
 /* A whole two-iteration substep of the fused tile plan in one launch, for

@@ -545,3 +545,83 @@ def test_tile_pair_tables_match_brute_force(tile):
         amax = max(amax, tile + len(r1))
         k1max = max(k1max, len(rows_t | rows_r1))
     assert (t["lmax"], t["amax"], t["k1max"]) == (lmax, amax, k1max)
+
+
+def _tile_rows(s, tile):
+    """Rows of a host scene in the fused kernel's order, with its tables."""
+    from xpbd_amd.solver import jacobi_tile_order, morton_order, pack_rows, tile_tables
+
+    h = s.host_arrays()
+    perm = morton_order(h["x"].astype(np.float64))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    ra = inv[h["a"]]
+    rb = np.where(h["b"] >= 0, inv[np.maximum(h["b"], 0)], -1)
+    rows, _ = pack_rows(h["type"], ra, rb, h["rest"], h["compliance"], h["anchor"], h["mu"])
+    rows = rows[jacobi_tile_order(ra, rb, h["type"], tile, s.num_bodies)]
+    return rows, tile_tables(rows, s.num_bodies, tile)
+
+
+@pytest.mark.parametrize("tile", [32, 64, 1024])
+def test_compact_tile_tables_describe_the_same_rows(tile):
+    """Each tile's rows split into compact DISTANCE rows (their (lo, hi)
+    decode back to the row's endpoints, rest and compliance are the row's),
+    rows read whole (ATTACH, and the FLOOR rows of a particle with two) and
+    single FLOOR rows, whose floor and friction land in the per-particle
+    table; halo entries name the owner tile's base for compact rows only."""
+    from xpbd_amd import Body, Solver
+    from xpbd_amd.scenes import cloth_sheet
+    from xpbd_amd.solver import compact_tile_tables
+
+    s, _ = cloth_sheet(Solver, res=20)
+    for i in range(0, 400, 37):
+        s.add_attach(Body(i), (0.1 * i, 1.0, 0.0), compliance=1e-4, rest=0.05)
+    s.add_floor_contact(Body(5), floor_y=-0.2, friction=0.3)      # particle 5: two FLOOR rows
+    rows, (ts, hs, hr) = _tile_rows(s, tile)
+    n = s.num_bodies
+    c = compact_tile_tables(rows, n, tile, ts, hr)
+    t = rows[:, 0] >> 28
+    a = (rows[:, 0] & 0x0FFFFFFF).astype(np.int64)
+    b = rows[:, 1].astype(np.int64)
+    lo = np.where(b >= 0, np.minimum(a, b), a)
+    hi = np.where(b >= 0, np.maximum(a, b), a)
+    nfl = np.bincount(a[t == 2], minlength=n)
+    seen_floor = []
+    for k in range(len(ts) - 1):
+        r0, r1, r2, r3 = ts[k], c["dist_end"][k], c["gen_end"][k], ts[k + 1]
+        assert r0 <= r1 <= r2 <= r3
+        base = k * tile
+        for j in range(r0, r1):
+            q = int(c["ci"][j, 0]) & 0xFFFFFFFF
+            assert t[j] == 0
+            assert (base + (q & 1023), base + (q >> 10)) == (lo[j], hi[j])
+            assert c["ci"][j, 1] == rows[j, 2]
+            assert c["cc"][j].view(np.int32) == rows[j, 3]
+        for j in range(r1, r2):
+            assert t[j] != 0 and not (t[j] == 2 and nfl[a[j]] == 1)
+        for j in range(r2, r3):
+            assert t[j] == 2 and nfl[a[j]] == 1
+            assert c["pf"][a[j], 0].view(np.int32) == rows[j, 2]
+            assert c["pf"][a[j], 1].view(np.int32) == rows[j, 3]
+            seen_floor.append(a[j])
+    assert sorted(seen_floor) == sorted(np.nonzero(nfl == 1)[0].tolist())
+    assert np.isneginf(c["pf"][nfl != 1, 0]).all()
+    assert (c["pf_part"] == a[c["pf_rows"]]).all()
+    assert (c["halo2"][:, 0] == hr).all()
+    own = (lo[hr] // tile) * tile
+    assert (c["halo2"][:, 1] == np.where(t[hr] == 0, own, -1)).all()
+    assert np.count_nonzero(t == 1) > 0 and nfl[nfl > 0].max() == 2
+
+
+def test_compact_rows_need_a_22_bit_span():
+    """A DISTANCE row whose larger endpoint lies 2^22 or more particles past
+    its tile's base is read whole; one just inside the span is compact."""
+    from xpbd_amd.solver import CLS_COMPACT, CLS_GENERAL, tile_row_class
+
+    tile = 512
+    a = np.array([0, 1, 700, 0])
+    b = np.array([(1 << 22) - 1, 1 << 22, 512 + (1 << 22) - 1, -1])
+    t = np.array([0, 0, 0, 1])
+    cls = tile_row_class(a, b, t, tile, 5_000_000)
+    assert cls.tolist() == [CLS_COMPACT, CLS_GENERAL, CLS_COMPACT, CLS_GENERAL]
+    assert (tile_row_class(a, b, t, 2048, 5_000_000) == CLS_GENERAL).all()

@@ -47,7 +47,15 @@ class ParticlesDesc(ctypes.Structure):
         ("pair_ring_idx", c_void_p), ("pair_p1_start", c_void_p), ("pair_p1", c_void_p),
         ("pair_p2_start", c_void_p), ("pair_p2", c_void_p),
         ("pair_lmax", c_int), ("pair_amax", c_int), ("pair_k1max", c_int),
+        ("compact", c_void_p),
     ]
+
+
+class TileCompact(ctypes.Structure):
+    """Mirror of ``xpbd_tile_compact`` (include/xpbd.h)."""
+
+    _fields_ = [(k, c_void_p) for k in ("dist_end", "gen_end", "ci", "cc", "halo2", "pf",
+                                        "pflam")] + [("general", c_int)]
 
 
 class RigidDesc(ctypes.Structure):
@@ -91,6 +99,16 @@ _SIGS = {
     "xpbd_jacobi_tile_next": (c_int, [c_void_p] * 11 + [c_float, c_int, c_float, c_float, c_float,
                                                         c_void_p, c_int, c_int, c_float, c_float,
                                                         c_float, c_void_p]),
+    # the same launches over compact tables (a TileCompact address in place
+    # of halo_rows)
+    "xpbd_jacobi_tile_cx": (c_int, [c_void_p] * 12 + [c_float, c_int, c_int, c_int, c_float,
+                                                       c_int, c_float, c_float, c_void_p]),
+    "xpbd_jacobi_tile_first_cx": (c_int, [c_void_p] * 11 + [c_float, c_float, c_float, c_float,
+                                                            c_void_p, c_int, c_int, c_float,
+                                                            c_void_p]),
+    "xpbd_jacobi_tile_next_cx": (c_int, [c_void_p] * 11 + [c_float, c_int, c_float, c_float,
+                                                           c_float, c_void_p, c_int, c_int,
+                                                           c_float, c_float, c_float, c_void_p]),
     "xpbd_integrate_to": (c_int, [c_void_p] * 5 + [c_float, c_float, c_float, c_float, c_void_p,
                                                    c_int]),
     "xpbd_solve_jacobi_dyn": (c_int, [c_void_p] * 6 + [c_float, c_int, c_void_p, c_int]),
@@ -156,6 +174,8 @@ _SIGS = {
     "xpbd_ws_features": (c_int, [_P] * 5 + [_F, _P, _I]),
     "xpbd_ws_net": (c_int, [_P] * 8 + [_F, _F, _F, _F, _P, _I, _I, _P, _I]),
     "xpbd_jacobi_tile_warm": (c_int, [_P] * 12 + [_F, _F, _F, _F, _P, _I, _I, _F, _I, _P, _I]),
+    "xpbd_jacobi_tile_warm_cx": (c_int, [_P] * 12 + [_F, _F, _F, _F, _P, _I, _I, _F, _I, _P,
+                                                     _I]),
     "xpbd_ws_decide": (c_int, [_P, _P, _P, _P]),
     "xpbd_ws_residual": (c_int, [_P] * 5 + [_F, _P, _I]),
     "xpbd_build_info": (ctypes.c_char_p, []),

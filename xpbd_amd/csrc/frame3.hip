@@ -26,6 +26,7 @@ enum : int { F_APPLY = 1, F_FINALIZE = 2, F_CAP = 4, F_INTEGRATE = 8 };
 
 struct Frame3 {
   xpbd_particles_desc d;
+  xpbd_tile_compact cx;  // copy of *desc.compact (d.compact points here)
   std::vector<int> color_starts;
   // captured frames: [0] plain, [1] with the judge of the learned start
   hipGraph_t graph[2] = {nullptr, nullptr};
@@ -75,6 +76,42 @@ struct TileRun {
   int li = 0;    // lam[li] holds the multipliers after the last launch
 };
 
+// The plain tile launches of a plan, over the compact tables when the plan
+// has them (identical results).
+int tile_launch(const xpbd_particles_desc& d, hipStream_t s, const float* xin, float* xout,
+                float* xp, const float* lin, float* lout, int first, int finalize, float* acc) {
+  if (d.compact)
+    return xpbd_jacobi_tile_cx(s, xin, xout, xp, d.v, d.rows, d.anchors, lin, lout,
+                               d.tile_row_start, d.halo_start, d.compact, d.h, first, d.tile_size,
+                               d.n_particles, d.relax, finalize, d.damping, d.max_speed, acc);
+  return xpbd_jacobi_tile(s, xin, xout, xp, d.v, d.rows, d.anchors, lin, lout, d.tile_row_start,
+                          d.halo_start, d.halo_rows, d.h, first, d.tile_size, d.n_particles,
+                          d.relax, finalize, d.damping, d.max_speed, acc);
+}
+
+int tile_launch_first(const xpbd_particles_desc& d, hipStream_t s, const float* xin, float* xout,
+                      float* xp, float* lout, float* acc) {
+  if (d.compact)
+    return xpbd_jacobi_tile_first_cx(s, xin, xout, xp, d.v, d.rows, d.anchors, lout,
+                                     d.tile_row_start, d.halo_start, d.compact, d.h, d.gx, d.gy,
+                                     d.gz, d.gscale, d.tile_size, d.n_particles, d.relax, acc);
+  return xpbd_jacobi_tile_first(s, xin, xout, xp, d.v, d.rows, d.anchors, lout, d.tile_row_start,
+                                d.halo_start, d.halo_rows, d.h, d.gx, d.gy, d.gz, d.gscale,
+                                d.tile_size, d.n_particles, d.relax, acc);
+}
+
+int tile_launch_next(const xpbd_particles_desc& d, hipStream_t s, const float* xin, float* xout,
+                     float* xp, const float* lin, float* lout, int first, float* acc) {
+  if (d.compact)
+    return xpbd_jacobi_tile_next_cx(s, xin, xout, xp, d.rows, d.anchors, lin, lout,
+                                    d.tile_row_start, d.halo_start, d.compact, d.h, first, d.gx,
+                                    d.gy, d.gz, d.gscale, d.tile_size, d.n_particles, d.relax,
+                                    d.damping, d.max_speed, acc);
+  return xpbd_jacobi_tile_next(s, xin, xout, xp, d.rows, d.anchors, lin, lout, d.tile_row_start,
+                               d.halo_start, d.halo_rows, d.h, first, d.gx, d.gy, d.gz, d.gscale,
+                               d.tile_size, d.n_particles, d.relax, d.damping, d.max_speed, acc);
+}
+
 // One substep of the fused tile plan (see issue_frame_tile) on stream s;
 // invert != 0 runs the start the solver is not using (the judge).
 int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r, int sub,
@@ -101,10 +138,16 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
                        d.gz, d.gscale, d.n_rows, predicted ? 0 : 1, d.ws_on, invert);
       if (rc) return rc;
     }
-    rc = xpbd_jacobi_tile_warm(s, in(), bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors, lams[1],
-                               d.tile_row_start, d.halo_start, d.halo_rows, r.y, d.h, d.gx, d.gy,
-                               d.gz, d.gscale, d.tile_size, n, d.relax, predicted ? 0 : 1,
-                               d.ws_on, invert);
+    if (d.compact)
+      rc = xpbd_jacobi_tile_warm_cx(s, in(), bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors,
+                                    lams[1], d.tile_row_start, d.halo_start, d.compact, r.y, d.h,
+                                    d.gx, d.gy, d.gz, d.gscale, d.tile_size, n, d.relax,
+                                    predicted ? 0 : 1, d.ws_on, invert);
+    else
+      rc = xpbd_jacobi_tile_warm(s, in(), bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors,
+                                 lams[1], d.tile_row_start, d.halo_start, d.halo_rows, r.y, d.h,
+                                 d.gx, d.gy, d.gz, d.gscale, d.tile_size, n, d.relax,
+                                 predicted ? 0 : 1, d.ws_on, invert);
     if (rc) return rc;
     r.src0 = nullptr;
     r.loc ^= 1;
@@ -115,9 +158,7 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
                                        d.gz, d.gscale, d.dyn_count, d.dyn_cap);
       if (rc) return rc;
     }
-    rc = xpbd_jacobi_tile_first(s, in(), bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors, lams[1],
-                                d.tile_row_start, d.halo_start, d.halo_rows, d.h, d.gx, d.gy,
-                                d.gz, d.gscale, d.tile_size, n, d.relax, r.acc);
+    rc = tile_launch_first(d, s, in(), bufs[r.loc ^ 1], r.xp, lams[1], r.acc);
     if (rc) return rc;
     r.src0 = nullptr;
     r.loc ^= 1;
@@ -153,15 +194,11 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
       if (rc) return rc;
     }
     if (it + 1 == it_n && integrate_next) {
-      rc = xpbd_jacobi_tile_next(s, bufs[r.loc], bufs[r.loc ^ 1], r.xp, d.rows, d.anchors,
-                                 lams[r.li], lams[r.li ^ 1], d.tile_row_start, d.halo_start,
-                                 d.halo_rows, d.h, first, d.gx, d.gy, d.gz, d.gscale,
-                                 d.tile_size, n, d.relax, d.damping, d.max_speed, r.acc);
+      rc = tile_launch_next(d, s, bufs[r.loc], bufs[r.loc ^ 1], r.xp, lams[r.li],
+                            lams[r.li ^ 1], first, r.acc);
     } else {
-      rc = xpbd_jacobi_tile(s, bufs[r.loc], bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors,
-                            lams[r.li], lams[r.li ^ 1], d.tile_row_start, d.halo_start,
-                            d.halo_rows, d.h, first, d.tile_size, n, d.relax, it + 1 == it_n,
-                            d.damping, d.max_speed, r.acc);
+      rc = tile_launch(d, s, bufs[r.loc], bufs[r.loc ^ 1], r.xp, lams[r.li], lams[r.li ^ 1],
+                       first, it + 1 == it_n ? 1 : 0, r.acc);
     }
     if (rc) return rc;
     r.loc ^= 1;
@@ -260,10 +297,16 @@ int issue_frame_tile(Frame3* f, hipStream_t s, bool judge) {
                          d.h, 0.0f, 0.0f, 0.0f, nullptr, d.n_rows, 0, d.ws_on, 0);
         if (rc) return rc;
       }
-      rc = xpbd_jacobi_tile_warm(s, bufs[cur], bufs[cur ^ 1], d.x_prev, d.v, d.rows, d.anchors,
-                                 lams[1], d.tile_row_start, d.halo_start, d.halo_rows, d.ws_y,
-                                 d.h, 0.0f, 0.0f, 0.0f, nullptr, d.tile_size, n, d.relax, 0,
-                                 d.ws_on, 0);
+      if (d.compact)
+        rc = xpbd_jacobi_tile_warm_cx(s, bufs[cur], bufs[cur ^ 1], d.x_prev, d.v, d.rows,
+                                      d.anchors, lams[1], d.tile_row_start, d.halo_start,
+                                      d.compact, d.ws_y, d.h, 0.0f, 0.0f, 0.0f, nullptr,
+                                      d.tile_size, n, d.relax, 0, d.ws_on, 0);
+      else
+        rc = xpbd_jacobi_tile_warm(s, bufs[cur], bufs[cur ^ 1], d.x_prev, d.v, d.rows,
+                                   d.anchors, lams[1], d.tile_row_start, d.halo_start,
+                                   d.halo_rows, d.ws_y, d.h, 0.0f, 0.0f, 0.0f, nullptr,
+                                   d.tile_size, n, d.relax, 0, d.ws_on, 0);
       if (rc) return rc;
       cur ^= 1;
       li = 1;
@@ -275,10 +318,8 @@ int issue_frame_tile(Frame3* f, hipStream_t s, bool judge) {
                                    it == 0, d.dyn_count, d.dyn_cap);
         if (rc) return rc;
       }
-      rc = xpbd_jacobi_tile(s, bufs[cur], bufs[nxt], d.x_prev, d.v, d.rows, d.anchors, lams[li],
-                            lams[li ^ 1], d.tile_row_start, d.halo_start, d.halo_rows, d.h,
-                            it == 0 && !warm, d.tile_size, n, d.relax, it + 1 == it_n,
-                            d.damping, d.max_speed, dyn ? d.acc : nullptr);
+      rc = tile_launch(d, s, bufs[cur], bufs[nxt], d.x_prev, lams[li], lams[li ^ 1],
+                       it == 0 && !warm ? 1 : 0, it + 1 == it_n ? 1 : 0, dyn ? d.acc : nullptr);
       if (rc) return rc;
       cur = nxt;
       li ^= 1;
@@ -412,9 +453,20 @@ int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle) {
        !desc->ws_jlam || !desc->ws_jlam_alt || !desc->ws_jy ||
        (desc->dyn_cap > 0 && (!desc->ws_jacc || !desc->ws_jdlam))))
     return (int)hipErrorInvalidValue;  // the judge runs in the fused plan's shape only
+  if (desc->compact &&
+      (desc->mode != XPBD_MODE_JACOBI_TILE || desc->tile_size < 64 || !desc->compact->dist_end ||
+       !desc->compact->ci ||
+       !desc->compact->cc || !desc->compact->halo2 ||
+       (desc->compact->pf &&
+        (!desc->compact->gen_end || !desc->compact->pflam || desc->pair_p1))))
+    return (int)hipErrorInvalidValue;  // per-particle floors: no kernel reading lam's FLOOR rows
   Frame3* f = new (std::nothrow) Frame3();
   if (!f) return (int)hipErrorOutOfMemory;
   f->d = *desc;
+  if (desc->compact) {
+    f->cx = *desc->compact;
+    f->d.compact = &f->cx;
+  }
   if (desc->mode == XPBD_MODE_GS && desc->n_rows > 0)
     f->color_starts.assign(desc->color_starts, desc->color_starts + desc->num_colors + 1);
   f->d.color_starts = nullptr;

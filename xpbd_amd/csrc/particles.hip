@@ -566,6 +566,243 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile(
 }
 
 // ---------------------------------------------------------------------------
+// The fused iteration over the compact tables (xpbd_tile_compact).
+//
+// A tile's own DISTANCE rows come as 12 bytes (a tile-relative endpoint pair
+// + rest in an int2, the compliance in a float: 4 bytes fewer than the int4
+// row), its other rows from R (GEN), and, with PF, the FLOOR row of each
+// particle that has exactly one is projected by the apply pass from 8 bytes
+// per particle: its correction joins the exact fixed-point sum like any
+// other row's, so the iteration is the int4 kernel's bit for bit; its
+// multiplier is updated in place, read and written only where the row is
+// active (or reset in a zero-start launch).  On 8 x 1024^2 hanging cloths
+// these cut the bytes a launch moves by ~17 %.
+//
+// The compact rows' loads are unconditional, their index clamped into the
+// segment (a clamped load re-reads one cached row), and they are issued one
+// pass ahead: with no branch around them the compiler can count the loads in
+// flight and wait only for the pass it projects (with the loads under a
+// branch it waits for all of them, the next pass's included).  Every option
+// is a template argument (the zero start and the streaming stores
+// included): the same kernel with a runtime flag tested in the apply pass
+// measured 7 % slower on the 8 x 1024^2 cloths (0.323 against 0.305 ms per
+// launch).
+// ---------------------------------------------------------------------------
+template <int MODE, bool GEN, bool PF, bool STREAM>
+__global__ __launch_bounds__(512) void k_jacobi_tile_cx(
+    const float4* __restrict__ Xin, float4* __restrict__ Xout, float4* __restrict__ XP,
+    float4* __restrict__ V, const int4* __restrict__ R, const float4* __restrict__ ANC,
+    const float* __restrict__ LAMin, float* __restrict__ LAMout,
+    const int* __restrict__ tile_row_start, const int* __restrict__ halo_start,
+    const int* __restrict__ dist_end, const int* __restrict__ gen_end,
+    const int2* __restrict__ CI, const float* __restrict__ CC, const int2* __restrict__ H2,
+    const float2* __restrict__ PFT, float* __restrict__ PFLAM, float h, int P, int n,
+    float relax, float damping, float max_speed, int do_cap, float4* __restrict__ ACC, float gx,
+    float gy, float gz, const float* __restrict__ gscale) {
+  constexpr int BLOCK = 512;
+  constexpr int U = 2;
+  constexpr bool PREDICT = MODE & TM_PREDICT, FINALIZE = MODE & TM_FINALIZE;
+  constexpr bool HAS_ACC = MODE & TM_ACC, HAS_ANCHOR = MODE & TM_ANCHOR;
+  constexpr bool NEXT = MODE & TM_NEXT;
+  constexpr bool ZERO_LAM = PREDICT || (MODE & TM_FIRST);
+  static_assert(!NEXT || (FINALIZE && !PREDICT), "NEXT extends a finalizing launch");
+  static_assert(!HAS_ANCHOR || GEN, "ATTACH rows are read from R");
+  extern __shared__ unsigned long long lds64[];
+  unsigned long long* sx = lds64;
+  unsigned long long* sy = lds64 + P;
+  unsigned long long* sz = lds64 + 2 * P;
+  unsigned* cnt = (unsigned*)(lds64 + 3 * P);
+  float4* spos = (float4*)(lds64 + tile_lds_pos_offset(P) / sizeof(unsigned long long));
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int p0 = tile * P;
+  if (p0 >= n) return;
+  const int p1 = min(p0 + P, n);
+  const int tid = threadIdx.x;
+  const float inv_dt2 = 1.0f / (h * h);
+  const int rb = tile_row_start[tile];
+  const int rd = dist_end[tile];
+  const int hb = halo_start[tile], he = halo_start[tile + 1];
+  const int n_cmp = rd - rb;
+  const int last = max(rd - 1, 0);  // a valid row index even for an empty segment
+  int2 q[U];
+  float cm[U], lm[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = min(rb + tid + u * BLOCK, last);
+    q[u] = CI[j];
+    cm[u] = CC[j];
+    lm[u] = ZERO_LAM ? 0.0f : LAMin[j];
+  }
+  auto predict = [&](float4 x4, int i) -> float4 {
+    if (!PREDICT || x4.w == 0.0f) return x4;
+    f3 g = mk3(gx, gy, gz);
+    if (gscale) g = g * gscale[i];
+    const f3 v = xyz(V[i]) + g * h;
+    const f3 x = xyz(x4) + v * h;
+    return make_float4(x.x, x.y, x.z, x4.w);
+  };
+  for (int k = tid; k < 3 * P; k += BLOCK) lds64[k] = 0ull;
+  for (int k = tid; k < P; k += BLOCK) cnt[k] = 0u;
+  for (int i = p0 + tid; i < p1; i += BLOCK) {
+    const float4 x0 = Xin[i];
+    spos[i - p0] = predict(x0, i);
+    if (PREDICT) XP[i] = make_float4(x0.x, x0.y, x0.z, 0.0f);
+  }
+  __syncthreads();
+  auto pos = [&](int i) -> float4 {
+    const unsigned off = (unsigned)(i - p0);
+    if (off < (unsigned)(p1 - p0)) return spos[off];
+    return predict(Xin[i], i);
+  };
+  auto prev = [&](int i) { return PREDICT ? xyz(Xin[i]) : xyz(XP[i]); };
+  auto add = [&](int i, f3 c) {
+    const unsigned off = (unsigned)(i - p0);
+    if (off >= (unsigned)P) return;
+    const bool ok = fabsf(c.x) <= kFixLimit && fabsf(c.y) <= kFixLimit && fabsf(c.z) <= kFixLimit;
+    atomicAdd(&sx[off], fix_bits(fminf(fmaxf(c.x, -kFixLimit), kFixLimit)));
+    atomicAdd(&sy[off], fix_bits(fminf(fmaxf(c.y, -kFixLimit), kFixLimit)));
+    atomicAdd(&sz[off], fix_bits(fminf(fmaxf(c.z, -kFixLimit), kFixLimit)));
+    atomicAdd(&cnt[off], ok ? 1u : (kFixPoison | 1u));
+  };
+  auto no_anchor = [] { return make_float4(0.f, 0.f, 0.f, 0.f); };
+  // a compact row of the tile based at `base`: DISTANCE, no type bits
+  auto compact = [&](int2 qq, float c, float l, int base) {
+    const unsigned v = (unsigned)qq.x;
+    const int4 row = make_int4(base + (int)(v & 1023u), base + (int)(v >> 10), qq.y,
+                               __float_as_int(c));
+    return project_row<true>(pos, prev, no_anchor, row, l, inv_dt2);
+  };
+  for (int k0 = tid; k0 < n_cmp; k0 += BLOCK * U) {
+    int2 nq[U];
+    float ncm[U], nlm[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = min(rb + k0 + (U + u) * BLOCK, last);
+      nq[u] = CI[j];
+      ncm[u] = CC[j];
+      nlm[u] = ZERO_LAM ? 0.0f : LAMin[j];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * BLOCK;
+      if (k < n_cmp) {
+        const Proj p = compact(q[u], cm[u], lm[u], p0);
+        LAMout[rb + k] = p.lam;
+        if (p.na >= 1) add(p.a, p.ca);
+        if (p.na == 2) add(p.b, p.cb);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      q[u] = nq[u];
+      cm[u] = ncm[u];
+      lm[u] = nlm[u];
+    }
+  }
+  if (GEN) {  // rows read whole: ATTACH, CONTACT, far DISTANCE, FLOOR without PF
+    const int rg = PF ? gen_end[tile] : tile_row_start[tile + 1];
+    for (int j = rd + tid; j < rg; j += BLOCK) {
+      const float l = ZERO_LAM ? 0.0f : LAMin[j];
+      const Proj p = project_row<true>(
+          pos, prev, [&]() { return HAS_ANCHOR ? ANC[j] : make_float4(0.f, 0.f, 0.f, 0.f); },
+          R[j], l, inv_dt2);
+      LAMout[j] = p.lam;
+      if (p.na >= 1) add(p.a, p.ca);
+      if (p.na == 2) add(p.b, p.cb);
+    }
+  }
+  // halo rows: owned by an earlier tile, projected again for this tile's
+  // endpoint (the owner writes their multiplier)
+  for (int k = hb + tid; k < he; k += BLOCK) {
+    const int2 e = H2[k];
+    const int j = e.x;
+    const float l = ZERO_LAM ? 0.0f : LAMin[j];
+    Proj p;
+    if (GEN && e.y < 0) {
+      p = project_row<true>(
+          pos, prev, [&]() { return HAS_ANCHOR ? ANC[j] : make_float4(0.f, 0.f, 0.f, 0.f); },
+          R[j], l, inv_dt2);
+    } else {
+      p = compact(CI[j], CC[j], l, e.y);
+    }
+    if (p.na >= 1) add(p.a, p.ca);
+    if (p.na == 2) add(p.b, p.cb);
+  }
+  __syncthreads();
+  for (int i = p0 + tid; i < p1; i += BLOCK) {
+    const int o = i - p0;
+    const float4 x4 = spos[o];
+    unsigned c = cnt[o];
+    unsigned long long s0 = sx[o], s1 = sy[o], s2 = sz[o];
+    if (PF) {  // the particle's FLOOR row (kernels.py:272-291)
+      const float2 pf = PFT[i];
+      const float C = x4.y - pf.x;
+      if (x4.w != 0.0f && C < 0.0f) {
+        const float wa = x4.w;
+        const float dlam = -C / wa;
+        PFLAM[i] = (ZERO_LAM ? 0.0f : PFLAM[i]) + dlam;
+        f3 corr = mk3(0.0f, wa * dlam, 0.0f);
+        if (pf.y > 0.0f) {  // x_prev: the substep's start (PREDICT wrote it above)
+          const f3 dp_rel = (xyz(x4) + corr) - xyz(XP[i]);
+          corr = corr + friction_delta(dp_rel, mk3(0.0f, 1.0f, 0.0f), -C, pf.y);
+        }
+        const bool ok = fabsf(corr.x) <= kFixLimit && fabsf(corr.y) <= kFixLimit &&
+                        fabsf(corr.z) <= kFixLimit;
+        s0 += fix_bits(fminf(fmaxf(corr.x, -kFixLimit), kFixLimit));
+        s1 += fix_bits(fminf(fmaxf(corr.y, -kFixLimit), kFixLimit));
+        s2 += fix_bits(fminf(fmaxf(corr.z, -kFixLimit), kFixLimit));
+        c += ok ? 1u : (kFixPoison | 1u);
+      } else if (ZERO_LAM && pf.x != -INFINITY) {
+        PFLAM[i] = 0.0f;
+      }
+    }
+    const unsigned adds = c & (kFixPoison - 1u);
+    float4 a = make_float4(0.f, 0.f, 0.f, (float)adds);
+    if (c) a = make_float4(unfix(s0, adds), unfix(s1, adds), unfix(s2, adds), a.w);
+    if (c >= kFixPoison) a.x = __int_as_float(0x7fc00000);
+    if (HAS_ACC) {  // corrections of the device-counted row segment
+      const float4 g = ACC[i];
+      if (g.w != 0.0f) {
+        a = make_float4(a.x + g.x, a.y + g.y, a.z + g.z, a.w + g.w);
+        ACC[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+    f3 x = xyz(x4);
+    if (a.w > 0.0f && x4.w > 0.0f) x = x + xyz(a) * (relax / a.w);
+    if (FINALIZE) {
+      f3 v;
+      if (x4.w == 0.0f) {
+        v = mk3(0.0f, 0.0f, 0.0f);
+      } else {
+        v = ((x - xyz(XP[i])) / h) * (1.0f - damping);
+      }
+      if (do_cap) {
+        const float speed = length(v);
+        if (speed > max_speed) v = v * (max_speed / speed);
+      }
+      if (NEXT) {  // the next substep's integrate (k_integrate_to)
+        XP[i] = make_float4(x.x, x.y, x.z, 0.0f);
+        if (x4.w != 0.0f) {
+          f3 g = mk3(gx, gy, gz);
+          if (gscale) g = g * gscale[i];
+          const f3 vn = v + g * h;
+          x = x + vn * h;
+        }
+      } else {
+        V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+      }
+    }
+    if (STREAM) {  // a frame's positions far exceed the caches: do not keep them
+      typedef float f4v __attribute__((ext_vector_type(4)));
+      const f4v ov = {x.x, x.y, x.z, x4.w};
+      __builtin_nontemporal_store(ov, (f4v*)&Xout[i]);
+    } else {
+      Xout[i] = make_float4(x.x, x.y, x.z, x4.w);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // A whole two-iteration substep in one launch (small scenes, where every
 // launch costs its latency rather than its bytes): the first iteration
 // (xpbd_jacobi_tile with first = 1) followed by the second with finalize
@@ -788,6 +1025,57 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
   }
 }
 
+// The learned start of one row (no reference counterpart): lambda0 from
+// the network output yv and the row's first plain step -C / (w + alpha~)
+// (one-sided rows only while violated), and the direction its endpoints
+// move along: a by nrm * wa * lam0, b (two) by -nrm * wb * lam0.
+struct WarmRow {
+  float lam0, wa, wb;
+  f3 nrm;
+  int a, b;
+  bool two;
+};
+template <class Pos, class Anchor>
+__device__ inline WarmRow warm_row(Pos pos, Anchor anchor, int4 row, float yv, float inv_dt2) {
+  WarmRow r;
+  const int t = row_type(row);
+  r.a = row_a(row);
+  r.b = row.y;
+  const float p0r = __int_as_float(row.z);
+  const float4 xa4 = pos(r.a);
+  r.wa = xa4.w;
+  r.wb = 0.0f;
+  float C, w, alpha = 0.0f;
+  r.nrm = mk3(0.0f, 1.0f, 0.0f);
+  bool ok = true;
+  if (t == T_FLOOR) {
+    C = xa4.y - p0r;
+    w = r.wa;
+  } else {
+    f3 pb;
+    if (t == T_ATTACH) {
+      pb = xyz(anchor());
+    } else {
+      const float4 xb4 = pos(r.b);
+      pb = xyz(xb4);
+      r.wb = xb4.w;
+    }
+    const f3 d = xyz(xa4) - pb;
+    const float L = length(d);
+    ok = L >= XPBD_EPS;
+    if (ok) r.nrm = d * __builtin_amdgcn_rcpf(L);
+    C = L - p0r;
+    w = r.wa + r.wb;
+    if (t != T_CONTACT) alpha = __int_as_float(row.w) * inv_dt2;
+  }
+  const float sden = w + alpha;
+  float step = (sden > 0.0f && ok) ? -C * __builtin_amdgcn_rcpf(sden) : 0.0f;
+  if (t == T_FLOOR || t == T_CONTACT) step = C < 0.0f ? step : 0.0f;
+  r.lam0 = xpbd_ws::start_lambda(yv, step);
+  r.two = t == T_DISTANCE || t == T_CONTACT;
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // Learned warm start launch (no reference counterpart; it replaces the zero
 // start of solver.py:431), run once per substep after the network pass
@@ -874,47 +1162,174 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_warm(
   };
   for (int k = threadIdx.x; k < total; k += BLOCK) {
     const int j = k < owned ? rb + k : halo_rows[hb + (k - owned)];
-    const int4 row = R[j];
-    const float yv = Y[j];
-    const int t = row_type(row);
-    const int a = row_a(row), bb = row.y;
-    const float p0r = __int_as_float(row.z);
-    const float4 xa4 = pos(a);
-    const float wa = xa4.w;
-    float C, w, wb = 0.0f, alpha = 0.0f;
-    f3 nrm = mk3(0.0f, 1.0f, 0.0f);
-    bool ok = true;
-    if (t == T_FLOOR) {
-      C = xa4.y - p0r;
-      w = wa;
-    } else {
-      f3 pb;
-      if (t == T_ATTACH) {
-        pb = xyz(ANC[j]);
-      } else {
-        const float4 xb4 = pos(bb);
-        pb = xyz(xb4);
-        wb = xb4.w;
-      }
-      const f3 d = xyz(xa4) - pb;
-      const float L = length(d);
-      ok = L >= XPBD_EPS;
-      if (ok) nrm = d * __builtin_amdgcn_rcpf(L);
-      C = L - p0r;
-      w = wa + wb;
-      if (t != T_CONTACT) alpha = __int_as_float(row.w) * inv_dt2;
-    }
-    const float sden = w + alpha;
-    float step = (sden > 0.0f && ok) ? -C * __builtin_amdgcn_rcpf(sden) : 0.0f;
-    if (t == T_FLOOR || t == T_CONTACT) step = C < 0.0f ? step : 0.0f;
-    const float lam0 = active ? xpbd_ws::start_lambda(yv, step) : 0.0f;
-    if (k < owned) LAMout[j] = lam0;
-    if (lam0 == 0.0f) continue;
-    add(a, nrm * (wa * lam0));
-    if (t == T_DISTANCE || t == T_CONTACT) add(bb, nrm * (-wb * lam0));
+    const WarmRow wr = warm_row(pos, [&]() { return ANC[j]; }, R[j], Y[j], inv_dt2);
+    if (k < owned) LAMout[j] = wr.lam0;
+    if (wr.lam0 == 0.0f) continue;
+    add(wr.a, wr.nrm * (wr.wa * wr.lam0));
+    if (wr.two) add(wr.b, wr.nrm * (-wr.wb * wr.lam0));
   }
   __syncthreads();
   for (int i = p0 + threadIdx.x; i < p1; i += BLOCK) {
+    const int o = i - p0;
+    const float4 x4 = spos[o];
+    const unsigned c = cnt[o];
+    const unsigned adds = c & (kFixPoison - 1u);
+    float4 a = make_float4(0.f, 0.f, 0.f, (float)adds);
+    if (c) a = make_float4(unfix(sx[o], adds), unfix(sy[o], adds), unfix(sz[o], adds), a.w);
+    if (c >= kFixPoison) a.x = __int_as_float(0x7fc00000);
+    f3 x = xyz(x4);
+    if (a.w > 0.0f && x4.w > 0.0f) x = x + xyz(a) * (relax / a.w);
+    Xout[i] = make_float4(x.x, x.y, x.z, x4.w);
+  }
+}
+
+// The learned start over the compact tables (xpbd_tile_compact): the same
+// rows, lambda0 and moves as k_jacobi_tile_warm, the compact rows' loads
+// unconditional and issued one pass ahead as in k_jacobi_tile_cx.  GEN:
+// some rows of [dist_end, gen_end) (PF) or [dist_end, end) are read from R.
+// PF: the single FLOOR rows [gen_end, end) are read from R too (the launch
+// needs their network output), and their lambda0 goes to PFLAM, where the
+// following k_jacobi_tile_cx launches keep it.
+template <bool PREDICT, bool GEN, bool PF>
+__global__ __launch_bounds__(512) void k_jacobi_tile_warm_cx(
+    const float4* __restrict__ Xin, float4* __restrict__ Xout, float4* __restrict__ XP,
+    const float4* __restrict__ V, const int4* __restrict__ R, const float4* __restrict__ ANC,
+    float* __restrict__ LAMout, const int* __restrict__ tile_row_start,
+    const int* __restrict__ halo_start, const int* __restrict__ dist_end,
+    const int* __restrict__ gen_end, const int2* __restrict__ CI, const float* __restrict__ CC,
+    const int2* __restrict__ H2, float* __restrict__ PFLAM, const float* __restrict__ Y,
+    float h, int P, int n, float relax, float gx, float gy, float gz,
+    const float* __restrict__ gscale, const int* __restrict__ ws_on, int invert) {
+  constexpr int BLOCK = 512;
+  constexpr int U = 2;
+  extern __shared__ unsigned long long lds64[];
+  const bool active = xpbd_ws::start_active(ws_on, invert);
+  unsigned long long* sx = lds64;
+  unsigned long long* sy = lds64 + P;
+  unsigned long long* sz = lds64 + 2 * P;
+  unsigned* cnt = (unsigned*)(lds64 + 3 * P);
+  float4* spos = (float4*)(lds64 + tile_lds_pos_offset(P) / sizeof(unsigned long long));
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int p0 = tile * P;
+  if (p0 >= n) return;
+  const int p1 = min(p0 + P, n);
+  const int tid = threadIdx.x;
+  const float inv_dt2 = 1.0f / (h * h);
+  auto predict = [&](float4 x4, int i) -> float4 {
+    if (!PREDICT || x4.w == 0.0f) return x4;
+    f3 g = mk3(gx, gy, gz);
+    if (gscale) g = g * gscale[i];
+    const f3 v = xyz(V[i]) + g * h;
+    const f3 x = xyz(x4) + v * h;
+    return make_float4(x.x, x.y, x.z, x4.w);
+  };
+  const int rb = tile_row_start[tile], re = tile_row_start[tile + 1];
+  if (!active) {  // the zero start: positions as predicted, multipliers 0
+    for (int i = p0 + tid; i < p1; i += BLOCK) {
+      const float4 x0 = Xin[i];
+      Xout[i] = predict(x0, i);
+      if (PREDICT) XP[i] = make_float4(x0.x, x0.y, x0.z, 0.0f);
+    }
+    for (int j = rb + tid; j < re; j += BLOCK) LAMout[j] = 0.0f;
+    if (PF) {  // the following launches start the single FLOOR rows from PFLAM
+      for (int j = gen_end[tile] + tid; j < re; j += BLOCK) PFLAM[row_a(R[j])] = 0.0f;
+    }
+    return;
+  }
+  const int rd = dist_end[tile];
+  const int hb = halo_start[tile], he = halo_start[tile + 1];
+  const int n_cmp = rd - rb;
+  const int last = max(rd - 1, 0);
+  int2 q[U];
+  float cm[U], yv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = min(rb + tid + u * BLOCK, last);
+    q[u] = CI[j];
+    cm[u] = CC[j];
+    yv[u] = Y[j];
+  }
+  for (int k = tid; k < 3 * P; k += BLOCK) lds64[k] = 0ull;
+  for (int k = tid; k < P; k += BLOCK) cnt[k] = 0u;
+  for (int i = p0 + tid; i < p1; i += BLOCK) {
+    const float4 x0 = Xin[i];
+    spos[i - p0] = predict(x0, i);
+    if (PREDICT) XP[i] = make_float4(x0.x, x0.y, x0.z, 0.0f);
+  }
+  __syncthreads();
+  auto pos = [&](int i) -> float4 {
+    const unsigned off = (unsigned)(i - p0);
+    if (off < (unsigned)(p1 - p0)) return spos[off];
+    return predict(Xin[i], i);
+  };
+  auto add = [&](int i, f3 c) {
+    const unsigned off = (unsigned)(i - p0);
+    if (off >= (unsigned)P) return;
+    const bool ok = fabsf(c.x) <= kFixLimit && fabsf(c.y) <= kFixLimit && fabsf(c.z) <= kFixLimit;
+    atomicAdd(&sx[off], fix_bits(fminf(fmaxf(c.x, -kFixLimit), kFixLimit)));
+    atomicAdd(&sy[off], fix_bits(fminf(fmaxf(c.y, -kFixLimit), kFixLimit)));
+    atomicAdd(&sz[off], fix_bits(fminf(fmaxf(c.z, -kFixLimit), kFixLimit)));
+    atomicAdd(&cnt[off], ok ? 1u : (kFixPoison | 1u));
+  };
+  auto no_anchor = [] { return make_float4(0.f, 0.f, 0.f, 0.f); };
+  auto compact = [](int2 qq, float c, int base) {  // DISTANCE: no type bits
+    const unsigned v = (unsigned)qq.x;
+    return make_int4(base + (int)(v & 1023u), base + (int)(v >> 10), qq.y, __float_as_int(c));
+  };
+  auto move = [&](const WarmRow& wr) {
+    if (wr.lam0 == 0.0f) return;
+    add(wr.a, wr.nrm * (wr.wa * wr.lam0));
+    if (wr.two) add(wr.b, wr.nrm * (-wr.wb * wr.lam0));
+  };
+  for (int k0 = tid; k0 < n_cmp; k0 += BLOCK * U) {
+    int2 nq[U];
+    float ncm[U], nyv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = min(rb + k0 + (U + u) * BLOCK, last);
+      nq[u] = CI[j];
+      ncm[u] = CC[j];
+      nyv[u] = Y[j];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * BLOCK;
+      if (k < n_cmp) {
+        const WarmRow wr = warm_row(pos, no_anchor, compact(q[u], cm[u], p0), yv[u], inv_dt2);
+        LAMout[rb + k] = wr.lam0;
+        move(wr);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      q[u] = nq[u];
+      cm[u] = ncm[u];
+      yv[u] = nyv[u];
+    }
+  }
+  const int rg = PF ? gen_end[tile] : re;
+  if (GEN) {  // every other owned row from R
+    for (int j = rd + tid; j < rg; j += BLOCK) {
+      const WarmRow wr = warm_row(pos, [&]() { return ANC[j]; }, R[j], Y[j], inv_dt2);
+      LAMout[j] = wr.lam0;
+      move(wr);
+    }
+  }
+  if (PF) {  // the single FLOOR rows: lambda0 per particle
+    for (int j = rg + tid; j < re; j += BLOCK) {
+      const WarmRow wr = warm_row(pos, no_anchor, R[j], Y[j], inv_dt2);
+      PFLAM[wr.a] = wr.lam0;
+      move(wr);
+    }
+  }
+  for (int k = hb + tid; k < he; k += BLOCK) {
+    const int2 e = H2[k];
+    const int j = e.x;
+    const int4 row = (GEN && e.y < 0) ? R[j] : compact(CI[j], CC[j], e.y);
+    move(warm_row(pos, [&]() { return ANC[j]; }, row, Y[j], inv_dt2));
+  }
+  __syncthreads();
+  for (int i = p0 + tid; i < p1; i += BLOCK) {
     const int o = i - p0;
     const float4 x4 = spos[o];
     const unsigned c = cnt[o];
@@ -1337,6 +1752,96 @@ int launch_tile(bool predict, hipStream_t s, const float* Xin, float* Xout, floa
 }
 
 
+
+// Launches of the compact-table kernel: every mode, with and without rows
+// read whole (GEN) and per-particle floors (PF).
+template <int MODE, bool GEN, bool PF>
+void launch_cx_mode(hipStream_t s, int nb, size_t shmem, const float* Xin, float* Xout,
+                    float* XP, float* V, const int* R, const float* ANC, const float* LAMin,
+                    float* LAMout, const int* trs, const int* hs, const xpbd_tile_compact* cx,
+                    float h, int P, int n, float relax, float damping, float max_speed,
+                    int do_cap, int stream_out, float* ACC, float gx, float gy, float gz,
+                    const float* gscale) {
+  auto k = stream_out ? k_jacobi_tile_cx<MODE, GEN, PF, true>
+                      : k_jacobi_tile_cx<MODE, GEN, PF, false>;
+  hipLaunchKernelGGL(k, dim3(nb), dim3(512), shmem, s,
+                     (const float4*)Xin, (float4*)Xout, (float4*)XP, (float4*)V, (const int4*)R,
+                     (const float4*)ANC, LAMin, LAMout, trs, hs, cx->dist_end, cx->gen_end,
+                     (const int2*)cx->ci, cx->cc, (const int2*)cx->halo2,
+                     (const float2*)cx->pf, cx->pflam, h, P, n, relax, damping, max_speed,
+                     do_cap, (float4*)ACC, gx, gy, gz, gscale);
+}
+
+template <bool GEN, bool PF>
+int launch_cx_gp(int mode, hipStream_t s, int nb, size_t shmem, const float* Xin, float* Xout,
+                 float* XP, float* V, const int* R, const float* ANC, const float* LAMin,
+                 float* LAMout, const int* trs, const int* hs, const xpbd_tile_compact* cx,
+                 float h, int P, int n, float relax, float damping, float max_speed, int do_cap,
+                 int stream_out, float* ACC, float gx, float gy, float gz, const float* gscale) {
+#define XPBD_CX_MODE(M)                                                                        \
+  case M:                                                                                      \
+    launch_cx_mode<M, GEN, PF>(s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin, LAMout, trs, hs, \
+                               cx, h, P, n, relax, damping, max_speed, do_cap, stream_out,     \
+                               ACC, gx, gy, gz, gscale);                                      \
+    return 0;
+#define XPBD_CX_BASES(X)                                                                       \
+  XPBD_CX_MODE(0 | (X))                                                                        \
+  XPBD_CX_MODE(TM_FIRST | (X))                                                                 \
+  XPBD_CX_MODE(TM_FINALIZE | (X))                                                              \
+  XPBD_CX_MODE(TM_FINALIZE | TM_FIRST | (X))                                                   \
+  XPBD_CX_MODE(TM_FINALIZE | TM_NEXT | (X))                                                    \
+  XPBD_CX_MODE(TM_FINALIZE | TM_NEXT | TM_FIRST | (X))                                         \
+  XPBD_CX_MODE(TM_PREDICT | (X))
+  switch (mode) {
+    XPBD_CX_BASES(0)
+    XPBD_CX_BASES(TM_ACC)
+  }
+  if constexpr (GEN) {
+    switch (mode) {
+      XPBD_CX_BASES(TM_ANCHOR)
+      XPBD_CX_BASES(TM_ANCHOR | TM_ACC)
+    }
+  }
+#undef XPBD_CX_BASES
+#undef XPBD_CX_MODE
+  return (int)hipErrorInvalidValue;
+}
+
+// The compact-table form of launch_tile (tile_size >= 64: 512-thread blocks).
+int launch_tile_cx(bool predict, hipStream_t s, const float* Xin, float* Xout, float* XP,
+                   float* V, const int* R, const float* ANC, const float* LAMin, float* LAMout,
+                   const int* tile_row_start, const int* halo_start, const xpbd_tile_compact* cx,
+                   float h, int first, int tile_size, int n, float relax, int finalize,
+                   float damping, float max_speed, float* ACC, float gx, float gy, float gz,
+                   const float* gscale) {
+  if (n <= 0) return 0;
+  if (!cx || tile_size < 64 || tile_size > XPBD_MAX_TILE || !cx->dist_end || !cx->ci ||
+      !cx->cc || !cx->halo2 || (cx->pf && (!cx->gen_end || !cx->pflam)))
+    return (int)hipErrorInvalidValue;
+  const bool gen = cx->general != 0;
+  const bool pf = cx->pf != nullptr;
+  const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
+  const int nb = blocks_for(n, tile_size);
+  const size_t shmem = tile_lds_bytes(tile_size);
+  // ATTACH rows are read whole: without such rows no anchor is read
+  const int mode = (predict ? TM_PREDICT : 0) | (finalize ? TM_FINALIZE : 0) |
+                   (finalize == 2 ? TM_NEXT : 0) | (ACC ? TM_ACC : 0) |
+                   (ANC && gen ? TM_ANCHOR : 0) | (first && !predict ? TM_FIRST : 0);
+  const int so = n >= kStreamOutParticles ? 1 : 0;
+  int rc;
+#define XPBD_CX_ARGS                                                                           \
+  mode, s, nb, shmem, Xin, Xout, XP, V, R, ANC, LAMin, LAMout, tile_row_start, halo_start, cx, \
+      h, tile_size, n, relax, damping, max_speed, do_cap, so, ACC, gx, gy, gz, gscale
+  if (gen)
+    rc = pf ? launch_cx_gp<true, true>(XPBD_CX_ARGS) : launch_cx_gp<true, false>(XPBD_CX_ARGS);
+  else
+    rc = pf ? launch_cx_gp<false, true>(XPBD_CX_ARGS) : launch_cx_gp<false, false>(XPBD_CX_ARGS);
+#undef XPBD_CX_ARGS
+  if (rc) return rc;
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1473,6 +1978,45 @@ int xpbd_jacobi_tile_first(void* stream, const float* X, float* Xout, float* XP,
                      n_particles, relax, 0, 0.0f, 0.0f, ACC, gx, gy, gz, gscale);
 }
 
+int xpbd_jacobi_tile_cx(void* stream, const float* Xin, float* Xout, const float* XP, float* V,
+                        const int* R, const float* ANC, const float* LAMin, float* LAMout,
+                        const int* tile_row_start, const int* halo_start,
+                        const xpbd_tile_compact* cx, float h, int first, int tile_size,
+                        int n_particles, float relax, int finalize, float damping,
+                        float max_speed, float* ACC) {
+  // the compact loop's multiplier loads are unconditional in a launch with
+  // lambda from LAMin: a zero start never reads it
+  return launch_tile_cx(false, (hipStream_t)stream, Xin, Xout, const_cast<float*>(XP), V, R,
+                        ANC, LAMin, LAMout, tile_row_start, halo_start, cx, h, first, tile_size,
+                        n_particles, relax, finalize ? 1 : 0, damping, max_speed, ACC, 0.0f,
+                        0.0f, 0.0f, nullptr);
+}
+
+int xpbd_jacobi_tile_next_cx(void* stream, const float* Xin, float* Xout, float* XP,
+                             const int* R, const float* ANC, const float* LAMin, float* LAMout,
+                             const int* tile_row_start, const int* halo_start,
+                             const xpbd_tile_compact* cx, float h, int first, float gx,
+                             float gy, float gz, const float* gscale, int tile_size,
+                             int n_particles, float relax, float damping, float max_speed,
+                             float* ACC) {
+  if (Xin == Xout) return (int)hipErrorInvalidValue;  // neighbours read Xin while tiles write
+  return launch_tile_cx(false, (hipStream_t)stream, Xin, Xout, XP, nullptr, R, ANC, LAMin,
+                        LAMout, tile_row_start, halo_start, cx, h, first, tile_size,
+                        n_particles, relax, 2, damping, max_speed, ACC, gx, gy, gz, gscale);
+}
+
+int xpbd_jacobi_tile_first_cx(void* stream, const float* X, float* Xout, float* XP,
+                              const float* V, const int* R, const float* ANC, float* LAMout,
+                              const int* tile_row_start, const int* halo_start,
+                              const xpbd_tile_compact* cx, float h, float gx, float gy, float gz,
+                              const float* gscale, int tile_size, int n_particles, float relax,
+                              float* ACC) {
+  if (X == Xout) return (int)hipErrorInvalidValue;  // neighbours read X while tiles write
+  return launch_tile_cx(true, (hipStream_t)stream, X, Xout, XP, const_cast<float*>(V), R, ANC,
+                        nullptr, LAMout, tile_row_start, halo_start, cx, h, 1, tile_size,
+                        n_particles, relax, 0, 0.0f, 0.0f, ACC, gx, gy, gz, gscale);
+}
+
 int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP, const float* V,
                           const int* R, const float* ANC, float* LAMout,
                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
@@ -1499,6 +2043,45 @@ int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP
     else XPBD_WARM_LAUNCH(kTileBlock, false);
   }
 #undef XPBD_WARM_LAUNCH
+  XPBD_LAUNCH_CHECK();
+  return 0;
+}
+
+int xpbd_jacobi_tile_warm_cx(void* stream, const float* Xin, float* Xout, float* XP,
+                             const float* V, const int* R, const float* ANC, float* LAMout,
+                             const int* tile_row_start, const int* halo_start,
+                             const xpbd_tile_compact* cx, const float* Y, float h, float gx,
+                             float gy, float gz, const float* gscale, int tile_size,
+                             int n_particles, float relax, int predict, const int* ws_on,
+                             int invert) {
+  if (n_particles <= 0) return 0;
+  if (!cx || !cx->dist_end || !cx->ci || !cx->cc || !cx->halo2 || tile_size < 64 ||
+      tile_size > XPBD_MAX_TILE || !Y || Xin == Xout ||
+      (cx->pf && (!cx->gen_end || !cx->pflam)))
+    return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = blocks_for(n_particles, tile_size);
+  const size_t shmem = tile_lds_bytes(tile_size);
+#define XPBD_WARM_CX(PR, GN, F)                                                               \
+  hipLaunchKernelGGL((k_jacobi_tile_warm_cx<PR, GN, F>), dim3(nb), dim3(512), shmem, s,       \
+                     (const float4*)Xin, (float4*)Xout, (float4*)XP, (const float4*)V,        \
+                     (const int4*)R, (const float4*)ANC, LAMout, tile_row_start, halo_start,  \
+                     cx->dist_end, cx->gen_end, (const int2*)cx->ci, cx->cc,                  \
+                     (const int2*)cx->halo2, cx->pflam, Y, h, tile_size, n_particles, relax,  \
+                     gx, gy, gz, gscale, ws_on, invert)
+#define XPBD_WARM_CX_PF(PR, GN)                                                               \
+  if (cx->pf) XPBD_WARM_CX(PR, GN, true);                                                     \
+  else XPBD_WARM_CX(PR, GN, false);
+  const bool gen = cx->general != 0;
+  if (predict) {
+    if (gen) { XPBD_WARM_CX_PF(true, true) }
+    else { XPBD_WARM_CX_PF(true, false) }
+  } else {
+    if (gen) { XPBD_WARM_CX_PF(false, true) }
+    else { XPBD_WARM_CX_PF(false, false) }
+  }
+#undef XPBD_WARM_CX_PF
+#undef XPBD_WARM_CX
   XPBD_LAUNCH_CHECK();
   return 0;
 }

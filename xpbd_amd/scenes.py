@@ -168,7 +168,8 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
     particles per tile are built on the device.
 
     Returns dict of torch tensors X, X_B, V, XP, ACC (float4 per particle), R
-    (int4 rows), ANC, LAM, LAM_B, TRS, HS, HR (tile tables) and counts.
+    (int4 rows), ANC, LAM, LAM_B, TRS, HS, HR (tile tables), CX (the compact
+    tables, include/xpbd.h xpbd_tile_compact) and counts.
     ``height`` <= 0 lays the sheets on the floor (every FLOOR row active,
     x_prev = x so friction reads it)."""
     import torch
@@ -245,7 +246,20 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
                             ).to(torch.int32)
     n_dist = int(is_d.sum())
     X = Xo[order].contiguous()
-    del a, b, t, lo, hi, rest, comp, p0, p1, is_d, Xo, ro, cross, ht, so
+    # compact tables (xpbd_tile_compact): every DISTANCE row is compact (a
+    # sheet spans 2^20 particles), every particle has one FLOOR row, so a
+    # tile's rows are its DISTANCE rows then its FLOOR rows
+    base = (lo // tile) * tile
+    assert int((hi - base).max()) < (1 << 22)
+    CI = torch.stack([torch.where(is_d, (lo - base) | ((hi - base) << 10),
+                                  torch.zeros_like(lo)).to(torch.int32), R[:, 2]], 1).contiguous()
+    CC = R[:, 3].view(torch.float32).clone()
+    DE = (TRS[:-1].long() + torch.bincount((lo // tile)[is_d], minlength=n_tiles)).to(torch.int32)
+    H2 = torch.stack([cross[so].to(torch.int32), base[cross[so]].to(torch.int32)], 1).contiguous()
+    PF = torch.empty((n, 2), device=device, dtype=torch.float32)
+    PF[a[~is_d], 0] = p0[~is_d]
+    PF[a[~is_d], 1] = p1[~is_d]
+    del a, b, t, lo, hi, rest, comp, p0, p1, is_d, Xo, ro, cross, ht, so, base
     return dict(
         X=X, X_B=X.clone(), V=torch.zeros((n, 4), device=device),
         XP=X.clone() if height <= 0 else torch.zeros((n, 4), device=device),
@@ -254,6 +268,9 @@ def packed_cloth_device(res: int, n_sheets: int, device, seed: int = 0, spacing:
         ANC=torch.zeros((1, 4), device=device), LAM=torch.zeros(m, device=device),
         LAM_B=torch.zeros(m, device=device), TRS=TRS, HS=HS,
         HR=HR if HR.numel() else torch.zeros(1, dtype=torch.int32, device=device),
+        CX=dict(dist_end=DE, gen_end=DE.clone(), ci=CI, cc=CC,
+                halo2=H2 if H2.numel() else torch.zeros((1, 2), dtype=torch.int32, device=device),
+                pf=PF, pflam=torch.zeros(n, device=device), general=0),
         n_particles=n, n_rows=m, n_distance=n_dist, tile=tile, n_halo=int(HR.numel()))
 
 

@@ -145,20 +145,97 @@ def jacobi_row_order(a, b) -> np.ndarray:
     return np.argsort(key, kind="stable")
 
 
-def jacobi_tile_order(a, b, types, tile: int) -> np.ndarray:
+# Row classes inside a tile of the fused kernel (the segments of
+# xpbd_tile_compact): DISTANCE rows stored compactly, rows read as int4,
+# and FLOOR rows of particles with exactly one FLOOR row (per-particle).
+CLS_COMPACT, CLS_GENERAL, CLS_FLOOR = 0, 1, 2
+COMPACT_SPAN = 1 << 22      # hi - tile base must stay below this (22-bit field)
+
+
+def tile_row_class(a, b, types, tile: int, n_particles: int) -> np.ndarray:
+    """Class of each row (CLS_*) for tiles of ``tile`` particles."""
+    a = np.asarray(a, np.int64)
+    b = np.asarray(b, np.int64)
+    types = np.asarray(types, np.int64)
+    lo = np.where(b >= 0, np.minimum(a, b), a)
+    hi = np.where(b >= 0, np.maximum(a, b), a)
+    cls = np.full(len(a), CLS_GENERAL, np.int64)
+    if tile <= 1024:
+        cls[(types == DISTANCE) & (hi - (lo // tile) * tile < COMPACT_SPAN)] = CLS_COMPACT
+    fl = types == FLOOR
+    if np.any(fl):
+        per = np.bincount(a[fl], minlength=max(n_particles, 1))
+        cls[fl & (per[np.where(fl, a, 0)] == 1)] = CLS_FLOOR
+    return cls
+
+
+def jacobi_tile_order(a, b, types, tile: int, n_particles: int | None = None) -> np.ndarray:
     """Stable row order of the fused tile kernel: grouped by the tile of the
-    smallest endpoint; inside a tile by type, then by rank among the rows of
-    the same smallest endpoint, then by that endpoint.  Rows of a wavefront
-    then take the same branch and mostly add into distinct particles."""
+    smallest endpoint; inside a tile by class (compact DISTANCE rows, rows
+    read whole, single FLOOR rows: the segments of the compact tables), then
+    by type, then by rank among the rows of the same smallest endpoint, then
+    by that endpoint.  Rows of a wavefront then take the same branch and
+    mostly add into distinct particles."""
     a = np.asarray(a, np.int64)
     b = np.asarray(b, np.int64)
     lo = np.where(b >= 0, np.minimum(a, b), a)
+    if n_particles is None:
+        n_particles = int(max(a.max(initial=-1), b.max(initial=-1))) + 1
+    cls = tile_row_class(a, b, types, tile, n_particles)
     first = jacobi_row_order(a, b)
     los = lo[first]
     rank = np.empty(len(lo), np.int64)
     rank[first] = np.arange(len(lo), dtype=np.int64) - np.searchsorted(los, los, "left")
-    key = np.lexsort((lo, rank, np.asarray(types, np.int64), lo // tile))
+    rank[cls == CLS_FLOOR] = 0
+    key = np.lexsort((lo, rank, np.asarray(types, np.int64), cls, lo // tile))
     return key.astype(np.int64)
+
+
+def compact_tile_tables(rows_sorted: np.ndarray, n_particles: int, tile: int, tile_row_start,
+                        halo_rows) -> dict:
+    """Compact tables (xpbd_tile_compact) of rows in ``jacobi_tile_order``:
+    per tile the ends of the compact and the general segment, the compact
+    rows (ci int32[M, 2]: tile-relative (lo, hi) | rest bits; cc float32[M]
+    compliance), the halo entries (int32[H, 2]: row, owner tile base or -1)
+    and the per-particle floors (pf float32[n, 2] floor_y / mu, floor_y =
+    -inf without one; pf_rows / pf_part: each such FLOOR row and its
+    particle, to read their multipliers back), and the counts of rows read
+    whole (n_general) and of per-particle FLOOR rows (n_floor)."""
+    m = rows_sorted.shape[0]
+    n_tiles = len(tile_row_start) - 1
+    t = (rows_sorted[:, 0] >> 28).astype(np.int64)
+    a = (rows_sorted[:, 0] & 0x0FFFFFFF).astype(np.int64)
+    b = rows_sorted[:, 1].astype(np.int64)
+    cls = tile_row_class(a, b, t, tile, n_particles)
+    lo = np.where(b >= 0, np.minimum(a, b), a)
+    hi = np.where(b >= 0, np.maximum(a, b), a)
+    lt = lo // tile
+    base = lt * tile
+    if m and np.any(np.diff(lt * 4 + cls) < 0):
+        raise ValueError("rows are not in jacobi_tile_order")
+    cnt = np.zeros((n_tiles, 3), np.int64)
+    np.add.at(cnt, (lt, cls), 1)
+    trs = np.asarray(tile_row_start, np.int64)
+    dist_end = (trs[:-1] + cnt[:, 0]).astype(np.int32)
+    gen_end = (dist_end + cnt[:, 1]).astype(np.int32)
+    ci = np.zeros((m, 2), np.int32)
+    c = cls == CLS_COMPACT
+    ci[c, 0] = ((lo[c] - base[c]) | ((hi[c] - base[c]) << 10)).astype(np.uint32).view(np.int32)
+    ci[:, 1] = rows_sorted[:, 2]
+    cc = np.ascontiguousarray(rows_sorted[:, 3]).view(np.float32).copy()
+    hr = np.asarray(halo_rows, np.int64)
+    halo2 = np.zeros((len(hr), 2), np.int32)
+    halo2[:, 0] = hr
+    halo2[:, 1] = np.where(cls[hr] == CLS_COMPACT, base[hr], -1)
+    pf = np.zeros((max(n_particles, 1), 2), np.float32)
+    pf[:, 0] = -np.inf
+    pf_rows = np.nonzero(cls == CLS_FLOOR)[0]
+    pf_part = a[pf_rows]
+    pf[pf_part, 0] = rows_sorted[pf_rows, 2].view(np.float32)
+    pf[pf_part, 1] = rows_sorted[pf_rows, 3].view(np.float32)
+    return dict(dist_end=dist_end, gen_end=gen_end, ci=ci, cc=cc, halo2=halo2, pf=pf,
+                pf_rows=pf_rows, pf_part=pf_part,
+                n_general=int(np.count_nonzero(cls == CLS_GENERAL)), n_floor=len(pf_rows))
 
 
 def choose_jacobi_tiling(rows_sorted: np.ndarray, block: int = 256) -> tuple[int, int]:
@@ -373,6 +450,12 @@ class Solver:
         self.tile_pair = "auto"
         self.block_frames = "auto"    # small scenes: whole frames in one workgroup
         self._block_active = False
+        # tile launches over the compact tables (xpbd_tile_compact): True,
+        # or False for the int4 rows and halo list alone (identical results)
+        self.compact_rows = True
+        self._cx = self._cx_dev = self._cx_struct = None
+        self._plan_pf = False         # the plan's FLOOR multipliers are per particle
+        self._plan_pair = False       # the plan runs two-iteration pair launches
         # device-side CONTACT generation (Jacobi tile mode): an extension
         # that replaces the per-frame host broad phase; see enable_device_contacts
         self._device_contacts = False
@@ -796,7 +879,7 @@ class Solver:
                 perm = np.zeros(0, np.int64)
             elif tile_mode:
                 self._tile = self.tile_size or default_tile_size(n_b)
-                perm = jacobi_tile_order(ra, rb, h["type"], self._tile)
+                perm = jacobi_tile_order(ra, rb, h["type"], self._tile, n_b)
             else:
                 perm = jacobi_row_order(ra, rb)
             self._color_starts = np.array([0, n_c], dtype=np.int32)
@@ -810,9 +893,13 @@ class Solver:
         self._tile_tabs = None
         self._rows_dev_order = rows if tile_mode else None   # for tile_pair_tables
         self._pair_dev = None
+        self._cx = self._cx_dev = self._cx_struct = None
+        self._plan_pf = False
         if tile_mode and n_c:
             self._tile = self.tile_size or default_tile_size(n_b)
             self._tile_tabs = tile_tables(rows, n_b, self._tile)
+            ts, _, hr = self._tile_tabs
+            self._cx = compact_tile_tables(rows, n_b, self._tile, ts, hr)
         self._pperm, self._pinv = pperm, pinv
         dev = self.device
         self.X = torch.from_numpy(X).to(dev)
@@ -830,6 +917,16 @@ class Solver:
             self._tile_dev = (torch.from_numpy(ts).to(dev), torch.from_numpy(hs).to(dev),
                               torch.from_numpy(np.ascontiguousarray(
                                   hr if len(hr) else np.zeros(1, np.int32))).to(dev))
+            if self._cx is not None:
+                c = self._cx
+                up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(  # noqa: E731
+                    a if len(a) else np.zeros((1,) + a.shape[1:], dt))).to(dev)
+                self._cx_dev = dict(dist_end=up(c["dist_end"], np.int32),
+                                    gen_end=up(c["gen_end"], np.int32),
+                                    ci=up(c["ci"], np.int32), cc=up(c["cc"], np.float32),
+                                    halo2=up(c["halo2"], np.int32), pf=up(c["pf"], np.float32),
+                                    pflam=torch.zeros(max(n_b, 1), dtype=torch.float32,
+                                                      device=dev))
         else:
             self.X_B = self.LAM_B = None
             self._tile_dev = None
@@ -943,8 +1040,15 @@ class Solver:
                               h["anchor"], h["mu"])
         if len(rows):
             m = len(rows)
-            self.R[:m].copy_(self._t(np.ascontiguousarray(rows[self._perm])))
+            rs = np.ascontiguousarray(rows[self._perm])
+            self.R[:m].copy_(self._t(rs))
             self.ANC[:m].copy_(self._t(np.ascontiguousarray(anc[self._perm])))
+            if self._cx_dev is not None:   # the compact copies of the same fields
+                ts, _, hr = self._tile_tabs
+                c = compact_tile_tables(rs, int(self.X.shape[0]), self._tile, ts, hr)
+                self._cx.update(c)
+                for k in ("ci", "cc", "pf"):
+                    self._cx_dev[k].copy_(self._t(np.ascontiguousarray(c[k])))
 
     # ---- the step (solver.py:412-467) --------------------------------------
     def _signature(self):
@@ -1015,6 +1119,7 @@ class Solver:
         self._drop_plan()
         n_b = int(self.X.shape[0])
         n_c = len(self._rt)
+        self._plan_pair = False
         d = N.ParticlesDesc()
         d.x, d.v, d.x_prev, d.acc = (N.ptr(self.X), N.ptr(self.V), N.ptr(self.XP),
                                      N.ptr(self.ACC))
@@ -1043,6 +1148,7 @@ class Solver:
                 d.dyn_rows, d.dyn_lam = N.ptr(self.CROWS), N.ptr(self.CLAM)
                 d.dyn_count, d.dyn_cap = N.ptr(self.CCOUNT), self._ccap
             pair = self._pair_tables(d, n_b, n_c)
+            self._plan_pair = pair is not None
             if pair is not None:
                 tabs, (lmax, amax, k1max), xp_alt = pair
                 d.x_prev_alt = N.ptr(xp_alt)
@@ -1081,8 +1187,27 @@ class Solver:
                     d.ws_jacc, d.ws_jdlam = N.ptr(j["acc"]), N.ptr(j["dl"])
             else:
                 self._judge = None
+        self._cx_struct = None
+        self._plan_pf = False
+        if (d.mode == MODE_JACOBI_TILE and self._cx_dev is not None and self.compact_rows
+                and n_c > 0 and self._tile >= 64):
+            c = self._cx_dev
+            cs = N.TileCompact()
+            for k in ("dist_end", "gen_end", "ci", "cc", "halo2"):
+                setattr(cs, k, N.ptr(c[k]))
+            # per-particle floors unless a kernel of the plan reads the FLOOR
+            # rows' multipliers from LAM (the pair launch)
+            if self._cx["n_floor"] and not d.pair_p1:
+                cs.pf, cs.pflam = N.ptr(c["pf"]), N.ptr(c["pflam"])
+                self._plan_pf = True
+            cs.general = int(self._cx["n_general"] > 0
+                             or (not self._plan_pf and self._cx["n_floor"] > 0))
+            self._cx_struct = cs
+            d.compact = N.ctypes.addressof(cs)
         self._block_active = self._use_block()
         if self._block_active:
+            d.compact = None
+            self._plan_pf = self._plan_pair = False
             d.block = 1
             d.mode = MODE_GS if self.solve_mode == "gs" else MODE_JACOBI
             d.anchors = N.ptr(self.ANC)
@@ -1220,8 +1345,11 @@ class Solver:
         launches = self.iterations + (1 if self._ws_theta is not None else 0)
         if self._tile_dev is not None and launches % 2 == 1 and not self._block_active:
             lam = self.LAM_B
+        dev = lam[:n_c].cpu().numpy()
+        if self._plan_pf:   # FLOOR rows kept per particle by the plan
+            dev[self._cx["pf_rows"]] = self._cx_dev["pflam"].cpu().numpy()[self._cx["pf_part"]]
         out = np.zeros(n_c, dtype=np.float32)
-        out[self._perm] = lam[:n_c].cpu().numpy()
+        out[self._perm] = dev
         return out
 
     @property

@@ -278,7 +278,7 @@ def roofline_scene(N, reps: int, frames: int) -> dict:
         "workload": f"{sheets} x {res}^2 cloth: {n} particles, {nd} DISTANCE + {m - nd} FLOOR "
                     f"rows ({sc['n_halo']} halo rows re-projected), jacobi 15x2, hanging at "
                     "y = 0.5 m (no FLOOR row active)",
-        "kernel": "k_jacobi_tile",
+        "kernel": "k_jacobi_tile_cx",
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": nbytes,
         "floor_rows_active": hang_active,
@@ -758,8 +758,15 @@ def main() -> int:
         print(f"[bench] roofline scene: kernel {scene['kernel_ms']:.3f} ms "
               f"{scene['achieved']} GB/s, frame {scene['frame_ms']} ms", file=sys.stderr,
               flush=True)
-        scene["traffic"] = load_traffic("k_jacobi_tile@8x1024^2")
+        scene["traffic"] = load_traffic("k_jacobi_tile_cx@8x1024^2")
         scene["traffic_frac"] = traffic_frac(scene["traffic"], scene["kernel_ms"])
+        rest = scene["resting"]
+        rest["traffic"] = load_traffic("k_jacobi_tile_cx@8x1024^2 resting")
+        rest["traffic_frac"] = traffic_frac(rest["traffic"], rest["kernel_ms"])
+        for key, part in (("k_jacobi_tile@8x1024^2", scene["int4_rows"]),
+                          ("k_jacobi_tile@8x1024^2 resting", rest["int4_rows"])):
+            part["traffic"] = load_traffic(key)
+            part["traffic_frac"] = traffic_frac(part["traffic"], part["kernel_ms"])
         torch.cuda.empty_cache()
 
     others = None

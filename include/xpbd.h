@@ -218,12 +218,16 @@ int xpbd_solve_jacobi_dyn_first(void* stream, const float* X, const float* V, co
  * count[0].  api[i] = API index of device particle i (NULL: identity);
  * excl = sorted keys (lo << 32 | hi) of the API index pairs joined by a
  * constraint (NULL with n_excl = 0: none).  radius[i] = 0 excludes
- * particle i. */
+ * particle i.  clist (optional): the device indices of the colliding
+ * particles (radius > 0) in ascending API index, nc of them; the search,
+ * its counts and scans then cover those only (the rows are the same).
+ * NULL: every particle is visited. */
 long long xpbd_contacts_workspace_bytes(int n);
 int xpbd_sphere_contacts(void* stream, void* ws, const float* X, const float* V,
                          const float* radius, const float* mu, const int* api,
-                         const long long* excl, int n_excl, double contact_margin, double dt,
-                         float r_max, int n, int cap, int* ROWS, int* count);
+                         const int* clist, int nc, const long long* excl, int n_excl,
+                         double contact_margin, double dt, float r_max, int n, int cap,
+                         int* ROWS, int* count);
 
 /* Whole frames of a small scene in one workgroup (solver.py:412-467): the
  * positions stay in LDS, phases are separated by workgroup barriers.

@@ -115,8 +115,9 @@ _SIGS = {
     "xpbd_solve_jacobi_dyn_first": (c_int, [c_void_p] * 6 + [c_float, c_float, c_float, c_float,
                                                              c_void_p, c_void_p, c_int]),
     "xpbd_contacts_workspace_bytes": (ctypes.c_longlong, [c_int]),
-    "xpbd_sphere_contacts": (c_int, [c_void_p] * 8 + [c_int, ctypes.c_double, ctypes.c_double,
-                                                      c_float, c_int, c_int, c_void_p, c_void_p]),
+    "xpbd_sphere_contacts": (c_int, [c_void_p] * 8 + [c_int, c_void_p, c_int, ctypes.c_double,
+                                                      ctypes.c_double, c_float, c_int, c_int,
+                                                      c_void_p, c_void_p]),
     "xpbd_frame_block": (c_int, [c_void_p] * 8 + [c_int] * 5 + [c_float] * 7 + [c_int, c_int]),
     "xpbd_frame_block_capacity": (c_int, [c_int]),
     "xpbd_solve_color": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -13,7 +13,10 @@
 // "Index" is the API particle index (`api`, when the device layout permutes
 // the particles), so the rows come out in exactly the reference's order.
 // Count per (offset, body), exclusive scan in offset-major order, emit; the
-// live row count stays on the device and gates the Jacobi projection.
+// live row count stays on the device and gates the Jacobi projection.  The
+// bodies are the colliding ones only (clist: their device indices in
+// ascending API index), so the counts, the scans and the cell table scale
+// with the colliders, not with every particle of the scene.
 #include <algorithm>
 #include <cmath>
 
@@ -56,12 +59,17 @@ __host__ __device__ inline int chash(int cx, int cy, int cz, int T) {
   return (int)(h & (unsigned)(T - 1));
 }
 
+// Body k of the candidate list: its device index (clist NULL: k itself).
+__device__ inline int body_of(const int* clist, int k) { return clist ? clist[k] : k; }
+
 __global__ __launch_bounds__(kB) void k_sphere_cells(const float4* __restrict__ X,
                                                      const float* __restrict__ radius,
                                                      const float* __restrict__ params, int T,
-                                                     int4* cells, int* slot_count, int n) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
+                                                     int4* cells, int* slot_count,
+                                                     const int* __restrict__ clist, int nb) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= nb) return;
+  const int i = body_of(clist, k);
   if (!(radius[i] > 0.0f)) {
     cells[i] = make_int4(0, 0, 0, -1);
     return;
@@ -76,9 +84,11 @@ __global__ __launch_bounds__(kB) void k_sphere_cells(const float4* __restrict__ 
 
 __global__ __launch_bounds__(kB) void k_sphere_scatter(const int4* __restrict__ cells,
                                                        const int* __restrict__ slot_start,
-                                                       int* cursor, int* bucket, int n) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
+                                                       int* cursor, int* bucket,
+                                                       const int* __restrict__ clist, int nb) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= nb) return;
+  const int i = body_of(clist, k);
   const int s = cells[i].w;
   if (s < 0) return;
   bucket[slot_start[s] + atomicAdd(&cursor[s], 1)] = i;
@@ -108,24 +118,29 @@ __device__ inline bool excluded(const long long* __restrict__ excl, int n_excl, 
   return l < n_excl && excl[l] == key;
 }
 
-// EMIT = false: cnt[o n + api(i)] = number of rows body i contributes at
-// offset o.  EMIT = true: those rows at off[o n + api(i)], partners in
-// ascending API index.
+// Bodies in API order: with clist, body k (device index clist[k]) has
+// rank k; without, device body i has rank api(i).  EMIT = false:
+// cnt[o nb + rank] = number of rows the body contributes at offset o.
+// EMIT = true: those rows at off[o nb + rank], partners in ascending API
+// index.
 template <bool EMIT>
 __global__ __launch_bounds__(kB) void k_sphere_query(
     const float4* __restrict__ X, const float* __restrict__ radius,
     const float* __restrict__ mu, const float* __restrict__ params,
     const int4* __restrict__ cells, const int* __restrict__ slot_start,
     const int* __restrict__ bucket, const int* __restrict__ api,
-    const long long* __restrict__ excl, int n_excl, int T, int n, int* __restrict__ cnt,
-    const int* __restrict__ off, int4* __restrict__ rows, int cap) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
+    const long long* __restrict__ excl, int n_excl, int T, const int* __restrict__ clist,
+    int n, int* __restrict__ cnt, const int* __restrict__ off, int4* __restrict__ rows,
+    int cap) {
+  const int k = blockIdx.x * kB + threadIdx.x;
+  if (k >= n) return;
+  const int i = body_of(clist, k);
   const int ai = api ? api[i] : i;
+  const int rk = clist ? k : ai;
   const int4 ci = cells[i];
   if (ci.w < 0) {
     if (!EMIT)
-      for (int o = 0; o < kOffsets; ++o) cnt[o * n + ai] = 0;
+      for (int o = 0; o < kOffsets; ++o) cnt[o * n + rk] = 0;
     return;
   }
   const float margin = params[0];
@@ -135,7 +150,7 @@ __global__ __launch_bounds__(kB) void k_sphere_query(
     const int3 d = cell_offset(o);
     const int cx = ci.x + d.x, cy = ci.y + d.y, cz = ci.z + d.z;
     const int s = chash(cx, cy, cz, T);
-    const int base = EMIT ? off[o * n + ai] : 0;
+    const int base = EMIT ? off[o * n + rk] : 0;
     int found = 0;
     for (int p = slot_start[s]; p < slot_start[s + 1]; ++p) {
       const int j = bucket[p];
@@ -172,7 +187,7 @@ __global__ __launch_bounds__(kB) void k_sphere_query(
         seg[r + 1] = v;
       }
     } else {
-      cnt[o * n + ai] = found;
+      cnt[o * n + rk] = found;
     }
   }
 }
@@ -230,40 +245,45 @@ long long xpbd_contacts_workspace_bytes(int n) {
 
 int xpbd_sphere_contacts(void* stream, void* ws, const float* X, const float* V,
                          const float* radius, const float* mu, const int* api,
-                         const long long* excl, int n_excl, double contact_margin, double dt,
-                         float r_max, int n, int cap, int* ROWS, int* count) {
-  if (n_excl < 0 || (n_excl > 0 && !excl) || (long long)kOffsets * n >= (1ll << 31))
+                         const int* clist, int nc, const long long* excl, int n_excl,
+                         double contact_margin, double dt, float r_max, int n, int cap,
+                         int* ROWS, int* count) {
+  if (n_excl < 0 || (n_excl > 0 && !excl) || (long long)kOffsets * n >= (1ll << 31) ||
+      (clist && (nc < 0 || nc > n)))
     return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
-  if (n <= 1 || !(r_max > 0.0f)) {
+  const int nbod = clist ? nc : n;  // bodies the candidate search visits
+  if (nbod <= 1 || !(r_max > 0.0f)) {
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(kB), 0, s, count, 0, 1);
     XPBD_LAUNCH_CHECK();
     return 0;
   }
   Ws w;
-  carve((char*)ws, w, n);
-  const int T = slots_for(n);
-  const int nb = nblk(n, kB);
+  carve((char*)ws, w, n);  // sized for every particle: nbod <= n fits
+  const int T = slots_for(nbod);
+  const int nb = nblk(nbod, kB);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(kB), 0, s, w.vmax, 0, 1);
   hipLaunchKernelGGL(k_fill, dim3(nblk(T + 1, kB)), dim3(kB), 0, s, w.slot_count, 0, T + 1);
   hipLaunchKernelGGL(k_fill, dim3(nblk(T, kB)), dim3(kB), 0, s, w.cursor, 0, T);
-  hipLaunchKernelGGL(k_vmax, dim3(min(nb, 1024)), dim3(kB), 0, s, (const float4*)V, n, w.vmax);
+  // the margin uses the largest velocity of every particle (solver.py:309)
+  hipLaunchKernelGGL(k_vmax, dim3(min(nblk(n, kB), 1024)), dim3(kB), 0, s, (const float4*)V, n,
+                     w.vmax);
   hipLaunchKernelGGL(k_contact_params, dim3(1), dim3(1), 0, s, w.vmax, contact_margin, dt, r_max,
                      w.params);
   hipLaunchKernelGGL(k_sphere_cells, dim3(nb), dim3(kB), 0, s, (const float4*)X, radius,
-                     w.params, T, w.cells, w.slot_count, n);
+                     w.params, T, w.cells, w.slot_count, clist, nbod);
   int rc = exclusive_scan(s, w.slot_count, w.slot_start, T + 1, w.scan, nullptr);
   if (rc) return rc;
   hipLaunchKernelGGL(k_sphere_scatter, dim3(nb), dim3(kB), 0, s, w.cells, w.slot_start, w.cursor,
-                     w.bucket, n);
+                     w.bucket, clist, nbod);
   hipLaunchKernelGGL(k_sphere_query<false>, dim3(nb), dim3(kB), 0, s, (const float4*)X, radius,
-                     mu, w.params, w.cells, w.slot_start, w.bucket, api, excl, n_excl, T, n,
-                     w.cnt, w.off, (int4*)ROWS, cap);
-  rc = exclusive_scan(s, w.cnt, w.off, kOffsets * n, w.scan, count);
+                     mu, w.params, w.cells, w.slot_start, w.bucket, api, excl, n_excl, T, clist,
+                     nbod, w.cnt, w.off, (int4*)ROWS, cap);
+  rc = exclusive_scan(s, w.cnt, w.off, kOffsets * nbod, w.scan, count);
   if (rc) return rc;
   hipLaunchKernelGGL(k_sphere_query<true>, dim3(nb), dim3(kB), 0, s, (const float4*)X, radius,
-                     mu, w.params, w.cells, w.slot_start, w.bucket, api, excl, n_excl, T, n,
-                     w.cnt, w.off, (int4*)ROWS, cap);
+                     mu, w.params, w.cells, w.slot_start, w.bucket, api, excl, n_excl, T, clist,
+                     nbod, w.cnt, w.off, (int4*)ROWS, cap);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

@@ -962,6 +962,13 @@ class Solver:
             self.CEXCL = torch.from_numpy(keys).to(dev) if len(keys) else None
             self.CRAD = torch.from_numpy(np.ascontiguousarray(rad)).to(dev)
             self.CMU = torch.from_numpy(np.ascontiguousarray(fr)).to(dev)
+            # the colliding particles' device indices in API order: the
+            # contact search visits these only
+            capi = np.nonzero(col & (rad_api > 0))[0]
+            cl = (pinv[capi] if pinv is not None else capi).astype(np.int32)
+            self._nclist = len(cl)
+            self.CLIST = torch.from_numpy(np.ascontiguousarray(
+                cl if len(cl) else np.zeros(1, np.int32))).to(dev)
             self.CWS = torch.empty(int(N.lib().xpbd_contacts_workspace_bytes(n_b)),
                                    dtype=torch.uint8, device=dev)
             self.CCOUNT = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -1018,6 +1025,7 @@ class Solver:
                 N.current_stream(), N.ptr(self.CWS), N.ptr(self.X), N.ptr(self.V),
                 N.ptr(self.CRAD), N.ptr(self.CMU),
                 None if self.CAPI is None else N.ptr(self.CAPI),
+                N.ptr(self.CLIST), int(self._nclist),
                 None if self.CEXCL is None else N.ptr(self.CEXCL),
                 0 if self.CEXCL is None else int(self.CEXCL.numel()),
                 self._contact_margin, self.dt, self._crmax,

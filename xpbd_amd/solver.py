@@ -513,18 +513,22 @@ class Solver:
     # ---- scene building (solver.py:130-228) ------------------------------
     def add_particle(self, position, mass, velocity=(0.0, 0.0, 0.0), shape=None, collide=True,
                      radius=None, friction=None) -> Body:
-        idx = len(self._x)
-        self._x.append(tuple(float(v) for v in position))
-        self._v.append(tuple(float(v) for v in velocity))
-        self._mass.append(float(mass))
-        sh = shape or Shape()
-        r = float(radius) if radius is not None else _shape_radius(sh)
-        self._radius.append(r)
-        self._collide.append(bool(collide) and r > 0.0 and mass > 0.0)
-        mu = float(friction) if friction is not None else self._default_friction
-        self._friction.append(max(0.0, mu))
+        """One particle (solver.py:130-151).  Its collision radius is
+        ``radius`` or the shape's bounding radius; it collides only when asked
+        to and when it has both a radius and a mass; friction is clamped at 0."""
+        sh = Shape() if shape is None else shape
+        r = _shape_radius(sh) if radius is None else float(radius)
+        m = float(mass)
+        mu = self._default_friction if friction is None else float(friction)
+        handle = Body(index=len(self._x), shape=sh)
+        for column, value in ((self._x, tuple(map(float, position))),
+                              (self._v, tuple(map(float, velocity))),
+                              (self._mass, m), (self._radius, r),
+                              (self._collide, bool(collide) and r > 0.0 and m > 0.0),
+                              (self._friction, max(0.0, mu))):
+            column.append(value)
         self._dirty = True
-        return Body(index=idx, shape=sh)
+        return handle
 
     def add_particles(self, positions, masses, velocities=None, radius=0.0, collide=False,
                       friction=None) -> np.ndarray:
@@ -545,6 +549,22 @@ class Solver:
         self._radius.extend(r)
         self._collide.extend(np.asarray(collide, bool) & (r > 0.0) & (m > 0.0))
         self._friction.extend(mu)
+        self._dirty = True
+        return np.arange(base, base + n, dtype=np.int64)
+
+    def _extend_rows(self, t, a, b, rest=0.0, comp=0.0, anchor=None, mu=0.0) -> np.ndarray:
+        """Append rows of type ``t`` over index arrays (scalars broadcast);
+        returns their row indices."""
+        a = np.asarray(a, np.int64).ravel()
+        n = a.shape[0]
+        base = len(self._rt)
+        self._rt.extend(np.full(n, t, np.int32))
+        self._ra.extend(a)
+        self._rb.extend(np.asarray(b, np.int64).ravel())
+        self._rrest.extend(np.broadcast_to(np.asarray(rest, np.float64), (n,)))
+        self._rcomp.extend(np.broadcast_to(np.asarray(comp, np.float64), (n,)))
+        self._ranch.extend(np.zeros((n, 3)) if anchor is None else np.asarray(anchor))
+        self._rmu.extend(np.broadcast_to(np.asarray(mu, np.float64), (n,)))
         self._dirty = True
         return np.arange(base, base + n, dtype=np.int64)
 
@@ -578,16 +598,7 @@ class Solver:
         if rest is None:
             x = self._x.array()
             rest = np.linalg.norm(x[a] - x[b], axis=1)
-        base = len(self._rt)
-        self._rt.extend(np.full(n, DISTANCE, np.int32))
-        self._ra.extend(a)
-        self._rb.extend(b)
-        self._rrest.extend(np.broadcast_to(np.asarray(rest, np.float64), (n,)))
-        self._rcomp.extend(np.broadcast_to(np.asarray(compliance, np.float64), (n,)))
-        self._ranch.extend(np.zeros((n, 3), np.float32))
-        self._rmu.extend(np.zeros(n, np.float32))
-        self._dirty = True
-        return np.arange(base, base + n, dtype=np.int64)
+        return self._extend_rows(DISTANCE, a, b, rest=rest, comp=compliance)
 
     def add_attach(self, body, world_point, compliance=0.0, rest=0.0) -> ConstraintHandle:
         """Compliant pin of a particle to a world point."""
@@ -610,16 +621,7 @@ class Solver:
               else np.broadcast_to(np.asarray(friction, np.float64), (n,)))
         anch = np.zeros((n, 3))
         anch[:, 1] = float(floor_y)
-        base = len(self._rt)
-        self._rt.extend(np.full(n, FLOOR))
-        self._ra.extend(a)
-        self._rb.extend(np.full(n, -1))
-        self._rrest.extend(np.zeros(n))
-        self._rcomp.extend(np.zeros(n))
-        self._ranch.extend(anch)
-        self._rmu.extend(np.maximum(0.0, mu))
-        self._dirty = True
-        return np.arange(base, base + n, dtype=np.int64)
+        return self._extend_rows(FLOOR, a, np.full(n, -1), anchor=anch, mu=np.maximum(0.0, mu))
 
     def enable_self_collision(self, enabled=True, margin=0.0, default_friction=None) -> None:
         """Per-frame sphere-sphere CONTACT generation between ``collide`` bodies."""
@@ -750,51 +752,49 @@ class Solver:
 
     # ---- host broad phase (solver.py:275-342) ------------------------------
     def _rebuild_contacts(self) -> None:
-        """Drop last frame's CONTACT rows and regenerate them from the current
-        positions, skipping pairs already joined by a constraint."""
+        """This frame's CONTACT rows (solver.py:275-342): last frame's are
+        dropped, then every pair of colliding bodies within reach gets one,
+        in the candidate order of ``grid_candidate_pairs``, unless a
+        constraint already joins the pair.  Reach is the sum of the radii
+        plus a margin that grows with the fastest velocity component
+        (margin + 3 dt max|v|, at most the largest radius); the pair test
+        and the row fields are float32 as on the device."""
         if not self._self_collision:
             return
-        t = self._rt.array()
-        keep_rows = t != CONTACT
+        static = self._rt.array() != CONTACT
         for col in (self._rt, self._ra, self._rb, self._rrest, self._rcomp, self._ranch,
                     self._rmu):
-            col.set(col.array()[keep_rows])
+            col.set(col.array()[static])
         self._contact_pool_start = len(self._rt)
+        self._dirty = True
+        bodies = np.flatnonzero(self._collide.array().astype(bool))
+        if bodies.size < 2:
+            return
+        x = self.positions() if self.X is not None else self._x.array().astype(np.float32)
+        rad = self._radius.array().astype(np.float32)
+        fr = self._friction.array().astype(np.float32)
+        fastest = float(np.abs(self.velocities()).max()) if self.V is not None else 0.0
+        r_big = float(rad[bodies].max())
+        margin = min(self._contact_margin + 3.0 * self.dt * fastest, r_big)
+        la, lb = grid_candidate_pairs(x[bodies], r_big * 2.0 + margin)
+        if la.size == 0:
+            return
+        ia, ib = bodies[la], bodies[lb]
+        gap = x[ia] - x[ib]
+        reach = rad[ia] + rad[ib] + margin
+        near = np.einsum("ij,ij->i", gap, gap) <= reach * reach
+        ia, ib = ia[near], ib[near]
+        # drop pairs a constraint joins: keys lo << 32 | hi of both sets
         ra, rb = self._ra.array(), self._rb.array()
         two = rb >= 0
-        existing = set(zip(np.minimum(ra[two], rb[two]).tolist(),
-                           np.maximum(ra[two], rb[two]).tolist()))
-        pos = (self.positions() if self.X is not None
-               else self._x.array().astype(np.float32))
-        collide = self._collide.array().astype(bool)
-        idx = np.nonzero(collide)[0]
-        if len(idx) < 2:
-            self._dirty = True
-            return
-        radii = self._radius.array().astype(np.float32)
-        friction = self._friction.array().astype(np.float32)
-        vmax = float(np.abs(self.velocities()).max()) if self.V is not None else 0.0
-        rmax = float(radii[idx].max())
-        margin = min(self._contact_margin + 3.0 * self.dt * vmax, rmax)
-        ga, gb = grid_candidate_pairs(pos[idx], float(radii[idx].max()) * 2.0 + margin)
-        if len(ga) == 0:
-            self._dirty = True
-            return
-        a_glob = idx[ga]
-        b_glob = idx[gb]
-        d = pos[a_glob] - pos[b_glob]
-        d2 = np.einsum("ij,ij->i", d, d)
-        rs = radii[a_glob] + radii[b_glob] + margin
-        keep = d2 <= rs * rs
-        a_glob = a_glob[keep]
-        b_glob = b_glob[keep]
-        mus = np.sqrt(friction[a_glob] * friction[b_glob])
-        rest = radii[a_glob] + radii[b_glob]
-        for a, b, mu, rst in zip(a_glob.tolist(), b_glob.tolist(), mus.tolist(), rest.tolist()):
-            if (min(a, b), max(a, b)) in existing:
-                continue
-            self._add_row(CONTACT, a, b, rest=float(rst), mu=float(mu))
-        self._dirty = True
+        joined = np.unique((np.minimum(ra[two], rb[two]).astype(np.int64) << 32)
+                           | np.maximum(ra[two], rb[two]).astype(np.int64))
+        key = (np.minimum(ia, ib) << 32) | np.maximum(ia, ib)
+        free = ~np.isin(key, joined, assume_unique=False)
+        ia, ib = ia[free], ib[free]
+        if ia.size:
+            self._extend_rows(CONTACT, ia, ib, rest=rad[ia] + rad[ib],
+                              mu=np.sqrt(fr[ia] * fr[ib]))
 
     # ---- flush: host lists -> device buffers (solver.py:346-408) ----------
     def host_arrays(self) -> dict:
@@ -1387,38 +1387,36 @@ _FWD_OFFSETS = (
 
 def grid_candidate_pairs(p: np.ndarray, cell: float) -> tuple[np.ndarray, np.ndarray]:
     """Candidate pairs of bodies in the same or an adjacent grid cell of side
-    ``cell`` (solver.py:520-562), in the reference's enumeration order: offset
-    by offset, body by body, partners in ascending index within a cell."""
+    ``cell`` (solver.py:520-562), in the reference's enumeration order: the
+    cell itself and then the 13 forward neighbour cells (``_FWD_OFFSETS``),
+    each as one block; inside a block body by body in ascending index, and
+    for each body the members of the target cell in ascending index (its
+    own cell: only those after it).  Computed for all 14 offsets at once
+    against the table of occupied cells."""
     m = p.shape[0]
     if m < 2 or cell <= 0.0:
         return np.zeros(0, np.int64), np.zeros(0, np.int64)
-    c = np.floor(p / cell).astype(np.int64)
-    c -= c.min(axis=0)
-    S = int(c.max()) + 2
-    cid = (c[:, 0] * S + c[:, 1]) * S + c[:, 2]
-    # bucket bodies by cell id: members of a cell in ascending body index
-    by_cell = np.argsort(cid, kind="stable")
-    sorted_cid = cid[by_cell]
-    bodies = np.arange(m, dtype=np.int64)
-    out_a, out_b = [], []
-    for off in ((0, 0, 0),) + _FWD_OFFSETS:
-        target = cid + (off[0] * S + off[1]) * S + off[2]
-        first = np.searchsorted(sorted_cid, target, "left")
-        cnt = np.searchsorted(sorted_cid, target, "right") - first
-        total = int(cnt.sum())
-        if total == 0:
-            continue
-        owner = np.repeat(bodies, cnt)
-        run_start = np.repeat(np.cumsum(cnt) - cnt, cnt)
-        partner = by_cell[np.repeat(first, cnt) + (np.arange(total) - run_start)]
-        if off == (0, 0, 0):
-            sel = partner > owner
-            owner, partner = owner[sel], partner[sel]
-        out_a.append(owner)
-        out_b.append(partner)
-    if not out_a:
+    ijk = np.floor(p / cell).astype(np.int64)
+    ijk -= ijk.min(axis=0)
+    span = int(ijk.max()) + 2              # room for the +1 neighbour on every axis
+    key = (ijk[:, 0] * span + ijk[:, 1]) * span + ijk[:, 2]
+    members = np.argsort(key, kind="stable")  # bodies cell by cell, ascending inside
+    occupied, first, size = np.unique(key[members], return_index=True, return_counts=True)
+    shift = np.array([(dx * span + dy) * span + dz for dx, dy, dz in ((0, 0, 0),) + _FWD_OFFSETS],
+                     np.int64)
+    target = shift[:, None] + key[None, :]     # (offset, body), offset-major
+    slot = np.minimum(np.searchsorted(occupied, target), len(occupied) - 1)
+    hit = occupied[slot] == target
+    fan = np.where(hit, size[slot], 0).ravel()
+    total = int(fan.sum())
+    if total == 0:
         return np.zeros(0, np.int64), np.zeros(0, np.int64)
-    return np.concatenate(out_a), np.concatenate(out_b)
+    owner = np.repeat(np.tile(np.arange(m, dtype=np.int64), len(shift)), fan)
+    block = np.repeat(np.repeat(np.arange(len(shift)), m), fan)
+    nth = np.arange(total) - np.repeat(np.cumsum(fan) - fan, fan)
+    partner = members[np.repeat(first[slot].ravel(), fan) + nth]
+    keep = (block > 0) | (partner > owner)
+    return owner[keep], partner[keep]
 
 
 def _shape_radius(shape) -> float:

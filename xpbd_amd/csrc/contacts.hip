@@ -82,16 +82,24 @@ __global__ __launch_bounds__(kB) void k_sphere_cells(const float4* __restrict__ 
   atomicAdd(&slot_count[s], 1);
 }
 
-__global__ __launch_bounds__(kB) void k_sphere_scatter(const int4* __restrict__ cells,
-                                                       const int* __restrict__ slot_start,
-                                                       int* cursor, int* bucket,
-                                                       const int* __restrict__ clist, int nb) {
+// Bucket records in slot order, so that a candidate is one load of each:
+// bkey = (cell x, y, z, API index), bpos = (x, y, z, radius), bref = (device
+// index, friction bits).
+__global__ __launch_bounds__(kB) void k_sphere_scatter(
+    const int4* __restrict__ cells, const int* __restrict__ slot_start, int* cursor,
+    const float4* __restrict__ X, const float* __restrict__ radius, const float* __restrict__ mu,
+    const int* __restrict__ api, int4* __restrict__ bkey, float4* __restrict__ bpos,
+    int2* __restrict__ bref, const int* __restrict__ clist, int nb) {
   const int k = blockIdx.x * kB + threadIdx.x;
   if (k >= nb) return;
   const int i = body_of(clist, k);
-  const int s = cells[i].w;
-  if (s < 0) return;
-  bucket[slot_start[s] + atomicAdd(&cursor[s], 1)] = i;
+  const int4 c = cells[i];
+  if (c.w < 0) return;
+  const int p = slot_start[c.w] + atomicAdd(&cursor[c.w], 1);
+  const float4 x = X[i];
+  bkey[p] = make_int4(c.x, c.y, c.z, api ? api[i] : i);
+  bpos[p] = make_float4(x.x, x.y, x.z, radius[i]);
+  bref[p] = make_int2(i, mu ? __float_as_int(mu[i]) : 0);
 }
 
 constexpr int kOffsets = 14;
@@ -118,77 +126,76 @@ __device__ inline bool excluded(const long long* __restrict__ excl, int n_excl, 
   return l < n_excl && excl[l] == key;
 }
 
-// Bodies in API order: with clist, body k (device index clist[k]) has
-// rank k; without, device body i has rank api(i).  EMIT = false:
-// cnt[o nb + rank] = number of rows the body contributes at offset o.
-// EMIT = true: those rows at off[o nb + rank], partners in ascending API
-// index.
+// One thread per (offset o, body k), offset-major.  Bodies in API order:
+// with clist, body k (device index clist[k]) has rank k; without, device
+// body i has rank api(i).  EMIT = false: cnt[o nb + rank] = number of rows
+// the body contributes at offset o.  EMIT = true: those rows at
+// off[o nb + rank], partners in ascending API index.
 template <bool EMIT>
 __global__ __launch_bounds__(kB) void k_sphere_query(
     const float4* __restrict__ X, const float* __restrict__ radius,
     const float* __restrict__ mu, const float* __restrict__ params,
     const int4* __restrict__ cells, const int* __restrict__ slot_start,
-    const int* __restrict__ bucket, const int* __restrict__ api,
+    const int4* __restrict__ bkey, const float4* __restrict__ bpos,
+    const int2* __restrict__ bref, const int* __restrict__ api,
     const long long* __restrict__ excl, int n_excl, int T, const int* __restrict__ clist,
     int n, int* __restrict__ cnt, const int* __restrict__ off, int4* __restrict__ rows,
     int cap) {
-  const int k = blockIdx.x * kB + threadIdx.x;
-  if (k >= n) return;
+  const long long t = (long long)blockIdx.x * kB + threadIdx.x;
+  if (t >= (long long)kOffsets * n) return;
+  const int o = (int)(t / n), k = (int)(t - (long long)o * n);
   const int i = body_of(clist, k);
   const int ai = api ? api[i] : i;
   const int rk = clist ? k : ai;
   const int4 ci = cells[i];
   if (ci.w < 0) {
-    if (!EMIT)
-      for (int o = 0; o < kOffsets; ++o) cnt[o * n + rk] = 0;
+    if (!EMIT) cnt[o * n + rk] = 0;
     return;
   }
   const float margin = params[0];
   const float4 xi = X[i];
   const float ri = radius[i];
-  for (int o = 0; o < kOffsets; ++o) {
-    const int3 d = cell_offset(o);
-    const int cx = ci.x + d.x, cy = ci.y + d.y, cz = ci.z + d.z;
-    const int s = chash(cx, cy, cz, T);
-    const int base = EMIT ? off[o * n + rk] : 0;
-    int found = 0;
-    for (int p = slot_start[s]; p < slot_start[s + 1]; ++p) {
-      const int j = bucket[p];
-      if (j == i) continue;
-      const int4 cj = cells[j];
-      if (cj.x != cx || cj.y != cy || cj.z != cz) continue;
-      const int aj = api ? api[j] : j;
-      if (o == 0 && aj <= ai) continue;
-      const float4 xj = X[j];
-      const float d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
-      const float dd = d0 * d0 + d1 * d1 + d2 * d2;
-      const float rs = ri + radius[j] + margin;
-      if (!(dd <= rs * rs)) continue;
-      if (n_excl > 0 && excluded(excl, n_excl, ai, aj)) continue;
-      if (EMIT && base + found < cap) {
-        const float rest = ri + radius[j];
-        const float m = sqrtf(mu[i] * mu[j]);
-        rows[base + found] =
-            make_int4(i | (T_CONTACT << 28), j, __float_as_int(rest), __float_as_int(m));
-      }
-      ++found;
+  const int3 d = cell_offset(o);
+  const int cx = ci.x + d.x, cy = ci.y + d.y, cz = ci.z + d.z;
+  const int s = chash(cx, cy, cz, T);
+  const int base = EMIT ? off[o * n + rk] : 0;
+  int found = 0;
+  const int pe = slot_start[s + 1];
+  for (int p = slot_start[s]; p < pe; ++p) {
+    const int4 key = bkey[p];
+    const float4 xj = bpos[p];
+    if (key.x != cx || key.y != cy || key.z != cz) continue;
+    const int aj = key.w;
+    if (aj == ai || (o == 0 && aj <= ai)) continue;
+    const float d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
+    const float dd = d0 * d0 + d1 * d1 + d2 * d2;
+    const float rs = ri + xj.w + margin;
+    if (!(dd <= rs * rs)) continue;
+    if (n_excl > 0 && excluded(excl, n_excl, ai, aj)) continue;
+    if (EMIT && base + found < cap) {
+      const int2 ref = bref[p];
+      const float rest = ri + xj.w;
+      const float m = sqrtf(mu[i] * __int_as_float(ref.y));
+      rows[base + found] =
+          make_int4(i | (T_CONTACT << 28), ref.x, __float_as_int(rest), __float_as_int(m));
     }
-    if (EMIT) {  // partners in ascending API index (insertion sort of the segment)
-      int4* seg = rows + base;
-      const int m = min(found, max(cap - base, 0));
-      for (int q = 1; q < m; ++q) {
-        const int4 v = seg[q];
-        const int kv = api ? api[v.y] : v.y;
-        int r = q - 1;
-        while (r >= 0 && (api ? api[seg[r].y] : seg[r].y) > kv) {
-          seg[r + 1] = seg[r];
-          --r;
-        }
-        seg[r + 1] = v;
+    ++found;
+  }
+  if (EMIT) {  // partners in ascending API index (insertion sort of the segment)
+    int4* seg = rows + base;
+    const int m = min(found, max(cap - base, 0));
+    for (int q = 1; q < m; ++q) {
+      const int4 v = seg[q];
+      const int kv = api ? api[v.y] : v.y;
+      int r = q - 1;
+      while (r >= 0 && (api ? api[seg[r].y] : seg[r].y) > kv) {
+        seg[r + 1] = seg[r];
+        --r;
       }
-    } else {
-      cnt[o * n + rk] = found;
+      seg[r + 1] = v;
     }
+  } else {
+    cnt[o * n + rk] = found;
   }
 }
 
@@ -207,7 +214,10 @@ struct Ws {
   int* vmax;
   float* params;
   int4* cells;
-  int *slot_count, *slot_start, *cursor, *bucket, *cnt, *off;
+  int *slot_count, *slot_start, *cursor, *cnt, *off;
+  int4* bkey;
+  float4* bpos;
+  int2* bref;
   ScanWs scan;
 };
 
@@ -226,7 +236,9 @@ size_t carve(char* base, Ws& w, int n) {
   w.slot_count = (int*)take(sizeof(int) * ((size_t)T + 1));
   w.slot_start = (int*)take(sizeof(int) * ((size_t)T + 1));
   w.cursor = (int*)take(sizeof(int) * (size_t)T);
-  w.bucket = (int*)take(sizeof(int) * (size_t)n);
+  w.bkey = (int4*)take(sizeof(int4) * (size_t)n);
+  w.bpos = (float4*)take(sizeof(float4) * (size_t)n);
+  w.bref = (int2*)take(sizeof(int2) * (size_t)n);
   const long long m = (long long)kOffsets * n;
   w.cnt = (int*)take(sizeof(int) * ((size_t)m + 1));
   w.off = (int*)take(sizeof(int) * ((size_t)m + 1));
@@ -275,15 +287,16 @@ int xpbd_sphere_contacts(void* stream, void* ws, const float* X, const float* V,
   int rc = exclusive_scan(s, w.slot_count, w.slot_start, T + 1, w.scan, nullptr);
   if (rc) return rc;
   hipLaunchKernelGGL(k_sphere_scatter, dim3(nb), dim3(kB), 0, s, w.cells, w.slot_start, w.cursor,
-                     w.bucket, clist, nbod);
-  hipLaunchKernelGGL(k_sphere_query<false>, dim3(nb), dim3(kB), 0, s, (const float4*)X, radius,
-                     mu, w.params, w.cells, w.slot_start, w.bucket, api, excl, n_excl, T, clist,
-                     nbod, w.cnt, w.off, (int4*)ROWS, cap);
+                     (const float4*)X, radius, mu, api, w.bkey, w.bpos, w.bref, clist, nbod);
+  const int nq = nblk((long long)kOffsets * nbod, kB);
+  hipLaunchKernelGGL(k_sphere_query<false>, dim3(nq), dim3(kB), 0, s, (const float4*)X, radius,
+                     mu, w.params, w.cells, w.slot_start, w.bkey, w.bpos, w.bref, api, excl,
+                     n_excl, T, clist, nbod, w.cnt, w.off, (int4*)ROWS, cap);
   rc = exclusive_scan(s, w.cnt, w.off, kOffsets * nbod, w.scan, count);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_sphere_query<true>, dim3(nb), dim3(kB), 0, s, (const float4*)X, radius,
-                     mu, w.params, w.cells, w.slot_start, w.bucket, api, excl, n_excl, T, clist,
-                     nbod, w.cnt, w.off, (int4*)ROWS, cap);
+  hipLaunchKernelGGL(k_sphere_query<true>, dim3(nq), dim3(kB), 0, s, (const float4*)X, radius,
+                     mu, w.params, w.cells, w.slot_start, w.bkey, w.bpos, w.bref, api, excl,
+                     n_excl, T, clist, nbod, w.cnt, w.off, (int4*)ROWS, cap);
   XPBD_LAUNCH_CHECK();
   return 0;
 }

@@ -11,9 +11,10 @@ Metric: constraint projections per second = rows x substeps x iterations /
 frame time (the reference's definition, examples/benchmark.py:123-130).
 
 Also reported on the same JSON line:
-* ``roofline``       — the dominant kernel (k_jacobi_tile: one fused Jacobi
-  iteration) at this workload: algorithmic bytes per launch / average launch
-  time, timed with HIP events on the launch stream, against 8 TB/s.
+* ``roofline``       — the kernel that dominates the headline frame (the
+  two-iteration k_jacobi_tile_pair launch when the plan uses it, else one
+  fused k_jacobi_tile iteration): algorithmic bytes per launch / average
+  launch time, timed with HIP events on the launch stream, against 8 TB/s.
 * ``roofline_scene`` — the same kernel and a full frame on 8 x 1024^2 cloths
   (58.6M rows, a working set far above the caches), built on the device.
 * ``cpu_baseline``   — the serial CPU restatement of the reference kernels
@@ -627,7 +628,10 @@ def scene_only(name: str, frames: int, warmup: int) -> dict:
     (the reference's N-box LBVH pile, N in 27, 100, 384, 1440), ``cloth128``
     (the headline cloth, Jacobi), ``stack-gs-device`` (the 8 x 8 x 5 sphere
     stack in GS mode with device CONTACT rows), ``tile-kernel`` (the
-    headline cloth's plain k_jacobi_tile launch replayed from a graph)."""
+    headline cloth's dominant launch replayed from a graph),
+    ``batched-plain4`` / ``batched-learned1`` (the batched world's particle
+    solver with four plain iterations, or one after the learned start with
+    its network fitted on the world first, judge on every 8th frame)."""
     import torch
 
     from xpbd_amd import Solver, Solver6DOF
@@ -645,6 +649,31 @@ def scene_only(name: str, frames: int, warmup: int) -> dict:
         s = sphere_stack(Solver, 8, 5, "gs", device="cuda:0")
         s.enable_self_collision(False)
         s.enable_device_contacts(True, margin=0.0)
+    elif name in ("batched-plain4", "batched-learned1"):
+        from xpbd_amd import warmstart as ws
+        from xpbd_amd.batch import BatchedWorld, draw
+
+        w = BatchedWorld(draw(1024, 0), device="cuda:0")
+        s = w.particles
+        s.block_frames = False
+        s.step(2)
+        torch.cuda.synchronize()
+        if name == "batched-learned1":
+            x0, v0 = s.positions(), s.velocities()
+
+            def restore():
+                if s._ws_theta is not None:
+                    s.set_warm_start(None)
+                s.iterations = 2
+                s.set_state(x0, v0)
+                return s
+
+            theta = ws.fit(restore, substeps=60, settle_frames=1, steps=800, net_every=15,
+                           pair_with_net=True, every=5)
+            restore()
+            s.warm_start_every = 15
+            s.set_warm_start(theta)
+        s.iterations = 1 if name == "batched-learned1" else 4
     else:
         raise SystemExit(f"unknown scene {name!r}")
     if name == "tile-kernel":

@@ -158,7 +158,7 @@ def headline_launch(N, s):
         def fn():
             N.check(lib.xpbd_jacobi_tile_pair(
                 N.current_stream(), *ptrs, h, s._tile, s.num_bodies, s.jacobi_relax, s.damping,
-                s.max_speed, *s.gravity, None, lmax, amax, k1max, 0), "xpbd_jacobi_tile_pair")
+                s.max_speed, *s.gravity, None, lmax, amax, k1max, 0, 0), "xpbd_jacobi_tile_pair")
         return fn, "k_jacobi_tile_pair", 2
     fn = tile_launcher(N, s.X, s.X_B, s.XP, s.V, s.R, s.ANC if s._has_attach else None, s.LAM,
                        s.LAM_B, ts, hs, hr, h, s._tile, s.num_bodies, cx=s._cx_struct)

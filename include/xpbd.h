@@ -397,7 +397,10 @@ typedef struct xpbd_particles_desc {
  * phase-1 rows with local endpoints, phase-2 rows).  x_prev is read from
  * XPin and written to XPout (!= XPin unless last): the next substep's start
  * positions, or with `last` a copy of XPin.  Multipliers of the second
- * iteration go to LAMout (owner tiles).  Dynamic LDS:
+ * iteration go to LAMout (owner tiles).  predict = 1 (not with last): the
+ * substep's integrate too (solver.py:427, kernels.py:49-73; the frame's first
+ * substep, as xpbd_jacobi_tile_first): Xin and V are the substep's start,
+ * x_prev is Xin (XPin is not read).  Dynamic LDS:
  * xpbd_jacobi_tile_pair_lds_bytes(lmax, amax, k1max) <= 64 KiB. */
 int xpbd_jacobi_tile_pair(void* stream, const float* Xin, float* Xout, const float* XPin,
                           float* XPout, float* V, const int* R, const float* ANC, float* LAMout,
@@ -405,7 +408,7 @@ int xpbd_jacobi_tile_pair(void* stream, const float* Xin, float* Xout, const flo
                           const int* p1_start, const int* p1, const int* p2_start, const int* p2,
                           float h, int tile_size, int n, float relax, float damping,
                           float max_speed, float gx, float gy, float gz, const float* gscale,
-                          int lmax, int amax, int k1max, int last);
+                          int lmax, int amax, int k1max, int last, int predict);
 size_t xpbd_jacobi_tile_pair_lds_bytes(int lmax, int amax, int k1max);
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);

@@ -148,7 +148,8 @@ _SIGS = {
     "xpbd6_velocity_all": (_I, [_P] * 5 + [_F, _F, _P, _F, _P, _P, _I] + [_P] * 6 + [_I, _P, _P]),
     "xpbd_frame6_create": (_I, [ctypes.POINTER(RigidDesc), ctypes.POINTER(c_void_p)]),
     "xpbd_frame6_run": (_I, [_P, _P, _I, _I]),
-    "xpbd_jacobi_tile_pair": (_I, [_P] * 16 + [_F, _I, _I, _F, _F, _F, _F, _F, _F, _P, _I, _I, _I, _I]),
+    "xpbd_jacobi_tile_pair": (_I, [_P] * 16 + [_F, _I, _I, _F, _F, _F, _F, _F, _F, _P, _I, _I, _I, _I,
+                                              _I]),
     "xpbd_jacobi_tile_pair_lds_bytes": (ctypes.c_size_t, [_I, _I, _I]),
     "xpbd6_substep_contacts": (_I, [_P, _P, _I]),
     "xpbd6_substep_velocity": (_I, [_P, _P, _I]),

@@ -47,6 +47,9 @@ struct Frame3 {
 // first launch is then a plain one with first = 1.  Launches
 // ping-pong between X and X_alt; `loc` tracks which one holds the state, and
 // a frame that ends in X_alt copies it back.
+// With the two-iteration tables (pair_p1; no device rows, no learned start)
+// each substep is one xpbd_jacobi_tile_pair launch, the frame's first with
+// its integrate folded in (predict) unless it is also the last.
 //
 // Otherwise (contact rows projected by a separate launch before each
 // iteration, or fuse_first off) a substep is one integrate pass (X -> X or
@@ -152,6 +155,20 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
     r.src0 = nullptr;
     r.loc ^= 1;
     r.li = 1;
+  } else if (!predicted && it_n == 2 && d.pair_p1 && r.xp_alt && !dyn && integrate_next) {
+    // the frame's first substep as one pair launch with the integrate folded
+    // in (x_prev = the start positions, written to the other x_prev buffer)
+    rc = xpbd_jacobi_tile_pair(s, in(), bufs[r.loc ^ 1], r.xp, r.xp_alt, d.v, d.rows, d.anchors,
+                               lams[0], d.pair_ring_start, d.pair_ring1_n, d.pair_ring_idx,
+                               d.pair_p1_start, d.pair_p1, d.pair_p2_start, d.pair_p2, d.h,
+                               d.tile_size, n, d.relax, d.damping, d.max_speed, d.gx, d.gy, d.gz,
+                               d.gscale, d.pair_lmax, d.pair_amax, d.pair_k1max, 0, 1);
+    if (rc) return rc;
+    r.src0 = nullptr;
+    r.loc ^= 1;
+    r.li = 0;
+    std::swap(r.xp, r.xp_alt);
+    return 0;
   } else if (!predicted && it_n >= 2) {
     if (dyn) {  // device-counted rows, first iteration on the predicted positions
       rc = xpbd_solve_jacobi_dyn_first(s, in(), d.v, d.dyn_rows, r.dlam, r.acc, d.h, d.gx, d.gy,
@@ -179,7 +196,7 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
                                d.pair_ring_idx, d.pair_p1_start, d.pair_p1, d.pair_p2_start,
                                d.pair_p2, d.h, d.tile_size, n, d.relax, d.damping, d.max_speed,
                                d.gx, d.gy, d.gz, d.gscale, d.pair_lmax, d.pair_amax,
-                               d.pair_k1max, last ? 1 : 0);
+                               d.pair_k1max, last ? 1 : 0, 0);
     if (rc) return rc;
     r.loc ^= 1;
     r.li = 0;
@@ -219,6 +236,9 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
 // first launch is then a plain one with first = 1.  Launches
 // ping-pong between X and X_alt; `loc` tracks which one holds the state, and
 // a frame that ends in X_alt copies it back.
+// With the two-iteration tables (pair_p1; no device rows, no learned start)
+// each substep is one xpbd_jacobi_tile_pair launch, the frame's first with
+// its integrate folded in (predict) unless it is also the last.
 //
 // Otherwise (contact rows projected by a separate launch before each
 // iteration, or fuse_first off) a substep is one integrate pass (X -> X or

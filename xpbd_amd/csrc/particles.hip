@@ -808,6 +808,10 @@ __global__ __launch_bounds__(512) void k_jacobi_tile_cx(
 // (xpbd_jacobi_tile with first = 1) followed by the second with finalize
 // and, unless LAST, the next substep's integrate (xpbd_jacobi_tile_next), or
 // finalize + cap (xpbd_jacobi_tile with finalize) when LAST.
+// PREDICT: the substep's integrate too (the frame's first substep,
+// xpbd_jacobi_tile_first): every local particle is predicted from X and V
+// at staging, its x_prev is X (XPin is not read); not with LAST, whose
+// velocity stores other blocks would read.
 //
 // A tile's second iteration reads the first iteration's positions of the
 // particles one row away (its ring 1), so the block also runs the first
@@ -850,7 +854,7 @@ __host__ __device__ inline size_t tile_pair_lds_bytes(int lmax, int amax, int k1
          (size_t)lmax * sizeof(int) + (size_t)k1max * (sizeof(int) + sizeof(float));
 }
 
-template <int BLOCK, bool LAST>
+template <int BLOCK, bool LAST, bool PREDICT>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
     const float4* __restrict__ Xin, float4* __restrict__ Xout, const float4* XPin,
     float4* XPout, float4* __restrict__ V, const int4* __restrict__ R,
@@ -919,8 +923,21 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
     }
     gidx[l] = g;
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    spos[l] = g >= 0 ? Xin[g] : z;
-    sprev[l] = g >= 0 ? XPin[g] : z;
+    float4 x4 = g >= 0 ? Xin[g] : z;
+    if (PREDICT) {  // integrate (kernels.py:49-73): x_prev = x, x += (v + g h) h
+      const float4 v4 = g >= 0 ? V[g] : z;
+      sprev[l] = make_float4(x4.x, x4.y, x4.z, 0.0f);
+      if (g >= 0 && x4.w != 0.0f) {
+        f3 gv = mk3(gx, gy, gz);
+        if (gscale) gv = gv * gscale[g];
+        const f3 vn = xyz(v4) + gv * h;
+        const f3 x = xyz(x4) + vn * h;
+        x4 = make_float4(x.x, x.y, x.z, x4.w);
+      }
+    } else {
+      sprev[l] = g >= 0 ? XPin[g] : z;
+    }
+    spos[l] = x4;
   }
   __syncthreads();
   auto pos = [&](int l) { return spos[l]; };
@@ -1864,30 +1881,32 @@ int xpbd_jacobi_tile_pair(void* stream, const float* Xin, float* Xout, const flo
                           const int* p1_start, const int* p1, const int* p2_start, const int* p2,
                           float h, int tile_size, int n, float relax, float damping,
                           float max_speed, float gx, float gy, float gz, const float* gscale,
-                          int lmax, int amax, int k1max, int last) {
+                          int lmax, int amax, int k1max, int last, int predict) {
   if (n <= 0) return 0;
   if (tile_size < 1 || tile_size > XPBD_MAX_TILE || amax < tile_size || lmax < amax ||
       k1max < 0 || !Xin || !Xout || !XPin || !XPout || !R || !LAMout || !ring_start ||
-      !ring1_n || !ring_idx || !p1_start || !p1 || !p2_start || !p2 || (last && !V) ||
-      Xin == Xout || (XPin == XPout && !last))
+      !ring1_n || !ring_idx || !p1_start || !p1 || !p2_start || !p2 || ((last || predict) && !V) ||
+      (last && predict) || Xin == Xout || (XPin == XPout && !last))
     return (int)hipErrorInvalidValue;
   const size_t shmem = tile_pair_lds_bytes(lmax, amax, k1max);
   if (shmem > kPairLdsMax) return (int)hipErrorInvalidValue;
   const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
   const int nb = blocks_for(n, tile_size);
   hipStream_t s = (hipStream_t)stream;
+#define XPBD_PAIR_LAUNCH(L, PR)                                                               \
+  hipLaunchKernelGGL((k_jacobi_tile_pair<kPairBlock, L, PR>), dim3(nb), dim3(kPairBlock),      \
+                     shmem, s, (const float4*)Xin, (float4*)Xout, (const float4*)XPin,         \
+                     (float4*)XPout, (float4*)V, (const int4*)R, (const float4*)ANC, LAMout,   \
+                     ring_start, ring1_n, ring_idx, p1_start, (const int4*)p1, p2_start, p2, h, \
+                     tile_size, n, relax, damping, max_speed, do_cap, gx, gy, gz, gscale, lmax, \
+                     amax, k1max)
   if (last)
-    hipLaunchKernelGGL((k_jacobi_tile_pair<kPairBlock, true>), dim3(nb), dim3(kPairBlock), shmem,
-                       s, (const float4*)Xin, (float4*)Xout, (const float4*)XPin, (float4*)XPout, (float4*)V, (const int4*)R,
-                       (const float4*)ANC, LAMout, ring_start, ring1_n, ring_idx, p1_start,
-                       (const int4*)p1, p2_start, p2, h, tile_size, n, relax, damping, max_speed,
-                       do_cap, gx, gy, gz, gscale, lmax, amax, k1max);
+    XPBD_PAIR_LAUNCH(true, false);
+  else if (predict)
+    XPBD_PAIR_LAUNCH(false, true);
   else
-    hipLaunchKernelGGL((k_jacobi_tile_pair<kPairBlock, false>), dim3(nb), dim3(kPairBlock),
-                       shmem, s, (const float4*)Xin, (float4*)Xout, (const float4*)XPin, (float4*)XPout, (float4*)V,
-                       (const int4*)R, (const float4*)ANC, LAMout, ring_start, ring1_n, ring_idx,
-                       p1_start, (const int4*)p1, p2_start, p2, h, tile_size, n, relax, damping,
-                       max_speed, do_cap, gx, gy, gz, gscale, lmax, amax, k1max);
+    XPBD_PAIR_LAUNCH(false, false);
+#undef XPBD_PAIR_LAUNCH
   XPBD_LAUNCH_CHECK();
   return 0;
 }

@@ -161,6 +161,10 @@ typedef struct xpbd_tile_compact {
   const float* pf;
   float* pflam;
   int general;
+  /* optional, launches with device-row sums (ACC): acc_tiles[t] = 0 when no
+   * particle of tile t can be an endpoint of a device row (no colliding
+   * particle), so the tile skips its ACC reads; NULL: every tile reads */
+  const int* acc_tiles;
 } xpbd_tile_compact;
 
 /* xpbd_jacobi_tile, xpbd_jacobi_tile_first and xpbd_jacobi_tile_next over

@@ -54,8 +54,8 @@ class ParticlesDesc(ctypes.Structure):
 class TileCompact(ctypes.Structure):
     """Mirror of ``xpbd_tile_compact`` (include/xpbd.h)."""
 
-    _fields_ = [(k, c_void_p) for k in ("dist_end", "gen_end", "ci", "cc", "halo2", "pf",
-                                        "pflam")] + [("general", c_int)]
+    _fields_ = ([(k, c_void_p) for k in ("dist_end", "gen_end", "ci", "cc", "halo2", "pf",
+                                         "pflam")] + [("general", c_int), ("acc_tiles", c_void_p)])
 
 
 class RigidDesc(ctypes.Structure):

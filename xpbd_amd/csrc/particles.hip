@@ -598,7 +598,7 @@ __global__ __launch_bounds__(512) void k_jacobi_tile_cx(
     const int2* __restrict__ CI, const float* __restrict__ CC, const int2* __restrict__ H2,
     const float2* __restrict__ PFT, float* __restrict__ PFLAM, float h, int P, int n,
     float relax, float damping, float max_speed, int do_cap, float4* __restrict__ ACC, float gx,
-    float gy, float gz, const float* __restrict__ gscale) {
+    float gy, float gz, const float* __restrict__ gscale, const int* __restrict__ acc_tiles) {
   constexpr int BLOCK = 512;
   constexpr int U = 2;
   constexpr bool PREDICT = MODE & TM_PREDICT, FINALIZE = MODE & TM_FINALIZE;
@@ -619,6 +619,8 @@ __global__ __launch_bounds__(512) void k_jacobi_tile_cx(
   const int p1 = min(p0 + P, n);
   const int tid = threadIdx.x;
   const float inv_dt2 = 1.0f / (h * h);
+  // tiles without a particle that device rows can reach skip the ACC read
+  const bool tile_acc = HAS_ACC && (!acc_tiles || acc_tiles[tile] != 0);
   const int rb = tile_row_start[tile];
   const int rd = dist_end[tile];
   const int hb = halo_start[tile], he = halo_start[tile + 1];
@@ -760,7 +762,7 @@ __global__ __launch_bounds__(512) void k_jacobi_tile_cx(
     float4 a = make_float4(0.f, 0.f, 0.f, (float)adds);
     if (c) a = make_float4(unfix(s0, adds), unfix(s1, adds), unfix(s2, adds), a.w);
     if (c >= kFixPoison) a.x = __int_as_float(0x7fc00000);
-    if (HAS_ACC) {  // corrections of the device-counted row segment
+    if (tile_acc) {  // corrections of the device-counted row segment
       const float4 g = ACC[i];
       if (g.w != 0.0f) {
         a = make_float4(a.x + g.x, a.y + g.y, a.z + g.z, a.w + g.w);
@@ -1788,7 +1790,7 @@ void launch_cx_mode(hipStream_t s, int nb, size_t shmem, const float* Xin, float
                      (const float4*)ANC, LAMin, LAMout, trs, hs, cx->dist_end, cx->gen_end,
                      (const int2*)cx->ci, cx->cc, (const int2*)cx->halo2,
                      (const float2*)cx->pf, cx->pflam, h, P, n, relax, damping, max_speed,
-                     do_cap, (float4*)ACC, gx, gy, gz, gscale);
+                     do_cap, (float4*)ACC, gx, gy, gz, gscale, cx->acc_tiles);
 }
 
 template <bool GEN, bool PF>

@@ -1210,6 +1210,17 @@ class Solver:
                 self._plan_pf = True
             cs.general = int(self._cx["n_general"] > 0
                              or (not self._plan_pf and self._cx["n_floor"] > 0))
+            if (getattr(self, "_device_contacts", False)
+                    and getattr(self, "CLIST", None) is not None):
+                import torch
+
+                # only tiles holding a colliding particle read the device-row sums
+                n_t = (n_b + self._tile - 1) // self._tile
+                at = torch.zeros(max(n_t, 1), dtype=torch.int32, device=self.X.device)
+                if self._nclist:
+                    at[self.CLIST[:self._nclist].long() // self._tile] = 1
+                self._acc_tiles = at
+                cs.acc_tiles = N.ptr(at)
             self._cx_struct = cs
             d.compact = N.ctypes.addressof(cs)
         self._block_active = self._use_block()

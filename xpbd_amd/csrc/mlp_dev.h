@@ -208,24 +208,25 @@ __device__ inline float forward16_t(const RegWeights& w, const float* b1, const 
   const int l = threadIdx.x & 63, g = l >> 4;
   bf16x4 h[4];
   for (int n = 0; n < 4; ++n) {  // Z1^T tile n: lane holds Z1[r][16n + 4g + j]
-    const f32x4 z = mfma(w.a1[n], xb, f32x4{0.f, 0.f, 0.f, 0.f});
+    // the bias enters as the accumulator's start value
     const float4 b = *(const float4*)&b1[16 * n + 4 * g];
-    h[n][0] = (short)to_bf16(gelu(z[0] + b.x));
-    h[n][1] = (short)to_bf16(gelu(z[1] + b.y));
-    h[n][2] = (short)to_bf16(gelu(z[2] + b.z));
-    h[n][3] = (short)to_bf16(gelu(z[3] + b.w));
+    const f32x4 z = mfma(w.a1[n], xb, f32x4{b.x, b.y, b.z, b.w});
+    h[n][0] = (short)to_bf16(gelu(z[0]));
+    h[n][1] = (short)to_bf16(gelu(z[1]));
+    h[n][2] = (short)to_bf16(gelu(z[2]));
+    h[n][3] = (short)to_bf16(gelu(z[3]));
   }
   float p = 0.0f;
   const bf16x8 hb[2] = {cat8(h[0], h[1]), cat8(h[2], h[3])};  // B slices of H1^T
-  for (int n = 0; n < 4; ++n) {  // Z2^T tile n = sum_m W2^T(n, m) H1^T(m)
-    f32x4 acc{0.f, 0.f, 0.f, 0.f};
-    for (int m = 0; m < 2; ++m) acc = mfma32(w.a2[n][m], hb[m], acc);
+  for (int n = 0; n < 4; ++n) {  // Z2^T tile n = b2 + sum_m W2^T(n, m) H1^T(m)
     const float4 b = *(const float4*)&b2[16 * n + 4 * g];
+    f32x4 acc{b.x, b.y, b.z, b.w};
+    for (int m = 0; m < 2; ++m) acc = mfma32(w.a2[n][m], hb[m], acc);
     const float4 c = *(const float4*)&w3[16 * n + 4 * g];
-    p = fmaf(gelu(acc[0] + b.x), c.x, p);
-    p = fmaf(gelu(acc[1] + b.y), c.y, p);
-    p = fmaf(gelu(acc[2] + b.z), c.z, p);
-    p = fmaf(gelu(acc[3] + b.w), c.w, p);
+    p = fmaf(gelu(acc[0]), c.x, p);
+    p = fmaf(gelu(acc[1]), c.y, p);
+    p = fmaf(gelu(acc[2]), c.z, p);
+    p = fmaf(gelu(acc[3]), c.w, p);
   }
   p += __shfl_xor(p, 16, 64);
   p += __shfl_xor(p, 32, 64);

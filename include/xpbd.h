@@ -414,6 +414,20 @@ int xpbd_jacobi_tile_pair(void* stream, const float* Xin, float* Xout, const flo
                           float max_speed, float gx, float gy, float gz, const float* gscale,
                           int lmax, int amax, int k1max, int last, int predict);
 size_t xpbd_jacobi_tile_pair_lds_bytes(int lmax, int amax, int k1max);
+/* The learned-start form of xpbd_jacobi_tile_pair (no reference counterpart;
+ * it replaces the zero start of solver.py:431): a one-iteration substep
+ * with the learned start, i.e. xpbd_jacobi_tile_warm (Y, ws_on, invert as
+ * there) followed by one xpbd_jacobi_tile / xpbd_jacobi_tile_next launch,
+ * bit for bit, with the same tables; Y NULL gives xpbd_jacobi_tile_pair. */
+int xpbd_jacobi_tile_pair_warm(void* stream, const float* Xin, float* Xout, const float* XPin,
+                               float* XPout, float* V, const int* R, const float* ANC,
+                               float* LAMout, const int* ring_start, const int* ring1_n,
+                               const int* ring_idx, const int* p1_start, const int* p1,
+                               const int* p2_start, const int* p2, float h, int tile_size, int n,
+                               float relax, float damping, float max_speed, float gx, float gy,
+                               float gz, const float* gscale, int lmax, int amax, int k1max,
+                               int last, int predict, const float* Y, const int* ws_on,
+                               int invert);
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);
 /* Runs `frames` frames on `stream`; use_graph != 0 captures the frame once

@@ -468,3 +468,36 @@ def test_warm_start_with_device_contact_rows_matches_eager(trained, fuse):
     piles = [sc for sc in spec.scenes if sc.kind == "pile"]
     for sc in piles:
         assert xa[sc.first:sc.first + sc.count, 1].min() > 0.05 - 0.01
+
+
+@pytest.mark.parametrize("off", [False, True])
+def test_one_launch_learned_substep_equals_start_launch_then_iteration(trained, off):
+    """One-iteration substeps with the learned start on a small cloth: the
+    plan's single launch per substep (xpbd_jacobi_tile_pair_warm: the start
+    on the tile and its ring, then the iteration) gives, bit for bit, the
+    states and multipliers of the plan with a start launch and an iteration
+    launch per substep; the judge runs every 2nd frame, the cloth rests on
+    the floor (friction active).  off: the start flag held at 'zero start'."""
+    theta, _ = trained
+    a, b = _cloth(), _cloth()
+    for s in (a, b):
+        s.step(30)
+        s.iterations = 1
+        s.warm_start_every = s.substeps
+        s.warm_start_judge = False if off else 2
+        s.set_warm_start(theta)
+    a.tile_pair = "auto"
+    b.tile_pair = False
+    for s in (a, b):
+        s._flush()
+        s._get_plan()
+        if off:
+            s.WSON.zero_()
+    assert a._plan_pair and not b._plan_pair
+    for f in range(6):
+        a.step()
+        b.step()
+        assert np.array_equal(a.positions(), b.positions()), f
+        assert np.array_equal(a.velocities(), b.velocities()), f
+        assert np.array_equal(a.lambdas(), b.lambdas()), f
+    assert a.warm_start_stats() == b.warm_start_stats()

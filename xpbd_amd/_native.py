@@ -151,6 +151,8 @@ _SIGS = {
     "xpbd_jacobi_tile_pair": (_I, [_P] * 16 + [_F, _I, _I, _F, _F, _F, _F, _F, _F, _P, _I, _I, _I, _I,
                                               _I]),
     "xpbd_jacobi_tile_pair_lds_bytes": (ctypes.c_size_t, [_I, _I, _I]),
+    "xpbd_jacobi_tile_pair_warm": (_I, [_P] * 16 + [_F, _I, _I, _F, _F, _F, _F, _F, _F, _P, _I,
+                                                   _I, _I, _I, _I, _P, _P, _I]),
     "xpbd6_substep_contacts": (_I, [_P, _P, _I]),
     "xpbd6_substep_velocity": (_I, [_P, _P, _I]),
     "xpbd6_substep_end": (_I, [_P, _P]),

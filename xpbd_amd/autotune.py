@@ -94,7 +94,7 @@ def tune(s, frames: int = 3, warmup: int = 1, verbose: bool = False):
     tile_only = (s._ws_theta is not None or s._gscale is not None
                  or getattr(s, "_device_contacts", False))
     judged = s._ws_theta is not None and int(s.warm_start_judge) > 0 and s.substeps >= 2
-    pair = s.iterations == 2 and s._ws_theta is None
+    pair = (s.iterations == 2) == (s._ws_theta is None) and s.iterations in (1, 2)
     for cand in candidates(s.num_bodies, tile_only=tile_only, fused_only=judged, pair=pair):
         _apply(s, cand)
         s.set_state(x0, v0)

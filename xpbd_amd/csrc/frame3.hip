@@ -141,6 +141,23 @@ int issue_fused_substep(const xpbd_particles_desc& d, hipStream_t s, TileRun& r,
                        d.gz, d.gscale, d.n_rows, predicted ? 0 : 1, d.ws_on, invert);
       if (rc) return rc;
     }
+    const bool last = !integrate_next;
+    if (it_n == 1 && d.pair_p1 && r.xp_alt && !dyn && next && !(last && !predicted)) {
+      // the learned start and the substep's iteration in one launch
+      float* xp_out = last ? d.x_prev : r.xp_alt;
+      rc = xpbd_jacobi_tile_pair_warm(
+          s, in(), bufs[r.loc ^ 1], r.xp, xp_out, d.v, d.rows, d.anchors, lams[0],
+          d.pair_ring_start, d.pair_ring1_n, d.pair_ring_idx, d.pair_p1_start, d.pair_p1,
+          d.pair_p2_start, d.pair_p2, d.h, d.tile_size, n, d.relax, d.damping, d.max_speed, d.gx,
+          d.gy, d.gz, d.gscale, d.pair_lmax, d.pair_amax, d.pair_k1max, last ? 1 : 0,
+          predicted ? 0 : 1, r.y, d.ws_on, invert);
+      if (rc) return rc;
+      r.src0 = nullptr;
+      r.loc ^= 1;
+      r.li = 0;
+      if (!last) std::swap(r.xp, r.xp_alt);
+      return 0;
+    }
     if (d.compact)
       rc = xpbd_jacobi_tile_warm_cx(s, in(), bufs[r.loc ^ 1], r.xp, d.v, d.rows, d.anchors,
                                     lams[1], d.tile_row_start, d.halo_start, d.compact, r.y, d.h,
@@ -276,7 +293,7 @@ int issue_frame_tile(Frame3* f, hipStream_t s, bool judge) {
   const bool next = d.fuse_first >= 2 && it_n >= 1;
   if (d.fuse_first && d.n_rows > 0 && (it_n >= 2 || (warm && it_n >= 1) || next)) {
     TileRun r{{d.x, d.x_alt}, {d.lam, d.lam_alt}, d.x_prev,
-              warm ? nullptr : d.x_prev_alt, d.ws_y, dyn ? d.acc : nullptr, d.dyn_lam, nullptr};
+              d.x_prev_alt, d.ws_y, dyn ? d.acc : nullptr, d.dyn_lam, nullptr};
     judge = judge && warm && d.ws_judge > 0;
     for (int sub = 0; sub < d.substeps; ++sub) {
       if (judge && sub == 0) {

@@ -804,12 +804,66 @@ __global__ __launch_bounds__(512) void k_jacobi_tile_cx(
   }
 }
 
+// The learned start of one row (no reference counterpart): lambda0 from
+// the network output yv and the row's first plain step -C / (w + alpha~)
+// (one-sided rows only while violated), and the direction its endpoints
+// move along: a by nrm * wa * lam0, b (two) by -nrm * wb * lam0.
+struct WarmRow {
+  float lam0, wa, wb;
+  f3 nrm;
+  int a, b;
+  bool two;
+};
+template <class Pos, class Anchor>
+__device__ inline WarmRow warm_row(Pos pos, Anchor anchor, int4 row, float yv, float inv_dt2) {
+  WarmRow r;
+  const int t = row_type(row);
+  r.a = row_a(row);
+  r.b = row.y;
+  const float p0r = __int_as_float(row.z);
+  const float4 xa4 = pos(r.a);
+  r.wa = xa4.w;
+  r.wb = 0.0f;
+  float C, w, alpha = 0.0f;
+  r.nrm = mk3(0.0f, 1.0f, 0.0f);
+  bool ok = true;
+  if (t == T_FLOOR) {
+    C = xa4.y - p0r;
+    w = r.wa;
+  } else {
+    f3 pb;
+    if (t == T_ATTACH) {
+      pb = xyz(anchor());
+    } else {
+      const float4 xb4 = pos(r.b);
+      pb = xyz(xb4);
+      r.wb = xb4.w;
+    }
+    const f3 d = xyz(xa4) - pb;
+    const float L = length(d);
+    ok = L >= XPBD_EPS;
+    if (ok) r.nrm = d * __builtin_amdgcn_rcpf(L);
+    C = L - p0r;
+    w = r.wa + r.wb;
+    if (t != T_CONTACT) alpha = __int_as_float(row.w) * inv_dt2;
+  }
+  const float sden = w + alpha;
+  float step = (sden > 0.0f && ok) ? -C * __builtin_amdgcn_rcpf(sden) : 0.0f;
+  if (t == T_FLOOR || t == T_CONTACT) step = C < 0.0f ? step : 0.0f;
+  r.lam0 = xpbd_ws::start_lambda(yv, step);
+  r.two = t == T_DISTANCE || t == T_CONTACT;
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // A whole two-iteration substep in one launch (small scenes, where every
 // launch costs its latency rather than its bytes): the first iteration
 // (xpbd_jacobi_tile with first = 1) followed by the second with finalize
 // and, unless LAST, the next substep's integrate (xpbd_jacobi_tile_next), or
 // finalize + cap (xpbd_jacobi_tile with finalize) when LAST.
+// WARM: the first iteration is replaced by the learned start of
+// xpbd_jacobi_tile_warm (network output Y per row, start flag ws_on), the
+// second is the substep's one plain iteration from its multipliers.
 // PREDICT: the substep's integrate too (the frame's first substep,
 // xpbd_jacobi_tile_first): every local particle is predicted from X and V
 // at staging, its x_prev is X (XPin is not read); not with LAST, whose
@@ -856,7 +910,7 @@ __host__ __device__ inline size_t tile_pair_lds_bytes(int lmax, int amax, int k1
          (size_t)lmax * sizeof(int) + (size_t)k1max * (sizeof(int) + sizeof(float));
 }
 
-template <int BLOCK, bool LAST, bool PREDICT>
+template <int BLOCK, bool LAST, bool PREDICT, bool WARM>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
     const float4* __restrict__ Xin, float4* __restrict__ Xout, const float4* XPin,
     float4* XPout, float4* __restrict__ V, const int4* __restrict__ R,
@@ -865,7 +919,8 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
     const int* __restrict__ ring_idx, const int* __restrict__ p1_start,
     const int4* __restrict__ p1, const int* __restrict__ p2_start, const int* __restrict__ p2,
     float h, int P, int n, float relax, float damping, float max_speed, int do_cap, float gx,
-    float gy, float gz, const float* __restrict__ gscale, int lmax, int amax, int k1max) {
+    float gy, float gz, const float* __restrict__ gscale, int lmax, int amax, int k1max,
+    const float* __restrict__ Y, const int* __restrict__ ws_on, int invert) {
   constexpr int U1 = 3;  // phase-1 rows per thread loaded ahead (3 x BLOCK per tile)
   constexpr int U2 = 2;  // phase-2 entries per thread loaded ahead
   extern __shared__ unsigned long long lds64[];
@@ -900,12 +955,19 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
   };
   int4 prow[U1];
   int pj[U1];
+  float py[U1];  // WARM: the rows' network outputs
 #pragma unroll
   for (int u = 0; u < U1; ++u) {
     const int k = threadIdx.x + u * BLOCK;
     pj[u] = -1;
-    if (k < n1) prow[u] = local_row(k, pj[u]);
+    py[u] = 0.0f;
+    if (k < n1) {
+      prow[u] = local_row(k, pj[u]);
+      if (WARM) py[u] = Y[pj[u]];
+    }
   }
+  // WARM with the start switched off: lambda0 = 0 and no move (the zero start)
+  const bool active = !WARM || xpbd_ws::start_active(ws_on, invert);
   int pe[U2];
 #pragma unroll
   for (int u = 0; u < U2; ++u) {
@@ -952,10 +1014,26 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
     atomicAdd(&sz[l], fix_bits(fminf(fmaxf(c.z, -kFixLimit), kFixLimit)));
     atomicAdd(&cnt[l], ok ? 1u : (kFixPoison | 1u));
   };
-  // phase 1: the first iteration of the tile and ring 1 (lambda from zero)
-  auto phase1 = [&](int k, int4 lrow, int j) {
+  // phase 1: the first iteration of the tile and ring 1 (lambda from zero),
+  // or (WARM) the learned start of k_jacobi_tile_warm on the same rows
+  auto phase1 = [&](int k, int4 lrow, int j, float yv) {
     srow[k] = lrow;
     sj[k] = j;
+    if (WARM) {
+      float lam0 = 0.0f;
+      if (active) {
+        const WarmRow wr = warm_row(
+            pos, [&]() { return ANC ? ANC[j] : make_float4(0.f, 0.f, 0.f, 0.f); }, lrow, yv,
+            inv_dt2);
+        lam0 = wr.lam0;
+        if (lam0 != 0.0f) {
+          add(wr.a, wr.nrm * (wr.wa * lam0), nacc);
+          if (wr.two) add(wr.b, wr.nrm * (-wr.wb * lam0), nacc);
+        }
+      }
+      slam[k] = lam0;
+      return;
+    }
     const Proj p = project_row<true>(
         pos, prev, [&]() { return ANC ? ANC[j] : make_float4(0.f, 0.f, 0.f, 0.f); }, lrow,
         0.0f, inv_dt2);
@@ -965,11 +1043,11 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
   };
 #pragma unroll
   for (int u = 0; u < U1; ++u)
-    if (pj[u] >= 0) phase1(threadIdx.x + u * BLOCK, prow[u], pj[u]);
+    if (pj[u] >= 0) phase1(threadIdx.x + u * BLOCK, prow[u], pj[u], py[u]);
   for (int k = threadIdx.x + U1 * BLOCK; k < n1; k += BLOCK) {
     int j;
     const int4 lrow = local_row(k, j);
-    phase1(k, lrow, j);
+    phase1(k, lrow, j, WARM ? Y[j] : 0.0f);
   }
   __syncthreads();
   // the first iteration's apply (k_jacobi_tile without finalize), in LDS
@@ -1044,57 +1122,6 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
     }
     Xout[i] = make_float4(x.x, x.y, x.z, x4.w);
   }
-}
-
-// The learned start of one row (no reference counterpart): lambda0 from
-// the network output yv and the row's first plain step -C / (w + alpha~)
-// (one-sided rows only while violated), and the direction its endpoints
-// move along: a by nrm * wa * lam0, b (two) by -nrm * wb * lam0.
-struct WarmRow {
-  float lam0, wa, wb;
-  f3 nrm;
-  int a, b;
-  bool two;
-};
-template <class Pos, class Anchor>
-__device__ inline WarmRow warm_row(Pos pos, Anchor anchor, int4 row, float yv, float inv_dt2) {
-  WarmRow r;
-  const int t = row_type(row);
-  r.a = row_a(row);
-  r.b = row.y;
-  const float p0r = __int_as_float(row.z);
-  const float4 xa4 = pos(r.a);
-  r.wa = xa4.w;
-  r.wb = 0.0f;
-  float C, w, alpha = 0.0f;
-  r.nrm = mk3(0.0f, 1.0f, 0.0f);
-  bool ok = true;
-  if (t == T_FLOOR) {
-    C = xa4.y - p0r;
-    w = r.wa;
-  } else {
-    f3 pb;
-    if (t == T_ATTACH) {
-      pb = xyz(anchor());
-    } else {
-      const float4 xb4 = pos(r.b);
-      pb = xyz(xb4);
-      r.wb = xb4.w;
-    }
-    const f3 d = xyz(xa4) - pb;
-    const float L = length(d);
-    ok = L >= XPBD_EPS;
-    if (ok) r.nrm = d * __builtin_amdgcn_rcpf(L);
-    C = L - p0r;
-    w = r.wa + r.wb;
-    if (t != T_CONTACT) alpha = __int_as_float(row.w) * inv_dt2;
-  }
-  const float sden = w + alpha;
-  float step = (sden > 0.0f && ok) ? -C * __builtin_amdgcn_rcpf(sden) : 0.0f;
-  if (t == T_FLOOR || t == T_CONTACT) step = C < 0.0f ? step : 0.0f;
-  r.lam0 = xpbd_ws::start_lambda(yv, step);
-  r.two = t == T_DISTANCE || t == T_CONTACT;
-  return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -1884,6 +1911,21 @@ int xpbd_jacobi_tile_pair(void* stream, const float* Xin, float* Xout, const flo
                           float h, int tile_size, int n, float relax, float damping,
                           float max_speed, float gx, float gy, float gz, const float* gscale,
                           int lmax, int amax, int k1max, int last, int predict) {
+  return xpbd_jacobi_tile_pair_warm(stream, Xin, Xout, XPin, XPout, V, R, ANC, LAMout,
+                                    ring_start, ring1_n, ring_idx, p1_start, p1, p2_start, p2, h,
+                                    tile_size, n, relax, damping, max_speed, gx, gy, gz, gscale,
+                                    lmax, amax, k1max, last, predict, nullptr, nullptr, 0);
+}
+
+int xpbd_jacobi_tile_pair_warm(void* stream, const float* Xin, float* Xout, const float* XPin,
+                               float* XPout, float* V, const int* R, const float* ANC,
+                               float* LAMout, const int* ring_start, const int* ring1_n,
+                               const int* ring_idx, const int* p1_start, const int* p1,
+                               const int* p2_start, const int* p2, float h, int tile_size, int n,
+                               float relax, float damping, float max_speed, float gx, float gy,
+                               float gz, const float* gscale, int lmax, int amax, int k1max,
+                               int last, int predict, const float* Y, const int* ws_on,
+                               int invert) {
   if (n <= 0) return 0;
   if (tile_size < 1 || tile_size > XPBD_MAX_TILE || amax < tile_size || lmax < amax ||
       k1max < 0 || !Xin || !Xout || !XPin || !XPout || !R || !LAMout || !ring_start ||
@@ -1895,19 +1937,28 @@ int xpbd_jacobi_tile_pair(void* stream, const float* Xin, float* Xout, const flo
   const int do_cap = (max_speed > 0.0f && std::isfinite(max_speed)) ? 1 : 0;
   const int nb = blocks_for(n, tile_size);
   hipStream_t s = (hipStream_t)stream;
-#define XPBD_PAIR_LAUNCH(L, PR)                                                               \
-  hipLaunchKernelGGL((k_jacobi_tile_pair<kPairBlock, L, PR>), dim3(nb), dim3(kPairBlock),      \
+#define XPBD_PAIR_LAUNCH(L, PR, W)                                                            \
+  hipLaunchKernelGGL((k_jacobi_tile_pair<kPairBlock, L, PR, W>), dim3(nb), dim3(kPairBlock),   \
                      shmem, s, (const float4*)Xin, (float4*)Xout, (const float4*)XPin,         \
                      (float4*)XPout, (float4*)V, (const int4*)R, (const float4*)ANC, LAMout,   \
                      ring_start, ring1_n, ring_idx, p1_start, (const int4*)p1, p2_start, p2, h, \
                      tile_size, n, relax, damping, max_speed, do_cap, gx, gy, gz, gscale, lmax, \
-                     amax, k1max)
-  if (last)
-    XPBD_PAIR_LAUNCH(true, false);
-  else if (predict)
-    XPBD_PAIR_LAUNCH(false, true);
-  else
-    XPBD_PAIR_LAUNCH(false, false);
+                     amax, k1max, Y, ws_on, invert)
+  if (Y) {
+    if (last)
+      XPBD_PAIR_LAUNCH(true, false, true);
+    else if (predict)
+      XPBD_PAIR_LAUNCH(false, true, true);
+    else
+      XPBD_PAIR_LAUNCH(false, false, true);
+  } else {
+    if (last)
+      XPBD_PAIR_LAUNCH(true, false, false);
+    else if (predict)
+      XPBD_PAIR_LAUNCH(false, true, false);
+    else
+      XPBD_PAIR_LAUNCH(false, false, false);
+  }
 #undef XPBD_PAIR_LAUNCH
   XPBD_LAUNCH_CHECK();
   return 0;

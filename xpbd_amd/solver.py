@@ -388,6 +388,8 @@ def tile_pair_tables(rows_sorted: np.ndarray, n_particles: int, tile: int):
         amax = max(amax, tile + len(r1))
         k1max = max(k1max, len(rows_1))
     cat = lambda xs, dt: (np.concatenate(xs) if xs else np.zeros(0, dt))  # noqa: E731
+    amax += amax & 1          # the 8-byte sums of amax particles precede 16-byte records
+    lmax = max(lmax, amax)
     return {
         "ring_start": np.asarray(ring_start, np.int32),
         "ring1_n": np.asarray(ring1_n, np.int32),
@@ -1076,9 +1078,12 @@ class Solver:
         the descriptor, when the plan can use it; None otherwise."""
         import torch
 
-        if self.tile_pair is False or d.fuse_first != 2 or self.iterations != 2:
+        # two plain iterations, or the learned start and one iteration
+        two = self.iterations == 2 and self._ws_theta is None
+        warm_one = self.iterations == 1 and self._ws_theta is not None
+        if self.tile_pair is False or d.fuse_first != 2 or not (two or warm_one):
             return None
-        if self.CROWS is not None or self._ws_theta is not None or n_c == 0:
+        if self.CROWS is not None or n_c == 0:
             return None
         if self.tile_pair == "auto" and n_c > self.PAIR_AUTO_ROWS:
             return None

@@ -249,6 +249,7 @@ int xpbd_frame_block(void* stream, float* X, float* V, float* XP, const int* R, 
  * zero start of solver.py:431), launched after xpbd_ws_net and before the
  * substep's Jacobi iterations, which then run with first = 0 from LAMout.
  * Every row starts from lambda0 = 4 sigmoid(Y[row]) (-C / (w + alpha~))
+ * (Y: the network pass's bfloat16 outputs)
  * (one-sided rows only while violated), written to LAMout; each particle
  * moves by relax x the mean over its rows of w_i grad C lambda0 (Xin ->
  * Xout, Xin != Xout).  predict = 1 folds integrate in as
@@ -261,7 +262,7 @@ int xpbd_frame_block(void* stream, float* X, float* V, float* XP, const int* R, 
 int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP, const float* V,
                           const int* R, const float* ANC, float* LAMout,
                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
-                          const float* Y, float h, float gx, float gy, float gz,
+                          const unsigned short* Y, float h, float gx, float gy, float gz,
                           const float* gscale, int tile_size, int n_particles, float relax,
                           int predict, const int* ws_on, int invert);
 /* xpbd_jacobi_tile_warm over the compact tables (cx replaces halo_rows):
@@ -271,7 +272,7 @@ int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP
 int xpbd_jacobi_tile_warm_cx(void* stream, const float* Xin, float* Xout, float* XP,
                              const float* V, const int* R, const float* ANC, float* LAMout,
                              const int* tile_row_start, const int* halo_start,
-                             const xpbd_tile_compact* cx, const float* Y, float h, float gx,
+                             const xpbd_tile_compact* cx, const unsigned short* Y, float h, float gx,
                              float gy, float gz, const float* gscale, int tile_size,
                              int n_particles, float relax, int predict, const int* ws_on,
                              int invert);
@@ -326,7 +327,7 @@ typedef struct xpbd_particles_desc {
    * then starts with xpbd_ws_net (outputs into ws_y) and
    * xpbd_jacobi_tile_warm */
   const void* ws_image;
-  float* ws_y;           /* float[n_rows] */
+  unsigned short* ws_y;  /* bfloat16 bits [n_rows] */
   /* Tile mode: 1 folds integrate into the first iteration of every substep
    * (xpbd_jacobi_tile_first; >= 2 iterations or the learned warm start), 2
    * folds it into the first substep's first iteration and into the previous
@@ -350,7 +351,7 @@ typedef struct xpbd_particles_desc {
    * ws_judge = k >= 1, the first substep of every k-th frame (counted by
    * the plan from its first run) is first run with the other start from the
    * same state into the judge buffers (ws_jx, ws_jx_alt, ws_jxp float4[n];
-   * ws_jlam, ws_jlam_alt, ws_jy float[n_rows]; ws_jacc float4[n] and
+   * ws_jlam, ws_jlam_alt float[n_rows], ws_jy bfloat16 bits[n_rows]; ws_jacc float4[n] and
    * ws_jdlam float[dyn_cap] with device rows), then with the solver's own;
    * both squared residuals go to ws_rsum (double[2], zero at the start) and
    * xpbd_ws_decide sets ws_on for the rest of the frame and the following
@@ -363,7 +364,7 @@ typedef struct xpbd_particles_desc {
   float* ws_jxp;
   float* ws_jlam;
   float* ws_jlam_alt;
-  float* ws_jy;
+  unsigned short* ws_jy;
   float* ws_jacc;
   float* ws_jdlam;
   double* ws_rsum;
@@ -426,7 +427,7 @@ int xpbd_jacobi_tile_pair_warm(void* stream, const float* Xin, float* Xout, cons
                                const int* p2_start, const int* p2, float h, int tile_size, int n,
                                float relax, float damping, float max_speed, float gx, float gy,
                                float gz, const float* gscale, int lmax, int amax, int k1max,
-                               int last, int predict, const float* Y, const int* ws_on,
+                               int last, int predict, const unsigned short* Y, const int* ws_on,
                                int invert);
 
 int xpbd_frame3_create(const xpbd_particles_desc* desc, void** handle);
@@ -694,11 +695,12 @@ int xpbd_ws_features(void* stream, const float* X, const float* XP, const int* R
 /* The network pass of a warm-started substep: per row the features (as
  * xpbd_ws_features; at the positions integrate would predict from Xin and V
  * when predict = 1, x_prev = Xin; otherwise at Xin with x_prev XP) and the
- * network's output Y[row] (as xpbd_mlp_forward on those features); image =
+ * network's output Y[row] (as xpbd_mlp_forward on those features, stored
+ * as bfloat16 bits, round to nearest even); image =
  * the network packed by xpbd_mlp_pack.  ws_on / invert as in
  * xpbd_jacobi_tile_warm: with the start off the launch does nothing. */
 int xpbd_ws_net(void* stream, const float* Xin, const float* XP, const float* V, const int* R,
-                const float* ANC, const void* image, float* Y, float h, float gx, float gy,
+                const float* ANC, const void* image, unsigned short* Y, float h, float gx, float gy,
                 float gz, const float* gscale, int n_rows, int predict, const int* ws_on,
                 int invert);
 int xpbd_ws_residual(void* stream, const float* X, const int* R, const float* ANC,

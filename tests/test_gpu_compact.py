@@ -276,7 +276,7 @@ def test_warm_cx_launch_equals_int4_launch(predict, general, pf, on):
     cs.general = int(general or not pf)
     X0 = s.X.clone()
     X0[::3, 1] = -0.002
-    Y = torch.randn(m, device=DEV)
+    Y = torch.randn(m, device=DEV).to(torch.bfloat16)
     ws_on = torch.full((1,), on, dtype=torch.int32, device=DEV)
     s._cx_dev["pflam"].fill_(123.0)
     outs = []
@@ -314,6 +314,7 @@ def test_solver_warm_start_compact_plan_equals_int4_plan():
     def build():
         s, _ = cloth_sheet(Solver, res=48, solve_mode="jacobi", device=DEV, iterations=1)
         s.tile_size = 64
+        s.tile_pair = False    # the start launch and the iteration as separate launches
         s.warm_start_judge = 2
         s.set_warm_start(theta)
         return s

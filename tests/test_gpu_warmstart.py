@@ -37,7 +37,7 @@ def _warm(s, src, dst, lam, theta, Y=None, predict=0):
     h = float(np.float32(s.dt / s.substeps))
     trs, hs, hr = (N.ptr(t) for t in s._tile_dev)
     if Y is None:
-        Y = torch.empty(s.num_constraints, device=DEV)
+        Y = torch.empty(s.num_constraints, dtype=torch.bfloat16, device=DEV)
     img = torch.empty(int(N.lib().xpbd_mlp_image_bytes()), dtype=torch.uint8, device=DEV)
     N.check(N.lib().xpbd_mlp_pack(None, N.ptr(theta), N.ptr(img)))
     N.check(N.lib().xpbd_ws_net(None, N.ptr(src), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R),
@@ -89,13 +89,13 @@ def test_warm_launch_matches_oracle():
     theta = _theta(7, b3=-1.0)
     Xout = torch.empty_like(s.X)
     lam = torch.zeros(m, device=DEV)
-    Y = torch.zeros(m, device=DEV)
+    Y = torch.zeros(m, dtype=torch.bfloat16, device=DEV)
     _warm(s, s.X, Xout, lam, theta, Y=Y)
     F = torch.empty((m, 12), device=DEV)
     N.check(N.lib().xpbd_ws_features(None, N.ptr(s.X), N.ptr(s.XP), N.ptr(s.R), N.ptr(s.ANC), h,
                                      N.ptr(F), m))
-    Yf = ws.predict(theta, F)
-    y_k, y_f = Y.cpu().numpy(), Yf.cpu().numpy()
+    Yf = ws.predict(theta, F).to(torch.bfloat16)     # the pass keeps bfloat16 outputs
+    y_k, y_f = Y.float().cpu().numpy(), Yf.float().cpu().numpy()
     assert np.array_equal(y_k, y_f), np.abs(y_k - y_f).max()
     omega = 4.0 / (1.0 + np.exp(-y_k))
     print(f"omega range {omega.min():.2f} .. {omega.max():.2f}")
@@ -138,7 +138,7 @@ def test_network_pass_in_64_row_groups_equals_features_then_forward(predict):
     theta = _theta(11, b3=0.5)
     img = torch.empty(int(N.lib().xpbd_mlp_image_bytes()), dtype=torch.uint8, device=DEV)
     N.check(N.lib().xpbd_mlp_pack(None, N.ptr(theta), N.ptr(img)))
-    Y = torch.full((m + 7,), 7.0, device=DEV)
+    Y = torch.full((m + 7,), 7.0, dtype=torch.bfloat16, device=DEV)
     N.check(N.lib().xpbd_ws_net(None, N.ptr(s.X), N.ptr(s.XP), N.ptr(s.V), N.ptr(s.R),
                                 N.ptr(s.ANC), N.ptr(img), N.ptr(Y), h, *s.gravity, None, m,
                                 predict, None, 0))
@@ -150,7 +150,7 @@ def test_network_pass_in_64_row_groups_equals_features_then_forward(predict):
     F = torch.empty((m, 12), device=DEV)
     N.check(N.lib().xpbd_ws_features(None, N.ptr(X), N.ptr(XP), N.ptr(s.R), N.ptr(s.ANC), h,
                                      N.ptr(F), m))
-    Yf = ws.predict(theta, F)
+    Yf = ws.predict(theta, F).to(torch.bfloat16)     # the pass keeps bfloat16 outputs
     assert torch.equal(Y[:m], Yf)
     assert bool((Y[m:] == 7.0).all())            # nothing written past the rows
 
@@ -221,7 +221,7 @@ def test_warm_launch_folded_integrate_equals_integrate_then_warm():
                                               N.ptr(s.XP), *s.gravity, h, None, n))
         Xo = torch.empty_like(s.X)
         lam = torch.zeros(m, device=DEV)
-        Y = torch.zeros(m, device=DEV)
+        Y = torch.zeros(m, dtype=torch.bfloat16, device=DEV)
         _warm(s, s.X if not predict else X0.clone(), Xo, lam, theta, Y=Y, predict=predict)
         res.append((Xo, lam, Y, s.XP.clone()))
     for name, a, b in zip(("X", "lambda", "Y", "x_prev"), *res):

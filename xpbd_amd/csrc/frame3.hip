@@ -71,7 +71,7 @@ struct TileRun {
   float* lam[2];
   float* xp;      // x_prev the substep reads (and, without pair launches, writes)
   float* xp_alt;  // the other x_prev buffer of the pair launches
-  float* y;
+  unsigned short* y;
   float* acc;
   float* dlam;
   const float* src0;

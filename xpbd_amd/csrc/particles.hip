@@ -920,7 +920,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
     const int4* __restrict__ p1, const int* __restrict__ p2_start, const int* __restrict__ p2,
     float h, int P, int n, float relax, float damping, float max_speed, int do_cap, float gx,
     float gy, float gz, const float* __restrict__ gscale, int lmax, int amax, int k1max,
-    const float* __restrict__ Y, const int* __restrict__ ws_on, int invert) {
+    const unsigned short* __restrict__ Y, const int* __restrict__ ws_on, int invert) {
   constexpr int U1 = 3;  // phase-1 rows per thread loaded ahead (3 x BLOCK per tile)
   constexpr int U2 = 2;  // phase-2 entries per thread loaded ahead
   extern __shared__ unsigned long long lds64[];
@@ -963,7 +963,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
     py[u] = 0.0f;
     if (k < n1) {
       prow[u] = local_row(k, pj[u]);
-      if (WARM) py[u] = Y[pj[u]];
+      if (WARM) py[u] = xpbd_ws::y_of(Y, pj[u]);
     }
   }
   // WARM with the start switched off: lambda0 = 0 and no move (the zero start)
@@ -1047,7 +1047,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
   for (int k = threadIdx.x + U1 * BLOCK; k < n1; k += BLOCK) {
     int j;
     const int4 lrow = local_row(k, j);
-    phase1(k, lrow, j, WARM ? Y[j] : 0.0f);
+    phase1(k, lrow, j, WARM ? xpbd_ws::y_of(Y, j) : 0.0f);
   }
   __syncthreads();
   // the first iteration's apply (k_jacobi_tile without finalize), in LDS
@@ -1148,7 +1148,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_warm(
     const float4* __restrict__ V, const int4* __restrict__ R, const float4* __restrict__ ANC,
     float* __restrict__ LAMout, const int* __restrict__ tile_row_start,
     const int* __restrict__ halo_start, const int* __restrict__ halo_rows,
-    const float* __restrict__ Y, float h, int P, int n, float relax, float gx, float gy,
+    const unsigned short* __restrict__ Y, float h, int P, int n, float relax, float gx, float gy,
     float gz, const float* __restrict__ gscale, const int* __restrict__ ws_on, int invert) {
   extern __shared__ unsigned long long lds64[];
   // switched off: lambda0 = 0 for every row, positions as predicted (the
@@ -1210,7 +1210,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_warm(
   };
   for (int k = threadIdx.x; k < total; k += BLOCK) {
     const int j = k < owned ? rb + k : halo_rows[hb + (k - owned)];
-    const WarmRow wr = warm_row(pos, [&]() { return ANC[j]; }, R[j], Y[j], inv_dt2);
+    const WarmRow wr = warm_row(pos, [&]() { return ANC[j]; }, R[j], xpbd_ws::y_of(Y, j), inv_dt2);
     if (k < owned) LAMout[j] = wr.lam0;
     if (wr.lam0 == 0.0f) continue;
     add(wr.a, wr.nrm * (wr.wa * wr.lam0));
@@ -1245,7 +1245,7 @@ __global__ __launch_bounds__(512) void k_jacobi_tile_warm_cx(
     float* __restrict__ LAMout, const int* __restrict__ tile_row_start,
     const int* __restrict__ halo_start, const int* __restrict__ dist_end,
     const int* __restrict__ gen_end, const int2* __restrict__ CI, const float* __restrict__ CC,
-    const int2* __restrict__ H2, float* __restrict__ PFLAM, const float* __restrict__ Y,
+    const int2* __restrict__ H2, float* __restrict__ PFLAM, const unsigned short* __restrict__ Y,
     float h, int P, int n, float relax, float gx, float gy, float gz,
     const float* __restrict__ gscale, const int* __restrict__ ws_on, int invert) {
   constexpr int BLOCK = 512;
@@ -1295,7 +1295,7 @@ __global__ __launch_bounds__(512) void k_jacobi_tile_warm_cx(
     const int j = min(rb + tid + u * BLOCK, last);
     q[u] = CI[j];
     cm[u] = CC[j];
-    yv[u] = Y[j];
+    yv[u] = xpbd_ws::y_of(Y, j);
   }
   for (int k = tid; k < 3 * P; k += BLOCK) lds64[k] = 0ull;
   for (int k = tid; k < P; k += BLOCK) cnt[k] = 0u;
@@ -1337,7 +1337,7 @@ __global__ __launch_bounds__(512) void k_jacobi_tile_warm_cx(
       const int j = min(rb + k0 + (U + u) * BLOCK, last);
       nq[u] = CI[j];
       ncm[u] = CC[j];
-      nyv[u] = Y[j];
+      nyv[u] = xpbd_ws::y_of(Y, j);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -1358,14 +1358,14 @@ __global__ __launch_bounds__(512) void k_jacobi_tile_warm_cx(
   const int rg = PF ? gen_end[tile] : re;
   if (GEN) {  // every other owned row from R
     for (int j = rd + tid; j < rg; j += BLOCK) {
-      const WarmRow wr = warm_row(pos, [&]() { return ANC[j]; }, R[j], Y[j], inv_dt2);
+      const WarmRow wr = warm_row(pos, [&]() { return ANC[j]; }, R[j], xpbd_ws::y_of(Y, j), inv_dt2);
       LAMout[j] = wr.lam0;
       move(wr);
     }
   }
   if (PF) {  // the single FLOOR rows: lambda0 per particle
     for (int j = rg + tid; j < re; j += BLOCK) {
-      const WarmRow wr = warm_row(pos, no_anchor, R[j], Y[j], inv_dt2);
+      const WarmRow wr = warm_row(pos, no_anchor, R[j], xpbd_ws::y_of(Y, j), inv_dt2);
       PFLAM[wr.a] = wr.lam0;
       move(wr);
     }
@@ -1374,7 +1374,7 @@ __global__ __launch_bounds__(512) void k_jacobi_tile_warm_cx(
     const int2 e = H2[k];
     const int j = e.x;
     const int4 row = (GEN && e.y < 0) ? R[j] : compact(CI[j], CC[j], e.y);
-    move(warm_row(pos, [&]() { return ANC[j]; }, row, Y[j], inv_dt2));
+    move(warm_row(pos, [&]() { return ANC[j]; }, row, xpbd_ws::y_of(Y, j), inv_dt2));
   }
   __syncthreads();
   for (int i = p0 + tid; i < p1; i += BLOCK) {
@@ -1924,7 +1924,7 @@ int xpbd_jacobi_tile_pair_warm(void* stream, const float* Xin, float* Xout, cons
                                const int* p2_start, const int* p2, float h, int tile_size, int n,
                                float relax, float damping, float max_speed, float gx, float gy,
                                float gz, const float* gscale, int lmax, int amax, int k1max,
-                               int last, int predict, const float* Y, const int* ws_on,
+                               int last, int predict, const unsigned short* Y, const int* ws_on,
                                int invert) {
   if (n <= 0) return 0;
   if (tile_size < 1 || tile_size > XPBD_MAX_TILE || amax < tile_size || lmax < amax ||
@@ -2095,7 +2095,7 @@ int xpbd_jacobi_tile_first_cx(void* stream, const float* X, float* Xout, float* 
 int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP, const float* V,
                           const int* R, const float* ANC, float* LAMout,
                           const int* tile_row_start, const int* halo_start, const int* halo_rows,
-                          const float* Y, float h, float gx, float gy, float gz,
+                          const unsigned short* Y, float h, float gx, float gy, float gz,
                           const float* gscale, int tile_size, int n_particles, float relax,
                           int predict, const int* ws_on, int invert) {
   if (n_particles <= 0) return 0;
@@ -2125,7 +2125,7 @@ int xpbd_jacobi_tile_warm(void* stream, const float* Xin, float* Xout, float* XP
 int xpbd_jacobi_tile_warm_cx(void* stream, const float* Xin, float* Xout, float* XP,
                              const float* V, const int* R, const float* ANC, float* LAMout,
                              const int* tile_row_start, const int* halo_start,
-                             const xpbd_tile_compact* cx, const float* Y, float h, float gx,
+                             const xpbd_tile_compact* cx, const unsigned short* Y, float h, float gx,
                              float gy, float gz, const float* gscale, int tile_size,
                              int n_particles, float relax, int predict, const int* ws_on,
                              int invert) {

@@ -109,7 +109,7 @@ template <bool PREDICT, int ROWS>
 __global__ __launch_bounds__(kNetBlock) void k_ws_net(
     const float4* __restrict__ Xin, const float4* __restrict__ XP, const float4* __restrict__ V,
     const int4* __restrict__ R, const float4* __restrict__ ANC,
-    const xpbd_mlp::Image* __restrict__ img, float* __restrict__ Y, float h, float gx, float gy,
+    const xpbd_mlp::Image* __restrict__ img, unsigned short* __restrict__ Y, float h, float gx, float gy,
     float gz, const float* __restrict__ gscale, int m, const int* __restrict__ ws_on,
     int invert) {
   using namespace xpbd_mlp;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kNetBlock) void k_ws_net(
       if (c * ROWS + 16 * t >= m) break;
       const bf16x4 xb = *(const bf16x4*)&s.ftile[wv][16 * t + r16][4 * g];
       const float y = forward16_t(rw, s.b1, s.b2, s.w3, xb, b3);
-      if (g == 0 && j < m) Y[j] = y;
+      if (g == 0 && j < m) Y[j] = xpbd_ws::y_bits(y);
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kNetBlock) void k_ws_net(
 
 template <int ROWS>
 void launch_net(hipStream_t s, const float* Xin, const float* XP, const float* V, const int* R,
-                const float* ANC, const xpbd_mlp::Image* img, float* Y, float h, float gx,
+                const float* ANC, const xpbd_mlp::Image* img, unsigned short* Y, float h, float gx,
                 float gy, float gz, const float* gscale, int n_rows, int predict,
                 const int* ws_on, int invert) {
   const int groups = (n_rows + ROWS - 1) / ROWS;
@@ -253,7 +253,7 @@ int xpbd_ws_features(void* stream, const float* X, const float* XP, const int* R
 }
 
 int xpbd_ws_net(void* stream, const float* Xin, const float* XP, const float* V, const int* R,
-                const float* ANC, const void* image, float* Y, float h, float gx, float gy,
+                const float* ANC, const void* image, unsigned short* Y, float h, float gx, float gy,
                 float gz, const float* gscale, int n_rows, int predict, const int* ws_on,
                 int invert) {
   if (n_rows <= 0) return 0;

@@ -47,6 +47,16 @@ __device__ inline void row_features(int t, float C, float w, float alpha, float 
 // Whether the learned start is in use for a launch: ws_on (device flag of
 // the frame plan's judge; NULL = always) and `invert` (the judge runs the
 // start the solver is not using).
+// The network's output per row is kept as bfloat16 bits (round to nearest
+// even): the start is a learned estimate, and the per-substep launches read
+// it for every row.
+__device__ inline unsigned short y_bits(float y) {
+  return __builtin_bit_cast(unsigned short, (__bf16)y);
+}
+__device__ inline float y_of(const unsigned short* Y, int j) {
+  return __uint_as_float((unsigned)Y[j] << 16);
+}
+
 __device__ inline bool start_active(const int* ws_on, int invert) {
   const bool on = ws_on ? (*ws_on != 0) : true;
   return on != (invert != 0);

@@ -933,7 +933,7 @@ class Solver:
             self.X_B = self.LAM_B = None
             self._tile_dev = None
         self._perm = perm
-        self.WSY = torch.empty(max(n_c, 1), dtype=torch.float32, device=dev)  # warm-start y
+        self.WSY = torch.empty(max(n_c, 1), dtype=torch.bfloat16, device=dev)  # warm-start y
         self.GS = None
         if self._gscale is not None:
             if not tile_mode:
@@ -1188,7 +1188,8 @@ class Solver:
                 m = max(n_c, 1)
                 f4 = lambda: torch.zeros((max(n_b, 1), 4), dtype=torch.float32, device=dev)  # noqa
                 f1 = lambda k: torch.zeros(k, dtype=torch.float32, device=dev)  # noqa
-                self._judge = dict(x=f4(), xa=f4(), xp=f4(), l=f1(m), la=f1(m), y=f1(m),
+                self._judge = dict(x=f4(), xa=f4(), xp=f4(), l=f1(m), la=f1(m),
+                                   y=torch.zeros(m, dtype=torch.bfloat16, device=dev),
                                    rs=torch.zeros(2, dtype=torch.float64, device=dev))
                 j = self._judge
                 d.ws_judge = int(self.warm_start_judge)

@@ -1060,17 +1060,16 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_tile_pair(
     return a;
   };
   for (int o = threadIdx.x; o < nacc; o += BLOCK) {
+    const float4 a = sums(o);
+    if (o < P) {  // cleared as read: phase 2 sums the tile again
+      sx[o] = sy[o] = sz[o] = 0ull;
+      cnt[o] = 0u;
+    }
     if (gidx[o] < 0) continue;
     const float4 x4 = spos[o];
-    const float4 a = sums(o);
     f3 x = xyz(x4);
     if (a.w > 0.0f && x4.w > 0.0f) x = x + xyz(a) * (relax / a.w);
     spos[o] = make_float4(x.x, x.y, x.z, x4.w);
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < P; k += BLOCK) {
-    sx[k] = sy[k] = sz[k] = 0ull;
-    cnt[k] = 0u;
   }
   __syncthreads();
   // phase 2: the second iteration of the tile's rows

@@ -523,14 +523,13 @@ def warm_start_section(res: int, frames: int) -> dict:
     }
 
 
-def warm_start_batched_section(frames: int) -> dict:
-    """Learned lambda warm start on the particle solver of the batched world
-    (1024 uneven scenes: chains, cloths, sphere piles with device contact
-    rows; per-scene gravity, compliance and friction): a network for one
-    pass per frame is fitted on the world itself (two rounds, on the GPU),
-    then each start runs on its own trajectory from the same state (RMS
-    residual over the static rows, 60 substeps) and whole frames are timed
-    through the captured frame plan."""
+def batched_particles(fit_log=None):
+    """The batched world's particle solver after two frames, with a
+    ``restore()`` that puts it back to that state (two plain iterations, no
+    learned start), and the network for one pass per frame fitted on the
+    world (two rounds, on the GPU) when ``fit_log`` is a list (it receives
+    the fit's log); returns (world, solver, restore, theta or None, fit
+    seconds)."""
     import torch
 
     from xpbd_amd import warmstart as ws
@@ -550,13 +549,31 @@ def warm_start_batched_section(frames: int) -> dict:
         s.set_state(x0, v0)
         return s
 
-    t0 = time.perf_counter()
+    theta, fit_s = None, 0.0
+    if fit_log is not None:
+        t0 = time.perf_counter()
+        theta = ws.fit(restore, substeps=60, settle_frames=1, steps=800, log=fit_log,
+                       net_every=15, pair_with_net=True, every=5)
+        torch.cuda.synchronize()
+        fit_s = time.perf_counter() - t0
+        restore()
+    return w, s, restore, theta, fit_s
+
+
+def warm_start_batched_section(frames: int) -> dict:
+    """Learned lambda warm start on the particle solver of the batched world
+    (1024 uneven scenes: chains, cloths, sphere piles with device contact
+    rows; per-scene gravity, compliance and friction): a network for one
+    pass per frame is fitted on the world itself (two rounds, on the GPU),
+    then each start runs on its own trajectory from the same state (RMS
+    residual over the static rows, 60 substeps) and whole frames are timed
+    through the captured frame plan."""
+    import torch
+
+    from xpbd_amd import warmstart as ws
+
     log = []
-    theta = ws.fit(restore, substeps=60, settle_frames=1, steps=800, log=log, net_every=15,
-                   pair_with_net=True, every=5)
-    torch.cuda.synchronize()
-    fit_s = time.perf_counter() - t0
-    restore()
+    w, s, restore, theta, fit_s = batched_particles(fit_log=log)
     s.step(4)
     resid, fallback = ws.Evaluator(s, theta).run(substeps=60, plain_iters=(1, 2, 4),
                                                  learned_iters=(1, 2), net_every=15)
@@ -650,30 +667,12 @@ def scene_only(name: str, frames: int, warmup: int) -> dict:
         s.enable_self_collision(False)
         s.enable_device_contacts(True, margin=0.0)
     elif name in ("batched-plain4", "batched-learned1"):
-        from xpbd_amd import warmstart as ws
-        from xpbd_amd.batch import BatchedWorld, draw
-
-        w = BatchedWorld(draw(1024, 0), device="cuda:0")
-        s = w.particles
-        s.block_frames = False
-        s.step(2)
-        torch.cuda.synchronize()
-        if name == "batched-learned1":
-            x0, v0 = s.positions(), s.velocities()
-
-            def restore():
-                if s._ws_theta is not None:
-                    s.set_warm_start(None)
-                s.iterations = 2
-                s.set_state(x0, v0)
-                return s
-
-            theta = ws.fit(restore, substeps=60, settle_frames=1, steps=800, net_every=15,
-                           pair_with_net=True, every=5)
-            restore()
+        learned = name == "batched-learned1"
+        w, s, _, theta, _ = batched_particles(fit_log=[] if learned else None)
+        if learned:
             s.warm_start_every = 15
             s.set_warm_start(theta)
-        s.iterations = 1 if name == "batched-learned1" else 4
+        s.iterations = 1 if learned else 4
     else:
         raise SystemExit(f"unknown scene {name!r}")
     if name == "tile-kernel":

@@ -37,9 +37,9 @@ def candidates(n_particles: int, tile_only: bool = False, fused_only: bool = Fal
                pair: bool = False):
     """Launch shapes to time; ``tile_only`` (learned warm start): no window
     or block shapes; ``fused_only`` (the learned start with its judge, which
-    runs in the fused plan): integrate folded in; ``pair`` (two iterations,
-    no warm start): the fused plan also without the one-launch substep
-    (fuse "two")."""
+    runs in the fused plan): integrate folded in; ``pair`` (two plain
+    iterations, or the learned start and one iteration): the fused plan also
+    without the one-launch substep (fuse "two")."""
     out = []
     for t in (32, 64, 128, 256, 512, 1024):
         if t <= max(64, 2 * n_particles):
@@ -94,7 +94,10 @@ def tune(s, frames: int = 3, warmup: int = 1, verbose: bool = False):
     tile_only = (s._ws_theta is not None or s._gscale is not None
                  or getattr(s, "_device_contacts", False))
     judged = s._ws_theta is not None and int(s.warm_start_judge) > 0 and s.substeps >= 2
-    pair = (s.iterations == 2) == (s._ws_theta is None) and s.iterations in (1, 2)
+    # plans that can run a substep as one launch: two plain iterations, or
+    # the learned start and one iteration
+    pair = ((s.iterations == 2 and s._ws_theta is None)
+            or (s.iterations == 1 and s._ws_theta is not None))
     for cand in candidates(s.num_bodies, tile_only=tile_only, fused_only=judged, pair=pair):
         _apply(s, cand)
         s.set_state(x0, v0)

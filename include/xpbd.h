@@ -389,6 +389,9 @@ typedef struct xpbd_particles_desc {
    * pf may be set only without the pair launch, which reads FLOOR
    * multipliers from lam. */
   const xpbd_tile_compact* compact;
+  /* optional: a second judge x_prev buffer (float4[n]), so that judged
+   * substeps can also run as one launch (xpbd_jacobi_tile_pair_warm) */
+  float* ws_jxp_alt;
 } xpbd_particles_desc;
 
 /* A whole two-iteration substep of the fused tile plan in one launch, for

@@ -47,7 +47,7 @@ class ParticlesDesc(ctypes.Structure):
         ("pair_ring_idx", c_void_p), ("pair_p1_start", c_void_p), ("pair_p1", c_void_p),
         ("pair_p2_start", c_void_p), ("pair_p2", c_void_p),
         ("pair_lmax", c_int), ("pair_amax", c_int), ("pair_k1max", c_int),
-        ("compact", c_void_p),
+        ("compact", c_void_p), ("ws_jxp_alt", c_void_p),
     ]
 
 

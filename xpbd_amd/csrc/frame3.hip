@@ -297,7 +297,7 @@ int issue_frame_tile(Frame3* f, hipStream_t s, bool judge) {
     judge = judge && warm && d.ws_judge > 0;
     for (int sub = 0; sub < d.substeps; ++sub) {
       if (judge && sub == 0) {
-        TileRun jr{{d.ws_jx, d.ws_jx_alt}, {d.ws_jlam, d.ws_jlam_alt}, d.ws_jxp, nullptr,
+        TileRun jr{{d.ws_jx, d.ws_jx_alt}, {d.ws_jlam, d.ws_jlam_alt}, d.ws_jxp, d.ws_jxp_alt,
                    d.ws_jy, dyn ? d.ws_jacc : nullptr, d.ws_jdlam, d.x};
         rc = issue_fused_substep(d, s, jr, 0, 1);
         if (rc) return rc;

@@ -1188,12 +1188,13 @@ class Solver:
                 m = max(n_c, 1)
                 f4 = lambda: torch.zeros((max(n_b, 1), 4), dtype=torch.float32, device=dev)  # noqa
                 f1 = lambda k: torch.zeros(k, dtype=torch.float32, device=dev)  # noqa
-                self._judge = dict(x=f4(), xa=f4(), xp=f4(), l=f1(m), la=f1(m),
+                self._judge = dict(x=f4(), xa=f4(), xp=f4(), xpa=f4(), l=f1(m), la=f1(m),
                                    y=torch.zeros(m, dtype=torch.bfloat16, device=dev),
                                    rs=torch.zeros(2, dtype=torch.float64, device=dev))
                 j = self._judge
                 d.ws_judge = int(self.warm_start_judge)
                 d.ws_jx, d.ws_jx_alt, d.ws_jxp = N.ptr(j["x"]), N.ptr(j["xa"]), N.ptr(j["xp"])
+                d.ws_jxp_alt = N.ptr(j["xpa"])
                 d.ws_jlam, d.ws_jlam_alt, d.ws_jy = N.ptr(j["l"]), N.ptr(j["la"]), N.ptr(j["y"])
                 d.ws_rsum, d.ws_stats = N.ptr(j["rs"]), N.ptr(self.WSST)
                 if self.CROWS is not None:
